@@ -1,0 +1,218 @@
+/*
+ * lima_hip.h -- C ABI of the MI355X-native LIMA force / minimization path.
+ *
+ * The library (lima_amd/lib/liblima_hip.so) replaces LIMA_ENGINE's per-step
+ * force path and its energy-minimization entry point.  Every entry point below
+ * names the reference interface it stands in for (paths are relative to the
+ * LIMA source tree).
+ *
+ * Conventions
+ *   - Plain C, no exceptions cross the ABI.  Every function returns an
+ *     lima_status; lima_last_error() returns a thread-local message for the
+ *     last failing call on the calling thread.
+ *   - The caller owns every host buffer it passes; lima_create deep-copies the
+ *     box.  The engine owns all device memory and frees it in lima_destroy.
+ *   - An engine handle is not thread-safe.  One HIP stream set per handle.
+ *   - Units are LIMA's (code/LIMA_BASE/include/Constants.h:43-66,
+ *     LimaTypes.cuh:297-334): length nm, positions as int32 Coord in 1e-8 nm
+ *     relative to an integer-nm origo, velocity nm/ns, mass kg/mol, charge
+ *     kC/mol, energy J/mol, force J/mol/nm, time steps given in fs.
+ */
+#ifndef LIMA_HIP_H
+#define LIMA_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LIMA_MAX_COMPOUND_PARTICLES 32 /* Constants.h:91 MAX_COMPOUND_PARTICLES */
+#define LIMA_LUT_SLOTS 5               /* LimaTypes.cuh:455 max_bonded_compounds */
+#define LIMA_NANO_TO_LIMA 100000000    /* LimaTypes.cuh:299 Coord::nanoToLima_i */
+
+typedef enum lima_status {
+    LIMA_OK = 0,
+    LIMA_ERR_HIP = 1,          /* a HIP runtime call failed */
+    LIMA_ERR_BAD_INPUT = 2,    /* descriptor / parameter validation failed */
+    LIMA_ERR_LIST_OVERFLOW = 3,/* a neighbour list or cell column overflowed */
+    LIMA_ERR_NONFINITE = 4,    /* NaN / inf force or energy detected */
+    LIMA_ERR_NO_DEVICE = 5,    /* no gfx950 device visible */
+    LIMA_ERR_UNSUPPORTED = 6   /* feature outside this implementation's scope */
+} lima_status;
+
+/* Boundary condition select, LIMA BoundaryConditionSelect (BoundaryConditionPublic.h).
+ * Only periodic boxes are implemented. */
+typedef enum lima_bc { LIMA_BC_NONE = 0, LIMA_BC_PBC = 1 } lima_bc;
+
+/*
+ * Box descriptor: the flat-array form of LIMA's Box (Simulation.cuh:137-157).
+ *
+ * Compounds keep LIMA's layout (Bodies.cuh:193-279): up to 32 particles per
+ * compound, an integer-nm origo (NodeIndex) and int32 Coord positions relative
+ * to it (CompoundCoords).  Per-particle arrays are [n_compounds][32].
+ * Tinymols are LIMA's single-site LJ solvents (BoxGrid.cuh:12-60): each has
+ * the NodeIndex of its solvent block and a Coord relative to that block.
+ *
+ * Exclusions are LIMA's BondedParticlesLUT set (LimaTypes.cuh:415-490,
+ * CompoundBuilder.cpp:17-93): bonded_lut[c][k][i] bit j is set when particle i
+ * of compound c and particle j of compound c+k-2 share any bonded term (or are
+ * the same particle).  Compounds c and c+k-2 with any bit set are "bonded
+ * compounds": their non-excluded pairs interact without cutoff
+ * (LennardJonesInteractions.cuh:121-161).
+ *
+ * Bonded terms are LIMA's BondGroup contents (Bodies.cuh:281-307) flattened;
+ * particle ids are compound*32 + slot.  Parameters are in LIMA's stored units
+ * (Bodies.cpp:9-47): dihedral k_phi is the GROMACS k converted by
+ * DihedralBond::Parameters::CreateFromCharmm, i.e. kJ*1000/2.
+ */
+typedef struct lima_box_desc {
+    double box_nm[3];                 /* orthorhombic edges [nm] */
+    int32_t n_compounds;
+    int32_t n_tinymols;
+
+    const int32_t*  compound_n_particles;  /* [n_compounds] */
+    const int32_t*  compound_origo;        /* [n_compounds][3] NodeIndex [nm] */
+    const int32_t*  compound_key_particle; /* [n_compounds] centerparticle_index */
+    const int32_t*  particle_coord;        /* [n_compounds][32][3] Coord rel. origo */
+    const float*    particle_velocity;     /* [n_compounds][32][3] nm/ns, NULL = 0 */
+    const float*    particle_charge;       /* [n_compounds][32] kC/mol */
+    const float*    particle_sigma;        /* [n_compounds][32] nm */
+    const float*    particle_epsilon;      /* [n_compounds][32] J/mol */
+    const float*    particle_mass;         /* [n_compounds][32] kg/mol */
+    const uint32_t* bonded_lut;            /* [n_compounds][5][32] */
+
+    const int32_t*  tinymol_block;         /* [n_tinymols][3] NodeIndex */
+    const int32_t*  tinymol_coord;         /* [n_tinymols][3] Coord rel. block */
+    const float*    tinymol_velocity;      /* [n_tinymols][3], NULL = 0 */
+    const float*    tinymol_sigma;         /* [n_tinymols] nm */
+    const float*    tinymol_epsilon;       /* [n_tinymols] J/mol */
+    const float*    tinymol_mass;          /* [n_tinymols] kg/mol */
+
+    int32_t n_bonds;     const int32_t* bond_ids;     const float* bond_params;     /* [n][2] ids, [n][2] b0 nm, kb J/mol/nm^2 */
+    int32_t n_angles;    const int32_t* angle_ids;    const float* angle_params;    /* [n][3] ids, [n][4] theta0 rad, ktheta J/mol/rad^2, ub0 nm, kub J/mol/nm^2 */
+    int32_t n_dihedrals; const int32_t* dihedral_ids; const float* dihedral_params; /* [n][4] ids, [n][3] phi0 rad, kphi J/mol, n */
+    int32_t n_impropers; const int32_t* improper_ids; const float* improper_params; /* [n][4] ids, [n][2] psi0 rad, kpsi J/mol/rad^2 */
+    int32_t n_pairs;     const int32_t* pair_ids;     const float* pair_params;     /* [n][2] ids, [n][2] sigma nm, eps J/mol */
+} lima_box_desc;
+
+/* LIMA SimParams (Simulation.cuh:17-46) plus the knobs of this implementation. */
+typedef struct lima_simparams {
+    int64_t n_steps;
+    float   dt_fs;                    /* SimParams::dt, given in fs */
+    int32_t em_variant;               /* SimParams::em_variant */
+    float   em_force_tolerance;       /* kJ/mol/nm */
+    int32_t bc_select;                /* lima_bc, PBC only */
+    int32_t enable_electrostatics;
+    float   cutoff_nm;
+    int32_t data_logging_interval;
+    int32_t apply_thermostat;
+    int64_t steps_per_temperature_measurement;
+    /* additions */
+    int32_t pme_grid[3];              /* 0 = derived from pme_spacing_nm */
+    float   pme_spacing_nm;           /* target PME spacing, default 0.12 */
+    float   nlist_skin_nm;            /* Verlet buffer, default 0.1 */
+    int32_t nlist_interval;           /* steps between list rebuilds, default 10 */
+    int32_t energy_interval;          /* steps between FP64 energy reductions (0 = never) */
+} lima_simparams;
+
+/* FP64 energy totals [J/mol]. */
+typedef struct lima_energy {
+    double lj;            /* LJ, cutoff pairs + intra/bonded-compound pairs */
+    double coulomb_sr;    /* erfc-Coulomb, same pair set */
+    double bonded;        /* bonds + angles(+UB) + dihedrals + impropers + 1-4 pairs */
+    double pme_recip;     /* 1/2 sum_i q_i phi_i */
+    double pme_self_lima; /* LIMA's per-charged-particle self term, PME.cuh:417-420,610-630 */
+    double pme_self_ewald;/* standard -kappa/sqrt(pi) k sum q^2 */
+    double potential;     /* lj + coulomb_sr + bonded + pme_recip + pme_self_ewald */
+    double potential_lima;/* sum of LIMA's per-particle potE (uses pme_self_lima) */
+    double kinetic;
+    double total;         /* potential + kinetic */
+} lima_energy;
+
+/* LIMA RunStatus mirror (Engine.cuh:63-75). */
+typedef struct lima_runstatus {
+    int64_t current_step;
+    float   current_temperature;
+    float   greatest_force;       /* kJ/mol/nm, max |F| at last check */
+    int32_t simulation_finished;
+    int32_t critical_error_occured;
+    int32_t n_list_rebuilds;
+} lima_runstatus;
+
+/* State read-back.  Arrays the caller provides (NULL = skip) in LIMA's
+ * [n_compounds][32] / [n_tinymols] layouts. */
+typedef struct lima_state_out {
+    int32_t* compound_origo;      /* [n_compounds][3] */
+    int32_t* particle_coord;      /* [n_compounds][32][3] */
+    float*   particle_velocity;   /* [n_compounds][32][3] */
+    float*   particle_force;      /* [n_compounds][32][3] J/mol/nm, forces of the last evaluation */
+    float*   particle_pot_energy; /* [n_compounds][32] J/mol, LIMA per-particle potE of the last energy evaluation */
+    int32_t* tinymol_block;       /* [n_tinymols][3] */
+    int32_t* tinymol_coord;       /* [n_tinymols][3] */
+    float*   tinymol_velocity;    /* [n_tinymols][3] */
+    float*   tinymol_force;       /* [n_tinymols][3] */
+    float*   tinymol_pot_energy;  /* [n_tinymols] */
+} lima_state_out;
+
+/* Steepest-descent result (Programs::EnergyMinimize, Programs.cpp:155-199). */
+typedef struct lima_em_result {
+    int64_t iterations;
+    int64_t accepted;
+    double  final_energy;         /* FP64 potential, J/mol */
+    double  final_fmax;           /* kJ/mol/nm */
+    double  wall_seconds;
+    int32_t converged;
+} lima_em_result;
+
+typedef struct lima_engine lima_engine;
+
+/* ---- engine lifetime: Engine(std::unique_ptr<Simulation>, BoundaryConditionSelect, ...)
+ *      code/LIMA_ENGINE/include/Engine.cuh:78-99, Engine.cu:19-81 ---- */
+lima_status lima_create(const lima_box_desc* box, const lima_simparams* params, lima_engine** out);
+/* Engine::~Engine / terminateSimulation, Engine.cu:192-199 */
+lima_status lima_destroy(lima_engine* engine);
+
+/* ---- Engine::step(), Engine.cu:148-159: nsteps velocity-Verlet steps ---- */
+lima_status lima_step(lima_engine* engine, int64_t nsteps);
+
+/* One force (+ FP64 energy) evaluation at the current positions, no integration.
+ * The force half of _deviceMaster (Engine.cu:297-352). */
+lima_status lima_compute_forces(lima_engine* engine, lima_energy* energy_out);
+
+/* Last FP64 energy totals (from lima_compute_forces, an energy step, or EM). */
+lima_status lima_get_energy(lima_engine* engine, lima_energy* energy_out);
+
+/* ---- Programs::EnergyMinimize, Programs.cpp:155-199 (EM entry, SimParams::em_variant):
+ *      steepest descent with FP64 energy acceptance ---- */
+lima_status lima_minimize(lima_engine* engine, double fmax_tol_kj, int64_t max_iter,
+                          double initial_step_nm, lima_em_result* result);
+
+/* ---- takeBackSim / CopyDataToHost, EngineHostside.h:10-14, SimulationDevice.cpp:74-96 ---- */
+lima_status lima_get_state(lima_engine* engine, lima_state_out* out);
+/* ---- Engine::runstatus, Engine.cuh:95 ---- */
+lima_status lima_get_status(lima_engine* engine, lima_runstatus* out);
+
+/* ---- NeighborLists::updateNlists, Neighborlists.cuh:294-317: force a rebuild and
+ *      read back the cluster list.  Counts: clusters, list entries. ---- */
+lima_status lima_get_neighbor_counts(lima_engine* engine, int64_t* n_clusters, int64_t* n_entries);
+/* cluster_atoms[n_clusters*8]: global particle index or -1; entry_offsets[n_clusters+1];
+ * entries[n_entries]: j particle index | (shift code << 24); entry_masks[n_entries]: bit i set = pair
+ * (cluster slot i, j) is computed in the cutoff kernel. */
+lima_status lima_get_neighbor_list(lima_engine* engine, int32_t* cluster_atoms, int64_t* entry_offsets,
+                                   uint32_t* entries, uint8_t* entry_masks);
+
+/* Thread-local message of the last error on this thread. */
+const char* lima_last_error(void);
+/* Per-phase GPU times [ms] accumulated since create (HIP events). */
+typedef struct lima_timings {
+    double pair_ms, intra_ms, bonded_ms, pme_ms, integrate_ms, nlist_ms, total_ms;
+    int64_t steps_timed;
+} lima_timings;
+lima_status lima_get_timings(lima_engine* engine, lima_timings* out);
+lima_status lima_set_timing(lima_engine* engine, int32_t enabled);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LIMA_HIP_H */

@@ -1,0 +1,279 @@
+"""lima_amd -- MI355X-native LIMA force / minimization path.
+
+Python access to the two shared libraries built in-tree:
+
+* ``lib/liblima_hip.so``  -- the product: HIP/CDNA4 kernels behind the C ABI of
+  ``include/lima_hip.h`` (stands in for LIMA_ENGINE's Engine and EM entry points).
+* ``lib/liblima_host.so`` -- host C++: the synthetic box generators of ``include/lima_gen.h``.
+
+Nothing here falls back to a CPU implementation: if the HIP library is missing,
+``load_hip()`` raises.
+"""
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+from ._abi import BoxDesc, SimParams, Energy, RunStatus, StateOut, EmResult, Timings
+
+LIB_DIR = Path(__file__).resolve().parent / "lib"
+REPO_ROOT = Path(__file__).resolve().parent.parent
+
+_host = None
+_hip = None
+
+
+class LimaError(RuntimeError):
+    pass
+
+
+def load_host():
+    global _host
+    if _host is None:
+        path = LIB_DIR / "liblima_host.so"
+        if not path.exists():
+            raise LimaError(f"{path} not built; run __graft_entry__.build()")
+        lib = C.CDLL(str(path))
+        pp = C.POINTER(C.c_void_p)
+        lib.lima_gen_water_lattice.argtypes = [C.c_int32, C.c_double, C.c_uint64, C.c_double, pp]
+        lib.lima_gen_chain_water.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_int32,
+                                             C.c_uint64, C.c_double, pp]
+        lib.lima_gen_em_system.argtypes = [C.c_int32, C.c_uint64, C.c_double, pp]
+        lib.lima_gen_large_water.argtypes = [C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double,
+                                             C.c_uint64, C.c_double, pp]
+        for f in ("lima_gen_water_lattice", "lima_gen_chain_water", "lima_gen_em_system", "lima_gen_large_water"):
+            getattr(lib, f).restype = C.c_int
+        lib.lima_box_describe.argtypes = [C.c_void_p, C.POINTER(BoxDesc)]
+        lib.lima_box_free.argtypes = [C.c_void_p]
+        lib.lima_box_hash.argtypes = [C.c_void_p]
+        lib.lima_box_hash.restype = C.c_uint64
+        lib.lima_box_n_particles.argtypes = [C.c_void_p]
+        lib.lima_box_n_particles.restype = C.c_int64
+        lib.lima_gen_last_error.restype = C.c_char_p
+        lib.lima_default_params.argtypes = [C.POINTER(SimParams), C.c_float, C.c_int64]
+        _host = lib
+    return _host
+
+
+def load_hip():
+    """Load the HIP product library.  Raises if it has not been built."""
+    global _hip
+    if _hip is None:
+        path = LIB_DIR / "liblima_hip.so"
+        if not path.exists():
+            raise LimaError(f"{path} not built; the HIP path has no CPU fallback")
+        lib = C.CDLL(str(path))
+        vp = C.c_void_p
+        sig = {
+            "lima_create": ([C.POINTER(BoxDesc), C.POINTER(SimParams), C.POINTER(vp)], C.c_int),
+            "lima_destroy": ([vp], C.c_int),
+            "lima_step": ([vp, C.c_int64], C.c_int),
+            "lima_compute_forces": ([vp, C.POINTER(Energy)], C.c_int),
+            "lima_get_energy": ([vp, C.POINTER(Energy)], C.c_int),
+            "lima_minimize": ([vp, C.c_double, C.c_int64, C.c_double, C.POINTER(EmResult)], C.c_int),
+            "lima_get_state": ([vp, C.POINTER(StateOut)], C.c_int),
+            "lima_get_status": ([vp, C.POINTER(RunStatus)], C.c_int),
+            "lima_get_neighbor_counts": ([vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
+            "lima_get_neighbor_list": ([vp, C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_uint32),
+                                        C.POINTER(C.c_uint8)], C.c_int),
+            "lima_get_timings": ([vp, C.POINTER(Timings)], C.c_int),
+            "lima_set_timing": ([vp, C.c_int32], C.c_int),
+        }
+        for name, (args, res) in sig.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = res
+        lib.lima_last_error.restype = C.c_char_p
+        _hip = lib
+    return _hip
+
+
+def default_params(dt_fs=2.0, n_steps=1000, **overrides):
+    p = SimParams()
+    load_host().lima_default_params(C.byref(p), dt_fs, n_steps)
+    for k, v in overrides.items():
+        if k == "pme_grid":
+            for d in range(3):
+                p.pme_grid[d] = v[d]
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def _arr(ptr, n, dtype):
+    if n == 0 or not ptr:
+        return np.zeros(0, dtype=dtype)
+    return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype, copy=True)
+
+
+class Box:
+    """Owning handle of a generated box (lima_box*) plus numpy copies of its arrays."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle)
+        self.desc = BoxDesc()
+        load_host().lima_box_describe(self._h, C.byref(self.desc))
+        d = self.desc
+        nc, nt = d.n_compounds, d.n_tinymols
+        self.n_compounds, self.n_tinymols = nc, nt
+        self.box_nm = np.array(list(d.box_nm))
+        self.n_particles_per_compound = _arr(d.compound_n_particles, nc, np.int32)
+        self.charge = _arr(d.particle_charge, nc * 32, np.float64).reshape(nc, 32)
+        self.mass = _arr(d.particle_mass, nc * 32, np.float64).reshape(nc, 32)
+        self.n_particles = int(load_host().lima_box_n_particles(self._h))
+        self.hash = int(load_host().lima_box_hash(self._h))
+
+    def __del__(self):
+        try:
+            if self._h:
+                load_host().lima_box_free(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    # ---- global particle order (compound particles, then tinymols) <-> LIMA layout ----
+    def layout_index(self):
+        """Index into the [n_compounds*32 + n_tinymols] LIMA layout for each global particle."""
+        idx = [c * 32 + s for c in range(self.n_compounds) for s in range(self.n_particles_per_compound[c])]
+        idx += [self.n_compounds * 32 + t for t in range(self.n_tinymols)]
+        return np.array(idx, dtype=np.int64)
+
+    def masses_global(self):
+        m = self.mass.reshape(-1)
+        li = self.layout_index()
+        out = np.empty(len(li))
+        nc32 = self.n_compounds * 32
+        for k, i in enumerate(li):
+            out[k] = m[i] if i < nc32 else 0.0
+        if self.n_tinymols:
+            tm = _arr(self.desc.tinymol_mass, self.n_tinymols, np.float64)
+            out[len(li) - self.n_tinymols:] = tm
+        return out
+
+
+def _gen(fn, *args):
+    h = C.c_void_p()
+    st = fn(*args, C.byref(h))
+    if st != 0:
+        raise LimaError(load_host().lima_gen_last_error().decode())
+    return Box(h.value)
+
+
+def gen_water_lattice(n_per_dim=10, box_nm=3.104, seed=0, temperature=300.0):
+    return _gen(load_host().lima_gen_water_lattice, n_per_dim, box_nm, seed, temperature)
+
+
+def gen_chain_water(box=(8.0, 8.0, 7.0), n_chains=128, chain_len=50, n_waters=12000, seed=1, temperature=300.0):
+    return _gen(load_host().lima_gen_chain_water, box[0], box[1], box[2], n_chains, chain_len, n_waters, seed, temperature)
+
+
+def gen_em_system(target_atoms, seed, noise_nm=0.02):
+    return _gen(load_host().lima_gen_em_system, target_atoms, seed, noise_nm)
+
+
+def gen_large_water(n_waters=333333, n_extra=1, extra_sigma=0.3, extra_eps_kj=0.5, extra_mass=39.948, seed=4,
+                    temperature=300.0):
+    return _gen(load_host().lima_gen_large_water, n_waters, n_extra, extra_sigma, extra_eps_kj, extra_mass, seed,
+                temperature)
+
+
+def em_batch_sizes(n_systems=256, lo=300, hi=3000, seed=1000):
+    """System sizes drawn log-uniform in [lo, hi] with the project RNG convention (SplitMix64)."""
+    sizes = []
+    x = seed & 0xFFFFFFFFFFFFFFFF
+    for _ in range(n_systems):
+        x = (x + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+        z = x
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+        z ^= z >> 31
+        u = (z >> 11) / float(1 << 53)
+        sizes.append(int(round(lo * (hi / lo) ** u)))
+    return sizes
+
+
+class Engine:
+    """Handle of a lima_engine (Engine.cuh:78-99) on the GPU."""
+
+    def __init__(self, box, params):
+        self.lib = load_hip()
+        self.box = box
+        h = C.c_void_p()
+        self._check(self.lib.lima_create(C.byref(box.desc), C.byref(params), C.byref(h)))
+        self._h = h
+
+    def _check(self, st):
+        if st != 0:
+            raise LimaError(f"status {st}: {self.lib.lima_last_error().decode()}")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.lima_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, n=1):
+        self._check(self.lib.lima_step(self._h, n))
+
+    def compute_forces(self):
+        e = Energy()
+        self._check(self.lib.lima_compute_forces(self._h, C.byref(e)))
+        return e
+
+    def energy(self):
+        e = Energy()
+        self._check(self.lib.lima_get_energy(self._h, C.byref(e)))
+        return e
+
+    def minimize(self, fmax_tol_kj=100.0, max_iter=10000, initial_step_nm=0.01):
+        r = EmResult()
+        self._check(self.lib.lima_minimize(self._h, fmax_tol_kj, max_iter, initial_step_nm, C.byref(r)))
+        return r
+
+    def status(self):
+        s = RunStatus()
+        self._check(self.lib.lima_get_status(self._h, C.byref(s)))
+        return s
+
+    def state(self):
+        """Forces, potE, velocities, positions in LIMA layout (numpy)."""
+        nc, nt = self.box.n_compounds, self.box.n_tinymols
+        st = StateOut()
+        out = {
+            "compound_origo": np.zeros((nc, 3), np.int32), "particle_coord": np.zeros((nc, 32, 3), np.int32),
+            "particle_velocity": np.zeros((nc, 32, 3), np.float32), "particle_force": np.zeros((nc, 32, 3), np.float32),
+            "particle_pot_energy": np.zeros((nc, 32), np.float32),
+            "tinymol_block": np.zeros((max(nt, 1), 3), np.int32), "tinymol_coord": np.zeros((max(nt, 1), 3), np.int32),
+            "tinymol_velocity": np.zeros((max(nt, 1), 3), np.float32), "tinymol_force": np.zeros((max(nt, 1), 3), np.float32),
+            "tinymol_pot_energy": np.zeros(max(nt, 1), np.float32),
+        }
+        for k, a in out.items():
+            setattr(st, k, a.ctypes.data_as(StateOut._fields_[[f[0] for f in StateOut._fields_].index(k)][1]))
+        self._check(self.lib.lima_get_state(self._h, C.byref(st)))
+        return out
+
+    def neighbor_list(self):
+        ncl, nent = C.c_int64(), C.c_int64()
+        self._check(self.lib.lima_get_neighbor_counts(self._h, C.byref(ncl), C.byref(nent)))
+        atoms = np.zeros(ncl.value * 8, np.int32)
+        off = np.zeros(ncl.value + 1, np.int64)
+        ent = np.zeros(max(nent.value, 1), np.uint32)
+        msk = np.zeros(max(nent.value, 1), np.uint8)
+        self._check(self.lib.lima_get_neighbor_list(
+            self._h, atoms.ctypes.data_as(C.POINTER(C.c_int32)), off.ctypes.data_as(C.POINTER(C.c_int64)),
+            ent.ctypes.data_as(C.POINTER(C.c_uint32)), msk.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return atoms, off, ent[:nent.value], msk[:nent.value]
+
+    def set_timing(self, on=True):
+        self._check(self.lib.lima_set_timing(self._h, 1 if on else 0))
+
+    def timings(self):
+        t = Timings()
+        self._check(self.lib.lima_get_timings(self._h, C.byref(t)))
+        return t

@@ -1,0 +1,176 @@
+// engine.h -- device state of one lima_engine and the launchers each .hip file provides.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/lima_hip.h"
+#include "lima_dev.h"
+
+namespace lima {
+
+// Arrays indexed by global particle (compound particles in compound order, then tinymols).
+struct Particles {
+    int64_t n = 0;
+    uint4* w = nullptr;        // wrapped Coord [0, Lq)
+    int4* img = nullptr;       // periodic image counters: unwrapped = w + img * Lq
+    float4* v = nullptr;       // velocity nm/ns
+    float4* fprev = nullptr;   // force of the previous step
+    float4* prm = nullptr;     // (charge kC/mol, sigma/2 nm, sqrt(eps J/mol), mass kg/mol)
+    int* comp = nullptr;       // compound id (tinymols: n_compounds + t)
+};
+
+// Cluster-pair neighbour list (definition in oracle/lima_oracle.cpp build_clusters).
+struct NList {
+    int ncx = 1, ncy = 1, ncol = 1;
+    int ncl = 0;               // clusters
+    int ncl_max = 0;
+    int64_t nent = 0, nent_cap = 0;
+    int* col_of = nullptr;     // [N]
+    int* col_count = nullptr;  // [ncol+1]
+    int* col_start = nullptr;  // [ncol+1]
+    int* col_fill = nullptr;   // [ncol]
+    unsigned long long* keys = nullptr;  // [N] (w_z << 32 | particle)
+    int* col_ncl = nullptr;    // [ncol+1] clusters per column -> exclusive scan
+    int* catoms = nullptr;     // [ncl_max*8] particle or -1
+    int* slot_of = nullptr;    // [N]
+    int* ccol = nullptr;       // [ncl_max] column of cluster
+    uint4* cref = nullptr;     // [ncl_max] cluster reference point (integer, centre of bbox)
+    uint4* cmin = nullptr;     // [ncl_max]
+    uint4* cmax = nullptr;     // [ncl_max]
+    float2* ljs = nullptr;     // [slots] (sigma/2, sqrt eps)
+    int* comps = nullptr;      // [slots] compound of slot (-1 padding)
+    float4* xq = nullptr;      // [slots] position relative to cluster ref [nm] + charge
+    int* ecount = nullptr;     // [ncl_max+1]
+    int* eoff = nullptr;       // [ncl_max+1]
+    float4* ent = nullptr;     // [nent_cap] (shift nm xyz, bits: j slot | mask << 24)
+    uint8_t* ecode = nullptr;  // [nent_cap] image shift code of each entry
+    int* flags = nullptr;      // [4] overflow flags
+};
+
+struct Pme {
+    int K[3] = {0, 0, 0};
+    int H = 0;                             // K[0]/2 + 1
+    long long* qgrid = nullptr;            // [Kz][Ky][Kx] fixed point charge * weight
+    float2* cgrid = nullptr;               // [Kz][Ky][H]
+    float* phi = nullptr;                  // [Kz][Ky][Kx]
+    float* green = nullptr;                // [Kz][Ky][H]
+    float2* tw[3] = {nullptr, nullptr, nullptr};   // twiddles exp(-2 pi i k / K_d)
+    int radix[3][8];
+    int nrad[3] = {0, 0, 0};
+    double eself_total = 0;                // -kappa/sqrt(pi_L) kC sum q^2
+    int64_t ncharged = 0;
+};
+
+struct Bonded {
+    int nb = 0, na = 0, nd = 0, ni = 0, np = 0;
+    int2* bid = nullptr; float2* bp = nullptr;
+    int4* aid = nullptr; float4* ap = nullptr;
+    int4* did = nullptr; float4* dp = nullptr;
+    int4* iid = nullptr; float2* ip = nullptr;
+    int2* pid = nullptr; float2* pp = nullptr;
+};
+
+// Device accumulators (FP64 energy reductions and scalars).
+enum Acc { ACC_LJ = 0, ACC_COUL, ACC_BOND, ACC_PME, ACC_KIN, ACC_N };
+
+struct SdState {            // steepest descent state, device resident
+    double e_acc, e_try;
+    float fmax_acc, fmax_try;
+    float h;
+    int accepted;
+    long long n_accepted;
+    double disp_since_build;
+};
+
+struct Engine {
+    lima_simparams prm{};
+    int nC = 0, nT = 0;
+    int64_t N = 0;
+    int64_t Lq[3]{};
+    double L[3]{};
+    float Lf[3]{};
+    float rc = 1.2f, rc2 = 1.44f, skin = 0.1f;
+    PairConst pc{};
+    double dt_ns = 2e-6;
+    bool elec = true;
+
+    Particles p;
+    // SD trial state
+    uint4* w_try = nullptr;
+    int4* img_try = nullptr;
+    float4* f_acc = nullptr;   // accepted forces, global order
+    SdState* sd = nullptr;
+
+    // compounds
+    int64_t* first = nullptr;  // [nC+1]
+    uint8_t* bmask = nullptr;  // [nC]
+    std::vector<int64_t> h_first;
+    std::vector<int> h_key;
+    std::vector<int> h_comp;   // global -> compound
+    // intra pairs work list
+    int n_intra = 0;
+    int4* intra_items = nullptr;    // (A, B, first A, first B)
+    uint32_t* intra_lut = nullptr;  // [n_intra*32]
+
+    Bonded bd;
+    NList nl;
+    Pme pme;
+
+    float* fs = nullptr;       // [3*slots] forces per cluster slot (SoA)
+    float* es = nullptr;       // [slots] per-slot potE
+    double* acc = nullptr;     // [ACC_N]
+    float* fmax = nullptr;     // [1]
+
+    hipStream_t s_main = nullptr, s_pme = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+
+    int64_t step = 0;
+    int64_t steps_since_build = 0;
+    int n_rebuilds = 0;
+    lima_energy energy{};
+    bool forces_valid = false;
+    bool critical = false;
+
+    // timing
+    bool timing = false;
+    hipEvent_t tev[16];
+    lima_timings tacc{};
+};
+
+#define HIP_CHECK(x)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (x);                                                                  \
+        if (e_ != hipSuccess) throw HipError(std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct HipError : std::runtime_error { using std::runtime_error::runtime_error; };
+struct LimaFail : std::runtime_error {
+    lima_status code;
+    LimaFail(lima_status c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+// ---- nlist.hip ----
+void nlist_alloc(Engine& e);
+void nlist_free(Engine& e);
+void nlist_build(Engine& e, const uint4* w);   // rebuilds clusters + lists from positions w
+
+// ---- forces.hip ----
+void launch_pack(Engine& e, const uint4* w, hipStream_t s);
+void launch_pair(Engine& e, bool energy, hipStream_t s);
+void launch_intra(Engine& e, const uint4* w, const int4* img, bool energy, hipStream_t s);
+void launch_bonded(Engine& e, const uint4* w, const int4* img, bool energy, hipStream_t s);
+
+// ---- pme.hip ----
+void pme_alloc(Engine& e, const std::vector<float>& charges_global);
+void pme_free(Engine& e);
+void launch_pme(Engine& e, const uint4* w, bool energy, hipStream_t s);
+
+// ---- scan helper (nlist.hip) ----
+void launch_exclusive_scan(const int* in, int* out, int n, hipStream_t s);
+
+}  // namespace lima

@@ -1,0 +1,768 @@
+// engine.hip -- C ABI of liblima_hip.so (include/lima_hip.h): engine lifetime, MD step,
+// steepest-descent minimisation, state read-back.
+//
+// Stands in for LIMA_ENGINE's Engine (code/LIMA_ENGINE/include/Engine.cuh:78-170,
+// Engine.cu:19-463) and Programs::EnergyMinimize (code/LIMA_MD/src/Programs.cpp:155-199).
+// Positions are integer Coords (1e-8 nm, LimaTypes.cuh:297-334) kept per particle as a wrapped
+// uint32 plus an image counter, so the unwrapped coordinate LIMA's compound-relative layout
+// represents is recoverable exactly and periodic wrapping never loses bits.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "engine.h"
+
+using namespace lima;
+
+struct lima_engine {
+    Engine e;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+inline int nblk(int64_t n, int b) { return (int)((n + b - 1) / b); }
+
+int64_t fdiv(int64_t a, int64_t b) { int64_t q = a / b; if ((a % b != 0) && ((a < 0) != (b < 0))) --q; return q; }
+
+// ---- integrator, EngineUtils.cuh:31-41 / EngineKernels.cuh:383-397 ----
+// v(t) = v(t-dt) + (F(t) + F(t-dt)) dt/2m ;  x += Coord{ v(t) dt + F(t) dt^2 / 2m } (truncating)
+struct IntegrateArgs {
+    int64_t N;
+    int nslots;
+    long long Lq[3];
+    float dt;
+    float thermostat;
+    const int* slot_of;
+    const float* fs;
+    uint4* w;
+    int4* img;
+    float4* v;
+    float4* fprev;
+    const float4* prm;
+    double* acc;
+};
+
+// w + d wrapped back into [0, Lq) for any d (a step may exceed a box edge when a run explodes).
+__device__ __forceinline__ void wrap_add(uint32_t& w, int& img, long long d, long long Lq) {
+    long long x = (long long)w + d;
+    if (x >= Lq || x < 0) {
+        long long q = x / Lq;
+        if (x - q * Lq < 0) q--;
+        x -= q * Lq;
+        img += (int)q;
+    }
+    w = (uint32_t)x;
+}
+
+__device__ __forceinline__ int coord_of(float nm) {   // Coord{Float3}: static_cast<int32_t>(pos * 1e8)
+    const float v = nm * LIMA_NANO_TO_LIMA_F;
+    return (int)fminf(fmaxf(v, -2.0e9f), 2.0e9f);
+}
+
+template <bool ENERGY>
+__global__ void k_integrate(IntegrateArgs a) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    float ke = 0.f;
+    if (p < a.N) {
+        const int s = a.slot_of[p];
+        const float fx = a.fs[s], fy = a.fs[a.nslots + s], fz = a.fs[2 * a.nslots + s];
+        const float m = a.prm[p].w;
+        const float4 fp = a.fprev[p];
+        float4 v = a.v[p];
+        const float kdt = a.dt * 0.5f / m;
+        v.x = v.x + (fx + fp.x) * kdt;
+        v.y = v.y + (fy + fp.y) * kdt;
+        v.z = v.z + (fz + fp.z) * kdt;
+        const float c2 = 0.5f / m * a.dt * a.dt;
+        uint4 w = a.w[p];
+        int4 im = a.img[p];
+        wrap_add(w.x, im.x, coord_of(v.x * a.dt + fx * c2), a.Lq[0]);
+        wrap_add(w.y, im.y, coord_of(v.y * a.dt + fy * c2), a.Lq[1]);
+        wrap_add(w.z, im.z, coord_of(v.z * a.dt + fz * c2), a.Lq[2]);
+        a.w[p] = w;
+        a.img[p] = im;
+        if (ENERGY) ke = 0.5f * m * (v.x * v.x + v.y * v.y + v.z * v.z);
+        v.x *= a.thermostat; v.y *= a.thermostat; v.z *= a.thermostat;
+        a.v[p] = v;
+        a.fprev[p] = make_float4(fx, fy, fz, 0.f);
+    }
+    if (ENERGY) {
+        const float t = wave_sum(ke);
+        if ((threadIdx.x & 63) == 0) atomicAdd(&a.acc[ACC_KIN], (double)t);
+    }
+}
+
+__global__ void k_kinetic(int64_t N, const float4* __restrict__ v, const float4* __restrict__ prm, double* acc) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    float ke = 0.f;
+    if (p < N) {
+        const float4 u = v[p];
+        ke = 0.5f * prm[p].w * (u.x * u.x + u.y * u.y + u.z * u.z);
+    }
+    const float t = wave_sum(ke);
+    if ((threadIdx.x & 63) == 0) atomicAdd(&acc[ACC_KIN], (double)t);
+}
+
+// Forces of the last evaluation in global order (w = per-particle potE) and max |F|.
+__global__ void k_forces_global(int64_t N, int nslots, const int* __restrict__ slot_of, const float* __restrict__ fs,
+                                const float* __restrict__ es, float4* __restrict__ out, float* fmax) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    float m = 0.f;
+    if (p < N) {
+        const int s = slot_of[p];
+        const float4 f = make_float4(fs[s], fs[nslots + s], fs[2 * nslots + s], es[s]);
+        out[p] = f;
+        m = sqrtf(f.x * f.x + f.y * f.y + f.z * f.z);
+        if (!(m <= 3.0e38f)) m = 3.0e38f;   // NaN / inf -> flagged as huge
+    }
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) atomic_max_pos(fmax, m);
+}
+
+__global__ void k_zero_d(double* p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0.0;
+}
+__global__ void k_zero_f(float* p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0.f;
+}
+
+// ---- steepest descent (GROMACS integrator=steep, dev/gmxscripts/em.mdp:2-5) ----
+__global__ void k_sd_trial(int64_t N, long long Lqx, long long Lqy, long long Lqz, const SdState* sd,
+                           const uint4* __restrict__ w, const int4* __restrict__ img, const float4* __restrict__ f,
+                           uint4* __restrict__ wt, int4* __restrict__ it) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= N) return;
+    const float k = sd->h / sd->fmax_acc;
+    const float4 F = f[p];
+    uint4 x = w[p];
+    int4 m = img[p];
+    wrap_add(x.x, m.x, coord_of(k * F.x), Lqx);
+    wrap_add(x.y, m.y, coord_of(k * F.y), Lqy);
+    wrap_add(x.z, m.z, coord_of(k * F.z), Lqz);
+    wt[p] = x;
+    it[p] = m;
+}
+
+__global__ void k_sd_accept(SdState* sd, const double* acc, double e_const, const float* fmax_try) {
+    const double e = acc[ACC_LJ] + acc[ACC_COUL] + acc[ACC_BOND] + acc[ACC_PME] + e_const;
+    sd->e_try = e;
+    sd->fmax_try = *fmax_try;
+    if (e < sd->e_acc) {
+        sd->accepted = 1;
+        sd->disp_since_build += sd->h;
+        sd->e_acc = e;
+        sd->fmax_acc = *fmax_try;
+        sd->h *= 1.2f;
+        sd->n_accepted++;
+    } else {
+        sd->accepted = 0;
+        sd->h *= 0.2f;
+    }
+}
+
+__global__ void k_sd_commit(int64_t N, const SdState* sd, const uint4* __restrict__ wt, const int4* __restrict__ it,
+                            const float4* __restrict__ ft, uint4* __restrict__ w, int4* __restrict__ img, float4* __restrict__ facc) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= N || !sd->accepted) return;
+    w[p] = wt[p];
+    img[p] = it[p];
+    facc[p] = ft[p];
+}
+
+// ---------------------------------------------------------------------------------------------
+
+void set_err(const std::string& s) { g_err = s; }
+
+template <class F>
+lima_status guarded(F&& f) {
+    try {
+        f();
+        return LIMA_OK;
+    } catch (const LimaFail& x) {
+        set_err(x.what());
+        return x.code;
+    } catch (const HipError& x) {
+        set_err(x.what());
+        return LIMA_ERR_HIP;
+    } catch (const std::exception& x) {
+        set_err(x.what());
+        return LIMA_ERR_BAD_INPUT;
+    }
+}
+
+template <class T>
+T* dalloc_copy(const std::vector<T>& v) {
+    T* p = nullptr;
+    if (v.empty()) { HIP_CHECK(hipMalloc(&p, sizeof(T))); return p; }
+    HIP_CHECK(hipMalloc(&p, v.size() * sizeof(T)));
+    HIP_CHECK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return p;
+}
+
+double e_const(const Engine& e) { return e.elec ? e.pme.eself_total : 0.0; }
+
+void evaluate(Engine& e, const uint4* w, const int4* img, bool energy) {
+    hipStream_t s = e.s_main;
+    hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(64), 0, s, e.acc, (int)ACC_N);
+    if (e.timing) HIP_CHECK(hipEventRecord(e.tev[0], s));
+    launch_pack(e, w, s);
+    launch_pair(e, energy, s);
+    if (e.timing) HIP_CHECK(hipEventRecord(e.tev[1], s));
+    launch_intra(e, w, img, energy, s);
+    if (e.timing) HIP_CHECK(hipEventRecord(e.tev[2], s));
+    launch_bonded(e, w, img, energy, s);
+    if (e.timing) HIP_CHECK(hipEventRecord(e.tev[3], s));
+    launch_pme(e, w, energy, s);
+    if (e.timing) HIP_CHECK(hipEventRecord(e.tev[4], s));
+}
+
+void fill_energy(Engine& e, bool with_kinetic) {
+    double acc[ACC_N];
+    HIP_CHECK(hipMemcpy(acc, e.acc, sizeof(acc), hipMemcpyDeviceToHost));
+    lima_energy& E = e.energy;
+    E.lj = acc[ACC_LJ];
+    E.coulomb_sr = acc[ACC_COUL];
+    E.bonded = acc[ACC_BOND];
+    E.pme_recip = acc[ACC_PME];
+    E.pme_self_ewald = e.elec ? e.pme.eself_total : 0.0;
+    E.pme_self_lima = e.elec ? 0.5 * e.pme.eself_total * (double)e.pme.ncharged : 0.0;
+    E.potential = E.lj + E.coulomb_sr + E.bonded + E.pme_recip + E.pme_self_ewald;
+    E.potential_lima = E.lj + E.coulomb_sr + E.bonded + E.pme_recip + E.pme_self_lima;
+    if (with_kinetic) E.kinetic = acc[ACC_KIN];
+    E.total = E.potential + E.kinetic;
+}
+
+void check_finite(Engine& e) {
+    if (!std::isfinite(e.energy.potential)) {
+        e.critical = true;
+        throw LimaFail(LIMA_ERR_NONFINITE, "non-finite energy");
+    }
+}
+
+void integrate(Engine& e, bool energy) {
+    IntegrateArgs a{};
+    a.N = e.N;
+    a.nslots = e.nl.ncl * CLUSTER;
+    for (int d = 0; d < 3; d++) a.Lq[d] = e.Lq[d];
+    a.dt = (float)e.dt_ns;
+    a.thermostat = 1.f;
+    a.slot_of = e.nl.slot_of; a.fs = e.fs; a.w = e.p.w; a.img = e.p.img; a.v = e.p.v; a.fprev = e.p.fprev;
+    a.prm = e.p.prm; a.acc = e.acc;
+    if (energy) hipLaunchKernelGGL(k_integrate<true>, dim3(nblk(e.N, 256)), dim3(256), 0, e.s_main, a);
+    else hipLaunchKernelGGL(k_integrate<false>, dim3(nblk(e.N, 256)), dim3(256), 0, e.s_main, a);
+}
+
+void md_steps(Engine& e, int64_t n) {
+    const int interval = e.prm.nlist_interval > 0 ? e.prm.nlist_interval : 10;
+    for (int64_t k = 0; k < n; k++) {
+        if (e.steps_since_build >= interval) {
+            if (e.timing) HIP_CHECK(hipEventRecord(e.tev[6], e.s_main));
+            nlist_build(e, e.p.w);
+            if (e.timing) {
+                HIP_CHECK(hipEventRecord(e.tev[7], e.s_main));
+                HIP_CHECK(hipEventSynchronize(e.tev[7]));
+                float ms;
+                HIP_CHECK(hipEventElapsedTime(&ms, e.tev[6], e.tev[7]));
+                e.tacc.nlist_ms += ms;
+            }
+        }
+        const bool energy = (e.prm.energy_interval > 0 && e.step % e.prm.energy_interval == 0) || k == n - 1;
+        evaluate(e, e.p.w, e.p.img, energy);
+        integrate(e, energy);
+        if (e.timing) {
+            HIP_CHECK(hipEventRecord(e.tev[5], e.s_main));
+            HIP_CHECK(hipEventSynchronize(e.tev[5]));
+            float t[5];
+            for (int i = 0; i < 5; i++) HIP_CHECK(hipEventElapsedTime(&t[i], e.tev[i], e.tev[i + 1]));
+            e.tacc.pair_ms += t[0]; e.tacc.intra_ms += t[1]; e.tacc.bonded_ms += t[2]; e.tacc.pme_ms += t[3];
+            e.tacc.integrate_ms += t[4];
+            float tot;
+            HIP_CHECK(hipEventElapsedTime(&tot, e.tev[0], e.tev[5]));
+            e.tacc.total_ms += tot;
+            e.tacc.steps_timed++;
+        }
+        e.step++;
+        e.steps_since_build++;
+        if (energy) {
+            fill_energy(e, true);
+            check_finite(e);
+        }
+    }
+    e.forces_valid = true;
+    HIP_CHECK(hipStreamSynchronize(e.s_main));
+    HIP_CHECK(hipGetLastError());
+}
+
+void create(Engine& e, const lima_box_desc* b, const lima_simparams* p) {
+    if (!b || !p) throw LimaFail(LIMA_ERR_BAD_INPUT, "null descriptor");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw LimaFail(LIMA_ERR_NO_DEVICE, "no HIP device visible");
+    e.prm = *p;
+    if (p->bc_select != LIMA_BC_PBC) throw LimaFail(LIMA_ERR_UNSUPPORTED, "only periodic boundary conditions are implemented");
+    if (!(p->cutoff_nm > 0.f)) throw LimaFail(LIMA_ERR_BAD_INPUT, "cutoff must be positive");
+    e.nC = b->n_compounds;
+    e.nT = b->n_tinymols;
+    if (e.nC < 0 || e.nT < 0) throw LimaFail(LIMA_ERR_BAD_INPUT, "negative counts");
+    e.rc = p->cutoff_nm;
+    e.rc2 = e.rc * e.rc;
+    e.skin = p->nlist_skin_nm > 0 ? p->nlist_skin_nm : 0.1f;
+    e.elec = p->enable_electrostatics != 0;
+    e.pc.kc = (float)(8.9875517873681764e9 / 1e-9 / 6.02214076e23 * 1000.f * 1000.f);   // PhysicsUtils.cuh:5
+    e.pc.kappa = 3.f / e.rc;                                                             // PhysicsUtils.cuh:60-62
+    e.pc.two_kappa_sqrtpi = 2.f * e.pc.kappa / sqrtf(LIMA_PI_F);
+    e.dt_ns = (double)(p->dt_fs * 1e-6f);
+    for (int d = 0; d < 3; d++) {
+        e.Lq[d] = llround(b->box_nm[d] * 1e8);
+        e.L[d] = (double)e.Lq[d] / 1e8;
+        e.Lf[d] = (float)e.L[d];
+        if (e.Lq[d] <= 0 || e.Lq[d] >= (1ll << 32)) throw LimaFail(LIMA_ERR_BAD_INPUT, "box edge must be in (0, 42.9) nm");
+        if (e.L[d] <= (double)e.rc + e.skin) throw LimaFail(LIMA_ERR_BAD_INPUT, "box edge must exceed cutoff + skin");
+    }
+    // ---- global particle order ----
+    e.h_first.assign(e.nC + 1, 0);
+    for (int c = 0; c < e.nC; c++) {
+        const int n = b->compound_n_particles[c];
+        if (n < 0 || n > 32) throw LimaFail(LIMA_ERR_BAD_INPUT, "compound particle count out of [0, 32]");
+        e.h_first[c + 1] = e.h_first[c] + n;
+    }
+    e.N = e.h_first[e.nC] + e.nT;
+    if (e.N <= 0) throw LimaFail(LIMA_ERR_BAD_INPUT, "empty box");
+    if (e.N >= (1 << 23)) throw LimaFail(LIMA_ERR_BAD_INPUT, "at most 2^23 particles");
+    const int64_t N = e.N;
+    std::vector<uint4> w(N);
+    std::vector<int4> img(N);
+    std::vector<float4> v(N), prm(N), zero4(N, make_float4(0.f, 0.f, 0.f, 0.f));
+    std::vector<int> comp(N);
+    std::vector<float> qv(N);
+    e.h_key.resize(e.nC);
+    auto put = [&](int64_t i, const int64_t* u) {
+        uint32_t ww[3];
+        int im[3];
+        for (int d = 0; d < 3; d++) {
+            const int64_t q = fdiv(u[d], e.Lq[d]);
+            ww[d] = (uint32_t)(u[d] - q * e.Lq[d]);
+            im[d] = (int)q;
+        }
+        w[i] = make_uint4(ww[0], ww[1], ww[2], 0);
+        img[i] = make_int4(im[0], im[1], im[2], 0);
+    };
+    for (int c = 0; c < e.nC; c++) {
+        e.h_key[c] = b->compound_key_particle ? b->compound_key_particle[c] : 0;
+        for (int s = 0; s < b->compound_n_particles[c]; s++) {
+            const int64_t i = e.h_first[c] + s;
+            const size_t ps = (size_t)c * 32 + s;
+            int64_t u[3];
+            for (int d = 0; d < 3; d++) u[d] = (int64_t)b->compound_origo[c * 3 + d] * 100000000ll + b->particle_coord[ps * 3 + d];
+            put(i, u);
+            const float* pv = b->particle_velocity ? &b->particle_velocity[ps * 3] : nullptr;
+            v[i] = make_float4(pv ? pv[0] : 0.f, pv ? pv[1] : 0.f, pv ? pv[2] : 0.f, 0.f);
+            const float q = e.elec ? b->particle_charge[ps] : 0.f;
+            const float m = b->particle_mass[ps];
+            if (!(m > 0.f)) throw LimaFail(LIMA_ERR_BAD_INPUT, "particle mass must be positive");
+            if (b->particle_sigma[ps] < 0.f || b->particle_epsilon[ps] < 0.f) throw LimaFail(LIMA_ERR_BAD_INPUT, "negative LJ parameter");
+            prm[i] = make_float4(q, 0.5f * b->particle_sigma[ps], sqrtf(b->particle_epsilon[ps]), m);
+            comp[i] = c;
+            qv[i] = q;
+        }
+    }
+    for (int t = 0; t < e.nT; t++) {
+        const int64_t i = e.h_first[e.nC] + t;
+        int64_t u[3];
+        for (int d = 0; d < 3; d++) u[d] = (int64_t)b->tinymol_block[t * 3 + d] * 100000000ll + b->tinymol_coord[t * 3 + d];
+        put(i, u);
+        const float* pv = b->tinymol_velocity ? &b->tinymol_velocity[t * 3] : nullptr;
+        v[i] = make_float4(pv ? pv[0] : 0.f, pv ? pv[1] : 0.f, pv ? pv[2] : 0.f, 0.f);
+        if (!(b->tinymol_mass[t] > 0.f)) throw LimaFail(LIMA_ERR_BAD_INPUT, "tinymol mass must be positive");
+        prm[i] = make_float4(0.f, 0.5f * b->tinymol_sigma[t], sqrtf(b->tinymol_epsilon[t]), b->tinymol_mass[t]);
+        comp[i] = e.nC + t;
+        qv[i] = 0.f;
+    }
+    e.h_comp = comp;
+    e.p.n = N;
+    e.p.w = dalloc_copy(w);
+    e.p.img = dalloc_copy(img);
+    e.p.v = dalloc_copy(v);
+    e.p.fprev = dalloc_copy(zero4);
+    e.p.prm = dalloc_copy(prm);
+    e.p.comp = dalloc_copy(comp);
+    HIP_CHECK(hipMalloc(&e.w_try, N * sizeof(uint4)));
+    HIP_CHECK(hipMalloc(&e.img_try, N * sizeof(int4)));
+    e.f_acc = dalloc_copy(zero4);
+    HIP_CHECK(hipMalloc(&e.sd, sizeof(SdState)));
+    // ---- compounds, exclusions, intra work items ----
+    std::vector<uint8_t> bm(std::max(e.nC, 1), 0);
+    const uint32_t* lut = b->bonded_lut;
+    auto lutrow = [&](int c, int k, int i) -> uint32_t { return lut ? lut[((size_t)c * 5 + k) * 32 + i] : (k == 2 ? (1u << i) : 0u); };
+    for (int c = 0; c < e.nC; c++) {
+        bm[c] = 1u << 2;
+        for (int k = 0; k < 5; k++) {
+            const int B = c + k - 2;
+            if (k == 2 || B < 0 || B >= e.nC) continue;
+            bool any = false;
+            for (int i = 0; i < 32; i++) any |= lutrow(c, k, i) != 0;
+            if (any) bm[c] |= (uint8_t)(1u << k);
+        }
+    }
+    std::vector<int4> items;
+    std::vector<uint32_t> ilut;
+    for (int A = 0; A < e.nC; A++) {
+        const int nA = b->compound_n_particles[A];
+        for (int k = 0; k < 5; k++) {
+            if (!((bm[A] >> k) & 1)) continue;
+            const int B = A + k - 2;
+            const int nB = b->compound_n_particles[B];
+            bool any = false;
+            uint32_t rows[32];
+            for (int i = 0; i < 32; i++) {
+                uint32_t r = lutrow(A, k, i);
+                if (A == B) r |= 1u << i;
+                rows[i] = r;
+                if (i < nA)
+                    for (int j = 0; j < nB; j++) any |= !((r >> j) & 1u);
+            }
+            if (!any) continue;
+            items.push_back(make_int4(A, B, 0, 0));
+            for (int i = 0; i < 32; i++) ilut.push_back(rows[i]);
+        }
+    }
+    e.n_intra = (int)items.size();
+    e.intra_items = dalloc_copy(items);
+    e.intra_lut = dalloc_copy(ilut);
+    e.first = dalloc_copy(e.h_first);
+    e.bmask = dalloc_copy(bm);
+    // ---- bonded terms ----
+    auto gid = [&](int32_t id) -> int {
+        const int c = id / 32, s = id % 32;
+        if (id < 0 || c >= e.nC || s >= b->compound_n_particles[c]) throw LimaFail(LIMA_ERR_BAD_INPUT, "bonded particle id out of range");
+        return (int)(e.h_first[c] + s);
+    };
+    {
+        std::vector<int2> bid(b->n_bonds); std::vector<float2> bp(b->n_bonds);
+        for (int k = 0; k < b->n_bonds; k++) {
+            bid[k] = make_int2(gid(b->bond_ids[2 * k]), gid(b->bond_ids[2 * k + 1]));
+            bp[k] = make_float2(b->bond_params[2 * k], b->bond_params[2 * k + 1]);
+        }
+        std::vector<int4> aid(b->n_angles); std::vector<float4> ap(b->n_angles);
+        for (int k = 0; k < b->n_angles; k++) {
+            aid[k] = make_int4(gid(b->angle_ids[3 * k]), gid(b->angle_ids[3 * k + 1]), gid(b->angle_ids[3 * k + 2]), 0);
+            ap[k] = make_float4(b->angle_params[4 * k], b->angle_params[4 * k + 1], b->angle_params[4 * k + 2], b->angle_params[4 * k + 3]);
+        }
+        std::vector<int4> did(b->n_dihedrals); std::vector<float4> dp(b->n_dihedrals);
+        for (int k = 0; k < b->n_dihedrals; k++) {
+            did[k] = make_int4(gid(b->dihedral_ids[4 * k]), gid(b->dihedral_ids[4 * k + 1]), gid(b->dihedral_ids[4 * k + 2]),
+                               gid(b->dihedral_ids[4 * k + 3]));
+            dp[k] = make_float4(b->dihedral_params[3 * k], b->dihedral_params[3 * k + 1], b->dihedral_params[3 * k + 2], 0.f);
+        }
+        std::vector<int4> iid(b->n_impropers); std::vector<float2> ip(b->n_impropers);
+        for (int k = 0; k < b->n_impropers; k++) {
+            iid[k] = make_int4(gid(b->improper_ids[4 * k]), gid(b->improper_ids[4 * k + 1]), gid(b->improper_ids[4 * k + 2]),
+                               gid(b->improper_ids[4 * k + 3]));
+            ip[k] = make_float2(b->improper_params[2 * k], b->improper_params[2 * k + 1]);
+        }
+        std::vector<int2> pid(b->n_pairs); std::vector<float2> pp(b->n_pairs);
+        for (int k = 0; k < b->n_pairs; k++) {
+            pid[k] = make_int2(gid(b->pair_ids[2 * k]), gid(b->pair_ids[2 * k + 1]));
+            pp[k] = make_float2(b->pair_params[2 * k], b->pair_params[2 * k + 1]);
+        }
+        e.bd.nb = b->n_bonds; e.bd.na = b->n_angles; e.bd.nd = b->n_dihedrals; e.bd.ni = b->n_impropers; e.bd.np = b->n_pairs;
+        e.bd.bid = dalloc_copy(bid); e.bd.bp = dalloc_copy(bp);
+        e.bd.aid = dalloc_copy(aid); e.bd.ap = dalloc_copy(ap);
+        e.bd.did = dalloc_copy(did); e.bd.dp = dalloc_copy(dp);
+        e.bd.iid = dalloc_copy(iid); e.bd.ip = dalloc_copy(ip);
+        e.bd.pid = dalloc_copy(pid); e.bd.pp = dalloc_copy(pp);
+    }
+    HIP_CHECK(hipMalloc(&e.acc, ACC_N * sizeof(double)));
+    HIP_CHECK(hipMalloc(&e.fmax, sizeof(float)));
+    HIP_CHECK(hipStreamCreate(&e.s_main));
+    HIP_CHECK(hipStreamCreate(&e.s_pme));
+    for (auto& ev : e.tev) HIP_CHECK(hipEventCreate(&ev));
+    nlist_alloc(e);
+    pme_alloc(e, qv);
+    nlist_build(e, e.p.w);
+    HIP_CHECK(hipStreamSynchronize(e.s_main));
+}
+
+void destroy(Engine& e) {
+    if (e.s_main) (void)hipStreamSynchronize(e.s_main);
+    nlist_free(e);
+    pme_free(e);
+    void* ptrs[] = {e.p.w, e.p.img, e.p.v, e.p.fprev, e.p.prm, e.p.comp, e.w_try, e.img_try, e.f_acc, e.sd, e.first,
+                    e.bmask, e.intra_items, e.intra_lut, e.bd.bid, e.bd.bp, e.bd.aid, e.bd.ap, e.bd.did, e.bd.dp,
+                    e.bd.iid, e.bd.ip, e.bd.pid, e.bd.pp, e.acc, e.fmax};
+    for (void* p : ptrs) if (p) (void)hipFree(p);
+    for (auto& ev : e.tev) if (ev) (void)hipEventDestroy(ev);
+    if (e.s_main) (void)hipStreamDestroy(e.s_main);
+    if (e.s_pme) (void)hipStreamDestroy(e.s_pme);
+}
+
+void forces_global(Engine& e) {
+    hipLaunchKernelGGL(k_zero_f, dim3(1), dim3(64), 0, e.s_main, e.fmax, 1);
+    hipLaunchKernelGGL(k_forces_global, dim3(nblk(e.N, 256)), dim3(256), 0, e.s_main, e.N, e.nl.ncl * CLUSTER, e.nl.slot_of,
+                       e.fs, e.es, e.f_acc, e.fmax);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* lima_last_error(void) { return g_err.c_str(); }
+
+lima_status lima_create(const lima_box_desc* box, const lima_simparams* params, lima_engine** out) {
+    if (!out) { set_err("null output handle"); return LIMA_ERR_BAD_INPUT; }
+    *out = nullptr;
+    auto* h = new lima_engine();
+    const lima_status st = guarded([&] { create(h->e, box, params); });
+    if (st != LIMA_OK) {
+        destroy(h->e);
+        delete h;
+        return st;
+    }
+    *out = h;
+    return LIMA_OK;
+}
+
+lima_status lima_destroy(lima_engine* h) {
+    if (!h) return LIMA_OK;
+    destroy(h->e);
+    delete h;
+    return LIMA_OK;
+}
+
+lima_status lima_step(lima_engine* h, int64_t nsteps) {
+    if (!h) { set_err("null engine"); return LIMA_ERR_BAD_INPUT; }
+    if (h->e.critical) { set_err("engine is in a critical error state"); return LIMA_ERR_NONFINITE; }
+    return guarded([&] { md_steps(h->e, nsteps); });
+}
+
+lima_status lima_compute_forces(lima_engine* h, lima_energy* out) {
+    if (!h) { set_err("null engine"); return LIMA_ERR_BAD_INPUT; }
+    return guarded([&] {
+        Engine& e = h->e;
+        nlist_build(e, e.p.w);
+        evaluate(e, e.p.w, e.p.img, true);
+        hipLaunchKernelGGL(k_kinetic, dim3(nblk(e.N, 256)), dim3(256), 0, e.s_main, e.N, e.p.v, e.p.prm, e.acc);
+        HIP_CHECK(hipStreamSynchronize(e.s_main));
+        HIP_CHECK(hipGetLastError());
+        fill_energy(e, true);
+        e.forces_valid = true;
+        if (out) *out = e.energy;
+    });
+}
+
+lima_status lima_get_energy(lima_engine* h, lima_energy* out) {
+    if (!h || !out) { set_err("null argument"); return LIMA_ERR_BAD_INPUT; }
+    *out = h->e.energy;
+    return LIMA_OK;
+}
+
+lima_status lima_minimize(lima_engine* h, double tol_kj, int64_t max_iter, double h0, lima_em_result* res) {
+    if (!h) { set_err("null engine"); return LIMA_ERR_BAD_INPUT; }
+    return guarded([&] {
+        Engine& e = h->e;
+        const auto t0 = std::chrono::steady_clock::now();
+        hipStream_t s = e.s_main;
+        // initial evaluation at the current positions
+        nlist_build(e, e.p.w);
+        evaluate(e, e.p.w, e.p.img, true);
+        forces_global(e);
+        SdState sd{};
+        {
+            double acc[ACC_N];
+            float fm;
+            HIP_CHECK(hipMemcpyAsync(acc, e.acc, sizeof(acc), hipMemcpyDeviceToHost, s));
+            HIP_CHECK(hipMemcpyAsync(&fm, e.fmax, sizeof(float), hipMemcpyDeviceToHost, s));
+            HIP_CHECK(hipStreamSynchronize(s));
+            sd.e_acc = acc[ACC_LJ] + acc[ACC_COUL] + acc[ACC_BOND] + acc[ACC_PME] + e_const(e);
+            sd.fmax_acc = fm;
+            sd.h = (float)h0;
+        }
+        HIP_CHECK(hipMemcpyAsync(e.sd, &sd, sizeof(sd), hipMemcpyHostToDevice, s));
+        const double tol = tol_kj * 1000.0;
+        int64_t it = 0;
+        bool conv = sd.fmax_acc < tol;
+        while (!conv && it < max_iter) {
+            if (sd.disp_since_build + sd.h > 0.5 * e.skin) {
+                nlist_build(e, e.p.w);
+                sd.disp_since_build = 0;
+                HIP_CHECK(hipMemcpyAsync(&e.sd->disp_since_build, &sd.disp_since_build, sizeof(double), hipMemcpyHostToDevice, s));
+            }
+            hipLaunchKernelGGL(k_sd_trial, dim3(nblk(e.N, 256)), dim3(256), 0, s, e.N, (long long)e.Lq[0], (long long)e.Lq[1],
+                               (long long)e.Lq[2], e.sd, e.p.w, e.p.img, e.f_acc, e.w_try, e.img_try);
+            evaluate(e, e.w_try, e.img_try, true);
+            // trial forces in global order -> p.fprev used as scratch
+            hipLaunchKernelGGL(k_zero_f, dim3(1), dim3(64), 0, s, e.fmax, 1);
+            hipLaunchKernelGGL(k_forces_global, dim3(nblk(e.N, 256)), dim3(256), 0, s, e.N, e.nl.ncl * CLUSTER, e.nl.slot_of,
+                               e.fs, e.es, e.p.fprev, e.fmax);
+            hipLaunchKernelGGL(k_sd_accept, dim3(1), dim3(1), 0, s, e.sd, e.acc, e_const(e), e.fmax);
+            hipLaunchKernelGGL(k_sd_commit, dim3(nblk(e.N, 256)), dim3(256), 0, s, e.N, e.sd, e.w_try, e.img_try, e.p.fprev,
+                               e.p.w, e.p.img, e.f_acc);
+            HIP_CHECK(hipMemcpyAsync(&sd, e.sd, sizeof(sd), hipMemcpyDeviceToHost, s));
+            HIP_CHECK(hipStreamSynchronize(s));
+            it++;
+            conv = sd.fmax_acc < tol;
+            if (!std::isfinite(sd.e_acc)) throw LimaFail(LIMA_ERR_NONFINITE, "non-finite energy in minimisation");
+            if (sd.h < 1e-12f) break;
+        }
+        // leave forces / energies of the accepted state in the engine
+        evaluate(e, e.p.w, e.p.img, true);
+        HIP_CHECK(hipMemsetAsync(e.p.fprev, 0, e.N * sizeof(float4), s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        fill_energy(e, false);
+        e.forces_valid = true;
+        if (res) {
+            res->iterations = it;
+            res->accepted = sd.n_accepted;
+            res->final_energy = sd.e_acc;
+            res->final_fmax = sd.fmax_acc / 1000.0;
+            res->converged = conv ? 1 : 0;
+            res->wall_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        }
+    });
+}
+
+lima_status lima_get_state(lima_engine* h, lima_state_out* out) {
+    if (!h || !out) { set_err("null argument"); return LIMA_ERR_BAD_INPUT; }
+    return guarded([&] {
+        Engine& e = h->e;
+        const int64_t N = e.N;
+        std::vector<uint4> w(N);
+        std::vector<int4> img(N);
+        std::vector<float4> v(N), f(N);
+        HIP_CHECK(hipMemcpy(w.data(), e.p.w, N * sizeof(uint4), hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(img.data(), e.p.img, N * sizeof(int4), hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(v.data(), e.p.v, N * sizeof(float4), hipMemcpyDeviceToHost));
+        if (e.forces_valid) {
+            forces_global(e);
+            HIP_CHECK(hipStreamSynchronize(e.s_main));
+            HIP_CHECK(hipMemcpy(f.data(), e.f_acc, N * sizeof(float4), hipMemcpyDeviceToHost));
+        } else {
+            std::fill(f.begin(), f.end(), make_float4(0.f, 0.f, 0.f, 0.f));
+        }
+        auto unw = [&](int64_t i, int d) -> int64_t {
+            const uint32_t ww = d == 0 ? w[i].x : (d == 1 ? w[i].y : w[i].z);
+            const int im = d == 0 ? img[i].x : (d == 1 ? img[i].y : img[i].z);
+            return (int64_t)ww + (int64_t)im * e.Lq[d];
+        };
+        auto wr = [&](int64_t i, int d) -> int64_t { return d == 0 ? w[i].x : (d == 1 ? w[i].y : w[i].z); };
+        for (int c = 0; c < e.nC; c++) {
+            const int64_t k = e.h_first[c] + e.h_key[c];
+            int64_t base[3];
+            for (int d = 0; d < 3; d++) {
+                const int64_t wk = wr(k, d);
+                const int32_t org = (int32_t)(wk / 100000000ll);
+                if (out->compound_origo) out->compound_origo[c * 3 + d] = org;
+                base[d] = unw(k, d) - wk + (int64_t)org * 100000000ll;
+            }
+            for (int s = 0; s < (int)(e.h_first[c + 1] - e.h_first[c]); s++) {
+                const int64_t i = e.h_first[c] + s;
+                const size_t ps = (size_t)c * 32 + s;
+                const float vv[3] = {v[i].x, v[i].y, v[i].z}, ff[3] = {f[i].x, f[i].y, f[i].z};
+                for (int d = 0; d < 3; d++) {
+                    if (out->particle_coord) out->particle_coord[ps * 3 + d] = (int32_t)(unw(i, d) - base[d]);
+                    if (out->particle_velocity) out->particle_velocity[ps * 3 + d] = vv[d];
+                    if (out->particle_force) out->particle_force[ps * 3 + d] = ff[d];
+                }
+                if (out->particle_pot_energy) out->particle_pot_energy[ps] = f[i].w;
+            }
+        }
+        for (int t = 0; t < e.nT; t++) {
+            const int64_t i = e.h_first[e.nC] + t;
+            const float vv[3] = {v[i].x, v[i].y, v[i].z}, ff[3] = {f[i].x, f[i].y, f[i].z};
+            for (int d = 0; d < 3; d++) {
+                const int64_t ww = wr(i, d);
+                const int32_t blk = (int32_t)(ww / 100000000ll);
+                if (out->tinymol_block) out->tinymol_block[t * 3 + d] = blk;
+                if (out->tinymol_coord) out->tinymol_coord[t * 3 + d] = (int32_t)(ww - (int64_t)blk * 100000000ll);
+                if (out->tinymol_velocity) out->tinymol_velocity[t * 3 + d] = vv[d];
+                if (out->tinymol_force) out->tinymol_force[t * 3 + d] = ff[d];
+            }
+            if (out->tinymol_pot_energy) out->tinymol_pot_energy[t] = f[i].w;
+        }
+    });
+}
+
+lima_status lima_get_status(lima_engine* h, lima_runstatus* out) {
+    if (!h || !out) { set_err("null argument"); return LIMA_ERR_BAD_INPUT; }
+    const Engine& e = h->e;
+    std::memset(out, 0, sizeof(*out));
+    out->current_step = e.step;
+    const double dof = 3.0 * (double)e.N - 3.0;
+    out->current_temperature = dof > 0 ? (float)(2.0 * e.energy.kinetic / (dof * (double)1.38066e-23f * 6.02214076e23)) : 0.f;
+    out->critical_error_occured = e.critical ? 1 : 0;
+    out->simulation_finished = (e.prm.n_steps > 0 && e.step >= e.prm.n_steps) ? 1 : 0;
+    out->n_list_rebuilds = e.n_rebuilds;
+    out->greatest_force = NAN;
+    return LIMA_OK;
+}
+
+lima_status lima_get_neighbor_counts(lima_engine* h, int64_t* ncl, int64_t* nent) {
+    if (!h || !ncl || !nent) { set_err("null argument"); return LIMA_ERR_BAD_INPUT; }
+    return guarded([&] {
+        Engine& e = h->e;
+        nlist_build(e, e.p.w);
+        HIP_CHECK(hipStreamSynchronize(e.s_main));
+        std::vector<float4> ent(e.nl.nent);
+        if (e.nl.nent) HIP_CHECK(hipMemcpy(ent.data(), e.nl.ent, e.nl.nent * sizeof(float4), hipMemcpyDeviceToHost));
+        int64_t n = 0;
+        for (auto& r : ent) {
+            uint32_t bits;
+            std::memcpy(&bits, &r.w, 4);
+            if (bits >> 24) n++;
+        }
+        *ncl = e.nl.ncl;
+        *nent = n;
+    });
+}
+
+lima_status lima_get_neighbor_list(lima_engine* h, int32_t* cluster_atoms, int64_t* offsets, uint32_t* entries, uint8_t* masks) {
+    if (!h) { set_err("null engine"); return LIMA_ERR_BAD_INPUT; }
+    return guarded([&] {
+        Engine& e = h->e;
+        const int ncl = e.nl.ncl;
+        std::vector<int> cat((size_t)ncl * CLUSTER), eoff(ncl + 1);
+        std::vector<float4> ent(e.nl.nent);
+        std::vector<uint8_t> code(e.nl.nent);
+        HIP_CHECK(hipMemcpy(cat.data(), e.nl.catoms, cat.size() * sizeof(int), hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(eoff.data(), e.nl.eoff, (ncl + 1) * sizeof(int), hipMemcpyDeviceToHost));
+        if (e.nl.nent) {
+            HIP_CHECK(hipMemcpy(ent.data(), e.nl.ent, e.nl.nent * sizeof(float4), hipMemcpyDeviceToHost));
+            HIP_CHECK(hipMemcpy(code.data(), e.nl.ecode, e.nl.nent, hipMemcpyDeviceToHost));
+        }
+        if (cluster_atoms) std::memcpy(cluster_atoms, cat.data(), cat.size() * sizeof(int));
+        int64_t n = 0;
+        for (int I = 0; I < ncl; I++) {
+            if (offsets) offsets[I] = n;
+            for (int k = eoff[I]; k < eoff[I + 1]; k++) {
+                uint32_t bits;
+                std::memcpy(&bits, &ent[k].w, 4);
+                const uint32_t m = bits >> 24;
+                if (!m) continue;
+                const int slot = (int)(bits & 0xffffffu);
+                if (entries) entries[n] = (uint32_t)cat[slot] | ((uint32_t)code[k] << 24);
+                if (masks) masks[n] = (uint8_t)m;
+                n++;
+            }
+        }
+        if (offsets) offsets[ncl] = n;
+    });
+}
+
+lima_status lima_get_timings(lima_engine* h, lima_timings* out) {
+    if (!h || !out) { set_err("null argument"); return LIMA_ERR_BAD_INPUT; }
+    *out = h->e.tacc;
+    return LIMA_OK;
+}
+
+lima_status lima_set_timing(lima_engine* h, int32_t enabled) {
+    if (!h) { set_err("null engine"); return LIMA_ERR_BAD_INPUT; }
+    h->e.timing = enabled != 0;
+    h->e.tacc = lima_timings{};
+    return LIMA_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,416 @@
+// forces.hip -- nonbonded and bonded force kernels (FP32 arithmetic, FP64 energy totals).
+//
+// Pair kernel: LIMA's cutoff pair path (compoundFarneighborShortrangeInteractionsKernel,
+// code/LIMA_ENGINE/src/EngineKernels.cuh:47-146 with LennardJonesInteractions.cuh:164-203).
+// A wave owns one i-cluster (8 particles) and walks its entry list 8 entries at a time:
+// lane = 8*jj + ii computes pair (i-slot ii, entry jj), so one wave instruction covers 64 pairs.
+// Newton's third law is applied per pair; the j force is summed over the 8 lanes that share
+// an entry with three DPP adds and added to the j slot with one float atomic per component.
+// FP32 VALU bound: about 55 VALU slots per evaluated pair.
+//
+// Intra kernel: LIMA's immediate-neighbour path (EngineKernels.cuh:150-263,
+// LennardJonesInteractions.cuh:121-161): pairs of the same or bonded compounds, BondedParticlesLUT
+// exclusions, no cutoff, unwrapped displacement, force on self only (as LIMA evaluates it).
+//
+// Bonded kernel: ForceComputations.cuh:17-264,335-376; one thread per term, forces added
+// to the particles' cluster slots with float atomics.
+#include "engine.h"
+
+namespace lima {
+
+namespace {
+
+inline int nblk(int64_t n, int b) { return (int)((n + b - 1) / b); }
+
+// block index remap so each XCD (blocks dispatched round-robin over 8 XCDs) gets a contiguous,
+// spatially coherent range of clusters and so shares j data in its own L2.
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+    const int q = nb >> 3, r = nb & 7;
+    const int x = b & 7, k = b >> 3;
+    return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
+__global__ void k_pack(int nslots, const int* __restrict__ catoms, const uint4* __restrict__ w, const float4* __restrict__ prm,
+                       const uint4* __restrict__ cref, long long Lqx, long long Lqy, long long Lqz,
+                       float4* __restrict__ xq, float* __restrict__ fs, float* __restrict__ es) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nslots) return;
+    const int a = catoms[k];
+    float4 out = make_float4(PAD_POS, PAD_POS, PAD_POS, 0.f);
+    if (a >= 0) {
+        const uint4 p = w[a];
+        const uint4 r = cref[k >> 3];
+        long long d[3] = {(long long)p.x - (long long)r.x, (long long)p.y - (long long)r.y, (long long)p.z - (long long)r.z};
+        const long long L[3] = {Lqx, Lqy, Lqz};
+        for (int i = 0; i < 3; i++) {
+            if (2 * d[i] > L[i]) d[i] -= L[i];
+            else if (2 * d[i] < -L[i]) d[i] += L[i];
+        }
+        out = make_float4((float)d[0] * LIMA_LIMA_TO_NANO_F, (float)d[1] * LIMA_LIMA_TO_NANO_F, (float)d[2] * LIMA_LIMA_TO_NANO_F, prm[a].x);
+    }
+    xq[k] = out;
+    fs[k] = 0.f;
+    fs[nslots + k] = 0.f;
+    fs[2 * nslots + k] = 0.f;
+    es[k] = 0.f;
+}
+
+struct PairArgs {
+    int ncl, nslots;
+    float rc2;
+    PairConst pc;
+    const float4* xq;
+    const float2* ljs;
+    const int* eoff;
+    const float4* ent;
+    float* fs;
+    float* es;
+    double* acc;
+};
+
+template <bool ENERGY>
+__global__ __launch_bounds__(256) void k_pair(PairArgs a) {
+    const int lane = threadIdx.x & 63;
+    const int I = xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+    if (I >= a.ncl) return;
+    const int ii = lane & 7, jj = lane >> 3;
+    const int islot = I * CLUSTER + ii;
+    const float4 xi = a.xq[islot];
+    const float2 pi = a.ljs[islot];
+    float fx = 0.f, fy = 0.f, fz = 0.f, ei = 0.f, elj = 0.f, ecl = 0.f;
+    const int e0 = a.eoff[I], e1 = a.eoff[I + 1];
+    for (int e = e0 + jj; e < e1; e += CLUSTER) {
+        const float4 rec = a.ent[e];
+        const uint32_t bits = __float_as_uint(rec.w);
+        const int b = (int)(bits & 0xffffffu);
+        const uint32_t m = bits >> 24;
+        const float4 xj = a.xq[b];
+        const float dx = (xj.x + rec.x) - xi.x, dy = (xj.y + rec.y) - xi.y, dz = (xj.z + rec.z) - xi.z;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        const bool inc = ((m >> ii) & 1u) && (r2 <= a.rc2) && (r2 > 0.f);
+        if (__ballot(inc) == 0ull) continue;
+        const float2 pj = a.ljs[b];
+        float el = 0.f, ec = 0.f;
+        float S = pair_scalar(inc ? r2 : 1.0f, pi.x + pj.x, pi.y * pj.y, xi.w * xj.w, a.pc, ENERGY ? &el : nullptr, &ec);
+        S = inc ? S : 0.f;
+        float tx = dx * S, ty = dy * S, tz = dz * S;
+        fx += tx; fy += ty; fz += tz;
+        tx = sum8(tx); ty = sum8(ty); tz = sum8(tz);
+        float ej = 0.f;
+        if (ENERGY) {
+            el = inc ? el : 0.f;
+            ec = inc ? ec : 0.f;
+            elj += el; ecl += ec;
+            ei += 0.5f * (el + ec);
+            ej = sum8(0.5f * (el + ec));
+        }
+        if (ii == 0) {
+            atomicAdd(&a.fs[b], -tx);
+            atomicAdd(&a.fs[a.nslots + b], -ty);
+            atomicAdd(&a.fs[2 * a.nslots + b], -tz);
+            if (ENERGY) atomicAdd(&a.es[b], ej);
+        }
+    }
+    // i force: lanes ii, ii+8, ..., ii+56 hold partial sums
+    for (int o = 8; o < 64; o <<= 1) {
+        fx += __shfl_xor(fx, o, 64);
+        fy += __shfl_xor(fy, o, 64);
+        fz += __shfl_xor(fz, o, 64);
+        if (ENERGY) ei += __shfl_xor(ei, o, 64);
+    }
+    if (jj == 0) {
+        atomicAdd(&a.fs[islot], fx);
+        atomicAdd(&a.fs[a.nslots + islot], fy);
+        atomicAdd(&a.fs[2 * a.nslots + islot], fz);
+        if (ENERGY) atomicAdd(&a.es[islot], ei);
+    }
+    if (ENERGY) {
+        const float tl = wave_sum(elj), tc = wave_sum(ecl);
+        if (lane == 0) {
+            atomicAdd(&a.acc[ACC_LJ], (double)tl);
+            atomicAdd(&a.acc[ACC_COUL], (double)tc);
+        }
+    }
+}
+
+struct IntraArgs {
+    int n_items, nslots;
+    long long Lq[3];
+    PairConst pc;
+    const int4* items;
+    const uint32_t* lut;
+    const uint4* w;
+    const int4* img;
+    const float4* prm;
+    const int* slot_of;
+    const int64_t* first;
+    float* fs;
+    float* es;
+    double* acc;
+};
+
+__device__ __forceinline__ void unwrapped3(const uint4* w, const int4* img, const long long* Lq, int a, long long* u) {
+    const uint4 p = w[a];
+    const int4 m = img[a];
+    u[0] = (long long)p.x + (long long)m.x * Lq[0];
+    u[1] = (long long)p.y + (long long)m.y * Lq[1];
+    u[2] = (long long)p.z + (long long)m.z * Lq[2];
+}
+
+// One wave per (A, B) item: lanes 0..31 take i = lane with j in [0,16), lanes 32..63 the same i
+// with j in [16,32).
+template <bool ENERGY>
+__global__ __launch_bounds__(256) void k_intra(IntraArgs a) {
+    const int lane = threadIdx.x & 63;
+    const int it = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (it >= a.n_items) return;
+    const int4 item = a.items[it];
+    const int A = item.x, B = item.y;
+    const int nA = (int)(a.first[A + 1] - a.first[A]), nB = (int)(a.first[B + 1] - a.first[B]);
+    const int i = lane & 31, half = lane >> 5;
+    float fx = 0.f, fy = 0.f, fz = 0.f, el_s = 0.f, ec_s = 0.f;
+    const int ai = (int)a.first[A] + i;
+    if (i < nA) {
+        long long ui[3];
+        unwrapped3(a.w, a.img, a.Lq, ai, ui);
+        const float4 pi = a.prm[ai];
+        const uint32_t row = a.lut[it * 32 + i];
+        for (int j = half * 16; j < half * 16 + 16 && j < nB; j++) {
+            if ((row >> j) & 1u) continue;
+            const int bj = (int)a.first[B] + j;
+            long long uj[3];
+            unwrapped3(a.w, a.img, a.Lq, bj, uj);
+            const float dx = (float)(uj[0] - ui[0]) * LIMA_LIMA_TO_NANO_F;
+            const float dy = (float)(uj[1] - ui[1]) * LIMA_LIMA_TO_NANO_F;
+            const float dz = (float)(uj[2] - ui[2]) * LIMA_LIMA_TO_NANO_F;
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            if (!(r2 > 0.f)) continue;
+            const float4 pj = a.prm[bj];
+            float el = 0.f, ec = 0.f;
+            const float S = pair_scalar(r2, pi.y + pj.y, pi.z * pj.z, pi.x * pj.x, a.pc, ENERGY ? &el : nullptr, &ec);
+            fx += dx * S; fy += dy * S; fz += dz * S;
+            if (ENERGY) { el_s += 0.5f * el; ec_s += 0.5f * ec; }
+        }
+    }
+    fx += __shfl_xor(fx, 32, 64);
+    fy += __shfl_xor(fy, 32, 64);
+    fz += __shfl_xor(fz, 32, 64);
+    float e_i = 0.f;
+    if (ENERGY) e_i = __shfl_xor(el_s + ec_s, 32, 64) + el_s + ec_s;
+    if (half == 0 && i < nA) {
+        const int s = a.slot_of[ai];
+        atomicAdd(&a.fs[s], fx);
+        atomicAdd(&a.fs[a.nslots + s], fy);
+        atomicAdd(&a.fs[2 * a.nslots + s], fz);
+        if (ENERGY) atomicAdd(&a.es[s], e_i);
+    }
+    if (ENERGY) {
+        const float tl = wave_sum(el_s), tc = wave_sum(ec_s);
+        if (lane == 0) {
+            atomicAdd(&a.acc[ACC_LJ], (double)tl);
+            atomicAdd(&a.acc[ACC_COUL], (double)tc);
+        }
+    }
+}
+
+struct BondArgs {
+    int nb, na, nd, ni, np, nslots;
+    long long Lq[3];
+    PairConst pc;
+    const int2* bid; const float2* bp;
+    const int4* aid; const float4* ap;
+    const int4* did; const float4* dp;
+    const int4* iid; const float2* ip;
+    const int2* pid; const float2* pp;
+    const uint4* w;
+    const int4* img;
+    const int* slot_of;
+    float* fs;
+    float* es;
+    double* acc;
+};
+
+struct v3 {
+    float x, y, z;
+};
+__device__ __forceinline__ v3 operator+(v3 a, v3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ v3 operator-(v3 a, v3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ v3 operator*(v3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ v3 operator-(v3 a) { return {-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ float dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ v3 cross(v3 a, v3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+__device__ __forceinline__ float len(v3 a) { return sqrtf(dot(a, a)); }
+__device__ __forceinline__ v3 nrm(v3 a) { return a * (1.0f / len(a)); }
+
+__device__ __forceinline__ v3 bdisp(const BondArgs& a, int from, int to) {   // unwrapped x_to - x_from [nm]
+    long long u0[3], u1[3];
+    unwrapped3(a.w, a.img, a.Lq, from, u0);
+    unwrapped3(a.w, a.img, a.Lq, to, u1);
+    return {(float)(u1[0] - u0[0]) * LIMA_LIMA_TO_NANO_F, (float)(u1[1] - u0[1]) * LIMA_LIMA_TO_NANO_F,
+            (float)(u1[2] - u0[2]) * LIMA_LIMA_TO_NANO_F};
+}
+
+__device__ __forceinline__ void addf(const BondArgs& a, int p, v3 f) {
+    const int s = a.slot_of[p];
+    atomicAdd(&a.fs[s], f.x);
+    atomicAdd(&a.fs[a.nslots + s], f.y);
+    atomicAdd(&a.fs[2 * a.nslots + s], f.z);
+}
+
+template <bool ENERGY>
+__global__ __launch_bounds__(256) void k_bonded(BondArgs a) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    float e = 0.f;
+    int ids[4];
+    int nat = 0;
+    if (t < a.nb) {   // calcSinglebondForces, ForceComputations.cuh:17-51
+        const int2 id = a.bid[t];
+        const float2 p = a.bp[t];
+        const v3 d = bdisp(a, id.y, id.x);   // pos_a - pos_b
+        const float r = len(d), err = r - p.x;
+        const v3 f = d * (-p.y * err / r);
+        addf(a, id.x, f);
+        addf(a, id.y, -f);
+        e = 0.5f * p.y * err * err;
+        ids[0] = id.x; ids[1] = id.y; nat = 2;
+    } else if (t < a.nb + a.na) {   // calcAnglebondForces :53-115, harmonic in theta + Urey-Bradley
+        const int k = t - a.nb;
+        const int4 id = a.aid[k];
+        const float4 p = a.ap[k];
+        const v3 r1 = bdisp(a, id.y, id.x), r2 = bdisp(a, id.y, id.z);
+        const float l1 = len(r1), l2 = len(r2);
+        const v3 v1 = r1 * (1.f / l1), v2 = r2 * (1.f / l2);
+        const float c = fminf(1.f, fmaxf(-1.f, dot(v1, v2)));
+        const float th = acosf(c), err = th - p.x;
+        const float s = fmaxf(sqrtf(fmaxf(0.f, 1.f - c * c)), 1e-6f);
+        const float tq = p.y * err;
+        v3 fl = (v2 - v1 * c) * (tq / (l1 * s));
+        v3 fr = (v1 - v2 * c) * (tq / (l2 * s));
+        e = 0.5f * p.y * err * err;
+        if (p.w != 0.f) {
+            const v3 d13 = bdisp(a, id.z, id.x);   // left - right
+            const float r13 = len(d13), eu = r13 - p.z;
+            const v3 fu = d13 * (-p.w * eu / r13);
+            fl = fl + fu;
+            fr = fr - fu;
+            e += 0.5f * p.w * eu * eu;
+        }
+        addf(a, id.x, fl);
+        addf(a, id.z, fr);
+        addf(a, id.y, -(fl + fr));
+        ids[0] = id.x; ids[1] = id.y; ids[2] = id.z; nat = 3;
+    } else if (t < a.nb + a.na + a.nd) {   // calcDihedralbondForces :119-210, phi_L = -atan2(sin, cos)
+        const int k = t - a.nb - a.na;
+        const int4 id = a.did[k];
+        const float4 p = a.dp[k];
+        const v3 b1 = bdisp(a, id.x, id.y), b2 = bdisp(a, id.y, id.z), b3 = bdisp(a, id.z, id.w);
+        const v3 m = cross(b1, b2), n = cross(b2, b3);
+        const float lb2 = len(b2);
+        const float phi = -atan2f(lb2 * dot(b1, n), dot(m, n));
+        const float arg = p.z * phi - p.x;
+        e = p.y * (1.f + cosf(arg));
+        const float dv = -p.y * p.z * sinf(arg);   // dV/dphi_L
+        const v3 g1 = m * (-lb2 / dot(m, m));
+        const v3 g4 = n * (lb2 / dot(n, n));
+        const float pp = dot(b1, b2) / (lb2 * lb2), qq = dot(b3, b2) / (lb2 * lb2);
+        const v3 g2 = g1 * (-(pp + 1.f)) + g4 * qq;
+        const v3 g3 = g1 * pp - g4 * (qq + 1.f);
+        addf(a, id.x, g1 * dv);
+        addf(a, id.y, g2 * dv);
+        addf(a, id.z, g3 * dv);
+        addf(a, id.w, g4 * dv);
+        ids[0] = id.x; ids[1] = id.y; ids[2] = id.z; ids[3] = id.w; nat = 4;
+    } else if (t < a.nb + a.na + a.nd + a.ni) {   // calcImproperdihedralbondForces :214-264
+        const int k = t - a.nb - a.na - a.nd;
+        const int4 id = a.iid[k];
+        const float2 p = a.ip[k];
+        const v3 pi = {0.f, 0.f, 0.f}, pj = bdisp(a, id.x, id.y), pk = bdisp(a, id.x, id.z), pl = bdisp(a, id.x, id.w);
+        const v3 ij = nrm(pj - pi), ik = nrm(pk - pi), il = nrm(pl - pi), lj = nrm(pj - pl), lk = nrm(pk - pl);
+        const v3 n1 = nrm(cross(ij, ik)), n2 = nrm(cross(lj, lk));
+        float ang = acosf(fminf(1.f, fmaxf(-1.f, dot(n1, n2))));
+        if (dot(n1, il) > 0.f) ang = -ang;
+        const float err = ang - p.x;
+        e = 0.5f * p.y * err * err;
+        const float tq = p.y * err;
+        const float dl = len(cross(pk - pj, pj - pl)) / len(pk - pj);
+        const float di = len(cross(pk - pj, pj - pi)) / len(pk - pj);
+        const v3 fl = n2 * (tq / dl);
+        const v3 fi = -n1 * (tq / di);
+        const v3 res = -(fi + fl);
+        const float ratio = len(pj - pi) / (len(pj - pi) + len(pk - pi));
+        addf(a, id.x, fi);
+        addf(a, id.y, res * (1.f - ratio));
+        addf(a, id.z, res * ratio);
+        addf(a, id.w, fl);
+        ids[0] = id.x; ids[1] = id.y; ids[2] = id.z; ids[3] = id.w; nat = 4;
+    } else if (t < a.nb + a.na + a.nd + a.ni + a.np) {   // computePairbondForces :335-376
+        const int k = t - a.nb - a.na - a.nd - a.ni;
+        const int2 id = a.pid[k];
+        const float2 p = a.pp[k];
+        const v3 d = bdisp(a, id.x, id.y);
+        const float r2 = dot(d, d), rinv2 = 1.f / r2;
+        float s = p.x * p.x * rinv2;
+        s = s * s * s;
+        const float S = 24.f * p.y * s * rinv2 * (1.f - 2.f * s);
+        addf(a, id.x, d * S);
+        addf(a, id.y, -(d * S));
+        e = 4.f * p.y * s * (s - 1.f);
+        ids[0] = id.x; ids[1] = id.y; nat = 2;
+    }
+    if (ENERGY) {
+        for (int k = 0; k < nat; k++) atomicAdd(&a.es[a.slot_of[ids[k]]], e / (float)nat);
+        const float tot = wave_sum(e);
+        if ((threadIdx.x & 63) == 0) atomicAdd(&a.acc[ACC_BOND], (double)tot);
+    }
+}
+
+}  // namespace
+
+void launch_pack(Engine& e, const uint4* w, hipStream_t s) {
+    const int nslots = e.nl.ncl * CLUSTER;
+    hipLaunchKernelGGL(k_pack, dim3(nblk(nslots, 256)), dim3(256), 0, s, nslots, e.nl.catoms, w, e.p.prm, e.nl.cref,
+                       (long long)e.Lq[0], (long long)e.Lq[1], (long long)e.Lq[2], e.nl.xq, e.fs, e.es);
+}
+
+void launch_pair(Engine& e, bool energy, hipStream_t s) {
+    PairArgs a{};
+    a.ncl = e.nl.ncl;
+    a.nslots = e.nl.ncl * CLUSTER;
+    a.rc2 = e.rc2;
+    a.pc = e.pc;
+    a.xq = e.nl.xq; a.ljs = e.nl.ljs; a.eoff = e.nl.eoff; a.ent = e.nl.ent;
+    a.fs = e.fs; a.es = e.es; a.acc = e.acc;
+    if (a.ncl == 0) return;
+    if (energy) hipLaunchKernelGGL(k_pair<true>, dim3(nblk(a.ncl, 4)), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(k_pair<false>, dim3(nblk(a.ncl, 4)), dim3(256), 0, s, a);
+}
+
+void launch_intra(Engine& e, const uint4* w, const int4* img, bool energy, hipStream_t s) {
+    if (e.n_intra == 0) return;
+    IntraArgs a{};
+    a.n_items = e.n_intra;
+    a.nslots = e.nl.ncl * CLUSTER;
+    for (int d = 0; d < 3; d++) a.Lq[d] = e.Lq[d];
+    a.pc = e.pc;
+    a.items = e.intra_items; a.lut = e.intra_lut; a.w = w; a.img = img; a.prm = e.p.prm;
+    a.slot_of = e.nl.slot_of; a.first = e.first; a.fs = e.fs; a.es = e.es; a.acc = e.acc;
+    if (energy) hipLaunchKernelGGL(k_intra<true>, dim3(nblk(a.n_items, 4)), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(k_intra<false>, dim3(nblk(a.n_items, 4)), dim3(256), 0, s, a);
+}
+
+void launch_bonded(Engine& e, const uint4* w, const int4* img, bool energy, hipStream_t s) {
+    BondArgs a{};
+    a.nb = e.bd.nb; a.na = e.bd.na; a.nd = e.bd.nd; a.ni = e.bd.ni; a.np = e.bd.np;
+    const int total = a.nb + a.na + a.nd + a.ni + a.np;
+    if (total == 0) return;
+    a.nslots = e.nl.ncl * CLUSTER;
+    for (int d = 0; d < 3; d++) a.Lq[d] = e.Lq[d];
+    a.pc = e.pc;
+    a.bid = e.bd.bid; a.bp = e.bd.bp; a.aid = e.bd.aid; a.ap = e.bd.ap; a.did = e.bd.did; a.dp = e.bd.dp;
+    a.iid = e.bd.iid; a.ip = e.bd.ip; a.pid = e.bd.pid; a.pp = e.bd.pp;
+    a.w = w; a.img = img; a.slot_of = e.nl.slot_of; a.fs = e.fs; a.es = e.es; a.acc = e.acc;
+    if (energy) hipLaunchKernelGGL(k_bonded<true>, dim3(nblk(total, 256)), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(k_bonded<false>, dim3(nblk(total, 256)), dim3(256), 0, s, a);
+}
+
+}  // namespace lima

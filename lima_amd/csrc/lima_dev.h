@@ -1,0 +1,77 @@
+// lima_dev.h -- device-side helpers shared by the CDNA4 kernels (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+// ---- LIMA constants (code/LIMA_BASE/include/Constants.h:43-66, PhysicsUtils.cuh:5) ----
+#define LIMA_PI_F 3.14159f
+#define LIMA_NANO_TO_LIMA_F 100000000.0f
+#define LIMA_LIMA_TO_NANO_F 1e-8f
+
+namespace lima {
+
+constexpr int WAVE = 64;
+constexpr int CLUSTER = 8;          // particles per cluster (pair-list granularity)
+constexpr float PAD_POS = 1.0e4f;   // position of padding slots, far outside every cutoff
+
+struct f3 { float x, y, z; };
+
+__device__ __forceinline__ int64_t unwrapped(uint32_t w, int32_t img, int64_t Lq) { return (int64_t)w + (int64_t)img * Lq; }
+
+// Sum over aligned groups of 8 lanes: quad_perm xor1, quad_perm xor2, row_half_mirror.
+// Every lane of the group ends up with the group sum.
+__device__ __forceinline__ float sum8(float v) {
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, false));
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x141, 0xF, 0xF, false));
+    return v;
+}
+
+// Wave-wide sum (any lane count), result valid in every lane.
+__device__ __forceinline__ float wave_sum(float v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_sum_d(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// float atomic max for non-negative values (IEEE order == integer order)
+__device__ __forceinline__ void atomic_max_pos(float* addr, float v) {
+    atomicMax(reinterpret_cast<unsigned int*>(addr), __float_as_uint(v));
+}
+
+// LIMA pair interaction in FP32 (LennardJonesInteractions.cuh:74-108,
+// DeviceAlgorithmsPrivate.cuh:38-95): returns S with force on self = diff * S,
+// diff = x_other - x_self.  sig = sigma_i/2 + sigma_j/2, eps = sqrt(eps_i) sqrt(eps_j).
+struct PairConst {
+    float kc;          // modifiedCoulombConstant
+    float kappa;       // 3 / cutoff
+    float two_kappa_sqrtpi;   // 2 kappa / sqrt(PI_L)
+};
+
+__device__ __forceinline__ float pair_scalar(float r2, float sig, float eps, float qq, const PairConst& c,
+                                             float* e_lj, float* e_c) {
+    const float rinv2 = 1.0f / r2;
+    float s = sig * sig * rinv2;
+    s = s * s * s;
+    const float flj = 24.0f * eps * s * rinv2 * (1.0f - 2.0f * s);
+    const float r = sqrtf(r2);
+    const float kr = c.kappa * r;
+    const float ec = erfcf(kr);
+    const float g = ec + c.two_kappa_sqrtpi * r * expf(-kr * kr);
+    const float fc = c.kc * qq * rinv2 / r * g;
+    if (e_lj) {
+        *e_lj = 4.0f * eps * s * (s - 1.0f);
+        *e_c = c.kc * qq * ec / r;
+    }
+    return flj - fc;
+}
+
+}  // namespace lima

@@ -1,0 +1,455 @@
+// pme.hip -- particle-mesh Ewald with LIMA's formulas (code/LIMA_ENGINE/src/PME.cuh).
+//
+//   spread   q * w_x w_y w_z at nodes ix-1..ix+2 of the centred cubic B-spline
+//            (PME.cuh:263-300, DeviceAlgorithmsPrivate.cuh:7-13), accumulated as int64 fixed
+//            point (2^32 per kC/mol) with global atomics, so the grid is order independent
+//            (LIMA uses int32 fixed point for the same reason, PME.cuh:249-252).
+//   solve    3D real FFT (own mixed-radix 2/3/4/5 Stockham in LDS), multiply by LIMA's
+//            influence function G(k) (PME.cuh:434-495, spline factor multiplied), inverse FFT,
+//            1/N (PME.cuh:585-602).  Five passes: x R2C, y, z (forward, G, inverse fused), y, x C2R.
+//   gather   phi and E = -grad phi by central differences of the grid (PME.cuh:374-411),
+//            rearranged into one 6x6x6 stencil: -sum_n w(n) (phi(n+1) - phi(n-1)) =
+//            sum_m phi(m) (w(m+1) - w(m-1)).  Force q E, potE 1/2 (q phi + Eself) (PME.cuh:414-420).
+// HBM / L2 bound: per step B = 48 N_q + 34 N_grid bytes of algorithmic traffic.
+#include <cmath>
+#include <vector>
+
+#include "engine.h"
+
+namespace lima {
+
+namespace {
+
+constexpr int MAXR = 5;
+
+inline int nblk(int64_t n, int b) { return (int)((n + b - 1) / b); }
+
+struct FftPlan {
+    int N, nrad;
+    int radix[8];
+    const float2* tw;
+};
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// multiply by -i (forward) or +i (inverse)
+__device__ __forceinline__ float2 rot(float2 a, bool inv) { return inv ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x); }
+
+__device__ __forceinline__ void dft_r(float2* u, int R, bool inv) {
+    if (R == 2) {
+        const float2 a = u[0], b = u[1];
+        u[0] = cadd(a, b);
+        u[1] = csub(a, b);
+    } else if (R == 3) {
+        const float s = 0.86602540378443864676f;
+        const float2 a = cadd(u[1], u[2]);
+        const float2 t = csub(u[0], cscale(a, 0.5f));
+        const float2 r = rot(cscale(csub(u[1], u[2]), s), inv);   // -i s (u1 - u2) forward
+        u[0] = cadd(u[0], a);
+        u[1] = cadd(t, r);
+        u[2] = csub(t, r);
+    } else if (R == 4) {
+        const float2 a0 = cadd(u[0], u[2]), a1 = csub(u[0], u[2]);
+        const float2 b0 = cadd(u[1], u[3]), b1 = rot(csub(u[1], u[3]), inv);
+        u[0] = cadd(a0, b0);
+        u[2] = csub(a0, b0);
+        u[1] = cadd(a1, b1);
+        u[3] = csub(a1, b1);
+    } else {   // R == 5
+        const float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
+        const float s1 = 0.95105651629515357212f, s2 = 0.58778525229247312917f;
+        const float2 a1 = cadd(u[1], u[4]), b1 = csub(u[1], u[4]);
+        const float2 a2 = cadd(u[2], u[3]), b2 = csub(u[2], u[3]);
+        const float2 t1 = cadd(u[0], cadd(cscale(a1, c1), cscale(a2, c2)));
+        const float2 t2 = cadd(u[0], cadd(cscale(a1, c2), cscale(a2, c1)));
+        const float2 r1 = rot(cadd(cscale(b1, s1), cscale(b2, s2)), inv);
+        const float2 r2 = rot(csub(cscale(b1, s2), cscale(b2, s1)), inv);
+        u[0] = cadd(u[0], cadd(a1, a2));
+        u[1] = cadd(t1, r1);
+        u[4] = csub(t1, r1);
+        u[2] = cadd(t2, r2);
+        u[3] = csub(t2, r2);
+    }
+}
+
+// Stockham auto-sort FFT of nl lines of length N in LDS (line-major); returns the buffer
+// holding the result (a or b).  All threads of the block participate.
+__device__ float2* fft_lines(float2* a, float2* b, int nl, const FftPlan& P, bool inv) {
+    const int N = P.N;
+    int p = 1;
+    for (int st = 0; st < P.nrad; st++) {
+        const int R = P.radix[st];
+        const int nb = N / R;
+        const int stride = N / (p * R);
+        for (int idx = threadIdx.x; idx < nl * nb; idx += blockDim.x) {
+            const int l = idx / nb, k = idx - l * nb;
+            const int i = k % p;
+            float2 u[MAXR];
+            for (int t = 0; t < R; t++) u[t] = a[l * N + k + t * nb];
+            for (int t = 1; t < R; t++) {
+                float2 w = P.tw[(t * i * stride) % N];
+                if (inv) w.y = -w.y;
+                u[t] = cmul(u[t], w);
+            }
+            dft_r(u, R, inv);
+            const int j = (k - i) * R + i;
+            for (int t = 0; t < R; t++) b[l * N + j + t * p] = u[t];
+        }
+        __syncthreads();
+        float2* tmp = a; a = b; b = tmp;
+        p *= R;
+    }
+    return a;
+}
+
+struct SpreadArgs {
+    int64_t N;
+    long long Lq[3];
+    int K[3];
+    const uint4* w;
+    const float4* prm;
+    long long* qgrid;
+};
+
+__device__ __forceinline__ void grid_coord(uint32_t w, long long Lq, int K, int* ix, float* f) {
+    const unsigned long long num = (unsigned long long)w * (unsigned long long)K;
+    const unsigned long long q = num / (unsigned long long)Lq;
+    *ix = (int)q;
+    *f = (float)(num - q * (unsigned long long)Lq) / (float)Lq;
+}
+
+__device__ __forceinline__ void bspline(float f, float* w) {   // DeviceAlgorithmsPrivate.cuh:9-12
+    const float g = 1.f - f;
+    w[0] = g * g * g * (1.f / 6.f);
+    w[1] = (4.f - 6.f * f * f + 3.f * f * f * f) * (1.f / 6.f);
+    w[2] = (1.f + 3.f * f + 3.f * f * f - 3.f * f * f * f) * (1.f / 6.f);
+    w[3] = f * f * f * (1.f / 6.f);
+}
+
+__device__ __forceinline__ int wrapi(int v, int K) { return v < 0 ? v + K : (v >= K ? v - K : v); }
+
+__global__ void k_spread(SpreadArgs a) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.N) return;
+    const float q = a.prm[p].x;
+    if (q == 0.f) return;
+    const uint4 pw = a.w[p];
+    int ix[3];
+    float f[3], wx[4], wy[4], wz[4];
+    grid_coord(pw.x, a.Lq[0], a.K[0], &ix[0], &f[0]);
+    grid_coord(pw.y, a.Lq[1], a.K[1], &ix[1], &f[1]);
+    grid_coord(pw.z, a.Lq[2], a.K[2], &ix[2], &f[2]);
+    bspline(f[0], wx); bspline(f[1], wy); bspline(f[2], wz);
+    for (int c = 0; c < 4; c++) {
+        const int Z = wrapi(ix[2] - 1 + c, a.K[2]);
+        for (int b = 0; b < 4; b++) {
+            const int Y = wrapi(ix[1] - 1 + b, a.K[1]);
+            const float qyz = q * wy[b] * wz[c];
+            long long* row = a.qgrid + ((int64_t)Z * a.K[1] + Y) * a.K[0];
+            for (int x = 0; x < 4; x++) {
+                const int X = wrapi(ix[0] - 1 + x, a.K[0]);
+                const long long v = __float2ll_rn(qyz * wx[x] * 4294967296.0f);
+                atomicAdd((unsigned long long*)&row[X], (unsigned long long)v);
+            }
+        }
+    }
+}
+
+// x pass, forward: TY rows of one z plane, fixed point -> density, complex FFT, keep H outputs.
+__global__ __launch_bounds__(256) void k_fft_x_fwd(FftPlan P, int Ky, int H, int TY, float scale,
+                                                   const long long* __restrict__ qgrid, float2* __restrict__ cg) {
+    extern __shared__ float2 lds[];
+    const int N = P.N;
+    const int z = blockIdx.y, y0 = blockIdx.x * TY;
+    const int nl = min(TY, Ky - y0);
+    float2* a = lds;
+    float2* b = lds + TY * N;
+    for (int i = threadIdx.x; i < nl * N; i += blockDim.x) {
+        const int l = i / N, x = i - l * N;
+        const long long v = qgrid[((int64_t)z * Ky + y0 + l) * N + x];
+        a[i] = make_float2(__ll2float_rn(v) * (1.0f / 4294967296.0f) * scale, 0.f);
+    }
+    __syncthreads();
+    const float2* r = fft_lines(a, b, nl, P, false);
+    for (int i = threadIdx.x; i < nl * H; i += blockDim.x) {
+        const int l = i / H, k = i - l * H;
+        cg[((int64_t)z * Ky + y0 + l) * H + k] = r[l * N + k];
+    }
+}
+
+// x pass, inverse: Hermitian extension of H outputs to N, inverse FFT, real part * norm.
+__global__ __launch_bounds__(256) void k_fft_x_inv(FftPlan P, int Ky, int H, int TY, float norm,
+                                                   const float2* __restrict__ cg, float* __restrict__ phi) {
+    extern __shared__ float2 lds[];
+    const int N = P.N;
+    const int z = blockIdx.y, y0 = blockIdx.x * TY;
+    const int nl = min(TY, Ky - y0);
+    float2* a = lds;
+    float2* b = lds + TY * N;
+    for (int i = threadIdx.x; i < nl * N; i += blockDim.x) {
+        const int l = i / N, k = i - l * N;
+        const float2* row = cg + ((int64_t)z * Ky + y0 + l) * H;
+        float2 v;
+        if (k < H) v = row[k];
+        else { v = row[N - k]; v.y = -v.y; }
+        a[i] = v;
+    }
+    __syncthreads();
+    const float2* r = fft_lines(a, b, nl, P, true);
+    for (int i = threadIdx.x; i < nl * N; i += blockDim.x) {
+        const int l = i / N, x = i - l * N;
+        phi[((int64_t)z * Ky + y0 + l) * N + x] = r[i].x * norm;
+    }
+}
+
+// Strided pass over lines (n * stride) for TK consecutive kx of one outer index.
+// mode 0: forward, 1: inverse, 2: forward * green, inverse (z pass).
+__global__ __launch_bounds__(256) void k_fft_strided(FftPlan P, int H, int TK, int64_t outer_stride, int64_t stride, int mode,
+                                                     int green_y_count, float2* __restrict__ cg, const float* __restrict__ green) {
+    extern __shared__ float2 lds[];
+    const int N = P.N;
+    const int o = blockIdx.y, k0 = blockIdx.x * TK;
+    const int nl = min(TK, H - k0);
+    float2* a = lds;
+    float2* b = lds + TK * N;
+    float2* base = cg + (int64_t)o * outer_stride + k0;
+    for (int i = threadIdx.x; i < nl * N; i += blockDim.x) {
+        const int n = i / nl, t = i - n * nl;
+        a[t * N + n] = base[(int64_t)n * stride + t];
+    }
+    __syncthreads();
+    float2* r = fft_lines(a, b, nl, P, mode == 1);
+    if (mode == 2) {
+        for (int i = threadIdx.x; i < nl * N; i += blockDim.x) {
+            const int t = i / N, n = i - t * N;
+            // z pass: line index n is z, outer o is y
+            const float g = green[((int64_t)n * green_y_count + o) * H + k0 + t];
+            r[i] = cscale(r[i], g);
+        }
+        __syncthreads();
+        float2* other = (r == a) ? b : a;
+        r = fft_lines(r, other, nl, P, true);
+    }
+    for (int i = threadIdx.x; i < nl * N; i += blockDim.x) {
+        const int n = i / nl, t = i - n * nl;
+        base[(int64_t)n * stride + t] = r[t * N + n];
+    }
+}
+
+struct GatherArgs {
+    int64_t N;
+    int nslots;
+    long long Lq[3];
+    int K[3];
+    float inv2h[3];
+    float eself;
+    const uint4* w;
+    const float4* prm;
+    const float* phi;
+    const int* slot_of;
+    float* fs;
+    float* es;
+    double* acc;
+};
+
+template <bool ENERGY>
+__global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    float ephi = 0.f;
+    if (p < a.N) {
+        const float q = a.prm[p].x;
+        if (q != 0.f) {
+            const uint4 pw = a.w[p];
+            int ix[3];
+            float f[3], wx[4], wy[4], wz[4];
+            grid_coord(pw.x, a.Lq[0], a.K[0], &ix[0], &f[0]);
+            grid_coord(pw.y, a.Lq[1], a.K[1], &ix[1], &f[1]);
+            grid_coord(pw.z, a.Lq[2], a.K[2], &ix[2], &f[2]);
+            bspline(f[0], wx); bspline(f[1], wy); bspline(f[2], wz);
+            // 6-point weights over nodes ix-2..ix+3: w4 (phi), dw (difference stencil)
+            float w6x[6], w6y[6], w6z[6], d6x[6], d6y[6], d6z[6];
+            for (int m = 0; m < 6; m++) {
+                const int k = m - 1;   // index into w for node ix-2+m
+                w6x[m] = (k >= 0 && k < 4) ? wx[k] : 0.f;
+                w6y[m] = (k >= 0 && k < 4) ? wy[k] : 0.f;
+                w6z[m] = (k >= 0 && k < 4) ? wz[k] : 0.f;
+                // E = (1/2h) sum_m phi(m) (w(m+1) - w(m-1)); node m+1 is w index m, node m-1 is m-2
+                const int lo = m - 2, hi = m;
+                d6x[m] = ((hi >= 0 && hi < 4) ? wx[hi] : 0.f) - ((lo >= 0 && lo < 4) ? wx[lo] : 0.f);
+                d6y[m] = ((hi >= 0 && hi < 4) ? wy[hi] : 0.f) - ((lo >= 0 && lo < 4) ? wy[lo] : 0.f);
+                d6z[m] = ((hi >= 0 && hi < 4) ? wz[hi] : 0.f) - ((lo >= 0 && lo < 4) ? wz[lo] : 0.f);
+            }
+            float ph = 0.f, ex = 0.f, ey = 0.f, ez = 0.f;
+            for (int mz = 0; mz < 6; mz++) {
+                const int Z = wrapi(wrapi(ix[2] - 2 + mz, a.K[2]), a.K[2]);
+                for (int my = 0; my < 6; my++) {
+                    const int Y = wrapi(wrapi(ix[1] - 2 + my, a.K[1]), a.K[1]);
+                    const float* row = a.phi + ((int64_t)Z * a.K[1] + Y) * a.K[0];
+                    const float wyz = w6y[my] * w6z[mz];
+                    const float dyz = d6y[my] * w6z[mz];
+                    const float wdz = w6y[my] * d6z[mz];
+                    for (int mx = 0; mx < 6; mx++) {
+                        const int X = wrapi(wrapi(ix[0] - 2 + mx, a.K[0]), a.K[0]);
+                        const float v = row[X];
+                        ph += w6x[mx] * wyz * v;
+                        ex += d6x[mx] * wyz * v;
+                        ey += w6x[mx] * dyz * v;
+                        ez += w6x[mx] * wdz * v;
+                    }
+                }
+            }
+            const int s = a.slot_of[p];
+            atomicAdd(&a.fs[s], q * ex * a.inv2h[0]);
+            atomicAdd(&a.fs[a.nslots + s], q * ey * a.inv2h[1]);
+            atomicAdd(&a.fs[2 * a.nslots + s], q * ez * a.inv2h[2]);
+            if (ENERGY) {
+                atomicAdd(&a.es[s], 0.5f * (q * ph + a.eself));
+                ephi = 0.5f * q * ph;
+            }
+        }
+    }
+    if (ENERGY) {
+        const float t = wave_sum(ephi);
+        if ((threadIdx.x & 63) == 0) atomicAdd(&a.acc[ACC_PME], (double)t);
+    }
+}
+
+__global__ void k_zero_ll(long long* p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+bool smooth235(int n) { for (int q : {2, 3, 5}) while (n % q == 0) n /= q; return n == 1; }
+
+int derive_grid(double L, double spacing) {   // same rule as the oracle
+    int n = (int)std::ceil(L / spacing - 1e-9);
+    if (n < 4) n = 4;
+    while (!smooth235(n)) n++;
+    return n;
+}
+
+void plan_radices(int N, int* radix, int* nrad) {
+    int n = N, k = 0;
+    while (n % 4 == 0) { radix[k++] = 4; n /= 4; }
+    while (n % 2 == 0) { radix[k++] = 2; n /= 2; }
+    while (n % 3 == 0) { radix[k++] = 3; n /= 3; }
+    while (n % 5 == 0) { radix[k++] = 5; n /= 5; }
+    if (n != 1 || k > 8) throw LimaFail(LIMA_ERR_BAD_INPUT, "PME grid dimension must factor into 2, 3, 5 (at most 8 factors)");
+    *nrad = k;
+}
+
+FftPlan make_plan(const Pme& m, int d) {
+    FftPlan P{};
+    P.N = m.K[d];
+    P.nrad = m.nrad[d];
+    for (int k = 0; k < 8; k++) P.radix[k] = m.radix[d][k];
+    P.tw = m.tw[d];
+    return P;
+}
+
+}  // namespace
+
+void pme_alloc(Engine& e, const std::vector<float>& q) {
+    Pme& m = e.pme;
+    const double spacing = e.prm.pme_spacing_nm > 0 ? e.prm.pme_spacing_nm : 0.12;
+    for (int d = 0; d < 3; d++) {
+        m.K[d] = e.prm.pme_grid[d] > 0 ? e.prm.pme_grid[d] : derive_grid(e.L[d], spacing);
+        if (m.K[d] < 4 || m.K[d] > 512) throw LimaFail(LIMA_ERR_BAD_INPUT, "PME grid dimension out of range [4, 512]");
+        plan_radices(m.K[d], m.radix[d], &m.nrad[d]);
+    }
+    m.H = m.K[0] / 2 + 1;
+    const int64_t NG = (int64_t)m.K[0] * m.K[1] * m.K[2];
+    const int64_t NC = (int64_t)m.H * m.K[1] * m.K[2];
+    HIP_CHECK(hipMalloc(&m.qgrid, NG * sizeof(long long)));
+    HIP_CHECK(hipMalloc(&m.phi, NG * sizeof(float)));
+    HIP_CHECK(hipMalloc(&m.cgrid, NC * sizeof(float2)));
+    HIP_CHECK(hipMalloc(&m.green, NC * sizeof(float)));
+    // twiddles exp(-2 pi i k / K) in double, stored as float
+    for (int d = 0; d < 3; d++) {
+        std::vector<float2> tw(m.K[d]);
+        for (int k = 0; k < m.K[d]; k++) {
+            const double ang = -2.0 * M_PI * (double)k / (double)m.K[d];
+            tw[k] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+        }
+        HIP_CHECK(hipMalloc(&m.tw[d], m.K[d] * sizeof(float2)));
+        HIP_CHECK(hipMemcpy(m.tw[d], tw.data(), m.K[d] * sizeof(float2), hipMemcpyHostToDevice));
+    }
+    // influence function, PME.cuh:434-495, in double (setup only), LIMA's PI constant
+    const double PI_L = (double)LIMA_PI_F;
+    const double kappa = (double)e.pc.kappa;
+    const double kc = (double)e.pc.kc;
+    double h[3];
+    for (int d = 0; d < 3; d++) h[d] = e.L[d] / m.K[d];
+    std::vector<float> g(NC);
+    for (int z = 0; z < m.K[2]; z++)
+        for (int y = 0; y < m.K[1]; y++)
+            for (int x = 0; x < m.H; x++) {
+                const int mi[3] = {x, y, z};
+                double k2 = 0, S = 1;
+                for (int d = 0; d < 3; d++) {
+                    const int half = m.K[d] / 2;
+                    const int mm = mi[d] <= half ? mi[d] : mi[d] - m.K[d];
+                    const double kd = 2.0 * PI_L * mm / e.L[d];
+                    k2 += kd * kd;
+                    const double kh = kd * h[d] * 0.5;
+                    S *= (std::fabs(kh) < 1e-14) ? 1.0 : std::pow(std::sin(kh) / kh, 4);
+                }
+                double G = 0;
+                if (k2 > 1e-14) G = 4.0 * PI_L / k2 * std::exp(-k2 / (4.0 * kappa * kappa)) * S * S * kc;
+                g[((size_t)z * m.K[1] + y) * m.H + x] = (float)G;
+            }
+    HIP_CHECK(hipMemcpy(m.green, g.data(), NC * sizeof(float), hipMemcpyHostToDevice));
+    double qsq = 0;
+    m.ncharged = 0;
+    for (float v : q) { qsq += (double)v * v; if (v != 0.f) m.ncharged++; }
+    // PME.cuh:610-630: selfenergyCorrection = -kappa / sqrt(PI) * sum q^2 * kC (stored as float)
+    m.eself_total = -kappa / std::sqrt(PI_L) * qsq * kc;
+}
+
+void pme_free(Engine& e) {
+    Pme& m = e.pme;
+    void* ptrs[] = {m.qgrid, m.phi, m.cgrid, m.green, m.tw[0], m.tw[1], m.tw[2]};
+    for (void* p : ptrs) if (p) (void)hipFree(p);
+}
+
+void launch_pme(Engine& e, const uint4* w, bool energy, hipStream_t s) {
+    if (!e.elec || e.pme.ncharged == 0) return;
+    Pme& m = e.pme;
+    const int Kx = m.K[0], Ky = m.K[1], Kz = m.K[2], H = m.H;
+    const int64_t NG = (int64_t)Kx * Ky * Kz;
+    hipLaunchKernelGGL(k_zero_ll, dim3(nblk(NG, 256)), dim3(256), 0, s, m.qgrid, NG);
+    SpreadArgs sa{};
+    sa.N = e.N;
+    for (int d = 0; d < 3; d++) { sa.Lq[d] = e.Lq[d]; sa.K[d] = m.K[d]; }
+    sa.w = w; sa.prm = e.p.prm; sa.qgrid = m.qgrid;
+    hipLaunchKernelGGL(k_spread, dim3(nblk(e.N, 256)), dim3(256), 0, s, sa);
+    const double vcell = (e.L[0] / Kx) * (e.L[1] / Ky) * (e.L[2] / Kz);
+    const FftPlan Px = make_plan(m, 0), Py = make_plan(m, 1), Pz = make_plan(m, 2);
+    // LDS tiles stay <= 48 KB: 2 buffers x lines x length x 8 B
+    const int TY = std::max(1, std::min(Ky, 3072 / Kx));
+    hipLaunchKernelGGL(k_fft_x_fwd, dim3(nblk(Ky, TY), Kz), dim3(256), 2 * TY * Kx * sizeof(float2), s, Px, Ky, H, TY,
+                       (float)(1.0 / vcell), m.qgrid, m.cgrid);
+    const int TKy = std::max(1, std::min(16, 3072 / Ky)), TKz = std::max(1, std::min(16, 3072 / Kz));
+    // y pass: outer z, lines along y (stride H)
+    hipLaunchKernelGGL(k_fft_strided, dim3(nblk(H, TKy), Kz), dim3(256), 2 * TKy * Ky * sizeof(float2), s, Py, H, TKy,
+                       (int64_t)Ky * H, (int64_t)H, 0, Ky, m.cgrid, m.green);
+    // z pass with the influence function: outer y, lines along z (stride Ky*H)
+    hipLaunchKernelGGL(k_fft_strided, dim3(nblk(H, TKz), Ky), dim3(256), 2 * TKz * Kz * sizeof(float2), s, Pz, H, TKz,
+                       (int64_t)H, (int64_t)Ky * H, 2, Ky, m.cgrid, m.green);
+    hipLaunchKernelGGL(k_fft_strided, dim3(nblk(H, TKy), Kz), dim3(256), 2 * TKy * Ky * sizeof(float2), s, Py, H, TKy,
+                       (int64_t)Ky * H, (int64_t)H, 1, Ky, m.cgrid, m.green);
+    hipLaunchKernelGGL(k_fft_x_inv, dim3(nblk(Ky, TY), Kz), dim3(256), 2 * TY * Kx * sizeof(float2), s, Px, Ky, H, TY,
+                       (float)(1.0 / (double)NG), m.cgrid, m.phi);
+    GatherArgs ga{};
+    ga.N = e.N;
+    ga.nslots = e.nl.ncl * CLUSTER;
+    for (int d = 0; d < 3; d++) { ga.Lq[d] = e.Lq[d]; ga.K[d] = m.K[d]; ga.inv2h[d] = (float)(m.K[d] / (2.0 * e.L[d])); }
+    ga.eself = (float)m.eself_total;
+    ga.w = w; ga.prm = e.p.prm; ga.phi = m.phi; ga.slot_of = e.nl.slot_of; ga.fs = e.fs; ga.es = e.es; ga.acc = e.acc;
+    if (energy) hipLaunchKernelGGL(k_gather<true>, dim3(nblk(e.N, 256)), dim3(256), 0, s, ga);
+    else hipLaunchKernelGGL(k_gather<false>, dim3(nblk(e.N, 256)), dim3(256), 0, s, ga);
+}
+
+}  // namespace lima
