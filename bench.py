@@ -1,0 +1,164 @@
+#!/usr/bin/env python3
+"""bench.py -- ns/day of the MI355X-native LIMA step (force path + integrator) on one GPU.
+
+Workload: the chain + water box of the mixed-precision config (8 x 8 x 7 nm, 128 chains of 50
+atoms in two leaflets along z, waters filling the rest; seed 1), cutoff 1.2 nm, PME spacing
+0.12 nm (72 x 72 x 60 grid), dt = 2 fs, FP32 forces with FP64 energy reductions.  The
+generated structure is relaxed with steepest descent (untimed), W warm-up steps run, then K
+steps are timed.  lima_step returns after the device has finished, so the host clock around
+it brackets exactly the K steps.
+
+Prints one JSON line (stdout).  Progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+FP32_VALU_PEAK_TFLOPS = 157.3      # MI355X published FP32 vector peak
+FLOPS_PER_PAIR = 64                # algorithmic FP32 flops per pair inside the cutoff
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def load_traffic(kernel, workload):
+    path = ROOT / "profiles" / "traffic.json"
+    if not path.exists():
+        return None
+    try:
+        data = json.loads(path.read_text())
+        return data.get(kernel, {}).get(workload)
+    except Exception:
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--warmup", type=int, default=500)
+    ap.add_argument("--em-iters", type=int, default=400)
+    ap.add_argument("--timing-steps", type=int, default=1000)
+    ap.add_argument("--cpu-steps", type=int, default=8, help="oracle MD steps in the CPU baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this path has no multi-GPU mode (--gpus must be 1)", file=sys.stderr)
+        return 2
+
+    import lima_amd as L
+
+    t = time.time()
+    box = L.gen_chain_water()
+    log(f"generated chain+water box: {box.n_particles} atoms, {box.n_compounds} compounds in {time.time() - t:.2f}s")
+    dt_fs = 2.0
+    p = L.default_params(dt_fs=dt_fs, n_steps=args.steps, nlist_interval=10, energy_interval=0)
+    eng = L.Engine(box, p)
+    log("engine created")
+    t = time.time()
+    r = eng.minimize(1000.0, args.em_iters, 0.01)
+    log(f"relaxed: {r.iterations} SD iterations, fmax {r.final_fmax:.1f} kJ/mol/nm, {time.time() - t:.1f}s")
+    t = time.time()
+    eng.step(args.warmup)
+    log(f"warm-up {args.warmup} steps in {time.time() - t:.2f}s")
+
+    t0 = time.perf_counter()
+    eng.step(args.steps)
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    ms_per_step = elapsed * 1e3 / args.steps
+    ns_day = args.steps / elapsed * dt_fs * 86400.0 / 1e6
+    log(f"timed {args.steps} steps: {ms_per_step:.4f} ms/step, {ns_day:.1f} ns/day")
+    e_end = eng.energy()
+    pairs = eng.pair_count()
+
+    # per-phase kernel times with HIP events recorded on the engine's stream (no host syncs)
+    eng.set_timing(True)
+    eng.step(args.timing_steps)
+    tm = eng.timings()
+    eng.set_timing(False)
+    n = max(tm.steps_timed, 1)
+    pair_ms = tm.pair_ms / n
+    phases = {"pair": pair_ms, "intra": tm.intra_ms / n, "bonded": tm.bonded_ms / n, "pme": tm.pme_ms / n,
+              "pack_integrate": tm.integrate_ms / n, "step_total": tm.total_ms / n,
+              "nlist_rebuild_per_step": tm.nlist_ms / n}
+    log("phase ms/step: " + ", ".join(f"{k} {v:.4f}" for k, v in phases.items()))
+    flops = pairs * FLOPS_PER_PAIR
+    achieved = flops / (pair_ms * 1e-3) / 1e12 if pair_ms > 0 else 0.0
+    workload = "chain_water_c1"
+    roofline = {"kernel": "k_pair", "bound": "valu", "achieved": round(achieved, 3), "peak": FP32_VALU_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(achieved / FP32_VALU_PEAK_TFLOPS, 5),
+                "traffic": load_traffic("k_pair", workload),
+                "work": f"{pairs} pairs inside 1.2 nm x {FLOPS_PER_PAIR} flop / {pair_ms * 1e3:.2f} us"}
+
+    accuracy = None
+    cpu = None
+    if not (args.no_accuracy and args.no_cpu_baseline):
+        from oracle.oracle import Oracle, set_threads
+        import numpy as np
+        from tests.helpers import rel_rms, layout_forces
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        set_threads(threads)
+        st = eng.state()
+        sbox = L.StateBox(box, st)
+        if not args.no_accuracy:
+            e_gpu = eng.compute_forces()
+            f_gpu = layout_forces(sbox, eng.state())
+            t = time.time()
+            orc = Oracle(sbox, p)
+            f_ref, _, e_ref = orc.forces()
+            err = rel_rms(f_gpu, f_ref)
+            e_rel = abs(e_gpu.potential - e_ref.potential) / abs(e_ref.potential)
+            accuracy = {"force_rel_rms_vs_fp64": float(f"{err:.3e}"), "potential_rel_err_vs_fp64": float(f"{e_rel:.3e}")}
+            log(f"accuracy vs FP64 oracle: force rel RMS {err:.2e}, potential rel {e_rel:.2e} ({time.time() - t:.1f}s)")
+        if not args.no_cpu_baseline:
+            orc = Oracle(sbox, p)
+            orc.step(1)   # warm-up
+            t = time.perf_counter()
+            orc.step(args.cpu_steps)
+            dtc = time.perf_counter() - t
+            cpu = {"value": round(args.cpu_steps / dtc * dt_fs * 86400.0 / 1e6, 4), "unit": "ns/day", "cores": threads,
+                   "kind": "port",
+                   "sample": f"{args.cpu_steps} FP64 oracle MD steps of the same box after 1 warm-up step "
+                             f"({dtc:.1f}s, {dtc / args.cpu_steps * 1e3:.0f} ms/step)"}
+            log(f"cpu baseline: {cpu}")
+
+    out = {
+        "metric": "ns/day (42k-atom box); time to Fmax<100 kJ/mol/nm, learned vs plain EM",
+        "value": round(ns_day, 3),
+        "unit": "ns/day",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": workload, "n_atoms": box.n_particles, "box_nm": [float(x) for x in box.box_nm],
+                   "chains": "128 x 50", "waters": (box.n_particles - 6400) // 3, "dt_fs": dt_fs, "cutoff_nm": 1.2,
+                   "pme_spacing_nm": 0.12, "nlist_interval": 10, "skin_nm": 0.1, "energy_reductions": "fp64"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "accuracy": accuracy,
+        "phases_ms_per_step": {k: round(v, 5) for k, v in phases.items()},
+        "pairs_in_cutoff": pairs,
+        "final_energy_kj": round(e_end.total / 1000.0, 3),
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

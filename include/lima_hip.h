@@ -210,7 +210,12 @@ typedef struct lima_timings {
     int64_t steps_timed;
 } lima_timings;
 lima_status lima_get_timings(lima_engine* engine, lima_timings* out);
+/* Enable per-phase HIP-event timing (no host synchronisation per step; the events are
+ * summed when lima_get_timings is called). */
 lima_status lima_set_timing(lima_engine* engine, int32_t enabled);
+/* Particle pairs inside the cutoff counted by the pair kernel at the last energy evaluation
+ * (the algorithmic work unit of the pair kernel); -1 for a null engine. */
+int64_t lima_get_pair_count(lima_engine* engine);
 
 #ifdef __cplusplus
 }

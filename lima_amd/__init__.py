@@ -79,6 +79,7 @@ def load_hip():
                                         C.POINTER(C.c_uint8)], C.c_int),
             "lima_get_timings": ([vp, C.POINTER(Timings)], C.c_int),
             "lima_set_timing": ([vp, C.c_int32], C.c_int),
+            "lima_get_pair_count": ([vp], C.c_int64),
         }
         for name, (args, res) in sig.items():
             fn = getattr(lib, name)
@@ -277,3 +278,36 @@ class Engine:
         t = Timings()
         self._check(self.lib.lima_get_timings(self._h, C.byref(t)))
         return t
+
+    def pair_count(self):
+        return int(self.lib.lima_get_pair_count(self._h))
+
+
+class StateBox:
+    """A box descriptor whose positions / velocities come from an engine state (LIMA layout),
+    with every other field shared with `base`.  Keeps the arrays alive."""
+
+    def __init__(self, base, state):
+        self.base = base
+        self.desc = BoxDesc()
+        C.pointer(self.desc)[0] = base.desc
+        self._origo = np.ascontiguousarray(state["compound_origo"], np.int32)
+        self._coord = np.ascontiguousarray(state["particle_coord"], np.int32)
+        self._vel = np.ascontiguousarray(state["particle_velocity"], np.float32)
+        self._tb = np.ascontiguousarray(state["tinymol_block"], np.int32)
+        self._tc = np.ascontiguousarray(state["tinymol_coord"], np.int32)
+        self._tv = np.ascontiguousarray(state["tinymol_velocity"], np.float32)
+        ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))
+        fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+        self.desc.compound_origo = ip(self._origo)
+        self.desc.particle_coord = ip(self._coord)
+        self.desc.particle_velocity = fp(self._vel)
+        if base.n_tinymols:
+            self.desc.tinymol_block = ip(self._tb)
+            self.desc.tinymol_coord = ip(self._tc)
+            self.desc.tinymol_velocity = fp(self._tv)
+        for k in ("n_compounds", "n_tinymols", "box_nm", "n_particles", "n_particles_per_compound"):
+            setattr(self, k, getattr(base, k))
+
+    def layout_index(self):
+        return self.base.layout_index()

@@ -76,7 +76,7 @@ struct Bonded {
 };
 
 // Device accumulators (FP64 energy reductions and scalars).
-enum Acc { ACC_LJ = 0, ACC_COUL, ACC_BOND, ACC_PME, ACC_KIN, ACC_N };
+enum Acc { ACC_LJ = 0, ACC_COUL, ACC_BOND, ACC_PME, ACC_KIN, ACC_PAIRS, ACC_N };
 
 struct SdState {            // steepest descent state, device resident
     double e_acc, e_try;
@@ -136,10 +136,14 @@ struct Engine {
     bool forces_valid = false;
     bool critical = false;
 
-    // timing
+    // timing: per step 7 events (before pack, after pack, pair, intra, bonded, pme, integrate),
+    // recorded into a ring without host syncs and summed when the ring is flushed
     bool timing = false;
-    hipEvent_t tev[16];
+    static constexpr int TRING = 256, TEV = 7;
+    hipEvent_t tev[TRING * TEV];
+    int tpos = 0;
     lima_timings tacc{};
+    int64_t pairs_last = 0;      // pairs inside the cutoff at the last energy evaluation
 };
 
 #define HIP_CHECK(x)                                                                          \

@@ -206,19 +206,43 @@ T* dalloc_copy(const std::vector<T>& v) {
 
 double e_const(const Engine& e) { return e.elec ? e.pme.eself_total : 0.0; }
 
-void evaluate(Engine& e, const uint4* w, const int4* img, bool energy) {
+hipEvent_t* tslot(Engine& e) { return &e.tev[e.tpos * Engine::TEV]; }
+
+void timing_flush(Engine& e) {
+    if (e.tpos == 0) return;
+    HIP_CHECK(hipEventSynchronize(e.tev[(e.tpos - 1) * Engine::TEV + Engine::TEV - 1]));
+    for (int k = 0; k < e.tpos; k++) {
+        hipEvent_t* ev = &e.tev[k * Engine::TEV];
+        float t[6];
+        for (int i = 0; i < 6; i++) HIP_CHECK(hipEventElapsedTime(&t[i], ev[i], ev[i + 1]));
+        e.tacc.pair_ms += t[1];
+        e.tacc.intra_ms += t[2];
+        e.tacc.bonded_ms += t[3];
+        e.tacc.pme_ms += t[4];
+        e.tacc.integrate_ms += t[0] + t[5];   // pack + integrate: the per-particle streaming kernels
+        float tot;
+        HIP_CHECK(hipEventElapsedTime(&tot, ev[0], ev[6]));
+        e.tacc.total_ms += tot;
+        e.tacc.steps_timed++;
+    }
+    e.tpos = 0;
+}
+
+void evaluate(Engine& e, const uint4* w, const int4* img, bool energy, bool timed = false) {
     hipStream_t s = e.s_main;
+    hipEvent_t* ev = timed ? tslot(e) : nullptr;
     hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(64), 0, s, e.acc, (int)ACC_N);
-    if (e.timing) HIP_CHECK(hipEventRecord(e.tev[0], s));
+    if (ev) HIP_CHECK(hipEventRecord(ev[0], s));
     launch_pack(e, w, s);
+    if (ev) HIP_CHECK(hipEventRecord(ev[1], s));
     launch_pair(e, energy, s);
-    if (e.timing) HIP_CHECK(hipEventRecord(e.tev[1], s));
+    if (ev) HIP_CHECK(hipEventRecord(ev[2], s));
     launch_intra(e, w, img, energy, s);
-    if (e.timing) HIP_CHECK(hipEventRecord(e.tev[2], s));
+    if (ev) HIP_CHECK(hipEventRecord(ev[3], s));
     launch_bonded(e, w, img, energy, s);
-    if (e.timing) HIP_CHECK(hipEventRecord(e.tev[3], s));
+    if (ev) HIP_CHECK(hipEventRecord(ev[4], s));
     launch_pme(e, w, energy, s);
-    if (e.timing) HIP_CHECK(hipEventRecord(e.tev[4], s));
+    if (ev) HIP_CHECK(hipEventRecord(ev[5], s));
 }
 
 void fill_energy(Engine& e, bool with_kinetic) {
@@ -235,6 +259,7 @@ void fill_energy(Engine& e, bool with_kinetic) {
     E.potential_lima = E.lj + E.coulomb_sr + E.bonded + E.pme_recip + E.pme_self_lima;
     if (with_kinetic) E.kinetic = acc[ACC_KIN];
     E.total = E.potential + E.kinetic;
+    e.pairs_last = (int64_t)acc[ACC_PAIRS];
 }
 
 void check_finite(Engine& e) {
@@ -261,30 +286,19 @@ void md_steps(Engine& e, int64_t n) {
     const int interval = e.prm.nlist_interval > 0 ? e.prm.nlist_interval : 10;
     for (int64_t k = 0; k < n; k++) {
         if (e.steps_since_build >= interval) {
-            if (e.timing) HIP_CHECK(hipEventRecord(e.tev[6], e.s_main));
-            nlist_build(e, e.p.w);
+            const auto t0 = std::chrono::steady_clock::now();
+            nlist_build(e, e.p.w);   // synchronises once (entry count read-back)
             if (e.timing) {
-                HIP_CHECK(hipEventRecord(e.tev[7], e.s_main));
-                HIP_CHECK(hipEventSynchronize(e.tev[7]));
-                float ms;
-                HIP_CHECK(hipEventElapsedTime(&ms, e.tev[6], e.tev[7]));
-                e.tacc.nlist_ms += ms;
+                HIP_CHECK(hipStreamSynchronize(e.s_main));
+                e.tacc.nlist_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
             }
         }
         const bool energy = (e.prm.energy_interval > 0 && e.step % e.prm.energy_interval == 0) || k == n - 1;
-        evaluate(e, e.p.w, e.p.img, energy);
+        evaluate(e, e.p.w, e.p.img, energy, e.timing);
         integrate(e, energy);
         if (e.timing) {
-            HIP_CHECK(hipEventRecord(e.tev[5], e.s_main));
-            HIP_CHECK(hipEventSynchronize(e.tev[5]));
-            float t[5];
-            for (int i = 0; i < 5; i++) HIP_CHECK(hipEventElapsedTime(&t[i], e.tev[i], e.tev[i + 1]));
-            e.tacc.pair_ms += t[0]; e.tacc.intra_ms += t[1]; e.tacc.bonded_ms += t[2]; e.tacc.pme_ms += t[3];
-            e.tacc.integrate_ms += t[4];
-            float tot;
-            HIP_CHECK(hipEventElapsedTime(&tot, e.tev[0], e.tev[5]));
-            e.tacc.total_ms += tot;
-            e.tacc.steps_timed++;
+            HIP_CHECK(hipEventRecord(tslot(e)[6], e.s_main));
+            if (++e.tpos == Engine::TRING) timing_flush(e);
         }
         e.step++;
         e.steps_since_build++;
@@ -754,15 +768,21 @@ lima_status lima_get_neighbor_list(lima_engine* h, int32_t* cluster_atoms, int64
 
 lima_status lima_get_timings(lima_engine* h, lima_timings* out) {
     if (!h || !out) { set_err("null argument"); return LIMA_ERR_BAD_INPUT; }
-    *out = h->e.tacc;
-    return LIMA_OK;
+    return guarded([&] {
+        timing_flush(h->e);
+        *out = h->e.tacc;
+    });
 }
 
 lima_status lima_set_timing(lima_engine* h, int32_t enabled) {
     if (!h) { set_err("null engine"); return LIMA_ERR_BAD_INPUT; }
-    h->e.timing = enabled != 0;
-    h->e.tacc = lima_timings{};
-    return LIMA_OK;
+    return guarded([&] {
+        timing_flush(h->e);
+        h->e.timing = enabled != 0;
+        h->e.tacc = lima_timings{};
+    });
 }
+
+int64_t lima_get_pair_count(lima_engine* h) { return h ? h->e.pairs_last : -1; }
 
 }  // extern "C"

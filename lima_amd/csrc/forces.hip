@@ -78,6 +78,7 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs a) {
     const float4 xi = a.xq[islot];
     const float2 pi = a.ljs[islot];
     float fx = 0.f, fy = 0.f, fz = 0.f, ei = 0.f, elj = 0.f, ecl = 0.f;
+    int npairs = 0;
     const int e0 = a.eoff[I], e1 = a.eoff[I + 1];
     for (int e = e0 + jj; e < e1; e += CLUSTER) {
         const float4 rec = a.ent[e];
@@ -88,7 +89,9 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs a) {
         const float dx = (xj.x + rec.x) - xi.x, dy = (xj.y + rec.y) - xi.y, dz = (xj.z + rec.z) - xi.z;
         const float r2 = dx * dx + dy * dy + dz * dz;
         const bool inc = ((m >> ii) & 1u) && (r2 <= a.rc2) && (r2 > 0.f);
-        if (__ballot(inc) == 0ull) continue;
+        const unsigned long long any = __ballot(inc);
+        if (any == 0ull) continue;
+        if (ENERGY) npairs += __popcll(any);
         const float2 pj = a.ljs[b];
         float el = 0.f, ec = 0.f;
         float S = pair_scalar(inc ? r2 : 1.0f, pi.x + pj.x, pi.y * pj.y, xi.w * xj.w, a.pc, ENERGY ? &el : nullptr, &ec);
@@ -129,6 +132,7 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs a) {
         if (lane == 0) {
             atomicAdd(&a.acc[ACC_LJ], (double)tl);
             atomicAdd(&a.acc[ACC_COUL], (double)tc);
+            atomicAdd(&a.acc[ACC_PAIRS], (double)npairs);
         }
     }
 }

@@ -433,29 +433,32 @@ lima_status lima_gen_chain_water(double bx, double by, double bz, int32_t n_chai
         B.box[0] = bx; B.box[1] = by; B.box[2] = bz;
         Hash solute(B.box, 0.5);
         // Two facing leaflets of chains along z: leaflet A points +z, leaflet B points -z.  The
-        // chains sit on a hexagonal lattice of 0.45 nm spacing (rows of 8) centred in x-y,
-        // leaflet A in the first half of the rows, B in the second, each chain all-trans with
-        // jittered dihedrals.
+        // chains sit on a hexagonal lattice of 0.5 nm spacing (rows of 8) centred in x-y,
+        // leaflet A in the first half of the rows, B in the second.
         const int per_row = 8;
         const int rows = (n_chains + per_row - 1) / per_row;
-        const double sp = 0.45, rsp = sp * std::sqrt(3.0) / 2.0;
+        const double sp = 0.5, rsp = sp * std::sqrt(3.0) / 2.0;
         const double x0 = 0.5 * (bx - (per_row - 0.5) * sp), y0 = 0.5 * (by - (rows - 1) * rsp);
         for (int c = 0; c < n_chains; c++) {
             const int row = c / per_row, col = c % per_row;
             const bool up = row < rows / 2 || rows == 1;
             V3 p0(x0 + col * sp + (row % 2) * 0.5 * sp + r.uniform(-0.03, 0.03), y0 + row * rsp + r.uniform(-0.03, 0.03),
                   up ? 0.35 + r.uniform(0, 0.1) : bz - 0.35 - r.uniform(0, 0.1));
-            const V3 dir(r.uniform(-0.05, 0.05), r.uniform(-0.05, 0.05), up ? 1.0 : -1.0);
+            // all-trans zig-zag whose axis is z: rise b sin(theta/2) per atom, alternating lateral
+            // offset +-b cos(theta/2)/2 along a random azimuth, plus a small positional jitter
             Molecule m;
             chain_params(m, len, r);
             chain_topology(m, len, r);
-            std::vector<V3> pos;
-            const double L3[3] = {bx, by, bz};
-            auto accept = [&](const V3& cand, const std::vector<V3>& cur) {
-                for (size_t j = 0; j + 3 < cur.size(); j++) if (mind_img(cand, cur[j], L3) < 0.3) return false;
-                return solute.map.empty() || solute.mindist(cand, 0.32) >= 0.32;
-            };
-            if (!grow_chain(pos, len, p0, dir, r, 0.92, 12.0, accept)) throw std::runtime_error("chain placement failed");
+            const double half = 0.5 * C_ANGLE * M_PI / 180.0;
+            const double rise = C_BOND * std::sin(half), lat = 0.5 * C_BOND * std::cos(half);
+            const double az = r.uniform(0, 2 * M_PI);
+            const V3 u(std::cos(az), std::sin(az), 0);
+            std::vector<V3> pos(len);
+            for (int k = 0; k < len; k++) {
+                const double sgn = (k % 2) ? 1.0 : -1.0;
+                pos[k] = p0 + V3(0, 0, (up ? 1.0 : -1.0) * rise * k) + u * (sgn * lat) +
+                         V3(r.uniform(-0.005, 0.005), r.uniform(-0.005, 0.005), r.uniform(-0.005, 0.005));
+            }
             for (int k = 0; k < len; k++) { m.atoms[k].pos = pos[k]; }
             for (auto& p : pos) solute.add(p);
             B.mols.push_back(std::move(m));
