@@ -81,6 +81,10 @@ def main():
     log(f"timed {args.steps} steps: {ms_per_step:.4f} ms/step, {ns_day:.1f} ns/day")
     e_end = eng.energy()
     pairs = eng.pair_count()
+    stt = eng.status()
+    list_slots = stt.list_entries * 8
+    log(f"list: {stt.n_list_rebuilds} rebuilds, {stt.list_violations} intervals with 2 max|dx| > skin, "
+        f"last max |dx| {stt.max_displacement_nm:.4f} nm")
 
     # per-phase kernel times with HIP events recorded on the engine's stream (no host syncs)
     eng.set_timing(True)
@@ -154,6 +158,10 @@ def main():
         "accuracy": accuracy,
         "phases_ms_per_step": {k: round(v, 5) for k, v in phases.items()},
         "pairs_in_cutoff": pairs,
+        "pair_slots_evaluated": list_slots,
+        "cluster_pair_efficiency": round(2 * pairs / list_slots, 4) if list_slots else None,
+        "list_rebuilds": stt.n_list_rebuilds,
+        "list_stale_intervals": stt.list_violations,
         "final_energy_kj": round(e_end.total / 1000.0, 3),
     }
     print(json.dumps(out), flush=True)

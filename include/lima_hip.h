@@ -138,6 +138,9 @@ typedef struct lima_runstatus {
     int32_t simulation_finished;
     int32_t critical_error_occured;
     int32_t n_list_rebuilds;
+    int64_t list_entries;         /* pair-list entries (8 pair slots each) of the last rebuild */
+    int64_t list_violations;      /* rebuilds after which 2 x max displacement exceeded the skin */
+    float   max_displacement_nm;  /* max displacement over the last completed list interval */
 } lima_runstatus;
 
 /* State read-back.  Arrays the caller provides (NULL = skip) in LIMA's

@@ -75,6 +75,9 @@ class RunStatus(C.Structure):
         ("simulation_finished", C.c_int32),
         ("critical_error_occured", C.c_int32),
         ("n_list_rebuilds", C.c_int32),
+        ("list_entries", C.c_int64),
+        ("list_violations", C.c_int64),
+        ("max_displacement_nm", C.c_float),
     ]
 
 

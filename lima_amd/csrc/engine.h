@@ -30,6 +30,7 @@ struct NList {
     int ncl = 0;               // clusters
     int ncl_max = 0;
     int64_t nent = 0, nent_cap = 0;
+    int cap = 2048;            // entry capacity per i-cluster (grows on overflow)
     int* col_of = nullptr;     // [N]
     int* col_count = nullptr;  // [ncol+1]
     int* col_start = nullptr;  // [ncol+1]
@@ -42,7 +43,7 @@ struct NList {
     uint4* cref = nullptr;     // [ncl_max] cluster reference point (integer, centre of bbox)
     uint4* cmin = nullptr;     // [ncl_max]
     uint4* cmax = nullptr;     // [ncl_max]
-    float2* ljs = nullptr;     // [slots] (sigma/2, sqrt eps)
+    float2* ljs = nullptr;     // [slots] (sigma/2, sqrt(24 eps))
     int* comps = nullptr;      // [slots] compound of slot (-1 padding)
     float4* xq = nullptr;      // [slots] position relative to cluster ref [nm] + charge
     int* ecount = nullptr;     // [ncl_max+1]
@@ -64,6 +65,18 @@ struct Pme {
     int nrad[3] = {0, 0, 0};
     double eself_total = 0;                // -kappa/sqrt(pi_L) kC sum q^2
     int64_t ncharged = 0;
+    // tiled spreading: charged particles binned every step by the tile of their base node;
+    // each tile accumulates an E^3 block (E = T + 3) in LDS
+    bool tiled = false;
+    int T[3] = {8, 8, 8}, margin = 1, E[3] = {13, 13, 13};   // tile edge divides K; E = T + 3 + 2 margin
+    int nt[3] = {0, 0, 0};
+    int ntiles = 0;
+    int bincap = 512;                      // charged particles per tile bin
+    int* bin_count = nullptr;              // [ntiles+1]
+    int* bin_atoms = nullptr;              // [ntiles][bincap]
+    long long* tblock = nullptr;           // [ntiles][E^3] per-tile blocks
+    int4* cover[3] = {nullptr, nullptr, nullptr};   // per grid coordinate: (tile a, local a, tile b, local b)
+    int* flags = nullptr;                  // [1] particle left its tile block
 };
 
 struct Bonded {
@@ -101,6 +114,11 @@ struct Engine {
 
     Particles p;
     // SD trial state
+    uint4* w_build = nullptr;  // positions at the last list rebuild (displacement check)
+    int4* img_build = nullptr;
+    float* dmax2 = nullptr;    // max squared displacement since the last rebuild [nm^2]
+    int64_t list_violations = 0;
+    double last_dmax = 0;
     uint4* w_try = nullptr;
     int4* img_try = nullptr;
     float4* f_acc = nullptr;   // accepted forces, global order
@@ -173,6 +191,7 @@ void launch_bonded(Engine& e, const uint4* w, const int4* img, bool energy, hipS
 void pme_alloc(Engine& e, const std::vector<float>& charges_global);
 void pme_free(Engine& e);
 void launch_pme(Engine& e, const uint4* w, bool energy, hipStream_t s);
+void pme_bin(Engine& e, const uint4* w, hipStream_t s);   // at list rebuild
 
 // ---- scan helper (nlist.hip) ----
 void launch_exclusive_scan(const int* in, int* out, int n, hipStream_t s);

@@ -37,6 +37,9 @@ struct IntegrateArgs {
     long long Lq[3];
     float dt;
     float thermostat;
+    const uint4* w_build;
+    const int4* img_build;
+    float* dmax2;
     const int* slot_of;
     const float* fs;
     uint4* w;
@@ -67,7 +70,7 @@ __device__ __forceinline__ int coord_of(float nm) {   // Coord{Float3}: static_c
 template <bool ENERGY>
 __global__ void k_integrate(IntegrateArgs a) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    float ke = 0.f;
+    float ke = 0.f, d2 = 0.f;
     if (p < a.N) {
         const int s = a.slot_of[p];
         const float fx = a.fs[s], fy = a.fs[a.nslots + s], fz = a.fs[2 * a.nslots + s];
@@ -86,11 +89,21 @@ __global__ void k_integrate(IntegrateArgs a) {
         wrap_add(w.z, im.z, coord_of(v.z * a.dt + fz * c2), a.Lq[2]);
         a.w[p] = w;
         a.img[p] = im;
+        {   // displacement since the list was built (unwrapped, exact integer difference)
+            const uint4 wb = a.w_build[p];
+            const int4 ib = a.img_build[p];
+            const float dx = (float)((long long)w.x - (long long)wb.x + (long long)(im.x - ib.x) * a.Lq[0]) * LIMA_LIMA_TO_NANO_F;
+            const float dy = (float)((long long)w.y - (long long)wb.y + (long long)(im.y - ib.y) * a.Lq[1]) * LIMA_LIMA_TO_NANO_F;
+            const float dz = (float)((long long)w.z - (long long)wb.z + (long long)(im.z - ib.z) * a.Lq[2]) * LIMA_LIMA_TO_NANO_F;
+            d2 = dx * dx + dy * dy + dz * dz;
+        }
         if (ENERGY) ke = 0.5f * m * (v.x * v.x + v.y * v.y + v.z * v.z);
         v.x *= a.thermostat; v.y *= a.thermostat; v.z *= a.thermostat;
         a.v[p] = v;
         a.fprev[p] = make_float4(fx, fy, fz, 0.f);
     }
+    d2 = wave_max(d2);
+    if ((threadIdx.x & 63) == 0) atomic_max_pos(a.dmax2, d2);
     if (ENERGY) {
         const float t = wave_sum(ke);
         if ((threadIdx.x & 63) == 0) atomicAdd(&a.acc[ACC_KIN], (double)t);
@@ -228,6 +241,10 @@ void timing_flush(Engine& e) {
     e.tpos = 0;
 }
 
+// One force evaluation.  PME (spread, FFT, gather) runs on its own stream between a fork
+// after the pack kernel (which zeroes the force slots) and a join before the caller's next
+// kernel, overlapping the pair / intra / bonded kernels (LIMA overlaps PME the same way,
+// Engine.cu:333).  With timing on, everything runs on one stream so each phase is timed alone.
 void evaluate(Engine& e, const uint4* w, const int4* img, bool energy, bool timed = false) {
     hipStream_t s = e.s_main;
     hipEvent_t* ev = timed ? tslot(e) : nullptr;
@@ -235,13 +252,21 @@ void evaluate(Engine& e, const uint4* w, const int4* img, bool energy, bool time
     if (ev) HIP_CHECK(hipEventRecord(ev[0], s));
     launch_pack(e, w, s);
     if (ev) HIP_CHECK(hipEventRecord(ev[1], s));
+    const bool split = !timed && e.elec && e.pme.ncharged > 0;
+    if (split) {
+        HIP_CHECK(hipEventRecord(e.ev_fork, s));
+        HIP_CHECK(hipStreamWaitEvent(e.s_pme, e.ev_fork, 0));
+        launch_pme(e, w, energy, e.s_pme);
+        HIP_CHECK(hipEventRecord(e.ev_join, e.s_pme));
+    }
     launch_pair(e, energy, s);
     if (ev) HIP_CHECK(hipEventRecord(ev[2], s));
     launch_intra(e, w, img, energy, s);
     if (ev) HIP_CHECK(hipEventRecord(ev[3], s));
     launch_bonded(e, w, img, energy, s);
     if (ev) HIP_CHECK(hipEventRecord(ev[4], s));
-    launch_pme(e, w, energy, s);
+    if (split) HIP_CHECK(hipStreamWaitEvent(s, e.ev_join, 0));
+    else launch_pme(e, w, energy, s);
     if (ev) HIP_CHECK(hipEventRecord(ev[5], s));
 }
 
@@ -278,6 +303,7 @@ void integrate(Engine& e, bool energy) {
     a.thermostat = 1.f;
     a.slot_of = e.nl.slot_of; a.fs = e.fs; a.w = e.p.w; a.img = e.p.img; a.v = e.p.v; a.fprev = e.p.fprev;
     a.prm = e.p.prm; a.acc = e.acc;
+    a.w_build = e.w_build; a.img_build = e.img_build; a.dmax2 = e.dmax2;
     if (energy) hipLaunchKernelGGL(k_integrate<true>, dim3(nblk(e.N, 256)), dim3(256), 0, e.s_main, a);
     else hipLaunchKernelGGL(k_integrate<false>, dim3(nblk(e.N, 256)), dim3(256), 0, e.s_main, a);
 }
@@ -404,6 +430,10 @@ void create(Engine& e, const lima_box_desc* b, const lima_simparams* p) {
     e.p.fprev = dalloc_copy(zero4);
     e.p.prm = dalloc_copy(prm);
     e.p.comp = dalloc_copy(comp);
+    HIP_CHECK(hipMalloc(&e.w_build, N * sizeof(uint4)));
+    HIP_CHECK(hipMalloc(&e.img_build, N * sizeof(int4)));
+    HIP_CHECK(hipMalloc(&e.dmax2, sizeof(float)));
+    HIP_CHECK(hipMemset(e.dmax2, 0, sizeof(float)));
     HIP_CHECK(hipMalloc(&e.w_try, N * sizeof(uint4)));
     HIP_CHECK(hipMalloc(&e.img_try, N * sizeof(int4)));
     e.f_acc = dalloc_copy(zero4);
@@ -494,6 +524,8 @@ void create(Engine& e, const lima_box_desc* b, const lima_simparams* p) {
     HIP_CHECK(hipMalloc(&e.fmax, sizeof(float)));
     HIP_CHECK(hipStreamCreate(&e.s_main));
     HIP_CHECK(hipStreamCreate(&e.s_pme));
+    HIP_CHECK(hipEventCreateWithFlags(&e.ev_fork, hipEventDisableTiming));
+    HIP_CHECK(hipEventCreateWithFlags(&e.ev_join, hipEventDisableTiming));
     for (auto& ev : e.tev) HIP_CHECK(hipEventCreate(&ev));
     nlist_alloc(e);
     pme_alloc(e, qv);
@@ -505,11 +537,13 @@ void destroy(Engine& e) {
     if (e.s_main) (void)hipStreamSynchronize(e.s_main);
     nlist_free(e);
     pme_free(e);
-    void* ptrs[] = {e.p.w, e.p.img, e.p.v, e.p.fprev, e.p.prm, e.p.comp, e.w_try, e.img_try, e.f_acc, e.sd, e.first,
+    void* ptrs[] = {e.p.w, e.p.img, e.p.v, e.p.fprev, e.p.prm, e.p.comp, e.w_build, e.img_build, e.dmax2, e.w_try, e.img_try, e.f_acc, e.sd, e.first,
                     e.bmask, e.intra_items, e.intra_lut, e.bd.bid, e.bd.bp, e.bd.aid, e.bd.ap, e.bd.did, e.bd.dp,
                     e.bd.iid, e.bd.ip, e.bd.pid, e.bd.pp, e.acc, e.fmax};
     for (void* p : ptrs) if (p) (void)hipFree(p);
     for (auto& ev : e.tev) if (ev) (void)hipEventDestroy(ev);
+    if (e.ev_fork) (void)hipEventDestroy(e.ev_fork);
+    if (e.ev_join) (void)hipEventDestroy(e.ev_join);
     if (e.s_main) (void)hipStreamDestroy(e.s_main);
     if (e.s_pme) (void)hipStreamDestroy(e.s_pme);
 }
@@ -710,6 +744,9 @@ lima_status lima_get_status(lima_engine* h, lima_runstatus* out) {
     out->critical_error_occured = e.critical ? 1 : 0;
     out->simulation_finished = (e.prm.n_steps > 0 && e.step >= e.prm.n_steps) ? 1 : 0;
     out->n_list_rebuilds = e.n_rebuilds;
+    out->list_entries = e.nl.nent;
+    out->list_violations = e.list_violations;
+    out->max_displacement_nm = (float)e.last_dmax;
     out->greatest_force = NAN;
     return LIMA_OK;
 }
@@ -720,15 +757,20 @@ lima_status lima_get_neighbor_counts(lima_engine* h, int64_t* ncl, int64_t* nent
         Engine& e = h->e;
         nlist_build(e, e.p.w);
         HIP_CHECK(hipStreamSynchronize(e.s_main));
-        std::vector<float4> ent(e.nl.nent);
-        if (e.nl.nent) HIP_CHECK(hipMemcpy(ent.data(), e.nl.ent, e.nl.nent * sizeof(float4), hipMemcpyDeviceToHost));
+        const int nc = e.nl.ncl;
+        std::vector<int> cnt(nc);
+        HIP_CHECK(hipMemcpy(cnt.data(), e.nl.ecount, nc * sizeof(int), hipMemcpyDeviceToHost));
         int64_t n = 0;
-        for (auto& r : ent) {
-            uint32_t bits;
-            std::memcpy(&bits, &r.w, 4);
-            if (bits >> 24) n++;
+        for (int I = 0; I < nc; I++) {
+            std::vector<float4> ent(cnt[I]);
+            if (cnt[I]) HIP_CHECK(hipMemcpy(ent.data(), e.nl.ent + (int64_t)I * e.nl.cap, cnt[I] * sizeof(float4), hipMemcpyDeviceToHost));
+            for (auto& r : ent) {
+                uint32_t bits;
+                std::memcpy(&bits, &r.w, 4);
+                if (bits >> 24) n++;
+            }
         }
-        *ncl = e.nl.ncl;
+        *ncl = nc;
         *nent = n;
     });
 }
@@ -738,20 +780,21 @@ lima_status lima_get_neighbor_list(lima_engine* h, int32_t* cluster_atoms, int64
     return guarded([&] {
         Engine& e = h->e;
         const int ncl = e.nl.ncl;
-        std::vector<int> cat((size_t)ncl * CLUSTER), eoff(ncl + 1);
-        std::vector<float4> ent(e.nl.nent);
-        std::vector<uint8_t> code(e.nl.nent);
+        const int64_t cap = e.nl.cap;
+        std::vector<int> cat((size_t)ncl * CLUSTER), cnt(ncl);
+        std::vector<float4> ent((size_t)ncl * cap);
+        std::vector<uint8_t> code((size_t)ncl * cap);
         HIP_CHECK(hipMemcpy(cat.data(), e.nl.catoms, cat.size() * sizeof(int), hipMemcpyDeviceToHost));
-        HIP_CHECK(hipMemcpy(eoff.data(), e.nl.eoff, (ncl + 1) * sizeof(int), hipMemcpyDeviceToHost));
-        if (e.nl.nent) {
-            HIP_CHECK(hipMemcpy(ent.data(), e.nl.ent, e.nl.nent * sizeof(float4), hipMemcpyDeviceToHost));
-            HIP_CHECK(hipMemcpy(code.data(), e.nl.ecode, e.nl.nent, hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(cnt.data(), e.nl.ecount, ncl * sizeof(int), hipMemcpyDeviceToHost));
+        if (ncl) {
+            HIP_CHECK(hipMemcpy(ent.data(), e.nl.ent, ent.size() * sizeof(float4), hipMemcpyDeviceToHost));
+            HIP_CHECK(hipMemcpy(code.data(), e.nl.ecode, code.size(), hipMemcpyDeviceToHost));
         }
         if (cluster_atoms) std::memcpy(cluster_atoms, cat.data(), cat.size() * sizeof(int));
         int64_t n = 0;
         for (int I = 0; I < ncl; I++) {
             if (offsets) offsets[I] = n;
-            for (int k = eoff[I]; k < eoff[I + 1]; k++) {
+            for (int64_t k = I * cap; k < I * cap + cnt[I]; k++) {
                 uint32_t bits;
                 std::memcpy(&bits, &ent[k].w, 4);
                 const uint32_t m = bits >> 24;

@@ -4,9 +4,9 @@
 // code/LIMA_ENGINE/src/EngineKernels.cuh:47-146 with LennardJonesInteractions.cuh:164-203).
 // A wave owns one i-cluster (8 particles) and walks its entry list 8 entries at a time:
 // lane = 8*jj + ii computes pair (i-slot ii, entry jj), so one wave instruction covers 64 pairs.
-// Newton's third law is applied per pair; the j force is summed over the 8 lanes that share
-// an entry with three DPP adds and added to the j slot with one float atomic per component.
-// FP32 VALU bound: about 55 VALU slots per evaluated pair.
+// The list is full (both sides of every pair, as LIMA evaluates them): only i forces are
+// accumulated, so the loop has no atomics and the pair forces are deterministic.
+// FP32 VALU bound: about 50 VALU slots per evaluated pair (pair_scalar_cut in lima_dev.h).
 //
 // Intra kernel: LIMA's immediate-neighbour path (EngineKernels.cuh:150-263,
 // LennardJonesInteractions.cuh:121-161): pairs of the same or bonded compounds, BondedParticlesLUT
@@ -56,83 +56,125 @@ __global__ void k_pack(int nslots, const int* __restrict__ catoms, const uint4* 
 }
 
 struct PairArgs {
-    int ncl, nslots;
+    int ncl, nslots, cap;
+    int split;           // waves per i-cluster (each takes a contiguous share of its entries)
     float rc2;
     PairConst pc;
     const float4* xq;
     const float2* ljs;
-    const int* eoff;
+    const int* ecount;
     const float4* ent;
     float* fs;
     float* es;
     double* acc;
 };
 
+// One i-cluster per `split` waves: lane = 8 jj + ii computes pair (i slot ii, entry jj) of 8
+// consecutive entries.  The list is full (every pair appears from both sides), so only the i
+// force is accumulated: no atomics and no cross-lane traffic inside the loop, and the pair
+// forces are deterministic.
+//
+// Entries are staged through LDS 64 at a time: lane l loads entry l of the chunk and its j
+// particle with one coalesced load each, adds the image shift to the j position and stores
+// (x, y, z, q) and (sigma/2, sqrt(24 eps), mask bits) in the wave's LDS slice.  The 8 lanes of a
+// group then read their entry with two broadcast ds_read_b128, so the texture path sees each
+// datum once instead of eight times.  Two LDS buffers: chunk c + 1 is loaded while c is computed.
+constexpr int PAIR_WAVES = 4;
+constexpr int CHUNK = 64;
+
 template <bool ENERGY>
 __global__ __launch_bounds__(256) void k_pair(PairArgs a) {
-    const int lane = threadIdx.x & 63;
-    const int I = xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+    __shared__ float4 sA[PAIR_WAVES][2][CHUNK];
+    __shared__ float4 sB[PAIR_WAVES][2][CHUNK];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int wv = xcd_block(blockIdx.x, gridDim.x) * PAIR_WAVES + wid;
+    const int I = wv / a.split, part = wv - I * a.split;
     if (I >= a.ncl) return;
     const int ii = lane & 7, jj = lane >> 3;
     const int islot = I * CLUSTER + ii;
     const float4 xi = a.xq[islot];
     const float2 pi = a.ljs[islot];
-    float fx = 0.f, fy = 0.f, fz = 0.f, ei = 0.f, elj = 0.f, ecl = 0.f;
+    float fx = 0.f, fy = 0.f, fz = 0.f, elj = 0.f, ecl = 0.f;
     int npairs = 0;
-    const int e0 = a.eoff[I], e1 = a.eoff[I + 1];
-    for (int e = e0 + jj; e < e1; e += CLUSTER) {
-        const float4 rec = a.ent[e];
-        const uint32_t bits = __float_as_uint(rec.w);
-        const int b = (int)(bits & 0xffffffu);
-        const uint32_t m = bits >> 24;
-        const float4 xj = a.xq[b];
-        const float dx = (xj.x + rec.x) - xi.x, dy = (xj.y + rec.y) - xi.y, dz = (xj.z + rec.z) - xi.z;
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        const bool inc = ((m >> ii) & 1u) && (r2 <= a.rc2) && (r2 > 0.f);
-        const unsigned long long any = __ballot(inc);
-        if (any == 0ull) continue;
-        if (ENERGY) npairs += __popcll(any);
-        const float2 pj = a.ljs[b];
-        float el = 0.f, ec = 0.f;
-        float S = pair_scalar(inc ? r2 : 1.0f, pi.x + pj.x, pi.y * pj.y, xi.w * xj.w, a.pc, ENERGY ? &el : nullptr, &ec);
-        S = inc ? S : 0.f;
-        float tx = dx * S, ty = dy * S, tz = dz * S;
-        fx += tx; fy += ty; fz += tz;
-        tx = sum8(tx); ty = sum8(ty); tz = sum8(tz);
-        float ej = 0.f;
-        if (ENERGY) {
-            el = inc ? el : 0.f;
-            ec = inc ? ec : 0.f;
-            elj += el; ecl += ec;
-            ei += 0.5f * (el + ec);
-            ej = sum8(0.5f * (el + ec));
+    const int ntot = a.ecount[I] / CLUSTER;
+    const int it0 = (int)((int64_t)ntot * part / a.split), it1 = (int)((int64_t)ntot * (part + 1) / a.split);
+    const int64_t e0 = (int64_t)I * a.cap + (int64_t)it0 * CLUSTER;
+    const int nent = (it1 - it0) * CLUSTER;          // entries of this wave (multiple of 8)
+    const int nchunk = (nent + CHUNK - 1) / CHUNK;   // wave-uniform
+
+    float4 ga = make_float4(0.f, 0.f, 0.f, 0.f), gb = make_float4(0.f, 0.f, 0.f, 0.f);
+    auto fetch = [&](int c) {   // global loads of chunk c into registers (lane l: entry l)
+        const int k = c * CHUNK + lane;
+        if (k < nent) {
+            const float4 rec = a.ent[e0 + k];
+            const uint32_t bits = __float_as_uint(rec.w);
+            const int b = (int)(bits & 0xffffffu);
+            const float4 xj = a.xq[b];
+            const float2 pj = a.ljs[b];
+            ga = make_float4(xj.x + rec.x, xj.y + rec.y, xj.z + rec.z, xj.w);
+            gb = make_float4(pj.x, pj.y, __uint_as_float(bits >> 24), 0.f);
+        } else {
+            ga = make_float4(PAD_POS, PAD_POS, PAD_POS, 0.f);
+            gb = make_float4(0.f, 0.f, __uint_as_float(0u), 0.f);
         }
-        if (ii == 0) {
-            atomicAdd(&a.fs[b], -tx);
-            atomicAdd(&a.fs[a.nslots + b], -ty);
-            atomicAdd(&a.fs[2 * a.nslots + b], -tz);
-            if (ENERGY) atomicAdd(&a.es[b], ej);
+    };
+    if (nchunk > 0) {
+        fetch(0);
+        sA[wid][0][lane] = ga;
+        sB[wid][0][lane] = gb;
+        __builtin_amdgcn_wave_barrier();   // the wave's own LDS slice: ordering only, no s_barrier
+    }
+    for (int c = 0; c < nchunk; c++) {
+        const int buf = c & 1;
+        if (c + 1 < nchunk) fetch(c + 1);   // in flight during the compute below
+        const int nit = min(CHUNK, nent - c * CHUNK) / CLUSTER;
+        for (int k = 0; k < nit; k++) {
+            const float4 A = sA[wid][buf][k * CLUSTER + jj];
+            const float4 B = sB[wid][buf][k * CLUSTER + jj];
+            const uint32_t m = __float_as_uint(B.z);
+            const float dx = A.x - xi.x, dy = A.y - xi.y, dz = A.z - xi.z;
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            const bool inc = ((m >> ii) & 1u) && (r2 <= a.rc2) && (r2 > 0.f);
+            const unsigned long long any = __ballot(inc);
+            if (any == 0ull) continue;
+            if (ENERGY) npairs += __popcll(any);
+            float el = 0.f, ec = 0.f;
+            float S = pair_scalar_cut(inc ? r2 : 1.0f, pi.x + B.x, pi.y * B.y, xi.w * A.w, a.pc, ENERGY ? &el : nullptr, &ec);
+            S = inc ? S : 0.f;
+            fx += dx * S; fy += dy * S; fz += dz * S;
+            if (ENERGY) {
+                elj += inc ? 0.5f * el : 0.f;   // each pair is evaluated from both sides: half per side
+                ecl += inc ? 0.5f * ec : 0.f;
+            }
         }
+        if (c + 1 < nchunk) {
+            sA[wid][buf ^ 1][lane] = ga;
+            sB[wid][buf ^ 1][lane] = gb;
+        }
+        __builtin_amdgcn_wave_barrier();
     }
     // i force: lanes ii, ii+8, ..., ii+56 hold partial sums
     for (int o = 8; o < 64; o <<= 1) {
         fx += __shfl_xor(fx, o, 64);
         fy += __shfl_xor(fy, o, 64);
         fz += __shfl_xor(fz, o, 64);
-        if (ENERGY) ei += __shfl_xor(ei, o, 64);
+        if (ENERGY) { elj += __shfl_xor(elj, o, 64); ecl += __shfl_xor(ecl, o, 64); }
     }
     if (jj == 0) {
+        // atomics: the intra, bonded and PME kernels (PME on its own stream) add to the same slots
         atomicAdd(&a.fs[islot], fx);
         atomicAdd(&a.fs[a.nslots + islot], fy);
         atomicAdd(&a.fs[2 * a.nslots + islot], fz);
-        if (ENERGY) atomicAdd(&a.es[islot], ei);
+        if (ENERGY) atomicAdd(&a.es[islot], elj + ecl);
     }
     if (ENERGY) {
-        const float tl = wave_sum(elj), tc = wave_sum(ecl);
+        float tl = (jj == 0) ? elj : 0.f, tc = (jj == 0) ? ecl : 0.f;
+        tl = wave_sum(tl);
+        tc = wave_sum(tc);
         if (lane == 0) {
             atomicAdd(&a.acc[ACC_LJ], (double)tl);
             atomicAdd(&a.acc[ACC_COUL], (double)tc);
-            atomicAdd(&a.acc[ACC_PAIRS], (double)npairs);
+            atomicAdd(&a.acc[ACC_PAIRS], (double)npairs * 0.5);   // both sides counted
         }
     }
 }
@@ -382,11 +424,16 @@ void launch_pair(Engine& e, bool energy, hipStream_t s) {
     a.nslots = e.nl.ncl * CLUSTER;
     a.rc2 = e.rc2;
     a.pc = e.pc;
-    a.xq = e.nl.xq; a.ljs = e.nl.ljs; a.eoff = e.nl.eoff; a.ent = e.nl.ent;
+    a.cap = e.nl.cap;
+    a.xq = e.nl.xq; a.ljs = e.nl.ljs; a.ecount = e.nl.ecount; a.ent = e.nl.ent;
     a.fs = e.fs; a.es = e.es; a.acc = e.acc;
     if (a.ncl == 0) return;
-    if (energy) hipLaunchKernelGGL(k_pair<true>, dim3(nblk(a.ncl, 4)), dim3(256), 0, s, a);
-    else hipLaunchKernelGGL(k_pair<false>, dim3(nblk(a.ncl, 4)), dim3(256), 0, s, a);
+    // enough waves to keep ~16 per SIMD in flight (256 CUs x 4 SIMDs) on small boxes
+    a.split = 1;
+    while (a.split < 8 && (int64_t)a.ncl * a.split < 16384) a.split *= 2;
+    const int64_t waves = (int64_t)a.ncl * a.split;
+    if (energy) hipLaunchKernelGGL(k_pair<true>, dim3(nblk(waves, PAIR_WAVES)), dim3(64 * PAIR_WAVES), 0, s, a);
+    else hipLaunchKernelGGL(k_pair<false>, dim3(nblk(waves, PAIR_WAVES)), dim3(64 * PAIR_WAVES), 0, s, a);
 }
 
 void launch_intra(Engine& e, const uint4* w, const int4* img, bool energy, hipStream_t s) {

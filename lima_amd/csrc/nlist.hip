@@ -7,8 +7,10 @@
 // (oracle/lima_oracle.cpp, build_clusters) for identical positions:
 //   columns  n_d = max(1, floor(L_d / cbrt(8 V / N))), column = floor(w_d n_d / Lq_d)
 //   clusters 8 consecutive particles of a column ordered by (w_z, particle index)
-//   entries  particle b (cluster >= I) with shift s when dist^2(w_b + s Lq, bbox_I) <= R^2,
-//            R = round((cutoff + skin) 1e8); 8-bit mask of the i slots the pair is computed for.
+//   entries  every particle b with shift s when dist^2(w_b + s Lq, bbox_I) <= R^2 (full list:
+//            each pair appears once from each side), R = round((cutoff + skin) 1e8); 8-bit mask of
+//            the i slots the pair is computed for (not intra, not the particle itself).
+// Each i-cluster owns a fixed-capacity slice of the entry array (no count pass / scan).
 // HBM bound: per rebuild it reads positions and writes the entry list (16 B / entry).
 #include <algorithm>
 #include <cmath>
@@ -120,7 +122,7 @@ __global__ void k_cluster_props(int ncl, int ncol, const int* __restrict__ col_c
         const uint32_t v[3] = {p.x, p.y, p.z};
         for (int d = 0; d < 3; d++) { mn[d] = min(mn[d], v[d]); mx[d] = max(mx[d], v[d]); }
         const float4 q = prm[a];
-        ljs[I * CLUSTER + s] = make_float2(q.y, q.z);
+        ljs[I * CLUSTER + s] = make_float2(q.y, q.z * 4.898979485566356f);   // (sigma/2, sqrt(24 eps))
         comps[I * CLUSTER + s] = comp[a];
     }
     cmin[I] = make_uint4(mn[0], mn[1], mn[2], 0);
@@ -130,7 +132,8 @@ __global__ void k_cluster_props(int ncl, int ncol, const int* __restrict__ col_c
 }
 
 struct NlArgs {
-    int ncl, ncx, ncy, nC;
+    int ncl, ncx, ncy, nC, cap;
+    int* flags;
     long long Lq[3];
     long long R, R2;
     const int* col_cl_start;
@@ -165,8 +168,18 @@ __device__ __forceinline__ bool col_range(long long lo, long long hi, long long 
     return true;
 }
 
-// One wave per i-cluster.  FILL = false counts entries, FILL = true writes them.
-template <bool FILL>
+// first index in [lo, hi) whose cluster max z >= z (clusters of a column are z ordered)
+__device__ __forceinline__ int first_cluster_reaching(const uint4* cmax, int lo, int hi, long long z) {
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if ((long long)cmax[mid].z < z) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// One wave per i-cluster, one pass: entries go to the cluster's fixed-capacity slice
+// [I cap, (I+1) cap); the count (padded to a multiple of 8 with mask-0 entries) goes to ecount[I].
+// A slice that would overflow sets flags[1] and the host rebuilds with a larger capacity.
 __global__ __launch_bounds__(256) void k_nlist(NlArgs a) {
     const int lane = threadIdx.x & 63;
     const int I = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -174,12 +187,11 @@ __global__ __launch_bounds__(256) void k_nlist(NlArgs a) {
     const uint4 bmn = a.cmin[I], bmx = a.cmax[I], refI = a.cref[I];
     const long long mn[3] = {(long long)bmn.x, (long long)bmn.y, (long long)bmn.z};
     const long long mx[3] = {(long long)bmx.x, (long long)bmx.y, (long long)bmx.z};
-    // i-slot compounds (lane-uniform copies)
     int ci[CLUSTER];
     for (int s = 0; s < CLUSTER; s++) ci[s] = a.comps[I * CLUSTER + s];
     const int jsub = lane >> 3, sl = lane & 7;
     int count = 0;
-    const int base = FILL ? a.eoff[I] : 0;
+    const int64_t base = (int64_t)I * a.cap;
     for (int sy = -1; sy <= 1; sy++) {
         int cy0, cy1;
         if (!col_range(mn[1] - a.R - sy * a.Lq[1], mx[1] + a.R - sy * a.Lq[1], a.Lq[1], a.ncy, &cy0, &cy1)) continue;
@@ -190,53 +202,55 @@ __global__ __launch_bounds__(256) void k_nlist(NlArgs a) {
                 for (int cx = cx0; cx <= cx1; cx++) {
                     const int col = cy * a.ncx + cx;
                     const int cs = a.col_cl_start[col], ce = a.col_cl_start[col + 1];
+                    if (cs >= ce) continue;
                     for (int sz = -1; sz <= 1; sz++) {
                         const long long zlo = mn[2] - a.R - sz * a.Lq[2], zhi = mx[2] + a.R - sz * a.Lq[2];
                         if (zhi < 0 || zlo > a.Lq[2] - 1) continue;
                         const int code = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
                         const long long sh[3] = {sx * a.Lq[0], sy * a.Lq[1], sz * a.Lq[2]};
-                        for (int Jb = max(cs, I); Jb < ce; Jb += CLUSTER) {
+                        const int Jfirst = first_cluster_reaching(a.cmax, cs, ce, zlo);
+                        for (int Jb = Jfirst; Jb < ce; Jb += CLUSTER) {
+                            if ((long long)a.cmin[Jb].z > zhi) break;   // z ordered: nothing further reaches
                             const int J = Jb + jsub;
                             bool emit = false;
                             uint32_t mask = 0;
                             int bslot = 0;
-                            if (J < ce && !(J == I && code < 13)) {
-                                const uint4 jmn = a.cmin[J], jmx = a.cmax[J];
-                                if ((long long)jmx.z >= zlo && (long long)jmn.z <= zhi) {
-                                    bslot = J * CLUSTER + sl;
-                                    const int b = a.catoms[bslot];
-                                    if (b >= 0) {
-                                        const uint4 pb = a.w[b];
-                                        const long long pv[3] = {(long long)pb.x + sh[0], (long long)pb.y + sh[1], (long long)pb.z + sh[2]};
-                                        long long d2 = 0;
-                                        for (int d = 0; d < 3; d++) {
-                                            long long g = 0;
-                                            if (pv[d] < mn[d]) g = mn[d] - pv[d];
-                                            else if (pv[d] > mx[d]) g = pv[d] - mx[d];
-                                            d2 += g * g;
+                            if (J < ce && (long long)a.cmin[J].z <= zhi) {
+                                bslot = J * CLUSTER + sl;
+                                const int b = a.catoms[bslot];
+                                if (b >= 0) {
+                                    const uint4 pb = a.w[b];
+                                    const long long pv[3] = {(long long)pb.x + sh[0], (long long)pb.y + sh[1], (long long)pb.z + sh[2]};
+                                    long long d2 = 0;
+                                    for (int d = 0; d < 3; d++) {
+                                        long long g = 0;
+                                        if (pv[d] < mn[d]) g = mn[d] - pv[d];
+                                        else if (pv[d] > mx[d]) g = pv[d] - mx[d];
+                                        d2 += g * g;
+                                    }
+                                    if (d2 <= a.R2) {
+                                        const int cb = a.comps[bslot];
+                                        for (int s = 0; s < CLUSTER; s++) {
+                                            if (ci[s] < 0) continue;
+                                            if (is_intra(ci[s], cb, a.nC, a.bmask)) continue;
+                                            if (J == I && code == 13 && s == sl) continue;   // no self pair
+                                            mask |= 1u << s;
                                         }
-                                        if (d2 <= a.R2) {
-                                            const int cb = a.comps[bslot];
-                                            for (int s = 0; s < CLUSTER; s++) {
-                                                if (ci[s] < 0) continue;
-                                                if (is_intra(ci[s], cb, a.nC, a.bmask)) continue;
-                                                if (J == I && code == 13 && s >= sl) continue;
-                                                mask |= 1u << s;
-                                            }
-                                            emit = mask != 0;
-                                        }
+                                        emit = mask != 0;
                                     }
                                 }
                             }
                             const unsigned long long bal = __ballot(emit);
-                            if (FILL && emit) {
-                                const int pos = base + count + __popcll(bal & ((1ull << lane) - 1ull));
-                                const uint4 rj = a.cref[J];
-                                const float sxf = (float)((long long)rj.x - (long long)refI.x + sh[0]) * LIMA_LIMA_TO_NANO_F;
-                                const float syf = (float)((long long)rj.y - (long long)refI.y + sh[1]) * LIMA_LIMA_TO_NANO_F;
-                                const float szf = (float)((long long)rj.z - (long long)refI.z + sh[2]) * LIMA_LIMA_TO_NANO_F;
-                                a.ent[pos] = make_float4(sxf, syf, szf, __uint_as_float((uint32_t)bslot | (mask << 24)));
-                                a.ecode[pos] = (uint8_t)code;
+                            if (emit) {
+                                const int pos = count + __popcll(bal & ((1ull << lane) - 1ull));
+                                if (pos < a.cap) {
+                                    const uint4 rj = a.cref[J];
+                                    const float sxf = (float)((long long)rj.x - (long long)refI.x + sh[0]) * LIMA_LIMA_TO_NANO_F;
+                                    const float syf = (float)((long long)rj.y - (long long)refI.y + sh[1]) * LIMA_LIMA_TO_NANO_F;
+                                    const float szf = (float)((long long)rj.z - (long long)refI.z + sh[2]) * LIMA_LIMA_TO_NANO_F;
+                                    a.ent[base + pos] = make_float4(sxf, syf, szf, __uint_as_float((uint32_t)bslot | (mask << 24)));
+                                    a.ecode[base + pos] = (uint8_t)code;
+                                }
                             }
                             count += __popcll(bal);
                         }
@@ -246,13 +260,17 @@ __global__ __launch_bounds__(256) void k_nlist(NlArgs a) {
         }
     }
     const int padded = (count + CLUSTER - 1) & ~(CLUSTER - 1);
-    if (FILL) {
-        for (int k = count + lane; k < padded; k += 64) {
-            a.ent[base + k] = make_float4(0.f, 0.f, 0.f, __uint_as_float((uint32_t)(I * CLUSTER)));
-            a.ecode[base + k] = 13;
-        }
-    } else if (lane == 0) {
+    if (padded > a.cap) {
+        if (lane == 0) { atomicOr(&a.flags[1], 1); a.ecount[I] = 0; }
+        return;
+    }
+    for (int k = count + lane; k < padded; k += 64) {
+        a.ent[base + k] = make_float4(0.f, 0.f, 0.f, __uint_as_float((uint32_t)(I * CLUSTER)));
+        a.ecode[base + k] = 13;
+    }
+    if (lane == 0) {
         a.ecount[I] = padded;
+        atomicAdd(&a.flags[2], padded);
     }
 }
 
@@ -329,6 +347,7 @@ void nlist_alloc(Engine& e) {
     HIP_CHECK(hipMalloc(&e.fs, 3 * slots * sizeof(float)));
     HIP_CHECK(hipMalloc(&e.es, slots * sizeof(float)));
     nl.nent_cap = 0;
+    nl.cap = 2048;
     nl.ent = nullptr;
     nl.ecode = nullptr;
 }
@@ -341,10 +360,31 @@ void nlist_free(Engine& e) {
     for (void* p : ptrs) if (p) (void)hipFree(p);
 }
 
+namespace {
+
+__global__ void k_snapshot(int64_t N, const uint4* __restrict__ w, const int4* __restrict__ img, uint4* __restrict__ wb,
+                           int4* __restrict__ ib, float* dmax2) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p == 0) *dmax2 = 0.f;
+    if (p >= N) return;
+    wb[p] = w[p];
+    ib[p] = img[p];
+}
+
+}  // namespace
+
 void nlist_build(Engine& e, const uint4* w) {
     NList& nl = e.nl;
     hipStream_t s = e.s_main;
     const int64_t N = e.N;
+    {   // a list is stale once two particles may have closed the skin: 2 max|dx| > skin
+        float d2 = 0.f;
+        HIP_CHECK(hipMemcpyAsync(&d2, e.dmax2, sizeof(float), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        e.last_dmax = std::sqrt((double)d2);
+        if (e.n_rebuilds > 0 && 2.0 * e.last_dmax > (double)e.skin) e.list_violations++;
+    }
+    if (w == e.p.w) hipLaunchKernelGGL(k_snapshot, dim3(nblk(N, 256)), dim3(256), 0, s, N, e.p.w, e.p.img, e.w_build, e.img_build, e.dmax2);
     hipLaunchKernelGGL(k_fill_int, dim3(nblk(nl.ncol + 1, 256)), dim3(256), 0, s, nl.col_count, nl.ncol + 1, 0);
     hipLaunchKernelGGL(k_fill_int, dim3(nblk(nl.ncol + 1, 256)), dim3(256), 0, s, nl.col_fill, nl.ncol + 1, 0);
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(4), 0, s, nl.flags, 4, 0);
@@ -374,23 +414,30 @@ void nlist_build(Engine& e, const uint4* w) {
     a.R = llround(((double)e.rc + (double)e.skin) * 1e8);
     a.R2 = a.R * a.R;
     a.col_cl_start = col_cl_start; a.catoms = nl.catoms; a.comps = nl.comps; a.bmask = e.bmask; a.w = w;
-    a.cref = nl.cref; a.cmin = nl.cmin; a.cmax = nl.cmax; a.ecount = nl.ecount; a.eoff = nl.eoff;
-    hipLaunchKernelGGL(k_nlist<false>, dim3(nblk(ncl, 4)), dim3(256), 0, s, a);
-    launch_exclusive_scan(nl.ecount, nl.eoff, ncl, s);
-    int total = 0;
-    HIP_CHECK(hipMemcpyAsync(&total, nl.eoff + ncl, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipStreamSynchronize(s));
-    if (total < 0) throw LimaFail(LIMA_ERR_LIST_OVERFLOW, "pair list exceeds 2^31 entries");
-    if (total > nl.nent_cap) {
-        if (nl.ent) HIP_CHECK(hipFree(nl.ent));
-        if (nl.ecode) HIP_CHECK(hipFree(nl.ecode));
-        nl.nent_cap = (int64_t)(total * 1.25) + 1024;
-        HIP_CHECK(hipMalloc(&nl.ent, nl.nent_cap * sizeof(float4)));
-        HIP_CHECK(hipMalloc(&nl.ecode, nl.nent_cap));
+    a.cref = nl.cref; a.cmin = nl.cmin; a.cmax = nl.cmax; a.ecount = nl.ecount; a.flags = nl.flags;
+    for (int attempt = 0;; attempt++) {
+        const int64_t need = (int64_t)nl.ncl_max * nl.cap;
+        if (need > nl.nent_cap) {
+            if (nl.ent) HIP_CHECK(hipFree(nl.ent));
+            if (nl.ecode) HIP_CHECK(hipFree(nl.ecode));
+            nl.ent = nullptr; nl.ecode = nullptr;
+            HIP_CHECK(hipMalloc(&nl.ent, need * sizeof(float4)));
+            HIP_CHECK(hipMalloc(&nl.ecode, need));
+            nl.nent_cap = need;
+        }
+        a.cap = nl.cap; a.ent = nl.ent; a.ecode = nl.ecode;
+        hipLaunchKernelGGL(k_nlist, dim3(nblk(ncl, 4)), dim3(256), 0, s, a);
+        HIP_CHECK(hipMemcpyAsync(hflags, nl.flags, sizeof(hflags), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        if (!hflags[1]) {
+            nl.nent = hflags[2];
+            break;
+        }
+        if (attempt >= 6) throw LimaFail(LIMA_ERR_LIST_OVERFLOW, "pair list capacity per cluster exceeded");
+        nl.cap = (nl.cap * 3 / 2 + 7) & ~7;
+        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(4), 0, s, nl.flags, 4, 0);
     }
-    nl.nent = total;
-    a.ent = nl.ent; a.ecode = nl.ecode;
-    hipLaunchKernelGGL(k_nlist<true>, dim3(nblk(ncl, 4)), dim3(256), 0, s, a);
+    pme_bin(e, w, s);
     HIP_CHECK(hipGetLastError());
     e.steps_since_build = 0;
     e.n_rebuilds++;

@@ -20,8 +20,6 @@ namespace lima {
 
 namespace {
 
-constexpr int MAXR = 5;
-
 inline int nblk(int64_t n, int b) { return (int)((n + b - 1) / b); }
 
 struct FftPlan {
@@ -37,7 +35,8 @@ __device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2
 // multiply by -i (forward) or +i (inverse)
 __device__ __forceinline__ float2 rot(float2 a, bool inv) { return inv ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x); }
 
-__device__ __forceinline__ void dft_r(float2* u, int R, bool inv) {
+template <int R>
+__device__ __forceinline__ void dft(float2* u, bool inv) {
     if (R == 2) {
         const float2 a = u[0], b = u[1];
         u[0] = cadd(a, b);
@@ -74,6 +73,38 @@ __device__ __forceinline__ void dft_r(float2* u, int R, bool inv) {
     }
 }
 
+// One radix-R Stockham stage over nl lines of length N (p = product of the earlier radices):
+// butterfly k of a line reads a[k + t N/R], twiddles by exp(-+2 pi i t (k mod p) / (p R)) and
+// writes b[(k - k mod p) R + k mod p + t p].  Index divisions use float reciprocals, exact for
+// these sizes (nl * N <= 64 K).
+template <int R>
+__device__ __forceinline__ void fft_stage(const float2* __restrict__ a, float2* __restrict__ b, int nl, int N, int p,
+                                          const float2* __restrict__ tw, bool inv) {
+    const int nb = N / R;
+    const int stride = N / (p * R);
+    const float rnb = 1.0f / (float)nb, rp = 1.0f / (float)p;
+    for (int idx = threadIdx.x; idx < nl * nb; idx += blockDim.x) {
+        const int l = (int)(((float)idx + 0.5f) * rnb);
+        const int k = idx - l * nb;
+        const int q = (int)(((float)k + 0.5f) * rp);
+        const int i = k - q * p;
+        const float2* src = a + l * N + k;
+        float2 u[R];
+#pragma unroll
+        for (int t = 0; t < R; t++) u[t] = src[t * nb];
+#pragma unroll
+        for (int t = 1; t < R; t++) {
+            float2 w = tw[t * i * stride];
+            if (inv) w.y = -w.y;
+            u[t] = cmul(u[t], w);
+        }
+        dft<R>(u, inv);
+        float2* dst = b + l * N + q * p * R + i;
+#pragma unroll
+        for (int t = 0; t < R; t++) dst[t * p] = u[t];
+    }
+}
+
 // Stockham auto-sort FFT of nl lines of length N in LDS (line-major); returns the buffer
 // holding the result (a or b).  All threads of the block participate.
 __device__ float2* fft_lines(float2* a, float2* b, int nl, const FftPlan& P, bool inv) {
@@ -81,21 +112,11 @@ __device__ float2* fft_lines(float2* a, float2* b, int nl, const FftPlan& P, boo
     int p = 1;
     for (int st = 0; st < P.nrad; st++) {
         const int R = P.radix[st];
-        const int nb = N / R;
-        const int stride = N / (p * R);
-        for (int idx = threadIdx.x; idx < nl * nb; idx += blockDim.x) {
-            const int l = idx / nb, k = idx - l * nb;
-            const int i = k % p;
-            float2 u[MAXR];
-            for (int t = 0; t < R; t++) u[t] = a[l * N + k + t * nb];
-            for (int t = 1; t < R; t++) {
-                float2 w = P.tw[(t * i * stride) % N];
-                if (inv) w.y = -w.y;
-                u[t] = cmul(u[t], w);
-            }
-            dft_r(u, R, inv);
-            const int j = (k - i) * R + i;
-            for (int t = 0; t < R; t++) b[l * N + j + t * p] = u[t];
+        switch (R) {
+            case 2: fft_stage<2>(a, b, nl, N, p, P.tw, inv); break;
+            case 3: fft_stage<3>(a, b, nl, N, p, P.tw, inv); break;
+            case 4: fft_stage<4>(a, b, nl, N, p, P.tw, inv); break;
+            default: fft_stage<5>(a, b, nl, N, p, P.tw, inv); break;
         }
         __syncthreads();
         float2* tmp = a; a = b; b = tmp;
@@ -154,6 +175,129 @@ __global__ void k_spread(SpreadArgs a) {
                 atomicAdd((unsigned long long*)&row[X], (unsigned long long)v);
             }
         }
+    }
+}
+
+struct TileArgs {
+    int64_t N;
+    long long Lq[3];
+    int K[3];
+    int T[3], margin, E[3];
+    int nt[3];
+    int bincap;
+    const uint4* w;
+    const float4* prm;
+    int* bin_atoms;
+    int* bin_count;
+    long long* tblock;
+    int* flags;
+};
+
+// Charged particles -> fixed-capacity bin of the tile holding their current base node.
+// The order inside a bin does not matter: the block sums are integer.
+__global__ void k_bin(TileArgs a) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.N) return;
+    if (a.prm[p].x == 0.f) return;
+    const uint4 pw = a.w[p];
+    int ix[3];
+    float f;
+    grid_coord(pw.x, a.Lq[0], a.K[0], &ix[0], &f);
+    grid_coord(pw.y, a.Lq[1], a.K[1], &ix[1], &f);
+    grid_coord(pw.z, a.Lq[2], a.K[2], &ix[2], &f);
+    const int t = ((ix[2] / a.T[2]) * a.nt[1] + ix[1] / a.T[1]) * a.nt[0] + ix[0] / a.T[0];
+    const int k = atomicAdd(&a.bin_count[t], 1);
+    if (k < a.bincap) a.bin_atoms[(int64_t)t * a.bincap + k] = (int)p;
+    else atomicOr(a.flags, 1);
+}
+
+// One block per tile: particles of the bin add q w_x w_y w_z (fixed point) into an E^3 LDS block
+// whose origin is the tile corner minus one node (E = T + 3); the block is stored as is.
+__global__ __launch_bounds__(256) void k_spread_tiles(TileArgs a) {
+    extern __shared__ unsigned long long blk[];
+    const int t = blockIdx.x;
+    const int Ex = a.E[0], Ey = a.E[1], E3 = a.E[0] * a.E[1] * a.E[2];
+    const int tx = t % a.nt[0], ty = (t / a.nt[0]) % a.nt[1], tz = t / (a.nt[0] * a.nt[1]);
+    const int o[3] = {tx * a.T[0] - 1 - a.margin, ty * a.T[1] - 1 - a.margin, tz * a.T[2] - 1 - a.margin};
+    for (int i = threadIdx.x; i < E3; i += blockDim.x) blk[i] = 0ull;
+    __syncthreads();
+    const int nb = min(a.bin_count[t], a.bincap);
+    for (int k = threadIdx.x; k < nb; k += blockDim.x) {
+        const int p = a.bin_atoms[(int64_t)t * a.bincap + k];
+        const float q = a.prm[p].x;
+        const uint4 pw = a.w[p];
+        int ix[3];
+        float f[3], wx[4], wy[4], wz[4];
+        grid_coord(pw.x, a.Lq[0], a.K[0], &ix[0], &f[0]);
+        grid_coord(pw.y, a.Lq[1], a.K[1], &ix[1], &f[1]);
+        grid_coord(pw.z, a.Lq[2], a.K[2], &ix[2], &f[2]);
+        bspline(f[0], wx); bspline(f[1], wy); bspline(f[2], wz);
+        int l[3];
+        bool ok = true;
+        for (int d = 0; d < 3; d++) {
+            int v = ix[d] - 1 - o[d];   // local index of the first stencil node
+            v = v < 0 ? v + a.K[d] : (v >= a.K[d] ? v - a.K[d] : v);
+            l[d] = v;
+            ok = ok && v >= 0 && v + 3 < a.E[d];
+        }
+        if (!ok) { atomicOr(a.flags, 2); continue; }   // cannot happen with current-position bins
+        for (int c = 0; c < 4; c++)
+            for (int b = 0; b < 4; b++) {
+                const float qyz = q * wy[b] * wz[c];
+                unsigned long long* row = blk + ((l[2] + c) * Ey + (l[1] + b)) * Ex + l[0];
+                for (int x = 0; x < 4; x++)
+                    atomicAdd(&row[x], (unsigned long long)__float2ll_rn(qyz * wx[x] * 4294967296.0f));
+            }
+    }
+    __syncthreads();
+    long long* out = a.tblock + (int64_t)t * E3;
+    for (int i = threadIdx.x; i < E3; i += blockDim.x) out[i] = (long long)blk[i];
+}
+
+// fixed-point charge at grid point (x, y, z) summed over the tile blocks that cover it
+__device__ __forceinline__ long long tiled_q(const long long* __restrict__ tblock, const int4* __restrict__ cx,
+                                            const int4* __restrict__ cy, const int4* __restrict__ cz, int x, int y, int z,
+                                            int Ex, int Ey, int E3, int ntx, int nty) {
+    const int4 X = cx[x], Y = cy[y], Z = cz[z];
+    long long acc = 0;
+    const int zt[2] = {Z.x, Z.z}, zl[2] = {Z.y, Z.w};
+    const int yt[2] = {Y.x, Y.z}, yl[2] = {Y.y, Y.w};
+    const int xt[2] = {X.x, X.z}, xl[2] = {X.y, X.w};
+    for (int c = 0; c < 2; c++) {
+        if (zt[c] < 0) continue;
+        for (int b = 0; b < 2; b++) {
+            if (yt[b] < 0) continue;
+            for (int k = 0; k < 2; k++) {
+                if (xt[k] < 0) continue;
+                const int t = (zt[c] * nty + yt[b]) * ntx + xt[k];
+                acc += tblock[(int64_t)t * E3 + (zl[c] * Ey + yl[b]) * Ex + xl[k]];
+            }
+        }
+    }
+    return acc;
+}
+
+// x pass, forward, reading the tile blocks instead of a global grid.
+__global__ __launch_bounds__(256) void k_fft_x_fwd_tiled(FftPlan P, int Ky, int H, int TY, float scale, const long long* __restrict__ tblock,
+                                                         const int4* __restrict__ cx, const int4* __restrict__ cy,
+                                                         const int4* __restrict__ cz, int Ex, int Ey, int E3, int ntx, int nty,
+                                                         float2* __restrict__ cg) {
+    extern __shared__ float2 lds[];
+    const int N = P.N;
+    const int z = blockIdx.y, y0 = blockIdx.x * TY;
+    const int nl = min(TY, Ky - y0);
+    float2* a = lds;
+    float2* b = lds + TY * N;
+    for (int i = threadIdx.x; i < nl * N; i += blockDim.x) {
+        const int l = i / N, x = i - l * N;
+        const long long v = tiled_q(tblock, cx, cy, cz, x, y0 + l, z, Ex, Ey, E3, ntx, nty);
+        a[i] = make_float2(__ll2float_rn(v) * (1.0f / 4294967296.0f) * scale, 0.f);
+    }
+    __syncthreads();
+    const float2* r = fft_lines(a, b, nl, P, false);
+    for (int i = threadIdx.x; i < nl * H; i += blockDim.x) {
+        const int l = i / H, k = i - l * H;
+        cg[((int64_t)z * Ky + y0 + l) * H + k] = r[l * N + k];
     }
 }
 
@@ -349,7 +493,34 @@ FftPlan make_plan(const Pme& m, int d) {
     return P;
 }
 
+TileArgs tile_args(Engine& e, const uint4* w) {
+    Pme& m = e.pme;
+    TileArgs a{};
+    a.N = e.N;
+    for (int d = 0; d < 3; d++) { a.Lq[d] = e.Lq[d]; a.K[d] = m.K[d]; a.nt[d] = m.nt[d]; }
+    for (int d = 0; d < 3; d++) { a.T[d] = m.T[d]; a.E[d] = m.E[d]; }
+    a.margin = m.margin;
+    a.bincap = m.bincap;
+    a.w = w; a.prm = e.p.prm; a.bin_atoms = m.bin_atoms; a.bin_count = m.bin_count; a.tblock = m.tblock; a.flags = m.flags;
+    return a;
+}
+
+__global__ void k_zero_int(int* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
 }  // namespace
+
+void pme_bin(Engine& e, const uint4* w, hipStream_t s) {   // checked at every list rebuild
+    (void)w;
+    Pme& m = e.pme;
+    if (!m.tiled || !e.elec || m.ncharged == 0) return;
+    int hflag = 0;
+    HIP_CHECK(hipMemcpyAsync(&hflag, m.flags, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    if (hflag) throw LimaFail(LIMA_ERR_LIST_OVERFLOW, "a PME tile bin overflowed (more than 512 charged particles in one tile)");
+}
 
 void pme_alloc(Engine& e, const std::vector<float>& q) {
     Pme& m = e.pme;
@@ -401,6 +572,53 @@ void pme_alloc(Engine& e, const std::vector<float>& q) {
                 g[((size_t)z * m.K[1] + y) * m.H + x] = (float)G;
             }
     HIP_CHECK(hipMemcpy(m.green, g.data(), NC * sizeof(float), hipMemcpyHostToDevice));
+    // tiled spreading needs every tile block to fit the grid without wrapping onto itself
+    {
+        double hmin = 1e30;
+        for (int d = 0; d < 3; d++) hmin = std::min(hmin, e.L[d] / m.K[d]);
+        (void)hmin;
+        m.margin = 0;   // bins are rebuilt from the current positions every step
+        // per axis a tile edge T dividing K (5..12, closest to 8): with T >= 3 + 2 margin every grid
+        // point is covered by at most two tile blocks per axis
+        m.tiled = true;
+        for (int d = 0; d < 3; d++) {
+            int best = 0;
+            for (int t = 5; t <= 12; t++)
+                if (m.K[d] % t == 0 && t >= 3 + 2 * m.margin && (best == 0 || std::abs(t - 8) < std::abs(best - 8))) best = t;
+            if (best == 0) { m.tiled = false; best = 8; }
+            m.T[d] = best;
+            m.E[d] = best + 3 + 2 * m.margin;
+            m.nt[d] = m.K[d] / best;
+            if (m.K[d] < m.E[d] || m.nt[d] < 2) m.tiled = false;
+        }
+        if ((int64_t)m.E[0] * m.E[1] * m.E[2] * 8 > 56 * 1024) m.tiled = false;   // LDS block size
+        m.ntiles = m.nt[0] * m.nt[1] * m.nt[2];
+        if (m.tiled) {
+            HIP_CHECK(hipMalloc(&m.bin_count, (m.ntiles + 1) * sizeof(int)));
+            HIP_CHECK(hipMalloc(&m.bin_atoms, (int64_t)m.ntiles * m.bincap * sizeof(int)));
+            HIP_CHECK(hipMalloc(&m.tblock, (int64_t)m.ntiles * m.E[0] * m.E[1] * m.E[2] * sizeof(long long)));
+            HIP_CHECK(hipMalloc(&m.flags, sizeof(int)));
+            HIP_CHECK(hipMemset(m.flags, 0, sizeof(int)));
+            // per grid coordinate: the (at most two) tiles whose block covers it and the local index
+            for (int d = 0; d < 3; d++) {
+                std::vector<int4> cov(m.K[d]);
+                for (int c = 0; c < m.K[d]; c++) {
+                    int found[2][2] = {{-1, 0}, {-1, 0}}, nf = 0;
+                    for (int t = 0; t < m.nt[d]; t++) {
+                        int loc = c - (t * m.T[d] - 1 - m.margin);
+                        loc = ((loc % m.K[d]) + m.K[d]) % m.K[d];
+                        if (loc < m.E[d]) {
+                            if (nf >= 2) throw LimaFail(LIMA_ERR_BAD_INPUT, "PME tile cover exceeds two tiles");
+                            found[nf][0] = t; found[nf][1] = loc; nf++;
+                        }
+                    }
+                    cov[c] = make_int4(found[0][0], found[0][1], found[1][0], found[1][1]);
+                }
+                HIP_CHECK(hipMalloc(&m.cover[d], m.K[d] * sizeof(int4)));
+                HIP_CHECK(hipMemcpy(m.cover[d], cov.data(), m.K[d] * sizeof(int4), hipMemcpyHostToDevice));
+            }
+        }
+    }
     double qsq = 0;
     m.ncharged = 0;
     for (float v : q) { qsq += (double)v * v; if (v != 0.f) m.ncharged++; }
@@ -410,7 +628,8 @@ void pme_alloc(Engine& e, const std::vector<float>& q) {
 
 void pme_free(Engine& e) {
     Pme& m = e.pme;
-    void* ptrs[] = {m.qgrid, m.phi, m.cgrid, m.green, m.tw[0], m.tw[1], m.tw[2]};
+    void* ptrs[] = {m.qgrid, m.phi, m.cgrid, m.green, m.tw[0], m.tw[1], m.tw[2], m.bin_count, m.bin_atoms, m.tblock,
+                    m.cover[0], m.cover[1], m.cover[2], m.flags};
     for (void* p : ptrs) if (p) (void)hipFree(p);
 }
 
@@ -419,18 +638,29 @@ void launch_pme(Engine& e, const uint4* w, bool energy, hipStream_t s) {
     Pme& m = e.pme;
     const int Kx = m.K[0], Ky = m.K[1], Kz = m.K[2], H = m.H;
     const int64_t NG = (int64_t)Kx * Ky * Kz;
-    hipLaunchKernelGGL(k_zero_ll, dim3(nblk(NG, 256)), dim3(256), 0, s, m.qgrid, NG);
-    SpreadArgs sa{};
-    sa.N = e.N;
-    for (int d = 0; d < 3; d++) { sa.Lq[d] = e.Lq[d]; sa.K[d] = m.K[d]; }
-    sa.w = w; sa.prm = e.p.prm; sa.qgrid = m.qgrid;
-    hipLaunchKernelGGL(k_spread, dim3(nblk(e.N, 256)), dim3(256), 0, s, sa);
     const double vcell = (e.L[0] / Kx) * (e.L[1] / Ky) * (e.L[2] / Kz);
     const FftPlan Px = make_plan(m, 0), Py = make_plan(m, 1), Pz = make_plan(m, 2);
     // LDS tiles stay <= 48 KB: 2 buffers x lines x length x 8 B
     const int TY = std::max(1, std::min(Ky, 3072 / Kx));
-    hipLaunchKernelGGL(k_fft_x_fwd, dim3(nblk(Ky, TY), Kz), dim3(256), 2 * TY * Kx * sizeof(float2), s, Px, Ky, H, TY,
-                       (float)(1.0 / vcell), m.qgrid, m.cgrid);
+    if (m.tiled) {
+        TileArgs ta = tile_args(e, w);
+        hipLaunchKernelGGL(k_zero_int, dim3(nblk(m.ntiles, 256)), dim3(256), 0, s, m.bin_count, m.ntiles);
+        hipLaunchKernelGGL(k_bin, dim3(nblk(e.N, 256)), dim3(256), 0, s, ta);
+        const int E3 = m.E[0] * m.E[1] * m.E[2];
+        hipLaunchKernelGGL(k_spread_tiles, dim3(m.ntiles), dim3(256), (size_t)E3 * sizeof(long long), s, ta);
+        hipLaunchKernelGGL(k_fft_x_fwd_tiled, dim3(nblk(Ky, TY), Kz), dim3(256), 2 * TY * Kx * sizeof(float2), s, Px, Ky, H,
+                           TY, (float)(1.0 / vcell), m.tblock, m.cover[0], m.cover[1], m.cover[2], m.E[0], m.E[1], E3, m.nt[0],
+                           m.nt[1], m.cgrid);
+    } else {
+        hipLaunchKernelGGL(k_zero_ll, dim3(nblk(NG, 256)), dim3(256), 0, s, m.qgrid, NG);
+        SpreadArgs sa{};
+        sa.N = e.N;
+        for (int d = 0; d < 3; d++) { sa.Lq[d] = e.Lq[d]; sa.K[d] = m.K[d]; }
+        sa.w = w; sa.prm = e.p.prm; sa.qgrid = m.qgrid;
+        hipLaunchKernelGGL(k_spread, dim3(nblk(e.N, 256)), dim3(256), 0, s, sa);
+        hipLaunchKernelGGL(k_fft_x_fwd, dim3(nblk(Ky, TY), Kz), dim3(256), 2 * TY * Kx * sizeof(float2), s, Px, Ky, H, TY,
+                           (float)(1.0 / vcell), m.qgrid, m.cgrid);
+    }
     const int TKy = std::max(1, std::min(16, 3072 / Ky)), TKz = std::max(1, std::min(16, 3072 / Kz));
     // y pass: outer z, lines along y (stride H)
     hipLaunchKernelGGL(k_fft_strided, dim3(nblk(H, TKy), Kz), dim3(256), 2 * TKy * Ky * sizeof(float2), s, Py, H, TKy,

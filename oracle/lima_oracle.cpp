@@ -523,12 +523,12 @@ void evaluate(lima_oracle* o, int comps, std::vector<double>& F, std::vector<dou
  *  - within a column particles are ordered by (w_z, global index) and cut into clusters
  *    of 8 consecutive particles; clusters are numbered column by column (x fastest).
  *  - cluster bounding box: integer min/max of the members' wrapped Coords.
- *  - cluster I lists particle b with image shift s in {-1,0,1}^3 when cluster(b) > I, or
- *    cluster(b) == I with s == 0 or a positive shift code, and the integer squared distance
- *    from w_b + s Lq to I's box is <= round((rc+skin) 1e8)^2.
+ *  - cluster I lists every particle b with image shift s in {-1,0,1}^3 whose integer squared
+ *    distance from w_b + s Lq to I's box is <= round((rc+skin) 1e8)^2 (a full list: the pair
+ *    kernel computes each pair from both sides, force on the i side only, as LIMA does).
  *  - the entry's 8-bit mask has bit i set for real slot i of I whose pair with b is not
- *    intra (same / bonded compound), and for s == 0 inside I only when slot i < slot(b).
- *    Entries with an empty mask are dropped.
+ *    intra (same / bonded compound) and is not b itself (s == 0).  Entries with an empty mask
+ *    are dropped.
  *  shift code = (sx+1) + 3 (sy+1) + 9 (sz+1); entry = global index of b | code << 24.
  * ------------------------------------------------------------------------------------- */
 struct ClusterList {
@@ -590,9 +590,7 @@ ClusterList build_clusters(const lima_oracle* o) {
         auto& lst = lists[I];
         for (int64_t b = 0; b < o->N; b++) {
             const int64_t J = clof[b];
-            if (J < I) continue;
             for (int code = 0; code < 27; code++) {
-                if (J == I && code < 13) continue;
                 const int s[3] = {code % 3 - 1, (code / 3) % 3 - 1, code / 9 - 1};
                 int64_t d2 = 0;
                 for (int d = 0; d < 3; d++) {
@@ -608,7 +606,7 @@ ClusterList build_clusters(const lima_oracle* o) {
                     const int32_t a = cl.atoms[I * 8 + i];
                     if (a < 0) continue;
                     if (o->intra(a, b)) continue;
-                    if (J == I && code == 13 && !(i < slotof[b])) continue;
+                    if (J == I && code == 13 && i == slotof[b]) continue;
                     m |= (uint8_t)(1u << i);
                 }
                 if (m) lst.push_back({(uint32_t)b | ((uint32_t)code << 24), m});
