@@ -91,6 +91,13 @@ struct Bonded {
 // Device accumulators (FP64 energy reductions and scalars).
 enum Acc { ACC_LJ = 0, ACC_COUL, ACC_BOND, ACC_PME, ACC_KIN, ACC_PAIRS, ACC_N };
 
+// FP64 energy reduction without same-address atomics: every wave of an energy-producing kernel
+// writes one record of ACC_N doubles into its kernel's region of `epart`; k_reduce_energy sums
+// the records in a fixed order (deterministic totals).
+struct ERegions {
+    int pair = 0, intra = 0, bond = 0, pme = 0, kin = 0, total = 0;
+};
+
 struct SdState {            // steepest descent state, device resident
     double e_acc, e_try;
     float fmax_acc, fmax_try;
@@ -116,7 +123,8 @@ struct Engine {
     // SD trial state
     uint4* w_build = nullptr;  // positions at the last list rebuild (displacement check)
     int4* img_build = nullptr;
-    float* dmax2 = nullptr;    // max squared displacement since the last rebuild [nm^2]
+    float* dmax2 = nullptr;    // [1] max squared displacement since the last rebuild [nm^2]
+    float* dpart = nullptr;    // [N/64] per-wave running max of the squared displacement
     int64_t list_violations = 0;
     double last_dmax = 0;
     uint4* w_try = nullptr;
@@ -142,6 +150,9 @@ struct Engine {
     float* fs = nullptr;       // [3*slots] forces per cluster slot (SoA)
     float* es = nullptr;       // [slots] per-slot potE
     double* acc = nullptr;     // [ACC_N]
+    double* epart = nullptr;   // [epart_cap * ACC_N] per-wave energy records
+    int64_t epart_cap = 0;
+    ERegions ereg;
     float* fmax = nullptr;     // [1]
 
     hipStream_t s_main = nullptr, s_pme = nullptr;
@@ -182,6 +193,9 @@ void nlist_free(Engine& e);
 void nlist_build(Engine& e, const uint4* w);   // rebuilds clusters + lists from positions w
 
 // ---- forces.hip ----
+int pair_split(const Engine& e);
+ERegions energy_regions(const Engine& e);
+void launch_reduce_energy(Engine& e, bool kinetic, hipStream_t s);
 void launch_pack(Engine& e, const uint4* w, hipStream_t s);
 void launch_pair(Engine& e, bool energy, hipStream_t s);
 void launch_intra(Engine& e, const uint4* w, const int4* img, bool energy, hipStream_t s);

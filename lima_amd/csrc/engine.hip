@@ -39,7 +39,8 @@ struct IntegrateArgs {
     float thermostat;
     const uint4* w_build;
     const int4* img_build;
-    float* dmax2;
+    float* dpart;          // per-wave running max of the squared displacement since the list build
+    double* epart;         // kinetic-energy region: one record per wave
     const int* slot_of;
     const float* fs;
     uint4* w;
@@ -47,7 +48,6 @@ struct IntegrateArgs {
     float4* v;
     float4* fprev;
     const float4* prm;
-    double* acc;
 };
 
 // w + d wrapped back into [0, Lq) for any d (a step may exceed a box edge when a run explodes).
@@ -103,22 +103,30 @@ __global__ void k_integrate(IntegrateArgs a) {
         a.fprev[p] = make_float4(fx, fy, fz, 0.f);
     }
     d2 = wave_max(d2);
-    if ((threadIdx.x & 63) == 0) atomic_max_pos(a.dmax2, d2);
+    const int64_t wv = p >> 6;
+    const bool wave_in = (p & ~63ll) < a.N;   // waves inside the per-wave arrays
+    if ((threadIdx.x & 63) == 0 && wave_in) a.dpart[wv] = fmaxf(a.dpart[wv], d2);
     if (ENERGY) {
-        const float t = wave_sum(ke);
-        if ((threadIdx.x & 63) == 0) atomicAdd(&a.acc[ACC_KIN], (double)t);
+        const double t = wave_sum_d((double)ke);
+        if ((threadIdx.x & 63) == 0 && wave_in) {
+            double* r = a.epart + wv * ACC_N;
+            r[ACC_LJ] = 0.0; r[ACC_COUL] = 0.0; r[ACC_BOND] = 0.0; r[ACC_PME] = 0.0; r[ACC_KIN] = t; r[ACC_PAIRS] = 0.0;
+        }
     }
 }
 
-__global__ void k_kinetic(int64_t N, const float4* __restrict__ v, const float4* __restrict__ prm, double* acc) {
+__global__ void k_kinetic(int64_t N, const float4* __restrict__ v, const float4* __restrict__ prm, double* epart) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     float ke = 0.f;
     if (p < N) {
         const float4 u = v[p];
         ke = 0.5f * prm[p].w * (u.x * u.x + u.y * u.y + u.z * u.z);
     }
-    const float t = wave_sum(ke);
-    if ((threadIdx.x & 63) == 0) atomicAdd(&acc[ACC_KIN], (double)t);
+    const double t = wave_sum_d((double)ke);
+    if ((threadIdx.x & 63) == 0 && (p & ~63ll) < N) {
+        double* r = epart + (p >> 6) * ACC_N;
+        r[ACC_LJ] = 0.0; r[ACC_COUL] = 0.0; r[ACC_BOND] = 0.0; r[ACC_PME] = 0.0; r[ACC_KIN] = t; r[ACC_PAIRS] = 0.0;
+    }
 }
 
 // Forces of the last evaluation in global order (w = per-particle potE) and max |F|.
@@ -248,7 +256,6 @@ void timing_flush(Engine& e) {
 void evaluate(Engine& e, const uint4* w, const int4* img, bool energy, bool timed = false) {
     hipStream_t s = e.s_main;
     hipEvent_t* ev = timed ? tslot(e) : nullptr;
-    hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(64), 0, s, e.acc, (int)ACC_N);
     if (ev) HIP_CHECK(hipEventRecord(ev[0], s));
     launch_pack(e, w, s);
     if (ev) HIP_CHECK(hipEventRecord(ev[1], s));
@@ -302,8 +309,8 @@ void integrate(Engine& e, bool energy) {
     a.dt = (float)e.dt_ns;
     a.thermostat = 1.f;
     a.slot_of = e.nl.slot_of; a.fs = e.fs; a.w = e.p.w; a.img = e.p.img; a.v = e.p.v; a.fprev = e.p.fprev;
-    a.prm = e.p.prm; a.acc = e.acc;
-    a.w_build = e.w_build; a.img_build = e.img_build; a.dmax2 = e.dmax2;
+    a.prm = e.p.prm; a.epart = e.epart + (int64_t)e.ereg.kin * ACC_N;
+    a.w_build = e.w_build; a.img_build = e.img_build; a.dpart = e.dpart;
     if (energy) hipLaunchKernelGGL(k_integrate<true>, dim3(nblk(e.N, 256)), dim3(256), 0, e.s_main, a);
     else hipLaunchKernelGGL(k_integrate<false>, dim3(nblk(e.N, 256)), dim3(256), 0, e.s_main, a);
 }
@@ -322,6 +329,7 @@ void md_steps(Engine& e, int64_t n) {
         const bool energy = (e.prm.energy_interval > 0 && e.step % e.prm.energy_interval == 0) || k == n - 1;
         evaluate(e, e.p.w, e.p.img, energy, e.timing);
         integrate(e, energy);
+        if (energy) launch_reduce_energy(e, true, e.s_main);
         if (e.timing) {
             HIP_CHECK(hipEventRecord(tslot(e)[6], e.s_main));
             if (++e.tpos == Engine::TRING) timing_flush(e);
@@ -434,6 +442,8 @@ void create(Engine& e, const lima_box_desc* b, const lima_simparams* p) {
     HIP_CHECK(hipMalloc(&e.img_build, N * sizeof(int4)));
     HIP_CHECK(hipMalloc(&e.dmax2, sizeof(float)));
     HIP_CHECK(hipMemset(e.dmax2, 0, sizeof(float)));
+    HIP_CHECK(hipMalloc(&e.dpart, ((N + 63) / 64) * sizeof(float)));
+    HIP_CHECK(hipMemset(e.dpart, 0, ((N + 63) / 64) * sizeof(float)));
     HIP_CHECK(hipMalloc(&e.w_try, N * sizeof(uint4)));
     HIP_CHECK(hipMalloc(&e.img_try, N * sizeof(int4)));
     e.f_acc = dalloc_copy(zero4);
@@ -537,7 +547,7 @@ void destroy(Engine& e) {
     if (e.s_main) (void)hipStreamSynchronize(e.s_main);
     nlist_free(e);
     pme_free(e);
-    void* ptrs[] = {e.p.w, e.p.img, e.p.v, e.p.fprev, e.p.prm, e.p.comp, e.w_build, e.img_build, e.dmax2, e.w_try, e.img_try, e.f_acc, e.sd, e.first,
+    void* ptrs[] = {e.p.w, e.p.img, e.p.v, e.p.fprev, e.p.prm, e.p.comp, e.w_build, e.img_build, e.dmax2, e.dpart, e.epart, e.w_try, e.img_try, e.f_acc, e.sd, e.first,
                     e.bmask, e.intra_items, e.intra_lut, e.bd.bid, e.bd.bp, e.bd.aid, e.bd.ap, e.bd.did, e.bd.dp,
                     e.bd.iid, e.bd.ip, e.bd.pid, e.bd.pp, e.acc, e.fmax};
     for (void* p : ptrs) if (p) (void)hipFree(p);
@@ -593,7 +603,9 @@ lima_status lima_compute_forces(lima_engine* h, lima_energy* out) {
         Engine& e = h->e;
         nlist_build(e, e.p.w);
         evaluate(e, e.p.w, e.p.img, true);
-        hipLaunchKernelGGL(k_kinetic, dim3(nblk(e.N, 256)), dim3(256), 0, e.s_main, e.N, e.p.v, e.p.prm, e.acc);
+        hipLaunchKernelGGL(k_kinetic, dim3(nblk(e.N, 256)), dim3(256), 0, e.s_main, e.N, e.p.v, e.p.prm,
+                           e.epart + (int64_t)e.ereg.kin * ACC_N);
+        launch_reduce_energy(e, true, e.s_main);
         HIP_CHECK(hipStreamSynchronize(e.s_main));
         HIP_CHECK(hipGetLastError());
         fill_energy(e, true);
@@ -617,6 +629,7 @@ lima_status lima_minimize(lima_engine* h, double tol_kj, int64_t max_iter, doubl
         // initial evaluation at the current positions
         nlist_build(e, e.p.w);
         evaluate(e, e.p.w, e.p.img, true);
+        launch_reduce_energy(e, false, s);
         forces_global(e);
         SdState sd{};
         {
@@ -642,6 +655,7 @@ lima_status lima_minimize(lima_engine* h, double tol_kj, int64_t max_iter, doubl
             hipLaunchKernelGGL(k_sd_trial, dim3(nblk(e.N, 256)), dim3(256), 0, s, e.N, (long long)e.Lq[0], (long long)e.Lq[1],
                                (long long)e.Lq[2], e.sd, e.p.w, e.p.img, e.f_acc, e.w_try, e.img_try);
             evaluate(e, e.w_try, e.img_try, true);
+            launch_reduce_energy(e, false, s);
             // trial forces in global order -> p.fprev used as scratch
             hipLaunchKernelGGL(k_zero_f, dim3(1), dim3(64), 0, s, e.fmax, 1);
             hipLaunchKernelGGL(k_forces_global, dim3(nblk(e.N, 256)), dim3(256), 0, s, e.N, e.nl.ncl * CLUSTER, e.nl.slot_of,
@@ -658,6 +672,7 @@ lima_status lima_minimize(lima_engine* h, double tol_kj, int64_t max_iter, doubl
         }
         // leave forces / energies of the accepted state in the engine
         evaluate(e, e.p.w, e.p.img, true);
+        launch_reduce_energy(e, false, s);
         HIP_CHECK(hipMemsetAsync(e.p.fprev, 0, e.N * sizeof(float4), s));
         HIP_CHECK(hipStreamSynchronize(s));
         fill_energy(e, false);

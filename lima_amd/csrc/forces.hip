@@ -66,7 +66,7 @@ struct PairArgs {
     const float4* ent;
     float* fs;
     float* es;
-    double* acc;
+    double* epart;   // this kernel's energy region: one record per wave
 };
 
 // One i-cluster per `split` waves: lane = 8 jj + ii computes pair (i slot ii, entry jj) of 8
@@ -94,7 +94,8 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs a) {
     const int islot = I * CLUSTER + ii;
     const float4 xi = a.xq[islot];
     const float2 pi = a.ljs[islot];
-    float fx = 0.f, fy = 0.f, fz = 0.f, elj = 0.f, ecl = 0.f;
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+    double elj = 0.0, ecl = 0.0;   // FP64 energy accumulation (energy steps only)
     int npairs = 0;
     const int ntot = a.ecount[I] / CLUSTER;
     const int it0 = (int)((int64_t)ntot * part / a.split), it1 = (int)((int64_t)ntot * (part + 1) / a.split);
@@ -142,9 +143,9 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs a) {
             float S = pair_scalar_cut(inc ? r2 : 1.0f, pi.x + B.x, pi.y * B.y, xi.w * A.w, a.pc, ENERGY ? &el : nullptr, &ec);
             S = inc ? S : 0.f;
             fx += dx * S; fy += dy * S; fz += dz * S;
-            if (ENERGY) {
-                elj += inc ? 0.5f * el : 0.f;   // each pair is evaluated from both sides: half per side
-                ecl += inc ? 0.5f * ec : 0.f;
+            if (ENERGY && inc) {
+                elj += 0.5 * (double)el;   // each pair is evaluated from both sides: half per side
+                ecl += 0.5 * (double)ec;
             }
         }
         if (c + 1 < nchunk) {
@@ -158,23 +159,22 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs a) {
         fx += __shfl_xor(fx, o, 64);
         fy += __shfl_xor(fy, o, 64);
         fz += __shfl_xor(fz, o, 64);
-        if (ENERGY) { elj += __shfl_xor(elj, o, 64); ecl += __shfl_xor(ecl, o, 64); }
     }
     if (jj == 0) {
         // atomics: the intra, bonded and PME kernels (PME on its own stream) add to the same slots
         atomicAdd(&a.fs[islot], fx);
         atomicAdd(&a.fs[a.nslots + islot], fy);
         atomicAdd(&a.fs[2 * a.nslots + islot], fz);
-        if (ENERGY) atomicAdd(&a.es[islot], elj + ecl);
     }
     if (ENERGY) {
-        float tl = (jj == 0) ? elj : 0.f, tc = (jj == 0) ? ecl : 0.f;
-        tl = wave_sum(tl);
-        tc = wave_sum(tc);
+        float ep = (float)(elj + ecl);
+        for (int o = 8; o < 64; o <<= 1) ep += __shfl_xor(ep, o, 64);
+        if (jj == 0) atomicAdd(&a.es[islot], ep);
+        const double tl = wave_sum_d(elj), tc = wave_sum_d(ecl);
         if (lane == 0) {
-            atomicAdd(&a.acc[ACC_LJ], (double)tl);
-            atomicAdd(&a.acc[ACC_COUL], (double)tc);
-            atomicAdd(&a.acc[ACC_PAIRS], (double)npairs * 0.5);   // both sides counted
+            double* r = a.epart + (int64_t)wv * ACC_N;
+            r[ACC_LJ] = tl; r[ACC_COUL] = tc; r[ACC_BOND] = 0.0; r[ACC_PME] = 0.0; r[ACC_KIN] = 0.0;
+            r[ACC_PAIRS] = (double)npairs * 0.5;   // both sides counted
         }
     }
 }
@@ -192,7 +192,7 @@ struct IntraArgs {
     const int64_t* first;
     float* fs;
     float* es;
-    double* acc;
+    double* epart;
 };
 
 __device__ __forceinline__ void unwrapped3(const uint4* w, const int4* img, const long long* Lq, int a, long long* u) {
@@ -251,10 +251,10 @@ __global__ __launch_bounds__(256) void k_intra(IntraArgs a) {
         if (ENERGY) atomicAdd(&a.es[s], e_i);
     }
     if (ENERGY) {
-        const float tl = wave_sum(el_s), tc = wave_sum(ec_s);
+        const double tl = wave_sum_d((double)el_s), tc = wave_sum_d((double)ec_s);
         if (lane == 0) {
-            atomicAdd(&a.acc[ACC_LJ], (double)tl);
-            atomicAdd(&a.acc[ACC_COUL], (double)tc);
+            double* r = a.epart + (int64_t)it * ACC_N;
+            r[ACC_LJ] = tl; r[ACC_COUL] = tc; r[ACC_BOND] = 0.0; r[ACC_PME] = 0.0; r[ACC_KIN] = 0.0; r[ACC_PAIRS] = 0.0;
         }
     }
 }
@@ -273,7 +273,7 @@ struct BondArgs {
     const int* slot_of;
     float* fs;
     float* es;
-    double* acc;
+    double* epart;
 };
 
 struct v3 {
@@ -405,12 +405,61 @@ __global__ __launch_bounds__(256) void k_bonded(BondArgs a) {
     }
     if (ENERGY) {
         for (int k = 0; k < nat; k++) atomicAdd(&a.es[a.slot_of[ids[k]]], e / (float)nat);
-        const float tot = wave_sum(e);
-        if ((threadIdx.x & 63) == 0) atomicAdd(&a.acc[ACC_BOND], (double)tot);
+        const double tot = wave_sum_d((double)e);
+        if ((threadIdx.x & 63) == 0 && (t & ~63) < a.nb + a.na + a.nd + a.ni + a.np) {   // waves inside the region
+            double* r = a.epart + (int64_t)(t >> 6) * ACC_N;
+            r[ACC_LJ] = 0.0; r[ACC_COUL] = 0.0; r[ACC_BOND] = tot; r[ACC_PME] = 0.0; r[ACC_KIN] = 0.0; r[ACC_PAIRS] = 0.0;
+        }
     }
 }
 
+// Sums the per-wave energy records of all regions in a fixed order: acc[c] = sum_r epart[r][c].
+// Records in [0, n0) are always present; [n0, n1) (PME) and [n1, n2) (kinetic) only when valid.
+__global__ __launch_bounds__(256) void k_reduce_energy(const double* __restrict__ epart, int n0, int n1, int n2, int pme_valid,
+                                                       int kin_valid, double* __restrict__ acc) {
+    __shared__ double red[ACC_N][4];
+    double s[ACC_N];
+    for (int c = 0; c < ACC_N; c++) s[c] = 0.0;
+    const int nrec = kin_valid ? n2 : (pme_valid ? n1 : n0);
+    for (int r = threadIdx.x; r < nrec; r += blockDim.x) {
+        if (r >= n0 && r < n1 && !pme_valid) continue;
+        for (int c = 0; c < ACC_N; c++) s[c] += epart[(int64_t)r * ACC_N + c];
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int c = 0; c < ACC_N; c++) {
+        const double v = wave_sum_d(s[c]);
+        if (lane == 0) red[c][wid] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < ACC_N) acc[threadIdx.x] = ((red[threadIdx.x][0] + red[threadIdx.x][1]) + red[threadIdx.x][2]) + red[threadIdx.x][3];
+}
+
 }  // namespace
+
+int pair_split(const Engine& e) {
+    // enough waves to keep ~16 per SIMD in flight (256 CUs x 4 SIMDs) on small boxes
+    int split = 1;
+    while (split < 8 && (int64_t)e.nl.ncl * split < 16384) split *= 2;
+    return split;
+}
+
+ERegions energy_regions(const Engine& e) {
+    ERegions r;
+    r.pair = 0;
+    r.intra = r.pair + e.nl.ncl * pair_split(e);
+    r.bond = r.intra + e.n_intra;
+    const int nterms = e.bd.nb + e.bd.na + e.bd.nd + e.bd.ni + e.bd.np;
+    r.pme = r.bond + (nterms + 63) / 64;
+    r.kin = r.pme + (int)((e.N + 63) / 64);
+    r.total = r.kin + (int)((e.N + 63) / 64);
+    return r;
+}
+
+void launch_reduce_energy(Engine& e, bool kinetic, hipStream_t s) {
+    const bool pme_valid = e.elec && e.pme.ncharged > 0;
+    hipLaunchKernelGGL(k_reduce_energy, dim3(1), dim3(256), 0, s, e.epart, e.ereg.pme, e.ereg.kin, e.ereg.total,
+                       pme_valid ? 1 : 0, kinetic ? 1 : 0, e.acc);
+}
 
 void launch_pack(Engine& e, const uint4* w, hipStream_t s) {
     const int nslots = e.nl.ncl * CLUSTER;
@@ -426,11 +475,9 @@ void launch_pair(Engine& e, bool energy, hipStream_t s) {
     a.pc = e.pc;
     a.cap = e.nl.cap;
     a.xq = e.nl.xq; a.ljs = e.nl.ljs; a.ecount = e.nl.ecount; a.ent = e.nl.ent;
-    a.fs = e.fs; a.es = e.es; a.acc = e.acc;
+    a.fs = e.fs; a.es = e.es; a.epart = e.epart + (int64_t)e.ereg.pair * ACC_N;
     if (a.ncl == 0) return;
-    // enough waves to keep ~16 per SIMD in flight (256 CUs x 4 SIMDs) on small boxes
-    a.split = 1;
-    while (a.split < 8 && (int64_t)a.ncl * a.split < 16384) a.split *= 2;
+    a.split = pair_split(e);
     const int64_t waves = (int64_t)a.ncl * a.split;
     if (energy) hipLaunchKernelGGL(k_pair<true>, dim3(nblk(waves, PAIR_WAVES)), dim3(64 * PAIR_WAVES), 0, s, a);
     else hipLaunchKernelGGL(k_pair<false>, dim3(nblk(waves, PAIR_WAVES)), dim3(64 * PAIR_WAVES), 0, s, a);
@@ -444,7 +491,7 @@ void launch_intra(Engine& e, const uint4* w, const int4* img, bool energy, hipSt
     for (int d = 0; d < 3; d++) a.Lq[d] = e.Lq[d];
     a.pc = e.pc;
     a.items = e.intra_items; a.lut = e.intra_lut; a.w = w; a.img = img; a.prm = e.p.prm;
-    a.slot_of = e.nl.slot_of; a.first = e.first; a.fs = e.fs; a.es = e.es; a.acc = e.acc;
+    a.slot_of = e.nl.slot_of; a.first = e.first; a.fs = e.fs; a.es = e.es; a.epart = e.epart + (int64_t)e.ereg.intra * ACC_N;
     if (energy) hipLaunchKernelGGL(k_intra<true>, dim3(nblk(a.n_items, 4)), dim3(256), 0, s, a);
     else hipLaunchKernelGGL(k_intra<false>, dim3(nblk(a.n_items, 4)), dim3(256), 0, s, a);
 }
@@ -459,7 +506,7 @@ void launch_bonded(Engine& e, const uint4* w, const int4* img, bool energy, hipS
     a.pc = e.pc;
     a.bid = e.bd.bid; a.bp = e.bd.bp; a.aid = e.bd.aid; a.ap = e.bd.ap; a.did = e.bd.did; a.dp = e.bd.dp;
     a.iid = e.bd.iid; a.ip = e.bd.ip; a.pid = e.bd.pid; a.pp = e.bd.pp;
-    a.w = w; a.img = img; a.slot_of = e.nl.slot_of; a.fs = e.fs; a.es = e.es; a.acc = e.acc;
+    a.w = w; a.img = img; a.slot_of = e.nl.slot_of; a.fs = e.fs; a.es = e.es; a.epart = e.epart + (int64_t)e.ereg.bond * ACC_N;
     if (energy) hipLaunchKernelGGL(k_bonded<true>, dim3(nblk(total, 256)), dim3(256), 0, s, a);
     else hipLaunchKernelGGL(k_bonded<false>, dim3(nblk(total, 256)), dim3(256), 0, s, a);
 }

@@ -363,12 +363,22 @@ void nlist_free(Engine& e) {
 namespace {
 
 __global__ void k_snapshot(int64_t N, const uint4* __restrict__ w, const int4* __restrict__ img, uint4* __restrict__ wb,
-                           int4* __restrict__ ib, float* dmax2) {
+                           int4* __restrict__ ib, float* dpart) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p == 0) *dmax2 = 0.f;
     if (p >= N) return;
+    if ((p & 63) == 0) dpart[p >> 6] = 0.f;
     wb[p] = w[p];
     ib[p] = img[p];
+}
+
+__global__ __launch_bounds__(256) void k_max_reduce(const float* __restrict__ v, int n, float* out) {
+    __shared__ float red[4];
+    float m = 0.f;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) m = fmaxf(m, v[i]);
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) *out = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
 }
 
 }  // namespace
@@ -379,12 +389,13 @@ void nlist_build(Engine& e, const uint4* w) {
     const int64_t N = e.N;
     {   // a list is stale once two particles may have closed the skin: 2 max|dx| > skin
         float d2 = 0.f;
+        hipLaunchKernelGGL(k_max_reduce, dim3(1), dim3(256), 0, s, e.dpart, (int)((N + 63) / 64), e.dmax2);
         HIP_CHECK(hipMemcpyAsync(&d2, e.dmax2, sizeof(float), hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipStreamSynchronize(s));
         e.last_dmax = std::sqrt((double)d2);
         if (e.n_rebuilds > 0 && 2.0 * e.last_dmax > (double)e.skin) e.list_violations++;
     }
-    if (w == e.p.w) hipLaunchKernelGGL(k_snapshot, dim3(nblk(N, 256)), dim3(256), 0, s, N, e.p.w, e.p.img, e.w_build, e.img_build, e.dmax2);
+    if (w == e.p.w) hipLaunchKernelGGL(k_snapshot, dim3(nblk(N, 256)), dim3(256), 0, s, N, e.p.w, e.p.img, e.w_build, e.img_build, e.dpart);
     hipLaunchKernelGGL(k_fill_int, dim3(nblk(nl.ncol + 1, 256)), dim3(256), 0, s, nl.col_count, nl.ncol + 1, 0);
     hipLaunchKernelGGL(k_fill_int, dim3(nblk(nl.ncol + 1, 256)), dim3(256), 0, s, nl.col_fill, nl.ncol + 1, 0);
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(4), 0, s, nl.flags, 4, 0);
@@ -438,6 +449,12 @@ void nlist_build(Engine& e, const uint4* w) {
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(4), 0, s, nl.flags, 4, 0);
     }
     pme_bin(e, w, s);
+    e.ereg = energy_regions(e);
+    if (e.ereg.total > e.epart_cap) {
+        if (e.epart) HIP_CHECK(hipFree(e.epart));
+        e.epart_cap = (int64_t)(e.ereg.total * 1.25) + 64;
+        HIP_CHECK(hipMalloc(&e.epart, e.epart_cap * ACC_N * sizeof(double)));
+    }
     HIP_CHECK(hipGetLastError());
     e.steps_since_build = 0;
     e.n_rebuilds++;

@@ -395,7 +395,7 @@ struct GatherArgs {
     const int* slot_of;
     float* fs;
     float* es;
-    double* acc;
+    double* epart;
 };
 
 template <bool ENERGY>
@@ -455,8 +455,11 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
         }
     }
     if (ENERGY) {
-        const float t = wave_sum(ephi);
-        if ((threadIdx.x & 63) == 0) atomicAdd(&a.acc[ACC_PME], (double)t);
+        const double t = wave_sum_d((double)ephi);
+        if ((threadIdx.x & 63) == 0 && (p & ~63ll) < a.N) {   // waves inside the region
+            double* r = a.epart + (p >> 6) * ACC_N;
+            r[ACC_LJ] = 0.0; r[ACC_COUL] = 0.0; r[ACC_BOND] = 0.0; r[ACC_PME] = t; r[ACC_KIN] = 0.0; r[ACC_PAIRS] = 0.0;
+        }
     }
 }
 
@@ -677,7 +680,8 @@ void launch_pme(Engine& e, const uint4* w, bool energy, hipStream_t s) {
     ga.nslots = e.nl.ncl * CLUSTER;
     for (int d = 0; d < 3; d++) { ga.Lq[d] = e.Lq[d]; ga.K[d] = m.K[d]; ga.inv2h[d] = (float)(m.K[d] / (2.0 * e.L[d])); }
     ga.eself = (float)m.eself_total;
-    ga.w = w; ga.prm = e.p.prm; ga.phi = m.phi; ga.slot_of = e.nl.slot_of; ga.fs = e.fs; ga.es = e.es; ga.acc = e.acc;
+    ga.w = w; ga.prm = e.p.prm; ga.phi = m.phi; ga.slot_of = e.nl.slot_of; ga.fs = e.fs; ga.es = e.es;
+    ga.epart = e.epart + (int64_t)e.ereg.pme * ACC_N;
     if (energy) hipLaunchKernelGGL(k_gather<true>, dim3(nblk(e.N, 256)), dim3(256), 0, s, ga);
     else hipLaunchKernelGGL(k_gather<false>, dim3(nblk(e.N, 256)), dim3(256), 0, s, ga);
 }

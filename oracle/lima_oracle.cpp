@@ -93,6 +93,7 @@ struct lima_oracle {
     // params
     double rc = 1.2, kappa = 2.5, dt_ns = 2e-6, skin = 0.1;
     bool elec = true;
+    bool pme_divide_modulus = false;    // test switch: textbook smooth-PME deconvolution
     int K[3]{};
     int64_t step = 0;
     // last evaluation
@@ -100,7 +101,9 @@ struct lima_oracle {
     lima_energy E{};
     bool have_forces = false;
     // scratch for PME inspection
-    std::vector<double> rho_last, phi_last;
+    std::vector<double> rho_last, phi_last;  // filled only while keep_grids is set
+    bool keep_grids = false;
+    std::vector<std::complex<double>> pme_grid;   // persistent work grid (reused across calls)
     // cluster list
     int ncol[2]{};
 
@@ -150,48 +153,92 @@ inline Vec pair_force(const Vec& diff, double sig, double eps, double qq, double
 /* ---- mixed-radix complex FFT in double (radix 2,3,4,5, generic odd prime by DFT) ---- */
 using cd = std::complex<double>;
 
-void fft1d_rec(const cd* in, cd* out, int n, int stride, int sign) {
+int nthreads() { return g_threads > 0 ? g_threads : omp_get_max_threads(); }
+
+/* Decimation in time: out[k + q m] = sum_t W_n^{t (k + q m)} FFT_m(in[t::r])[k], n = r m.
+ * W holds W_N^j = exp(sign 2 pi i j / N) for the top-level length N; scr needs 2 n entries. */
+void fft1d_rec(const cd* in, size_t stride, cd* out, int n, const cd* W, int N, cd* scr) {
     if (n == 1) { out[0] = in[0]; return; }
     int r = 0;
     for (int p : {4, 2, 3, 5}) if (n % p == 0) { r = p; break; }
     if (!r) { for (int p = 7; p <= n; p += 2) if (n % p == 0) { r = p; break; } }
     const int m = n / r;
-    std::vector<cd> sub(n);
-    for (int t = 0; t < r; t++) fft1d_rec(in + t * stride, sub.data() + t * m, m, stride * r, sign);
-    const double tw = sign * 2.0 * M_PI / n;
+    cd* sub = scr;
+    for (int t = 0; t < r; t++) fft1d_rec(in + t * stride, stride * r, sub + t * m, m, W, N, scr + n);
+    // W_n^{t (k + q m)} = W_n^{t k} W_r^{t q}: pre-twiddle, then an r-point DFT
+    const long step = N / n;
+    cd Wr[64];
+    const bool small = r <= 8;
+    if (small) for (int j = 0; j < r * r; j++) Wr[j] = W[(long)((j / r) * (j % r) % r) * (N / r)];
+    cd x[8];
     for (int k = 0; k < m; k++) {
-        for (int q = 0; q < r; q++) {
-            cd acc = 0;
-            for (int t = 0; t < r; t++) {
-                const long e = (long)t * (k + (long)q * m);
-                acc += sub[t * m + k] * std::polar(1.0, tw * (double)(e % n));
+        if (small) {
+            for (int t = 0; t < r; t++) x[t] = sub[t * m + k] * W[(long)t * k * step];
+            for (int q = 0; q < r; q++) {
+                cd acc = 0;
+                for (int t = 0; t < r; t++) acc += x[t] * Wr[t * r + q];
+                out[k + q * m] = acc;
             }
-            out[k + q * m] = acc;
+        } else {
+            for (int q = 0; q < r; q++) {
+                cd acc = 0;
+                for (int t = 0; t < r; t++) {
+                    const long e = ((long)t * (k + (long)q * m)) % n;
+                    acc += sub[t * m + k] * W[e * step];
+                }
+                out[k + q * m] = acc;
+            }
         }
     }
 }
 
+std::vector<cd> twiddles(int n, int sign) {
+    std::vector<cd> W(n);
+    for (int j = 0; j < n; j++) W[j] = std::polar(1.0, sign * 2.0 * M_PI * (double)j / (double)n);
+    return W;
+}
+
 void fft3d(std::vector<cd>& a, int nx, int ny, int nz, int sign) {
-    std::vector<cd> line, res;
-    // x
-    line.resize(nx); res.resize(nx);
-    for (int z = 0; z < nz; z++) for (int y = 0; y < ny; y++) {
-        cd* p = &a[((size_t)z * ny + y) * nx];
-        for (int x = 0; x < nx; x++) line[x] = p[x];
-        fft1d_rec(line.data(), res.data(), nx, 1, sign);
-        for (int x = 0; x < nx; x++) p[x] = res[x];
-    }
-    line.resize(ny); res.resize(ny);
-    for (int z = 0; z < nz; z++) for (int x = 0; x < nx; x++) {
-        for (int y = 0; y < ny; y++) line[y] = a[((size_t)z * ny + y) * nx + x];
-        fft1d_rec(line.data(), res.data(), ny, 1, sign);
-        for (int y = 0; y < ny; y++) a[((size_t)z * ny + y) * nx + x] = res[y];
-    }
-    line.resize(nz); res.resize(nz);
-    for (int y = 0; y < ny; y++) for (int x = 0; x < nx; x++) {
-        for (int z = 0; z < nz; z++) line[z] = a[((size_t)z * ny + y) * nx + x];
-        fft1d_rec(line.data(), res.data(), nz, 1, sign);
-        for (int z = 0; z < nz; z++) a[((size_t)z * ny + y) * nx + x] = res[z];
+    const int dims[3] = {nx, ny, nz};
+    const size_t strides[3] = {1, (size_t)nx, (size_t)nx * ny};
+    constexpr int B = 16;   // neighbouring x-lines transformed together (contiguous gathers)
+    for (int ax = 0; ax < 3; ax++) {
+        const int n = dims[ax];
+        const std::vector<cd> W = twiddles(n, sign);
+        if (ax == 0) {
+#pragma omp parallel num_threads(nthreads())
+            {
+                std::vector<cd> res(n), scr(2 * (size_t)n);
+#pragma omp for schedule(static)
+                for (long l = 0; l < (long)ny * nz; l++) {
+                    cd* p = &a[(size_t)l * nx];
+                    fft1d_rec(p, 1, res.data(), n, W.data(), n, scr.data());
+                    std::copy(res.begin(), res.end(), p);
+                }
+            }
+            continue;
+        }
+        const int o = ax == 1 ? 2 : 1;                  // the other non-x axis
+        const int nxb = (nx + B - 1) / B;
+        const long nblk = (long)dims[o] * nxb;
+#pragma omp parallel num_threads(nthreads())
+        {
+            std::vector<cd> tile((size_t)n * B), res(n), scr(2 * (size_t)n);
+#pragma omp for schedule(static)
+            for (long blk = 0; blk < nblk; blk++) {
+                const int xb = (int)(blk % nxb) * B, io = (int)(blk / nxb);
+                const int bw = std::min(B, nx - xb);
+                cd* base = &a[io * strides[o] + xb];
+                for (int i = 0; i < n; i++)
+                    for (int b = 0; b < bw; b++) tile[(size_t)i * B + b] = base[i * strides[ax] + b];
+                for (int b = 0; b < bw; b++) {
+                    fft1d_rec(&tile[b], B, res.data(), n, W.data(), n, scr.data());
+                    for (int i = 0; i < n; i++) tile[(size_t)i * B + b] = res[i];
+                }
+                for (int i = 0; i < n; i++)
+                    for (int b = 0; b < bw; b++) base[i * strides[ax] + b] = tile[(size_t)i * B + b];
+            }
+        }
     }
 }
 
@@ -422,7 +469,10 @@ void pme(lima_oracle* o, std::vector<double>& F, std::vector<double>& pot, lima_
     double h[3];
     for (int d = 0; d < 3; d++) h[d] = o->L[d] / o->K[d];
     const double inv_vcell = 1.0 / (h[0] * h[1] * h[2]);
-    std::vector<cd> grid(NG, 0.0);
+    std::vector<cd>& grid = o->pme_grid;
+    grid.resize(NG);
+#pragma omp parallel for schedule(static) num_threads(nthreads())
+    for (long g = 0; g < (long)NG; g++) grid[g] = 0.0;
     struct Sten { int ix[3]; double w[3][4]; };
     std::vector<Sten> st(o->N);
     double qsq = 0;
@@ -444,30 +494,48 @@ void pme(lima_oracle* o, std::vector<double>& F, std::vector<double>& pot, lima_
             grid[((size_t)Z * Ky + Y) * Kx + X] += o->q[i] * s.w[0][a] * s.w[1][b] * s.w[2][c] * inv_vcell;
         }
     }
-    o->rho_last.resize(NG);
-    for (size_t g = 0; g < NG; g++) o->rho_last[g] = grid[g].real();
+    if (o->keep_grids) {
+        o->rho_last.resize(NG);
+        for (size_t g = 0; g < NG; g++) o->rho_last[g] = grid[g].real();
+    }
     fft3d(grid, Kx, Ky, Kz, -1);
     const double twopi = 2.0 * (double)PI_L;
     const double fourpi = 4.0 * (double)PI_L;
-    for (int z = 0; z < Kz; z++) for (int y = 0; y < Ky; y++) for (int x = 0; x < Kx; x++) {
-        const int mi[3] = {x, y, z};
-        double k[3], S = 1.0, k2 = 0;
-        for (int d = 0; d < 3; d++) {
-            const int half = o->K[d] / 2;
-            const int m = mi[d] <= half ? mi[d] : mi[d] - o->K[d];
-            k[d] = twopi * m / o->L[d];
-            k2 += k[d] * k[d];
-            const double kh = k[d] * h[d] * 0.5;
-            S *= (std::fabs(kh) < 1e-14) ? 1.0 : std::pow(std::sin(kh) / kh, 4);
+    // per-axis wave numbers and spline factors; G is separable apart from exp(-k^2 / 4 kappa^2)
+    std::vector<double> kk[3], sf[3];
+    for (int d = 0; d < 3; d++) {
+        kk[d].resize(o->K[d]);
+        sf[d].resize(o->K[d]);
+        const int half = o->K[d] / 2;
+        for (int i = 0; i < o->K[d]; i++) {
+            const int m = i <= half ? i : i - o->K[d];
+            const double k = twopi * m / o->L[d];
+            const double kh = k * h[d] * 0.5;
+            kk[d][i] = k * k;
+            sf[d][i] = (std::fabs(kh) < 1e-14) ? 1.0 : std::pow(std::sin(kh) / kh, 4);
         }
+    }
+#pragma omp parallel for schedule(static) num_threads(nthreads())
+    for (int z = 0; z < Kz; z++) for (int y = 0; y < Ky; y++) for (int x = 0; x < Kx; x++) {
+        const double k2 = kk[0][x] + kk[1][y] + kk[2][z];
+        const double S = sf[0][x] * sf[1][y] * sf[2][z];
         double G = 0.0;
-        if (k2 > 1e-14) G = fourpi / k2 * std::exp(-k2 / (4.0 * o->kappa * o->kappa)) * S * S * KC;
+        // LIMA multiplies the Green's function by the spline transform (sinc^4 per axis, squared:
+        // spread and gather), where smooth PME divides by it.  The product keeps LIMA's form; the
+        // divided form exists so tests can compare the rest of the pipeline with a direct Ewald sum.
+        if (k2 > 1e-14) {
+            G = fourpi / k2 * std::exp(-k2 / (4.0 * o->kappa * o->kappa)) * KC;
+            G = o->pme_divide_modulus ? G / (S * S) : G * S * S;
+        }
         grid[((size_t)z * Ky + y) * Kx + x] *= G;
     }
     fft3d(grid, Kx, Ky, Kz, +1);
-    std::vector<double> phi(NG);
-    for (size_t g = 0; g < NG; g++) phi[g] = grid[g].real() / (double)NG;
-    o->phi_last = phi;
+    // phi = Re(IFFT) / NG, kept in place in the real parts of the work grid
+    const double inv_ng = 1.0 / (double)NG;
+    if (o->keep_grids) {
+        o->phi_last.resize(NG);
+        for (size_t g = 0; g < NG; g++) o->phi_last[g] = grid[g].real() * inv_ng;
+    }
     const double eself = -o->kappa / std::sqrt((double)PI_L) * qsq * KC;
     double erec = 0;
     int64_t ncharged = 0;
@@ -479,7 +547,7 @@ void pme(lima_oracle* o, std::vector<double>& F, std::vector<double>& pot, lima_
         for (int a = 0; a < 4; a++) for (int b = 0; b < 4; b++) for (int c = 0; c < 4; c++) {
             const int X = (int)floormod(s.ix[0] - 1 + a, Kx), Y = (int)floormod(s.ix[1] - 1 + b, Ky), Z = (int)floormod(s.ix[2] - 1 + c, Kz);
             auto at = [&](int xx, int yy, int zz) {
-                return phi[((size_t)floormod(zz, Kz) * Ky + floormod(yy, Ky)) * Kx + floormod(xx, Kx)];
+                return grid[((size_t)floormod(zz, Kz) * Ky + floormod(yy, Ky)) * Kx + floormod(xx, Kx)].real() * inv_ng;
             };
             const double w = s.w[0][a] * s.w[1][b] * s.w[2][c];
             ph += w * at(X, Y, Z);
@@ -892,6 +960,8 @@ int lima_oracle_pairs(lima_oracle* o, double radius_nm, int32_t* pi, int32_t* pj
     return 0;
 }
 
+void lima_oracle_set_pme_divide_modulus(lima_oracle* o, int on) { o->pme_divide_modulus = on != 0; }
+
 int lima_oracle_pme_grid_dims(lima_oracle* o, int32_t* dims) {
     for (int d = 0; d < 3; d++) dims[d] = o->K[d];
     return 0;
@@ -900,7 +970,9 @@ int lima_oracle_pme_grid_dims(lima_oracle* o, int32_t* dims) {
 int lima_oracle_pme_grids(lima_oracle* o, double* rho, double* phi) {
     std::vector<double> F(o->N * 3), pot(o->N);
     lima_energy E{};
+    o->keep_grids = true;
     pme(o, F, pot, E);
+    o->keep_grids = false;
     if (rho) std::memcpy(rho, o->rho_last.data(), o->rho_last.size() * 8);
     if (phi) std::memcpy(phi, o->phi_last.data(), o->phi_last.size() * 8);
     return 0;

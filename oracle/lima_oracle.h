@@ -67,6 +67,9 @@ int lima_oracle_pairs(lima_oracle* o, double radius_nm, int32_t* pi, int32_t* pj
 
 /* PME building blocks exposed for pinning tests. */
 int lima_oracle_pme_grid_dims(lima_oracle* o, int32_t* dims);
+/* Test switch: divide the Green's function by the B-spline transform (textbook smooth PME)
+ * instead of multiplying (LIMA).  Off by default. */
+void lima_oracle_set_pme_divide_modulus(lima_oracle* o, int on);
 /* Charge-density grid after spreading [Kz][Ky][Kx] and potential grid after the solve. */
 int lima_oracle_pme_grids(lima_oracle* o, double* rho, double* phi);
 /* 1D/3D complex FFT used by the oracle (forward sign -1), for tests against numpy. */

@@ -43,6 +43,7 @@ def lib():
             "lima_oracle_pair_count": ([vp, C.c_double], C.c_int64),
             "lima_oracle_pairs": ([vp, C.c_double, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
             "lima_oracle_pme_grid_dims": ([vp, C.POINTER(C.c_int32)], C.c_int),
+            "lima_oracle_set_pme_divide_modulus": ([vp, C.c_int], None),
             "lima_oracle_pme_grids": ([vp, d, d], C.c_int),
             "lima_oracle_fft3d": ([C.c_int, C.c_int, C.c_int, d, d, d, d, C.c_int], C.c_int),
         }
@@ -160,6 +161,9 @@ class Oracle:
         ip = C.POINTER(C.c_int32)
         lib().lima_oracle_pairs(self._h, radius_nm, pi.ctypes.data_as(ip), pj.ctypes.data_as(ip), ps.ctypes.data_as(ip))
         return pi[:n], pj[:n], ps[:n]
+
+    def set_pme_divide_modulus(self, on=True):
+        lib().lima_oracle_set_pme_divide_modulus(self._h, 1 if on else 0)
 
     def pme_dims(self):
         d = np.zeros(3, np.int32)
