@@ -20,8 +20,9 @@ from pathlib import Path
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
-FP32_VALU_PEAK_TFLOPS = 157.3      # MI355X published FP32 vector peak
+FP32_PEAK_TFLOPS = 157.3           # MI355X published dense FP32 peak
 FLOPS_PER_PAIR = 64                # algorithmic FP32 flops per pair inside the cutoff
+COPY_CALIBRATION_BYTES = 512 << 20
 
 
 def log(msg):
@@ -29,14 +30,32 @@ def log(msg):
 
 
 def load_traffic(kernel, workload):
-    path = ROOT / "profiles" / "traffic.json"
+    """HBM bytes per launch of `kernel`, from the committed counter summary
+    profiles/traffic_<kernel>_<workload>.txt (line 'traffic_bytes_per_launch = N'), or None."""
+    path = ROOT / "profiles" / f"traffic_{kernel}_{workload}.txt"
     if not path.exists():
         return None
-    try:
-        data = json.loads(path.read_text())
-        return data.get(kernel, {}).get(workload)
-    except Exception:
-        return None
+    for line in path.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch"):
+            try:
+                return int(float(line.split("=")[1]))
+            except ValueError:
+                return None
+    return None
+
+
+def calibrate_copy(repeats=5):
+    """A plain device-to-device copy of known size (512 MiB read + 512 MiB written per launch):
+    the FETCH_SIZE / WRITE_SIZE counters read on it give the factor that converts those counters
+    into bytes for the kernels measured."""
+    import torch
+    src = torch.ones(COPY_CALIBRATION_BYTES // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(repeats):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": COPY_CALIBRATION_BYTES, "launches": repeats}), flush=True)
+    return 0
 
 
 def main():
@@ -49,12 +68,16 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=8, help="oracle MD steps in the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run the copy kernel used to calibrate the HBM byte counters")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: this path has no multi-GPU mode (--gpus must be 1)", file=sys.stderr)
         return 2
+    if args.calibrate_copy:
+        return calibrate_copy()
 
     import lima_amd as L
 
@@ -100,8 +123,9 @@ def main():
     flops = pairs * FLOPS_PER_PAIR
     achieved = flops / (pair_ms * 1e-3) / 1e12 if pair_ms > 0 else 0.0
     workload = "chain_water_c1"
-    roofline = {"kernel": "k_pair", "bound": "valu", "achieved": round(achieved, 3), "peak": FP32_VALU_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(achieved / FP32_VALU_PEAK_TFLOPS, 5),
+    roofline = {"kernel": "k_pair", "bound": "mfma", "pipe": "fp32 vector ALU (no matrix instructions)",
+                "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 5),
                 "traffic": load_traffic("k_pair", workload),
                 "work": f"{pairs} pairs inside 1.2 nm x {FLOPS_PER_PAIR} flop / {pair_ms * 1e3:.2f} us"}
 
