@@ -220,6 +220,43 @@ lima_status lima_set_timing(lima_engine* engine, int32_t enabled);
  * (the algorithmic work unit of the pair kernel); -1 for a null engine. */
 int64_t lima_get_pair_count(lima_engine* engine);
 
+/* ---- Batched minimisation: Programs::EnergyMinimize (Programs.cpp:155-199) applied to many
+ * independent small systems at once.  LIMA minimises one Simulation per Environment; here one
+ * handle holds n systems, each minimised by steepest descent with FP64 acceptance exactly as
+ * lima_minimize does, all of them in one persistent launch per workgroup shape (one workgroup
+ * per system, no host round trip per iteration).  Limits per system: 1..8192 atoms, box edges
+ * in (cutoff, 10) nm, PME grid edges in [4, 32]. ---- */
+typedef struct lima_batch lima_batch;
+
+/* Workgroup shape per system-size bucket: systems with n atoms use the first bucket with
+ * n <= bucket_max_atoms.  threads: 256, 512 or 1024; i_per_thread: atoms a thread holds in
+ * registers per pass over the j tiles (1, 2 or 4).  us_per_iter: measured by the autotuner. */
+typedef struct lima_batch_tuning {
+    int32_t n_buckets;
+    int32_t bucket_max_atoms[8];
+    int32_t threads[8];
+    int32_t i_per_thread[8];
+    double  us_per_iter[8];
+} lima_batch_tuning;
+
+/* boxes[n_systems]; params apply to every system (cutoff, electrostatics, PME spacing/grid). */
+lima_status lima_batch_create(const lima_box_desc* boxes, int32_t n_systems, const lima_simparams* params,
+                              lima_batch** out);
+lima_status lima_batch_destroy(lima_batch* batch);
+/* Times every candidate shape for trial_iterations SD iterations on up to 8 systems per size
+ * bucket, keeps the fastest per bucket, and restores the starting positions. */
+lima_status lima_batch_autotune(lima_batch* batch, int32_t trial_iterations, lima_batch_tuning* out);
+lima_status lima_batch_set_tuning(lima_batch* batch, const lima_batch_tuning* tuning);
+lima_status lima_batch_get_tuning(lima_batch* batch, lima_batch_tuning* out);
+/* Minimises every system from its current positions; results[n_systems] per system
+ * (wall_seconds: the system's own time inside the launch); *wall_seconds: the whole batch. */
+lima_status lima_batch_minimize(lima_batch* batch, double fmax_tol_kj, int64_t max_iter, double initial_step_nm,
+                                lima_em_result* results, double* wall_seconds);
+/* State of one system in LIMA's layout (velocities are zero; forces of the accepted state). */
+lima_status lima_batch_get_state(lima_batch* batch, int32_t system, lima_state_out* out);
+/* Restores the positions every system had at lima_batch_create. */
+lima_status lima_batch_reset(lima_batch* batch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -15,7 +15,7 @@ from pathlib import Path
 
 import numpy as np
 
-from ._abi import BoxDesc, SimParams, Energy, RunStatus, StateOut, EmResult, Timings
+from ._abi import BoxDesc, SimParams, Energy, RunStatus, StateOut, EmResult, Timings, BatchTuning
 
 LIB_DIR = Path(__file__).resolve().parent / "lib"
 REPO_ROOT = Path(__file__).resolve().parent.parent
@@ -80,6 +80,15 @@ def load_hip():
             "lima_get_timings": ([vp, C.POINTER(Timings)], C.c_int),
             "lima_set_timing": ([vp, C.c_int32], C.c_int),
             "lima_get_pair_count": ([vp], C.c_int64),
+            "lima_batch_create": ([C.POINTER(BoxDesc), C.c_int32, C.POINTER(SimParams), C.POINTER(vp)], C.c_int),
+            "lima_batch_destroy": ([vp], C.c_int),
+            "lima_batch_autotune": ([vp, C.c_int32, C.POINTER(BatchTuning)], C.c_int),
+            "lima_batch_set_tuning": ([vp, C.POINTER(BatchTuning)], C.c_int),
+            "lima_batch_get_tuning": ([vp, C.POINTER(BatchTuning)], C.c_int),
+            "lima_batch_minimize": ([vp, C.c_double, C.c_int64, C.c_double, C.POINTER(EmResult), C.POINTER(C.c_double)],
+                                    C.c_int),
+            "lima_batch_get_state": ([vp, C.c_int32, C.POINTER(StateOut)], C.c_int),
+            "lima_batch_reset": ([vp], C.c_int),
         }
         for name, (args, res) in sig.items():
             fn = getattr(lib, name)
@@ -194,6 +203,23 @@ def em_batch_sizes(n_systems=256, lo=300, hi=3000, seed=1000):
     return sizes
 
 
+def _state_arrays(nc, nt):
+    """numpy buffers for a lima_state_out of a box with nc compounds and nt tinymols."""
+    out = {
+        "compound_origo": np.zeros((nc, 3), np.int32), "particle_coord": np.zeros((nc, 32, 3), np.int32),
+        "particle_velocity": np.zeros((nc, 32, 3), np.float32), "particle_force": np.zeros((nc, 32, 3), np.float32),
+        "particle_pot_energy": np.zeros((nc, 32), np.float32),
+        "tinymol_block": np.zeros((max(nt, 1), 3), np.int32), "tinymol_coord": np.zeros((max(nt, 1), 3), np.int32),
+        "tinymol_velocity": np.zeros((max(nt, 1), 3), np.float32), "tinymol_force": np.zeros((max(nt, 1), 3), np.float32),
+        "tinymol_pot_energy": np.zeros(max(nt, 1), np.float32),
+    }
+    st = StateOut()
+    names = [f[0] for f in StateOut._fields_]
+    for k, a in out.items():
+        setattr(st, k, a.ctypes.data_as(StateOut._fields_[names.index(k)][1]))
+    return out, st
+
+
 class Engine:
     """Handle of a lima_engine (Engine.cuh:78-99) on the GPU."""
 
@@ -244,18 +270,7 @@ class Engine:
 
     def state(self):
         """Forces, potE, velocities, positions in LIMA layout (numpy)."""
-        nc, nt = self.box.n_compounds, self.box.n_tinymols
-        st = StateOut()
-        out = {
-            "compound_origo": np.zeros((nc, 3), np.int32), "particle_coord": np.zeros((nc, 32, 3), np.int32),
-            "particle_velocity": np.zeros((nc, 32, 3), np.float32), "particle_force": np.zeros((nc, 32, 3), np.float32),
-            "particle_pot_energy": np.zeros((nc, 32), np.float32),
-            "tinymol_block": np.zeros((max(nt, 1), 3), np.int32), "tinymol_coord": np.zeros((max(nt, 1), 3), np.int32),
-            "tinymol_velocity": np.zeros((max(nt, 1), 3), np.float32), "tinymol_force": np.zeros((max(nt, 1), 3), np.float32),
-            "tinymol_pot_energy": np.zeros(max(nt, 1), np.float32),
-        }
-        for k, a in out.items():
-            setattr(st, k, a.ctypes.data_as(StateOut._fields_[[f[0] for f in StateOut._fields_].index(k)][1]))
+        out, st = _state_arrays(self.box.n_compounds, self.box.n_tinymols)
         self._check(self.lib.lima_get_state(self._h, C.byref(st)))
         return out
 
@@ -281,6 +296,65 @@ class Engine:
 
     def pair_count(self):
         return int(self.lib.lima_get_pair_count(self._h))
+
+
+class Batch:
+    """Handle of a lima_batch: many independent systems minimised together on the GPU."""
+
+    def __init__(self, boxes, params):
+        self.lib = load_hip()
+        self.boxes = list(boxes)
+        arr = (BoxDesc * len(self.boxes))()
+        for k, b in enumerate(self.boxes):
+            arr[k] = b.desc
+        self._descs = arr
+        h = C.c_void_p()
+        self._check(self.lib.lima_batch_create(arr, len(self.boxes), C.byref(params), C.byref(h)))
+        self._h = h
+
+    def _check(self, st):
+        if st != 0:
+            raise LimaError(f"status {st}: {self.lib.lima_last_error().decode()}")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.lima_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def autotune(self, trial_iterations=4):
+        t = BatchTuning()
+        self._check(self.lib.lima_batch_autotune(self._h, trial_iterations, C.byref(t)))
+        return t
+
+    def tuning(self):
+        t = BatchTuning()
+        self._check(self.lib.lima_batch_get_tuning(self._h, C.byref(t)))
+        return t
+
+    def set_tuning(self, t):
+        self._check(self.lib.lima_batch_set_tuning(self._h, C.byref(t)))
+
+    def minimize(self, fmax_tol_kj=100.0, max_iter=10000, initial_step_nm=0.01):
+        """Returns (list of EmResult, wall seconds of the whole batch)."""
+        res = (EmResult * len(self.boxes))()
+        wall = C.c_double()
+        self._check(self.lib.lima_batch_minimize(self._h, fmax_tol_kj, max_iter, initial_step_nm, res, C.byref(wall)))
+        return list(res), wall.value
+
+    def state(self, k):
+        b = self.boxes[k]
+        out, st = _state_arrays(b.n_compounds, b.n_tinymols)
+        self._check(self.lib.lima_batch_get_state(self._h, k, C.byref(st)))
+        return out
+
+    def reset(self):
+        self._check(self.lib.lima_batch_reset(self._h))
 
 
 class StateBox:

@@ -113,3 +113,17 @@ class Timings(C.Structure):
         ("pme_ms", C.c_double), ("integrate_ms", C.c_double), ("nlist_ms", C.c_double),
         ("total_ms", C.c_double), ("steps_timed", C.c_int64),
     ]
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [
+        ("n_buckets", C.c_int32),
+        ("bucket_max_atoms", C.c_int32 * 8),
+        ("threads", C.c_int32 * 8),
+        ("i_per_thread", C.c_int32 * 8),
+        ("us_per_iter", C.c_double * 8),
+    ]
+
+    def as_list(self):
+        return [{"max_atoms": self.bucket_max_atoms[k], "threads": self.threads[k], "i_per_thread": self.i_per_thread[k],
+                 "us_per_iter": round(self.us_per_iter[k], 2)} for k in range(self.n_buckets)]

@@ -1,0 +1,1202 @@
+// batch.hip -- batched steepest-descent minimisation of many small periodic systems in one
+// persistent launch per workgroup shape.
+//
+// Each workgroup owns one system and runs its whole minimisation (Programs::EnergyMinimize,
+// Programs.cpp:155-199, with steepest descent and FP64 acceptance instead of LIMA's ADAM EM):
+//
+//   E, F <- forces(x); h <- h0
+//   while Fmax >= tol and iterations < max:
+//       x' = x + trunc(h F / Fmax)            (integer Coord, per atom)
+//       E', F' <- forces(x')
+//       accept (x, F, E <- x', F', E'; h *= 1.2) if E' < E in FP64, else h *= 0.2
+//
+// Force evaluation inside the workgroup, with no inter-workgroup synchronisation:
+//   - nonbonded cutoff pairs (LennardJonesInteractions.cuh:74-203): all pairs i, j of the system
+//     with j-tiles staged in LDS (broadcast reads); boxes are 1.5-3.5 nm against a 1.2 nm
+//     cutoff, so the cutoff sphere covers 25-100 % of the box and a cell list would not prune.
+//     Every periodic image within the cutoff counts (two candidates per axis when an edge is
+//     shorter than 2 rc).  Pairs inside one compound or between bonded compounds are skipped
+//     here and computed by the intra pass.
+//   - intra pairs (LennardJonesInteractions.cuh:121-161): same / bonded compound, BondedParticlesLUT
+//     exclusions, no cutoff, unwrapped displacement.
+//   - bonded terms (ForceComputations.cuh): each atom sums the terms it belongs to from a per-atom
+//     incidence list (the term is evaluated once per member atom, so no atomics).
+//   - PME (PME.cuh): int32 fixed-point spreading into an LDS grid, DFT passes on the per-system
+//     half spectrum, LIMA's influence function, finite-difference gather from an LDS potential.
+//   Energies are reduced in FP64 in a fixed order; the run is deterministic.
+//
+// Block shapes (threads per workgroup, i atoms held in registers per pass) come from
+// lima_batch_autotune, which times every candidate shape on the batch's own systems.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "engine.h"
+#include "lima_terms.h"
+
+namespace lima {
+namespace {
+
+constexpr int KMAX = 32;                       // largest PME grid edge handled in registers
+constexpr int MAX_ATOMS = 8192;                // atoms per system
+constexpr float FIX_SCALE = 1048576.0f;        // 2^20 fixed-point units per kC/mol of spread charge
+constexpr int NBUCKET = 4;
+constexpr int BUCKET_MAX[NBUCKET] = {600, 1200, 2000, MAX_ATOMS};
+
+inline int nblk(int64_t n, int b) { return (int)((n + b - 1) / b); }
+
+struct SysMeta {
+    int n, atom0, nc, comp0;
+    int Lq[3], hLq[3];
+    float L[3];
+    int K[3], H, NG;
+    int spec0;          // offset of this system's half spectrum and influence function
+    int tw0[3];         // twiddle table offsets per axis
+    float fix_to_rho;   // fixed point -> charge density: 1 / (2^20 vcell)
+    float inv_ng;
+    float inv2h[3];
+    double eself;       // -kappa / sqrt(pi_L) k_C sum q^2 (standard Ewald self energy)
+    int multi;          // an edge shorter than 2 rc: up to two images per axis
+    int elec;           // system has charges and electrostatics on
+    int lds_bytes;
+};
+
+struct SysOut {
+    double e_acc;       // accepted potential energy [J/mol], incl. the Ewald self energy
+    double e_parts[4];  // lj, coulomb short range, bonded, pme reciprocal of the accepted state
+    float fmax;         // [J/mol/nm]
+    float h;
+    int iterations, accepted, converged, cur;
+    unsigned long long t0, t1;   // s_memrealtime (100 MHz) at start / end
+};
+
+struct BatchArgs {
+    const SysMeta* meta;
+    const int* order;            // system of each workgroup
+    int4* u0; int4* u1;          // unwrapped Coord (two buffers: accepted / trial)
+    float4* f0; float4* f1;      // forces
+    const float4* prm;           // (q kC/mol, sigma/2 nm, sqrt(eps), sqrt(24 eps))
+    const int* comp;             // local compound of each atom
+    const int2* cinfo;           // per compound (first atom local, atom count)
+    const int* cbm;              // per compound bonded-compound mask (bit k: compound c+k-2)
+    const uint32_t* clut;        // per compound [5][32] exclusion rows
+    const int* inc_off;          // per atom incidence offsets (global atom index, N+1)
+    const int* inc;              // (type << 29) | (role << 27) | term
+    const int2* bid; const float2* bp;
+    const int4* aid; const float4* ap;
+    const int4* did; const float4* dp;
+    const int4* iid; const float2* ip;
+    const int2* pid; const float2* pp;
+    const float* green;
+    float2* spec;
+    const float2* twid;
+    SysOut* out;
+    PairConst pc;
+    float rc, rc2;
+    double tol;                  // Fmax tolerance [J/mol/nm]
+    double h0;                   // initial step [nm]
+    int max_iter;
+};
+
+enum { T_BOND = 0, T_ANGLE = 1, T_DIH = 2, T_IMP = 3, T_PAIR = 4 };
+
+__device__ __forceinline__ int fmodi(int x, int L) {
+    int r = x % L;
+    return r < 0 ? r + L : r;
+}
+
+__device__ __forceinline__ v3 udisp(const int4* u, int a, int b) {   // x_b - x_a [nm], unwrapped
+    const int4 ua = u[a], ub = u[b];
+    return {(float)(ub.x - ua.x) * LIMA_LIMA_TO_NANO_F, (float)(ub.y - ua.y) * LIMA_LIMA_TO_NANO_F,
+            (float)(ub.z - ua.z) * LIMA_LIMA_TO_NANO_F};
+}
+
+// One cutoff pair evaluated from i's side: force on i += d S, half the pair energy on i.
+__device__ __forceinline__ void pair_eval(float x, float y, float z, float r2, float4 pi, float4 pj, const PairConst& pc,
+                                          v3& acc, float& elj, float& ec) {
+    float e1, e2;
+    const float S = pair_scalar_cut(r2, pi.y + pj.y, pi.w * pj.w, pi.x * pj.x, pc, &e1, &e2);
+    acc.x += x * S;
+    acc.y += y * S;
+    acc.z += z * S;
+    elj += 0.5f * e1;
+    ec += 0.5f * e2;
+}
+
+template <bool MULTI>
+__device__ __forceinline__ void pair_cut(const int4& wi, const float4& pi, int ci, int bmi, const int4& wj, const float4& pj,
+                                         const SysMeta& m, const BatchArgs& a, v3& acc, float& elj, float& ec) {
+    const int dc = wj.w - ci;
+    if ((unsigned)(dc + 2) <= 4u && ((bmi >> (dc + 2)) & 1)) return;   // same or bonded compound: intra pass
+    int dx = wj.x - wi.x, dy = wj.y - wi.y, dz = wj.z - wi.z;
+    dx += dx < -m.hLq[0] ? m.Lq[0] : 0; dx -= dx >= m.hLq[0] ? m.Lq[0] : 0;
+    dy += dy < -m.hLq[1] ? m.Lq[1] : 0; dy -= dy >= m.hLq[1] ? m.Lq[1] : 0;
+    dz += dz < -m.hLq[2] ? m.Lq[2] : 0; dz -= dz >= m.hLq[2] ? m.Lq[2] : 0;
+    const float x = (float)dx * LIMA_LIMA_TO_NANO_F, y = (float)dy * LIMA_LIMA_TO_NANO_F, z = (float)dz * LIMA_LIMA_TO_NANO_F;
+    if (!MULTI) {
+        const float r2 = x * x + y * y + z * z;
+        if (r2 <= a.rc2) pair_eval(x, y, z, r2, pi, pj, a.pc, acc, elj, ec);
+    } else {
+        // second candidate image per axis: the one on the other side, when within the cutoff
+        const float ax = x - copysignf(m.L[0], x), ay = y - copysignf(m.L[1], y), az = z - copysignf(m.L[2], z);
+        const int nx = fabsf(ax) <= a.rc ? 2 : 1, ny = fabsf(ay) <= a.rc ? 2 : 1, nz = fabsf(az) <= a.rc ? 2 : 1;
+        for (int kz = 0; kz < nz; kz++) {
+            const float zz = kz ? az : z;
+            for (int ky = 0; ky < ny; ky++) {
+                const float yy = ky ? ay : y;
+                for (int kx = 0; kx < nx; kx++) {
+                    const float xx = kx ? ax : x;
+                    const float r2 = xx * xx + yy * yy + zz * zz;
+                    if (r2 <= a.rc2) pair_eval(xx, yy, zz, r2, pi, pj, a.pc, acc, elj, ec);
+                }
+            }
+        }
+    }
+}
+
+// intra pairs of atom i (local li): its compound and the bonded compounds, LUT-excluded, no cutoff
+__device__ __forceinline__ void intra_atom(const BatchArgs& a, const SysMeta& m, const int4* u, int li, int ci, int bmi, float4 pi,
+                                           v3& acc, float& elj, float& ec) {
+    const int gi = m.atom0 + li;
+    const int2 ca = a.cinfo[m.comp0 + ci];
+    const int si = li - ca.x;
+    for (int k = 0; k < 5; k++) {
+        if (!((bmi >> k) & 1)) continue;
+        const int B = ci + k - 2;
+        const uint32_t row = a.clut[((size_t)(m.comp0 + ci) * 5 + k) * 32 + si];
+        const int2 cb = a.cinfo[m.comp0 + B];
+        for (int sj = 0; sj < cb.y; sj++) {
+            if ((row >> sj) & 1u) continue;
+            const int gj = m.atom0 + cb.x + sj;
+            const v3 d = udisp(u, gi, gj);
+            const float r2 = dot(d, d);
+            const float4 pj = a.prm[gj];
+            float e1, e2;
+            const float S = pair_scalar(r2, pi.y + pj.y, pi.z * pj.z, pi.x * pj.x, a.pc, &e1, &e2);
+            acc = acc + d * S;
+            elj += 0.5f * e1;
+            ec += 0.5f * e2;
+        }
+    }
+}
+
+// bonded terms containing atom gi, summed from its incidence list; the term energy is counted by
+// the incidence with role 0
+__device__ __forceinline__ void bonded_atom(const BatchArgs& a, const int4* u, int gi, v3& acc, float& eb) {
+    const int e0 = a.inc_off[gi], e1 = a.inc_off[gi + 1];
+    for (int q = e0; q < e1; q++) {
+        const int code = a.inc[q];
+        const int type = (code >> 29) & 7, role = (code >> 27) & 3, t = code & ((1 << 27) - 1);
+        v3 f = {0.f, 0.f, 0.f};
+        float e = 0.f;
+        if (type == T_BOND) {
+            const int2 id = a.bid[t];
+            v3 fa;
+            e = term_bond(udisp(u, id.y, id.x), a.bp[t], &fa);
+            f = role == 0 ? fa : -fa;
+        } else if (type == T_ANGLE) {
+            const int4 id = a.aid[t];
+            v3 fl, fr;
+            e = term_angle(udisp(u, id.y, id.x), udisp(u, id.y, id.z), udisp(u, id.z, id.x), a.ap[t], &fl, &fr);
+            f = role == 0 ? fl : (role == 2 ? fr : -(fl + fr));
+        } else if (type == T_DIH) {
+            const int4 id = a.did[t];
+            v3 fd[4];
+            e = term_dihedral(udisp(u, id.x, id.y), udisp(u, id.y, id.z), udisp(u, id.z, id.w), a.dp[t], fd);
+            f = role == 0 ? fd[0] : (role == 1 ? fd[1] : (role == 2 ? fd[2] : fd[3]));
+        } else if (type == T_IMP) {
+            const int4 id = a.iid[t];
+            v3 fd[4];
+            e = term_improper(udisp(u, id.x, id.y), udisp(u, id.x, id.z), udisp(u, id.x, id.w), a.ip[t], fd);
+            f = role == 0 ? fd[0] : (role == 1 ? fd[1] : (role == 2 ? fd[2] : fd[3]));
+        } else {
+            const int2 id = a.pid[t];
+            v3 fa;
+            e = term_pair14(udisp(u, id.x, id.y), a.pp[t], &fa);
+            f = role == 0 ? fa : -fa;
+        }
+        acc = acc + f;
+        if (role == 0) eb += e;
+    }
+}
+
+// Length-K DFT of a line held in registers, one output at a time:
+// out[k] = sum_j in[j] W^(j k), W = exp(-+2 pi i / K) from the twiddle table.
+template <bool INV>
+__device__ __forceinline__ float2 dft_out(const float2* in, int K, int k, const float2* tw) {
+    float re = 0.f, im = 0.f;
+    int idx = 0;
+#pragma unroll
+    for (int j = 0; j < KMAX; j++) {
+        if (j < K) {
+            const float2 w = tw[idx];
+            const float wi = INV ? -w.y : w.y;
+            re += in[j].x * w.x - in[j].y * wi;
+            im += in[j].x * wi + in[j].y * w.x;
+            idx += k;
+            idx -= idx >= K ? K : 0;
+        }
+    }
+    return make_float2(re, im);
+}
+
+template <int T>
+struct BlockShared {
+    double red[T / 64][5];
+    float fred[T / 64];
+    double res[5];
+    float fres;
+    float2 tw[3][KMAX];
+};
+
+// FP64 sums of v[0..3] and max of fm over the workgroup, in a fixed order; every thread gets them.
+template <int T>
+__device__ __forceinline__ void block_reduce(BlockShared<T>& sh, double* v, float fm) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int c = 0; c < 4; c++) v[c] = wave_sum_d(v[c]);
+    fm = wave_max(fm);
+    if (lane == 0) {
+        for (int c = 0; c < 4; c++) sh.red[wv][c] = v[c];
+        sh.fred[wv] = fm;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s[4] = {0, 0, 0, 0};
+        float f = 0.f;
+        for (int w = 0; w < T / 64; w++) {
+            for (int c = 0; c < 4; c++) s[c] += sh.red[w][c];
+            f = fmaxf(f, sh.fred[w]);
+        }
+        for (int c = 0; c < 4; c++) sh.res[c] = s[c];
+        sh.fres = f;
+    }
+    __syncthreads();
+    for (int c = 0; c < 4; c++) v[c] = sh.res[c];
+}
+
+struct Eval {
+    double lj, coul, bond, pme;
+    float fmax;
+    __device__ double total() const { return lj + coul + bond + pme; }
+};
+
+// Forces and energy of the system at positions u (global arrays); writes f and returns the FP64
+// energy parts and Fmax (identical in every thread).
+template <int T, int RB, bool MULTI>
+__device__ __forceinline__ Eval evaluate(const BatchArgs& a, const SysMeta& m, BlockShared<T>& sh, unsigned char* lds,
+                                         const int4* u, float4* f) {
+    const int tid = threadIdx.x;
+    const int n = m.n;
+    double elj_d = 0.0, ec_d = 0.0, eb_d = 0.0, ep_d = 0.0;
+    int4* tw4 = reinterpret_cast<int4*>(lds);
+    float4* tp4 = reinterpret_cast<float4*>(lds + T * sizeof(int4));
+    // ---- pairs, intra pairs and bonded terms, RB atoms per thread per pass ----
+    for (int c0 = 0; c0 < n; c0 += T * RB) {
+        int4 wi[RB];
+        float4 pi[RB];
+        int ci[RB], bmi[RB];
+        v3 acc[RB];
+        float elj[RB], ec[RB];
+#pragma unroll
+        for (int r = 0; r < RB; r++) {
+            const int li = c0 + r * T + tid;
+            acc[r] = {0.f, 0.f, 0.f};
+            elj[r] = 0.f;
+            ec[r] = 0.f;
+            if (li < n) {
+                const int4 uu = u[m.atom0 + li];
+                ci[r] = a.comp[m.atom0 + li];
+                wi[r] = make_int4(fmodi(uu.x, m.Lq[0]), fmodi(uu.y, m.Lq[1]), fmodi(uu.z, m.Lq[2]), ci[r]);
+                pi[r] = a.prm[m.atom0 + li];
+                bmi[r] = a.cbm[m.comp0 + ci[r]];
+            } else {
+                ci[r] = -1000; bmi[r] = 0; wi[r] = make_int4(0, 0, 0, 0); pi[r] = make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+        }
+        for (int j0 = 0; j0 < n; j0 += T) {
+            __syncthreads();
+            const int lj = j0 + tid;
+            if (lj < n) {
+                const int4 uu = u[m.atom0 + lj];
+                tw4[tid] = make_int4(fmodi(uu.x, m.Lq[0]), fmodi(uu.y, m.Lq[1]), fmodi(uu.z, m.Lq[2]), a.comp[m.atom0 + lj]);
+                tp4[tid] = a.prm[m.atom0 + lj];
+            }
+            __syncthreads();
+            const int nj = min(T, n - j0);
+            for (int jj = 0; jj < nj; jj++) {
+                const int4 wj = tw4[jj];
+                const float4 pj = tp4[jj];
+#pragma unroll
+                for (int r = 0; r < RB; r++)
+                    if (c0 + r * T + tid < n) pair_cut<MULTI>(wi[r], pi[r], ci[r], bmi[r], wj, pj, m, a, acc[r], elj[r], ec[r]);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < RB; r++) {
+            const int li = c0 + r * T + tid;
+            if (li >= n) continue;
+            intra_atom(a, m, u, li, ci[r], bmi[r], pi[r], acc[r], elj[r], ec[r]);
+            float eb = 0.f;
+            bonded_atom(a, u, m.atom0 + li, acc[r], eb);
+            f[m.atom0 + li] = make_float4(acc[r].x, acc[r].y, acc[r].z, 0.f);
+            elj_d += (double)elj[r];
+            ec_d += (double)ec[r];
+            eb_d += (double)eb;
+        }
+    }
+    // ---- PME ----
+    if (m.elec) {
+        __syncthreads();   // LDS switches from pair tiles to the grid
+        int* qg = reinterpret_cast<int*>(lds);
+        float* phi = reinterpret_cast<float*>(lds);
+        const int K0 = m.K[0], K1 = m.K[1], K2 = m.K[2], H = m.H;
+        for (int g = tid; g < m.NG; g += T) qg[g] = 0;
+        for (int d = 0; d < 3; d++)
+            for (int k = tid; k < m.K[d]; k += T) sh.tw[d][k] = a.twid[m.tw0[d] + k];
+        __syncthreads();
+        for (int li = tid; li < n; li += T) {   // spread q w_x w_y w_z (int32 fixed point)
+            const float q = a.prm[m.atom0 + li].x;
+            if (q == 0.f) continue;
+            const int4 uu = u[m.atom0 + li];
+            int ix[3];
+            float fr[3], wx[4], wy[4], wz[4];
+            grid_coord((uint32_t)fmodi(uu.x, m.Lq[0]), m.Lq[0], K0, &ix[0], &fr[0]);
+            grid_coord((uint32_t)fmodi(uu.y, m.Lq[1]), m.Lq[1], K1, &ix[1], &fr[1]);
+            grid_coord((uint32_t)fmodi(uu.z, m.Lq[2]), m.Lq[2], K2, &ix[2], &fr[2]);
+            bspline(fr[0], wx); bspline(fr[1], wy); bspline(fr[2], wz);
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const int Z = wrapi(ix[2] - 1 + c, K2);
+#pragma unroll
+                for (int b = 0; b < 4; b++) {
+                    const int Y = wrapi(ix[1] - 1 + b, K1);
+                    const float qyz = q * wy[b] * wz[c];
+                    int* row = qg + (Z * K1 + Y) * K0;
+#pragma unroll
+                    for (int x = 0; x < 4; x++) atomicAdd(&row[wrapi(ix[0] - 1 + x, K0)], __float2int_rn(qyz * wx[x] * FIX_SCALE));
+                }
+            }
+        }
+        __syncthreads();
+        float2* spec = a.spec + m.spec0;
+        // x forward, real -> half spectrum: lines (y, z)
+        for (int l = tid; l < K1 * K2; l += T) {
+            float2 line[KMAX];
+            const int* src = qg + l * K0;
+#pragma unroll
+            for (int j = 0; j < KMAX; j++) line[j] = make_float2(j < K0 ? (float)src[j] * m.fix_to_rho : 0.f, 0.f);
+            float2* dst = spec + l * H;
+            for (int k = 0; k < H; k++) dst[k] = dft_out<false>(line, K0, k, sh.tw[0]);
+        }
+        __syncthreads();
+        // y forward: lines (x < H, z), stride H
+        for (int l = tid; l < H * K2; l += T) {
+            float2 line[KMAX];
+            const int x = l % H, z = l / H;
+            float2* base = spec + z * K1 * H + x;
+#pragma unroll
+            for (int j = 0; j < KMAX; j++) line[j] = j < K1 ? base[j * H] : make_float2(0.f, 0.f);
+            for (int k = 0; k < K1; k++) base[k * H] = dft_out<false>(line, K1, k, sh.tw[1]);
+        }
+        __syncthreads();
+        // z forward times the influence function, then z inverse: lines (x < H, y), stride K1 H
+        for (int l = tid; l < H * K1; l += T) {
+            float2 line[KMAX];
+            const int x = l % H, y = l / H;
+            float2* base = spec + y * H + x;
+            const float* g = a.green + m.spec0 + y * H + x;
+#pragma unroll
+            for (int j = 0; j < KMAX; j++) line[j] = j < K2 ? base[j * K1 * H] : make_float2(0.f, 0.f);
+            for (int k = 0; k < K2; k++) {
+                const float2 v = dft_out<false>(line, K2, k, sh.tw[2]);
+                const float gk = g[k * K1 * H];
+                base[k * K1 * H] = make_float2(v.x * gk, v.y * gk);
+            }
+        }
+        __syncthreads();
+        for (int l = tid; l < H * K1; l += T) {
+            float2 line[KMAX];
+            const int x = l % H, y = l / H;
+            float2* base = spec + y * H + x;
+#pragma unroll
+            for (int j = 0; j < KMAX; j++) line[j] = j < K2 ? base[j * K1 * H] : make_float2(0.f, 0.f);
+            for (int k = 0; k < K2; k++) base[k * K1 * H] = dft_out<true>(line, K2, k, sh.tw[2]);
+        }
+        __syncthreads();
+        for (int l = tid; l < H * K2; l += T) {
+            float2 line[KMAX];
+            const int x = l % H, z = l / H;
+            float2* base = spec + z * K1 * H + x;
+#pragma unroll
+            for (int j = 0; j < KMAX; j++) line[j] = j < K1 ? base[j * H] : make_float2(0.f, 0.f);
+            for (int k = 0; k < K1; k++) base[k * H] = dft_out<true>(line, K1, k, sh.tw[1]);
+        }
+        __syncthreads();
+        // x inverse, half spectrum -> real potential (Hermitian sum) / NG into LDS
+        for (int l = tid; l < K1 * K2; l += T) {
+            float2 line[KMAX];
+            const float2* src = spec + l * H;
+#pragma unroll
+            for (int j = 0; j < KMAX / 2 + 1; j++) line[j] = j < H ? src[j] : make_float2(0.f, 0.f);
+            float* dst = phi + l * K0;
+            const bool even = (K0 & 1) == 0;
+            for (int xj = 0; xj < K0; xj++) {
+                float s = line[0].x;
+                int idx = xj;
+#pragma unroll
+                for (int k = 1; k < KMAX / 2 + 1; k++) {
+                    if (k < H) {
+                        const float2 w = sh.tw[0][idx];   // exp(-i ...); the inverse uses the conjugate
+                        const float re = line[k].x * w.x + line[k].y * w.y;
+                        s += (even && k == H - 1) ? re : 2.f * re;
+                        idx += xj;
+                        idx -= idx >= K0 ? K0 : 0;
+                    }
+                }
+                dst[xj] = s * m.inv_ng;
+            }
+        }
+        __syncthreads();
+        // gather: phi and the finite-difference field, force q E, energy q phi / 2
+        for (int li = tid; li < n; li += T) {
+            const float q = a.prm[m.atom0 + li].x;
+            if (q == 0.f) continue;
+            const int4 uu = u[m.atom0 + li];
+            int ix[3];
+            float fr[3], wx[4], wy[4], wz[4];
+            grid_coord((uint32_t)fmodi(uu.x, m.Lq[0]), m.Lq[0], K0, &ix[0], &fr[0]);
+            grid_coord((uint32_t)fmodi(uu.y, m.Lq[1]), m.Lq[1], K1, &ix[1], &fr[1]);
+            grid_coord((uint32_t)fmodi(uu.z, m.Lq[2]), m.Lq[2], K2, &ix[2], &fr[2]);
+            bspline(fr[0], wx); bspline(fr[1], wy); bspline(fr[2], wz);
+            float w6x[6], w6y[6], w6z[6], d6x[6], d6y[6], d6z[6];
+            gather_weights(wx, w6x, d6x);
+            gather_weights(wy, w6y, d6y);
+            gather_weights(wz, w6z, d6z);
+            float ph = 0.f, ex = 0.f, ey = 0.f, ez = 0.f;
+#pragma unroll
+            for (int mz = 0; mz < 6; mz++) {
+                const int Z = wrapi(wrapi(ix[2] - 2 + mz, K2), K2);
+#pragma unroll
+                for (int my = 0; my < 6; my++) {
+                    const int Y = wrapi(wrapi(ix[1] - 2 + my, K1), K1);
+                    const float* row = phi + (Z * K1 + Y) * K0;
+                    const float wyz = w6y[my] * w6z[mz], dyz = d6y[my] * w6z[mz], wdz = w6y[my] * d6z[mz];
+#pragma unroll
+                    for (int mx = 0; mx < 6; mx++) {
+                        const float v = row[wrapi(wrapi(ix[0] - 2 + mx, K0), K0)];
+                        ph += w6x[mx] * wyz * v;
+                        ex += d6x[mx] * wyz * v;
+                        ey += w6x[mx] * dyz * v;
+                        ez += w6x[mx] * wdz * v;
+                    }
+                }
+            }
+            float4 fo = f[m.atom0 + li];
+            fo.x += q * ex * m.inv2h[0];
+            fo.y += q * ey * m.inv2h[1];
+            fo.z += q * ez * m.inv2h[2];
+            f[m.atom0 + li] = fo;
+            ep_d += (double)(0.5f * q * ph);
+        }
+    }
+    // ---- Fmax and the FP64 energy sums ----
+    float fm = 0.f;
+    for (int li = tid; li < n; li += T) {
+        const float4 fo = f[m.atom0 + li];
+        fm = fmaxf(fm, sqrtf(fo.x * fo.x + fo.y * fo.y + fo.z * fo.z));
+    }
+    double v[4] = {elj_d, ec_d, eb_d, ep_d};
+    block_reduce<T>(sh, v, fm);
+    return Eval{v[0], v[1], v[2], v[3], sh.fres};
+}
+
+template <int T, int RB, bool MULTI>
+__device__ __forceinline__ void minimize_system(const BatchArgs& a, const SysMeta& m, SysOut* out, BlockShared<T>& sh,
+                                                unsigned char* lds) {
+    const int tid = threadIdx.x;
+    Eval acc_e{0.0, 0.0, 0.0, 0.0, 0.f};
+    double h = a.h0;
+    int cur = 0, it = 0, nacc = 0;
+    bool first = true, conv = false;
+    for (;;) {
+        // the first pass evaluates the starting state (buffer 0); later passes a trial step
+        const int tgt = first ? 0 : 1 - cur;
+        int4* ut = (tgt ? a.u1 : a.u0) + m.atom0;
+        float4* ft = (tgt ? a.f1 : a.f0) + m.atom0;
+        if (!first) {
+            const float k = (float)(h / (double)acc_e.fmax);
+            const int4* uc = (cur ? a.u1 : a.u0) + m.atom0;
+            const float4* fc = (cur ? a.f1 : a.f0) + m.atom0;
+            for (int li = tid; li < m.n; li += T) {
+                const int4 x = uc[li];
+                const float4 g = fc[li];
+                ut[li] = make_int4(x.x + (int)fminf(fmaxf(k * g.x * LIMA_NANO_TO_LIMA_F, -2.0e8f), 2.0e8f),
+                                   x.y + (int)fminf(fmaxf(k * g.y * LIMA_NANO_TO_LIMA_F, -2.0e8f), 2.0e8f),
+                                   x.z + (int)fminf(fmaxf(k * g.z * LIMA_NANO_TO_LIMA_F, -2.0e8f), 2.0e8f), 0);
+            }
+            __syncthreads();
+        }
+        const Eval ev = evaluate<T, RB, MULTI>(a, m, sh, lds, ut - m.atom0, ft - m.atom0);
+        if (first) {
+            acc_e = ev;
+            first = false;
+        } else {
+            if (ev.total() < acc_e.total()) {   // FP64 acceptance
+                cur = tgt;
+                acc_e = ev;
+                h *= 1.2;
+                nacc++;
+            } else {
+                h *= 0.2;
+            }
+            it++;
+        }
+        conv = (double)acc_e.fmax < a.tol;
+        if (conv || it >= a.max_iter || h < 1e-12) break;
+    }
+    if (cur == 1) {   // the accepted state goes back to buffer 0
+        __syncthreads();
+        for (int li = tid; li < m.n; li += T) {
+            a.u0[m.atom0 + li] = a.u1[m.atom0 + li];
+            a.f0[m.atom0 + li] = a.f1[m.atom0 + li];
+        }
+    }
+    if (tid == 0) {
+        out->e_acc = acc_e.total() + m.eself;
+        out->e_parts[0] = acc_e.lj;
+        out->e_parts[1] = acc_e.coul;
+        out->e_parts[2] = acc_e.bond;
+        out->e_parts[3] = acc_e.pme;
+        out->fmax = acc_e.fmax;
+        out->h = (float)h;
+        out->iterations = it;
+        out->accepted = nacc;
+        out->converged = conv ? 1 : 0;
+        out->cur = 0;
+    }
+}
+
+template <int T, int RB>
+__global__ __launch_bounds__(T) void k_em_batch(BatchArgs a) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ BlockShared<T> sh;
+    const int s = a.order[blockIdx.x];
+    const SysMeta m = a.meta[s];
+    SysOut* out = a.out + s;
+    if (threadIdx.x == 0) out->t0 = __builtin_amdgcn_s_memrealtime();
+    if (m.multi) minimize_system<T, RB, true>(a, m, out, sh, lds);
+    else minimize_system<T, RB, false>(a, m, out, sh, lds);
+    __syncthreads();
+    if (threadIdx.x == 0) out->t1 = __builtin_amdgcn_s_memrealtime();
+}
+
+// ---------------------------------------------------------------------------------------------
+
+struct Shape { int threads, rb; };
+constexpr Shape SHAPES[] = {{256, 1}, {256, 2}, {256, 4}, {512, 1}, {512, 2}, {512, 4}, {1024, 1}, {1024, 2}, {1024, 4}};
+constexpr int NSHAPES = (int)(sizeof(SHAPES) / sizeof(SHAPES[0]));
+
+typedef void (*KernelFn)(BatchArgs);
+KernelFn shape_kernel(int i) {
+    switch (i) {
+        case 0: return k_em_batch<256, 1>;
+        case 1: return k_em_batch<256, 2>;
+        case 2: return k_em_batch<256, 4>;
+        case 3: return k_em_batch<512, 1>;
+        case 4: return k_em_batch<512, 2>;
+        case 5: return k_em_batch<512, 4>;
+        case 6: return k_em_batch<1024, 1>;
+        case 7: return k_em_batch<1024, 2>;
+        default: return k_em_batch<1024, 4>;
+    }
+}
+
+int smooth235(int n) { for (int p : {2, 3, 5}) while (n % p == 0) n /= p; return n == 1; }
+int derive_grid_b(double L, double spacing) {
+    int n = (int)std::ceil(L / spacing - 1e-9);
+    if (n < 4) n = 4;
+    while (!smooth235(n)) n++;
+    return n;
+}
+int64_t fdiv64(int64_t a, int64_t b) { int64_t q = a / b; if ((a % b != 0) && ((a < 0) != (b < 0))) --q; return q; }
+
+template <class T>
+T* dcopy(const std::vector<T>& v) {
+    T* p = nullptr;
+    if (v.empty()) return nullptr;
+    HIP_CHECK(hipMalloc(&p, v.size() * sizeof(T)));
+    HIP_CHECK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return p;
+}
+
+}  // namespace
+
+struct Batch {
+    int nsys = 0;
+    int64_t N = 0;
+    lima_simparams prm{};
+    PairConst pc{};
+    float rc = 1.2f;
+    std::vector<SysMeta> meta;
+    std::vector<int> h_ncomp;                  // compounds per system
+    std::vector<std::vector<int>> h_cn;        // per system: atoms per compound
+    std::vector<std::vector<int>> h_key;       // per system: key particle per compound
+    std::vector<int> h_ntiny;
+    std::vector<int4> u_init;
+    // device
+    SysMeta* d_meta = nullptr;
+    int* d_order = nullptr;
+    int4 *u0 = nullptr, *u1 = nullptr, *u_start = nullptr;
+    float4 *f0 = nullptr, *f1 = nullptr;
+    float4* prm4 = nullptr;
+    int* comp = nullptr;
+    int2* cinfo = nullptr;
+    int* cbm = nullptr;
+    uint32_t* clut = nullptr;
+    int *inc_off = nullptr, *inc = nullptr;
+    int2* bid = nullptr; float2* bp = nullptr;
+    int4* aid = nullptr; float4* ap = nullptr;
+    int4* did = nullptr; float4* dp = nullptr;
+    int4* iid = nullptr; float2* ip = nullptr;
+    int2* pid = nullptr; float2* pp = nullptr;
+    float* green = nullptr;
+    float2* spec = nullptr;
+    float2* twid = nullptr;
+    SysOut* out = nullptr;
+    hipStream_t streams[NSHAPES] = {};
+    lima_batch_tuning tuning{};
+    bool tuned = false;
+    bool forces_valid = false;
+};
+
+namespace {
+
+void batch_free(Batch& b) {
+    for (auto& s : b.streams) if (s) { (void)hipStreamSynchronize(s); (void)hipStreamDestroy(s); s = nullptr; }
+    void* ptrs[] = {b.d_meta, b.d_order, b.u0, b.u1, b.u_start, b.f0, b.f1, b.prm4, b.comp, b.cinfo, b.cbm, b.clut, b.inc_off, b.inc,
+                    b.bid, b.bp, b.aid, b.ap, b.did, b.dp, b.iid, b.ip, b.pid, b.pp, b.green, b.spec, b.twid, b.out};
+    for (void* p : ptrs) if (p) (void)hipFree(p);
+}
+
+void batch_create(Batch& B, const lima_box_desc* boxes, int nsys, const lima_simparams* p) {
+    if (!boxes || !p || nsys <= 0) throw LimaFail(LIMA_ERR_BAD_INPUT, "null descriptor or empty batch");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw LimaFail(LIMA_ERR_NO_DEVICE, "no HIP device visible");
+    if (p->bc_select != LIMA_BC_PBC) throw LimaFail(LIMA_ERR_UNSUPPORTED, "only periodic boundary conditions are implemented");
+    if (!(p->cutoff_nm > 0.f)) throw LimaFail(LIMA_ERR_BAD_INPUT, "cutoff must be positive");
+    B.prm = *p;
+    B.nsys = nsys;
+    B.rc = p->cutoff_nm;
+    const bool elec = p->enable_electrostatics != 0;
+    B.pc.kc = (float)(8.9875517873681764e9 / 1e-9 / 6.02214076e23 * 1000.f * 1000.f);   // PhysicsUtils.cuh:5
+    B.pc.kappa = 3.f / B.rc;                                                             // PhysicsUtils.cuh:60-62
+    B.pc.two_kappa_sqrtpi = 2.f * B.pc.kappa / sqrtf(LIMA_PI_F);
+    const double spacing = p->pme_spacing_nm > 0 ? p->pme_spacing_nm : 0.12;
+
+    std::vector<int4> u;
+    std::vector<float4> prm;
+    std::vector<int> comp, cbm;
+    std::vector<int2> cinfo;
+    std::vector<uint32_t> clut;
+    std::vector<int2> bid, pid; std::vector<float2> bp, pp, ip;
+    std::vector<int4> aid, did, iid; std::vector<float4> ap, dp;
+    std::vector<std::vector<int>> incl;    // per global atom
+    std::vector<float> green;
+    std::vector<float2> twid;
+    int spec_total = 0;
+    B.meta.resize(nsys);
+    B.h_ncomp.resize(nsys);
+    B.h_cn.resize(nsys);
+    B.h_key.resize(nsys);
+    B.h_ntiny.resize(nsys);
+    for (int s = 0; s < nsys; s++) {
+        const lima_box_desc* b = &boxes[s];
+        SysMeta& m = B.meta[s];
+        std::memset(&m, 0, sizeof(m));
+        if (b->n_compounds < 0 || b->n_tinymols < 0) throw LimaFail(LIMA_ERR_BAD_INPUT, "negative counts");
+        const int nC = b->n_compounds, nT = b->n_tinymols;
+        m.atom0 = (int)u.size();
+        m.comp0 = (int)cinfo.size();
+        m.nc = nC + nT;     // tinymols become one-atom compounds
+        B.h_ncomp[s] = nC;
+        B.h_ntiny[s] = nT;
+        double L[3];
+        for (int d = 0; d < 3; d++) {
+            const long long Lq = llround(b->box_nm[d] * 1e8);
+            if (Lq <= 0 || Lq >= 1000000000ll) throw LimaFail(LIMA_ERR_BAD_INPUT, "batched systems need box edges in (0, 10) nm");
+            m.Lq[d] = (int)Lq;
+            m.hLq[d] = (int)(Lq / 2);
+            L[d] = (double)Lq / 1e8;
+            m.L[d] = (float)L[d];
+            if (L[d] <= (double)B.rc) throw LimaFail(LIMA_ERR_BAD_INPUT, "box edge must exceed the cutoff");
+            if (L[d] < 2.0 * (double)B.rc) m.multi = 1;
+        }
+        std::vector<int> first(nC + 1, 0);
+        for (int c = 0; c < nC; c++) {
+            const int n = b->compound_n_particles[c];
+            if (n < 0 || n > 32) throw LimaFail(LIMA_ERR_BAD_INPUT, "compound particle count out of [0, 32]");
+            first[c + 1] = first[c] + n;
+        }
+        m.n = first[nC] + nT;
+        if (m.n <= 0 || m.n > MAX_ATOMS) throw LimaFail(LIMA_ERR_BAD_INPUT, "batched systems hold 1 .. 8192 atoms");
+        B.h_cn[s].assign(b->compound_n_particles, b->compound_n_particles + nC);
+        B.h_key[s].resize(nC);
+        double qsq = 0.0;
+        int ncharged = 0;
+        std::vector<float> qv;
+        for (int c = 0; c < nC; c++) {
+            B.h_key[s][c] = b->compound_key_particle ? b->compound_key_particle[c] : 0;
+            for (int k = 0; k < b->compound_n_particles[c]; k++) {
+                const size_t ps = (size_t)c * 32 + k;
+                int64_t uu[3];
+                for (int d = 0; d < 3; d++) uu[d] = (int64_t)b->compound_origo[c * 3 + d] * 100000000ll + b->particle_coord[ps * 3 + d];
+                for (int d = 0; d < 3; d++)
+                    if (uu[d] < -1000000000ll || uu[d] > 1000000000ll) throw LimaFail(LIMA_ERR_BAD_INPUT, "batched positions must lie within 10 nm of the origin");
+                u.push_back(make_int4((int)uu[0], (int)uu[1], (int)uu[2], 0));
+                const float q = elec ? b->particle_charge[ps] : 0.f;
+                const float sg = b->particle_sigma[ps], ep = b->particle_epsilon[ps];
+                if (sg < 0.f || ep < 0.f) throw LimaFail(LIMA_ERR_BAD_INPUT, "negative LJ parameter");
+                prm.push_back(make_float4(q, 0.5f * sg, sqrtf(ep), sqrtf(24.f * ep)));
+                comp.push_back(c);
+                qsq += (double)q * q;
+                ncharged += q != 0.f;
+            }
+        }
+        for (int t = 0; t < nT; t++) {
+            int64_t uu[3];
+            for (int d = 0; d < 3; d++) uu[d] = (int64_t)b->tinymol_block[t * 3 + d] * 100000000ll + b->tinymol_coord[t * 3 + d];
+            u.push_back(make_int4((int)uu[0], (int)uu[1], (int)uu[2], 0));
+            const float sg = b->tinymol_sigma[t], ep = b->tinymol_epsilon[t];
+            prm.push_back(make_float4(0.f, 0.5f * sg, sqrtf(ep), sqrtf(24.f * ep)));
+            comp.push_back(nC + t);
+        }
+        // compounds: (first, count), bonded mask, exclusion rows (self bits on the diagonal block)
+        const uint32_t* lut = b->bonded_lut;
+        auto lutrow = [&](int c, int k, int i) -> uint32_t { return lut ? lut[((size_t)c * 5 + k) * 32 + i] : (k == 2 ? (1u << i) : 0u); };
+        for (int c = 0; c < nC + nT; c++) {
+            const bool tiny = c >= nC;
+            cinfo.push_back(make_int2(tiny ? first[nC] + (c - nC) : first[c], tiny ? 1 : b->compound_n_particles[c]));
+            int bm = 1 << 2;
+            for (int k = 0; k < 5; k++) {
+                const int Bc = c + k - 2;
+                for (int i = 0; i < 32; i++) {
+                    uint32_t r = 0;
+                    if (!tiny && Bc >= 0 && Bc < nC) r = lutrow(c, k, i);
+                    if (k == 2) r |= 1u << i;
+                    clut.push_back(r);
+                    if (k != 2 && r) bm |= 1 << k;
+                }
+            }
+            cbm.push_back(bm);
+        }
+        // bonded terms: ids compound*32 + slot -> global atom index; incidences per atom
+        incl.resize(u.size());
+        auto gid = [&](int32_t id) -> int {
+            const int c = id / 32, k = id % 32;
+            if (id < 0 || c >= nC || k >= b->compound_n_particles[c]) throw LimaFail(LIMA_ERR_BAD_INPUT, "bonded particle id out of range");
+            return m.atom0 + first[c] + k;
+        };
+        auto add_inc = [&](int atom, int type, int role, int term) {
+            if (term >= (1 << 27)) throw LimaFail(LIMA_ERR_BAD_INPUT, "too many bonded terms in the batch");
+            incl[atom].push_back((type << 29) | (role << 27) | term);
+        };
+        for (int k = 0; k < b->n_bonds; k++) {
+            const int t = (int)bid.size();
+            bid.push_back(make_int2(gid(b->bond_ids[2 * k]), gid(b->bond_ids[2 * k + 1])));
+            bp.push_back(make_float2(b->bond_params[2 * k], b->bond_params[2 * k + 1]));
+            add_inc(bid[t].x, T_BOND, 0, t); add_inc(bid[t].y, T_BOND, 1, t);
+        }
+        for (int k = 0; k < b->n_angles; k++) {
+            const int t = (int)aid.size();
+            aid.push_back(make_int4(gid(b->angle_ids[3 * k]), gid(b->angle_ids[3 * k + 1]), gid(b->angle_ids[3 * k + 2]), 0));
+            ap.push_back(make_float4(b->angle_params[4 * k], b->angle_params[4 * k + 1], b->angle_params[4 * k + 2], b->angle_params[4 * k + 3]));
+            add_inc(aid[t].x, T_ANGLE, 0, t); add_inc(aid[t].y, T_ANGLE, 1, t); add_inc(aid[t].z, T_ANGLE, 2, t);
+        }
+        for (int k = 0; k < b->n_dihedrals; k++) {
+            const int t = (int)did.size();
+            did.push_back(make_int4(gid(b->dihedral_ids[4 * k]), gid(b->dihedral_ids[4 * k + 1]), gid(b->dihedral_ids[4 * k + 2]),
+                                    gid(b->dihedral_ids[4 * k + 3])));
+            dp.push_back(make_float4(b->dihedral_params[3 * k], b->dihedral_params[3 * k + 1], b->dihedral_params[3 * k + 2], 0.f));
+            add_inc(did[t].x, T_DIH, 0, t); add_inc(did[t].y, T_DIH, 1, t); add_inc(did[t].z, T_DIH, 2, t); add_inc(did[t].w, T_DIH, 3, t);
+        }
+        for (int k = 0; k < b->n_impropers; k++) {
+            const int t = (int)iid.size();
+            iid.push_back(make_int4(gid(b->improper_ids[4 * k]), gid(b->improper_ids[4 * k + 1]), gid(b->improper_ids[4 * k + 2]),
+                                    gid(b->improper_ids[4 * k + 3])));
+            ip.push_back(make_float2(b->improper_params[2 * k], b->improper_params[2 * k + 1]));
+            add_inc(iid[t].x, T_IMP, 0, t); add_inc(iid[t].y, T_IMP, 1, t); add_inc(iid[t].z, T_IMP, 2, t); add_inc(iid[t].w, T_IMP, 3, t);
+        }
+        for (int k = 0; k < b->n_pairs; k++) {
+            const int t = (int)pid.size();
+            pid.push_back(make_int2(gid(b->pair_ids[2 * k]), gid(b->pair_ids[2 * k + 1])));
+            pp.push_back(make_float2(b->pair_params[2 * k], b->pair_params[2 * k + 1]));
+            add_inc(pid[t].x, T_PAIR, 0, t); add_inc(pid[t].y, T_PAIR, 1, t);
+        }
+        // PME grid, twiddles and LIMA's influence function (PME.cuh:434-495) in double
+        m.elec = (elec && ncharged > 0) ? 1 : 0;
+        for (int d = 0; d < 3; d++) {
+            m.K[d] = p->pme_grid[d] > 0 ? p->pme_grid[d] : derive_grid_b(L[d], spacing);
+            if (m.K[d] < 4 || m.K[d] > KMAX) throw LimaFail(LIMA_ERR_BAD_INPUT, "batched systems need PME grid edges in [4, 32]");
+            m.tw0[d] = (int)twid.size();
+            for (int k = 0; k < m.K[d]; k++) {
+                const double ang = -2.0 * M_PI * (double)k / (double)m.K[d];
+                twid.push_back(make_float2((float)std::cos(ang), (float)std::sin(ang)));
+            }
+            m.inv2h[d] = (float)(m.K[d] / (2.0 * L[d]));
+        }
+        m.H = m.K[0] / 2 + 1;
+        m.NG = m.K[0] * m.K[1] * m.K[2];
+        m.spec0 = spec_total;
+        const int nspec = m.H * m.K[1] * m.K[2];
+        spec_total += nspec;
+        const double vcell = (L[0] / m.K[0]) * (L[1] / m.K[1]) * (L[2] / m.K[2]);
+        m.fix_to_rho = (float)(1.0 / (FIX_SCALE * vcell));
+        m.inv_ng = (float)(1.0 / (double)m.NG);
+        {
+            const double PI_L = (double)LIMA_PI_F, kappa = (double)B.pc.kappa, kc = (double)B.pc.kc;
+            green.resize(spec_total);
+            for (int z = 0; z < m.K[2]; z++)
+                for (int y = 0; y < m.K[1]; y++)
+                    for (int x = 0; x < m.H; x++) {
+                        const int mi[3] = {x, y, z};
+                        double k2 = 0, S = 1;
+                        for (int d = 0; d < 3; d++) {
+                            const int half = m.K[d] / 2;
+                            const int mm = mi[d] <= half ? mi[d] : mi[d] - m.K[d];
+                            const double kd = 2.0 * PI_L * mm / L[d];
+                            k2 += kd * kd;
+                            const double kh = kd * (L[d] / m.K[d]) * 0.5;
+                            S *= (std::fabs(kh) < 1e-14) ? 1.0 : std::pow(std::sin(kh) / kh, 4);
+                        }
+                        double G = 0;
+                        if (k2 > 1e-14) G = 4.0 * PI_L / k2 * std::exp(-k2 / (4.0 * kappa * kappa)) * S * S * kc;
+                        green[m.spec0 + ((size_t)z * m.K[1] + y) * m.H + x] = (float)G;
+                    }
+            m.eself = -(double)B.pc.kappa / std::sqrt(PI_L) * qsq * kc;
+        }
+        if (!m.elec) m.eself = 0.0;
+        m.lds_bytes = m.elec ? m.NG * 4 : 0;
+    }
+    B.N = (int64_t)u.size();
+    // incidence CSR
+    std::vector<int> inc_off(B.N + 1, 0), inc;
+    for (int64_t i = 0; i < B.N; i++) {
+        inc_off[i + 1] = inc_off[i] + (int)incl[i].size();
+        inc.insert(inc.end(), incl[i].begin(), incl[i].end());
+    }
+    if (inc.empty()) inc.push_back(0);
+    B.u_init = u;
+    B.d_meta = dcopy(B.meta);
+    B.u0 = dcopy(u);
+    B.u_start = dcopy(u);
+    HIP_CHECK(hipMalloc(&B.u1, B.N * sizeof(int4)));
+    HIP_CHECK(hipMalloc(&B.f0, B.N * sizeof(float4)));
+    HIP_CHECK(hipMalloc(&B.f1, B.N * sizeof(float4)));
+    HIP_CHECK(hipMemset(B.f0, 0, B.N * sizeof(float4)));
+    B.prm4 = dcopy(prm);
+    B.comp = dcopy(comp);
+    B.cinfo = dcopy(cinfo);
+    B.cbm = dcopy(cbm);
+    B.clut = dcopy(clut);
+    B.inc_off = dcopy(inc_off);
+    B.inc = dcopy(inc);
+    B.bid = dcopy(bid); B.bp = dcopy(bp);
+    B.aid = dcopy(aid); B.ap = dcopy(ap);
+    B.did = dcopy(did); B.dp = dcopy(dp);
+    B.iid = dcopy(iid); B.ip = dcopy(ip);
+    B.pid = dcopy(pid); B.pp = dcopy(pp);
+    B.green = dcopy(green);
+    HIP_CHECK(hipMalloc(&B.spec, std::max(spec_total, 1) * sizeof(float2)));
+    B.twid = dcopy(twid);
+    HIP_CHECK(hipMalloc(&B.out, nsys * sizeof(SysOut)));
+    HIP_CHECK(hipMemset(B.out, 0, nsys * sizeof(SysOut)));
+    HIP_CHECK(hipMalloc(&B.d_order, nsys * sizeof(int)));
+    for (auto& st : B.streams) HIP_CHECK(hipStreamCreate(&st));
+    // default shapes until autotuned: 1024 threads, two atoms per thread per pass
+    B.tuning.n_buckets = NBUCKET;
+    for (int k = 0; k < NBUCKET; k++) {
+        B.tuning.bucket_max_atoms[k] = BUCKET_MAX[k];
+        B.tuning.threads[k] = 1024;
+        B.tuning.i_per_thread[k] = 2;
+        B.tuning.us_per_iter[k] = 0.0;
+    }
+}
+
+int shape_index(int threads, int rb) {
+    for (int i = 0; i < NSHAPES; i++) if (SHAPES[i].threads == threads && SHAPES[i].rb == rb) return i;
+    return -1;
+}
+
+int bucket_of(const lima_batch_tuning& t, int n) {
+    for (int k = 0; k < t.n_buckets; k++) if (n <= t.bucket_max_atoms[k]) return k;
+    return t.n_buckets - 1;
+}
+
+BatchArgs make_args(const Batch& B, double tol_j, int max_iter, double h0) {
+    BatchArgs a{};
+    a.meta = B.d_meta; a.order = B.d_order;
+    a.u0 = B.u0; a.u1 = B.u1; a.f0 = B.f0; a.f1 = B.f1;
+    a.prm = B.prm4; a.comp = B.comp; a.cinfo = B.cinfo; a.cbm = B.cbm; a.clut = B.clut;
+    a.inc_off = B.inc_off; a.inc = B.inc;
+    a.bid = B.bid; a.bp = B.bp; a.aid = B.aid; a.ap = B.ap; a.did = B.did; a.dp = B.dp; a.iid = B.iid; a.ip = B.ip;
+    a.pid = B.pid; a.pp = B.pp;
+    a.green = B.green; a.spec = B.spec; a.twid = B.twid; a.out = B.out;
+    a.pc = B.pc; a.rc = B.rc; a.rc2 = B.rc * B.rc;
+    a.tol = tol_j; a.h0 = h0; a.max_iter = max_iter;
+    return a;
+}
+
+// Launch the given systems, grouped by shape (one launch per shape, each on its own stream,
+// largest systems first), and wait for all of them.
+void launch_groups(Batch& B, const std::vector<int>& systems, const std::vector<int>& shape_of, const BatchArgs& a) {
+    std::vector<int> order;
+    std::vector<std::pair<int, int>> ranges(NSHAPES, {0, 0});
+    for (int sh = 0; sh < NSHAPES; sh++) {
+        std::vector<int> g;
+        for (int s : systems) if (shape_of[s] == sh) g.push_back(s);
+        std::stable_sort(g.begin(), g.end(), [&](int x, int y) { return B.meta[x].n > B.meta[y].n; });
+        ranges[sh] = {(int)order.size(), (int)g.size()};
+        order.insert(order.end(), g.begin(), g.end());
+    }
+    HIP_CHECK(hipMemcpy(B.d_order, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice));
+    for (int sh = 0; sh < NSHAPES; sh++) {
+        const int cnt = ranges[sh].second;
+        if (cnt == 0) continue;
+        int lds = SHAPES[sh].threads * 32;
+        for (int k = 0; k < cnt; k++) lds = std::max(lds, B.meta[order[ranges[sh].first + k]].lds_bytes);
+        BatchArgs ag = a;
+        ag.order = B.d_order + ranges[sh].first;
+        KernelFn fn = shape_kernel(sh);
+        HIP_CHECK(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+        hipLaunchKernelGGL(fn, dim3(cnt), dim3(SHAPES[sh].threads), (size_t)lds, B.streams[sh], ag);
+        HIP_CHECK(hipGetLastError());
+    }
+    for (int sh = 0; sh < NSHAPES; sh++) HIP_CHECK(hipStreamSynchronize(B.streams[sh]));
+}
+
+std::vector<int> shapes_from_tuning(const Batch& B) {
+    std::vector<int> shape_of(B.nsys);
+    for (int s = 0; s < B.nsys; s++) {
+        const int k = bucket_of(B.tuning, B.meta[s].n);
+        int sh = shape_index(B.tuning.threads[k], B.tuning.i_per_thread[k]);
+        if (sh < 0) sh = shape_index(1024, 2);
+        shape_of[s] = sh;
+    }
+    return shape_of;
+}
+
+}  // namespace
+}  // namespace lima
+
+using lima::Batch;
+using lima::LimaFail;
+using lima::HipError;
+
+struct lima_batch {
+    Batch b;
+};
+
+namespace {
+template <class F>
+lima_status bguard(F&& f) {
+    try {
+        f();
+        return LIMA_OK;
+    } catch (const LimaFail& x) {
+        lima::g_err = x.what();
+        return x.code;
+    } catch (const HipError& x) {
+        lima::g_err = x.what();
+        return LIMA_ERR_HIP;
+    } catch (const std::exception& x) {
+        lima::g_err = x.what();
+        return LIMA_ERR_BAD_INPUT;
+    }
+}
+}  // namespace
+
+extern "C" {
+
+lima_status lima_batch_create(const lima_box_desc* boxes, int32_t n_systems, const lima_simparams* params, lima_batch** out) {
+    if (!out) { lima::g_err = "null output handle"; return LIMA_ERR_BAD_INPUT; }
+    *out = nullptr;
+    lima_batch* h = new lima_batch();
+    const lima_status st = bguard([&] { lima::batch_create(h->b, boxes, n_systems, params); });
+    if (st != LIMA_OK) {
+        lima::batch_free(h->b);
+        delete h;
+        return st;
+    }
+    *out = h;
+    return LIMA_OK;
+}
+
+lima_status lima_batch_destroy(lima_batch* h) {
+    if (!h) return LIMA_OK;
+    lima::batch_free(h->b);
+    delete h;
+    return LIMA_OK;
+}
+
+lima_status lima_batch_autotune(lima_batch* h, int32_t trial_iterations, lima_batch_tuning* out) {
+    if (!h) { lima::g_err = "null batch"; return LIMA_ERR_BAD_INPUT; }
+    return bguard([&] {
+        Batch& B = h->b;
+        using namespace lima;
+        const int iters = std::max(1, (int)trial_iterations);
+        // per bucket: up to 8 systems spread over the bucket's size range
+        std::vector<std::vector<int>> members(NBUCKET);
+        for (int s = 0; s < B.nsys; s++) members[bucket_of(B.tuning, B.meta[s].n)].push_back(s);
+        std::vector<int> sample;
+        for (auto& g : members) {
+            std::sort(g.begin(), g.end(), [&](int x, int y) { return B.meta[x].n < B.meta[y].n; });
+            const int take = std::min<int>(8, (int)g.size());
+            for (int k = 0; k < take; k++) sample.push_back(g[(size_t)k * g.size() / take]);
+        }
+        std::vector<std::vector<double>> cost(NSHAPES, std::vector<double>(NBUCKET, 0.0));
+        std::vector<SysOut> res(B.nsys);
+        for (int sh = 0; sh < NSHAPES; sh++) {
+            std::vector<int> shape_of(B.nsys, sh);
+            HIP_CHECK(hipMemcpy(B.u0, B.u_start, B.N * sizeof(int4), hipMemcpyDeviceToDevice));
+            // tolerance 0: every system runs exactly `iters` trial steps
+            launch_groups(B, sample, shape_of, make_args(B, 0.0, iters, 0.01));
+            HIP_CHECK(hipMemcpy(res.data(), B.out, B.nsys * sizeof(SysOut), hipMemcpyDeviceToHost));
+            for (int s : sample) {
+                const double us = (double)(res[s].t1 - res[s].t0) * 0.01 / (double)(res[s].iterations + 1);
+                cost[sh][bucket_of(B.tuning, B.meta[s].n)] += us;
+            }
+        }
+        for (int k = 0; k < NBUCKET; k++) {
+            if (members[k].empty()) continue;
+            int best = shape_index(1024, 2);
+            for (int sh = 0; sh < NSHAPES; sh++) if (cost[sh][k] < cost[best][k]) best = sh;
+            const int take = std::min<int>(8, (int)members[k].size());
+            B.tuning.threads[k] = SHAPES[best].threads;
+            B.tuning.i_per_thread[k] = SHAPES[best].rb;
+            B.tuning.us_per_iter[k] = cost[best][k] / take;
+        }
+        HIP_CHECK(hipMemcpy(B.u0, B.u_start, B.N * sizeof(int4), hipMemcpyDeviceToDevice));
+        B.tuned = true;
+        B.forces_valid = false;
+        if (out) *out = B.tuning;
+    });
+}
+
+lima_status lima_batch_set_tuning(lima_batch* h, const lima_batch_tuning* t) {
+    if (!h || !t) { lima::g_err = "null argument"; return LIMA_ERR_BAD_INPUT; }
+    if (t->n_buckets < 1 || t->n_buckets > 8) { lima::g_err = "n_buckets out of [1, 8]"; return LIMA_ERR_BAD_INPUT; }
+    for (int k = 0; k < t->n_buckets; k++)
+        if (lima::shape_index(t->threads[k], t->i_per_thread[k]) < 0) {
+            lima::g_err = "unsupported shape (threads 256/512/1024, i_per_thread 1/2/4)";
+            return LIMA_ERR_BAD_INPUT;
+        }
+    h->b.tuning = *t;
+    h->b.tuned = true;
+    return LIMA_OK;
+}
+
+lima_status lima_batch_minimize(lima_batch* h, double fmax_tol_kj, int64_t max_iter, double initial_step_nm, lima_em_result* results,
+                                double* wall_seconds) {
+    if (!h) { lima::g_err = "null batch"; return LIMA_ERR_BAD_INPUT; }
+    if (max_iter < 0 || max_iter > (1ll << 30)) { lima::g_err = "max_iter out of range"; return LIMA_ERR_BAD_INPUT; }
+    return bguard([&] {
+        using namespace lima;
+        Batch& B = h->b;
+        std::vector<int> all(B.nsys);
+        for (int s = 0; s < B.nsys; s++) all[s] = s;
+        const std::vector<int> shape_of = shapes_from_tuning(B);
+        const auto t0 = std::chrono::steady_clock::now();
+        launch_groups(B, all, shape_of, make_args(B, fmax_tol_kj * 1000.0, (int)max_iter, initial_step_nm));
+        const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        std::vector<SysOut> res(B.nsys);
+        HIP_CHECK(hipMemcpy(res.data(), B.out, B.nsys * sizeof(SysOut), hipMemcpyDeviceToHost));
+        B.forces_valid = true;
+        bool bad = false;
+        for (int s = 0; s < B.nsys; s++) {
+            if (!std::isfinite(res[s].e_acc)) bad = true;
+            if (results) {
+                results[s].iterations = res[s].iterations;
+                results[s].accepted = res[s].accepted;
+                results[s].final_energy = res[s].e_acc;
+                results[s].final_fmax = (double)res[s].fmax / 1000.0;
+                results[s].converged = res[s].converged;
+                results[s].wall_seconds = (double)(res[s].t1 - res[s].t0) * 1e-8;
+            }
+        }
+        if (wall_seconds) *wall_seconds = wall;
+        if (bad) throw LimaFail(LIMA_ERR_NONFINITE, "non-finite energy in batched minimisation");
+    });
+}
+
+lima_status lima_batch_get_state(lima_batch* h, int32_t system, lima_state_out* out) {
+    if (!h || !out) { lima::g_err = "null argument"; return LIMA_ERR_BAD_INPUT; }
+    if (system < 0 || system >= h->b.nsys) { lima::g_err = "system index out of range"; return LIMA_ERR_BAD_INPUT; }
+    return bguard([&] {
+        using namespace lima;
+        Batch& B = h->b;
+        const SysMeta& m = B.meta[system];
+        std::vector<int4> u(m.n);
+        std::vector<float4> f(m.n);
+        HIP_CHECK(hipMemcpy(u.data(), B.u0 + m.atom0, m.n * sizeof(int4), hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(f.data(), B.f0 + m.atom0, m.n * sizeof(float4), hipMemcpyDeviceToHost));
+        if (!B.forces_valid) std::fill(f.begin(), f.end(), make_float4(0.f, 0.f, 0.f, 0.f));
+        const int nC = B.h_ncomp[system];
+        auto comp_u = [&](int i, int d) -> int64_t { return d == 0 ? u[i].x : (d == 1 ? u[i].y : u[i].z); };
+        int first = 0;
+        for (int c = 0; c < nC; c++) {
+            const int nc = B.h_cn[system][c];
+            const int k = first + B.h_key[system][c];
+            int64_t base[3];
+            for (int d = 0; d < 3; d++) {
+                const int64_t wk = comp_u(k, d) - fdiv64(comp_u(k, d), m.Lq[d]) * m.Lq[d];
+                const int32_t org = (int32_t)(wk / 100000000ll);
+                if (out->compound_origo) out->compound_origo[c * 3 + d] = org;
+                base[d] = comp_u(k, d) - wk + (int64_t)org * 100000000ll;
+            }
+            for (int sl = 0; sl < nc; sl++) {
+                const int i = first + sl;
+                const size_t ps = (size_t)c * 32 + sl;
+                const float ff[3] = {f[i].x, f[i].y, f[i].z};
+                for (int d = 0; d < 3; d++) {
+                    if (out->particle_coord) out->particle_coord[ps * 3 + d] = (int32_t)(comp_u(i, d) - base[d]);
+                    if (out->particle_velocity) out->particle_velocity[ps * 3 + d] = 0.f;
+                    if (out->particle_force) out->particle_force[ps * 3 + d] = ff[d];
+                }
+                if (out->particle_pot_energy) out->particle_pot_energy[ps] = 0.f;
+            }
+            first += nc;
+        }
+        for (int t = 0; t < B.h_ntiny[system]; t++) {
+            const int i = first + t;
+            for (int d = 0; d < 3; d++) {
+                const int64_t ww = comp_u(i, d) - fdiv64(comp_u(i, d), m.Lq[d]) * m.Lq[d];
+                const int32_t blk = (int32_t)(ww / 100000000ll);
+                if (out->tinymol_block) out->tinymol_block[t * 3 + d] = blk;
+                if (out->tinymol_coord) out->tinymol_coord[t * 3 + d] = (int32_t)(ww - (int64_t)blk * 100000000ll);
+                if (out->tinymol_velocity) out->tinymol_velocity[t * 3 + d] = 0.f;
+                if (out->tinymol_force) out->tinymol_force[t * 3 + d] = (&f[i].x)[d];
+            }
+            if (out->tinymol_pot_energy) out->tinymol_pot_energy[t] = 0.f;
+        }
+    });
+}
+
+lima_status lima_batch_reset(lima_batch* h) {
+    if (!h) { lima::g_err = "null batch"; return LIMA_ERR_BAD_INPUT; }
+    return bguard([&] {
+        Batch& B = h->b;
+        HIP_CHECK(hipMemcpy(B.u0, B.u_start, B.N * sizeof(int4), hipMemcpyDeviceToDevice));
+        B.forces_valid = false;
+    });
+}
+
+lima_status lima_batch_get_tuning(lima_batch* h, lima_batch_tuning* out) {
+    if (!h || !out) { lima::g_err = "null argument"; return LIMA_ERR_BAD_INPUT; }
+    *out = h->b.tuning;
+    return LIMA_OK;
+}
+
+}  // extern "C"

@@ -181,6 +181,8 @@ struct Engine {
         if (e_ != hipSuccess) throw HipError(std::string(#x) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+extern thread_local std::string g_err;   // lima_last_error() text, set by every failing C-ABI call
+
 struct HipError : std::runtime_error { using std::runtime_error::runtime_error; };
 struct LimaFail : std::runtime_error {
     lima_status code;

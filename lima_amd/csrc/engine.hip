@@ -21,9 +21,11 @@ struct lima_engine {
     Engine e;
 };
 
-namespace {
+namespace lima {
+thread_local std::string g_err;   // message of the last failing C-ABI call on this thread
+}
 
-thread_local std::string g_err;
+namespace {
 
 inline int nblk(int64_t n, int b) { return (int)((n + b - 1) / b); }
 

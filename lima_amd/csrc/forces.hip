@@ -15,6 +15,7 @@
 // Bonded kernel: ForceComputations.cuh:17-264,335-376; one thread per term, forces added
 // to the particles' cluster slots with float atomics.
 #include "engine.h"
+#include "lima_terms.h"
 
 namespace lima {
 
@@ -276,18 +277,6 @@ struct BondArgs {
     double* epart;
 };
 
-struct v3 {
-    float x, y, z;
-};
-__device__ __forceinline__ v3 operator+(v3 a, v3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
-__device__ __forceinline__ v3 operator-(v3 a, v3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
-__device__ __forceinline__ v3 operator*(v3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
-__device__ __forceinline__ v3 operator-(v3 a) { return {-a.x, -a.y, -a.z}; }
-__device__ __forceinline__ float dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ v3 cross(v3 a, v3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
-__device__ __forceinline__ float len(v3 a) { return sqrtf(dot(a, a)); }
-__device__ __forceinline__ v3 nrm(v3 a) { return a * (1.0f / len(a)); }
-
 __device__ __forceinline__ v3 bdisp(const BondArgs& a, int from, int to) {   // unwrapped x_to - x_from [nm]
     long long u0[3], u1[3];
     unwrapped3(a.w, a.img, a.Lq, from, u0);
@@ -311,36 +300,16 @@ __global__ __launch_bounds__(256) void k_bonded(BondArgs a) {
     int nat = 0;
     if (t < a.nb) {   // calcSinglebondForces, ForceComputations.cuh:17-51
         const int2 id = a.bid[t];
-        const float2 p = a.bp[t];
-        const v3 d = bdisp(a, id.y, id.x);   // pos_a - pos_b
-        const float r = len(d), err = r - p.x;
-        const v3 f = d * (-p.y * err / r);
+        v3 f;
+        e = term_bond(bdisp(a, id.y, id.x), a.bp[t], &f);
         addf(a, id.x, f);
         addf(a, id.y, -f);
-        e = 0.5f * p.y * err * err;
         ids[0] = id.x; ids[1] = id.y; nat = 2;
     } else if (t < a.nb + a.na) {   // calcAnglebondForces :53-115, harmonic in theta + Urey-Bradley
         const int k = t - a.nb;
         const int4 id = a.aid[k];
-        const float4 p = a.ap[k];
-        const v3 r1 = bdisp(a, id.y, id.x), r2 = bdisp(a, id.y, id.z);
-        const float l1 = len(r1), l2 = len(r2);
-        const v3 v1 = r1 * (1.f / l1), v2 = r2 * (1.f / l2);
-        const float c = fminf(1.f, fmaxf(-1.f, dot(v1, v2)));
-        const float th = acosf(c), err = th - p.x;
-        const float s = fmaxf(sqrtf(fmaxf(0.f, 1.f - c * c)), 1e-6f);
-        const float tq = p.y * err;
-        v3 fl = (v2 - v1 * c) * (tq / (l1 * s));
-        v3 fr = (v1 - v2 * c) * (tq / (l2 * s));
-        e = 0.5f * p.y * err * err;
-        if (p.w != 0.f) {
-            const v3 d13 = bdisp(a, id.z, id.x);   // left - right
-            const float r13 = len(d13), eu = r13 - p.z;
-            const v3 fu = d13 * (-p.w * eu / r13);
-            fl = fl + fu;
-            fr = fr - fu;
-            e += 0.5f * p.w * eu * eu;
-        }
+        v3 fl, fr;
+        e = term_angle(bdisp(a, id.y, id.x), bdisp(a, id.y, id.z), bdisp(a, id.z, id.x), a.ap[k], &fl, &fr);
         addf(a, id.x, fl);
         addf(a, id.z, fr);
         addf(a, id.y, -(fl + fr));
@@ -348,59 +317,30 @@ __global__ __launch_bounds__(256) void k_bonded(BondArgs a) {
     } else if (t < a.nb + a.na + a.nd) {   // calcDihedralbondForces :119-210, phi_L = -atan2(sin, cos)
         const int k = t - a.nb - a.na;
         const int4 id = a.did[k];
-        const float4 p = a.dp[k];
-        const v3 b1 = bdisp(a, id.x, id.y), b2 = bdisp(a, id.y, id.z), b3 = bdisp(a, id.z, id.w);
-        const v3 m = cross(b1, b2), n = cross(b2, b3);
-        const float lb2 = len(b2);
-        const float phi = -atan2f(lb2 * dot(b1, n), dot(m, n));
-        const float arg = p.z * phi - p.x;
-        e = p.y * (1.f + cosf(arg));
-        const float dv = -p.y * p.z * sinf(arg);   // dV/dphi_L
-        const v3 g1 = m * (-lb2 / dot(m, m));
-        const v3 g4 = n * (lb2 / dot(n, n));
-        const float pp = dot(b1, b2) / (lb2 * lb2), qq = dot(b3, b2) / (lb2 * lb2);
-        const v3 g2 = g1 * (-(pp + 1.f)) + g4 * qq;
-        const v3 g3 = g1 * pp - g4 * (qq + 1.f);
-        addf(a, id.x, g1 * dv);
-        addf(a, id.y, g2 * dv);
-        addf(a, id.z, g3 * dv);
-        addf(a, id.w, g4 * dv);
+        v3 f[4];
+        e = term_dihedral(bdisp(a, id.x, id.y), bdisp(a, id.y, id.z), bdisp(a, id.z, id.w), a.dp[k], f);
+        addf(a, id.x, f[0]);
+        addf(a, id.y, f[1]);
+        addf(a, id.z, f[2]);
+        addf(a, id.w, f[3]);
         ids[0] = id.x; ids[1] = id.y; ids[2] = id.z; ids[3] = id.w; nat = 4;
     } else if (t < a.nb + a.na + a.nd + a.ni) {   // calcImproperdihedralbondForces :214-264
         const int k = t - a.nb - a.na - a.nd;
         const int4 id = a.iid[k];
-        const float2 p = a.ip[k];
-        const v3 pi = {0.f, 0.f, 0.f}, pj = bdisp(a, id.x, id.y), pk = bdisp(a, id.x, id.z), pl = bdisp(a, id.x, id.w);
-        const v3 ij = nrm(pj - pi), ik = nrm(pk - pi), il = nrm(pl - pi), lj = nrm(pj - pl), lk = nrm(pk - pl);
-        const v3 n1 = nrm(cross(ij, ik)), n2 = nrm(cross(lj, lk));
-        float ang = acosf(fminf(1.f, fmaxf(-1.f, dot(n1, n2))));
-        if (dot(n1, il) > 0.f) ang = -ang;
-        const float err = ang - p.x;
-        e = 0.5f * p.y * err * err;
-        const float tq = p.y * err;
-        const float dl = len(cross(pk - pj, pj - pl)) / len(pk - pj);
-        const float di = len(cross(pk - pj, pj - pi)) / len(pk - pj);
-        const v3 fl = n2 * (tq / dl);
-        const v3 fi = -n1 * (tq / di);
-        const v3 res = -(fi + fl);
-        const float ratio = len(pj - pi) / (len(pj - pi) + len(pk - pi));
-        addf(a, id.x, fi);
-        addf(a, id.y, res * (1.f - ratio));
-        addf(a, id.z, res * ratio);
-        addf(a, id.w, fl);
+        v3 f[4];
+        e = term_improper(bdisp(a, id.x, id.y), bdisp(a, id.x, id.z), bdisp(a, id.x, id.w), a.ip[k], f);
+        addf(a, id.x, f[0]);
+        addf(a, id.y, f[1]);
+        addf(a, id.z, f[2]);
+        addf(a, id.w, f[3]);
         ids[0] = id.x; ids[1] = id.y; ids[2] = id.z; ids[3] = id.w; nat = 4;
     } else if (t < a.nb + a.na + a.nd + a.ni + a.np) {   // computePairbondForces :335-376
         const int k = t - a.nb - a.na - a.nd - a.ni;
         const int2 id = a.pid[k];
-        const float2 p = a.pp[k];
-        const v3 d = bdisp(a, id.x, id.y);
-        const float r2 = dot(d, d), rinv2 = 1.f / r2;
-        float s = p.x * p.x * rinv2;
-        s = s * s * s;
-        const float S = 24.f * p.y * s * rinv2 * (1.f - 2.f * s);
-        addf(a, id.x, d * S);
-        addf(a, id.y, -(d * S));
-        e = 4.f * p.y * s * (s - 1.f);
+        v3 f;
+        e = term_pair14(bdisp(a, id.x, id.y), a.pp[k], &f);
+        addf(a, id.x, f);
+        addf(a, id.y, -f);
         ids[0] = id.x; ids[1] = id.y; nat = 2;
     }
     if (ENERGY) {

@@ -1,0 +1,145 @@
+// lima_terms.h -- LIMA's bonded-term and PME-stencil arithmetic as device functions, shared by the
+// per-step engine kernels and the batched minimiser so both evaluate identical formulas.
+#pragma once
+
+#include "lima_dev.h"
+
+namespace lima {
+
+struct v3 {
+    float x, y, z;
+};
+__device__ __forceinline__ v3 operator+(v3 a, v3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ v3 operator-(v3 a, v3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ v3 operator*(v3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ v3 operator-(v3 a) { return {-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ float dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ v3 cross(v3 a, v3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+__device__ __forceinline__ float len(v3 a) { return sqrtf(dot(a, a)); }
+__device__ __forceinline__ v3 nrm(v3 a) { return a * (1.0f / len(a)); }
+
+// ---- bonded terms (ForceComputations.cuh).  Each returns the term energy [J/mol] and the
+// forces [J/mol/nm] on its atoms; arguments are displacements in nm. ----
+
+// calcSinglebondForces, ForceComputations.cuh:17-51.  d = x_a - x_b; p = (b0, kb).
+// Force on a is *fa, on b -*fa.
+__device__ __forceinline__ float term_bond(v3 d, float2 p, v3* fa) {
+    const float r = len(d), err = r - p.x;
+    *fa = d * (-p.y * err / r);
+    return 0.5f * p.y * err * err;
+}
+
+// calcAnglebondForces, ForceComputations.cuh:53-115: harmonic in theta plus Urey-Bradley on r13.
+// r1 = x_l - x_m, r2 = x_r - x_m, d13 = x_l - x_r; p = (theta0, ktheta, ub0, kub).
+// Forces on l and r; the middle atom takes -(fl + fr).
+__device__ __forceinline__ float term_angle(v3 r1, v3 r2, v3 d13, float4 p, v3* fl, v3* fr) {
+    const float l1 = len(r1), l2 = len(r2);
+    const v3 v1 = r1 * (1.f / l1), v2 = r2 * (1.f / l2);
+    const float c = fminf(1.f, fmaxf(-1.f, dot(v1, v2)));
+    const float th = acosf(c), err = th - p.x;
+    const float s = fmaxf(sqrtf(fmaxf(0.f, 1.f - c * c)), 1e-6f);
+    const float tq = p.y * err;
+    v3 a = (v2 - v1 * c) * (tq / (l1 * s));
+    v3 b = (v1 - v2 * c) * (tq / (l2 * s));
+    float e = 0.5f * p.y * err * err;
+    if (p.w != 0.f) {
+        const float r13 = len(d13), eu = r13 - p.z;
+        const v3 fu = d13 * (-p.w * eu / r13);
+        a = a + fu;
+        b = b - fu;
+        e += 0.5f * p.w * eu * eu;
+    }
+    *fl = a;
+    *fr = b;
+    return e;
+}
+
+// calcDihedralbondForces, ForceComputations.cuh:119-210: V = k (1 + cos(n phi_L - phi0)) with
+// phi_L = -atan2(sin, cos), as the exact gradient of that energy.  b1 = x1 - x0, b2 = x2 - x1,
+// b3 = x3 - x2; p = (phi0, k, n).
+__device__ __forceinline__ float term_dihedral(v3 b1, v3 b2, v3 b3, float4 p, v3* f) {
+    const v3 m = cross(b1, b2), n = cross(b2, b3);
+    const float lb2 = len(b2);
+    const float phi = -atan2f(lb2 * dot(b1, n), dot(m, n));
+    const float arg = p.z * phi - p.x;
+    const float dv = -p.y * p.z * sinf(arg);   // dV/dphi_L
+    const v3 g1 = m * (-lb2 / dot(m, m));
+    const v3 g4 = n * (lb2 / dot(n, n));
+    const float pp = dot(b1, b2) / (lb2 * lb2), qq = dot(b3, b2) / (lb2 * lb2);
+    const v3 g2 = g1 * (-(pp + 1.f)) + g4 * qq;
+    const v3 g3 = g1 * pp - g4 * (qq + 1.f);
+    f[0] = g1 * dv;
+    f[1] = g2 * dv;
+    f[2] = g3 * dv;
+    f[3] = g4 * dv;
+    return p.y * (1.f + cosf(arg));
+}
+
+// calcImproperdihedralbondForces, ForceComputations.cuh:214-264 (LIMA's force split, not a
+// gradient).  pj, pk, pl are x_j - x_i, x_k - x_i, x_l - x_i; p = (psi0, kpsi).
+__device__ __forceinline__ float term_improper(v3 pj, v3 pk, v3 pl, float2 p, v3* f) {
+    const v3 pi = {0.f, 0.f, 0.f};
+    const v3 ij = nrm(pj - pi), ik = nrm(pk - pi), il = nrm(pl - pi), lj = nrm(pj - pl), lk = nrm(pk - pl);
+    const v3 n1 = nrm(cross(ij, ik)), n2 = nrm(cross(lj, lk));
+    float ang = acosf(fminf(1.f, fmaxf(-1.f, dot(n1, n2))));
+    if (dot(n1, il) > 0.f) ang = -ang;
+    const float err = ang - p.x;
+    const float tq = p.y * err;
+    const float dl = len(cross(pk - pj, pj - pl)) / len(pk - pj);
+    const float di = len(cross(pk - pj, pj - pi)) / len(pk - pj);
+    const v3 fl = n2 * (tq / dl);
+    const v3 fi = -n1 * (tq / di);
+    const v3 res = -(fi + fl);
+    const float ratio = len(pj - pi) / (len(pj - pi) + len(pk - pi));
+    f[0] = fi;
+    f[1] = res * (1.f - ratio);
+    f[2] = res * ratio;
+    f[3] = fl;
+    return 0.5f * p.y * err * err;
+}
+
+// computePairbondForces, ForceComputations.cuh:335-376: 1-4 LJ with the pair's own sigma / eps,
+// no Coulomb.  d = x_b - x_a; p = (sigma, eps).  Force on a is *fa, on b -*fa.
+__device__ __forceinline__ float term_pair14(v3 d, float2 p, v3* fa) {
+    const float r2 = dot(d, d), rinv2 = 1.f / r2;
+    float s = p.x * p.x * rinv2;
+    s = s * s * s;
+    *fa = d * (24.f * p.y * s * rinv2 * (1.f - 2.f * s));
+    return 4.f * p.y * s * (s - 1.f);
+}
+
+// ---- PME stencil (PME.cuh:263-300, DeviceAlgorithmsPrivate.cuh:7-13) ----
+
+// Grid coordinate of a wrapped integer Coord w in [0, Lq): base node ix = floor(w K / Lq) and
+// fraction f, exact in integers.
+__device__ __forceinline__ void grid_coord(uint32_t w, long long Lq, int K, int* ix, float* f) {
+    const unsigned long long num = (unsigned long long)w * (unsigned long long)K;
+    const unsigned long long q = num / (unsigned long long)Lq;
+    *ix = (int)q;
+    *f = (float)(num - q * (unsigned long long)Lq) / (float)Lq;
+}
+
+// Cubic B-spline weights at nodes ix-1 .. ix+2 (DeviceAlgorithmsPrivate.cuh:9-12).
+__device__ __forceinline__ void bspline(float f, float* w) {
+    const float g = 1.f - f;
+    w[0] = g * g * g * (1.f / 6.f);
+    w[1] = (4.f - 6.f * f * f + 3.f * f * f * f) * (1.f / 6.f);
+    w[2] = (1.f + 3.f * f + 3.f * f * f - 3.f * f * f * f) * (1.f / 6.f);
+    w[3] = f * f * f * (1.f / 6.f);
+}
+
+__device__ __forceinline__ int wrapi(int v, int K) { return v < 0 ? v + K : (v >= K ? v - K : v); }
+
+// Gather weights over the 6 nodes ix-2 .. ix+3 of one axis: w6 interpolates phi, d6 is the
+// central-difference stencil of the field (PME.cuh:374-411) folded onto the same nodes:
+// -sum_n w(n) (phi(n+1) - phi(n-1)) = sum_m phi(m) (w(m+1) - w(m-1)).
+__device__ __forceinline__ void gather_weights(const float* w, float* w6, float* d6) {
+    for (int m = 0; m < 6; m++) {
+        const int k = m - 1;
+        w6[m] = (k >= 0 && k < 4) ? w[k] : 0.f;
+        const int lo = m - 2, hi = m;
+        d6[m] = ((hi >= 0 && hi < 4) ? w[hi] : 0.f) - ((lo >= 0 && lo < 4) ? w[lo] : 0.f);
+    }
+}
+
+}  // namespace lima

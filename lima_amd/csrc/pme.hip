@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "engine.h"
+#include "lima_terms.h"
 
 namespace lima {
 
@@ -133,23 +134,6 @@ struct SpreadArgs {
     const float4* prm;
     long long* qgrid;
 };
-
-__device__ __forceinline__ void grid_coord(uint32_t w, long long Lq, int K, int* ix, float* f) {
-    const unsigned long long num = (unsigned long long)w * (unsigned long long)K;
-    const unsigned long long q = num / (unsigned long long)Lq;
-    *ix = (int)q;
-    *f = (float)(num - q * (unsigned long long)Lq) / (float)Lq;
-}
-
-__device__ __forceinline__ void bspline(float f, float* w) {   // DeviceAlgorithmsPrivate.cuh:9-12
-    const float g = 1.f - f;
-    w[0] = g * g * g * (1.f / 6.f);
-    w[1] = (4.f - 6.f * f * f + 3.f * f * f * f) * (1.f / 6.f);
-    w[2] = (1.f + 3.f * f + 3.f * f * f - 3.f * f * f * f) * (1.f / 6.f);
-    w[3] = f * f * f * (1.f / 6.f);
-}
-
-__device__ __forceinline__ int wrapi(int v, int K) { return v < 0 ? v + K : (v >= K ? v - K : v); }
 
 __global__ void k_spread(SpreadArgs a) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -412,19 +396,11 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
             grid_coord(pw.y, a.Lq[1], a.K[1], &ix[1], &f[1]);
             grid_coord(pw.z, a.Lq[2], a.K[2], &ix[2], &f[2]);
             bspline(f[0], wx); bspline(f[1], wy); bspline(f[2], wz);
-            // 6-point weights over nodes ix-2..ix+3: w4 (phi), dw (difference stencil)
+            // 6-point weights over nodes ix-2..ix+3: w6 (phi), d6 (difference stencil)
             float w6x[6], w6y[6], w6z[6], d6x[6], d6y[6], d6z[6];
-            for (int m = 0; m < 6; m++) {
-                const int k = m - 1;   // index into w for node ix-2+m
-                w6x[m] = (k >= 0 && k < 4) ? wx[k] : 0.f;
-                w6y[m] = (k >= 0 && k < 4) ? wy[k] : 0.f;
-                w6z[m] = (k >= 0 && k < 4) ? wz[k] : 0.f;
-                // E = (1/2h) sum_m phi(m) (w(m+1) - w(m-1)); node m+1 is w index m, node m-1 is m-2
-                const int lo = m - 2, hi = m;
-                d6x[m] = ((hi >= 0 && hi < 4) ? wx[hi] : 0.f) - ((lo >= 0 && lo < 4) ? wx[lo] : 0.f);
-                d6y[m] = ((hi >= 0 && hi < 4) ? wy[hi] : 0.f) - ((lo >= 0 && lo < 4) ? wy[lo] : 0.f);
-                d6z[m] = ((hi >= 0 && hi < 4) ? wz[hi] : 0.f) - ((lo >= 0 && lo < 4) ? wz[lo] : 0.f);
-            }
+            gather_weights(wx, w6x, d6x);
+            gather_weights(wy, w6y, d6y);
+            gather_weights(wz, w6z, d6z);
             float ph = 0.f, ex = 0.f, ey = 0.f, ez = 0.f;
             for (int mz = 0; mz < 6; mz++) {
                 const int Z = wrapi(wrapi(ix[2] - 2 + mz, a.K[2]), a.K[2]);
