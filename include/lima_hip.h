@@ -229,13 +229,13 @@ int64_t lima_get_pair_count(lima_engine* engine);
 typedef struct lima_batch lima_batch;
 
 /* Workgroup shape per system-size bucket: systems with n atoms use the first bucket with
- * n <= bucket_max_atoms.  threads: 256, 512 or 1024; i_per_thread: atoms a thread holds in
- * registers per pass over the j tiles (1, 2 or 4).  us_per_iter: measured by the autotuner. */
+ * n <= bucket_max_atoms.  threads: 256, 512 or 1024; pair_unroll: unroll factor of the pair loop
+ * (1, 2 or 4).  us_per_iter: cost of one SD iteration measured by the autotuner. */
 typedef struct lima_batch_tuning {
     int32_t n_buckets;
     int32_t bucket_max_atoms[8];
     int32_t threads[8];
-    int32_t i_per_thread[8];
+    int32_t pair_unroll[8];
     double  us_per_iter[8];
 } lima_batch_tuning;
 
@@ -256,6 +256,10 @@ lima_status lima_batch_minimize(lima_batch* batch, double fmax_tol_kj, int64_t m
 lima_status lima_batch_get_state(lima_batch* batch, int32_t system, lima_state_out* out);
 /* Restores the positions every system had at lima_batch_create. */
 lima_status lima_batch_reset(lima_batch* batch);
+/* Time each system spent in the last lima_batch_minimize, by phase, as seen by its workgroup's
+ * first thread: seconds[n_systems][5] = cutoff tile pairs, per-atom sums + intra pairs + bonded
+ * terms, PME, reductions, trial steps. */
+lima_status lima_batch_get_phase_times(lima_batch* batch, double* seconds);
 
 #ifdef __cplusplus
 }

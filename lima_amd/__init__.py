@@ -89,6 +89,7 @@ def load_hip():
                                     C.c_int),
             "lima_batch_get_state": ([vp, C.c_int32, C.POINTER(StateOut)], C.c_int),
             "lima_batch_reset": ([vp], C.c_int),
+            "lima_batch_get_phase_times": ([vp, C.POINTER(C.c_double)], C.c_int),
         }
         for name, (args, res) in sig.items():
             fn = getattr(lib, name)
@@ -355,6 +356,13 @@ class Batch:
 
     def reset(self):
         self._check(self.lib.lima_batch_reset(self._h))
+
+    def phase_times(self):
+        """[n_systems, 5] seconds: cutoff tile pairs, per-atom sums + intra + bonded, PME, reductions,
+        trial steps."""
+        out = np.zeros((len(self.boxes), 5))
+        self._check(self.lib.lima_batch_get_phase_times(self._h, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
 
 
 class StateBox:

@@ -120,10 +120,10 @@ class BatchTuning(C.Structure):
         ("n_buckets", C.c_int32),
         ("bucket_max_atoms", C.c_int32 * 8),
         ("threads", C.c_int32 * 8),
-        ("i_per_thread", C.c_int32 * 8),
+        ("pair_unroll", C.c_int32 * 8),
         ("us_per_iter", C.c_double * 8),
     ]
 
     def as_list(self):
-        return [{"max_atoms": self.bucket_max_atoms[k], "threads": self.threads[k], "i_per_thread": self.i_per_thread[k],
+        return [{"max_atoms": self.bucket_max_atoms[k], "threads": self.threads[k], "pair_unroll": self.pair_unroll[k],
                  "us_per_iter": round(self.us_per_iter[k], 2)} for k in range(self.n_buckets)]

@@ -11,9 +11,10 @@
 //       accept (x, F, E <- x', F', E'; h *= 1.2) if E' < E in FP64, else h *= 0.2
 //
 // Force evaluation inside the workgroup, with no inter-workgroup synchronisation:
-//   - nonbonded cutoff pairs (LennardJonesInteractions.cuh:74-203): all pairs i, j of the system
-//     with j-tiles staged in LDS (broadcast reads); boxes are 1.5-3.5 nm against a 1.2 nm
-//     cutoff, so the cutoff sphere covers 25-100 % of the box and a cell list would not prune.
+//   - nonbonded cutoff pairs (LennardJonesInteractions.cuh:74-203): every pair of the system once,
+//     as pairs of 64-atom tiles with the j tile rotating across the wave; boxes are 1.5-3.5 nm
+//     against a 1.2 nm cutoff, so the cutoff sphere covers 25-100 % of the box and a cell list
+//     would not prune.
 //     Every periodic image within the cutoff counts (two candidates per axis when an edge is
 //     shorter than 2 rc).  Pairs inside one compound or between bonded compounds are skipped
 //     here and computed by the intra pass.
@@ -25,8 +26,8 @@
 //     half spectrum, LIMA's influence function, finite-difference gather from an LDS potential.
 //   Energies are reduced in FP64 in a fixed order; the run is deterministic.
 //
-// Block shapes (threads per workgroup, i atoms held in registers per pass) come from
-// lima_batch_autotune, which times every candidate shape on the batch's own systems.
+// Block shapes (threads per workgroup, unroll of the pair loop) come from lima_batch_autotune,
+// which times every candidate shape on the batch's own systems.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -54,6 +55,7 @@ struct SysMeta {
     float L[3];
     int K[3], H, NG;
     int spec0;          // offset of this system's half spectrum and influence function
+    int part0;          // offset (float4) of the tile-pair partial forces: [NT(NT+1)/2][2][64]
     int tw0[3];         // twiddle table offsets per axis
     float fix_to_rho;   // fixed point -> charge density: 1 / (2^20 vcell)
     float inv_ng;
@@ -71,6 +73,7 @@ struct SysOut {
     float h;
     int iterations, accepted, converged, cur;
     unsigned long long t0, t1;   // s_memrealtime (100 MHz) at start / end
+    unsigned long long ph[5];    // ticks in tile pairs, per-atom sums + intra + bonded, PME, reductions, trial steps
 };
 
 struct BatchArgs {
@@ -92,6 +95,7 @@ struct BatchArgs {
     const int2* pid; const float2* pp;
     const float* green;
     float2* spec;
+    float4* part;
     const float2* twid;
     SysOut* out;
     PairConst pc;
@@ -114,34 +118,42 @@ __device__ __forceinline__ v3 udisp(const int4* u, int a, int b) {   // x_b - x_
             (float)(ub.z - ua.z) * LIMA_LIMA_TO_NANO_F};
 }
 
-// One cutoff pair evaluated from i's side: force on i += d S, half the pair energy on i.
-__device__ __forceinline__ void pair_eval(float x, float y, float z, float r2, float4 pi, float4 pj, const PairConst& pc,
-                                          v3& acc, float& elj, float& ec) {
+// One cutoff pair computed once: force on i += d S, on j -= d S, the whole pair energy.
+// pj = (q, sigma/2, sqrt(24 eps)) of j.
+__device__ __forceinline__ void pair_eval(float x, float y, float z, float r2, const float4& pi, float qj, float sj, float ej,
+                                          const PairConst& pc, v3& fi, v3& fj, float& elj, float& ec) {
     float e1, e2;
-    const float S = pair_scalar_cut(r2, pi.y + pj.y, pi.w * pj.w, pi.x * pj.x, pc, &e1, &e2);
-    acc.x += x * S;
-    acc.y += y * S;
-    acc.z += z * S;
-    elj += 0.5f * e1;
-    ec += 0.5f * e2;
+    const float S = pair_scalar_cut(r2, pi.y + sj, pi.w * ej, pi.x * qj, pc, &e1, &e2);
+    // explicit FMAs: the same rounding whatever the unrolling or scheduling
+    fi.x = fmaf(x, S, fi.x); fi.y = fmaf(y, S, fi.y); fi.z = fmaf(z, S, fi.z);
+    fj.x = fmaf(-x, S, fj.x); fj.y = fmaf(-y, S, fj.y); fj.z = fmaf(-z, S, fj.z);
+    elj += e1;
+    ec += e2;
 }
 
+// Box constants of one system held in registers for the pair loop.
+struct PairBox {
+    int Lx, Ly, Lz, hx, hy, hz;
+    float fx, fy, fz;
+};
+
+// Cutoff pair (i, j) over every periodic image within the cutoff.  `excl` marks a pair of one
+// compound or of bonded compounds (left to the intra pass).  One predicate guards the pair
+// arithmetic, so a wave branches once per step.
 template <bool MULTI>
-__device__ __forceinline__ void pair_cut(const int4& wi, const float4& pi, int ci, int bmi, const int4& wj, const float4& pj,
-                                         const SysMeta& m, const BatchArgs& a, v3& acc, float& elj, float& ec) {
-    const int dc = wj.w - ci;
-    if ((unsigned)(dc + 2) <= 4u && ((bmi >> (dc + 2)) & 1)) return;   // same or bonded compound: intra pass
-    int dx = wj.x - wi.x, dy = wj.y - wi.y, dz = wj.z - wi.z;
-    dx += dx < -m.hLq[0] ? m.Lq[0] : 0; dx -= dx >= m.hLq[0] ? m.Lq[0] : 0;
-    dy += dy < -m.hLq[1] ? m.Lq[1] : 0; dy -= dy >= m.hLq[1] ? m.Lq[1] : 0;
-    dz += dz < -m.hLq[2] ? m.Lq[2] : 0; dz -= dz >= m.hLq[2] ? m.Lq[2] : 0;
+__device__ __forceinline__ void pair_cut(const int4& wi, const float4& pi, bool excl, int jx, int jy, int jz, float qj, float sj,
+                                         float ej, const PairBox& bx, const BatchArgs& a, v3& fi, v3& fj, float& elj, float& ec) {
+    int dx = jx - wi.x, dy = jy - wi.y, dz = jz - wi.z;
+    dx += dx < -bx.hx ? bx.Lx : 0; dx -= dx >= bx.hx ? bx.Lx : 0;
+    dy += dy < -bx.hy ? bx.Ly : 0; dy -= dy >= bx.hy ? bx.Ly : 0;
+    dz += dz < -bx.hz ? bx.Lz : 0; dz -= dz >= bx.hz ? bx.Lz : 0;
     const float x = (float)dx * LIMA_LIMA_TO_NANO_F, y = (float)dy * LIMA_LIMA_TO_NANO_F, z = (float)dz * LIMA_LIMA_TO_NANO_F;
     if (!MULTI) {
         const float r2 = x * x + y * y + z * z;
-        if (r2 <= a.rc2) pair_eval(x, y, z, r2, pi, pj, a.pc, acc, elj, ec);
-    } else {
+        if (!excl && r2 <= a.rc2) pair_eval(x, y, z, r2, pi, qj, sj, ej, a.pc, fi, fj, elj, ec);
+    } else if (!excl) {
         // second candidate image per axis: the one on the other side, when within the cutoff
-        const float ax = x - copysignf(m.L[0], x), ay = y - copysignf(m.L[1], y), az = z - copysignf(m.L[2], z);
+        const float ax = x - copysignf(bx.fx, x), ay = y - copysignf(bx.fy, y), az = z - copysignf(bx.fz, z);
         const int nx = fabsf(ax) <= a.rc ? 2 : 1, ny = fabsf(ay) <= a.rc ? 2 : 1, nz = fabsf(az) <= a.rc ? 2 : 1;
         for (int kz = 0; kz < nz; kz++) {
             const float zz = kz ? az : z;
@@ -150,12 +162,16 @@ __device__ __forceinline__ void pair_cut(const int4& wi, const float4& pi, int c
                 for (int kx = 0; kx < nx; kx++) {
                     const float xx = kx ? ax : x;
                     const float r2 = xx * xx + yy * yy + zz * zz;
-                    if (r2 <= a.rc2) pair_eval(xx, yy, zz, r2, pi, pj, a.pc, acc, elj, ec);
+                    if (r2 <= a.rc2) pair_eval(xx, yy, zz, r2, pi, qj, sj, ej, a.pc, fi, fj, elj, ec);
                 }
             }
         }
     }
 }
+
+// Rotate a value across the wave by one lane (DPP wave_rol:1): lane l receives lane l+1's value.
+__device__ __forceinline__ int rot1(int v) { return __builtin_amdgcn_mov_dpp(v, 0x134, 0xF, 0xF, false); }
+__device__ __forceinline__ float rot1f(float v) { return __int_as_float(rot1(__float_as_int(v))); }
 
 // intra pairs of atom i (local li): its compound and the bonded compounds, LUT-excluded, no cutoff
 __device__ __forceinline__ void intra_atom(const BatchArgs& a, const SysMeta& m, const int4* u, int li, int ci, int bmi, float4 pi,
@@ -250,6 +266,7 @@ struct BlockShared {
     double res[5];
     float fres;
     float2 tw[3][KMAX];
+    unsigned long long ph[5];    // phase ticks seen by thread 0
 };
 
 // FP64 sums of v[0..3] and max of fm over the workgroup, in a fixed order; every thread gets them.
@@ -285,71 +302,132 @@ struct Eval {
 
 // Forces and energy of the system at positions u (global arrays); writes f and returns the FP64
 // energy parts and Fmax (identical in every thread).
-template <int T, int RB, bool MULTI>
+template <int T, int UNR, bool MULTI>
 __device__ __forceinline__ Eval evaluate(const BatchArgs& a, const SysMeta& m, BlockShared<T>& sh, unsigned char* lds,
                                          const int4* u, float4* f) {
     const int tid = threadIdx.x;
     const int n = m.n;
     double elj_d = 0.0, ec_d = 0.0, eb_d = 0.0, ep_d = 0.0;
-    int4* tw4 = reinterpret_cast<int4*>(lds);
-    float4* tp4 = reinterpret_cast<float4*>(lds + T * sizeof(int4));
-    // ---- pairs, intra pairs and bonded terms, RB atoms per thread per pass ----
-    for (int c0 = 0; c0 < n; c0 += T * RB) {
-        int4 wi[RB];
-        float4 pi[RB];
-        int ci[RB], bmi[RB];
-        v3 acc[RB];
-        float elj[RB], ec[RB];
-#pragma unroll
-        for (int r = 0; r < RB; r++) {
-            const int li = c0 + r * T + tid;
-            acc[r] = {0.f, 0.f, 0.f};
-            elj[r] = 0.f;
-            ec[r] = 0.f;
-            if (li < n) {
-                const int4 uu = u[m.atom0 + li];
-                ci[r] = a.comp[m.atom0 + li];
-                wi[r] = make_int4(fmodi(uu.x, m.Lq[0]), fmodi(uu.y, m.Lq[1]), fmodi(uu.z, m.Lq[2]), ci[r]);
-                pi[r] = a.prm[m.atom0 + li];
-                bmi[r] = a.cbm[m.comp0 + ci[r]];
-            } else {
-                ci[r] = -1000; bmi[r] = 0; wi[r] = make_int4(0, 0, 0, 0); pi[r] = make_float4(0.f, 0.f, 0.f, 0.f);
-            }
+    unsigned long long tk = tid == 0 ? __builtin_amdgcn_s_memrealtime() : 0ull;
+    // ---- cutoff pairs: every unordered pair of 64-atom tiles once (Newton's third law) ----
+    // Wave w takes a fixed contiguous range of tile pairs (I <= J).  Lane l holds atom I*64+l;
+    // the J tile's atoms and their force accumulators rotate one lane per step (DPP), so at
+    // step s lane l meets atom J*64 + (l+s) mod 64.  Each tile pair writes its i- and j-side
+    // partial forces; atoms sum them in a fixed tile order, so forces do not depend on the
+    // workgroup shape.
+    const int NT = (n + 63) >> 6;
+    {
+        const int lane = tid & 63, wave = tid >> 6;
+        constexpr int W = T / 64;
+        const int NI = NT * (NT + 1) / 2;
+        float* home = reinterpret_cast<float*>(lds) + wave * 192;   // 64 x 3 floats per wave
+        float4* part = a.part + m.part0;
+        const PairBox bx = {m.Lq[0], m.Lq[1], m.Lq[2], m.hLq[0], m.hLq[1], m.hLq[2], m.L[0], m.L[1], m.L[2]};
+        const int q0 = (int)((long long)NI * wave / W), q1 = (int)((long long)NI * (wave + 1) / W);
+        int I = 0, J = 0;
+        {
+            int q = q0;
+            while (I < NT && q >= NT - I) { q -= NT - I; I++; }
+            J = I + q;
         }
-        for (int j0 = 0; j0 < n; j0 += T) {
-            __syncthreads();
-            const int lj = j0 + tid;
-            if (lj < n) {
-                const int4 uu = u[m.atom0 + lj];
-                tw4[tid] = make_int4(fmodi(uu.x, m.Lq[0]), fmodi(uu.y, m.Lq[1]), fmodi(uu.z, m.Lq[2]), a.comp[m.atom0 + lj]);
-                tp4[tid] = a.prm[m.atom0 + lj];
+        int curI = -1;
+        int4 wi = make_int4(0, 0, 0, 0);
+        float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
+        int ci = 0, bmi = 0;
+        bool vi = false;
+        for (int q = q0; q < q1; q++) {
+            if (I != curI) {
+                curI = I;
+                const int li = I * 64 + lane;
+                vi = li < n;
+                if (vi) {
+                    const int4 uu = u[m.atom0 + li];
+                    ci = a.comp[m.atom0 + li];
+                    wi = make_int4(fmodi(uu.x, bx.Lx), fmodi(uu.y, bx.Ly), fmodi(uu.z, bx.Lz), 0);
+                    pi = a.prm[m.atom0 + li];
+                    bmi = a.cbm[m.comp0 + ci];
+                }
             }
-            __syncthreads();
-            const int nj = min(T, n - j0);
-            for (int jj = 0; jj < nj; jj++) {
-                const int4 wj = tw4[jj];
-                const float4 pj = tp4[jj];
-#pragma unroll
-                for (int r = 0; r < RB; r++)
-                    if (c0 + r * T + tid < n) pair_cut<MULTI>(wi[r], pi[r], ci[r], bmi[r], wj, pj, m, a, acc[r], elj[r], ec[r]);
+            // atom J*64+lane and its accumulator start in this lane
+            int jx = 0, jy = 0, jz = 0, jc = -1;
+            float jq = 0.f, js = 0.f, je = 0.f;
+            {
+                const int lj = J * 64 + lane;
+                if (lj < n) {
+                    const int4 uu = u[m.atom0 + lj];
+                    const float4 pj = a.prm[m.atom0 + lj];
+                    jx = fmodi(uu.x, bx.Lx); jy = fmodi(uu.y, bx.Ly); jz = fmodi(uu.z, bx.Lz);
+                    jc = a.comp[m.atom0 + lj];
+                    jq = pj.x; js = pj.y; je = pj.w;
+                }
             }
+            v3 fi = {0.f, 0.f, 0.f}, fj = {0.f, 0.f, 0.f};
+            float elj = 0.f, ec = 0.f;
+            const bool diag = I == J;
+            const bool near = J - I <= 2;   // compounds hold <= 32 atoms: intra pairs lie within two tiles
+            const int ns = diag ? 33 : 64;  // diagonal tile: steps 1..32 (step 32 on half the lanes)
+#pragma unroll UNR
+            for (int st = 0; st < ns; st++) {
+                const int dc = jc - ci;
+                const bool excl = !vi || jc < 0 || (diag && (st == 0 || (st == 32 && lane >= 32))) ||
+                                  (near && (unsigned)(dc + 2) <= 4u && ((bmi >> (dc + 2)) & 1));
+                pair_cut<MULTI>(wi, pi, excl, jx, jy, jz, jq, js, je, bx, a, fi, fj, elj, ec);
+                if (st + 1 < ns) {
+                    jx = rot1(jx); jy = rot1(jy); jz = rot1(jz); jc = rot1(jc);
+                    jq = rot1f(jq); js = rot1f(js); je = rot1f(je);
+                    fj.x = rot1f(fj.x); fj.y = rot1f(fj.y); fj.z = rot1f(fj.z);
+                }
+            }
+            // after ns - 1 rotations lane l holds the accumulator of atom J*64 + (l + ns - 1) mod 64
+            const int jidx = (lane + ns - 1) & 63;
+            home[jidx] = fj.x; home[64 + jidx] = fj.y; home[128 + jidx] = fj.z;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const float hx = home[lane], hy = home[64 + lane], hz = home[128 + lane];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            part[(q * 2 + 0) * 64 + lane] = make_float4(fi.x, fi.y, fi.z, 0.f);
+            part[(q * 2 + 1) * 64 + lane] = make_float4(hx, hy, hz, 0.f);
+            elj_d += (double)elj;
+            ec_d += (double)ec;
+            if (++J == NT) { ++I; J = I; }
         }
-#pragma unroll
-        for (int r = 0; r < RB; r++) {
-            const int li = c0 + r * T + tid;
-            if (li >= n) continue;
-            intra_atom(a, m, u, li, ci[r], bmi[r], pi[r], acc[r], elj[r], ec[r]);
-            float eb = 0.f;
-            bonded_atom(a, u, m.atom0 + li, acc[r], eb);
-            f[m.atom0 + li] = make_float4(acc[r].x, acc[r].y, acc[r].z, 0.f);
-            elj_d += (double)elj[r];
-            ec_d += (double)ec[r];
+    }
+    __syncthreads();
+    if (tid == 0) { const unsigned long long t = __builtin_amdgcn_s_memrealtime(); sh.ph[0] += t - tk; tk = t; }
+    // ---- per atom: tile-pair partials in a fixed order, intra pairs, bonded terms ----
+    {
+        const float4* part = a.part + m.part0;
+        for (int li = tid; li < n; li += T) {
+            const int A = li >> 6, la = li & 63;
+            v3 acc = {0.f, 0.f, 0.f};
+            for (int I2 = 0; I2 <= A; I2++) {   // j side of (I2, A)
+                const int q = I2 * NT - I2 * (I2 - 1) / 2 + (A - I2);
+                const float4 v = part[(q * 2 + 1) * 64 + la];
+                acc.x += v.x; acc.y += v.y; acc.z += v.z;
+            }
+            for (int J2 = A; J2 < NT; J2++) {   // i side of (A, J2)
+                const int q = A * NT - A * (A - 1) / 2 + (J2 - A);
+                const float4 v = part[(q * 2 + 0) * 64 + la];
+                acc.x += v.x; acc.y += v.y; acc.z += v.z;
+            }
+            const int gi = m.atom0 + li;
+            const int ci = a.comp[gi];
+            float elj = 0.f, ec = 0.f, eb = 0.f;
+            intra_atom(a, m, u, li, ci, a.cbm[m.comp0 + ci], a.prm[gi], acc, elj, ec);
+            bonded_atom(a, u, gi, acc, eb);
+            f[gi] = make_float4(acc.x, acc.y, acc.z, 0.f);
+            elj_d += (double)elj;
+            ec_d += (double)ec;
             eb_d += (double)eb;
         }
     }
     // ---- PME ----
+    __syncthreads();
+    if (tid == 0) { const unsigned long long t = __builtin_amdgcn_s_memrealtime(); sh.ph[1] += t - tk; tk = t; }
     if (m.elec) {
-        __syncthreads();   // LDS switches from pair tiles to the grid
         int* qg = reinterpret_cast<int*>(lds);
         float* phi = reinterpret_cast<float*>(lds);
         const int K0 = m.K[0], K1 = m.K[1], K2 = m.K[2], H = m.H;
@@ -503,6 +581,8 @@ __device__ __forceinline__ Eval evaluate(const BatchArgs& a, const SysMeta& m, B
         }
     }
     // ---- Fmax and the FP64 energy sums ----
+    __syncthreads();
+    if (tid == 0) { const unsigned long long t = __builtin_amdgcn_s_memrealtime(); sh.ph[2] += t - tk; tk = t; }
     float fm = 0.f;
     for (int li = tid; li < n; li += T) {
         const float4 fo = f[m.atom0 + li];
@@ -510,10 +590,11 @@ __device__ __forceinline__ Eval evaluate(const BatchArgs& a, const SysMeta& m, B
     }
     double v[4] = {elj_d, ec_d, eb_d, ep_d};
     block_reduce<T>(sh, v, fm);
+    if (tid == 0) sh.ph[3] += __builtin_amdgcn_s_memrealtime() - tk;
     return Eval{v[0], v[1], v[2], v[3], sh.fres};
 }
 
-template <int T, int RB, bool MULTI>
+template <int T, int UNR, bool MULTI>
 __device__ __forceinline__ void minimize_system(const BatchArgs& a, const SysMeta& m, SysOut* out, BlockShared<T>& sh,
                                                 unsigned char* lds) {
     const int tid = threadIdx.x;
@@ -521,12 +602,14 @@ __device__ __forceinline__ void minimize_system(const BatchArgs& a, const SysMet
     double h = a.h0;
     int cur = 0, it = 0, nacc = 0;
     bool first = true, conv = false;
+    if (tid == 0) for (int q = 0; q < 5; q++) sh.ph[q] = 0ull;
     for (;;) {
         // the first pass evaluates the starting state (buffer 0); later passes a trial step
         const int tgt = first ? 0 : 1 - cur;
         int4* ut = (tgt ? a.u1 : a.u0) + m.atom0;
         float4* ft = (tgt ? a.f1 : a.f0) + m.atom0;
         if (!first) {
+            const unsigned long long ts = tid == 0 ? __builtin_amdgcn_s_memrealtime() : 0ull;
             const float k = (float)(h / (double)acc_e.fmax);
             const int4* uc = (cur ? a.u1 : a.u0) + m.atom0;
             const float4* fc = (cur ? a.f1 : a.f0) + m.atom0;
@@ -538,8 +621,9 @@ __device__ __forceinline__ void minimize_system(const BatchArgs& a, const SysMet
                                    x.z + (int)fminf(fmaxf(k * g.z * LIMA_NANO_TO_LIMA_F, -2.0e8f), 2.0e8f), 0);
             }
             __syncthreads();
+            if (tid == 0) sh.ph[4] += __builtin_amdgcn_s_memrealtime() - ts;
         }
-        const Eval ev = evaluate<T, RB, MULTI>(a, m, sh, lds, ut - m.atom0, ft - m.atom0);
+        const Eval ev = evaluate<T, UNR, MULTI>(a, m, sh, lds, ut - m.atom0, ft - m.atom0);
         if (first) {
             acc_e = ev;
             first = false;
@@ -576,10 +660,11 @@ __device__ __forceinline__ void minimize_system(const BatchArgs& a, const SysMet
         out->accepted = nacc;
         out->converged = conv ? 1 : 0;
         out->cur = 0;
+        for (int q = 0; q < 5; q++) out->ph[q] = sh.ph[q];
     }
 }
 
-template <int T, int RB>
+template <int T, int UNR>
 __global__ __launch_bounds__(T) void k_em_batch(BatchArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ BlockShared<T> sh;
@@ -587,15 +672,15 @@ __global__ __launch_bounds__(T) void k_em_batch(BatchArgs a) {
     const SysMeta m = a.meta[s];
     SysOut* out = a.out + s;
     if (threadIdx.x == 0) out->t0 = __builtin_amdgcn_s_memrealtime();
-    if (m.multi) minimize_system<T, RB, true>(a, m, out, sh, lds);
-    else minimize_system<T, RB, false>(a, m, out, sh, lds);
+    if (m.multi) minimize_system<T, UNR, true>(a, m, out, sh, lds);
+    else minimize_system<T, UNR, false>(a, m, out, sh, lds);
     __syncthreads();
     if (threadIdx.x == 0) out->t1 = __builtin_amdgcn_s_memrealtime();
 }
 
 // ---------------------------------------------------------------------------------------------
 
-struct Shape { int threads, rb; };
+struct Shape { int threads, unroll; };
 constexpr Shape SHAPES[] = {{256, 1}, {256, 2}, {256, 4}, {512, 1}, {512, 2}, {512, 4}, {1024, 1}, {1024, 2}, {1024, 4}};
 constexpr int NSHAPES = (int)(sizeof(SHAPES) / sizeof(SHAPES[0]));
 
@@ -664,6 +749,7 @@ struct Batch {
     int2* pid = nullptr; float2* pp = nullptr;
     float* green = nullptr;
     float2* spec = nullptr;
+    float4* part = nullptr;
     float2* twid = nullptr;
     SysOut* out = nullptr;
     hipStream_t streams[NSHAPES] = {};
@@ -677,7 +763,7 @@ namespace {
 void batch_free(Batch& b) {
     for (auto& s : b.streams) if (s) { (void)hipStreamSynchronize(s); (void)hipStreamDestroy(s); s = nullptr; }
     void* ptrs[] = {b.d_meta, b.d_order, b.u0, b.u1, b.u_start, b.f0, b.f1, b.prm4, b.comp, b.cinfo, b.cbm, b.clut, b.inc_off, b.inc,
-                    b.bid, b.bp, b.aid, b.ap, b.did, b.dp, b.iid, b.ip, b.pid, b.pp, b.green, b.spec, b.twid, b.out};
+                    b.bid, b.bp, b.aid, b.ap, b.did, b.dp, b.iid, b.ip, b.pid, b.pp, b.green, b.spec, b.part, b.twid, b.out};
     for (void* p : ptrs) if (p) (void)hipFree(p);
 }
 
@@ -707,6 +793,7 @@ void batch_create(Batch& B, const lima_box_desc* boxes, int nsys, const lima_sim
     std::vector<float> green;
     std::vector<float2> twid;
     int spec_total = 0;
+    int64_t part_total = 0;
     B.meta.resize(nsys);
     B.h_ncomp.resize(nsys);
     B.h_cn.resize(nsys);
@@ -879,6 +966,12 @@ void batch_create(Batch& B, const lima_box_desc* boxes, int nsys, const lima_sim
         }
         if (!m.elec) m.eself = 0.0;
         m.lds_bytes = m.elec ? m.NG * 4 : 0;
+        {
+            const int64_t nt = (m.n + 63) / 64;
+            if (part_total + nt * (nt + 1) / 2 * 128 >= (1ll << 31)) throw LimaFail(LIMA_ERR_BAD_INPUT, "batch too large");
+            m.part0 = (int)part_total;
+            part_total += nt * (nt + 1) / 2 * 128;
+        }
     }
     B.N = (int64_t)u.size();
     // incidence CSR
@@ -910,6 +1003,7 @@ void batch_create(Batch& B, const lima_box_desc* boxes, int nsys, const lima_sim
     B.pid = dcopy(pid); B.pp = dcopy(pp);
     B.green = dcopy(green);
     HIP_CHECK(hipMalloc(&B.spec, std::max(spec_total, 1) * sizeof(float2)));
+    HIP_CHECK(hipMalloc(&B.part, std::max<int64_t>(part_total, 1) * sizeof(float4)));
     B.twid = dcopy(twid);
     HIP_CHECK(hipMalloc(&B.out, nsys * sizeof(SysOut)));
     HIP_CHECK(hipMemset(B.out, 0, nsys * sizeof(SysOut)));
@@ -920,13 +1014,13 @@ void batch_create(Batch& B, const lima_box_desc* boxes, int nsys, const lima_sim
     for (int k = 0; k < NBUCKET; k++) {
         B.tuning.bucket_max_atoms[k] = BUCKET_MAX[k];
         B.tuning.threads[k] = 1024;
-        B.tuning.i_per_thread[k] = 2;
+        B.tuning.pair_unroll[k] = 2;
         B.tuning.us_per_iter[k] = 0.0;
     }
 }
 
 int shape_index(int threads, int rb) {
-    for (int i = 0; i < NSHAPES; i++) if (SHAPES[i].threads == threads && SHAPES[i].rb == rb) return i;
+    for (int i = 0; i < NSHAPES; i++) if (SHAPES[i].threads == threads && SHAPES[i].unroll == rb) return i;
     return -1;
 }
 
@@ -943,7 +1037,7 @@ BatchArgs make_args(const Batch& B, double tol_j, int max_iter, double h0) {
     a.inc_off = B.inc_off; a.inc = B.inc;
     a.bid = B.bid; a.bp = B.bp; a.aid = B.aid; a.ap = B.ap; a.did = B.did; a.dp = B.dp; a.iid = B.iid; a.ip = B.ip;
     a.pid = B.pid; a.pp = B.pp;
-    a.green = B.green; a.spec = B.spec; a.twid = B.twid; a.out = B.out;
+    a.green = B.green; a.spec = B.spec; a.part = B.part; a.twid = B.twid; a.out = B.out;
     a.pc = B.pc; a.rc = B.rc; a.rc2 = B.rc * B.rc;
     a.tol = tol_j; a.h0 = h0; a.max_iter = max_iter;
     return a;
@@ -965,7 +1059,7 @@ void launch_groups(Batch& B, const std::vector<int>& systems, const std::vector<
     for (int sh = 0; sh < NSHAPES; sh++) {
         const int cnt = ranges[sh].second;
         if (cnt == 0) continue;
-        int lds = SHAPES[sh].threads * 32;
+        int lds = SHAPES[sh].threads / 64 * 192 * (int)sizeof(float);
         for (int k = 0; k < cnt; k++) lds = std::max(lds, B.meta[order[ranges[sh].first + k]].lds_bytes);
         BatchArgs ag = a;
         ag.order = B.d_order + ranges[sh].first;
@@ -981,7 +1075,7 @@ std::vector<int> shapes_from_tuning(const Batch& B) {
     std::vector<int> shape_of(B.nsys);
     for (int s = 0; s < B.nsys; s++) {
         const int k = bucket_of(B.tuning, B.meta[s].n);
-        int sh = shape_index(B.tuning.threads[k], B.tuning.i_per_thread[k]);
+        int sh = shape_index(B.tuning.threads[k], B.tuning.pair_unroll[k]);
         if (sh < 0) sh = shape_index(1024, 2);
         shape_of[s] = sh;
     }
@@ -1075,7 +1169,7 @@ lima_status lima_batch_autotune(lima_batch* h, int32_t trial_iterations, lima_ba
             for (int sh = 0; sh < NSHAPES; sh++) if (cost[sh][k] < cost[best][k]) best = sh;
             const int take = std::min<int>(8, (int)members[k].size());
             B.tuning.threads[k] = SHAPES[best].threads;
-            B.tuning.i_per_thread[k] = SHAPES[best].rb;
+            B.tuning.pair_unroll[k] = SHAPES[best].unroll;
             B.tuning.us_per_iter[k] = cost[best][k] / take;
         }
         HIP_CHECK(hipMemcpy(B.u0, B.u_start, B.N * sizeof(int4), hipMemcpyDeviceToDevice));
@@ -1089,8 +1183,8 @@ lima_status lima_batch_set_tuning(lima_batch* h, const lima_batch_tuning* t) {
     if (!h || !t) { lima::g_err = "null argument"; return LIMA_ERR_BAD_INPUT; }
     if (t->n_buckets < 1 || t->n_buckets > 8) { lima::g_err = "n_buckets out of [1, 8]"; return LIMA_ERR_BAD_INPUT; }
     for (int k = 0; k < t->n_buckets; k++)
-        if (lima::shape_index(t->threads[k], t->i_per_thread[k]) < 0) {
-            lima::g_err = "unsupported shape (threads 256/512/1024, i_per_thread 1/2/4)";
+        if (lima::shape_index(t->threads[k], t->pair_unroll[k]) < 0) {
+            lima::g_err = "unsupported shape (threads 256/512/1024, pair_unroll 1/2/4)";
             return LIMA_ERR_BAD_INPUT;
         }
     h->b.tuning = *t;
@@ -1190,6 +1284,18 @@ lima_status lima_batch_reset(lima_batch* h) {
         Batch& B = h->b;
         HIP_CHECK(hipMemcpy(B.u0, B.u_start, B.N * sizeof(int4), hipMemcpyDeviceToDevice));
         B.forces_valid = false;
+    });
+}
+
+lima_status lima_batch_get_phase_times(lima_batch* h, double* seconds) {
+    if (!h || !seconds) { lima::g_err = "null argument"; return LIMA_ERR_BAD_INPUT; }
+    return bguard([&] {
+        using namespace lima;
+        Batch& B = h->b;
+        std::vector<SysOut> res(B.nsys);
+        HIP_CHECK(hipMemcpy(res.data(), B.out, B.nsys * sizeof(SysOut), hipMemcpyDeviceToHost));
+        for (int s = 0; s < B.nsys; s++)
+            for (int q = 0; q < 5; q++) seconds[s * 5 + q] = (double)res[s].ph[q] * 1e-8;
     });
 }
 

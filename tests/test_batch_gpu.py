@@ -22,7 +22,7 @@ def shape_tuning(threads, rb):
     t.n_buckets = 1
     t.bucket_max_atoms[0] = 8192
     t.threads[0] = threads
-    t.i_per_thread[0] = rb
+    t.pair_unroll[0] = rb
     return t
 
 
@@ -59,10 +59,18 @@ def test_batch_forces_independent_of_shape(batch4, threads, rb):
     b.set_tuning(shape_tuning(threads, rb))
     b.reset()
     res, _ = b.minimize(100.0, 0, 0.01)
+    p = L.default_params()
     for k in range(len(boxes)):
-        # same j order for every shape: forces agree to the last bits, energies to FP64 rounding
-        assert np.array_equal(b.state(k)["particle_force"], ref[k])
-        assert res[k].final_energy == pytest.approx(ref_res[k].final_energy, rel=1e-12)
+        f = b.state(k)["particle_force"]
+        d = np.abs(f - ref[k]).max() / np.abs(ref[k]).max()
+        f_ref, _, _ = Oracle(boxes[k], p).forces()
+        err = rel_rms(layout_forces(boxes[k], b.state(k)), f_ref)
+        print(f"system {k}: max |dF| / max |F| vs shape (1024, 1) {d:.2e}; rel RMS vs oracle {err:.2e}", flush=True)
+        assert err < 1e-5
+        # same pairs in the same order for every shape; only the instruction schedule of the
+        # unrolled pair loop differs, which moves the last bits
+        assert d < 1e-6
+        assert res[k].final_energy == pytest.approx(ref_res[k].final_energy, rel=1e-9)
 
 
 def test_batch_minimize_converges_like_the_engine(batch4):
@@ -79,8 +87,10 @@ def test_batch_minimize_converges_like_the_engine(batch4):
               f"engine {r.iterations} it E {r.final_energy:.6e}", flush=True)
         assert res[k].converged == 1 and res[k].final_fmax < 100.0
         assert res[k].final_energy < e0
-        # steepest descent from the same start reaches the same minimum within 0.1 %
-        assert res[k].final_energy == pytest.approx(r.final_energy, rel=1e-3)
+        # two FP32 paths take the same first steps and then drift apart (the accept / reject
+        # sequence amplifies rounding; the engine's float atomics also vary run to run), ending in
+        # neighbouring minima of the rough landscape
+        assert res[k].final_energy == pytest.approx(r.final_energy, rel=1e-2)
     print(f"batch wall {wall:.3f}s", flush=True)
 
 
@@ -93,6 +103,27 @@ def test_batch_minimize_matches_oracle_small():
     print(f"batch {res[0].iterations} it E {res[0].final_energy:.6e} | oracle {r.iterations} it E {r.final_energy:.6e}")
     assert res[0].converged == 1 and r.converged == 1
     assert res[0].final_energy == pytest.approx(r.final_energy, rel=1e-3)
+
+
+@pytest.mark.parametrize("n,seed", [(300, 1000), (700, 1001)])
+def test_batch_first_iterations_match_oracle(n, seed):
+    """The first SD iterations follow the FP64 oracle's trajectory: same accept / reject sequence
+    and the same energy after 30 iterations, to FP32 rounding of the forces and steps measured on
+    the starting energy scale (the energy itself falls through zero on the way)."""
+    box = L.gen_em_system(n, seed, 0.02)
+    b = L.Batch([box], L.default_params())
+    res, _ = b.minimize(100.0, 30, 0.01)
+    o = Oracle(box, L.default_params())
+    e0 = o.forces()[2].potential
+    r = o.minimize(tol=100.0, max_iter=30, h0=0.01)
+    print(f"n={n}: E0 {e0:.6e} | batch acc {res[0].accepted} E {res[0].final_energy:.8e} Fmax {res[0].final_fmax:.2f} | "
+          f"oracle acc {r.accepted} E {r.final_energy:.8e} Fmax {r.final_fmax:.2f}")
+    assert res[0].iterations == r.iterations == 30
+    assert res[0].accepted == r.accepted
+    assert abs(res[0].final_energy - r.final_energy) < 5e-6 * abs(e0)
+    # Fmax is the force on one atom, often in a steep clash where a few Coord units of step
+    # rounding move it by 1e-3
+    assert res[0].final_fmax == pytest.approx(r.final_fmax, rel=1e-2)
 
 
 def test_batch_is_deterministic(batch4):
@@ -111,7 +142,7 @@ def test_batch_autotune_picks_valid_shapes(batch4):
     boxes, b = batch4
     t = b.autotune(3)
     for k in range(t.n_buckets):
-        assert t.threads[k] in (256, 512, 1024) and t.i_per_thread[k] in (1, 2, 4)
+        assert t.threads[k] in (256, 512, 1024) and t.pair_unroll[k] in (1, 2, 4)
     print(t.as_list())
     # tuning leaves the starting positions in place
     b.reset()
