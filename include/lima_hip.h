@@ -256,6 +256,50 @@ lima_status lima_batch_minimize(lima_batch* batch, double fmax_tol_kj, int64_t m
 lima_status lima_batch_get_state(lima_batch* batch, int32_t system, lima_state_out* out);
 /* Restores the positions every system had at lima_batch_create. */
 lima_status lima_batch_reset(lima_batch* batch);
+/* ---- Learned step proposals.  A per-atom MLP (32 features: F / Fmax, |F| / Fmax, and 28 counts of
+ * neighbours within 1.0 nm in 14 distance bins per charge sign; two SiLU layers of 64; output a
+ * step scale s in (0, 4)) proposes x' = x + h s_i F_i / Fmax.  The proposal is kept only if the FP64
+ * energy decreases; otherwise the plain steepest-descent step is taken, so every accepted state is
+ * one the plain method would also accept.  No counterpart in LIMA; the minimisation is still
+ * Programs::EnergyMinimize's (Programs.cpp:155-199). ---- */
+typedef struct lima_mlp lima_mlp;
+int32_t lima_mlp_n_params(void);
+lima_status lima_mlp_create(uint64_t seed, lima_mlp** out);
+lima_status lima_mlp_destroy(lima_mlp* model);
+/* FP32 parameters: W1 [32][64], b1 [64], W2 [64][64], b2 [64], w3 [64], b3 */
+lima_status lima_mlp_get_params(lima_mlp* model, float* params);
+lima_status lima_mlp_set_params(lima_mlp* model, const float* params);
+/* s for n feature rows [n][32] (BF16 MFMA inference, FP32 accumulation) */
+lima_status lima_mlp_forward(lima_mlp* model, const float* features, int64_t n, float* scale_out);
+/* Adam on 1/2 (s - t)^2, BF16 MFMA forward and backward with FP32 accumulation and master weights;
+ * batch a multiple of 16.  loss_per_epoch[epochs] (may be NULL), *seconds (may be NULL). */
+lima_status lima_mlp_train(lima_mlp* model, const float* features, const float* targets, int64_t n, int32_t epochs,
+                           int32_t batch, float learning_rate, uint64_t seed, float* loss_per_epoch, double* seconds);
+/* Gradient of sum_samples 1/2 (s - t)^2 w.r.t. the FP32 parameters (the training kernel's own
+ * product), grad_out[lima_mlp_n_params()]; *loss = the summed loss. */
+lima_status lima_mlp_gradient(lima_mlp* model, const float* features, const float* targets, int64_t n, float* grad_out,
+                              double* loss);
+/* Batched minimisation with learned proposals; the step-scale features' neighbour histograms are
+ * refreshed every feature_interval iterations.  Systems of up to 4096 atoms. */
+lima_status lima_batch_minimize_learned(lima_batch* batch, lima_mlp* model, double fmax_tol_kj, int64_t max_iter,
+                                        double initial_step_nm, int32_t feature_interval, lima_em_result* results,
+                                        double* wall_seconds);
+/* Plain batched minimisation that records, at every accepted state, the features, the plain step
+ * and the Coord of every atom_stride-th atom (up to max_states states per system). */
+lima_status lima_batch_record(lima_batch* batch, double fmax_tol_kj, int64_t max_iter, double initial_step_nm,
+                              int32_t atom_stride, int32_t max_states, int64_t* n_states);
+/* Training samples from the recording: features [n][32] and the target step scale: the atom's
+ * displacement over the next `lookahead` accepted states projected on its plain step, in units of
+ * that step, clipped to [0.05, 3.95].  With features == NULL only *n_samples is set. */
+lima_status lima_batch_training_set(lima_batch* batch, int32_t lookahead, float* features, float* targets, int64_t capacity,
+                                    int64_t* n_samples);
+/* Feature rows [n_atoms][32] of one system as last computed (slots 4..31: neighbour histogram,
+ * count / 8; slots 0..3 are filled from the forces at inference time and read here as 0). */
+lima_status lima_batch_get_features(lima_batch* batch, int32_t system, float* out);
+/* Per system [n][3]: force evaluations, iterations whose learned proposal was accepted, recorded states
+ * of the last call. */
+lima_status lima_batch_get_counters(lima_batch* batch, int64_t* out);
+
 /* Time each system spent in the last lima_batch_minimize, by phase, as seen by its workgroup's
  * first thread: seconds[n_systems][5] = cutoff tile pairs, per-atom sums + intra pairs + bonded
  * terms, PME, reductions, trial steps. */

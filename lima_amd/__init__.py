@@ -90,6 +90,23 @@ def load_hip():
             "lima_batch_get_state": ([vp, C.c_int32, C.POINTER(StateOut)], C.c_int),
             "lima_batch_reset": ([vp], C.c_int),
             "lima_batch_get_phase_times": ([vp, C.POINTER(C.c_double)], C.c_int),
+            "lima_batch_minimize_learned": ([vp, vp, C.c_double, C.c_int64, C.c_double, C.c_int32, C.POINTER(EmResult),
+                                             C.POINTER(C.c_double)], C.c_int),
+            "lima_batch_record": ([vp, C.c_double, C.c_int64, C.c_double, C.c_int32, C.c_int32, C.POINTER(C.c_int64)], C.c_int),
+            "lima_batch_training_set": ([vp, C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int64,
+                                         C.POINTER(C.c_int64)], C.c_int),
+            "lima_batch_get_counters": ([vp, C.POINTER(C.c_int64)], C.c_int),
+            "lima_batch_get_features": ([vp, C.c_int32, C.POINTER(C.c_float)], C.c_int),
+            "lima_mlp_n_params": ([], C.c_int32),
+            "lima_mlp_create": ([C.c_uint64, C.POINTER(vp)], C.c_int),
+            "lima_mlp_destroy": ([vp], C.c_int),
+            "lima_mlp_get_params": ([vp, C.POINTER(C.c_float)], C.c_int),
+            "lima_mlp_set_params": ([vp, C.POINTER(C.c_float)], C.c_int),
+            "lima_mlp_forward": ([vp, C.POINTER(C.c_float), C.c_int64, C.POINTER(C.c_float)], C.c_int),
+            "lima_mlp_gradient": ([vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int64, C.POINTER(C.c_float),
+                                   C.POINTER(C.c_double)], C.c_int),
+            "lima_mlp_train": ([vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int64, C.c_int32, C.c_int32, C.c_float,
+                                C.c_uint64, C.POINTER(C.c_float), C.POINTER(C.c_double)], C.c_int),
         }
         for name, (args, res) in sig.items():
             fn = getattr(lib, name)
@@ -357,12 +374,110 @@ class Batch:
     def reset(self):
         self._check(self.lib.lima_batch_reset(self._h))
 
+    def minimize_learned(self, model, fmax_tol_kj=100.0, max_iter=10000, initial_step_nm=0.01, feature_interval=4):
+        """Learned proposals with the plain step as fallback.  Returns (results, wall seconds)."""
+        res = (EmResult * len(self.boxes))()
+        wall = C.c_double()
+        self._check(self.lib.lima_batch_minimize_learned(self._h, model._h, fmax_tol_kj, max_iter, initial_step_nm,
+                                                         feature_interval, res, C.byref(wall)))
+        return list(res), wall.value
+
+    def record(self, fmax_tol_kj=100.0, max_iter=10000, initial_step_nm=0.01, atom_stride=16, max_states=2000):
+        n = C.c_int64()
+        self._check(self.lib.lima_batch_record(self._h, fmax_tol_kj, max_iter, initial_step_nm, atom_stride, max_states,
+                                               C.byref(n)))
+        return n.value
+
+    def training_set(self, lookahead=4):
+        n = C.c_int64()
+        self._check(self.lib.lima_batch_training_set(self._h, lookahead, None, None, 0, C.byref(n)))
+        x = np.zeros((n.value, 32), np.float32)
+        t = np.zeros(n.value, np.float32)
+        fp = C.POINTER(C.c_float)
+        self._check(self.lib.lima_batch_training_set(self._h, lookahead, x.ctypes.data_as(fp), t.ctypes.data_as(fp), n.value,
+                                                     C.byref(n)))
+        return x, t
+
+    def counters(self):
+        """[n_systems, 3]: force evaluations, accepted learned proposals, recorded states."""
+        out = np.zeros((len(self.boxes), 3), np.int64)
+        self._check(self.lib.lima_batch_get_counters(self._h, out.ctypes.data_as(C.POINTER(C.c_int64))))
+        return out
+
+    def features(self, k):
+        out = np.zeros((self.boxes[k].n_particles, 32), np.float32)
+        self._check(self.lib.lima_batch_get_features(self._h, k, out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
     def phase_times(self):
         """[n_systems, 5] seconds: cutoff tile pairs, per-atom sums + intra + bonded, PME, reductions,
         trial steps."""
         out = np.zeros((len(self.boxes), 5))
         self._check(self.lib.lima_batch_get_phase_times(self._h, out.ctypes.data_as(C.POINTER(C.c_double))))
         return out
+
+
+class Mlp:
+    """Handle of a lima_mlp: the per-atom step-scale model (BF16 MFMA inference and training)."""
+
+    def __init__(self, seed=0):
+        self.lib = load_hip()
+        h = C.c_void_p()
+        self._check(self.lib.lima_mlp_create(seed, C.byref(h)))
+        self._h = h
+        self.n_params = int(self.lib.lima_mlp_n_params())
+
+    def _check(self, st):
+        if st != 0:
+            raise LimaError(f"status {st}: {self.lib.lima_last_error().decode()}")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.lima_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        p = np.zeros(self.n_params, np.float32)
+        self._check(self.lib.lima_mlp_get_params(self._h, p.ctypes.data_as(C.POINTER(C.c_float))))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        self._check(self.lib.lima_mlp_set_params(self._h, p.ctypes.data_as(C.POINTER(C.c_float))))
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        out = np.zeros(len(x), np.float32)
+        fp = C.POINTER(C.c_float)
+        self._check(self.lib.lima_mlp_forward(self._h, x.ctypes.data_as(fp), len(x), out.ctypes.data_as(fp)))
+        return out
+
+    def gradient(self, x, t):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        g = np.zeros(self.n_params, np.float32)
+        loss = C.c_double()
+        fp = C.POINTER(C.c_float)
+        self._check(self.lib.lima_mlp_gradient(self._h, x.ctypes.data_as(fp), t.ctypes.data_as(fp), len(x),
+                                               g.ctypes.data_as(fp), C.byref(loss)))
+        return g, loss.value
+
+    def train(self, x, t, epochs=4, batch=1024, lr=1e-3, seed=0):
+        """Returns (mean loss per epoch, seconds)."""
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        losses = np.zeros(epochs, np.float32)
+        sec = C.c_double()
+        fp = C.POINTER(C.c_float)
+        self._check(self.lib.lima_mlp_train(self._h, x.ctypes.data_as(fp), t.ctypes.data_as(fp), len(x), epochs, batch, lr, seed,
+                                            losses.ctypes.data_as(fp), C.byref(sec)))
+        return losses, sec.value
 
 
 class StateBox:

@@ -37,6 +37,7 @@
 
 #include "engine.h"
 #include "lima_terms.h"
+#include "lima_mlp.h"
 
 namespace lima {
 namespace {
@@ -72,6 +73,9 @@ struct SysOut {
     float fmax;         // [J/mol/nm]
     float h;
     int iterations, accepted, converged, cur;
+    int evaluations;             // force evaluations (learned mode: up to two per iteration)
+    int learned_accepted;        // iterations whose learned proposal was accepted
+    int rec_iters;               // recorded states (recording mode)
     unsigned long long t0, t1;   // s_memrealtime (100 MHz) at start / end
     unsigned long long ph[5];    // ticks in tile pairs, per-atom sums + intra + bonded, PME, reductions, trial steps
 };
@@ -103,7 +107,21 @@ struct BatchArgs {
     double tol;                  // Fmax tolerance [J/mol/nm]
     double h0;                   // initial step [nm]
     int max_iter;
+    int mode;                    // MODE_PLAIN, MODE_LEARNED, MODE_RECORD
+    MlpWeights mlp;              // learned mode: step-scale model
+    unsigned short* feat;        // [N][32] BF16 features (slots 4..31: neighbour histogram)
+    float* scale;                // [N] learned step scale of the current state
+    int feat_interval;           // iterations between histogram refreshes (learned mode)
+    float* rec;                  // recording: [rec_info.x + (state * nsamp + k) * REC_W]
+    const int2* rec_info;        // per system (offset in floats, sampled atoms)
+    int rec_stride, rec_cap;     // sampled atoms li = k * rec_stride; states per system
 };
+
+enum { MODE_PLAIN = 0, MODE_LEARNED = 1, MODE_RECORD = 2 };
+constexpr int REC_W = 40;              // floats per record: 32 features, step (3), Coord (3), pad
+constexpr float HIST_R = 1.0f;         // neighbour histogram radius [nm]
+constexpr int HIST_BINS = 14;          // bins per charge sign
+constexpr int HIST_MAX_ATOMS = 4096;   // byte counters in LDS: 28 per atom
 
 enum { T_BOND = 0, T_ANGLE = 1, T_DIH = 2, T_IMP = 3, T_PAIR = 4 };
 
@@ -292,6 +310,120 @@ __device__ __forceinline__ void block_reduce(BlockShared<T>& sh, double* v, floa
     }
     __syncthreads();
     for (int c = 0; c < 4; c++) v[c] = sh.res[c];
+}
+
+// Neighbour histograms (features 4..31): pairs within HIST_R counted in HIST_BINS distance bins per
+// charge sign of the neighbour, every periodic image, each pair once through the same tile-pair walk
+// as the forces (j positions rotate across the wave).  Byte counters live in LDS ([n][28]); integer
+// increments make the result independent of order.  Written as BF16 count / 8.
+template <int T, bool MULTI>
+__device__ __forceinline__ void hist_pass(const BatchArgs& a, const SysMeta& m, unsigned char* lds, const int4* u) {
+    const int tid = threadIdx.x, n = m.n;
+    unsigned* hc = reinterpret_cast<unsigned*>(lds);
+    for (int k = tid; k < n * 7; k += T) hc[k] = 0u;
+    __syncthreads();
+    const int lane = tid & 63, wave = tid >> 6;
+    constexpr int W = T / 64;
+    const int NT = (n + 63) >> 6, NI = NT * (NT + 1) / 2;
+    const PairBox bx = {m.Lq[0], m.Lq[1], m.Lq[2], m.hLq[0], m.hLq[1], m.hLq[2], m.L[0], m.L[1], m.L[2]};
+    const float r2max = HIST_R * HIST_R, binv = (float)HIST_BINS / HIST_R;
+    const int q0 = (int)((long long)NI * wave / W), q1 = (int)((long long)NI * (wave + 1) / W);
+    int I = 0, J = 0;
+    {
+        int q = q0;
+        while (I < NT && q >= NT - I) { q -= NT - I; I++; }
+        J = I + q;
+    }
+    int curI = -1, li = 0, ineg = 0;
+    int4 wi = make_int4(0, 0, 0, 0);
+    bool vi = false;
+    for (int q = q0; q < q1; q++) {
+        if (I != curI) {
+            curI = I;
+            li = I * 64 + lane;
+            vi = li < n;
+            if (vi) {
+                const int4 uu = u[m.atom0 + li];
+                wi = make_int4(fmodi(uu.x, bx.Lx), fmodi(uu.y, bx.Ly), fmodi(uu.z, bx.Lz), 0);
+                ineg = a.prm[m.atom0 + li].x < 0.f ? HIST_BINS : 0;
+            }
+        }
+        int jx = 0, jy = 0, jz = 0, jl = -1, jneg = 0;
+        {
+            const int lj = J * 64 + lane;
+            if (lj < n) {
+                const int4 uu = u[m.atom0 + lj];
+                jx = fmodi(uu.x, bx.Lx); jy = fmodi(uu.y, bx.Ly); jz = fmodi(uu.z, bx.Lz);
+                jl = lj;
+                jneg = a.prm[m.atom0 + lj].x < 0.f ? HIST_BINS : 0;
+            }
+        }
+        const bool diag = I == J;
+        const int ns = diag ? 33 : 64;
+        for (int st = 0; st < ns; st++) {
+            const bool skip = !vi || jl < 0 || (diag && (st == 0 || (st == 32 && lane >= 32)));
+            if (!skip) {
+                int dx = jx - wi.x, dy = jy - wi.y, dz = jz - wi.z;
+                dx += dx < -bx.hx ? bx.Lx : 0; dx -= dx >= bx.hx ? bx.Lx : 0;
+                dy += dy < -bx.hy ? bx.Ly : 0; dy -= dy >= bx.hy ? bx.Ly : 0;
+                dz += dz < -bx.hz ? bx.Lz : 0; dz -= dz >= bx.hz ? bx.Lz : 0;
+                const float x = (float)dx * LIMA_LIMA_TO_NANO_F, y = (float)dy * LIMA_LIMA_TO_NANO_F, z = (float)dz * LIMA_LIMA_TO_NANO_F;
+                const float ax = x - copysignf(bx.fx, x), ay = y - copysignf(bx.fy, y), az = z - copysignf(bx.fz, z);
+                const int nx = MULTI && fabsf(ax) <= HIST_R ? 2 : 1, ny = MULTI && fabsf(ay) <= HIST_R ? 2 : 1;
+                const int nz = MULTI && fabsf(az) <= HIST_R ? 2 : 1;
+                for (int kz = 0; kz < nz; kz++)
+                    for (int ky = 0; ky < ny; ky++)
+                        for (int kx = 0; kx < nx; kx++) {
+                            const float xx = kx ? ax : x, yy = ky ? ay : y, zz = kz ? az : z;
+                            const float r2 = xx * xx + yy * yy + zz * zz;
+                            if (r2 <= r2max) {
+                                const int b = min(HIST_BINS - 1, (int)(sqrtf(r2) * binv));
+                                const int bi = b + jneg, bj = b + ineg;
+                                atomicAdd(&hc[li * 7 + (bi >> 2)], 1u << ((bi & 3) * 8));
+                                atomicAdd(&hc[jl * 7 + (bj >> 2)], 1u << ((bj & 3) * 8));
+                            }
+                        }
+            }
+            if (st + 1 < ns) { jx = rot1(jx); jy = rot1(jy); jz = rot1(jz); jl = rot1(jl); jneg = rot1(jneg); }
+        }
+        if (++J == NT) { ++I; J = I; }
+    }
+    __syncthreads();
+    for (int k = tid; k < n; k += T) {
+        unsigned short* fo = a.feat + (int64_t)(m.atom0 + k) * 32;
+        for (int b = 0; b < 2 * HIST_BINS; b++) {
+            const unsigned c = (hc[k * 7 + (b >> 2)] >> ((b & 3) * 8)) & 255u;
+            fo[4 + b] = f2bf((float)c * 0.125f);
+        }
+    }
+    __syncthreads();
+}
+
+// Learned step scale of every atom: features = (F / Fmax, |F| / Fmax, histogram), one MFMA forward
+// per 16 atoms per wave.
+template <int T>
+__device__ __forceinline__ void infer_scales(const BatchArgs& a, const SysMeta& m, const float4* F, float fmax) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
+    constexpr int W = T / 64;
+    const float inv = 1.f / fmax;
+    const bf16x4 zero = pack4(0.f, 0.f, 0.f, 0.f);
+    for (int base = wave * 16; base < m.n; base += W * 16) {
+        const int li = base + r;
+        const bool ok = li < m.n;
+        const unsigned short* fr = a.feat + (int64_t)(m.atom0 + (ok ? li : 0)) * 32;
+        bf16x4 xb[2];
+        if (g == 0) {
+            const float4 f = ok ? F[m.atom0 + li] : make_float4(0.f, 0.f, 0.f, 0.f);
+            xb[0] = pack4(f.x * inv, f.y * inv, f.z * inv, sqrtf(f.x * f.x + f.y * f.y + f.z * f.z) * inv);
+        } else {
+            xb[0] = ok ? *reinterpret_cast<const bf16x4*>(fr + 4 * g) : zero;
+        }
+        xb[1] = ok ? *reinterpret_cast<const bf16x4*>(fr + 16 + 4 * g) : zero;
+        f32x4 z1[4], z2[4];
+        bf16x4 h1[4];
+        const float z3 = mlp_forward16(a.mlp, xb, z1, h1, z2);
+        if (g == 0 && ok) a.scale[m.atom0 + li] = mlp_scale(z3);
+    }
 }
 
 struct Eval {
@@ -594,52 +726,98 @@ __device__ __forceinline__ Eval evaluate(const BatchArgs& a, const SysMeta& m, B
     return Eval{v[0], v[1], v[2], v[3], sh.fres};
 }
 
+// Write the sampled atoms of the accepted state: features (F / Fmax, |F| / Fmax, histogram), the
+// plain step h F / Fmax the minimiser would take from here, and the Coord.
+template <int T>
+__device__ __forceinline__ void record_state(const BatchArgs& a, const SysMeta& m, int sys, int state, const int4* u, const float4* F, float fmax,
+                             double h) {
+    const int2 ri = a.rec_info[sys];
+    const float inv = 1.f / fmax, k = (float)(h / (double)fmax);
+    for (int q = threadIdx.x; q < ri.y; q += T) {
+        const int li = q * a.rec_stride;
+        float* e = a.rec + ri.x + ((int64_t)state * ri.y + q) * REC_W;
+        const float4 f = F[m.atom0 + li];
+        const int4 x = u[m.atom0 + li];
+        e[0] = f.x * inv; e[1] = f.y * inv; e[2] = f.z * inv; e[3] = sqrtf(f.x * f.x + f.y * f.y + f.z * f.z) * inv;
+        const unsigned short* fr = a.feat + (int64_t)(m.atom0 + li) * 32;
+        for (int b = 4; b < 32; b++) e[b] = bf2f(fr[b]);
+        e[32] = k * f.x; e[33] = k * f.y; e[34] = k * f.z;
+        e[35] = __int_as_float(x.x); e[36] = __int_as_float(x.y); e[37] = __int_as_float(x.z);
+        e[38] = 0.f; e[39] = 0.f;
+    }
+}
+
+// Trial step from the accepted state: x' = x + trunc(h s F / Fmax), s = 1 (plain) or the learned scale.
+template <int T>
+__device__ __forceinline__ void trial_step(const SysMeta& m, const int4* uc, const float4* fc, int4* ut, const float* scale, double h,
+                                           float fmax) {
+    const float k = (float)(h / (double)fmax);
+    for (int li = threadIdx.x; li < m.n; li += T) {
+        const int4 x = uc[li];
+        const float4 g = fc[li];
+        const float kk = scale ? k * scale[li] : k;
+        ut[li] = make_int4(x.x + (int)fminf(fmaxf(kk * g.x * LIMA_NANO_TO_LIMA_F, -2.0e8f), 2.0e8f),
+                           x.y + (int)fminf(fmaxf(kk * g.y * LIMA_NANO_TO_LIMA_F, -2.0e8f), 2.0e8f),
+                           x.z + (int)fminf(fmaxf(kk * g.z * LIMA_NANO_TO_LIMA_F, -2.0e8f), 2.0e8f), 0);
+    }
+}
+
 template <int T, int UNR, bool MULTI>
-__device__ __forceinline__ void minimize_system(const BatchArgs& a, const SysMeta& m, SysOut* out, BlockShared<T>& sh,
+__device__ __forceinline__ void minimize_system(const BatchArgs& a, const SysMeta& m, int sys, SysOut* out, BlockShared<T>& sh,
                                                 unsigned char* lds) {
     const int tid = threadIdx.x;
+    if (tid == 0) for (int q = 0; q < 5; q++) sh.ph[q] = 0ull;
+    const bool learned = a.mode == MODE_LEARNED, record = a.mode == MODE_RECORD;
     Eval acc_e{0.0, 0.0, 0.0, 0.0, 0.f};
     double h = a.h0;
-    int cur = 0, it = 0, nacc = 0;
-    bool first = true, conv = false;
-    if (tid == 0) for (int q = 0; q < 5; q++) sh.ph[q] = 0ull;
+    int cur = 0, it = 0, nacc = 0, nevals = 0, nlearn = 0, nrec = 0;
+    // pass kinds: 0 the starting state, 1 a learned proposal, 2 the plain step
+    int kind = 0;
+    bool conv = false;
     for (;;) {
-        // the first pass evaluates the starting state (buffer 0); later passes a trial step
-        const int tgt = first ? 0 : 1 - cur;
-        int4* ut = (tgt ? a.u1 : a.u0) + m.atom0;
-        float4* ft = (tgt ? a.f1 : a.f0) + m.atom0;
-        if (!first) {
-            const unsigned long long ts = tid == 0 ? __builtin_amdgcn_s_memrealtime() : 0ull;
-            const float k = (float)(h / (double)acc_e.fmax);
+        const int tgt = kind == 0 ? 0 : 1 - cur;
+        int4* ut = tgt ? a.u1 : a.u0;
+        float4* ft = tgt ? a.f1 : a.f0;
+        if (kind != 0) {
             const int4* uc = (cur ? a.u1 : a.u0) + m.atom0;
             const float4* fc = (cur ? a.f1 : a.f0) + m.atom0;
-            for (int li = tid; li < m.n; li += T) {
-                const int4 x = uc[li];
-                const float4 g = fc[li];
-                ut[li] = make_int4(x.x + (int)fminf(fmaxf(k * g.x * LIMA_NANO_TO_LIMA_F, -2.0e8f), 2.0e8f),
-                                   x.y + (int)fminf(fmaxf(k * g.y * LIMA_NANO_TO_LIMA_F, -2.0e8f), 2.0e8f),
-                                   x.z + (int)fminf(fmaxf(k * g.z * LIMA_NANO_TO_LIMA_F, -2.0e8f), 2.0e8f), 0);
+            if (kind == 1) {
+                if (it % a.feat_interval == 0) hist_pass<T, MULTI>(a, m, lds, cur ? a.u1 : a.u0);
+                infer_scales<T>(a, m, cur ? a.f1 : a.f0, acc_e.fmax);
+                __syncthreads();
             }
+            const unsigned long long ts = tid == 0 ? __builtin_amdgcn_s_memrealtime() : 0ull;
+            trial_step<T>(m, uc, fc, ut + m.atom0, kind == 1 ? a.scale + m.atom0 : nullptr, h, acc_e.fmax);
             __syncthreads();
             if (tid == 0) sh.ph[4] += __builtin_amdgcn_s_memrealtime() - ts;
         }
-        const Eval ev = evaluate<T, UNR, MULTI>(a, m, sh, lds, ut - m.atom0, ft - m.atom0);
-        if (first) {
+        const Eval ev = evaluate<T, UNR, MULTI>(a, m, sh, lds, ut, ft);
+        nevals++;
+        bool step_done = false, accepted = false;
+        if (kind == 0) {
             acc_e = ev;
-            first = false;
-        } else {
-            if (ev.total() < acc_e.total()) {   // FP64 acceptance
-                cur = tgt;
-                acc_e = ev;
-                h *= 1.2;
-                nacc++;
-            } else {
-                h *= 0.2;
-            }
+            accepted = true;   // for recording the starting state
+        } else if (ev.total() < acc_e.total()) {   // FP64 acceptance
+            cur = tgt;
+            acc_e = ev;
+            accepted = true;
+            step_done = true;
+            nlearn += kind == 1;
+        } else if (kind == 2) {
+            step_done = true;
+        }
+        if (kind != 0 && step_done) {
+            if (accepted) { h *= 1.2; nacc++; } else { h *= 0.2; }
             it++;
         }
+        if (record && accepted && nrec < a.rec_cap) {
+            hist_pass<T, MULTI>(a, m, lds, cur ? a.u1 : a.u0);
+            record_state<T>(a, m, sys, nrec++, cur ? a.u1 : a.u0, cur ? a.f1 : a.f0, acc_e.fmax, h);
+        }
+        if (kind == 1 && !step_done) { kind = 2; continue; }   // proposal refused: plain step, same iteration
         conv = (double)acc_e.fmax < a.tol;
         if (conv || it >= a.max_iter || h < 1e-12) break;
+        kind = learned ? 1 : 2;
     }
     if (cur == 1) {   // the accepted state goes back to buffer 0
         __syncthreads();
@@ -660,6 +838,9 @@ __device__ __forceinline__ void minimize_system(const BatchArgs& a, const SysMet
         out->accepted = nacc;
         out->converged = conv ? 1 : 0;
         out->cur = 0;
+        out->evaluations = nevals;
+        out->learned_accepted = nlearn;
+        out->rec_iters = nrec;
         for (int q = 0; q < 5; q++) out->ph[q] = sh.ph[q];
     }
 }
@@ -672,8 +853,8 @@ __global__ __launch_bounds__(T) void k_em_batch(BatchArgs a) {
     const SysMeta m = a.meta[s];
     SysOut* out = a.out + s;
     if (threadIdx.x == 0) out->t0 = __builtin_amdgcn_s_memrealtime();
-    if (m.multi) minimize_system<T, UNR, true>(a, m, out, sh, lds);
-    else minimize_system<T, UNR, false>(a, m, out, sh, lds);
+    if (m.multi) minimize_system<T, UNR, true>(a, m, s, out, sh, lds);
+    else minimize_system<T, UNR, false>(a, m, s, out, sh, lds);
     __syncthreads();
     if (threadIdx.x == 0) out->t1 = __builtin_amdgcn_s_memrealtime();
 }
@@ -752,6 +933,13 @@ struct Batch {
     float4* part = nullptr;
     float2* twid = nullptr;
     SysOut* out = nullptr;
+    unsigned short* feat = nullptr;   // [N][32] BF16 features (learned and recording modes)
+    float* scale = nullptr;           // [N]
+    float* rec = nullptr;             // recorded states
+    int2* rec_info = nullptr;         // per system (offset, sampled atoms)
+    int64_t rec_floats = 0;
+    int rec_stride = 0, rec_cap = 0;
+    std::vector<int2> h_rec_info;
     hipStream_t streams[NSHAPES] = {};
     lima_batch_tuning tuning{};
     bool tuned = false;
@@ -763,7 +951,8 @@ namespace {
 void batch_free(Batch& b) {
     for (auto& s : b.streams) if (s) { (void)hipStreamSynchronize(s); (void)hipStreamDestroy(s); s = nullptr; }
     void* ptrs[] = {b.d_meta, b.d_order, b.u0, b.u1, b.u_start, b.f0, b.f1, b.prm4, b.comp, b.cinfo, b.cbm, b.clut, b.inc_off, b.inc,
-                    b.bid, b.bp, b.aid, b.ap, b.did, b.dp, b.iid, b.ip, b.pid, b.pp, b.green, b.spec, b.part, b.twid, b.out};
+                    b.bid, b.bp, b.aid, b.ap, b.did, b.dp, b.iid, b.ip, b.pid, b.pp, b.green, b.spec, b.part, b.twid, b.out,
+                    b.feat, b.scale, b.rec, b.rec_info};
     for (void* p : ptrs) if (p) (void)hipFree(p);
 }
 
@@ -1040,7 +1229,21 @@ BatchArgs make_args(const Batch& B, double tol_j, int max_iter, double h0) {
     a.green = B.green; a.spec = B.spec; a.part = B.part; a.twid = B.twid; a.out = B.out;
     a.pc = B.pc; a.rc = B.rc; a.rc2 = B.rc * B.rc;
     a.tol = tol_j; a.h0 = h0; a.max_iter = max_iter;
+    a.mode = MODE_PLAIN;
+    a.feat = B.feat; a.scale = B.scale; a.feat_interval = 1;
+    a.rec = B.rec; a.rec_info = B.rec_info; a.rec_stride = B.rec_stride; a.rec_cap = B.rec_cap;
     return a;
+}
+
+void ensure_features(Batch& B) {
+    if (!B.feat) {
+        HIP_CHECK(hipMalloc(&B.feat, B.N * 32 * sizeof(unsigned short)));
+        HIP_CHECK(hipMemset(B.feat, 0, B.N * 32 * sizeof(unsigned short)));
+    }
+    if (!B.scale) HIP_CHECK(hipMalloc(&B.scale, B.N * sizeof(float)));
+    for (int s = 0; s < B.nsys; s++)
+        if (B.meta[s].n > HIST_MAX_ATOMS)
+            throw LimaFail(LIMA_ERR_BAD_INPUT, "learned proposals and recording support systems of up to 4096 atoms");
 }
 
 // Launch the given systems, grouped by shape (one launch per shape, each on its own stream,
@@ -1060,7 +1263,11 @@ void launch_groups(Batch& B, const std::vector<int>& systems, const std::vector<
         const int cnt = ranges[sh].second;
         if (cnt == 0) continue;
         int lds = SHAPES[sh].threads / 64 * 192 * (int)sizeof(float);
-        for (int k = 0; k < cnt; k++) lds = std::max(lds, B.meta[order[ranges[sh].first + k]].lds_bytes);
+        for (int k = 0; k < cnt; k++) {
+            const SysMeta& m = B.meta[order[ranges[sh].first + k]];
+            lds = std::max(lds, m.lds_bytes);
+            if (a.mode != MODE_PLAIN) lds = std::max(lds, m.n * 28);   // histogram byte counters
+        }
         BatchArgs ag = a;
         ag.order = B.d_order + ranges[sh].first;
         KernelFn fn = shape_kernel(sh);
@@ -1222,6 +1429,165 @@ lima_status lima_batch_minimize(lima_batch* h, double fmax_tol_kj, int64_t max_i
         }
         if (wall_seconds) *wall_seconds = wall;
         if (bad) throw LimaFail(LIMA_ERR_NONFINITE, "non-finite energy in batched minimisation");
+    });
+}
+
+lima_status lima_batch_minimize_learned(lima_batch* h, lima_mlp* model, double fmax_tol_kj, int64_t max_iter, double initial_step_nm,
+                                        int32_t feature_interval, lima_em_result* results, double* wall_seconds) {
+    if (!h || !model) { lima::g_err = "null batch or model"; return LIMA_ERR_BAD_INPUT; }
+    if (max_iter < 0 || max_iter > (1ll << 30) || feature_interval < 1) {
+        lima::g_err = "max_iter out of range or feature_interval < 1";
+        return LIMA_ERR_BAD_INPUT;
+    }
+    return bguard([&] {
+        using namespace lima;
+        Batch& B = h->b;
+        ensure_features(B);
+        std::vector<int> all(B.nsys);
+        for (int s = 0; s < B.nsys; s++) all[s] = s;
+        const std::vector<int> shape_of = shapes_from_tuning(B);
+        BatchArgs ag = make_args(B, fmax_tol_kj * 1000.0, (int)max_iter, initial_step_nm);
+        ag.mode = MODE_LEARNED;
+        ag.mlp = mlp_view(model);
+        ag.feat_interval = feature_interval;
+        const auto t0 = std::chrono::steady_clock::now();
+        launch_groups(B, all, shape_of, ag);
+        const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        std::vector<SysOut> res(B.nsys);
+        HIP_CHECK(hipMemcpy(res.data(), B.out, B.nsys * sizeof(SysOut), hipMemcpyDeviceToHost));
+        B.forces_valid = true;
+        bool bad = false;
+        for (int s = 0; s < B.nsys; s++) {
+            if (!std::isfinite(res[s].e_acc)) bad = true;
+            if (results) {
+                results[s].iterations = res[s].iterations;
+                results[s].accepted = res[s].accepted;
+                results[s].final_energy = res[s].e_acc;
+                results[s].final_fmax = (double)res[s].fmax / 1000.0;
+                results[s].converged = res[s].converged;
+                results[s].wall_seconds = (double)(res[s].t1 - res[s].t0) * 1e-8;
+            }
+        }
+        if (wall_seconds) *wall_seconds = wall;
+        if (bad) throw LimaFail(LIMA_ERR_NONFINITE, "non-finite energy in batched minimisation");
+    });
+}
+
+lima_status lima_batch_record(lima_batch* h, double fmax_tol_kj, int64_t max_iter, double initial_step_nm, int32_t atom_stride,
+                              int32_t max_states, int64_t* n_states) {
+    if (!h) { lima::g_err = "null batch"; return LIMA_ERR_BAD_INPUT; }
+    if (max_iter < 0 || max_iter > (1ll << 30) || atom_stride < 1 || max_states < 1) {
+        lima::g_err = "bad recording parameters";
+        return LIMA_ERR_BAD_INPUT;
+    }
+    return bguard([&] {
+        using namespace lima;
+        Batch& B = h->b;
+        ensure_features(B);
+        // per system: sampled atoms 0, stride, 2 stride, ...; max_states recorded accepted states
+        B.h_rec_info.resize(B.nsys);
+        int64_t off = 0;
+        for (int s = 0; s < B.nsys; s++) {
+            const int ns = (B.meta[s].n + atom_stride - 1) / atom_stride;
+            B.h_rec_info[s] = make_int2((int)off, ns);
+            off += (int64_t)ns * max_states * REC_W;
+            if (off >= (1ll << 31)) throw LimaFail(LIMA_ERR_BAD_INPUT, "recording buffer exceeds 2^31 floats; raise the stride");
+        }
+        if (B.rec) { (void)hipFree(B.rec); B.rec = nullptr; }
+        if (B.rec_info) { (void)hipFree(B.rec_info); B.rec_info = nullptr; }
+        HIP_CHECK(hipMalloc(&B.rec, std::max<int64_t>(off, 1) * sizeof(float)));
+        B.rec_info = dcopy(B.h_rec_info);
+        B.rec_floats = off;
+        B.rec_stride = atom_stride;
+        B.rec_cap = max_states;
+        std::vector<int> all(B.nsys);
+        for (int s = 0; s < B.nsys; s++) all[s] = s;
+        const std::vector<int> shape_of = shapes_from_tuning(B);
+        BatchArgs ag = make_args(B, fmax_tol_kj * 1000.0, (int)max_iter, initial_step_nm);
+        ag.mode = MODE_RECORD;
+        launch_groups(B, all, shape_of, ag);
+        std::vector<SysOut> res(B.nsys);
+        HIP_CHECK(hipMemcpy(res.data(), B.out, B.nsys * sizeof(SysOut), hipMemcpyDeviceToHost));
+        B.forces_valid = true;
+        int64_t tot = 0;
+        for (int s = 0; s < B.nsys; s++) tot += res[s].rec_iters;
+        if (n_states) *n_states = tot;
+    });
+}
+
+lima_status lima_batch_training_set(lima_batch* h, int32_t lookahead, float* features, float* targets, int64_t capacity,
+                                    int64_t* n_samples) {
+    if (!h || !n_samples || lookahead < 1) { lima::g_err = "bad argument"; return LIMA_ERR_BAD_INPUT; }
+    return bguard([&] {
+        using namespace lima;
+        Batch& B = h->b;
+        if (!B.rec) throw LimaFail(LIMA_ERR_BAD_INPUT, "nothing recorded: call lima_batch_record first");
+        std::vector<SysOut> res(B.nsys);
+        HIP_CHECK(hipMemcpy(res.data(), B.out, B.nsys * sizeof(SysOut), hipMemcpyDeviceToHost));
+        std::vector<float> rec(B.rec_floats);
+        HIP_CHECK(hipMemcpy(rec.data(), B.rec, B.rec_floats * sizeof(float), hipMemcpyDeviceToHost));
+        // target: displacement over the next `lookahead` accepted states, projected on the plain step
+        // d taken from this state, in units of |d|; clipped to the model's range (0, 4)
+        int64_t cnt = 0;
+        for (int s = 0; s < B.nsys; s++) {
+            const int2 ri = B.h_rec_info[s];
+            const int nst = res[s].rec_iters;
+            for (int st = 0; st + lookahead < nst; st++)
+                for (int q = 0; q < ri.y; q++) {
+                    const float* e0 = &rec[ri.x + ((int64_t)st * ri.y + q) * REC_W];
+                    const float* e1 = &rec[ri.x + ((int64_t)(st + lookahead) * ri.y + q) * REC_W];
+                    const double d[3] = {e0[32], e0[33], e0[34]};
+                    const double dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+                    if (dd < 1e-14) continue;   // atoms the step does not move carry no signal
+                    double proj = 0.0;
+                    for (int c = 0; c < 3; c++) {
+                        int32_t u0, u1;
+                        std::memcpy(&u0, &e0[35 + c], 4);
+                        std::memcpy(&u1, &e1[35 + c], 4);
+                        proj += (double)(u1 - u0) * 1e-8 * d[c];
+                    }
+                    double t = proj / dd;
+                    t = std::min(3.95, std::max(0.05, t));
+                    if (features && targets && cnt < capacity) {
+                        std::memcpy(&features[cnt * 32], e0, 32 * sizeof(float));
+                        targets[cnt] = (float)t;
+                    }
+                    cnt++;
+                }
+        }
+        *n_samples = cnt;
+    });
+}
+
+lima_status lima_batch_get_features(lima_batch* h, int32_t system, float* out) {
+    if (!h || !out) { lima::g_err = "null argument"; return LIMA_ERR_BAD_INPUT; }
+    if (system < 0 || system >= h->b.nsys) { lima::g_err = "system index out of range"; return LIMA_ERR_BAD_INPUT; }
+    return bguard([&] {
+        using namespace lima;
+        Batch& B = h->b;
+        if (!B.feat) throw LimaFail(LIMA_ERR_BAD_INPUT, "no features computed yet");
+        const SysMeta& m = B.meta[system];
+        std::vector<unsigned short> f((size_t)m.n * 32);
+        HIP_CHECK(hipMemcpy(f.data(), B.feat + (int64_t)m.atom0 * 32, f.size() * sizeof(unsigned short), hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < f.size(); k++) {
+            const uint32_t bits = (uint32_t)f[k] << 16;
+            std::memcpy(&out[k], &bits, 4);
+        }
+    });
+}
+
+lima_status lima_batch_get_counters(lima_batch* h, int64_t* out) {
+    if (!h || !out) { lima::g_err = "null argument"; return LIMA_ERR_BAD_INPUT; }
+    return bguard([&] {
+        using namespace lima;
+        Batch& B = h->b;
+        std::vector<SysOut> res(B.nsys);
+        HIP_CHECK(hipMemcpy(res.data(), B.out, B.nsys * sizeof(SysOut), hipMemcpyDeviceToHost));
+        for (int s = 0; s < B.nsys; s++) {
+            out[s * 3 + 0] = res[s].evaluations;
+            out[s * 3 + 1] = res[s].learned_accepted;
+            out[s * 3 + 2] = res[s].rec_iters;
+        }
     });
 }
 

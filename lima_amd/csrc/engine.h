@@ -178,7 +178,7 @@ struct Engine {
 #define HIP_CHECK(x)                                                                          \
     do {                                                                                      \
         hipError_t e_ = (x);                                                                  \
-        if (e_ != hipSuccess) throw HipError(std::string(#x) + ": " + hipGetErrorString(e_)); \
+        if (e_ != hipSuccess) throw ::lima::HipError(std::string(#x) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
 extern thread_local std::string g_err;   // lima_last_error() text, set by every failing C-ABI call
