@@ -58,6 +58,83 @@ def calibrate_copy(repeats=5):
     return 0
 
 
+def em_bench(args):
+    """Time to Fmax < 100 kJ/mol/nm for the batch of independent systems: plain steepest descent,
+    then learned proposals (model trained on plain trajectories of fresh systems), same start."""
+    import numpy as np
+    import lima_amd as L
+    t = time.time()
+    sizes = L.em_batch_sizes(args.em_systems, seed=1000)
+    boxes = [L.gen_em_system(int(n), 1000 + k, 0.02) for k, n in enumerate(sizes)]
+    natoms = int(sum(b.n_particles for b in boxes))
+    log(f"em: generated {len(boxes)} systems ({natoms} atoms) in {time.time() - t:.1f}s")
+    p = L.default_params()
+    plain = L.Batch(boxes, p)
+    tune = plain.autotune(3)
+    log(f"em: autotuned shapes {tune.as_list()}")
+    res_p, wall_p = plain.minimize(100.0, args.em_max_iter, 0.01)
+    cnt_p = plain.counters()
+    log(f"em: plain {wall_p:.3f}s, {sum(r.iterations for r in res_p)} iterations")
+    # training data: plain trajectories of fresh systems
+    t = time.time()
+    tsizes = L.em_batch_sizes(args.em_train_systems, seed=2000)
+    tboxes = [L.gen_em_system(int(n), 2000 + k, 0.02) for k, n in enumerate(tsizes)]
+    tb = L.Batch(tboxes, p)
+    tb.set_tuning(tune)
+    states = tb.record(100.0, args.em_max_iter, 0.01, atom_stride=args.em_stride, max_states=4000)
+    x, tgt = tb.training_set(lookahead=1)
+    rec_s = time.time() - t
+    model = L.Mlp(seed=1)
+    losses, train_s = model.train(x, tgt, epochs=args.em_epochs, batch=1024, lr=1e-3, seed=3)
+    log(f"em: {states} recorded states, {len(tgt)} samples ({rec_s:.1f}s), trained {args.em_epochs} epochs in "
+        f"{train_s:.2f}s, loss {losses[0]:.4f} -> {losses[-1]:.4f}")
+    learned = L.Batch(boxes, p)
+    learned.set_tuning(tune)
+    res_l, wall_l = learned.minimize_learned(model, 100.0, args.em_max_iter, 0.01, feature_interval=4,
+                                            grow_on_learned=True)
+    cnt_l = learned.counters()
+    log(f"em: learned {wall_l:.3f}s, {sum(r.iterations for r in res_l)} iterations")
+    # control: the same learned path with a model whose output is a constant scale, which makes every
+    # proposal the plain step; it separates the path's overhead from the effect of the trained scales
+    const = L.Mlp(seed=0)
+    prm = np.zeros(const.n_params, np.float32)
+    const.set_params(prm)     # 4 sigmoid(0) = 2 everywhere
+    control = L.Batch(boxes, p)
+    control.set_tuning(tune)
+    res_c, wall_c = control.minimize_learned(const, 100.0, args.em_max_iter, 0.01, feature_interval=4,
+                                              grow_on_learned=True)
+    cnt_c = control.counters()
+    log(f"em: constant-scale control {wall_c:.3f}s, {sum(r.iterations for r in res_c)} iterations")
+    ep = np.array([r.final_energy for r in res_p])
+    el = np.array([r.final_energy for r in res_l])
+    rel = (el - ep) / np.abs(ep)
+
+    def side(res, wall, cnt):
+        return {"seconds": round(wall, 4), "converged": int(sum(r.converged for r in res)),
+                "iterations_total": int(sum(r.iterations for r in res)),
+                "iterations_max": int(max(r.iterations for r in res)),
+                "force_evaluations_total": int(cnt[:, 0].sum()),
+                "slowest_system_seconds": round(max(r.wall_seconds for r in res), 4)}
+
+    return {
+        "workload": f"{len(boxes)} chain+water systems, 300-3000 atoms log-uniform (seeds 1000..{999 + len(boxes)}), "
+                    f"{natoms} atoms, 0.02 nm noise, Fmax < 100 kJ/mol/nm, h0 0.01 nm",
+        "plain": side(res_p, wall_p, cnt_p),
+        "learned": dict(side(res_l, wall_l, cnt_l), learned_proposals_accepted=int(cnt_l[:, 1].sum())),
+        "constant_scale_control": dict(side(res_c, wall_c, cnt_c), learned_proposals_accepted=int(cnt_c[:, 1].sum())),
+        "speedup_time_to_fmax": round(wall_p / wall_l, 3) if wall_l > 0 else None,
+        "final_energy_total_rel_diff": float(f"{(el.sum() - ep.sum()) / abs(ep.sum()):.3e}"),
+        "final_energy_per_system": {"median_abs_rel_diff": float(f"{np.median(np.abs(rel)):.3e}"),
+                                    "within_0.1pct": int(np.sum(np.abs(rel) < 1e-3)),
+                                    "learned_lower": int(np.sum(el < ep))},
+        "training": {"systems": len(tboxes), "seeds": f"2000..{1999 + len(tboxes)}", "recorded_states": int(states),
+                     "samples": int(len(tgt)), "atom_stride": args.em_stride, "target": "secant step estimate to the next accepted state, log-mapped onto (0.05, 3.95)", "epochs": args.em_epochs,
+                     "record_seconds": round(rec_s, 2), "train_seconds": round(train_s, 3),
+                     "loss_first_last": [round(float(losses[0]), 5), round(float(losses[-1]), 5)]},
+        "autotuned_shapes": tune.as_list(),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,6 +147,12 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run the copy kernel used to calibrate the HBM byte counters")
+    ap.add_argument("--no-em", action="store_true", help="skip the batched-minimisation measurement")
+    ap.add_argument("--em-systems", type=int, default=256)
+    ap.add_argument("--em-train-systems", type=int, default=100)
+    ap.add_argument("--em-max-iter", type=int, default=20000)
+    ap.add_argument("--em-stride", type=int, default=64, help="record every k-th atom for training")
+    ap.add_argument("--em-epochs", type=int, default=4)
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -161,6 +244,10 @@ def main():
                              f"({dtc:.1f}s, {dtc / args.cpu_steps * 1e3:.0f} ms/step)"}
             log(f"cpu baseline: {cpu}")
 
+    em = None
+    if not args.no_em:
+        em = em_bench(args)
+
     out = {
         "metric": "ns/day (42k-atom box); time to Fmax<100 kJ/mol/nm, learned vs plain EM",
         "value": round(ns_day, 3),
@@ -187,6 +274,7 @@ def main():
         "list_rebuilds": stt.n_list_rebuilds,
         "list_stale_intervals": stt.list_violations,
         "final_energy_kj": round(e_end.total / 1000.0, 3),
+        "em_batch": em,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -256,12 +256,13 @@ lima_status lima_batch_minimize(lima_batch* batch, double fmax_tol_kj, int64_t m
 lima_status lima_batch_get_state(lima_batch* batch, int32_t system, lima_state_out* out);
 /* Restores the positions every system had at lima_batch_create. */
 lima_status lima_batch_reset(lima_batch* batch);
-/* ---- Learned step proposals.  A per-atom MLP (32 features: F / Fmax, |F| / Fmax, and 28 counts of
+/* ---- Learned step proposals.  A per-atom MLP (32 features: F / Fmax, 1 + log10(|F| / Fmax) / 6, and 28 counts of
  * neighbours within 1.0 nm in 14 distance bins per charge sign; two SiLU layers of 64; output a
- * step scale s in (0, 4)) proposes x' = x + h s_i F_i / Fmax.  The proposal is kept only if the FP64
- * energy decreases; otherwise the plain steepest-descent step is taken, so every accepted state is
- * one the plain method would also accept.  No counterpart in LIMA; the minimisation is still
- * Programs::EnergyMinimize's (Programs.cpp:155-199). ---- */
+ * step scale s in (0, 4)) proposes x' = x + h s_i F_i / max_j (s_j |F_j|): a per-atom preconditioned
+ * step whose largest displacement is h, as for the plain step (a constant s gives the plain step).
+ * The proposal is kept only if the FP64 energy decreases; otherwise the plain steepest-descent step
+ * is taken.  No counterpart in LIMA; the minimisation is still Programs::EnergyMinimize's
+ * (Programs.cpp:155-199). ---- */
 typedef struct lima_mlp lima_mlp;
 int32_t lima_mlp_n_params(void);
 lima_status lima_mlp_create(uint64_t seed, lima_mlp** out);
@@ -280,17 +281,21 @@ lima_status lima_mlp_train(lima_mlp* model, const float* features, const float* 
 lima_status lima_mlp_gradient(lima_mlp* model, const float* features, const float* targets, int64_t n, float* grad_out,
                               double* loss);
 /* Batched minimisation with learned proposals; the step-scale features' neighbour histograms are
- * refreshed every feature_interval iterations.  Systems of up to 4096 atoms. */
+ * refreshed every feature_interval iterations; grow_on_learned != 0 grows h by 1.2 after an accepted
+ * proposal as after an accepted plain step (0 keeps h, so h tracks the plain step alone).
+ * Systems of up to 4096 atoms. */
 lima_status lima_batch_minimize_learned(lima_batch* batch, lima_mlp* model, double fmax_tol_kj, int64_t max_iter,
-                                        double initial_step_nm, int32_t feature_interval, lima_em_result* results,
-                                        double* wall_seconds);
+                                        double initial_step_nm, int32_t feature_interval, int32_t grow_on_learned,
+                                        lima_em_result* results, double* wall_seconds);
 /* Plain batched minimisation that records, at every accepted state, the features, the plain step
  * and the Coord of every atom_stride-th atom (up to max_states states per system). */
 lima_status lima_batch_record(lima_batch* batch, double fmax_tol_kj, int64_t max_iter, double initial_step_nm,
                               int32_t atom_stride, int32_t max_states, int64_t* n_states);
-/* Training samples from the recording: features [n][32] and the target step scale: the atom's
- * displacement over the next `lookahead` accepted states projected on its plain step, in units of
- * that step, clipped to [0.05, 3.95].  With features == NULL only *n_samples is set. */
+/* Training samples from the recording: features [n][32] and the target step scale.  rho, the secant
+ * estimate of the atom's best step along its displacement D to the state `lookahead` accepted states
+ * later in units of its plain step (|D| f / (f - f') / |d|, f and f' the force components along D
+ * before and after), mapped log-linearly from [0.1, 1000] onto [0.05, 3.95].  With features == NULL
+ * only *n_samples is set. */
 lima_status lima_batch_training_set(lima_batch* batch, int32_t lookahead, float* features, float* targets, int64_t capacity,
                                     int64_t* n_samples);
 /* Feature rows [n_atoms][32] of one system as last computed (slots 4..31: neighbour histogram,

@@ -90,8 +90,8 @@ def load_hip():
             "lima_batch_get_state": ([vp, C.c_int32, C.POINTER(StateOut)], C.c_int),
             "lima_batch_reset": ([vp], C.c_int),
             "lima_batch_get_phase_times": ([vp, C.POINTER(C.c_double)], C.c_int),
-            "lima_batch_minimize_learned": ([vp, vp, C.c_double, C.c_int64, C.c_double, C.c_int32, C.POINTER(EmResult),
-                                             C.POINTER(C.c_double)], C.c_int),
+            "lima_batch_minimize_learned": ([vp, vp, C.c_double, C.c_int64, C.c_double, C.c_int32, C.c_int32,
+                                             C.POINTER(EmResult), C.POINTER(C.c_double)], C.c_int),
             "lima_batch_record": ([vp, C.c_double, C.c_int64, C.c_double, C.c_int32, C.c_int32, C.POINTER(C.c_int64)], C.c_int),
             "lima_batch_training_set": ([vp, C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int64,
                                          C.POINTER(C.c_int64)], C.c_int),
@@ -374,12 +374,16 @@ class Batch:
     def reset(self):
         self._check(self.lib.lima_batch_reset(self._h))
 
-    def minimize_learned(self, model, fmax_tol_kj=100.0, max_iter=10000, initial_step_nm=0.01, feature_interval=4):
-        """Learned proposals with the plain step as fallback.  Returns (results, wall seconds)."""
+    def minimize_learned(self, model, fmax_tol_kj=100.0, max_iter=10000, initial_step_nm=0.01, feature_interval=4,
+                         grow_on_learned=True):
+        """Learned proposals with the plain step as fallback.  Returns (results, wall seconds).
+
+        With grow_on_learned (the default) h grows by 1.2 after every accepted step, learned or plain,
+        so a model with a constant scale reproduces the plain minimiser step for step."""
         res = (EmResult * len(self.boxes))()
         wall = C.c_double()
         self._check(self.lib.lima_batch_minimize_learned(self._h, model._h, fmax_tol_kj, max_iter, initial_step_nm,
-                                                         feature_interval, res, C.byref(wall)))
+                                                         feature_interval, 1 if grow_on_learned else 0, res, C.byref(wall)))
         return list(res), wall.value
 
     def record(self, fmax_tol_kj=100.0, max_iter=10000, initial_step_nm=0.01, atom_stride=16, max_states=2000):
@@ -388,7 +392,7 @@ class Batch:
                                                C.byref(n)))
         return n.value
 
-    def training_set(self, lookahead=4):
+    def training_set(self, lookahead=1):
         n = C.c_int64()
         self._check(self.lib.lima_batch_training_set(self._h, lookahead, None, None, 0, C.byref(n)))
         x = np.zeros((n.value, 32), np.float32)

@@ -115,10 +115,11 @@ struct BatchArgs {
     float* rec;                  // recording: [rec_info.x + (state * nsamp + k) * REC_W]
     const int2* rec_info;        // per system (offset in floats, sampled atoms)
     int rec_stride, rec_cap;     // sampled atoms li = k * rec_stride; states per system
+    int grow_on_learned;         // learned mode: h *= 1.2 after an accepted proposal (else h is kept)
 };
 
 enum { MODE_PLAIN = 0, MODE_LEARNED = 1, MODE_RECORD = 2 };
-constexpr int REC_W = 40;              // floats per record: 32 features, step (3), Coord (3), pad
+constexpr int REC_W = 48;              // floats per record: 32 features, step (3), Coord (3), force (3), pad
 constexpr float HIST_R = 1.0f;         // neighbour histogram radius [nm]
 constexpr int HIST_BINS = 14;          // bins per charge sign
 constexpr int HIST_MAX_ATOMS = 4096;   // byte counters in LDS: 28 per atom
@@ -399,31 +400,47 @@ __device__ __forceinline__ void hist_pass(const BatchArgs& a, const SysMeta& m, 
     __syncthreads();
 }
 
-// Learned step scale of every atom: features = (F / Fmax, |F| / Fmax, histogram), one MFMA forward
-// per 16 atoms per wave.
+// |F| / Fmax on a log scale, 1 + log10(max(r, 1e-6)) / 6 in [0, 1]: the plain step of an atom is
+// proportional to it, so its optimal relative step spans decades.
+__device__ __forceinline__ float force_feature(float r) { return 1.f + log10f(fmaxf(r, 1e-6f)) * (1.f / 6.f); }
+
+// Learned step scale of every atom: features = (F / Fmax, log |F| / Fmax, histogram), one MFMA forward
+// per 16 atoms per wave.  Returns S = max_i s_i |F_i| (every thread), the normaliser of the proposal.
 template <int T>
-__device__ __forceinline__ void infer_scales(const BatchArgs& a, const SysMeta& m, const float4* F, float fmax) {
+__device__ __forceinline__ float infer_scales(const BatchArgs& a, const SysMeta& m, const float4* F, float fmax, BlockShared<T>& sh) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
     constexpr int W = T / 64;
     const float inv = 1.f / fmax;
     const bf16x4 zero = pack4(0.f, 0.f, 0.f, 0.f);
+    float smax = 0.f;
     for (int base = wave * 16; base < m.n; base += W * 16) {
         const int li = base + r;
         const bool ok = li < m.n;
         const unsigned short* fr = a.feat + (int64_t)(m.atom0 + (ok ? li : 0)) * 32;
+        const float4 f = ok ? F[m.atom0 + li] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const float fl = sqrtf(f.x * f.x + f.y * f.y + f.z * f.z);
         bf16x4 xb[2];
-        if (g == 0) {
-            const float4 f = ok ? F[m.atom0 + li] : make_float4(0.f, 0.f, 0.f, 0.f);
-            xb[0] = pack4(f.x * inv, f.y * inv, f.z * inv, sqrtf(f.x * f.x + f.y * f.y + f.z * f.z) * inv);
-        } else {
-            xb[0] = ok ? *reinterpret_cast<const bf16x4*>(fr + 4 * g) : zero;
-        }
+        if (g == 0) xb[0] = pack4(f.x * inv, f.y * inv, f.z * inv, force_feature(fl * inv));
+        else xb[0] = ok ? *reinterpret_cast<const bf16x4*>(fr + 4 * g) : zero;
         xb[1] = ok ? *reinterpret_cast<const bf16x4*>(fr + 16 + 4 * g) : zero;
         f32x4 z1[4], z2[4];
         bf16x4 h1[4];
-        const float z3 = mlp_forward16(a.mlp, xb, z1, h1, z2);
-        if (g == 0 && ok) a.scale[m.atom0 + li] = mlp_scale(z3);
+        const float sc = mlp_scale(mlp_forward16(a.mlp, xb, z1, h1, z2));
+        if (g == 0 && ok) {
+            a.scale[m.atom0 + li] = sc;
+            smax = fmaxf(smax, sc * fl);
+        }
     }
+    smax = wave_max(smax);
+    if (lane == 0) sh.fred[wave] = smax;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float v = 0.f;
+        for (int w = 0; w < W; w++) v = fmaxf(v, sh.fred[w]);
+        sh.fres = v;
+    }
+    __syncthreads();
+    return sh.fres;
 }
 
 struct Eval {
@@ -738,16 +755,18 @@ __device__ __forceinline__ void record_state(const BatchArgs& a, const SysMeta& 
         float* e = a.rec + ri.x + ((int64_t)state * ri.y + q) * REC_W;
         const float4 f = F[m.atom0 + li];
         const int4 x = u[m.atom0 + li];
-        e[0] = f.x * inv; e[1] = f.y * inv; e[2] = f.z * inv; e[3] = sqrtf(f.x * f.x + f.y * f.y + f.z * f.z) * inv;
+        e[0] = f.x * inv; e[1] = f.y * inv; e[2] = f.z * inv; e[3] = force_feature(sqrtf(f.x * f.x + f.y * f.y + f.z * f.z) * inv);
         const unsigned short* fr = a.feat + (int64_t)(m.atom0 + li) * 32;
         for (int b = 4; b < 32; b++) e[b] = bf2f(fr[b]);
         e[32] = k * f.x; e[33] = k * f.y; e[34] = k * f.z;
         e[35] = __int_as_float(x.x); e[36] = __int_as_float(x.y); e[37] = __int_as_float(x.z);
-        e[38] = 0.f; e[39] = 0.f;
+        e[38] = f.x; e[39] = f.y; e[40] = f.z;
+        for (int b = 41; b < REC_W; b++) e[b] = 0.f;
     }
 }
 
-// Trial step from the accepted state: x' = x + trunc(h s F / Fmax), s = 1 (plain) or the learned scale.
+// Trial step from the accepted state: x' = x + trunc(h s F / norm), s = 1 (plain, norm = Fmax) or the
+// learned scale (norm = max s |F|).
 template <int T>
 __device__ __forceinline__ void trial_step(const SysMeta& m, const int4* uc, const float4* fc, int4* ut, const float* scale, double h,
                                            float fmax) {
@@ -773,6 +792,7 @@ __device__ __forceinline__ void minimize_system(const BatchArgs& a, const SysMet
     int cur = 0, it = 0, nacc = 0, nevals = 0, nlearn = 0, nrec = 0;
     // pass kinds: 0 the starting state, 1 a learned proposal, 2 the plain step
     int kind = 0;
+    float snorm = 1.f;
     bool conv = false;
     for (;;) {
         const int tgt = kind == 0 ? 0 : 1 - cur;
@@ -783,11 +803,11 @@ __device__ __forceinline__ void minimize_system(const BatchArgs& a, const SysMet
             const float4* fc = (cur ? a.f1 : a.f0) + m.atom0;
             if (kind == 1) {
                 if (it % a.feat_interval == 0) hist_pass<T, MULTI>(a, m, lds, cur ? a.u1 : a.u0);
-                infer_scales<T>(a, m, cur ? a.f1 : a.f0, acc_e.fmax);
-                __syncthreads();
+                snorm = infer_scales<T>(a, m, cur ? a.f1 : a.f0, acc_e.fmax, sh);
             }
             const unsigned long long ts = tid == 0 ? __builtin_amdgcn_s_memrealtime() : 0ull;
-            trial_step<T>(m, uc, fc, ut + m.atom0, kind == 1 ? a.scale + m.atom0 : nullptr, h, acc_e.fmax);
+            // plain: h F / Fmax; learned: h s F / max(s |F|), the same largest displacement h
+            trial_step<T>(m, uc, fc, ut + m.atom0, kind == 1 ? a.scale + m.atom0 : nullptr, h, kind == 1 ? snorm : acc_e.fmax);
             __syncthreads();
             if (tid == 0) sh.ph[4] += __builtin_amdgcn_s_memrealtime() - ts;
         }
@@ -807,7 +827,12 @@ __device__ __forceinline__ void minimize_system(const BatchArgs& a, const SysMet
             step_done = true;
         }
         if (kind != 0 && step_done) {
-            if (accepted) { h *= 1.2; nacc++; } else { h *= 0.2; }
+            if (accepted) {
+                if (kind == 2 || a.grow_on_learned) h *= 1.2;
+                nacc++;
+            } else {
+                h *= 0.2;
+            }
             it++;
         }
         if (record && accepted && nrec < a.rec_cap) {
@@ -1433,7 +1458,8 @@ lima_status lima_batch_minimize(lima_batch* h, double fmax_tol_kj, int64_t max_i
 }
 
 lima_status lima_batch_minimize_learned(lima_batch* h, lima_mlp* model, double fmax_tol_kj, int64_t max_iter, double initial_step_nm,
-                                        int32_t feature_interval, lima_em_result* results, double* wall_seconds) {
+                                        int32_t feature_interval, int32_t grow_on_learned, lima_em_result* results,
+                                        double* wall_seconds) {
     if (!h || !model) { lima::g_err = "null batch or model"; return LIMA_ERR_BAD_INPUT; }
     if (max_iter < 0 || max_iter > (1ll << 30) || feature_interval < 1) {
         lima::g_err = "max_iter out of range or feature_interval < 1";
@@ -1450,6 +1476,7 @@ lima_status lima_batch_minimize_learned(lima_batch* h, lima_mlp* model, double f
         ag.mode = MODE_LEARNED;
         ag.mlp = mlp_view(model);
         ag.feat_interval = feature_interval;
+        ag.grow_on_learned = grow_on_learned ? 1 : 0;
         const auto t0 = std::chrono::steady_clock::now();
         launch_groups(B, all, shape_of, ag);
         const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -1524,10 +1551,20 @@ lima_status lima_batch_training_set(lima_batch* h, int32_t lookahead, float* fea
         if (!B.rec) throw LimaFail(LIMA_ERR_BAD_INPUT, "nothing recorded: call lima_batch_record first");
         std::vector<SysOut> res(B.nsys);
         HIP_CHECK(hipMemcpy(res.data(), B.out, B.nsys * sizeof(SysOut), hipMemcpyDeviceToHost));
+        // copy only the recorded states of each system
         std::vector<float> rec(B.rec_floats);
-        HIP_CHECK(hipMemcpy(rec.data(), B.rec, B.rec_floats * sizeof(float), hipMemcpyDeviceToHost));
-        // target: displacement over the next `lookahead` accepted states, projected on the plain step
-        // d taken from this state, in units of |d|; clipped to the model's range (0, 4)
+        for (int s = 0; s < B.nsys; s++) {
+            const int2 ri = B.h_rec_info[s];
+            const int64_t used = (int64_t)res[s].rec_iters * ri.y * REC_W;
+            if (used > 0) HIP_CHECK(hipMemcpy(rec.data() + ri.x, B.rec + ri.x, used * sizeof(float), hipMemcpyDeviceToHost));
+        }
+        // target: the secant estimate of the atom's best step along its actual displacement D from
+        // this accepted state to the one `lookahead` accepted states later.  With f, f' the force
+        // components along D before and after, the force along D vanishes at |D| f / (f - f'); in
+        // units of the atom's plain step |d| that is the ratio rho (soft atoms with small forces:
+        // rho >> 1; stiff or overshooting ones: rho < 1).  rho in [0.1, 1000] maps log-linearly onto
+        // the model's range [0.05, 3.95] (s ~ rho^0.47 up to a constant, which the proposal's
+        // normalisation removes).
         int64_t cnt = 0;
         for (int s = 0; s < B.nsys; s++) {
             const int2 ri = B.h_rec_info[s];
@@ -1536,18 +1573,26 @@ lima_status lima_batch_training_set(lima_batch* h, int32_t lookahead, float* fea
                 for (int q = 0; q < ri.y; q++) {
                     const float* e0 = &rec[ri.x + ((int64_t)st * ri.y + q) * REC_W];
                     const float* e1 = &rec[ri.x + ((int64_t)(st + lookahead) * ri.y + q) * REC_W];
-                    const double d[3] = {e0[32], e0[33], e0[34]};
-                    const double dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-                    if (dd < 1e-14) continue;   // atoms the step does not move carry no signal
-                    double proj = 0.0;
+                    const double d2 = (double)e0[32] * e0[32] + (double)e0[33] * e0[33] + (double)e0[34] * e0[34];
+                    if (d2 < 1e-14) continue;   // atoms the step does not move carry no signal
+                    double D[3], DD = 0.0;
                     for (int c = 0; c < 3; c++) {
                         int32_t u0, u1;
                         std::memcpy(&u0, &e0[35 + c], 4);
                         std::memcpy(&u1, &e1[35 + c], 4);
-                        proj += (double)(u1 - u0) * 1e-8 * d[c];
+                        D[c] = (double)(u1 - u0) * 1e-8;
+                        DD += D[c] * D[c];
                     }
-                    double t = proj / dd;
-                    t = std::min(3.95, std::max(0.05, t));
+                    if (DD < 1e-16) continue;
+                    const double nD = std::sqrt(DD);
+                    double f0 = 0.0, f1 = 0.0;
+                    for (int c = 0; c < 3; c++) { f0 += e0[38 + c] * D[c] / nD; f1 += e1[38 + c] * D[c] / nD; }
+                    double rho;
+                    if (f0 <= 0.0) rho = 0.1;
+                    else if (f0 - f1 <= 0.0) rho = 1000.0;
+                    else rho = (f0 / (f0 - f1)) * nD / std::sqrt(d2);
+                    const double u = std::min(1.0, std::max(0.0, (std::log10(std::max(rho, 1e-30)) + 1.0) / 4.0));
+                    const double t = 0.05 * std::pow(3.95 / 0.05, u);
                     if (features && targets && cnt < capacity) {
                         std::memcpy(&features[cnt * 32], e0, 32 * sizeof(float));
                         targets[cnt] = (float)t;

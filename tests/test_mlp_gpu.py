@@ -114,7 +114,7 @@ def test_recording_and_training_set():
     boxes = [L.gen_em_system(n, s, 0.02) for n, s in ((300, 2000), (600, 2001))]
     b = L.Batch(boxes, L.default_params())
     states = b.record(100.0, 400, 0.01, atom_stride=8, max_states=500)
-    x, t = b.training_set(lookahead=4)
+    x, t = b.training_set(lookahead=1)
     print(f"{states} states, {len(t)} samples, target mean {t.mean():.3f}")
     assert states > 100 and len(t) > 1000
     assert np.all((t >= 0.05) & (t <= 3.95))
@@ -123,3 +123,32 @@ def test_recording_and_training_set():
     m = L.Mlp(seed=9)
     losses, _ = m.train(x, t, epochs=3, batch=512, lr=1e-3, seed=4)
     assert losses[-1] < losses[0]
+
+
+def test_constant_scale_model_reproduces_plain_minimiser():
+    """A model whose output is the same scale for every atom makes each proposal the plain step (the
+    proposal is normalised by max s |F|), so the learned path follows the plain minimiser: step for
+    step over the first iterations (until FP32 roundings of the two step formulas part them), and to
+    the same convergence over a full run."""
+    boxes = [L.gen_em_system(n, s, 0.02) for n, s in ((400, 1020), (1100, 1021))]
+    p = L.default_params()
+    m = L.Mlp(seed=0)
+    m.set_params(np.zeros(m.n_params, np.float32))      # 4 sigmoid(0) = 2 for every atom
+    for iters, tol in ((25, 1e-3), (5000, 100.0)):
+        plain = L.Batch(boxes, p)
+        rp, _ = plain.minimize(tol, iters, 0.01)
+        learned = L.Batch(boxes, p)
+        rl, _ = learned.minimize_learned(m, tol, iters, 0.01, feature_interval=4, grow_on_learned=True)
+        cnt = learned.counters()
+        for k in range(len(boxes)):
+            print(f"max_iter {iters} system {k}: plain {rp[k].iterations} it {rp[k].accepted} acc E {rp[k].final_energy:.10e} | "
+                  f"constant-scale {rl[k].iterations} it {rl[k].accepted} acc {cnt[k, 1]} learned E {rl[k].final_energy:.10e}")
+            if iters == 25:
+                assert rl[k].iterations == rp[k].iterations == 25
+                assert rl[k].accepted == rp[k].accepted and cnt[k, 1] == rl[k].accepted
+                assert rl[k].final_energy == pytest.approx(rp[k].final_energy, rel=2e-5)   # FP32 |F| of the two norms
+            else:
+                assert rl[k].converged == 1
+                assert abs(rl[k].iterations - rp[k].iterations) <= 0.15 * rp[k].iterations
+                assert cnt[k, 1] >= 0.8 * rl[k].accepted     # nearly every accepted step is a proposal
+                assert rl[k].final_energy == pytest.approx(rp[k].final_energy, rel=3e-3)
