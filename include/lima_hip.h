@@ -305,9 +305,10 @@ lima_status lima_batch_get_features(lima_batch* batch, int32_t system, float* ou
  * of the last call. */
 lima_status lima_batch_get_counters(lima_batch* batch, int64_t* out);
 
-/* Time each system spent in the last lima_batch_minimize, by phase, as seen by its workgroup's
- * first thread: seconds[n_systems][5] = cutoff tile pairs, per-atom sums + intra pairs + bonded
- * terms, PME, reductions, trial steps. */
+/* Time each system spent in the last batched minimisation, by phase, as seen by its workgroup's
+ * first thread: seconds[n_systems][7] = cutoff tile pairs, per-atom sums + intra pairs + bonded
+ * terms, PME, reductions, trial steps, feature histograms, scale inference (the last two in the
+ * learned mode only). */
 lima_status lima_batch_get_phase_times(lima_batch* batch, double* seconds);
 
 #ifdef __cplusplus

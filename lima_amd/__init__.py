@@ -414,9 +414,9 @@ class Batch:
         return out
 
     def phase_times(self):
-        """[n_systems, 5] seconds: cutoff tile pairs, per-atom sums + intra + bonded, PME, reductions,
-        trial steps."""
-        out = np.zeros((len(self.boxes), 5))
+        """[n_systems, 7] seconds: cutoff tile pairs, per-atom sums + intra + bonded, PME, reductions,
+        trial steps, feature histograms, scale inference."""
+        out = np.zeros((len(self.boxes), 7))
         self._check(self.lib.lima_batch_get_phase_times(self._h, out.ctypes.data_as(C.POINTER(C.c_double))))
         return out
 

@@ -77,7 +77,8 @@ struct SysOut {
     int learned_accepted;        // iterations whose learned proposal was accepted
     int rec_iters;               // recorded states (recording mode)
     unsigned long long t0, t1;   // s_memrealtime (100 MHz) at start / end
-    unsigned long long ph[5];    // ticks in tile pairs, per-atom sums + intra + bonded, PME, reductions, trial steps
+    unsigned long long ph[7];    // ticks in tile pairs, per-atom sums + intra + bonded, PME, reductions, trial steps,
+                                 // feature histograms, scale inference
 };
 
 struct BatchArgs {
@@ -285,7 +286,7 @@ struct BlockShared {
     double res[5];
     float fres;
     float2 tw[3][KMAX];
-    unsigned long long ph[5];    // phase ticks seen by thread 0
+    unsigned long long ph[7];    // phase ticks seen by thread 0
 };
 
 // FP64 sums of v[0..3] and max of fm over the workgroup, in a fixed order; every thread gets them.
@@ -785,7 +786,7 @@ template <int T, int UNR, bool MULTI>
 __device__ __forceinline__ void minimize_system(const BatchArgs& a, const SysMeta& m, int sys, SysOut* out, BlockShared<T>& sh,
                                                 unsigned char* lds) {
     const int tid = threadIdx.x;
-    if (tid == 0) for (int q = 0; q < 5; q++) sh.ph[q] = 0ull;
+    if (tid == 0) for (int q = 0; q < 7; q++) sh.ph[q] = 0ull;
     const bool learned = a.mode == MODE_LEARNED, record = a.mode == MODE_RECORD;
     Eval acc_e{0.0, 0.0, 0.0, 0.0, 0.f};
     double h = a.h0;
@@ -802,8 +803,14 @@ __device__ __forceinline__ void minimize_system(const BatchArgs& a, const SysMet
             const int4* uc = (cur ? a.u1 : a.u0) + m.atom0;
             const float4* fc = (cur ? a.f1 : a.f0) + m.atom0;
             if (kind == 1) {
-                if (it % a.feat_interval == 0) hist_pass<T, MULTI>(a, m, lds, cur ? a.u1 : a.u0);
+                unsigned long long t0 = tid == 0 ? __builtin_amdgcn_s_memrealtime() : 0ull;
+                if (it % a.feat_interval == 0) {
+                    hist_pass<T, MULTI>(a, m, lds, cur ? a.u1 : a.u0);
+                    __syncthreads();
+                    if (tid == 0) { const unsigned long long t1 = __builtin_amdgcn_s_memrealtime(); sh.ph[5] += t1 - t0; t0 = t1; }
+                }
                 snorm = infer_scales<T>(a, m, cur ? a.f1 : a.f0, acc_e.fmax, sh);
+                if (tid == 0) sh.ph[6] += __builtin_amdgcn_s_memrealtime() - t0;
             }
             const unsigned long long ts = tid == 0 ? __builtin_amdgcn_s_memrealtime() : 0ull;
             // plain: h F / Fmax; learned: h s F / max(s |F|), the same largest displacement h
@@ -866,7 +873,7 @@ __device__ __forceinline__ void minimize_system(const BatchArgs& a, const SysMet
         out->evaluations = nevals;
         out->learned_accepted = nlearn;
         out->rec_iters = nrec;
-        for (int q = 0; q < 5; q++) out->ph[q] = sh.ph[q];
+        for (int q = 0; q < 7; q++) out->ph[q] = sh.ph[q];
     }
 }
 
@@ -1706,7 +1713,7 @@ lima_status lima_batch_get_phase_times(lima_batch* h, double* seconds) {
         std::vector<SysOut> res(B.nsys);
         HIP_CHECK(hipMemcpy(res.data(), B.out, B.nsys * sizeof(SysOut), hipMemcpyDeviceToHost));
         for (int s = 0; s < B.nsys; s++)
-            for (int q = 0; q < 5; q++) seconds[s * 5 + q] = (double)res[s].ph[q] * 1e-8;
+            for (int q = 0; q < 7; q++) seconds[s * 7 + q] = (double)res[s].ph[q] * 1e-8;
     });
 }
 
