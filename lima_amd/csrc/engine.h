@@ -29,7 +29,7 @@ struct NList {
     int ncx = 1, ncy = 1, ncol = 1;
     int ncl = 0;               // clusters
     int ncl_max = 0;
-    int64_t nent = 0, nent_cap = 0;
+    int64_t nent_cap = 0;      // entry slots allocated (ncl_max * cap)
     int cap = 2048;            // entry capacity per i-cluster (grows on overflow)
     int* col_of = nullptr;     // [N]
     int* col_count = nullptr;  // [ncol+1]
@@ -45,6 +45,7 @@ struct NList {
     uint4* cmax = nullptr;     // [ncl_max]
     float2* ljs = nullptr;     // [slots] (sigma/2, sqrt(24 eps))
     int* comps = nullptr;      // [slots] compound of slot (-1 padding)
+    uint4* cw = nullptr;       // [slots] wrapped Coord of the slot's particle (w = 1 for padding)
     float4* xq = nullptr;      // [slots] position relative to cluster ref [nm] + charge
     int* ecount = nullptr;     // [ncl_max+1]
     int* eoff = nullptr;       // [ncl_max+1]

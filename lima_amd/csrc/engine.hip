@@ -761,7 +761,11 @@ lima_status lima_get_status(lima_engine* h, lima_runstatus* out) {
     out->critical_error_occured = e.critical ? 1 : 0;
     out->simulation_finished = (e.prm.n_steps > 0 && e.step >= e.prm.n_steps) ? 1 : 0;
     out->n_list_rebuilds = e.n_rebuilds;
-    out->list_entries = e.nl.nent;
+    if (e.nl.ncl > 0 && e.nl.ecount) {   // padded entry slots of the current list (summed here, not per rebuild)
+        std::vector<int> cnt(e.nl.ncl);
+        if (hipMemcpy(cnt.data(), e.nl.ecount, cnt.size() * sizeof(int), hipMemcpyDeviceToHost) == hipSuccess)
+            for (int c : cnt) out->list_entries += c;
+    }
     out->list_violations = e.list_violations;
     out->max_displacement_nm = (float)e.last_dmax;
     out->greatest_force = NAN;

@@ -28,26 +28,72 @@ __global__ void k_fill_int(int* p, int n, int v) {
     if (i < n) p[i] = v;
 }
 
-__global__ void k_col_assign(int64_t N, const uint4* __restrict__ w, long long Lqx, long long Lqy, int ncx, int ncy,
-                             int* __restrict__ col_of, int* __restrict__ col_count) {
-    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= N) return;
-    const uint4 p = w[a];
+__device__ __forceinline__ int column_of(const uint4 p, long long Lqx, long long Lqy, int ncx, int ncy) {
     const int cx = (int)(((unsigned long long)p.x * (unsigned long long)ncx) / (unsigned long long)Lqx);
     const int cy = (int)(((unsigned long long)p.y * (unsigned long long)ncy) / (unsigned long long)Lqy);
-    const int c = cy * ncx + cx;
-    col_of[a] = c;
-    atomicAdd(&col_count[c], 1);
+    return cy * ncx + cx;
 }
 
-__global__ void k_col_scatter(int64_t N, const uint4* __restrict__ w, const int* __restrict__ col_of,
-                              const int* __restrict__ col_start, int* __restrict__ col_fill,
-                              unsigned long long* __restrict__ keys) {
-    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= N) return;
-    const int c = col_of[a];
-    const int pos = col_start[c] + atomicAdd(&col_fill[c], 1);
-    keys[pos] = ((unsigned long long)w[a].z << 32) | (unsigned long long)(uint32_t)a;
+constexpr int COL_ITEMS = 4;      // particles per thread in the column count / scatter
+constexpr int COL_LDS_MAX = 8192; // columns counted in LDS; more columns use global atomics directly
+
+// Column of every particle and the column populations.  Counts are gathered per block in LDS and
+// added to the global counters once per (block, column): same-address global atomics from every
+// particle serialise in a few L2 channels.
+__global__ __launch_bounds__(256) void k_col_assign(int64_t N, const uint4* __restrict__ w, long long Lqx, long long Lqy, int ncx,
+                                                    int ncy, int* __restrict__ col_of, int* __restrict__ col_count) {
+    extern __shared__ int hist[];
+    const int ncol = ncx * ncy;
+    const bool lds = ncol <= COL_LDS_MAX;
+    if (lds) for (int c = threadIdx.x; c < ncol; c += blockDim.x) hist[c] = 0;
+    __syncthreads();
+    const int64_t a0 = (int64_t)blockIdx.x * blockDim.x * COL_ITEMS;
+#pragma unroll
+    for (int k = 0; k < COL_ITEMS; k++) {
+        const int64_t a = a0 + (int64_t)k * blockDim.x + threadIdx.x;
+        if (a < N) {
+            const int c = column_of(w[a], Lqx, Lqy, ncx, ncy);
+            col_of[a] = c;
+            atomicAdd(lds ? &hist[c] : &col_count[c], 1);
+        }
+    }
+    __syncthreads();
+    if (lds)
+        for (int c = threadIdx.x; c < ncol; c += blockDim.x)
+            if (hist[c]) atomicAdd(&col_count[c], hist[c]);
+}
+
+// (w_z, particle) keys into their column's segment (order inside a column is settled by the sort).
+// Each block reserves one range per column it touches and places its particles in it by their LDS rank.
+__global__ __launch_bounds__(256) void k_col_scatter(int64_t N, int ncol, const uint4* __restrict__ w, const int* __restrict__ col_of,
+                                                     const int* __restrict__ col_start, int* __restrict__ col_fill,
+                                                     unsigned long long* __restrict__ keys) {
+    extern __shared__ int hist[];
+    const bool lds = ncol <= COL_LDS_MAX;
+    if (lds) for (int c = threadIdx.x; c < ncol; c += blockDim.x) hist[c] = 0;
+    __syncthreads();
+    const int64_t a0 = (int64_t)blockIdx.x * blockDim.x * COL_ITEMS;
+    int rank[COL_ITEMS], col[COL_ITEMS];
+#pragma unroll
+    for (int k = 0; k < COL_ITEMS; k++) {
+        const int64_t a = a0 + (int64_t)k * blockDim.x + threadIdx.x;
+        col[k] = a < N ? col_of[a] : -1;
+        rank[k] = 0;
+        if (col[k] >= 0) rank[k] = lds ? atomicAdd(&hist[col[k]], 1) : col_start[col[k]] + atomicAdd(&col_fill[col[k]], 1);
+    }
+    __syncthreads();
+    if (lds)
+        for (int c = threadIdx.x; c < ncol; c += blockDim.x)
+            if (hist[c]) hist[c] = col_start[c] + atomicAdd(&col_fill[c], hist[c]);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < COL_ITEMS; k++) {
+        const int64_t a = a0 + (int64_t)k * blockDim.x + threadIdx.x;
+        if (col[k] >= 0) {
+            const int pos = lds ? hist[col[k]] + rank[k] : rank[k];
+            keys[pos] = ((unsigned long long)w[a].z << 32) | (unsigned long long)(uint32_t)a;
+        }
+    }
 }
 
 // One block per column: bitonic sort of the column's (w_z, particle) keys in LDS.
@@ -103,32 +149,45 @@ __global__ void k_assign_slots(int64_t N, int ncol, const int* __restrict__ col_
     slot_of[a] = I * CLUSTER + k % CLUSTER;
 }
 
-__global__ void k_cluster_props(int ncl, int ncol, const int* __restrict__ col_cl_start, const int* __restrict__ catoms,
-                                const uint4* __restrict__ w, const float4* __restrict__ prm, const int* __restrict__ comp,
-                                int* __restrict__ ccol, uint4* __restrict__ cref, uint4* __restrict__ cmin, uint4* __restrict__ cmax,
-                                float2* __restrict__ ljs, int* __restrict__ comps) {
-    const int I = blockIdx.x * blockDim.x + threadIdx.x;
-    if (I >= ncl) return;
-    ccol[I] = upper_bound_int(col_cl_start, ncol + 1, I) - 1;
-    uint32_t mn[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, mx[3] = {0, 0, 0};
-    for (int s = 0; s < CLUSTER; s++) {
-        const int a = catoms[I * CLUSTER + s];
-        if (a < 0) {
-            ljs[I * CLUSTER + s] = make_float2(0.f, 0.f);
-            comps[I * CLUSTER + s] = -1;
-            continue;
-        }
+// One thread per cluster slot; the 8 slots of a cluster are 8 consecutive lanes, which reduce the
+// cluster's bounding box with xor shuffles.
+__global__ __launch_bounds__(256) void k_cluster_props(int ncl, int ncol, const int* __restrict__ col_cl_start,
+                                                       const int* __restrict__ catoms, const uint4* __restrict__ w,
+                                                       const float4* __restrict__ prm, const int* __restrict__ comp,
+                                                       int* __restrict__ ccol, uint4* __restrict__ cref, uint4* __restrict__ cmin,
+                                                       uint4* __restrict__ cmax, float2* __restrict__ ljs, int* __restrict__ comps,
+                                                       uint4* __restrict__ cw) {
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    const int I = slot / CLUSTER;
+    const bool live = I < ncl;   // whole 8-lane groups share I
+    const int a = live ? catoms[slot] : -1;
+    uint32_t mn[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, mx[3] = {0u, 0u, 0u};
+    if (a >= 0) {
         const uint4 p = w[a];
-        const uint32_t v[3] = {p.x, p.y, p.z};
-        for (int d = 0; d < 3; d++) { mn[d] = min(mn[d], v[d]); mx[d] = max(mx[d], v[d]); }
+        mn[0] = mx[0] = p.x; mn[1] = mx[1] = p.y; mn[2] = mx[2] = p.z;
         const float4 q = prm[a];
-        ljs[I * CLUSTER + s] = make_float2(q.y, q.z * 4.898979485566356f);   // (sigma/2, sqrt(24 eps))
-        comps[I * CLUSTER + s] = comp[a];
+        ljs[slot] = make_float2(q.y, q.z * 4.898979485566356f);   // (sigma/2, sqrt(24 eps))
+        comps[slot] = comp[a];
+        cw[slot] = make_uint4(p.x, p.y, p.z, 0u);
+    } else if (live) {
+        ljs[slot] = make_float2(0.f, 0.f);
+        comps[slot] = -1;
+        cw[slot] = make_uint4(0u, 0u, 0u, 1u);
     }
-    cmin[I] = make_uint4(mn[0], mn[1], mn[2], 0);
-    cmax[I] = make_uint4(mx[0], mx[1], mx[2], 0);
-    cref[I] = make_uint4((uint32_t)(((uint64_t)mn[0] + mx[0]) >> 1), (uint32_t)(((uint64_t)mn[1] + mx[1]) >> 1),
-                         (uint32_t)(((uint64_t)mn[2] + mx[2]) >> 1), 0);
+#pragma unroll
+    for (int o = 1; o < CLUSTER; o <<= 1)
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            mn[d] = min(mn[d], (uint32_t)__shfl_xor((int)mn[d], o, 64));
+            mx[d] = max(mx[d], (uint32_t)__shfl_xor((int)mx[d], o, 64));
+        }
+    if (live && (slot & (CLUSTER - 1)) == 0) {
+        ccol[I] = upper_bound_int(col_cl_start, ncol + 1, I) - 1;
+        cmin[I] = make_uint4(mn[0], mn[1], mn[2], 0);
+        cmax[I] = make_uint4(mx[0], mx[1], mx[2], 0);
+        cref[I] = make_uint4((uint32_t)(((uint64_t)mn[0] + mx[0]) >> 1), (uint32_t)(((uint64_t)mn[1] + mx[1]) >> 1),
+                             (uint32_t)(((uint64_t)mn[2] + mx[2]) >> 1), 0);
+    }
 }
 
 struct NlArgs {
@@ -140,7 +199,7 @@ struct NlArgs {
     const int* catoms;
     const int* comps;
     const uint8_t* bmask;
-    const uint4* w;
+    const uint4* cw;
     const uint4* cref;
     const uint4* cmin;
     const uint4* cmax;
@@ -149,14 +208,6 @@ struct NlArgs {
     float4* ent;
     uint8_t* ecode;
 };
-
-__device__ __forceinline__ bool is_intra(int ca, int cb, int nC, const uint8_t* bmask) {
-    if (ca == cb) return true;
-    if (ca < 0 || cb < 0 || ca >= nC || cb >= nC) return false;
-    const int d = cb - ca;
-    if (d < -2 || d > 2) return false;
-    return (bmask[ca] >> (d + 2)) & 1;
-}
 
 // range of column indices whose [lo, hi] interval (after shift) is non-empty
 __device__ __forceinline__ bool col_range(long long lo, long long hi, long long Lq, int n, int* c0, int* c1) {
@@ -168,7 +219,9 @@ __device__ __forceinline__ bool col_range(long long lo, long long hi, long long 
     return true;
 }
 
-// first index in [lo, hi) whose cluster max z >= z (clusters of a column are z ordered)
+// first index in [lo, hi) whose cluster max z >= z (clusters of a column are z ordered: a cluster
+// holds 8 consecutive particles of the column's z order, so both its min z and max z are
+// non-decreasing along the column)
 __device__ __forceinline__ int first_cluster_reaching(const uint4* cmax, int lo, int hi, long long z) {
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
@@ -177,86 +230,224 @@ __device__ __forceinline__ int first_cluster_reaching(const uint4* cmax, int lo,
     return lo;
 }
 
-// One wave per i-cluster, one pass: entries go to the cluster's fixed-capacity slice
-// [I cap, (I+1) cap); the count (padded to a multiple of 8 with mask-0 entries) goes to ecount[I].
-// A slice that would overflow sets flags[1] and the host rebuilds with a larger capacity.
+// first index in [lo, hi) whose cluster min z > z
+__device__ __forceinline__ int first_cluster_beyond(const uint4* cmin, int lo, int hi, long long z) {
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if ((long long)cmin[mid].z <= z) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ long long gap1(long long lo, long long hi, long long mn, long long mx) {   // [lo, hi] to [mn, mx]
+    return lo > mx ? lo - mx : (hi < mn ? mn - hi : 0);
+}
+
+// The shifted column ranges along one axis: for shift s in -1..1, columns [c0[s], c1[s]] (c1 < c0:
+// none) whose slab lies within R of [mn, mx].
+struct AxisRanges {
+    int c0[3], c1[3], n;
+};
+__device__ __forceinline__ AxisRanges axis_ranges(long long mn, long long mx, long long R, long long Lq, int ncol) {
+    AxisRanges r;
+    r.n = 0;
+    for (int s = 0; s < 3; s++) {
+        int a0 = 0, a1 = -1;
+        if (!col_range(mn - R - (s - 1) * Lq, mx + R - (s - 1) * Lq, Lq, ncol, &a0, &a1)) { a0 = 0; a1 = -1; }
+        r.c0[s] = a0; r.c1[s] = a1;
+        r.n += a1 - a0 + 1;
+    }
+    return r;
+}
+// k-th column of the concatenated ranges -> (column, shift index 0..2)
+__device__ __forceinline__ void axis_pick(const AxisRanges& r, int k, int* col, int* s) {
+    int t = 0;
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        const int len = r.c1[q] - r.c0[q] + 1;
+        if (k >= 0 && k < len && t == 0) { *col = r.c0[q] + k; *s = q; t = 1; }
+        k -= len;
+    }
+}
+
+constexpr int NL_FLAT = 512;   // flat j-cluster table per wave (LDS), filled in chunks
+
+// Image shift of code = sx + 3 sy + 9 sz (each 0..2 for -1, 0, +1 box lengths).
+__device__ __forceinline__ long long shift_of(int c, long long Lq) { return c == 0 ? -Lq : (c == 2 ? Lq : 0ll); }
+
+// Is the shifted box of j-cluster J within R of the i box?  A particle is never nearer than its
+// cluster's box, so dropping J here never drops an entry.
+__device__ __forceinline__ bool jbox_near(const NlArgs& a, int J, int code, const long long* mn, const long long* mx) {
+    const uint4 lo = a.cmin[J], hi = a.cmax[J];
+    const long long sx = shift_of(code % 3, a.Lq[0]), sy = shift_of((code / 3) % 3, a.Lq[1]), sz = shift_of(code / 9, a.Lq[2]);
+    const long long bx = min(gap1((long long)lo.x + sx, (long long)hi.x + sx, mn[0], mx[0]), a.R + 1);
+    const long long by = min(gap1((long long)lo.y + sy, (long long)hi.y + sy, mn[1], mx[1]), a.R + 1);
+    const long long bz = min(gap1((long long)lo.z + sz, (long long)hi.z + sz, mn[2], mx[2]), a.R + 1);
+    return bx * bx + by * by + bz * bz <= a.R2;
+}
+
+// One wave per i-cluster.  Candidate j-clusters are z-ranges of x-y columns under the 27 image
+// shifts.  Lanes resolve up to 64 (y column, x column, z shift) units at once: the unit is dropped
+// when the column's x-y slab lies farther than R from the i box, else two binary searches on the
+// column's z-ordered cluster bounds give its j-cluster range.  A wave prefix sum expands the ranges
+// into an LDS table of (J, shift); lanes test 64 table entries at a time for a shifted box within R
+// of the i box and pack the survivors into a second table.  The wave takes 8 survivors x 8 slots
+// per pass, with the next pass's loads in flight, and tests each particle against the i box.  Entries go to the cluster's
+// fixed-capacity slice [I cap, (I+1) cap); the count (padded to a multiple of 8 with mask-0
+// entries) goes to ecount[I].  A slice that would overflow sets flags[1] and the host rebuilds
+// with a larger capacity.
 __global__ __launch_bounds__(256) void k_nlist(NlArgs a) {
+    __shared__ uint32_t flat_all[4][NL_FLAT], cand_all[4][NL_FLAT];
     const int lane = threadIdx.x & 63;
     const int I = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (I >= a.ncl) return;
+    if (I >= a.ncl) return;   // uniform per wave
+    uint32_t* flat = flat_all[threadIdx.x >> 6];   // candidates that passed the box test
+    uint32_t* cand = cand_all[threadIdx.x >> 6];   // all candidates of the current chunk
     const uint4 bmn = a.cmin[I], bmx = a.cmax[I], refI = a.cref[I];
     const long long mn[3] = {(long long)bmn.x, (long long)bmn.y, (long long)bmn.z};
     const long long mx[3] = {(long long)bmx.x, (long long)bmx.y, (long long)bmx.z};
     int ci[CLUSTER];
-    for (int s = 0; s < CLUSTER; s++) ci[s] = a.comps[I * CLUSTER + s];
+    uint32_t bmi[CLUSTER];   // exclusion bits of slot s's compound, 0 when it has none
+    int ci_lo = 0x7fffffff, ci_hi = -0x7fffffff;
+    uint32_t valid = 0;      // i slots holding a particle
+#pragma unroll
+    for (int s = 0; s < CLUSTER; s++) {
+        ci[s] = a.comps[I * CLUSTER + s];
+        bmi[s] = (ci[s] >= 0 && ci[s] < a.nC) ? (uint32_t)a.bmask[ci[s]] : 0u;
+        if (ci[s] >= 0) {
+            valid |= 1u << s;
+            ci_lo = min(ci_lo, ci[s]);
+            ci_hi = max(ci_hi, ci[s]);
+        }
+    }
+    const AxisRanges ry = axis_ranges(mn[1], mx[1], a.R, a.Lq[1], a.ncy);
+    const AxisRanges rx = axis_ranges(mn[0], mx[0], a.R, a.Lq[0], a.ncx);
+    const int nunits = ry.n * rx.n * 3;
     const int jsub = lane >> 3, sl = lane & 7;
     int count = 0;
     const int64_t base = (int64_t)I * a.cap;
-    for (int sy = -1; sy <= 1; sy++) {
-        int cy0, cy1;
-        if (!col_range(mn[1] - a.R - sy * a.Lq[1], mx[1] + a.R - sy * a.Lq[1], a.Lq[1], a.ncy, &cy0, &cy1)) continue;
-        for (int cy = cy0; cy <= cy1; cy++) {
-            for (int sx = -1; sx <= 1; sx++) {
-                int cx0, cx1;
-                if (!col_range(mn[0] - a.R - sx * a.Lq[0], mx[0] + a.R - sx * a.Lq[0], a.Lq[0], a.ncx, &cx0, &cx1)) continue;
-                for (int cx = cx0; cx <= cx1; cx++) {
+    for (int ub = 0; ub < nunits; ub += 64) {
+        // ---- this lane's unit: a z range of one shifted column ----
+        int jf = 0, nj = 0, code = 13;
+        {
+            const int u = ub + lane;
+            if (u < nunits) {
+                const int szi = u % 3, t = u / 3;
+                int cx = 0, sxi = 0, cy = 0, syi = 0;
+                axis_pick(rx, t % rx.n, &cx, &sxi);
+                axis_pick(ry, t / rx.n, &cy, &syi);
+                const long long shx = shift_of(sxi, a.Lq[0]), shy = shift_of(syi, a.Lq[1]), shz = shift_of(szi, a.Lq[2]);
+                // the column's slab holds x in [ceil(cx Lq / n), ceil((cx+1) Lq / n) - 1], inside
+                // [floor(cx Lq / n), floor((cx+1) Lq / n)]
+                const long long xlo = (long long)(((unsigned long long)cx * (unsigned long long)a.Lq[0]) / (unsigned long long)a.ncx);
+                const long long xhi = (long long)(((unsigned long long)(cx + 1) * (unsigned long long)a.Lq[0]) / (unsigned long long)a.ncx);
+                const long long ylo = (long long)(((unsigned long long)cy * (unsigned long long)a.Lq[1]) / (unsigned long long)a.ncy);
+                const long long yhi = (long long)(((unsigned long long)(cy + 1) * (unsigned long long)a.Lq[1]) / (unsigned long long)a.ncy);
+                const long long gx = min(gap1(xlo + shx, xhi + shx, mn[0], mx[0]), a.R + 1);
+                const long long gy = min(gap1(ylo + shy, yhi + shy, mn[1], mx[1]), a.R + 1);
+                const long long zlo = mn[2] - a.R - shz, zhi = mx[2] + a.R - shz;
+                if (gx * gx + gy * gy <= a.R2 && zhi >= 0 && zlo <= a.Lq[2] - 1) {
                     const int col = cy * a.ncx + cx;
                     const int cs = a.col_cl_start[col], ce = a.col_cl_start[col + 1];
-                    if (cs >= ce) continue;
-                    for (int sz = -1; sz <= 1; sz++) {
-                        const long long zlo = mn[2] - a.R - sz * a.Lq[2], zhi = mx[2] + a.R - sz * a.Lq[2];
-                        if (zhi < 0 || zlo > a.Lq[2] - 1) continue;
-                        const int code = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
-                        const long long sh[3] = {sx * a.Lq[0], sy * a.Lq[1], sz * a.Lq[2]};
-                        const int Jfirst = first_cluster_reaching(a.cmax, cs, ce, zlo);
-                        for (int Jb = Jfirst; Jb < ce; Jb += CLUSTER) {
-                            if ((long long)a.cmin[Jb].z > zhi) break;   // z ordered: nothing further reaches
-                            const int J = Jb + jsub;
-                            bool emit = false;
-                            uint32_t mask = 0;
-                            int bslot = 0;
-                            if (J < ce && (long long)a.cmin[J].z <= zhi) {
-                                bslot = J * CLUSTER + sl;
-                                const int b = a.catoms[bslot];
-                                if (b >= 0) {
-                                    const uint4 pb = a.w[b];
-                                    const long long pv[3] = {(long long)pb.x + sh[0], (long long)pb.y + sh[1], (long long)pb.z + sh[2]};
-                                    long long d2 = 0;
-                                    for (int d = 0; d < 3; d++) {
-                                        long long g = 0;
-                                        if (pv[d] < mn[d]) g = mn[d] - pv[d];
-                                        else if (pv[d] > mx[d]) g = pv[d] - mx[d];
-                                        d2 += g * g;
-                                    }
-                                    if (d2 <= a.R2) {
-                                        const int cb = a.comps[bslot];
-                                        for (int s = 0; s < CLUSTER; s++) {
-                                            if (ci[s] < 0) continue;
-                                            if (is_intra(ci[s], cb, a.nC, a.bmask)) continue;
-                                            if (J == I && code == 13 && s == sl) continue;   // no self pair
-                                            mask |= 1u << s;
-                                        }
-                                        emit = mask != 0;
-                                    }
-                                }
-                            }
-                            const unsigned long long bal = __ballot(emit);
-                            if (emit) {
-                                const int pos = count + __popcll(bal & ((1ull << lane) - 1ull));
-                                if (pos < a.cap) {
-                                    const uint4 rj = a.cref[J];
-                                    const float sxf = (float)((long long)rj.x - (long long)refI.x + sh[0]) * LIMA_LIMA_TO_NANO_F;
-                                    const float syf = (float)((long long)rj.y - (long long)refI.y + sh[1]) * LIMA_LIMA_TO_NANO_F;
-                                    const float szf = (float)((long long)rj.z - (long long)refI.z + sh[2]) * LIMA_LIMA_TO_NANO_F;
-                                    a.ent[base + pos] = make_float4(sxf, syf, szf, __uint_as_float((uint32_t)bslot | (mask << 24)));
-                                    a.ecode[base + pos] = (uint8_t)code;
-                                }
-                            }
-                            count += __popcll(bal);
-                        }
-                    }
+                    const int f0 = first_cluster_reaching(a.cmax, cs, ce, zlo);
+                    const int f1 = first_cluster_beyond(a.cmin, f0, ce, zhi);
+                    jf = f0;
+                    nj = f1 - f0;
+                    code = sxi + 3 * syi + 9 * szi;
                 }
             }
+        }
+        // ---- candidates of all units, in chunks of NL_FLAT: expand, filter by box, pack ----
+        int incl = nj;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += y;
+        }
+        const int ncand = __shfl(incl, 63, 64);
+        for (int k0 = 0; k0 < ncand; k0 += NL_FLAT) {
+            const int k1 = min(ncand, k0 + NL_FLAT);
+            for (int q = max(incl - nj, k0); q < min(incl, k1); q++)
+                cand[q - k0] = (uint32_t)(jf + (q - (incl - nj))) | ((uint32_t)code << 24);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // one candidate per lane: keep it when its shifted box lies within R of the i box
+            int c1 = 0;   // survivors packed into flat[0, c1)
+            for (int r = k0; r < k1; r += 64) {
+                const int k = r + lane;
+                bool near = false;
+                uint32_t t = 0;
+                if (k < k1) {
+                    t = cand[k - k0];
+                    near = jbox_near(a, (int)(t & 0xffffffu), (int)(t >> 24), mn, mx);
+                }
+                const unsigned long long bal = __ballot(near);
+                if (near) flat[c1 + __popcll(bal & ((1ull << lane) - 1ull))] = t;
+                c1 += __popcll(bal);
+            }
+            const int c0 = 0;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // ---- 8 j-clusters x 8 slots per pass; next pass's loads in flight ----
+            auto fetch = [&](int f, int& J, int& cd, uint4& pb, int& cb, uint4& rj) {
+                const bool ok = f < c1;
+                const uint32_t t = flat[ok ? f - c0 : 0];
+                J = ok ? (int)(t & 0xffffffu) : I;   // past the end: a valid cluster, never tested
+                cd = ok ? (int)(t >> 24) : -1;
+                pb = a.cw[J * CLUSTER + sl];
+                cb = a.comps[J * CLUSTER + sl];
+                rj = a.cref[J];
+            };
+            int J, cd, cb, nJ, ncd, ncb;
+            uint4 pb, rj, npb, nrj;
+            fetch(c0 + jsub, J, cd, pb, cb, rj);
+            for (int f0 = c0; f0 < c1; f0 += CLUSTER) {
+                fetch(f0 + CLUSTER + jsub, nJ, ncd, npb, ncb, nrj);
+                uint32_t mask = 0;
+                if (cd >= 0 && pb.w == 0u) {
+                    const long long sx = shift_of(cd % 3, a.Lq[0]), sy = shift_of((cd / 3) % 3, a.Lq[1]);
+                    const long long sz = shift_of(cd / 9, a.Lq[2]);
+                    const long long gx = min(gap1((long long)pb.x + sx, (long long)pb.x + sx, mn[0], mx[0]), a.R + 1);
+                    const long long gy = min(gap1((long long)pb.y + sy, (long long)pb.y + sy, mn[1], mx[1]), a.R + 1);
+                    const long long gz = min(gap1((long long)pb.z + sz, (long long)pb.z + sz, mn[2], mx[2]), a.R + 1);
+                    if (gx * gx + gy * gy + gz * gz <= a.R2) {
+                        if (cb < ci_lo - 2 || cb > ci_hi + 2) {
+                            mask = valid;   // no slot's compound is cb or bonded to it
+                        } else {
+                            const bool cbin = cb >= 0 && cb < a.nC;
+#pragma unroll
+                            for (int s = 0; s < CLUSTER; s++) {
+                                if (ci[s] < 0) continue;
+                                const int dd = cb - ci[s];
+                                const bool intra = ci[s] == cb || (cbin && bmi[s] != 0u && dd >= -2 && dd <= 2 && ((bmi[s] >> (dd + 2)) & 1u));
+                                if (!intra) mask |= 1u << s;
+                            }
+                        }
+                        if (J == I && cd == 13) mask &= ~(1u << sl);   // no self pair
+                    }
+                }
+                const bool emit = mask != 0;
+                const unsigned long long bal = __ballot(emit);
+                if (emit) {
+                    const int pos = count + __popcll(bal & ((1ull << lane) - 1ull));
+                    if (pos < a.cap) {
+                        // shift between the cluster reference points: the int64 difference is exact in
+                        // a double, so the float is the same single rounding as a direct conversion
+                        const float sxf = (float)(double)((long long)rj.x - (long long)refI.x + shift_of(cd % 3, a.Lq[0])) * LIMA_LIMA_TO_NANO_F;
+                        const float syf = (float)(double)((long long)rj.y - (long long)refI.y + shift_of((cd / 3) % 3, a.Lq[1])) * LIMA_LIMA_TO_NANO_F;
+                        const float szf = (float)(double)((long long)rj.z - (long long)refI.z + shift_of(cd / 9, a.Lq[2])) * LIMA_LIMA_TO_NANO_F;
+                        a.ent[base + pos] = make_float4(sxf, syf, szf, __uint_as_float((uint32_t)(J * CLUSTER + sl) | (mask << 24)));
+                        a.ecode[base + pos] = (uint8_t)cd;
+                    }
+                }
+                count += __popcll(bal);
+                J = nJ; cd = ncd; pb = npb; cb = ncb; rj = nrj;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
     }
     const int padded = (count + CLUSTER - 1) & ~(CLUSTER - 1);
@@ -268,10 +459,7 @@ __global__ __launch_bounds__(256) void k_nlist(NlArgs a) {
         a.ent[base + k] = make_float4(0.f, 0.f, 0.f, __uint_as_float((uint32_t)(I * CLUSTER)));
         a.ecode[base + k] = 13;
     }
-    if (lane == 0) {
-        a.ecount[I] = padded;
-        atomicAdd(&a.flags[2], padded);
-    }
+    if (lane == 0) a.ecount[I] = padded;
 }
 
 // Single-block exclusive scan of n ints; out[n] = total.
@@ -325,6 +513,7 @@ void nlist_alloc(Engine& e) {
     nl.ncy = std::max(1, (int)std::floor(e.L[1] / a));
     nl.ncol = nl.ncx * nl.ncy;
     nl.ncl_max = (int)(e.N / CLUSTER) + nl.ncol + 1;
+    if (nl.ncl_max >= (1 << 24)) throw LimaFail(LIMA_ERR_UNSUPPORTED, "more than 2^24 particle clusters");
     const int64_t slots = (int64_t)nl.ncl_max * CLUSTER;
     HIP_CHECK(hipMalloc(&nl.col_of, e.N * sizeof(int)));
     HIP_CHECK(hipMalloc(&nl.col_count, (nl.ncol + 1) * sizeof(int)));
@@ -340,6 +529,7 @@ void nlist_alloc(Engine& e) {
     HIP_CHECK(hipMalloc(&nl.cmax, nl.ncl_max * sizeof(uint4)));
     HIP_CHECK(hipMalloc(&nl.ljs, slots * sizeof(float2)));
     HIP_CHECK(hipMalloc(&nl.comps, slots * sizeof(int)));
+    HIP_CHECK(hipMalloc(&nl.cw, slots * sizeof(uint4)));
     HIP_CHECK(hipMalloc(&nl.xq, slots * sizeof(float4)));
     HIP_CHECK(hipMalloc(&nl.ecount, (nl.ncl_max + 1) * sizeof(int)));
     HIP_CHECK(hipMalloc(&nl.eoff, (nl.ncl_max + 2) * sizeof(int)));
@@ -355,7 +545,7 @@ void nlist_alloc(Engine& e) {
 void nlist_free(Engine& e) {
     NList& nl = e.nl;
     void* ptrs[] = {nl.col_of, nl.col_count, nl.col_start, nl.col_fill, nl.keys, nl.col_ncl, nl.catoms, nl.slot_of,
-                    nl.ccol, nl.cref, nl.cmin, nl.cmax, nl.ljs, nl.comps, nl.xq, nl.ecount, nl.eoff, nl.flags,
+                    nl.ccol, nl.cref, nl.cmin, nl.cmax, nl.ljs, nl.comps, nl.cw, nl.xq, nl.ecount, nl.eoff, nl.flags,
                     nl.ent, nl.ecode, e.fs, e.es};
     for (void* p : ptrs) if (p) (void)hipFree(p);
 }
@@ -399,10 +589,12 @@ void nlist_build(Engine& e, const uint4* w) {
     hipLaunchKernelGGL(k_fill_int, dim3(nblk(nl.ncol + 1, 256)), dim3(256), 0, s, nl.col_count, nl.ncol + 1, 0);
     hipLaunchKernelGGL(k_fill_int, dim3(nblk(nl.ncol + 1, 256)), dim3(256), 0, s, nl.col_fill, nl.ncol + 1, 0);
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(4), 0, s, nl.flags, 4, 0);
-    hipLaunchKernelGGL(k_col_assign, dim3(nblk(N, 256)), dim3(256), 0, s, N, w, (long long)e.Lq[0], (long long)e.Lq[1],
-                       nl.ncx, nl.ncy, nl.col_of, nl.col_count);
+    const size_t col_lds = nl.ncol <= COL_LDS_MAX ? (size_t)nl.ncol * sizeof(int) : 0;
+    hipLaunchKernelGGL(k_col_assign, dim3(nblk(N, 256 * COL_ITEMS)), dim3(256), col_lds, s, N, w, (long long)e.Lq[0],
+                       (long long)e.Lq[1], nl.ncx, nl.ncy, nl.col_of, nl.col_count);
     launch_exclusive_scan(nl.col_count, nl.col_start, nl.ncol, s);
-    hipLaunchKernelGGL(k_col_scatter, dim3(nblk(N, 256)), dim3(256), 0, s, N, w, nl.col_of, nl.col_start, nl.col_fill, nl.keys);
+    hipLaunchKernelGGL(k_col_scatter, dim3(nblk(N, 256 * COL_ITEMS)), dim3(256), col_lds, s, N, nl.ncol, w, nl.col_of, nl.col_start,
+                       nl.col_fill, nl.keys);
     hipLaunchKernelGGL(k_col_sort, dim3(nl.ncol), dim3(1024), 0, s, nl.col_start, nl.keys, nl.col_ncl, nl.flags);
     launch_exclusive_scan(nl.col_ncl, nl.col_count, nl.ncol, s);   // col_count now holds the cluster start per column
     int hflags[4];
@@ -417,14 +609,14 @@ void nlist_build(Engine& e, const uint4* w) {
     hipLaunchKernelGGL(k_fill_int, dim3(nblk((int64_t)ncl * CLUSTER, 256)), dim3(256), 0, s, nl.catoms, ncl * CLUSTER, -1);
     hipLaunchKernelGGL(k_assign_slots, dim3(nblk(N, 256)), dim3(256), 0, s, N, nl.ncol, nl.col_start, col_cl_start, nl.keys,
                        nl.catoms, nl.slot_of);
-    hipLaunchKernelGGL(k_cluster_props, dim3(nblk(ncl, 256)), dim3(256), 0, s, ncl, nl.ncol, col_cl_start, nl.catoms, w,
-                       e.p.prm, e.p.comp, nl.ccol, nl.cref, nl.cmin, nl.cmax, nl.ljs, nl.comps);
+    hipLaunchKernelGGL(k_cluster_props, dim3(nblk((int64_t)ncl * CLUSTER, 256)), dim3(256), 0, s, ncl, nl.ncol, col_cl_start,
+                       nl.catoms, w, e.p.prm, e.p.comp, nl.ccol, nl.cref, nl.cmin, nl.cmax, nl.ljs, nl.comps, nl.cw);
     NlArgs a{};
     a.ncl = ncl; a.ncx = nl.ncx; a.ncy = nl.ncy; a.nC = e.nC;
     for (int d = 0; d < 3; d++) a.Lq[d] = e.Lq[d];
     a.R = llround(((double)e.rc + (double)e.skin) * 1e8);
     a.R2 = a.R * a.R;
-    a.col_cl_start = col_cl_start; a.catoms = nl.catoms; a.comps = nl.comps; a.bmask = e.bmask; a.w = w;
+    a.col_cl_start = col_cl_start; a.catoms = nl.catoms; a.comps = nl.comps; a.bmask = e.bmask; a.cw = nl.cw;
     a.cref = nl.cref; a.cmin = nl.cmin; a.cmax = nl.cmax; a.ecount = nl.ecount; a.flags = nl.flags;
     for (int attempt = 0;; attempt++) {
         const int64_t need = (int64_t)nl.ncl_max * nl.cap;
@@ -440,10 +632,7 @@ void nlist_build(Engine& e, const uint4* w) {
         hipLaunchKernelGGL(k_nlist, dim3(nblk(ncl, 4)), dim3(256), 0, s, a);
         HIP_CHECK(hipMemcpyAsync(hflags, nl.flags, sizeof(hflags), hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipStreamSynchronize(s));
-        if (!hflags[1]) {
-            nl.nent = hflags[2];
-            break;
-        }
+        if (!hflags[1]) break;
         if (attempt >= 6) throw LimaFail(LIMA_ERR_LIST_OVERFLOW, "pair list capacity per cluster exceeded");
         nl.cap = (nl.cap * 3 / 2 + 7) & ~7;
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(4), 0, s, nl.flags, 4, 0);
