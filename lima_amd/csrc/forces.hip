@@ -390,7 +390,8 @@ ERegions energy_regions(const Engine& e) {
     r.bond = r.intra + e.n_intra;
     const int nterms = e.bd.nb + e.bd.na + e.bd.nd + e.bd.ni + e.bd.np;
     r.pme = r.bond + (nterms + 63) / 64;
-    r.kin = r.pme + (int)((e.N + 63) / 64);
+    // PME records: one per gather wave (4 per tile block when tiled)
+    r.kin = r.pme + (e.pme.tiled ? e.pme.ntiles * 4 : (int)((e.N + 63) / 64));
     r.total = r.kin + (int)((e.N + 63) / 64);
     return r;
 }

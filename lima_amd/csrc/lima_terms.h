@@ -113,10 +113,15 @@ __device__ __forceinline__ float term_pair14(v3 d, float2 p, v3* fa) {
 // Grid coordinate of a wrapped integer Coord w in [0, Lq): base node ix = floor(w K / Lq) and
 // fraction f, exact in integers.
 __device__ __forceinline__ void grid_coord(uint32_t w, long long Lq, int K, int* ix, float* f) {
-    const unsigned long long num = (unsigned long long)w * (unsigned long long)K;
-    const unsigned long long q = num / (unsigned long long)Lq;
+    // ix = floor(w K / Lq) exactly: w K < 2^41 and Lq are exact doubles, so the double product with
+    // 1/Lq is within one unit of the floor, and one integer correction step makes it exact
+    const long long num = (long long)((unsigned long long)w * (unsigned long long)K);
+    long long q = (long long)((double)num * (1.0 / (double)Lq));
+    long long r = num - q * Lq;
+    if (r < 0) { q--; r += Lq; }
+    else if (r >= Lq) { q++; r -= Lq; }
     *ix = (int)q;
-    *f = (float)(num - q * (unsigned long long)Lq) / (float)Lq;
+    *f = (float)r / (float)Lq;
 }
 
 // Cubic B-spline weights at nodes ix-1 .. ix+2 (DeviceAlgorithmsPrivate.cuh:9-12).

@@ -177,22 +177,49 @@ struct TileArgs {
     int* flags;
 };
 
-// Charged particles -> fixed-capacity bin of the tile holding their current base node.
-// The order inside a bin does not matter: the block sums are integer.
-__global__ void k_bin(TileArgs a) {
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= a.N) return;
-    if (a.prm[p].x == 0.f) return;
-    const uint4 pw = a.w[p];
-    int ix[3];
-    float f;
-    grid_coord(pw.x, a.Lq[0], a.K[0], &ix[0], &f);
-    grid_coord(pw.y, a.Lq[1], a.K[1], &ix[1], &f);
-    grid_coord(pw.z, a.Lq[2], a.K[2], &ix[2], &f);
-    const int t = ((ix[2] / a.T[2]) * a.nt[1] + ix[1] / a.T[1]) * a.nt[0] + ix[0] / a.T[0];
-    const int k = atomicAdd(&a.bin_count[t], 1);
-    if (k < a.bincap) a.bin_atoms[(int64_t)t * a.bincap + k] = (int)p;
-    else atomicOr(a.flags, 1);
+constexpr int BIN_ITEMS = 1;          // particles per thread in the binning kernel
+constexpr int BIN_LDS_MAX = 16384;    // tiles counted in LDS; more tiles use global atomics directly
+
+// Charged particles -> fixed-capacity bin of the tile holding their current base node.  The order
+// inside a bin does not matter: the block sums are integer.  Each block ranks its particles per
+// tile in LDS and reserves one range per (block, tile) with a single global atomic.  The tile
+// counters start at zero: the tiled gather clears each counter after its last use.
+__global__ __launch_bounds__(256) void k_bin(TileArgs a) {
+    extern __shared__ int hist[];
+    const int ntiles = a.nt[0] * a.nt[1] * a.nt[2];
+    const bool lds = ntiles <= BIN_LDS_MAX;
+    if (lds) for (int t = threadIdx.x; t < ntiles; t += blockDim.x) hist[t] = 0;
+    __syncthreads();
+    const int64_t p0 = (int64_t)blockIdx.x * blockDim.x * BIN_ITEMS;
+    int tile[BIN_ITEMS], rank[BIN_ITEMS];
+#pragma unroll
+    for (int k = 0; k < BIN_ITEMS; k++) {
+        const int64_t p = p0 + (int64_t)k * blockDim.x + threadIdx.x;
+        tile[k] = -1;
+        rank[k] = 0;
+        if (p < a.N && a.prm[p].x != 0.f) {
+            const uint4 pw = a.w[p];
+            int ix[3];
+            float f;
+            grid_coord(pw.x, a.Lq[0], a.K[0], &ix[0], &f);
+            grid_coord(pw.y, a.Lq[1], a.K[1], &ix[1], &f);
+            grid_coord(pw.z, a.Lq[2], a.K[2], &ix[2], &f);
+            tile[k] = ((ix[2] / a.T[2]) * a.nt[1] + ix[1] / a.T[1]) * a.nt[0] + ix[0] / a.T[0];
+            rank[k] = lds ? atomicAdd(&hist[tile[k]], 1) : atomicAdd(&a.bin_count[tile[k]], 1);
+        }
+    }
+    __syncthreads();
+    if (lds)
+        for (int t = threadIdx.x; t < ntiles; t += blockDim.x)
+            if (hist[t]) hist[t] = atomicAdd(&a.bin_count[t], hist[t]);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < BIN_ITEMS; k++) {
+        if (tile[k] < 0) continue;
+        const int pos = (lds ? hist[tile[k]] : 0) + rank[k];
+        if (pos < a.bincap) a.bin_atoms[(int64_t)tile[k] * a.bincap + pos] = (int)(p0 + (int64_t)k * blockDim.x + threadIdx.x);
+        else atomicOr(a.flags, 1);
+    }
 }
 
 // One block per tile: particles of the bin add q w_x w_y w_z (fixed point) into an E^3 LDS block
@@ -439,6 +466,125 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
     }
 }
 
+struct GatherTileArgs {
+    int K[3], T[3], nt[3];
+    long long Lq[3];
+    int bincap;
+    int nslots;
+    float inv2h[3];
+    float eself;
+    const uint4* w;
+    const float4* prm;
+    const float* phi;
+    const int* slot_of;
+    const int* bin_atoms;
+    int* bin_count;
+    float* fs;
+    float* es;
+    double* epart;
+};
+
+__device__ __forceinline__ float pick6(const float* v, int i) {   // v[i] for a per-lane i in 0..5, as a masked sum (no stack array)
+    float r = 0.f;
+#pragma unroll
+    for (int m = 0; m < 6; m++) r = fmaf(v[m], (float)(m == i), r);
+    return r;
+}
+
+// One block per spreading tile: the potential on nodes corner-2 .. corner+T+2 of the tile (every node
+// a 6-point gather stencil of a particle based in the tile reads) is staged in LDS.  Each particle
+// of the tile's bin is gathered by 8 lanes: lane j < 6 sums the 36 nodes of plane mz = j, and xor
+// shuffles add the partial sums.  The block clears the tile's bin counter for
+// the next step's binning.  Energy: one FP64 record per wave.
+template <bool ENERGY>
+__global__ __launch_bounds__(256) void k_gather_tiles(GatherTileArgs a) {
+    extern __shared__ float phib[];
+    const int t = blockIdx.x;
+    const int G0 = a.T[0] + 5, G1 = a.T[1] + 5, G2 = a.T[2] + 5;
+    const int tx = t % a.nt[0], ty = (t / a.nt[0]) % a.nt[1], tz = t / (a.nt[0] * a.nt[1]);
+    const int o0 = tx * a.T[0] - 2, o1 = ty * a.T[1] - 2, o2 = tz * a.T[2] - 2;
+    {   // rows of the block: 32 lanes per row (G0 <= 17)
+        const int lx = threadIdx.x & 31;
+        const float rG1 = 1.f / (float)G1;
+        for (int r = threadIdx.x >> 5; r < G1 * G2; r += blockDim.x >> 5) {
+            const int lz = (int)(((float)r + 0.5f) * rG1), ly = r - lz * G1;
+            if (lx < G0) {
+                const int X = wrapi(o0 + lx, a.K[0]), Y = wrapi(o1 + ly, a.K[1]), Z = wrapi(o2 + lz, a.K[2]);
+                phib[r * G0 + lx] = a.phi[((int64_t)Z * a.K[1] + Y) * a.K[0] + X];
+            }
+        }
+    }
+    __syncthreads();
+    const int nb = min(a.bin_count[t], a.bincap);
+    const int j = threadIdx.x & 7;
+    float ephi = 0.f;
+    for (int k0 = 0; k0 < nb; k0 += blockDim.x >> 3) {   // uniform trip count per block
+        const int k = k0 + (threadIdx.x >> 3);
+        const bool live = k < nb;
+        const int p = live ? a.bin_atoms[(int64_t)t * a.bincap + k] : a.bin_atoms[(int64_t)t * a.bincap];
+        float q = 0.f, ph = 0.f, ex = 0.f, ey = 0.f, ez = 0.f;
+        if (live) {
+            q = a.prm[p].x;
+            const uint4 pw = a.w[p];
+            int ix[3];
+            float f[3], wx[4], wy[4], wz[4];
+            grid_coord(pw.x, a.Lq[0], a.K[0], &ix[0], &f[0]);
+            grid_coord(pw.y, a.Lq[1], a.K[1], &ix[1], &f[1]);
+            grid_coord(pw.z, a.Lq[2], a.K[2], &ix[2], &f[2]);
+            bspline(f[0], wx); bspline(f[1], wy); bspline(f[2], wz);
+            float w6x[6], w6y[6], w6z[6], d6x[6], d6y[6], d6z[6];
+            gather_weights(wx, w6x, d6x);
+            gather_weights(wy, w6y, d6y);
+            gather_weights(wz, w6z, d6z);
+            // the particle's base node lies in this tile (binned from the same positions), so its
+            // stencil ix-2 .. ix+3 is local nodes ix - corner .. ix - corner + 5
+            const int l0 = ix[0] - tx * a.T[0], l1 = ix[1] - ty * a.T[1], l2 = ix[2] - tz * a.T[2];
+            if (j < 6) {
+                const float wz6 = pick6(w6z, j), dz6 = pick6(d6z, j);
+#pragma unroll
+                for (int my = 0; my < 6; my++) {
+                    const float wyz = w6y[my] * wz6, dyz = d6y[my] * wz6, wdz = w6y[my] * dz6;
+                    const float* rowp = phib + ((l2 + j) * G1 + (l1 + my)) * G0 + l0;
+#pragma unroll
+                    for (int mx = 0; mx < 6; mx++) {
+                        const float v = rowp[mx];
+                        ph += w6x[mx] * wyz * v;
+                        ex += d6x[mx] * wyz * v;
+                        ey += w6x[mx] * dyz * v;
+                        ez += w6x[mx] * wdz * v;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int o = 1; o < 8; o <<= 1) {
+            ph += __shfl_xor(ph, o, 64);
+            ex += __shfl_xor(ex, o, 64);
+            ey += __shfl_xor(ey, o, 64);
+            ez += __shfl_xor(ez, o, 64);
+        }
+        if (live && j == 0) {
+            const int s = a.slot_of[p];
+            atomicAdd(&a.fs[s], q * ex * a.inv2h[0]);
+            atomicAdd(&a.fs[a.nslots + s], q * ey * a.inv2h[1]);
+            atomicAdd(&a.fs[2 * a.nslots + s], q * ez * a.inv2h[2]);
+            if (ENERGY) {
+                atomicAdd(&a.es[s], 0.5f * (q * ph + a.eself));
+                ephi += 0.5f * q * ph;
+            }
+        }
+    }
+    if (ENERGY) {
+        const double e = wave_sum_d((double)ephi);
+        if ((threadIdx.x & 63) == 0) {
+            double* r = a.epart + ((int64_t)t * (blockDim.x >> 6) + (threadIdx.x >> 6)) * ACC_N;
+            r[ACC_LJ] = 0.0; r[ACC_COUL] = 0.0; r[ACC_BOND] = 0.0; r[ACC_PME] = e; r[ACC_KIN] = 0.0; r[ACC_PAIRS] = 0.0;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) a.bin_count[t] = 0;
+}
+
 __global__ void k_zero_ll(long long* p, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
@@ -484,10 +630,6 @@ TileArgs tile_args(Engine& e, const uint4* w) {
     return a;
 }
 
-__global__ void k_zero_int(int* p, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0;
-}
 
 }  // namespace
 
@@ -574,6 +716,7 @@ void pme_alloc(Engine& e, const std::vector<float>& q) {
         m.ntiles = m.nt[0] * m.nt[1] * m.nt[2];
         if (m.tiled) {
             HIP_CHECK(hipMalloc(&m.bin_count, (m.ntiles + 1) * sizeof(int)));
+            HIP_CHECK(hipMemset(m.bin_count, 0, (m.ntiles + 1) * sizeof(int)));   // later cleared by the gather
             HIP_CHECK(hipMalloc(&m.bin_atoms, (int64_t)m.ntiles * m.bincap * sizeof(int)));
             HIP_CHECK(hipMalloc(&m.tblock, (int64_t)m.ntiles * m.E[0] * m.E[1] * m.E[2] * sizeof(long long)));
             HIP_CHECK(hipMalloc(&m.flags, sizeof(int)));
@@ -620,11 +763,11 @@ void launch_pme(Engine& e, const uint4* w, bool energy, hipStream_t s) {
     const double vcell = (e.L[0] / Kx) * (e.L[1] / Ky) * (e.L[2] / Kz);
     const FftPlan Px = make_plan(m, 0), Py = make_plan(m, 1), Pz = make_plan(m, 2);
     // LDS tiles stay <= 48 KB: 2 buffers x lines x length x 8 B
-    const int TY = std::max(1, std::min(Ky, 3072 / Kx));
+    const int TY = std::max(1, std::min(8, 3072 / Kx));   // lines per workgroup: a few hundred workgroups per pass
     if (m.tiled) {
         TileArgs ta = tile_args(e, w);
-        hipLaunchKernelGGL(k_zero_int, dim3(nblk(m.ntiles, 256)), dim3(256), 0, s, m.bin_count, m.ntiles);
-        hipLaunchKernelGGL(k_bin, dim3(nblk(e.N, 256)), dim3(256), 0, s, ta);
+        hipLaunchKernelGGL(k_bin, dim3(nblk(e.N, 256 * BIN_ITEMS)), dim3(256),
+                           m.ntiles <= BIN_LDS_MAX ? (size_t)m.ntiles * sizeof(int) : 0, s, ta);
         const int E3 = m.E[0] * m.E[1] * m.E[2];
         hipLaunchKernelGGL(k_spread_tiles, dim3(m.ntiles), dim3(256), (size_t)E3 * sizeof(long long), s, ta);
         hipLaunchKernelGGL(k_fft_x_fwd_tiled, dim3(nblk(Ky, TY), Kz), dim3(256), 2 * TY * Kx * sizeof(float2), s, Px, Ky, H,
@@ -640,7 +783,7 @@ void launch_pme(Engine& e, const uint4* w, bool energy, hipStream_t s) {
         hipLaunchKernelGGL(k_fft_x_fwd, dim3(nblk(Ky, TY), Kz), dim3(256), 2 * TY * Kx * sizeof(float2), s, Px, Ky, H, TY,
                            (float)(1.0 / vcell), m.qgrid, m.cgrid);
     }
-    const int TKy = std::max(1, std::min(16, 3072 / Ky)), TKz = std::max(1, std::min(16, 3072 / Kz));
+    const int TKy = std::max(1, std::min(8, 3072 / Ky)), TKz = std::max(1, std::min(8, 3072 / Kz));
     // y pass: outer z, lines along y (stride H)
     hipLaunchKernelGGL(k_fft_strided, dim3(nblk(H, TKy), Kz), dim3(256), 2 * TKy * Ky * sizeof(float2), s, Py, H, TKy,
                        (int64_t)Ky * H, (int64_t)H, 0, Ky, m.cgrid, m.green);
@@ -651,6 +794,23 @@ void launch_pme(Engine& e, const uint4* w, bool energy, hipStream_t s) {
                        (int64_t)Ky * H, (int64_t)H, 1, Ky, m.cgrid, m.green);
     hipLaunchKernelGGL(k_fft_x_inv, dim3(nblk(Ky, TY), Kz), dim3(256), 2 * TY * Kx * sizeof(float2), s, Px, Ky, H, TY,
                        (float)(1.0 / (double)NG), m.cgrid, m.phi);
+    if (m.tiled) {
+        GatherTileArgs gt{};
+        for (int d = 0; d < 3; d++) {
+            gt.K[d] = m.K[d]; gt.T[d] = m.T[d]; gt.nt[d] = m.nt[d]; gt.Lq[d] = e.Lq[d];
+            gt.inv2h[d] = (float)(m.K[d] / (2.0 * e.L[d]));
+        }
+        gt.bincap = m.bincap;
+        gt.nslots = e.nl.ncl * CLUSTER;
+        gt.eself = (float)m.eself_total;
+        gt.w = w; gt.prm = e.p.prm; gt.phi = m.phi; gt.slot_of = e.nl.slot_of; gt.bin_atoms = m.bin_atoms;
+        gt.bin_count = m.bin_count; gt.fs = e.fs; gt.es = e.es;
+        gt.epart = e.epart + (int64_t)e.ereg.pme * ACC_N;
+        const size_t lds = (size_t)(m.T[0] + 5) * (m.T[1] + 5) * (m.T[2] + 5) * sizeof(float);
+        if (energy) hipLaunchKernelGGL(k_gather_tiles<true>, dim3(m.ntiles), dim3(256), lds, s, gt);
+        else hipLaunchKernelGGL(k_gather_tiles<false>, dim3(m.ntiles), dim3(256), lds, s, gt);
+        return;
+    }
     GatherArgs ga{};
     ga.N = e.N;
     ga.nslots = e.nl.ncl * CLUSTER;
