@@ -88,7 +88,7 @@ struct BatchArgs {
     float4* f0; float4* f1;      // forces
     const float4* prm;           // (q kC/mol, sigma/2 nm, sqrt(eps), sqrt(24 eps))
     const int* comp;             // local compound of each atom
-    const int2* cinfo;           // per compound (first atom local, atom count)
+    const int4* cinfo;           // per compound (first atom local, atom count, key atom local)
     const int* cbm;              // per compound bonded-compound mask (bit k: compound c+k-2)
     const uint32_t* clut;        // per compound [5][32] exclusion rows
     const int* inc_off;          // per atom incidence offsets (global atom index, N+1)
@@ -132,10 +132,19 @@ __device__ __forceinline__ int fmodi(int x, int L) {
     return r < 0 ? r + L : r;
 }
 
-__device__ __forceinline__ v3 udisp(const int4* u, int a, int b) {   // x_b - x_a [nm], unwrapped
+__device__ __forceinline__ int min_image(int d, int Lq) {   // representative of d mod Lq in [-Lq/2, Lq/2]
+    int r = d % Lq;
+    r += r < 0 ? Lq : 0;
+    r -= 2 * r > Lq ? Lq : 0;
+    return r;
+}
+
+// x_b - x_a [nm] of bonded particles: the nearest image (LIMA's hyperpos result for particles a
+// few bonds apart; the Coords of different compounds may differ by whole boxes)
+__device__ __forceinline__ v3 bdisp(const int4* u, int a, int b, const SysMeta& m) {
     const int4 ua = u[a], ub = u[b];
-    return {(float)(ub.x - ua.x) * LIMA_LIMA_TO_NANO_F, (float)(ub.y - ua.y) * LIMA_LIMA_TO_NANO_F,
-            (float)(ub.z - ua.z) * LIMA_LIMA_TO_NANO_F};
+    return {(float)min_image(ub.x - ua.x, m.Lq[0]) * LIMA_LIMA_TO_NANO_F, (float)min_image(ub.y - ua.y, m.Lq[1]) * LIMA_LIMA_TO_NANO_F,
+            (float)min_image(ub.z - ua.z, m.Lq[2]) * LIMA_LIMA_TO_NANO_F};
 }
 
 // One cutoff pair computed once: force on i += d S, on j -= d S, the whole pair energy.
@@ -197,17 +206,29 @@ __device__ __forceinline__ float rot1f(float v) { return __int_as_float(rot1(__f
 __device__ __forceinline__ void intra_atom(const BatchArgs& a, const SysMeta& m, const int4* u, int li, int ci, int bmi, float4 pi,
                                            v3& acc, float& elj, float& ec) {
     const int gi = m.atom0 + li;
-    const int2 ca = a.cinfo[m.comp0 + ci];
+    const int4 ca = a.cinfo[m.comp0 + ci];
     const int si = li - ca.x;
+    const int4 ui = u[gi];
     for (int k = 0; k < 5; k++) {
         if (!((bmi >> k) & 1)) continue;
         const int B = ci + k - 2;
         const uint32_t row = a.clut[((size_t)(m.comp0 + ci) * 5 + k) * 32 + si];
-        const int2 cb = a.cinfo[m.comp0 + B];
+        const int4 cb = a.cinfo[m.comp0 + B];
+        // within the compound unwrapped; a bonded compound moves to the image whose key particle
+        // is nearest to this compound's (LIMA's hyperpos of the neighbour's origo)
+        int sx = 0, sy = 0, sz = 0;
+        if (B != ci) {
+            const int4 ka = u[m.atom0 + ca.z], kb = u[m.atom0 + cb.z];
+            sx = min_image(kb.x - ka.x, m.Lq[0]) - (kb.x - ka.x);
+            sy = min_image(kb.y - ka.y, m.Lq[1]) - (kb.y - ka.y);
+            sz = min_image(kb.z - ka.z, m.Lq[2]) - (kb.z - ka.z);
+        }
         for (int sj = 0; sj < cb.y; sj++) {
             if ((row >> sj) & 1u) continue;
             const int gj = m.atom0 + cb.x + sj;
-            const v3 d = udisp(u, gi, gj);
+            const int4 uj = u[gj];
+            const v3 d = {(float)(uj.x - ui.x + sx) * LIMA_LIMA_TO_NANO_F, (float)(uj.y - ui.y + sy) * LIMA_LIMA_TO_NANO_F,
+                          (float)(uj.z - ui.z + sz) * LIMA_LIMA_TO_NANO_F};
             const float r2 = dot(d, d);
             const float4 pj = a.prm[gj];
             float e1, e2;
@@ -221,7 +242,7 @@ __device__ __forceinline__ void intra_atom(const BatchArgs& a, const SysMeta& m,
 
 // bonded terms containing atom gi, summed from its incidence list; the term energy is counted by
 // the incidence with role 0
-__device__ __forceinline__ void bonded_atom(const BatchArgs& a, const int4* u, int gi, v3& acc, float& eb) {
+__device__ __forceinline__ void bonded_atom(const BatchArgs& a, const SysMeta& m, const int4* u, int gi, v3& acc, float& eb) {
     const int e0 = a.inc_off[gi], e1 = a.inc_off[gi + 1];
     for (int q = e0; q < e1; q++) {
         const int code = a.inc[q];
@@ -231,27 +252,27 @@ __device__ __forceinline__ void bonded_atom(const BatchArgs& a, const int4* u, i
         if (type == T_BOND) {
             const int2 id = a.bid[t];
             v3 fa;
-            e = term_bond(udisp(u, id.y, id.x), a.bp[t], &fa);
+            e = term_bond(bdisp(u, id.y, id.x, m), a.bp[t], &fa);
             f = role == 0 ? fa : -fa;
         } else if (type == T_ANGLE) {
             const int4 id = a.aid[t];
             v3 fl, fr;
-            e = term_angle(udisp(u, id.y, id.x), udisp(u, id.y, id.z), udisp(u, id.z, id.x), a.ap[t], &fl, &fr);
+            e = term_angle(bdisp(u, id.y, id.x, m), bdisp(u, id.y, id.z, m), bdisp(u, id.z, id.x, m), a.ap[t], &fl, &fr);
             f = role == 0 ? fl : (role == 2 ? fr : -(fl + fr));
         } else if (type == T_DIH) {
             const int4 id = a.did[t];
             v3 fd[4];
-            e = term_dihedral(udisp(u, id.x, id.y), udisp(u, id.y, id.z), udisp(u, id.z, id.w), a.dp[t], fd);
+            e = term_dihedral(bdisp(u, id.x, id.y, m), bdisp(u, id.y, id.z, m), bdisp(u, id.z, id.w, m), a.dp[t], fd);
             f = role == 0 ? fd[0] : (role == 1 ? fd[1] : (role == 2 ? fd[2] : fd[3]));
         } else if (type == T_IMP) {
             const int4 id = a.iid[t];
             v3 fd[4];
-            e = term_improper(udisp(u, id.x, id.y), udisp(u, id.x, id.z), udisp(u, id.x, id.w), a.ip[t], fd);
+            e = term_improper(bdisp(u, id.x, id.y, m), bdisp(u, id.x, id.z, m), bdisp(u, id.x, id.w, m), a.ip[t], fd);
             f = role == 0 ? fd[0] : (role == 1 ? fd[1] : (role == 2 ? fd[2] : fd[3]));
         } else {
             const int2 id = a.pid[t];
             v3 fa;
-            e = term_pair14(udisp(u, id.x, id.y), a.pp[t], &fa);
+            e = term_pair14(bdisp(u, id.x, id.y, m), a.pp[t], &fa);
             f = role == 0 ? fa : -fa;
         }
         acc = acc + f;
@@ -567,7 +588,7 @@ __device__ __forceinline__ Eval evaluate(const BatchArgs& a, const SysMeta& m, B
             const int ci = a.comp[gi];
             float elj = 0.f, ec = 0.f, eb = 0.f;
             intra_atom(a, m, u, li, ci, a.cbm[m.comp0 + ci], a.prm[gi], acc, elj, ec);
-            bonded_atom(a, u, gi, acc, eb);
+            bonded_atom(a, m, u, gi, acc, eb);
             f[gi] = make_float4(acc.x, acc.y, acc.z, 0.f);
             elj_d += (double)elj;
             ec_d += (double)ec;
@@ -951,7 +972,7 @@ struct Batch {
     float4 *f0 = nullptr, *f1 = nullptr;
     float4* prm4 = nullptr;
     int* comp = nullptr;
-    int2* cinfo = nullptr;
+    int4* cinfo = nullptr;
     int* cbm = nullptr;
     uint32_t* clut = nullptr;
     int *inc_off = nullptr, *inc = nullptr;
@@ -1006,7 +1027,7 @@ void batch_create(Batch& B, const lima_box_desc* boxes, int nsys, const lima_sim
     std::vector<int4> u;
     std::vector<float4> prm;
     std::vector<int> comp, cbm;
-    std::vector<int2> cinfo;
+    std::vector<int4> cinfo;
     std::vector<uint32_t> clut;
     std::vector<int2> bid, pid; std::vector<float2> bp, pp, ip;
     std::vector<int4> aid, did, iid; std::vector<float4> ap, dp;
@@ -1086,7 +1107,9 @@ void batch_create(Batch& B, const lima_box_desc* boxes, int nsys, const lima_sim
         auto lutrow = [&](int c, int k, int i) -> uint32_t { return lut ? lut[((size_t)c * 5 + k) * 32 + i] : (k == 2 ? (1u << i) : 0u); };
         for (int c = 0; c < nC + nT; c++) {
             const bool tiny = c >= nC;
-            cinfo.push_back(make_int2(tiny ? first[nC] + (c - nC) : first[c], tiny ? 1 : b->compound_n_particles[c]));
+            const int cf = tiny ? first[nC] + (c - nC) : first[c];
+            const int ck = cf + (tiny || !b->compound_key_particle ? 0 : b->compound_key_particle[c]);
+            cinfo.push_back(make_int4(cf, tiny ? 1 : b->compound_n_particles[c], ck, 0));
             int bm = 1 << 2;
             for (int k = 0; k < 5; k++) {
                 const int Bc = c + k - 2;

@@ -203,6 +203,7 @@ void launch_pack(Engine& e, const uint4* w, hipStream_t s);
 void launch_pair(Engine& e, bool energy, hipStream_t s);
 void launch_intra(Engine& e, const uint4* w, const int4* img, bool energy, hipStream_t s);
 void launch_bonded(Engine& e, const uint4* w, const int4* img, bool energy, hipStream_t s);
+void launch_local(Engine& e, const uint4* w, const int4* img, bool energy, hipStream_t s);   // intra + bonded, one launch
 
 // ---- pme.hip ----
 void pme_alloc(Engine& e, const std::vector<float>& charges_global);

@@ -270,10 +270,14 @@ void evaluate(Engine& e, const uint4* w, const int4* img, bool energy, bool time
     }
     launch_pair(e, energy, s);
     if (ev) HIP_CHECK(hipEventRecord(ev[2], s));
-    launch_intra(e, w, img, energy, s);
-    if (ev) HIP_CHECK(hipEventRecord(ev[3], s));
-    launch_bonded(e, w, img, energy, s);
-    if (ev) HIP_CHECK(hipEventRecord(ev[4], s));
+    if (ev) {   // timed: each phase alone
+        launch_intra(e, w, img, energy, s);
+        HIP_CHECK(hipEventRecord(ev[3], s));
+        launch_bonded(e, w, img, energy, s);
+        HIP_CHECK(hipEventRecord(ev[4], s));
+    } else {
+        launch_local(e, w, img, energy, s);
+    }
     if (split) HIP_CHECK(hipStreamWaitEvent(s, e.ev_join, 0));
     else launch_pme(e, w, energy, s);
     if (ev) HIP_CHECK(hipEventRecord(ev[5], s));
@@ -482,7 +486,8 @@ void create(Engine& e, const lima_box_desc* b, const lima_simparams* p) {
                     for (int j = 0; j < nB; j++) any |= !((r >> j) & 1u);
             }
             if (!any) continue;
-            items.push_back(make_int4(A, B, 0, 0));
+            // (A, B, key particle of A, key particle of B): the keys fix B's image for A != B
+            items.push_back(make_int4(A, B, (int)(e.h_first[A] + e.h_key[A]), (int)(e.h_first[B] + e.h_key[B])));
             for (int i = 0; i < 32; i++) ilut.push_back(rows[i]);
         }
     }

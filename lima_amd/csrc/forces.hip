@@ -10,7 +10,9 @@
 //
 // Intra kernel: LIMA's immediate-neighbour path (EngineKernels.cuh:150-263,
 // LennardJonesInteractions.cuh:121-161): pairs of the same or bonded compounds, BondedParticlesLUT
-// exclusions, no cutoff, unwrapped displacement, force on self only (as LIMA evaluates it).
+// exclusions, no cutoff, force on self only (as LIMA evaluates it).  Within a compound the
+// displacement is unwrapped; a bonded compound is first moved to the image whose key particle is
+// nearest (LIMA's hyperpos of the neighbour's origo, EngineKernels.cuh:235-236).
 //
 // Bonded kernel: ForceComputations.cuh:17-264,335-376; one thread per term, forces added
 // to the particles' cluster slots with float atomics.
@@ -204,16 +206,48 @@ __device__ __forceinline__ void unwrapped3(const uint4* w, const int4* img, cons
     u[2] = (long long)p.z + (long long)m.z * Lq[2];
 }
 
-// One wave per (A, B) item: lanes 0..31 take i = lane with j in [0,16), lanes 32..63 the same i
+// One wave per (A, B) item: compound B's particles (unwrapped Coord, parameters) are staged in the
+// wave's LDS slice, then lanes 0..31 take i = lane with j in [0,16) and lanes 32..63 the same i
 // with j in [16,32).
+struct IntraLds {
+    long long u[4][32][3];
+    float4 p[4][32];
+};
+
 template <bool ENERGY>
-__global__ __launch_bounds__(256) void k_intra(IntraArgs a) {
-    const int lane = threadIdx.x & 63;
-    const int it = blockIdx.x * 4 + (threadIdx.x >> 6);
+__device__ __forceinline__ void intra_items(const IntraArgs& a, int blk, IntraLds& L) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int it = blk * 4 + wid;
     if (it >= a.n_items) return;
     const int4 item = a.items[it];
     const int A = item.x, B = item.y;
     const int nA = (int)(a.first[A + 1] - a.first[A]), nB = (int)(a.first[B + 1] - a.first[B]);
+    long long sh[3] = {0, 0, 0};   // B's image: its key particle nearest to A's (LIMA's hyperpos)
+    if (A != B) {
+        long long ka[3], kb[3];
+        unwrapped3(a.w, a.img, a.Lq, item.z, ka);
+        unwrapped3(a.w, a.img, a.Lq, item.w, kb);
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const long long dk = kb[d] - ka[d];
+            long long r = dk % a.Lq[d];
+            r += r < 0 ? a.Lq[d] : 0;
+            r -= 2 * r > a.Lq[d] ? a.Lq[d] : 0;
+            sh[d] = r - dk;
+        }
+    }
+    if (lane < nB) {
+        const int bj = (int)a.first[B] + lane;
+        long long ub[3];
+        unwrapped3(a.w, a.img, a.Lq, bj, ub);
+        L.u[wid][lane][0] = ub[0] + sh[0];
+        L.u[wid][lane][1] = ub[1] + sh[1];
+        L.u[wid][lane][2] = ub[2] + sh[2];
+        L.p[wid][lane] = a.prm[bj];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const int i = lane & 31, half = lane >> 5;
     float fx = 0.f, fy = 0.f, fz = 0.f, el_s = 0.f, ec_s = 0.f;
     const int ai = (int)a.first[A] + i;
@@ -224,15 +258,13 @@ __global__ __launch_bounds__(256) void k_intra(IntraArgs a) {
         const uint32_t row = a.lut[it * 32 + i];
         for (int j = half * 16; j < half * 16 + 16 && j < nB; j++) {
             if ((row >> j) & 1u) continue;
-            const int bj = (int)a.first[B] + j;
-            long long uj[3];
-            unwrapped3(a.w, a.img, a.Lq, bj, uj);
+            const long long* uj = L.u[wid][j];
             const float dx = (float)(uj[0] - ui[0]) * LIMA_LIMA_TO_NANO_F;
             const float dy = (float)(uj[1] - ui[1]) * LIMA_LIMA_TO_NANO_F;
             const float dz = (float)(uj[2] - ui[2]) * LIMA_LIMA_TO_NANO_F;
             const float r2 = dx * dx + dy * dy + dz * dz;
             if (!(r2 > 0.f)) continue;
-            const float4 pj = a.prm[bj];
+            const float4 pj = L.p[wid][j];
             float el = 0.f, ec = 0.f;
             const float S = pair_scalar(r2, pi.y + pj.y, pi.z * pj.z, pi.x * pj.x, a.pc, ENERGY ? &el : nullptr, &ec);
             fx += dx * S; fy += dy * S; fz += dz * S;
@@ -277,12 +309,18 @@ struct BondArgs {
     double* epart;
 };
 
-__device__ __forceinline__ v3 bdisp(const BondArgs& a, int from, int to) {   // unwrapped x_to - x_from [nm]
-    long long u0[3], u1[3];
-    unwrapped3(a.w, a.img, a.Lq, from, u0);
-    unwrapped3(a.w, a.img, a.Lq, to, u1);
-    return {(float)(u1[0] - u0[0]) * LIMA_LIMA_TO_NANO_F, (float)(u1[1] - u0[1]) * LIMA_LIMA_TO_NANO_F,
-            (float)(u1[2] - u0[2]) * LIMA_LIMA_TO_NANO_F};
+// x_to - x_from [nm] of two bonded particles.  Bonded displacements are far shorter than half the
+// box, so the minimum image of the wrapped Coord difference is the same integer as the unwrapped
+// difference LIMA uses (no image counters needed).
+__device__ __forceinline__ v3 bdisp(const BondArgs& a, int from, int to) {
+    const uint4 p0 = a.w[from], p1 = a.w[to];
+    long long d[3] = {(long long)p1.x - (long long)p0.x, (long long)p1.y - (long long)p0.y, (long long)p1.z - (long long)p0.z};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        if (2 * d[k] > a.Lq[k]) d[k] -= a.Lq[k];
+        else if (2 * d[k] < -a.Lq[k]) d[k] += a.Lq[k];
+    }
+    return {(float)d[0] * LIMA_LIMA_TO_NANO_F, (float)d[1] * LIMA_LIMA_TO_NANO_F, (float)d[2] * LIMA_LIMA_TO_NANO_F};
 }
 
 __device__ __forceinline__ void addf(const BondArgs& a, int p, v3 f) {
@@ -293,8 +331,8 @@ __device__ __forceinline__ void addf(const BondArgs& a, int p, v3 f) {
 }
 
 template <bool ENERGY>
-__global__ __launch_bounds__(256) void k_bonded(BondArgs a) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void bonded_terms(const BondArgs& a, int blk) {
+    const int t = blk * blockDim.x + threadIdx.x;
     float e = 0.f;
     int ids[4];
     int nat = 0;
@@ -351,6 +389,16 @@ __global__ __launch_bounds__(256) void k_bonded(BondArgs a) {
             r[ACC_LJ] = 0.0; r[ACC_COUL] = 0.0; r[ACC_BOND] = tot; r[ACC_PME] = 0.0; r[ACC_KIN] = 0.0; r[ACC_PAIRS] = 0.0;
         }
     }
+}
+
+// Intra-compound pairs and bonded terms in one launch: blocks [0, nib) take intra items (4 per
+// block), the rest bonded terms (256 per block).  Both are short, latency-bound passes; sharing a
+// launch lets their memory latencies overlap.
+template <bool ENERGY>
+__global__ __launch_bounds__(256) void k_local(IntraArgs ia, BondArgs ba, int nib) {
+    __shared__ IntraLds L;
+    if ((int)blockIdx.x < nib) intra_items<ENERGY>(ia, blockIdx.x, L);
+    else bonded_terms<ENERGY>(ba, blockIdx.x - nib);
 }
 
 // Sums the per-wave energy records of all regions in a fixed order: acc[c] = sum_r epart[r][c].
@@ -424,8 +472,7 @@ void launch_pair(Engine& e, bool energy, hipStream_t s) {
     else hipLaunchKernelGGL(k_pair<false>, dim3(nblk(waves, PAIR_WAVES)), dim3(64 * PAIR_WAVES), 0, s, a);
 }
 
-void launch_intra(Engine& e, const uint4* w, const int4* img, bool energy, hipStream_t s) {
-    if (e.n_intra == 0) return;
+IntraArgs intra_args(Engine& e, const uint4* w, const int4* img) {
     IntraArgs a{};
     a.n_items = e.n_intra;
     a.nslots = e.nl.ncl * CLUSTER;
@@ -433,23 +480,42 @@ void launch_intra(Engine& e, const uint4* w, const int4* img, bool energy, hipSt
     a.pc = e.pc;
     a.items = e.intra_items; a.lut = e.intra_lut; a.w = w; a.img = img; a.prm = e.p.prm;
     a.slot_of = e.nl.slot_of; a.first = e.first; a.fs = e.fs; a.es = e.es; a.epart = e.epart + (int64_t)e.ereg.intra * ACC_N;
-    if (energy) hipLaunchKernelGGL(k_intra<true>, dim3(nblk(a.n_items, 4)), dim3(256), 0, s, a);
-    else hipLaunchKernelGGL(k_intra<false>, dim3(nblk(a.n_items, 4)), dim3(256), 0, s, a);
+    return a;
 }
 
-void launch_bonded(Engine& e, const uint4* w, const int4* img, bool energy, hipStream_t s) {
+BondArgs bond_args(Engine& e, const uint4* w, const int4* img) {
     BondArgs a{};
     a.nb = e.bd.nb; a.na = e.bd.na; a.nd = e.bd.nd; a.ni = e.bd.ni; a.np = e.bd.np;
-    const int total = a.nb + a.na + a.nd + a.ni + a.np;
-    if (total == 0) return;
     a.nslots = e.nl.ncl * CLUSTER;
     for (int d = 0; d < 3; d++) a.Lq[d] = e.Lq[d];
     a.pc = e.pc;
     a.bid = e.bd.bid; a.bp = e.bd.bp; a.aid = e.bd.aid; a.ap = e.bd.ap; a.did = e.bd.did; a.dp = e.bd.dp;
     a.iid = e.bd.iid; a.ip = e.bd.ip; a.pid = e.bd.pid; a.pp = e.bd.pp;
     a.w = w; a.img = img; a.slot_of = e.nl.slot_of; a.fs = e.fs; a.es = e.es; a.epart = e.epart + (int64_t)e.ereg.bond * ACC_N;
-    if (energy) hipLaunchKernelGGL(k_bonded<true>, dim3(nblk(total, 256)), dim3(256), 0, s, a);
-    else hipLaunchKernelGGL(k_bonded<false>, dim3(nblk(total, 256)), dim3(256), 0, s, a);
+    return a;
+}
+
+void launch_local_blocks(Engine& e, const uint4* w, const int4* img, bool energy, bool intra, bool bonded, hipStream_t s) {
+    const IntraArgs ia = intra_args(e, w, img);
+    const BondArgs ba = bond_args(e, w, img);
+    const int nterms = ba.nb + ba.na + ba.nd + ba.ni + ba.np;
+    const int nib = intra ? nblk(ia.n_items, 4) : 0;
+    const int nbb = bonded ? nblk(nterms, 256) : 0;
+    if (nib + nbb == 0) return;
+    if (energy) hipLaunchKernelGGL(k_local<true>, dim3(nib + nbb), dim3(256), 0, s, ia, ba, nib);
+    else hipLaunchKernelGGL(k_local<false>, dim3(nib + nbb), dim3(256), 0, s, ia, ba, nib);
+}
+
+void launch_local(Engine& e, const uint4* w, const int4* img, bool energy, hipStream_t s) {
+    launch_local_blocks(e, w, img, energy, true, true, s);
+}
+
+void launch_intra(Engine& e, const uint4* w, const int4* img, bool energy, hipStream_t s) {
+    launch_local_blocks(e, w, img, energy, true, false, s);
+}
+
+void launch_bonded(Engine& e, const uint4* w, const int4* img, bool energy, hipStream_t s) {
+    launch_local_blocks(e, w, img, energy, false, true, s);
 }
 
 }  // namespace lima

@@ -259,9 +259,23 @@ int derive_grid(double L, double spacing) {
     return n;
 }
 
-Vec disp(const lima_oracle* o, int64_t a, int64_t b) {  // unwrapped x_b - x_a [nm]
-    return Vec((double)(o->u[b * 3 + 0] - o->u[a * 3 + 0]), (double)(o->u[b * 3 + 1] - o->u[a * 3 + 1]),
-               (double)(o->u[b * 3 + 2] - o->u[a * 3 + 2])) * (1.0 / NANO_TO_LIMA);
+// Displacements between the particles of one compound are unwrapped (LIMA keeps a compound's
+// particles relative to one origo).  Between compounds LIMA moves the other compound's origo to
+// its nearest image first (BoundaryCondition::applyHyperpos_Return, EngineKernels.cuh:235-236);
+// here the compound's key particle stands for its origo.  Bonded terms join nearby particles, so
+// the nearest image of each displacement is the hyperpos result.
+inline int64_t min_image(int64_t d, int64_t Lq) {   // representative of d mod Lq in [-Lq/2, Lq/2]
+    int64_t r = d % Lq;
+    if (r < 0) r += Lq;
+    if (2 * r > Lq) r -= Lq;
+    return r;
+}
+
+
+Vec bdisp(const lima_oracle* o, int64_t a, int64_t b) {  // nearest image of x_b - x_a [nm]
+    double d[3];
+    for (int k = 0; k < 3; k++) d[k] = (double)min_image(o->u[b * 3 + k] - o->u[a * 3 + k], o->Lq[k]);
+    return Vec(d[0], d[1], d[2]) * (1.0 / NANO_TO_LIMA);
 }
 
 /* ---------------------------------------------------------------------------------------
@@ -332,7 +346,7 @@ void nb_cutoff(lima_oracle* o, std::vector<double>& F, std::vector<double>& pot,
     elj += elj_t; ec += ec_t;
 }
 
-/* Intra / bonded-compound pairs, no cutoff, LUT exclusions, unwrapped displacement.
+/* Intra / bonded-compound pairs, no cutoff, LUT exclusions, compound-frame displacement.
  * EngineKernels.cuh:197-263 with LennardJonesInteractions.cuh:121-161. */
 void nb_intra(lima_oracle* o, std::vector<double>& F, std::vector<double>& pot, double& elj, double& ec) {
     double elj_t = 0, ec_t = 0;
@@ -343,6 +357,14 @@ void nb_intra(lima_oracle* o, std::vector<double>& F, std::vector<double>& pot, 
             if (!((o->bmask[A] >> k) & 1)) continue;
             const int B = A + k - 2;
             const int nB = (int)(o->first[B + 1] - o->first[B]);
+            int64_t sh[3] = {0, 0, 0};   // image shift of compound B: its key particle nearest to A's
+            if (B != A) {
+                const int64_t ka = o->gidx(A, o->key[A]), kb = o->gidx(B, o->key[B]);
+                for (int d = 0; d < 3; d++) {
+                    const int64_t dk = o->u[kb * 3 + d] - o->u[ka * 3 + d];
+                    sh[d] = min_image(dk, o->Lq[d]) - dk;
+                }
+            }
             for (int si = 0; si < nA; si++) {
                 const int64_t i = o->gidx(A, si);
                 Vec fi; double el = 0, e2 = 0;
@@ -350,7 +372,9 @@ void nb_intra(lima_oracle* o, std::vector<double>& F, std::vector<double>& pot, 
                     if (o->lut_excluded(A, si, B, sj)) continue;
                     const int64_t j = o->gidx(B, sj);
                     if (j == i) continue;
-                    const Vec d = disp(o, i, j);
+                    const Vec d((double)(o->u[j * 3 + 0] - o->u[i * 3 + 0] + sh[0]) / NANO_TO_LIMA,
+                                (double)(o->u[j * 3 + 1] - o->u[i * 3 + 1] + sh[1]) / NANO_TO_LIMA,
+                                (double)(o->u[j * 3 + 2] - o->u[i * 3 + 2] + sh[2]) / NANO_TO_LIMA);
                     const double sg = (o->sig[i] + o->sig[j]) * 0.5, ep = std::sqrt(o->eps[i] * o->eps[j]);
                     fi += pair_force(d, sg, ep, o->q[i] * o->q[j], o->kappa, el, e2);
                 }
@@ -371,7 +395,7 @@ double bonded(lima_oracle* o, std::vector<double>& F, std::vector<double>& pot) 
     double etot = 0;
     // bonds, calcSinglebondForces :17-51 (MD form; SD uses the exact gradient)
     for (auto& t : o->bonds) {
-        const Vec dif = disp(o, t.a[1], t.a[0]);   // pos_a - pos_b
+        const Vec dif = bdisp(o, t.a[1], t.a[0]);   // pos_a - pos_b
         const double r = dif.len(), err = r - t.p[0];
         const double e = 0.5 * t.p[1] * err * err;
         const Vec f = dif * (-t.p[1] * err / r);
@@ -380,7 +404,7 @@ double bonded(lima_oracle* o, std::vector<double>& F, std::vector<double>& pot) 
     }
     // angles + Urey-Bradley, calcAnglebondForces :53-115 (harmonic in theta; exact gradient)
     for (auto& t : o->angles) {
-        const Vec r1 = disp(o, t.a[1], t.a[0]), r2 = disp(o, t.a[1], t.a[2]);
+        const Vec r1 = bdisp(o, t.a[1], t.a[0]), r2 = bdisp(o, t.a[1], t.a[2]);
         const double l1 = r1.len(), l2 = r2.len();
         const Vec v1 = r1 * (1.0 / l1), v2 = r2 * (1.0 / l2);
         double c = v1.dot(v2); c = std::min(1.0, std::max(-1.0, c));
@@ -392,7 +416,7 @@ double bonded(lima_oracle* o, std::vector<double>& F, std::vector<double>& pot) 
         const Vec dr = (v1 - v2 * c) * (-1.0 / (l2 * s));
         Vec fl = dl * (-t.p[1] * err), fr = dr * (-t.p[1] * err);
         if (t.p[3] != 0.0) {
-            const Vec d13 = disp(o, t.a[2], t.a[0]);   // left - right
+            const Vec d13 = bdisp(o, t.a[2], t.a[0]);   // left - right
             const double r13 = d13.len(), eu = r13 - t.p[2];
             e += 0.5 * t.p[3] * eu * eu;
             const Vec fu = d13 * (-t.p[3] * eu / r13);
@@ -405,7 +429,7 @@ double bonded(lima_oracle* o, std::vector<double>& F, std::vector<double>& pot) 
     // proper dihedrals, calcDihedralbondForces :119-210.  phi_L = -atan2(sin, cos) with
     // A = r12 x r23, B = r23 x r34 (LIMA's sign convention), V = k(1 + cos(n phi_L - phi0)).
     for (auto& t : o->dihedrals) {
-        const Vec b1 = disp(o, t.a[0], t.a[1]), b2 = disp(o, t.a[1], t.a[2]), b3 = disp(o, t.a[2], t.a[3]);
+        const Vec b1 = bdisp(o, t.a[0], t.a[1]), b2 = bdisp(o, t.a[1], t.a[2]), b3 = bdisp(o, t.a[2], t.a[3]);
         const Vec m = b1.cross(b2), n = b2.cross(b3);
         const double lb2 = b2.len();
         const double phi_bk = std::atan2(lb2 * b1.dot(n), m.dot(n));   // Blondel-Karplus angle
@@ -426,7 +450,7 @@ double bonded(lima_oracle* o, std::vector<double>& F, std::vector<double>& pot) 
     }
     // improper dihedrals, calcImproperdihedralbondForces :214-264 (LIMA's force split, not a gradient)
     for (auto& t : o->impropers) {
-        const Vec pi(0, 0, 0), pj = disp(o, t.a[0], t.a[1]), pk = disp(o, t.a[0], t.a[2]), pl = disp(o, t.a[0], t.a[3]);
+        const Vec pi(0, 0, 0), pj = bdisp(o, t.a[0], t.a[1]), pk = bdisp(o, t.a[0], t.a[2]), pl = bdisp(o, t.a[0], t.a[3]);
         const Vec ij = (pj - pi).norm(), ik = (pk - pi).norm(), il = (pl - pi).norm();
         const Vec lj = (pj - pl).norm(), lk = (pk - pl).norm();
         const Vec n1 = ij.cross(ik).norm(), n2 = lj.cross(lk).norm();
@@ -446,7 +470,7 @@ double bonded(lima_oracle* o, std::vector<double>& F, std::vector<double>& pot) 
     }
     // 1-4 pair LJ, computePairbondForces :335-376
     for (auto& t : o->pairs) {
-        const Vec d = disp(o, t.a[0], t.a[1]);   // other - self (self = a0)
+        const Vec d = bdisp(o, t.a[0], t.a[1]);   // other - self (self = a0)
         double el = 0, ec = 0;
         const Vec f = pair_force(d, t.p[0], t.p[1], 0.0, o->kappa, el, ec);
         addF(F, t.a[0], f); addF(F, t.a[1], -f);

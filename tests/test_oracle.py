@@ -149,6 +149,44 @@ class _Maker:
 BASE4 = np.array([[1.0, 1.0, 1.0], [1.15, 1.02, 1.01], [1.2, 1.16, 1.05], [1.35, 1.2, 1.12]])
 
 
+def _helix_chain(n=40):
+    k = np.arange(n)
+    return np.stack([0.5 + 0.3 * np.cos(1.7 * k), 0.5 + 0.3 * np.sin(1.7 * k), 0.5 + 0.045 * k], 1)
+
+
+@pytest.mark.parametrize("components", [BONDED, NB_INTRA])
+def test_chain_forces_invariant_across_the_periodic_boundary(components):
+    """A 40-particle chain is two compounds (32 + 8).  Translated so that its compounds' key
+    particles wrap to opposite sides of the box, every bonded term and bonded-compound pair must
+    see the same geometry: LIMA moves a neighbour compound to its nearest image (hyperpos,
+    EngineKernels.cuh:235-236).  Unwrapped origo differences would stretch the bonds that join the
+    two compounds by a whole box length."""
+    pos = _helix_chain()
+    n = len(pos)
+    bonds = [(i, i + 1, 0.15, 2e5) for i in range(n - 1)]
+    angles = [(i, i + 1, i + 2, 111.0, 400.0, 0.0, 0.0) for i in range(n - 2)]
+    dihedrals = [(i, i + 1, i + 2, i + 3, 0.0, 2.0, 3) for i in range(n - 3)]
+    q = np.where(np.arange(n) % 2 == 0, 0.3, -0.3)
+
+    def forces(shift):
+        b = CustomBox([3.0] * 3, pos + shift, q, np.full(n, 0.33), np.full(n, 0.5), np.full(n, 12.0),
+                      bonds=bonds, angles=angles, dihedrals=dihedrals)
+        o = Oracle(b, params())
+        f, _, e = o.forces(components)
+        return f[b.gid], e, b
+
+    f0, e0, b0 = forces(np.zeros(3))
+    f1, e1, b1 = forces(np.array([2.3, 0.0, 1.6]))   # compound 0's key near x = 2.8, compound 1 wraps past 3.0
+    assert len(set(map(tuple, b1.origo.tolist()))) == 2
+    assert np.abs(b1.origo[0] - b1.origo[1]).max() >= 2   # the two origos sit on opposite sides of the box
+    scale = np.abs(f0).max()
+    assert scale > 0
+    assert np.abs(f1 - f0).max() < 1e-6 * scale
+    tot0 = e0.bonded + e0.lj + e0.coulomb_sr
+    tot1 = e1.bonded + e1.lj + e1.coulomb_sr
+    assert tot1 == pytest.approx(tot0, rel=1e-9, abs=1e-9)
+
+
 @pytest.mark.parametrize("term", ["bond", "angle", "urey_bradley", "dihedral", "dihedral_phase", "pair14"])
 def test_bonded_forces_are_energy_gradients(term):
     rng = np.random.default_rng(3)
