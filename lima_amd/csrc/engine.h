@@ -46,6 +46,8 @@ struct NList {
     float2* ljs = nullptr;     // [slots] (sigma/2, sqrt(24 eps))
     int* comps = nullptr;      // [slots] compound of slot (-1 padding)
     uint4* cw = nullptr;       // [slots] wrapped Coord of the slot's particle (w = 1 for padding)
+    long long* xb[2] = {nullptr, nullptr};   // [n+1] first Coord of each x / y column: ceil(c Lq / n)
+    double rinv[2] = {0.0, 0.0};             // n / Lq per axis
     float4* xq = nullptr;      // [slots] position relative to cluster ref [nm] + charge
     int* ecount = nullptr;     // [ncl_max+1]
     int* eoff = nullptr;       // [ncl_max+1]

@@ -28,10 +28,19 @@ __global__ void k_fill_int(int* p, int n, int v) {
     if (i < n) p[i] = v;
 }
 
-__device__ __forceinline__ int column_of(const uint4 p, long long Lqx, long long Lqy, int ncx, int ncy) {
-    const int cx = (int)(((unsigned long long)p.x * (unsigned long long)ncx) / (unsigned long long)Lqx);
-    const int cy = (int)(((unsigned long long)p.y * (unsigned long long)ncy) / (unsigned long long)Lqy);
-    return cy * ncx + cx;
+// Column of coordinate v in [0, Lq): floor(v n / Lq), from a double estimate corrected against
+// the exact column starts xb[c] = ceil(c Lq / n) (no 64-bit integer division).
+__device__ __forceinline__ int col_of(long long v, const long long* __restrict__ xb, int n, double rinv) {
+    int c = (int)((double)v * rinv);
+    c = min(max(c, 0), n - 1);
+    if (xb[c] > v) c--;
+    else if (c + 1 < n && xb[c + 1] <= v) c++;
+    return c;
+}
+
+__device__ __forceinline__ int column_of(const uint4 p, const long long* xbx, const long long* xby, int ncx, int ncy, double rx,
+                                         double ry) {
+    return col_of((long long)p.y, xby, ncy, ry) * ncx + col_of((long long)p.x, xbx, ncx, rx);
 }
 
 constexpr int COL_ITEMS = 4;      // particles per thread in the column count / scatter
@@ -40,8 +49,9 @@ constexpr int COL_LDS_MAX = 8192; // columns counted in LDS; more columns use gl
 // Column of every particle and the column populations.  Counts are gathered per block in LDS and
 // added to the global counters once per (block, column): same-address global atomics from every
 // particle serialise in a few L2 channels.
-__global__ __launch_bounds__(256) void k_col_assign(int64_t N, const uint4* __restrict__ w, long long Lqx, long long Lqy, int ncx,
-                                                    int ncy, int* __restrict__ col_of, int* __restrict__ col_count) {
+__global__ __launch_bounds__(256) void k_col_assign(int64_t N, const uint4* __restrict__ w, const long long* __restrict__ xbx,
+                                                    const long long* __restrict__ xby, double rx, double ry, int ncx, int ncy,
+                                                    int* __restrict__ col_of, int* __restrict__ col_count) {
     extern __shared__ int hist[];
     const int ncol = ncx * ncy;
     const bool lds = ncol <= COL_LDS_MAX;
@@ -52,7 +62,7 @@ __global__ __launch_bounds__(256) void k_col_assign(int64_t N, const uint4* __re
     for (int k = 0; k < COL_ITEMS; k++) {
         const int64_t a = a0 + (int64_t)k * blockDim.x + threadIdx.x;
         if (a < N) {
-            const int c = column_of(w[a], Lqx, Lqy, ncx, ncy);
+            const int c = column_of(w[a], xbx, xby, ncx, ncy, rx, ry);
             col_of[a] = c;
             atomicAdd(lds ? &hist[c] : &col_count[c], 1);
         }
@@ -200,6 +210,8 @@ struct NlArgs {
     const int* comps;
     const uint8_t* bmask;
     const uint4* cw;
+    const long long* xb[2];   // column starts along x, y
+    double rinv[2];
     const uint4* cref;
     const uint4* cmin;
     const uint4* cmax;
@@ -210,12 +222,13 @@ struct NlArgs {
 };
 
 // range of column indices whose [lo, hi] interval (after shift) is non-empty
-__device__ __forceinline__ bool col_range(long long lo, long long hi, long long Lq, int n, int* c0, int* c1) {
+__device__ __forceinline__ bool col_range(long long lo, long long hi, long long Lq, int n, const long long* xb, double rinv, int* c0,
+                                          int* c1) {
     if (hi < 0 || lo > Lq - 1) return false;
     lo = lo < 0 ? 0 : lo;
     hi = hi > Lq - 1 ? Lq - 1 : hi;
-    *c0 = (int)(((unsigned long long)lo * (unsigned long long)n) / (unsigned long long)Lq);
-    *c1 = (int)(((unsigned long long)hi * (unsigned long long)n) / (unsigned long long)Lq);
+    *c0 = col_of(lo, xb, n, rinv);
+    *c1 = col_of(hi, xb, n, rinv);
     return true;
 }
 
@@ -248,12 +261,13 @@ __device__ __forceinline__ long long gap1(long long lo, long long hi, long long 
 struct AxisRanges {
     int c0[3], c1[3], n;
 };
-__device__ __forceinline__ AxisRanges axis_ranges(long long mn, long long mx, long long R, long long Lq, int ncol) {
+__device__ __forceinline__ AxisRanges axis_ranges(long long mn, long long mx, long long R, long long Lq, int ncol, const long long* xb,
+                                                  double rinv) {
     AxisRanges r;
     r.n = 0;
     for (int s = 0; s < 3; s++) {
         int a0 = 0, a1 = -1;
-        if (!col_range(mn - R - (s - 1) * Lq, mx + R - (s - 1) * Lq, Lq, ncol, &a0, &a1)) { a0 = 0; a1 = -1; }
+        if (!col_range(mn - R - (s - 1) * Lq, mx + R - (s - 1) * Lq, Lq, ncol, xb, rinv, &a0, &a1)) { a0 = 0; a1 = -1; }
         r.c0[s] = a0; r.c1[s] = a1;
         r.n += a1 - a0 + 1;
     }
@@ -296,6 +310,7 @@ __device__ __forceinline__ bool jbox_near(const NlArgs& a, int J, int code, cons
 // fixed-capacity slice [I cap, (I+1) cap); the count (padded to a multiple of 8 with mask-0
 // entries) goes to ecount[I].  A slice that would overflow sets flags[1] and the host rebuilds
 // with a larger capacity.
+template <bool NARROW>
 __global__ __launch_bounds__(256) void k_nlist(NlArgs a) {
     __shared__ uint32_t flat_all[4][NL_FLAT], cand_all[4][NL_FLAT];
     const int lane = threadIdx.x & 63;
@@ -320,8 +335,19 @@ __global__ __launch_bounds__(256) void k_nlist(NlArgs a) {
             ci_hi = max(ci_hi, ci[s]);
         }
     }
-    const AxisRanges ry = axis_ranges(mn[1], mx[1], a.R, a.Lq[1], a.ncy);
-    const AxisRanges rx = axis_ranges(mn[0], mx[0], a.R, a.Lq[0], a.ncx);
+    // the i-cluster's distinct compounds: id, the slots holding it, its bonded-compound bits
+    int dc_id[CLUSTER], ndc = 0;
+    uint32_t dc_slots[CLUSTER], dc_bm[CLUSTER];
+#pragma unroll
+    for (int s = 0; s < CLUSTER; s++) {
+        if (ci[s] < 0) continue;
+        bool seen = false;
+        for (int k = 0; k < ndc; k++)
+            if (dc_id[k] == ci[s]) { dc_slots[k] |= 1u << s; seen = true; }
+        if (!seen) { dc_id[ndc] = ci[s]; dc_slots[ndc] = 1u << s; dc_bm[ndc] = bmi[s]; ndc++; }
+    }
+    const AxisRanges ry = axis_ranges(mn[1], mx[1], a.R, a.Lq[1], a.ncy, a.xb[1], a.rinv[1]);
+    const AxisRanges rx = axis_ranges(mn[0], mx[0], a.R, a.Lq[0], a.ncx, a.xb[0], a.rinv[0]);
     const int nunits = ry.n * rx.n * 3;
     const int jsub = lane >> 3, sl = lane & 7;
     int count = 0;
@@ -337,12 +363,9 @@ __global__ __launch_bounds__(256) void k_nlist(NlArgs a) {
                 axis_pick(rx, t % rx.n, &cx, &sxi);
                 axis_pick(ry, t / rx.n, &cy, &syi);
                 const long long shx = shift_of(sxi, a.Lq[0]), shy = shift_of(syi, a.Lq[1]), shz = shift_of(szi, a.Lq[2]);
-                // the column's slab holds x in [ceil(cx Lq / n), ceil((cx+1) Lq / n) - 1], inside
-                // [floor(cx Lq / n), floor((cx+1) Lq / n)]
-                const long long xlo = (long long)(((unsigned long long)cx * (unsigned long long)a.Lq[0]) / (unsigned long long)a.ncx);
-                const long long xhi = (long long)(((unsigned long long)(cx + 1) * (unsigned long long)a.Lq[0]) / (unsigned long long)a.ncx);
-                const long long ylo = (long long)(((unsigned long long)cy * (unsigned long long)a.Lq[1]) / (unsigned long long)a.ncy);
-                const long long yhi = (long long)(((unsigned long long)(cy + 1) * (unsigned long long)a.Lq[1]) / (unsigned long long)a.ncy);
+                // the column's slab holds x in [ceil(cx Lq / n), ceil((cx+1) Lq / n) - 1]
+                const long long xlo = a.xb[0][cx], xhi = a.xb[0][cx + 1] - 1;
+                const long long ylo = a.xb[1][cy], yhi = a.xb[1][cy + 1] - 1;
                 const long long gx = min(gap1(xlo + shx, xhi + shx, mn[0], mx[0]), a.R + 1);
                 const long long gy = min(gap1(ylo + shy, yhi + shy, mn[1], mx[1]), a.R + 1);
                 const long long zlo = mn[2] - a.R - shz, zhi = mx[2] + a.R - shz;
@@ -407,23 +430,36 @@ __global__ __launch_bounds__(256) void k_nlist(NlArgs a) {
                 fetch(f0 + CLUSTER + jsub, nJ, ncd, npb, ncb, nrj);
                 uint32_t mask = 0;
                 if (cd >= 0 && pb.w == 0u) {
-                    const long long sx = shift_of(cd % 3, a.Lq[0]), sy = shift_of((cd / 3) % 3, a.Lq[1]);
-                    const long long sz = shift_of(cd / 9, a.Lq[2]);
-                    const long long gx = min(gap1((long long)pb.x + sx, (long long)pb.x + sx, mn[0], mx[0]), a.R + 1);
-                    const long long gy = min(gap1((long long)pb.y + sy, (long long)pb.y + sy, mn[1], mx[1]), a.R + 1);
-                    const long long gz = min(gap1((long long)pb.z + sz, (long long)pb.z + sz, mn[2], mx[2]), a.R + 1);
-                    if (gx * gx + gy * gy + gz * gz <= a.R2) {
+                    // shift code cd = sx + 3 sy + 9 sz, each 0..2 (division-free for cd < 27)
+                    const int czi = (cd * 57) >> 9, rem = cd - 9 * czi, cyi = (rem * 11) >> 5, cxi = rem - 3 * cyi;
+                    bool in;
+                    if (NARROW) {
+                        // Lq + R + cluster extents < 2^31: every difference below is an exact int32
+                        const int px = (int)(pb.x + (uint32_t)shift_of(cxi, a.Lq[0])), py = (int)(pb.y + (uint32_t)shift_of(cyi, a.Lq[1]));
+                        const int pz = (int)(pb.z + (uint32_t)shift_of(czi, a.Lq[2]));
+                        const uint32_t R1 = (uint32_t)a.R + 1u;
+                        const uint32_t gx = min((uint32_t)max(0, max((int)bmn.x - px, px - (int)bmx.x)), R1);
+                        const uint32_t gy = min((uint32_t)max(0, max((int)bmn.y - py, py - (int)bmx.y)), R1);
+                        const uint32_t gz = min((uint32_t)max(0, max((int)bmn.z - pz, pz - (int)bmx.z)), R1);
+                        in = (unsigned long long)gx * gx + (unsigned long long)gy * gy + (unsigned long long)gz * gz <= (unsigned long long)a.R2;
+                    } else {
+                        const long long sx = shift_of(cxi, a.Lq[0]), sy = shift_of(cyi, a.Lq[1]), sz = shift_of(czi, a.Lq[2]);
+                        const long long gx = min(gap1((long long)pb.x + sx, (long long)pb.x + sx, mn[0], mx[0]), a.R + 1);
+                        const long long gy = min(gap1((long long)pb.y + sy, (long long)pb.y + sy, mn[1], mx[1]), a.R + 1);
+                        const long long gz = min(gap1((long long)pb.z + sz, (long long)pb.z + sz, mn[2], mx[2]), a.R + 1);
+                        in = gx * gx + gy * gy + gz * gz <= a.R2;
+                    }
+                    if (in) {
                         if (cb < ci_lo - 2 || cb > ci_hi + 2) {
                             mask = valid;   // no slot's compound is cb or bonded to it
                         } else {
                             const bool cbin = cb >= 0 && cb < a.nC;
-#pragma unroll
-                            for (int s = 0; s < CLUSTER; s++) {
-                                if (ci[s] < 0) continue;
-                                const int dd = cb - ci[s];
-                                const bool intra = ci[s] == cb || (cbin && bmi[s] != 0u && dd >= -2 && dd <= 2 && ((bmi[s] >> (dd + 2)) & 1u));
-                                if (!intra) mask |= 1u << s;
+                            uint32_t excl = 0;
+                            for (int k = 0; k < ndc; k++) {
+                                const int dd = cb - dc_id[k];
+                                if (dd == 0 || (cbin && dd >= -2 && dd <= 2 && ((dc_bm[k] >> (dd + 2)) & 1u))) excl |= dc_slots[k];
                             }
+                            mask = valid & ~excl;
                         }
                         if (J == I && cd == 13) mask &= ~(1u << sl);   // no self pair
                     }
@@ -433,11 +469,18 @@ __global__ __launch_bounds__(256) void k_nlist(NlArgs a) {
                 if (emit) {
                     const int pos = count + __popcll(bal & ((1ull << lane) - 1ull));
                     if (pos < a.cap) {
-                        // shift between the cluster reference points: the int64 difference is exact in
-                        // a double, so the float is the same single rounding as a direct conversion
-                        const float sxf = (float)(double)((long long)rj.x - (long long)refI.x + shift_of(cd % 3, a.Lq[0])) * LIMA_LIMA_TO_NANO_F;
-                        const float syf = (float)(double)((long long)rj.y - (long long)refI.y + shift_of((cd / 3) % 3, a.Lq[1])) * LIMA_LIMA_TO_NANO_F;
-                        const float szf = (float)(double)((long long)rj.z - (long long)refI.z + shift_of(cd / 9, a.Lq[2])) * LIMA_LIMA_TO_NANO_F;
+                        // shift between the cluster reference points (exact integer, then one rounding)
+                        const int czi = (cd * 57) >> 9, rem = cd - 9 * czi, cyi = (rem * 11) >> 5, cxi = rem - 3 * cyi;
+                        float sxf, syf, szf;
+                        if (NARROW) {
+                            sxf = (float)(int)(rj.x - refI.x + (uint32_t)shift_of(cxi, a.Lq[0])) * LIMA_LIMA_TO_NANO_F;
+                            syf = (float)(int)(rj.y - refI.y + (uint32_t)shift_of(cyi, a.Lq[1])) * LIMA_LIMA_TO_NANO_F;
+                            szf = (float)(int)(rj.z - refI.z + (uint32_t)shift_of(czi, a.Lq[2])) * LIMA_LIMA_TO_NANO_F;
+                        } else {
+                            sxf = (float)(double)((long long)rj.x - (long long)refI.x + shift_of(cxi, a.Lq[0])) * LIMA_LIMA_TO_NANO_F;
+                            syf = (float)(double)((long long)rj.y - (long long)refI.y + shift_of(cyi, a.Lq[1])) * LIMA_LIMA_TO_NANO_F;
+                            szf = (float)(double)((long long)rj.z - (long long)refI.z + shift_of(czi, a.Lq[2])) * LIMA_LIMA_TO_NANO_F;
+                        }
                         a.ent[base + pos] = make_float4(sxf, syf, szf, __uint_as_float((uint32_t)(J * CLUSTER + sl) | (mask << 24)));
                         a.ecode[base + pos] = (uint8_t)cd;
                     }
@@ -512,6 +555,15 @@ void nlist_alloc(Engine& e) {
     nl.ncx = std::max(1, (int)std::floor(e.L[0] / a));
     nl.ncy = std::max(1, (int)std::floor(e.L[1] / a));
     nl.ncol = nl.ncx * nl.ncy;
+    for (int d = 0; d < 2; d++) {   // column starts ceil(c Lq / n), exact integer arithmetic
+        const int n = d == 0 ? nl.ncx : nl.ncy;
+        const unsigned long long Lq = (unsigned long long)e.Lq[d];
+        std::vector<long long> xb(n + 1);
+        for (int c = 0; c <= n; c++) xb[c] = (long long)(((unsigned long long)c * Lq + (unsigned long long)n - 1) / (unsigned long long)n);
+        HIP_CHECK(hipMalloc(&nl.xb[d], (n + 1) * sizeof(long long)));
+        HIP_CHECK(hipMemcpy(nl.xb[d], xb.data(), (n + 1) * sizeof(long long), hipMemcpyHostToDevice));
+        nl.rinv[d] = (double)n / (double)e.Lq[d];
+    }
     nl.ncl_max = (int)(e.N / CLUSTER) + nl.ncol + 1;
     if (nl.ncl_max >= (1 << 24)) throw LimaFail(LIMA_ERR_UNSUPPORTED, "more than 2^24 particle clusters");
     const int64_t slots = (int64_t)nl.ncl_max * CLUSTER;
@@ -545,7 +597,7 @@ void nlist_alloc(Engine& e) {
 void nlist_free(Engine& e) {
     NList& nl = e.nl;
     void* ptrs[] = {nl.col_of, nl.col_count, nl.col_start, nl.col_fill, nl.keys, nl.col_ncl, nl.catoms, nl.slot_of,
-                    nl.ccol, nl.cref, nl.cmin, nl.cmax, nl.ljs, nl.comps, nl.cw, nl.xq, nl.ecount, nl.eoff, nl.flags,
+                    nl.ccol, nl.cref, nl.cmin, nl.cmax, nl.ljs, nl.comps, nl.cw, nl.xb[0], nl.xb[1], nl.xq, nl.ecount, nl.eoff, nl.flags,
                     nl.ent, nl.ecode, e.fs, e.es};
     for (void* p : ptrs) if (p) (void)hipFree(p);
 }
@@ -590,8 +642,8 @@ void nlist_build(Engine& e, const uint4* w) {
     hipLaunchKernelGGL(k_fill_int, dim3(nblk(nl.ncol + 1, 256)), dim3(256), 0, s, nl.col_fill, nl.ncol + 1, 0);
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(4), 0, s, nl.flags, 4, 0);
     const size_t col_lds = nl.ncol <= COL_LDS_MAX ? (size_t)nl.ncol * sizeof(int) : 0;
-    hipLaunchKernelGGL(k_col_assign, dim3(nblk(N, 256 * COL_ITEMS)), dim3(256), col_lds, s, N, w, (long long)e.Lq[0],
-                       (long long)e.Lq[1], nl.ncx, nl.ncy, nl.col_of, nl.col_count);
+    hipLaunchKernelGGL(k_col_assign, dim3(nblk(N, 256 * COL_ITEMS)), dim3(256), col_lds, s, N, w, nl.xb[0], nl.xb[1], nl.rinv[0],
+                       nl.rinv[1], nl.ncx, nl.ncy, nl.col_of, nl.col_count);
     launch_exclusive_scan(nl.col_count, nl.col_start, nl.ncol, s);
     hipLaunchKernelGGL(k_col_scatter, dim3(nblk(N, 256 * COL_ITEMS)), dim3(256), col_lds, s, N, nl.ncol, w, nl.col_of, nl.col_start,
                        nl.col_fill, nl.keys);
@@ -617,6 +669,7 @@ void nlist_build(Engine& e, const uint4* w) {
     a.R = llround(((double)e.rc + (double)e.skin) * 1e8);
     a.R2 = a.R * a.R;
     a.col_cl_start = col_cl_start; a.catoms = nl.catoms; a.comps = nl.comps; a.bmask = e.bmask; a.cw = nl.cw;
+    a.xb[0] = nl.xb[0]; a.xb[1] = nl.xb[1]; a.rinv[0] = nl.rinv[0]; a.rinv[1] = nl.rinv[1];
     a.cref = nl.cref; a.cmin = nl.cmin; a.cmax = nl.cmax; a.ecount = nl.ecount; a.flags = nl.flags;
     for (int attempt = 0;; attempt++) {
         const int64_t need = (int64_t)nl.ncl_max * nl.cap;
@@ -629,7 +682,11 @@ void nlist_build(Engine& e, const uint4* w) {
             nl.nent_cap = need;
         }
         a.cap = nl.cap; a.ent = nl.ent; a.ecode = nl.ecode;
-        hipLaunchKernelGGL(k_nlist, dim3(nblk(ncl, 4)), dim3(256), 0, s, a);
+        // 32-bit coordinate differences are exact when the box, the list radius and a column's
+        // width stay below 2^31 Coord units together (boxes up to ~10 nm)
+        const long long Lmax = std::max(a.Lq[0], std::max(a.Lq[1], a.Lq[2]));
+        if (2 * Lmax + 2 * a.R < (1ll << 31)) hipLaunchKernelGGL(k_nlist<true>, dim3(nblk(ncl, 4)), dim3(256), 0, s, a);
+        else hipLaunchKernelGGL(k_nlist<false>, dim3(nblk(ncl, 4)), dim3(256), 0, s, a);
         HIP_CHECK(hipMemcpyAsync(hflags, nl.flags, sizeof(hflags), hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipStreamSynchronize(s));
         if (!hflags[1]) break;
