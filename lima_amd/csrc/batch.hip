@@ -152,7 +152,7 @@ __device__ __forceinline__ v3 bdisp(const int4* u, int a, int b, const SysMeta& 
 __device__ __forceinline__ void pair_eval(float x, float y, float z, float r2, const float4& pi, float qj, float sj, float ej,
                                           const PairConst& pc, v3& fi, v3& fj, float& elj, float& ec) {
     float e1, e2;
-    const float S = pair_scalar_cut(r2, pi.y + sj, pi.w * ej, pi.x * qj, pc, &e1, &e2);
+    const float S = pair_scalar_cut(r2, pi.y + sj, pi.w * ej, pi.x * qj * pc.kc, pc, &e1, &e2);
     // explicit FMAs: the same rounding whatever the unrolling or scheduling
     fi.x = fmaf(x, S, fi.x); fi.y = fmaf(y, S, fi.y); fi.z = fmaf(z, S, fi.z);
     fj.x = fmaf(-x, S, fj.x); fj.y = fmaf(-y, S, fj.y); fj.z = fmaf(-z, S, fj.z);

@@ -97,6 +97,7 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs a) {
     const int islot = I * CLUSTER + ii;
     const float4 xi = a.xq[islot];
     const float2 pi = a.ljs[islot];
+    const float qi_kc = xi.w * a.pc.kc;
     float fx = 0.f, fy = 0.f, fz = 0.f;
     double elj = 0.0, ecl = 0.0;   // FP64 energy accumulation (energy steps only)
     int npairs = 0;
@@ -131,19 +132,19 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs a) {
     for (int c = 0; c < nchunk; c++) {
         const int buf = c & 1;
         if (c + 1 < nchunk) fetch(c + 1);   // in flight during the compute below
-        const int nit = min(CHUNK, nent - c * CHUNK) / CLUSTER;
+        const int nit = __builtin_amdgcn_readfirstlane(min(CHUNK, nent - c * CHUNK) / CLUSTER);   // wave-uniform
         for (int k = 0; k < nit; k++) {
             const float4 A = sA[wid][buf][k * CLUSTER + jj];
             const float4 B = sB[wid][buf][k * CLUSTER + jj];
             const uint32_t m = __float_as_uint(B.z);
             const float dx = A.x - xi.x, dy = A.y - xi.y, dz = A.z - xi.z;
             const float r2 = dx * dx + dy * dy + dz * dz;
-            const bool inc = ((m >> ii) & 1u) && (r2 <= a.rc2) && (r2 > 0.f);
+            const bool inc = ((m >> ii) & 1u) && (r2 <= a.rc2);   // the list masks exclude self and padding pairs
             const unsigned long long any = __ballot(inc);
             if (any == 0ull) continue;
             if (ENERGY) npairs += __popcll(any);
             float el = 0.f, ec = 0.f;
-            float S = pair_scalar_cut(inc ? r2 : 1.0f, pi.x + B.x, pi.y * B.y, xi.w * A.w, a.pc, ENERGY ? &el : nullptr, &ec);
+            float S = pair_scalar_cut(inc ? r2 : 1.0f, pi.x + B.x, pi.y * B.y, qi_kc * A.w, a.pc, ENERGY ? &el : nullptr, &ec);
             S = inc ? S : 0.f;
             fx += dx * S; fy += dy * S; fz += dz * S;
             if (ENERGY && inc) {

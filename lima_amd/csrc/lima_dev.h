@@ -80,16 +80,16 @@ __device__ __forceinline__ float pair_scalar(float r2, float sig, float eps, flo
 // The a_k are a relative-error least-squares fit of R on [0, 3.25] (max relative error 1e-7 in
 // exact arithmetic, 4e-7 absolute on erfc when evaluated in FP32).  exp(-x^2) serves both the
 // erfc and the Gaussian term of the force.  eps24 = 24 eps.
-__device__ __forceinline__ float pair_scalar_cut(float r2, float sig, float eps24, float qq, const PairConst& c,
+__device__ __forceinline__ float pair_scalar_cut(float r2, float sig, float eps24, float qq_kc, const PairConst& c,
                                                  float* e_lj, float* e_c) {
+    // qq_kc = kC q_i q_j; exp(-x^2) = exp2(r^2 (-kappa^2 log2 e)), the constant hoisted by the caller's loop
     const float rinv = __builtin_amdgcn_rsqf(r2);
     const float rinv2 = rinv * rinv;
     float s = sig * sig * rinv2;
     s = s * s * s;
     const float flj = eps24 * s * rinv2 * (1.0f - 2.0f * s);
-    const float x = c.kappa * (r2 * rinv);
-    const float ex = __builtin_amdgcn_exp2f(-(x * x) * 1.4426950408889634f);
-    const float t = __builtin_amdgcn_rcpf(1.0f + 0.35f * x);
+    const float ex = __builtin_amdgcn_exp2f(r2 * (-(c.kappa * c.kappa) * 1.4426950408889634f));
+    const float t = __builtin_amdgcn_rcpf(1.0f + (0.35f * c.kappa) * (r2 * rinv));   // 1 / (1 + 0.35 kappa r)
     float R = -2.035868019e-01f;
     R = R * t + 7.992476821e-01f;
     R = R * t - 7.338286042e-01f;
@@ -97,11 +97,11 @@ __device__ __forceinline__ float pair_scalar_cut(float r2, float sig, float eps2
     R = R * t - 4.074399173e-02f;
     R = R * t + 2.456984967e-01f;
     R = R * t + 1.931048036e-01f;
-    const float erfcv = ex * (R * t);
-    const float fc = c.kc * qq * rinv2 * (erfcv * rinv + c.two_kappa_sqrtpi * ex);
+    const float Rt = R * t;   // erfc(x) = ex Rt
+    const float fc = (qq_kc * rinv2) * (ex * fmaf(rinv, Rt, c.two_kappa_sqrtpi));
     if (e_lj) {
         *e_lj = (eps24 * (1.0f / 6.0f)) * s * (s - 1.0f);
-        *e_c = c.kc * qq * erfcv * rinv;
+        *e_c = qq_kc * (ex * Rt) * rinv;
     }
     return flj - fc;
 }
