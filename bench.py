@@ -148,6 +148,9 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run the copy kernel used to calibrate the HBM byte counters")
     ap.add_argument("--no-em", action="store_true", help="skip the batched-minimisation measurement")
+    ap.add_argument("--launch-mode", type=int, default=0,
+                    help="lima_simparams.launch_mode: 0 step graph with PME on a second stream; +1 eager launches, "
+                         "+2 PME in line on one stream")
     ap.add_argument("--em-systems", type=int, default=256)
     ap.add_argument("--em-train-systems", type=int, default=100)
     ap.add_argument("--em-max-iter", type=int, default=20000)
@@ -168,7 +171,8 @@ def main():
     box = L.gen_chain_water()
     log(f"generated chain+water box: {box.n_particles} atoms, {box.n_compounds} compounds in {time.time() - t:.2f}s")
     dt_fs = 2.0
-    p = L.default_params(dt_fs=dt_fs, n_steps=args.steps, nlist_interval=10, energy_interval=0)
+    p = L.default_params(dt_fs=dt_fs, n_steps=args.steps, nlist_interval=10, energy_interval=0,
+                         launch_mode=args.launch_mode)
     eng = L.Engine(box, p)
     log("engine created")
     t = time.time()
@@ -178,13 +182,16 @@ def main():
     eng.step(args.warmup)
     log(f"warm-up {args.warmup} steps in {time.time() - t:.2f}s")
 
+    host0 = eng.timings()
     t0 = time.perf_counter()
     eng.step(args.steps)
     t1 = time.perf_counter()
+    host1 = eng.timings()
+    host_issue_us = (host1.host_issue_ms - host0.host_issue_ms) * 1e3 / max(host1.host_steps - host0.host_steps, 1)
     elapsed = t1 - t0
     ms_per_step = elapsed * 1e3 / args.steps
     ns_day = args.steps / elapsed * dt_fs * 86400.0 / 1e6
-    log(f"timed {args.steps} steps: {ms_per_step:.4f} ms/step, {ns_day:.1f} ns/day")
+    log(f"timed {args.steps} steps: {ms_per_step:.4f} ms/step, {ns_day:.1f} ns/day; host issue {host_issue_us:.1f} us/step")
     e_end = eng.energy()
     pairs = eng.pair_count()
     stt = eng.status()
@@ -263,11 +270,14 @@ def main():
         "data": "synthetic",
         "config": {"workload": workload, "n_atoms": box.n_particles, "box_nm": [float(x) for x in box.box_nm],
                    "chains": "128 x 50", "waters": (box.n_particles - 6400) // 3, "dt_fs": dt_fs, "cutoff_nm": 1.2,
-                   "pme_spacing_nm": 0.12, "nlist_interval": 10, "skin_nm": 0.1, "energy_reductions": "fp64"},
+                   "pme_spacing_nm": 0.12, "nlist_interval": 10, "skin_nm": 0.1, "energy_reductions": "fp64",
+                   "launch": ("eager" if args.launch_mode & 1 else "step graph") +
+                             (", one stream" if args.launch_mode & 2 else ", PME on a second stream")},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "accuracy": accuracy,
         "phases_ms_per_step": {k: round(v, 5) for k, v in phases.items()},
+        "host_issue_us_per_step": round(host_issue_us, 2),
         "pairs_in_cutoff": pairs,
         "pair_slots_evaluated": list_slots,
         "cluster_pair_efficiency": round(2 * pairs / list_slots, 4) if list_slots else None,

@@ -114,7 +114,14 @@ typedef struct lima_simparams {
     float   nlist_skin_nm;            /* Verlet buffer, default 0.1 */
     int32_t nlist_interval;           /* steps between list rebuilds, default 10 */
     int32_t energy_interval;          /* steps between FP64 energy reductions (0 = never) */
+    int32_t launch_mode;              /* 0 = each MD step replayed as a HIP graph (re-captured after every
+                                         list rebuild) with PME on a second stream; or-ed flags
+                                         LIMA_LAUNCH_EAGER (launch kernels one by one) and
+                                         LIMA_LAUNCH_ONE_STREAM (PME in line on the main stream) */
 } lima_simparams;
+
+#define LIMA_LAUNCH_EAGER 1
+#define LIMA_LAUNCH_ONE_STREAM 2
 
 /* FP64 energy totals [J/mol]. */
 typedef struct lima_energy {
@@ -211,6 +218,10 @@ const char* lima_last_error(void);
 typedef struct lima_timings {
     double pair_ms, intra_ms, bonded_ms, pme_ms, integrate_ms, nlist_ms, total_ms;
     int64_t steps_timed;
+    /* host side of lima_step, accumulated over every call: time spent issuing steps (list
+     * rebuilds included, the final wait for the device excluded), and the steps issued */
+    double host_issue_ms;
+    int64_t host_steps;
 } lima_timings;
 lima_status lima_get_timings(lima_engine* engine, lima_timings* out);
 /* Enable per-phase HIP-event timing (no host synchronisation per step; the events are

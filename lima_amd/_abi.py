@@ -55,6 +55,7 @@ class SimParams(C.Structure):
         ("nlist_skin_nm", C.c_float),
         ("nlist_interval", C.c_int32),
         ("energy_interval", C.c_int32),
+        ("launch_mode", C.c_int32),
     ]
 
 
@@ -112,6 +113,7 @@ class Timings(C.Structure):
         ("pair_ms", C.c_double), ("intra_ms", C.c_double), ("bonded_ms", C.c_double),
         ("pme_ms", C.c_double), ("integrate_ms", C.c_double), ("nlist_ms", C.c_double),
         ("total_ms", C.c_double), ("steps_timed", C.c_int64),
+        ("host_issue_ms", C.c_double), ("host_steps", C.c_int64),
     ]
 
 

@@ -92,6 +92,8 @@ struct Bonded {
 };
 
 // Device accumulators (FP64 energy reductions and scalars).
+constexpr int LAUNCH_EAGER = LIMA_LAUNCH_EAGER, LAUNCH_ONE_STREAM = LIMA_LAUNCH_ONE_STREAM;
+
 enum Acc { ACC_LJ = 0, ACC_COUL, ACC_BOND, ACC_PME, ACC_KIN, ACC_PAIRS, ACC_N };
 
 // FP64 energy reduction without same-address atomics: every wave of an energy-producing kernel
@@ -157,9 +159,16 @@ struct Engine {
     int64_t epart_cap = 0;
     ERegions ereg;
     float* fmax = nullptr;     // [1]
+    float* thermo = nullptr;   // [2] velocity scale factor applied by the integrator, last measured temperature [K]
+    bool temp_measured = false;
 
     hipStream_t s_main = nullptr, s_pme = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // one MD step (evaluation without energies + integration) as an instantiated HIP graph; its
+    // kernel arguments depend on the list, so every list rebuild invalidates it
+    hipGraphExec_t step_graph = nullptr;
+    bool step_graph_valid = false;
+    int64_t graph_captures = 0;
 
     int64_t step = 0;
     int64_t steps_since_build = 0;
@@ -176,6 +185,8 @@ struct Engine {
     int tpos = 0;
     lima_timings tacc{};
     int64_t pairs_last = 0;      // pairs inside the cutoff at the last energy evaluation
+    double host_issue_ms = 0;    // host time spent issuing MD steps (excluding the final device wait)
+    int64_t host_steps = 0;
 };
 
 #define HIP_CHECK(x)                                                                          \

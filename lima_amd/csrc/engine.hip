@@ -38,7 +38,7 @@ struct IntegrateArgs {
     int nslots;
     long long Lq[3];
     float dt;
-    float thermostat;
+    const float* thermostat;   // [0] velocity scale factor (Thermostat.cuh:56-67), set on the device
     const uint4* w_build;
     const int4* img_build;
     float* dpart;          // per-wave running max of the squared displacement since the list build
@@ -100,7 +100,8 @@ __global__ void k_integrate(IntegrateArgs a) {
             d2 = dx * dx + dy * dy + dz * dz;
         }
         if (ENERGY) ke = 0.5f * m * (v.x * v.x + v.y * v.y + v.z * v.z);
-        v.x *= a.thermostat; v.y *= a.thermostat; v.z *= a.thermostat;
+        const float ts = a.thermostat[0];
+        v.x *= ts; v.y *= ts; v.z *= ts;
         a.v[p] = v;
         a.fprev[p] = make_float4(fx, fy, fz, 0.f);
     }
@@ -128,6 +129,32 @@ __global__ void k_kinetic(int64_t N, const float4* __restrict__ v, const float4*
     if ((threadIdx.x & 63) == 0 && (p & ~63ll) < N) {
         double* r = epart + (p >> 6) * ACC_N;
         r[ACC_LJ] = 0.0; r[ACC_COUL] = 0.0; r[ACC_BOND] = 0.0; r[ACC_PME] = 0.0; r[ACC_KIN] = t; r[ACC_PAIRS] = 0.0;
+    }
+}
+
+// Thermostat::Temperature (Thermostat.cuh:87-103) with ComputeThermostatScalar (:56-67): the
+// kinetic energy of the stored velocities is summed in FP64 in a fixed order, T = 2 KE / (dof k_B N_A)
+// with dof = 3N - 3 (BoxBuilder.cpp:108), and the velocity scale is clamp(310 K / T, 1 -+ max_scalar),
+// max_scalar = 0.001 / steps_per_temperature_measurement.  The temperature is always recorded; the
+// scale the integrator applies from the next step on is replaced only when the thermostat is on
+// (Engine.cu:167-174).
+__global__ __launch_bounds__(256) void k_thermostat(const double* __restrict__ kin, int nrec, double dof,
+                                                    float max_scalar, int apply, float* thermo) {
+    __shared__ double red[256];
+    double t = 0.0;
+    for (int r = threadIdx.x; r < nrec; r += 256) t += kin[(int64_t)r * ACC_N + ACC_KIN];
+    red[threadIdx.x] = t;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const float T = dof > 0.0 ? (float)(red[0] * (2.0 / dof) / ((double)1.38066e-23f * 6.02214076e23)) : 0.f;
+        const float safe = T == 0.f ? 1.f : T;
+        const float sc = fminf(fmaxf(310.f / safe, 1.f - max_scalar), 1.f + max_scalar);
+        thermo[1] = T;
+        if (apply) thermo[0] = sc;
     }
 }
 
@@ -261,13 +288,13 @@ void evaluate(Engine& e, const uint4* w, const int4* img, bool energy, bool time
     if (ev) HIP_CHECK(hipEventRecord(ev[0], s));
     launch_pack(e, w, s);
     if (ev) HIP_CHECK(hipEventRecord(ev[1], s));
-    const bool split = !timed && e.elec && e.pme.ncharged > 0;
+    const bool split = !timed && !(e.prm.launch_mode & LAUNCH_ONE_STREAM) && e.elec && e.pme.ncharged > 0;
     if (split) {
         HIP_CHECK(hipEventRecord(e.ev_fork, s));
         HIP_CHECK(hipStreamWaitEvent(e.s_pme, e.ev_fork, 0));
-        launch_pme(e, w, energy, e.s_pme);
-        HIP_CHECK(hipEventRecord(e.ev_join, e.s_pme));
     }
+    // the short-range kernels are submitted first: they start while the host is still
+    // submitting the PME chain
     launch_pair(e, energy, s);
     if (ev) HIP_CHECK(hipEventRecord(ev[2], s));
     if (ev) {   // timed: each phase alone
@@ -278,8 +305,13 @@ void evaluate(Engine& e, const uint4* w, const int4* img, bool energy, bool time
     } else {
         launch_local(e, w, img, energy, s);
     }
-    if (split) HIP_CHECK(hipStreamWaitEvent(s, e.ev_join, 0));
-    else launch_pme(e, w, energy, s);
+    if (split) {
+        launch_pme(e, w, energy, e.s_pme);
+        HIP_CHECK(hipEventRecord(e.ev_join, e.s_pme));
+        HIP_CHECK(hipStreamWaitEvent(s, e.ev_join, 0));
+    } else {
+        launch_pme(e, w, energy, s);
+    }
     if (ev) HIP_CHECK(hipEventRecord(ev[5], s));
 }
 
@@ -313,7 +345,7 @@ void integrate(Engine& e, bool energy) {
     a.nslots = e.nl.ncl * CLUSTER;
     for (int d = 0; d < 3; d++) a.Lq[d] = e.Lq[d];
     a.dt = (float)e.dt_ns;
-    a.thermostat = 1.f;
+    a.thermostat = e.thermo;
     a.slot_of = e.nl.slot_of; a.fs = e.fs; a.w = e.p.w; a.img = e.p.img; a.v = e.p.v; a.fprev = e.p.fprev;
     a.prm = e.p.prm; a.epart = e.epart + (int64_t)e.ereg.kin * ACC_N;
     a.w_build = e.w_build; a.img_build = e.img_build; a.dpart = e.dpart;
@@ -321,7 +353,65 @@ void integrate(Engine& e, bool energy) {
     else hipLaunchKernelGGL(k_integrate<false>, dim3(nblk(e.N, 256)), dim3(256), 0, e.s_main, a);
 }
 
+// LIMA measures the temperature in hostMaster when the logging buffer is full and the step is a
+// multiple of steps_per_temperature_measurement (Engine.cu:162-174; IsBufferFull is
+// step % (5 * data_logging_interval) == 0, SimulationDevice.cuh:50-54).
+bool temperature_due(const Engine& e) {
+    const int64_t spt = e.prm.steps_per_temperature_measurement;
+    const int64_t buf = 5 * (int64_t)e.prm.data_logging_interval;
+    return spt > 0 && buf > 0 && e.step > 0 && e.step % buf == 0 && e.step % spt == 0;
+}
+
+void measure_temperature(Engine& e) {
+    double* kin = e.epart + (int64_t)e.ereg.kin * ACC_N;
+    hipLaunchKernelGGL(k_kinetic, dim3(nblk(e.N, 256)), dim3(256), 0, e.s_main, e.N, e.p.v, e.p.prm, kin);
+    const float max_scalar = 0.001f / (float)e.prm.steps_per_temperature_measurement;
+    hipLaunchKernelGGL(k_thermostat, dim3(1), dim3(256), 0, e.s_main, (const double*)kin, (int)((e.N + 63) / 64),
+                       3.0 * (double)e.N - 3.0, max_scalar, e.prm.apply_thermostat ? 1 : 0, e.thermo);
+    e.temp_measured = true;
+}
+
+// Captures one MD step without energies (pack, PME on the second stream, pair, local terms,
+// join, integrator) into a graph.  Replaying it replaces ~16 host launch and event calls per step
+// by one graph launch; the kernels, their arguments and the fork/join are those of the eager path.
+// An existing executable graph is updated in place when the topology is unchanged.
+void capture_step_graph(Engine& e) {
+    hipGraph_t g = nullptr;
+    HIP_CHECK(hipStreamBeginCapture(e.s_main, hipStreamCaptureModeThreadLocal));
+    try {
+        evaluate(e, e.p.w, e.p.img, false);
+        integrate(e, false);
+    } catch (...) {
+        hipGraph_t junk = nullptr;
+        (void)hipStreamEndCapture(e.s_main, &junk);
+        if (junk) (void)hipGraphDestroy(junk);
+        throw;
+    }
+    HIP_CHECK(hipStreamEndCapture(e.s_main, &g));
+    if (e.step_graph) {
+        hipGraphNode_t err_node = nullptr;
+        hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+        if (hipGraphExecUpdate(e.step_graph, g, &err_node, &res) != hipSuccess || res != hipGraphExecUpdateSuccess) {
+            (void)hipGetLastError();
+            (void)hipGraphExecDestroy(e.step_graph);
+            e.step_graph = nullptr;
+        }
+    }
+    if (!e.step_graph) {
+        const hipError_t err = hipGraphInstantiate(&e.step_graph, g, nullptr, nullptr, 0);
+        if (err != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            e.step_graph = nullptr;
+            throw HipError(std::string("hipGraphInstantiate: ") + hipGetErrorString(err));
+        }
+    }
+    HIP_CHECK(hipGraphDestroy(g));
+    e.step_graph_valid = true;
+    e.graph_captures++;
+}
+
 void md_steps(Engine& e, int64_t n) {
+    const auto t_issue = std::chrono::steady_clock::now();
     const int interval = e.prm.nlist_interval > 0 ? e.prm.nlist_interval : 10;
     for (int64_t k = 0; k < n; k++) {
         if (e.steps_since_build >= interval) {
@@ -333,8 +423,13 @@ void md_steps(Engine& e, int64_t n) {
             }
         }
         const bool energy = (e.prm.energy_interval > 0 && e.step % e.prm.energy_interval == 0) || k == n - 1;
-        evaluate(e, e.p.w, e.p.img, energy, e.timing);
-        integrate(e, energy);
+        if (!energy && !e.timing && !(e.prm.launch_mode & LAUNCH_EAGER)) {
+            if (!e.step_graph_valid) capture_step_graph(e);
+            HIP_CHECK(hipGraphLaunch(e.step_graph, e.s_main));
+        } else {
+            evaluate(e, e.p.w, e.p.img, energy, e.timing);
+            integrate(e, energy);
+        }
         if (energy) launch_reduce_energy(e, true, e.s_main);
         if (e.timing) {
             HIP_CHECK(hipEventRecord(tslot(e)[6], e.s_main));
@@ -342,12 +437,15 @@ void md_steps(Engine& e, int64_t n) {
         }
         e.step++;
         e.steps_since_build++;
+        if (temperature_due(e)) measure_temperature(e);
         if (energy) {
             fill_energy(e, true);
             check_finite(e);
         }
     }
     e.forces_valid = true;
+    e.host_issue_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_issue).count();
+    e.host_steps += n;
     HIP_CHECK(hipStreamSynchronize(e.s_main));
     HIP_CHECK(hipGetLastError());
 }
@@ -357,6 +455,10 @@ void create(Engine& e, const lima_box_desc* b, const lima_simparams* p) {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw LimaFail(LIMA_ERR_NO_DEVICE, "no HIP device visible");
     e.prm = *p;
+    if (p->launch_mode < 0 || p->launch_mode > (LAUNCH_EAGER | LAUNCH_ONE_STREAM))
+        throw LimaFail(LIMA_ERR_BAD_INPUT, "launch_mode must be a combination of LIMA_LAUNCH_EAGER and LIMA_LAUNCH_ONE_STREAM");
+    if (p->apply_thermostat && p->steps_per_temperature_measurement <= 0)
+        throw LimaFail(LIMA_ERR_BAD_INPUT, "the thermostat needs steps_per_temperature_measurement > 0");
     if (p->bc_select != LIMA_BC_PBC) throw LimaFail(LIMA_ERR_UNSUPPORTED, "only periodic boundary conditions are implemented");
     if (!(p->cutoff_nm > 0.f)) throw LimaFail(LIMA_ERR_BAD_INPUT, "cutoff must be positive");
     e.nC = b->n_compounds;
@@ -448,6 +550,11 @@ void create(Engine& e, const lima_box_desc* b, const lima_simparams* p) {
     HIP_CHECK(hipMalloc(&e.img_build, N * sizeof(int4)));
     HIP_CHECK(hipMalloc(&e.dmax2, sizeof(float)));
     HIP_CHECK(hipMemset(e.dmax2, 0, sizeof(float)));
+    {   // thermostat scale 1 (Engine.cu:130-131), no temperature measured yet
+        const float t0[2] = {1.f, 0.f};
+        HIP_CHECK(hipMalloc(&e.thermo, sizeof(t0)));
+        HIP_CHECK(hipMemcpy(e.thermo, t0, sizeof(t0), hipMemcpyHostToDevice));
+    }
     HIP_CHECK(hipMalloc(&e.dpart, ((N + 63) / 64) * sizeof(float)));
     HIP_CHECK(hipMemset(e.dpart, 0, ((N + 63) / 64) * sizeof(float)));
     HIP_CHECK(hipMalloc(&e.w_try, N * sizeof(uint4)));
@@ -554,11 +661,13 @@ void destroy(Engine& e) {
     if (e.s_main) (void)hipStreamSynchronize(e.s_main);
     nlist_free(e);
     pme_free(e);
-    void* ptrs[] = {e.p.w, e.p.img, e.p.v, e.p.fprev, e.p.prm, e.p.comp, e.w_build, e.img_build, e.dmax2, e.dpart, e.epart, e.w_try, e.img_try, e.f_acc, e.sd, e.first,
+    void* ptrs[] = {e.p.w, e.p.img, e.p.v, e.p.fprev, e.p.prm, e.p.comp, e.w_build, e.img_build, e.dmax2, e.thermo, e.dpart, e.epart, e.w_try, e.img_try, e.f_acc, e.sd, e.first,
                     e.bmask, e.intra_items, e.intra_lut, e.bd.bid, e.bd.bp, e.bd.aid, e.bd.ap, e.bd.did, e.bd.dp,
                     e.bd.iid, e.bd.ip, e.bd.pid, e.bd.pp, e.acc, e.fmax};
     for (void* p : ptrs) if (p) (void)hipFree(p);
     for (auto& ev : e.tev) if (ev) (void)hipEventDestroy(ev);
+    if (e.step_graph) (void)hipGraphExecDestroy(e.step_graph);
+    e.step_graph = nullptr;
     if (e.ev_fork) (void)hipEventDestroy(e.ev_fork);
     if (e.ev_join) (void)hipEventDestroy(e.ev_join);
     if (e.s_main) (void)hipStreamDestroy(e.s_main);
@@ -761,8 +870,11 @@ lima_status lima_get_status(lima_engine* h, lima_runstatus* out) {
     const Engine& e = h->e;
     std::memset(out, 0, sizeof(*out));
     out->current_step = e.step;
-    const double dof = 3.0 * (double)e.N - 3.0;
-    out->current_temperature = dof > 0 ? (float)(2.0 * e.energy.kinetic / (dof * (double)1.38066e-23f * 6.02214076e23)) : 0.f;
+    out->current_temperature = NAN;   // RunStatus default until the first measurement (Engine.cuh:67)
+    if (e.temp_measured) {
+        float t[2];
+        if (hipMemcpy(t, e.thermo, sizeof(t), hipMemcpyDeviceToHost) == hipSuccess) out->current_temperature = t[1];
+    }
     out->critical_error_occured = e.critical ? 1 : 0;
     out->simulation_finished = (e.prm.n_steps > 0 && e.step >= e.prm.n_steps) ? 1 : 0;
     out->n_list_rebuilds = e.n_rebuilds;
@@ -840,6 +952,8 @@ lima_status lima_get_timings(lima_engine* h, lima_timings* out) {
     return guarded([&] {
         timing_flush(h->e);
         *out = h->e.tacc;
+        out->host_issue_ms = h->e.host_issue_ms;
+        out->host_steps = h->e.host_steps;
     });
 }
 

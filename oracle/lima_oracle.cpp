@@ -96,6 +96,12 @@ struct lima_oracle {
     bool pme_divide_modulus = false;    // test switch: textbook smooth-PME deconvolution
     int K[3]{};
     int64_t step = 0;
+    // thermostat (Thermostat.cuh:56-103, Engine.cu:162-174)
+    bool thermo_apply = false;
+    int64_t thermo_every = 200;       // steps_per_temperature_measurement
+    int64_t log_interval = 5;         // data_logging_interval
+    float thermo_scale = 1.f;         // DeviceConstants::thermostatScalar, initially 1 (Engine.cu:130-131)
+    float temperature = NAN;          // RunStatus::current_temperature
     // last evaluation
     std::vector<double> F, pot;
     lima_energy E{};
@@ -798,6 +804,9 @@ int lima_oracle_create(const lima_box_desc* b, const lima_simparams* p, lima_ora
         o->dt_ns = (double)(p->dt_fs * 1e-6f);
         o->skin = p->nlist_skin_nm > 0 ? p->nlist_skin_nm : 0.1;
         o->elec = p->enable_electrostatics != 0;
+        o->thermo_apply = p->apply_thermostat != 0;
+        o->thermo_every = p->steps_per_temperature_measurement;
+        o->log_interval = p->data_logging_interval;
         const double spacing = p->pme_spacing_nm > 0 ? p->pme_spacing_nm : 0.12;
         for (int d = 0; d < 3; d++) o->K[d] = p->pme_grid[d] > 0 ? p->pme_grid[d] : derive_grid(o->L[d], spacing);
         for (int d = 0; d < 3; d++) if (o->L[d] <= o->rc + o->skin) throw std::runtime_error("box edge must exceed cutoff + skin");
@@ -830,6 +839,32 @@ int lima_oracle_forces(lima_oracle* o, int comps, double* force, double* pot, li
     return 0;
 }
 
+/* LIMA's thermostat.  hostMaster measures the temperature after a step when the logging buffer
+ * is full (step % (5 * data_logging_interval) == 0, SimulationDevice.cuh:50-54) and the step is a
+ * positive multiple of steps_per_temperature_measurement (Engine.cu:162-174).  Temperature from
+ * the kinetic energy of the stored velocities with dof = 3N - 3 (Thermostat.cuh:87-103,
+ * PhysicsUtils.cuh:32-34, BoxBuilder.cpp:108); the scale clamp(310 K / T, 1 -+ 0.001 / every)
+ * (Thermostat.cuh:56-67) replaces the integrator's scale only when apply_thermostat is set. */
+static void thermostat(lima_oracle* o) {
+    const int64_t buf = 5 * o->log_interval;
+    if (o->thermo_every <= 0 || buf <= 0 || o->step <= 0 || o->step % buf != 0 || o->step % o->thermo_every != 0) return;
+    double ke = 0;
+    for (int64_t i = 0; i < o->N * 3; i++) ke += 0.5 * o->mass[i / 3] * o->v[i] * o->v[i];
+    const double dof = 3.0 * (double)o->N - 3.0;
+    const float T = dof > 0 ? (float)(ke * (2.0 / dof) / ((double)1.38066e-23f * 6.02214076e23)) : 0.f;
+    o->temperature = T;
+    if (!o->thermo_apply) return;
+    const float safe = T == 0.f ? 1.f : T;
+    const float lim = 0.001f / (float)o->thermo_every;
+    o->thermo_scale = std::clamp(310.f / safe, 1.f - lim, 1.f + lim);
+}
+
+int lima_oracle_thermostat(lima_oracle* o, float* out) {
+    out[0] = o->thermo_scale;
+    out[1] = o->temperature;
+    return 0;
+}
+
 /* Velocity Verlet as LIMA integrates it (EngineUtils.cuh:31-41, EngineKernels.cuh:383-397):
  * v(t) = v(t-dt) + (F(t) + F(t-dt)) dt/2m ; x += Coord{v(t) dt + F(t) dt^2/2m} (truncation). */
 int lima_oracle_step(lima_oracle* o, int64_t nsteps) {
@@ -844,7 +879,7 @@ int lima_oracle_step(lima_oracle* o, int64_t nsteps) {
                 const double vn = o->v[i * 3 + d] + (f + o->fprev[i * 3 + d]) * (dt * 0.5 / m);
                 const double dx = vn * dt + f * (0.5 / m * dt * dt);
                 o->u[i * 3 + d] += (int64_t)std::trunc(dx * NANO_TO_LIMA);
-                o->v[i * 3 + d] = vn;
+                o->v[i * 3 + d] = vn * (double)o->thermo_scale;   // EngineKernels.cuh:406-411
                 o->fprev[i * 3 + d] = f;
                 ke += 0.5 * m * vn * vn;
             }
@@ -852,6 +887,7 @@ int lima_oracle_step(lima_oracle* o, int64_t nsteps) {
         o->E.kinetic = ke;
         o->E.total = o->E.potential + ke;
         o->step++;
+        thermostat(o);
     }
     o->have_forces = false;
     return 0;

@@ -42,8 +42,13 @@ enum {
  * pot [n_compounds*32 + n_tinymols] (LIMA per-particle potE), either may be NULL. */
 int lima_oracle_forces(lima_oracle* o, int components, double* force, double* pot, lima_energy* e);
 
-/* nsteps of LIMA's velocity Verlet on integer Coord positions. */
+/* nsteps of LIMA's velocity Verlet on integer Coord positions, with LIMA's thermostat
+ * (temperature measured, and the velocity scale replaced when apply_thermostat is set). */
 int lima_oracle_step(lima_oracle* o, int64_t nsteps);
+
+/* out[0] = velocity scale the integrator applies, out[1] = last measured temperature [K]
+ * (NaN before the first measurement). */
+int lima_oracle_thermostat(lima_oracle* o, float* out);
 
 /* Steepest descent, FP64 acceptance.  trace_energy / trace_fmax (may be NULL)
  * receive the accepted energy and fmax after every iteration (max_iter entries). */

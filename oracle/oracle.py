@@ -33,6 +33,7 @@ def lib():
             "lima_oracle_set_threads": ([C.c_int], None),
             "lima_oracle_forces": ([vp, C.c_int, d, d, C.POINTER(Energy)], C.c_int),
             "lima_oracle_step": ([vp, C.c_int64], C.c_int),
+            "lima_oracle_thermostat": ([vp, C.POINTER(C.c_float)], C.c_int),
             "lima_oracle_minimize": ([vp, C.c_double, C.c_int64, C.c_double, C.POINTER(EmResult), d, d], C.c_int),
             "lima_oracle_get_state": ([vp, C.POINTER(StateOut)], C.c_int),
             "lima_oracle_get_positions": ([vp, C.POINTER(C.c_int64)], C.c_int),
@@ -107,6 +108,12 @@ class Oracle:
 
     def step(self, n):
         lib().lima_oracle_step(self._h, n)
+
+    def thermostat(self):
+        """(velocity scale applied by the integrator, last measured temperature K or NaN)."""
+        out = (C.c_float * 2)()
+        lib().lima_oracle_thermostat(self._h, out)
+        return float(out[0]), float(out[1])
 
     def minimize(self, tol=100.0, max_iter=1000, h0=0.01, trace=False):
         r = EmResult()

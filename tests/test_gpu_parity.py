@@ -75,6 +75,57 @@ def test_md_trajectory_matches_oracle():
     assert abs(eg.kinetic - orc.forces()[2].kinetic) < 1e-3 * abs(eo.kinetic) + 1.0
 
 
+def _velocities(box, state):
+    li = box.layout_index()
+    v = np.concatenate([state["particle_velocity"].reshape(-1, 3), state["tinymol_velocity"][:box.n_tinymols]])
+    return v[li].astype(np.float64)
+
+
+def test_thermostat_trajectory_matches_oracle():
+    """Thermostat on, measured every 5 steps: temperatures, positions and velocities follow the
+    oracle.  Over 20 steps the scale (1 + 0.0002 per step) changes velocities by ~0.3%, far above
+    the tolerance, so a thermostat that is not applied, or applied at the wrong steps, fails."""
+    box = L.gen_water_lattice(10, 3.104, 0, 300.0)
+    p = L.default_params(dt_fs=1.0, apply_thermostat=1, steps_per_temperature_measurement=5, data_logging_interval=1)
+    eng = L.Engine(box, p)
+    orc = Oracle(box, p)
+    assert np.isnan(eng.status().current_temperature)
+    eng.step(20)
+    orc.step(20)
+    scale_o, T_o = orc.thermostat()
+    T_g = eng.status().current_temperature
+    print(f"T gpu {T_g:.4f} oracle {T_o:.4f}, oracle scale {scale_o:.7f}")
+    assert scale_o != 1.0
+    assert T_g == pytest.approx(T_o, rel=1e-5)
+    st = eng.state()
+    d = (unwrapped_positions(box, st) - orc.positions()) * 1e-8
+    assert float(np.sqrt((d ** 2).sum(1).mean())) < 1e-5
+    vg, vo = _velocities(box, st), _velocities(box, orc.state())
+    err = float(np.sqrt(((vg - vo) ** 2).sum() / (vo ** 2).sum()))
+    print("velocity rel RMS", err)
+    assert err < 2e-4
+
+
+def test_graph_replay_matches_eager_launches():
+    """The MD step replayed as a HIP graph (re-captured at each list rebuild) gives the trajectory
+    of the eager launches.  Float atomics make the force sums order dependent in the last bits,
+    so the comparison allows a few Coord units (1e-8 nm) of difference."""
+    box = L.gen_em_system(1200, 1003, 0.0)
+    runs = []
+    for mode in (0, 1):
+        eng = L.Engine(box, L.default_params(dt_fs=1.0, launch_mode=mode))
+        eng.step(35)                             # crosses three list rebuilds
+        st = eng.state()
+        runs.append((unwrapped_positions(box, st), _velocities(box, st), eng.energy().potential, eng.status()))
+    (xg, vg, eg, sg), (xe, ve, ee, se) = runs
+    assert sg.current_step == se.current_step == 35 and sg.n_list_rebuilds == se.n_list_rebuilds
+    dx = np.abs(xg - xe).max()
+    print("max |dx| Coord units", dx, "E graph", eg, "eager", ee)
+    assert dx <= 100
+    assert float(np.sqrt(((vg - ve) ** 2).sum() / (ve ** 2).sum())) < 1e-5
+    assert abs(eg - ee) <= 1e-6 * abs(ee)
+
+
 def test_minimize_decreases_energy():
     box = L.gen_em_system(300, 1000, 0.02)
     p = L.default_params(dt_fs=1.0)

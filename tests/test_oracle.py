@@ -14,6 +14,7 @@ mistake (dropped term, wrong sign, transposed index, wrong unit) fails:
   neighbour list     expanded cluster list == brute-force O(N^2) pair set
   integrator         harmonic bond frequency sqrt(k/mu)/2pi within 1% (ForceCorrectness.h:169-230)
   minimiser          monotone energy; a stretched bond relaxes to b0
+  thermostat         force-free gas: velocities v0 s^k with LIMA's clamp and measurement gating
 """
 import ctypes
 import re
@@ -380,13 +381,15 @@ def test_pme_two_charges_within_lima_tolerances(p0, p1, mirror):
 
 # ---------------------------------------------------------------------------------------------
 def _expand(box, atoms, off, ent, msk, radius):
-    """Cluster list -> set of (i, j, shift) pairs with i < j that lie within `radius` (integer test)."""
+    """Cluster list -> (set of (i, j, shift) pairs with i < j that lie within `radius` (integer
+    test), number of (entry, slot) items inside the radius)."""
     o = Oracle(box, params())
     pos = o.positions()
     Lq = np.round(box.box_nm * 1e8).astype(np.int64)
     W = np.mod(pos, Lq)
     R = int(round(radius * 1e8))
     out = set()
+    n = 0
     for I in range(len(off) - 1):
         for k in range(off[I], off[I + 1]):
             j = int(ent[k] & 0xFFFFFF)
@@ -399,7 +402,8 @@ def _expand(box, atoms, off, ent, msk, radius):
                 d = W[j] + s * Lq - W[i]
                 if int((d * d).sum()) <= R * R:
                     out.add((i, j, code) if i < j else (j, i, 26 - code))
-    return out
+                    n += 1
+    return out, n
 
 
 @pytest.mark.parametrize("which", ["water_small", "em_system"])
@@ -409,11 +413,12 @@ def test_cluster_list_covers_exactly_the_bruteforce_pairs(which):
     o = Oracle(box, p)
     atoms, off, ent, msk = o.neighbor_list()
     radius = 1.2 + 0.1
-    got = _expand(box, atoms, off, ent, msk, radius)
+    got, n_items = _expand(box, atoms, off, ent, msk, radius)
     pi, pj, ps = o.pairs(radius)
     brute = set(zip(pi.tolist(), pj.tolist(), ps.tolist()))
     assert got == brute
-    # a full list: every pair appears from both sides
+    # a full list: every pair (and image) inside the radius appears exactly twice, once per side
+    assert n_items == 2 * len(brute)
     assert len(ent) > 0
 
 
@@ -463,6 +468,66 @@ def test_integrator_conserves_energy_water():
     ke = kinetic(o)
     assert abs(tot[-1] - tot[0]) < 1e-2 * ke
     assert tot.std() < 5e-3 * ke
+
+
+def _free_gas(T_target, n=8, seed=3):
+    """n LJ-free tinymols (epsilon 0, no charges): forces vanish, so velocities change only
+    through the thermostat scale.  Velocities [nm/ns] drawn and rescaled so that
+    2 KE / ((3n - 3) k_B N_A) equals T_target exactly (in FP64)."""
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(0.5, 5.5, (n, 3))
+    m_g = 18.0
+    v = rng.normal(0.0, 1.0, (n, 3))
+    ke = 0.5 * (m_g / 1000.0) * (v ** 2).sum()
+    kB = float(np.float32(1.38066e-23))
+    v *= np.sqrt(T_target * (3 * n - 3) * kB * N_A / 2.0 / ke)
+    v = v.astype(np.float32).astype(np.float64)
+    box = CustomBox([6.0] * 3, pos, [0.0] * n, [0.3] * n, [0.0] * n, [m_g] * n, velocity=v, tinymol=[True] * n)
+    T = 2.0 * 0.5 * (np.float32(m_g / 1000.0) * (v ** 2).sum()) / ((3 * n - 3) * kB * N_A)
+    return box, v, T
+
+
+def test_thermostat_clamps_and_gates_like_lima():
+    """LIMA's thermostat (Thermostat.cuh:56-103, Engine.cu:162-174): the temperature is measured
+    after steps that are multiples of both steps_per_temperature_measurement and
+    5 * data_logging_interval, from T = 2 KE / (dof k_B N_A), dof = 3N - 3; the velocity scale
+    clamp(310 / T, 1 -+ 0.001 / every) multiplies every later step's velocities.  On a force-free
+    gas the velocities after k steps are therefore v0 * s^(k - first measurement step)."""
+    box, v0, T0 = _free_gas(1000.0)
+    p = params(dt_fs=1.0, enable_electrostatics=0, apply_thermostat=1, steps_per_temperature_measurement=10,
+               data_logging_interval=1)
+    o = Oracle(box, p)
+    o.step(9)
+    assert np.isnan(o.thermostat()[1])
+    o.step(16)                                   # measured at steps 10 and 20
+    s_expected = float(np.float32(1.0) - np.float32(np.float32(0.001) / np.float32(10.0)))
+    scale, T = o.thermostat()
+    assert scale == s_expected
+    v = o.state()["tinymol_velocity"][:8].astype(np.float64)
+    assert np.allclose(v, v0 * s_expected ** 15, rtol=1e-6, atol=0)
+    assert T == pytest.approx(T0 * s_expected ** 20, rel=1e-5)
+    # 5 * data_logging_interval = 15 does not divide 10 or 20: no measurement before step 30
+    o2 = Oracle(box, params(dt_fs=1.0, enable_electrostatics=0, apply_thermostat=1,
+                            steps_per_temperature_measurement=10, data_logging_interval=3))
+    o2.step(25)
+    assert np.isnan(o2.thermostat()[1]) and o2.thermostat()[0] == 1.0
+    assert np.allclose(o2.state()["tinymol_velocity"][:8], v0, rtol=1e-6, atol=0)
+    # measured but not applied
+    o3 = Oracle(box, params(dt_fs=1.0, enable_electrostatics=0, apply_thermostat=0,
+                            steps_per_temperature_measurement=5, data_logging_interval=1))
+    o3.step(5)
+    assert o3.thermostat()[0] == 1.0 and o3.thermostat()[1] == pytest.approx(T0, rel=1e-5)
+
+
+def test_thermostat_unclamped_scale_is_target_over_temperature():
+    box, v0, T0 = _free_gas(309.9)
+    o = Oracle(box, params(dt_fs=1.0, enable_electrostatics=0, apply_thermostat=1,
+                           steps_per_temperature_measurement=1, data_logging_interval=1))
+    o.step(5)                                    # first measurement at step 5 (buffer of 5 steps)
+    scale, T = o.thermostat()
+    assert T == pytest.approx(T0, rel=1e-5)
+    assert scale == pytest.approx(310.0 / T0, rel=1e-6)
+    assert abs(scale - 1.0) < 0.001
 
 
 def test_minimizer_monotone_and_bond_reaches_b0():
