@@ -92,7 +92,7 @@ struct Bonded {
 };
 
 // Device accumulators (FP64 energy reductions and scalars).
-constexpr int LAUNCH_EAGER = LIMA_LAUNCH_EAGER, LAUNCH_ONE_STREAM = LIMA_LAUNCH_ONE_STREAM;
+constexpr int LAUNCH_GRAPH = LIMA_LAUNCH_GRAPH, LAUNCH_ONE_STREAM = LIMA_LAUNCH_ONE_STREAM;
 
 enum Acc { ACC_LJ = 0, ACC_COUL, ACC_BOND, ACC_PME, ACC_KIN, ACC_PAIRS, ACC_N };
 

@@ -292,9 +292,9 @@ void evaluate(Engine& e, const uint4* w, const int4* img, bool energy, bool time
     if (split) {
         HIP_CHECK(hipEventRecord(e.ev_fork, s));
         HIP_CHECK(hipStreamWaitEvent(e.s_pme, e.ev_fork, 0));
+        launch_pme(e, w, energy, e.s_pme);
+        HIP_CHECK(hipEventRecord(e.ev_join, e.s_pme));
     }
-    // the short-range kernels are submitted first: they start while the host is still
-    // submitting the PME chain
     launch_pair(e, energy, s);
     if (ev) HIP_CHECK(hipEventRecord(ev[2], s));
     if (ev) {   // timed: each phase alone
@@ -305,13 +305,8 @@ void evaluate(Engine& e, const uint4* w, const int4* img, bool energy, bool time
     } else {
         launch_local(e, w, img, energy, s);
     }
-    if (split) {
-        launch_pme(e, w, energy, e.s_pme);
-        HIP_CHECK(hipEventRecord(e.ev_join, e.s_pme));
-        HIP_CHECK(hipStreamWaitEvent(s, e.ev_join, 0));
-    } else {
-        launch_pme(e, w, energy, s);
-    }
+    if (split) HIP_CHECK(hipStreamWaitEvent(s, e.ev_join, 0));
+    else launch_pme(e, w, energy, s);
     if (ev) HIP_CHECK(hipEventRecord(ev[5], s));
 }
 
@@ -423,7 +418,7 @@ void md_steps(Engine& e, int64_t n) {
             }
         }
         const bool energy = (e.prm.energy_interval > 0 && e.step % e.prm.energy_interval == 0) || k == n - 1;
-        if (!energy && !e.timing && !(e.prm.launch_mode & LAUNCH_EAGER)) {
+        if (!energy && !e.timing && (e.prm.launch_mode & LAUNCH_GRAPH)) {
             if (!e.step_graph_valid) capture_step_graph(e);
             HIP_CHECK(hipGraphLaunch(e.step_graph, e.s_main));
         } else {
@@ -455,8 +450,8 @@ void create(Engine& e, const lima_box_desc* b, const lima_simparams* p) {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw LimaFail(LIMA_ERR_NO_DEVICE, "no HIP device visible");
     e.prm = *p;
-    if (p->launch_mode < 0 || p->launch_mode > (LAUNCH_EAGER | LAUNCH_ONE_STREAM))
-        throw LimaFail(LIMA_ERR_BAD_INPUT, "launch_mode must be a combination of LIMA_LAUNCH_EAGER and LIMA_LAUNCH_ONE_STREAM");
+    if (p->launch_mode < 0 || p->launch_mode > (LAUNCH_GRAPH | LAUNCH_ONE_STREAM))
+        throw LimaFail(LIMA_ERR_BAD_INPUT, "launch_mode must be a combination of LIMA_LAUNCH_GRAPH and LIMA_LAUNCH_ONE_STREAM");
     if (p->apply_thermostat && p->steps_per_temperature_measurement <= 0)
         throw LimaFail(LIMA_ERR_BAD_INPUT, "the thermostat needs steps_per_temperature_measurement > 0");
     if (p->bc_select != LIMA_BC_PBC) throw LimaFail(LIMA_ERR_UNSUPPORTED, "only periodic boundary conditions are implemented");

@@ -15,6 +15,7 @@ mistake (dropped term, wrong sign, transposed index, wrong unit) fails:
   integrator         harmonic bond frequency sqrt(k/mu)/2pi within 1% (ForceCorrectness.h:169-230)
   minimiser          monotone energy; a stretched bond relaxes to b0
   thermostat         force-free gas: velocities v0 s^k with LIMA's clamp and measurement gating
+  improper           axis geometry: closed-form energy, rotational gradient on i and l, zero net force
 """
 import ctypes
 import re
@@ -25,7 +26,7 @@ import pytest
 from scipy.special import erfc
 
 import lima_amd as L
-from lima_amd.boxdesc import CustomBox
+from lima_amd.boxdesc import CustomBox, DEG_TO_RAD_L
 from oracle.oracle import Oracle, fft3d, NB_CUTOFF, NB_INTRA, BONDED, PME, ALL
 
 ROOT = Path(__file__).resolve().parent.parent
@@ -218,6 +219,62 @@ def test_bonded_closed_form_energies():
     pos_c = [[1, 1, 1], [1.1, 1.1, 1], [1.25, 1.1, 1], [1.35, 1.0, 1]]   # cis: phi = 0 -> k (1 + 1)
     bc = CustomBox([5.0] * 3, pos_c, [0] * 4, [0] * 4, [0] * 4, [12.0] * 4, dihedrals=[(0, 1, 2, 3, 0.0, 4.0, 3)])
     assert Oracle(bc, params(enable_electrostatics=0)).forces(BONDED)[2].bonded == pytest.approx(2 * 4.0 * 1000 / 2, rel=1e-6)
+
+
+def _improper_box(phi, theta=0.0, a1=0.12, a2=0.12, r=0.13, psi0_deg=0.0, k_kj=300.0):
+    """j and k on an axis along z, i at radius r and angle theta around it, l at radius r and
+    angle phi.  LIMA's plane normals (j - i) x (k - i) and (j - l) x (k - l) then point along the
+    tangents of i and l, so the improper angle is the rotation theta - phi between the half planes
+    (negative for l ahead of i, by LIMA's sign rule n1 . (l - i) > 0)."""
+    c = np.array([1.5, 1.5, 1.5])
+    pos = np.array([[r * np.cos(theta), r * np.sin(theta), 0.0], [0.0, 0.0, -a1], [0.0, 0.0, a2],
+                    [r * np.cos(phi), r * np.sin(phi), 0.0]]) + c
+    return CustomBox([4.0] * 3, pos, [0] * 4, [0] * 4, [0] * 4, [12.0] * 4, impropers=[(0, 1, 2, 3, psi0_deg, k_kj)])
+
+
+def _improper_eval(**kw):
+    b = _improper_box(**kw)
+    f, _, e = Oracle(b, params(enable_electrostatics=0)).forces(BONDED)
+    return f[b.gid], e.bonded
+
+
+@pytest.mark.parametrize("phi,psi0_deg", [(0.3, 0.0), (1.0, 20.0), (2.0, -15.0)])
+def test_improper_closed_form_and_rotational_gradient(phi, psi0_deg):
+    """LIMA's improper (ForceComputations.cuh:214-249) is a heuristic force split, not the gradient
+    of its energy, so it cannot be pinned by finite differences in general.  In the axis geometry
+    above it is pinned independently: the energy is 1/2 k (psi - psi0)^2 with psi = theta - phi
+    (geometry, not LIMA's formula), the forces on i and l along their rotation tangents equal
+    -dE/ds (central differences of the FP64 energy under rotation about the axis), the four forces
+    sum to zero, and the symmetric j / k pair shares the residual equally."""
+    k_kj, r = 300.0, 0.13
+    f, e = _improper_eval(phi=phi, psi0_deg=psi0_deg, k_kj=k_kj, r=r)
+    psi0 = float(np.float32(psi0_deg) * DEG_TO_RAD_L)
+    assert e == pytest.approx(0.5 * k_kj * 1000.0 * (-phi - psi0) ** 2, rel=1e-6)
+    h = 1e-2   # E is exactly quadratic in the rotation angle, so central differences are exact;
+    #            a large step keeps the 1e-8 nm Coord rounding of the rotated positions negligible
+    dE_l = (_improper_eval(phi=phi + h, psi0_deg=psi0_deg, k_kj=k_kj, r=r)[1]
+            - _improper_eval(phi=phi - h, psi0_deg=psi0_deg, k_kj=k_kj, r=r)[1]) / (2 * h)
+    dE_i = (_improper_eval(phi=phi, theta=h, psi0_deg=psi0_deg, k_kj=k_kj, r=r)[1]
+            - _improper_eval(phi=phi, theta=-h, psi0_deg=psi0_deg, k_kj=k_kj, r=r)[1]) / (2 * h)
+    t_l = np.array([-np.sin(phi), np.cos(phi), 0.0])
+    t_i = np.array([0.0, 1.0, 0.0])
+    scale = np.abs(f).max()
+    assert f[3] @ t_l == pytest.approx(-dE_l / r, rel=1e-5, abs=1e-6 * scale)
+    assert f[0] @ t_i == pytest.approx(-dE_i / r, rel=1e-5, abs=1e-6 * scale)
+    assert np.abs(f.sum(0)).max() < 1e-9 * scale
+    assert np.allclose(f[1], f[2], rtol=1e-9, atol=1e-9 * scale)
+
+
+def test_improper_residual_goes_mostly_to_the_nearer_axis_atom():
+    """With j nearer to i than k, j takes the larger share of the residual -(F_i + F_l), both
+    shares parallel to it (lever split, ForceComputations.cuh:241-244)."""
+    f, _ = _improper_eval(phi=1.0, a1=0.08, a2=0.2)
+    res = -(f[0] + f[3])
+    assert np.linalg.norm(f[1]) > np.linalg.norm(f[2]) > 0
+    for fj in (f[1], f[2]):
+        assert np.linalg.norm(np.cross(fj, res)) < 1e-9 * np.linalg.norm(fj) * np.linalg.norm(res)
+        assert fj @ res > 0
+    assert np.allclose(f[1] + f[2], res, rtol=1e-9, atol=1e-9 * np.abs(res).max())
 
 
 def test_nonbonded_forces_are_energy_gradients():
