@@ -125,7 +125,6 @@ constexpr float HIST_R = 1.0f;         // neighbour histogram radius [nm]
 constexpr int HIST_BINS = 14;          // bins per charge sign
 constexpr int HIST_MAX_ATOMS = 4096;   // byte counters in LDS: 28 per atom
 
-enum { T_BOND = 0, T_ANGLE = 1, T_DIH = 2, T_IMP = 3, T_PAIR = 4 };
 
 __device__ __forceinline__ int fmodi(int x, int L) {
     int r = x % L;

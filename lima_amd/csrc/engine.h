@@ -89,7 +89,13 @@ struct Bonded {
     int4* did = nullptr; float4* dp = nullptr;
     int4* iid = nullptr; float2* ip = nullptr;
     int2* pid = nullptr; float2* pp = nullptr;
+    // incidence lanes: one lane per (atom, term) pair, the lanes of an atom contiguous and inside
+    // one wave; lane_atom = atom | (last lane of the atom's run within the wave) << 24
+    int n_lanes = 0;             // multiple of 64
+    uint32_t* lane_code = nullptr;   // inc_code, or LANE_PAD
+    int* lane_atom = nullptr;
 };
+constexpr uint32_t LANE_PAD = 0xffffffffu;
 
 // Device accumulators (FP64 energy reductions and scalars).
 constexpr int LAUNCH_GRAPH = LIMA_LAUNCH_GRAPH, LAUNCH_ONE_STREAM = LIMA_LAUNCH_ONE_STREAM;
@@ -152,8 +158,10 @@ struct Engine {
     NList nl;
     Pme pme;
 
-    float* fs = nullptr;       // [3*slots] forces per cluster slot (SoA)
+    float* fs = nullptr;       // [3*slots] forces per cluster slot (SoA), summed with atomics
     float* es = nullptr;       // [slots] per-slot potE
+    float* fb = nullptr;       // [3*slots] bonded force per slot, stored once by the atom's bonded lanes
+    float* eb = nullptr;       // [slots] bonded potE share per slot (energy evaluations)
     double* acc = nullptr;     // [ACC_N]
     double* epart = nullptr;   // [epart_cap * ACC_N] per-wave energy records
     int64_t epart_cap = 0;

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "engine.h"
+#include "lima_terms.h"
 
 using namespace lima;
 
@@ -45,6 +46,7 @@ struct IntegrateArgs {
     double* epart;         // kinetic-energy region: one record per wave
     const int* slot_of;
     const float* fs;
+    const float* fb;
     uint4* w;
     int4* img;
     float4* v;
@@ -75,7 +77,7 @@ __global__ void k_integrate(IntegrateArgs a) {
     float ke = 0.f, d2 = 0.f;
     if (p < a.N) {
         const int s = a.slot_of[p];
-        const float fx = a.fs[s], fy = a.fs[a.nslots + s], fz = a.fs[2 * a.nslots + s];
+        const float fx = a.fs[s] + a.fb[s], fy = a.fs[a.nslots + s] + a.fb[a.nslots + s], fz = a.fs[2 * a.nslots + s] + a.fb[2 * a.nslots + s];
         const float m = a.prm[p].w;
         const float4 fp = a.fprev[p];
         float4 v = a.v[p];
@@ -160,12 +162,14 @@ __global__ __launch_bounds__(256) void k_thermostat(const double* __restrict__ k
 
 // Forces of the last evaluation in global order (w = per-particle potE) and max |F|.
 __global__ void k_forces_global(int64_t N, int nslots, const int* __restrict__ slot_of, const float* __restrict__ fs,
-                                const float* __restrict__ es, float4* __restrict__ out, float* fmax) {
+                                const float* __restrict__ es, const float* __restrict__ fb, const float* __restrict__ eb,
+                                float4* __restrict__ out, float* fmax) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     float m = 0.f;
     if (p < N) {
         const int s = slot_of[p];
-        const float4 f = make_float4(fs[s], fs[nslots + s], fs[2 * nslots + s], es[s]);
+        const float4 f = make_float4(fs[s] + fb[s], fs[nslots + s] + fb[nslots + s], fs[2 * nslots + s] + fb[2 * nslots + s],
+                                     es[s] + eb[s]);
         out[p] = f;
         m = sqrtf(f.x * f.x + f.y * f.y + f.z * f.z);
         if (!(m <= 3.0e38f)) m = 3.0e38f;   // NaN / inf -> flagged as huge
@@ -341,7 +345,7 @@ void integrate(Engine& e, bool energy) {
     for (int d = 0; d < 3; d++) a.Lq[d] = e.Lq[d];
     a.dt = (float)e.dt_ns;
     a.thermostat = e.thermo;
-    a.slot_of = e.nl.slot_of; a.fs = e.fs; a.w = e.p.w; a.img = e.p.img; a.v = e.p.v; a.fprev = e.p.fprev;
+    a.slot_of = e.nl.slot_of; a.fs = e.fs; a.fb = e.fb; a.w = e.p.w; a.img = e.p.img; a.v = e.p.v; a.fprev = e.p.fprev;
     a.prm = e.p.prm; a.epart = e.epart + (int64_t)e.ereg.kin * ACC_N;
     a.w_build = e.w_build; a.img_build = e.img_build; a.dpart = e.dpart;
     if (energy) hipLaunchKernelGGL(k_integrate<true>, dim3(nblk(e.N, 256)), dim3(256), 0, e.s_main, a);
@@ -632,6 +636,53 @@ void create(Engine& e, const lima_box_desc* b, const lima_simparams* p) {
             pid[k] = make_int2(gid(b->pair_ids[2 * k]), gid(b->pair_ids[2 * k + 1]));
             pp[k] = make_float2(b->pair_params[2 * k], b->pair_params[2 * k + 1]);
         }
+        // incidence lists: every atom of a term gets (type, role, term); atoms in global order
+        {
+            const int64_t nterms = (int64_t)b->n_bonds + b->n_angles + b->n_dihedrals + b->n_impropers + b->n_pairs;
+            if (nterms >= INC_MAX_TERMS) throw LimaFail(LIMA_ERR_BAD_INPUT, "too many bonded terms");
+            if (N >= (1 << 24)) throw LimaFail(LIMA_ERR_BAD_INPUT, "bonded lanes address at most 2^24 particles");
+            std::vector<int> cnt(N + 1, 0);
+            auto count = [&](int atom) { cnt[atom]++; };
+            for (auto& t : bid) { count(t.x); count(t.y); }
+            for (auto& t : aid) { count(t.x); count(t.y); count(t.z); }
+            for (auto& t : did) { count(t.x); count(t.y); count(t.z); count(t.w); }
+            for (auto& t : iid) { count(t.x); count(t.y); count(t.z); count(t.w); }
+            for (auto& t : pid) { count(t.x); count(t.y); }
+            std::vector<int> off(N + 1, 0);
+            for (int64_t i = 0; i < N; i++) {
+                if (cnt[i] > 64) throw LimaFail(LIMA_ERR_UNSUPPORTED, "a particle belongs to more than 64 bonded terms");
+                off[i + 1] = off[i] + cnt[i];
+            }
+            std::vector<uint32_t> inc(off[N]);
+            std::vector<int> fill(off.begin(), off.end() - 1);
+            auto add = [&](int atom, int type, int role, int term) { inc[fill[atom]++] = inc_code(type, role, term); };
+            for (int k = 0; k < b->n_bonds; k++) { add(bid[k].x, T_BOND, 0, k); add(bid[k].y, T_BOND, 1, k); }
+            for (int k = 0; k < b->n_angles; k++) { add(aid[k].x, T_ANGLE, 0, k); add(aid[k].y, T_ANGLE, 1, k); add(aid[k].z, T_ANGLE, 2, k); }
+            for (int k = 0; k < b->n_dihedrals; k++) {
+                add(did[k].x, T_DIH, 0, k); add(did[k].y, T_DIH, 1, k); add(did[k].z, T_DIH, 2, k); add(did[k].w, T_DIH, 3, k);
+            }
+            for (int k = 0; k < b->n_impropers; k++) {
+                add(iid[k].x, T_IMP, 0, k); add(iid[k].y, T_IMP, 1, k); add(iid[k].z, T_IMP, 2, k); add(iid[k].w, T_IMP, 3, k);
+            }
+            for (int k = 0; k < b->n_pairs; k++) { add(pid[k].x, T_PAIR, 0, k); add(pid[k].y, T_PAIR, 1, k); }
+            // lanes: an atom's incidences in consecutive lanes of one wave (runs of at most 64)
+            std::vector<uint32_t> lcode;
+            std::vector<int> latom;
+            for (int64_t i = 0; i < N; i++) {
+                for (int q = off[i]; q < off[i + 1]; q += 64) {
+                    const int run = std::min(64, off[i + 1] - q);
+                    const int inwave = (int)(lcode.size() & 63);
+                    if (inwave + run > 64)   // pad to the next wave
+                        while (lcode.size() & 63) { lcode.push_back(LANE_PAD); latom.push_back((int)((lcode.size() - 1) & 63) << 24); }
+                    const int last = (int)((lcode.size() + run - 1) & 63);
+                    for (int r = 0; r < run; r++) { lcode.push_back(inc[q + r]); latom.push_back((int)i | (last << 24)); }
+                }
+            }
+            while (lcode.size() & 63) { lcode.push_back(LANE_PAD); latom.push_back((int)((lcode.size() - 1) & 63) << 24); }
+            e.bd.n_lanes = (int)lcode.size();
+            e.bd.lane_code = dalloc_copy(lcode);
+            e.bd.lane_atom = dalloc_copy(latom);
+        }
         e.bd.nb = b->n_bonds; e.bd.na = b->n_angles; e.bd.nd = b->n_dihedrals; e.bd.ni = b->n_impropers; e.bd.np = b->n_pairs;
         e.bd.bid = dalloc_copy(bid); e.bd.bp = dalloc_copy(bp);
         e.bd.aid = dalloc_copy(aid); e.bd.ap = dalloc_copy(ap);
@@ -658,7 +709,7 @@ void destroy(Engine& e) {
     pme_free(e);
     void* ptrs[] = {e.p.w, e.p.img, e.p.v, e.p.fprev, e.p.prm, e.p.comp, e.w_build, e.img_build, e.dmax2, e.thermo, e.dpart, e.epart, e.w_try, e.img_try, e.f_acc, e.sd, e.first,
                     e.bmask, e.intra_items, e.intra_lut, e.bd.bid, e.bd.bp, e.bd.aid, e.bd.ap, e.bd.did, e.bd.dp,
-                    e.bd.iid, e.bd.ip, e.bd.pid, e.bd.pp, e.acc, e.fmax};
+                    e.bd.iid, e.bd.ip, e.bd.pid, e.bd.pp, e.bd.lane_code, e.bd.lane_atom, e.acc, e.fmax};
     for (void* p : ptrs) if (p) (void)hipFree(p);
     for (auto& ev : e.tev) if (ev) (void)hipEventDestroy(ev);
     if (e.step_graph) (void)hipGraphExecDestroy(e.step_graph);
@@ -672,7 +723,7 @@ void destroy(Engine& e) {
 void forces_global(Engine& e) {
     hipLaunchKernelGGL(k_zero_f, dim3(1), dim3(64), 0, e.s_main, e.fmax, 1);
     hipLaunchKernelGGL(k_forces_global, dim3(nblk(e.N, 256)), dim3(256), 0, e.s_main, e.N, e.nl.ncl * CLUSTER, e.nl.slot_of,
-                       e.fs, e.es, e.f_acc, e.fmax);
+                       e.fs, e.es, e.fb, e.eb, e.f_acc, e.fmax);
 }
 
 }  // namespace
@@ -770,7 +821,7 @@ lima_status lima_minimize(lima_engine* h, double tol_kj, int64_t max_iter, doubl
             // trial forces in global order -> p.fprev used as scratch
             hipLaunchKernelGGL(k_zero_f, dim3(1), dim3(64), 0, s, e.fmax, 1);
             hipLaunchKernelGGL(k_forces_global, dim3(nblk(e.N, 256)), dim3(256), 0, s, e.N, e.nl.ncl * CLUSTER, e.nl.slot_of,
-                               e.fs, e.es, e.p.fprev, e.fmax);
+                               e.fs, e.es, e.fb, e.eb, e.p.fprev, e.fmax);
             hipLaunchKernelGGL(k_sd_accept, dim3(1), dim3(1), 0, s, e.sd, e.acc, e_const(e), e.fmax);
             hipLaunchKernelGGL(k_sd_commit, dim3(nblk(e.N, 256)), dim3(256), 0, s, e.N, e.sd, e.w_try, e.img_try, e.p.fprev,
                                e.p.w, e.p.img, e.f_acc);

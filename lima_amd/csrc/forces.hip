@@ -14,8 +14,8 @@
 // displacement is unwrapped; a bonded compound is first moved to the image whose key particle is
 // nearest (LIMA's hyperpos of the neighbour's origo, EngineKernels.cuh:235-236).
 //
-// Bonded kernel: ForceComputations.cuh:17-264,335-376; one thread per term, forces added
-// to the particles' cluster slots with float atomics.
+// Bonded kernel: ForceComputations.cuh:17-264,335-376; one lane per (atom, term) incidence, the
+// lanes of an atom summed in-wave and added to its cluster slot once.
 #include "engine.h"
 #include "lima_terms.h"
 
@@ -302,11 +302,16 @@ struct BondArgs {
     const int4* did; const float4* dp;
     const int4* iid; const float2* ip;
     const int2* pid; const float2* pp;
+    int n_lanes;
+    const uint32_t* lane_code;
+    const int* lane_atom;
     const uint4* w;
     const int4* img;
     const int* slot_of;
     float* fs;
     float* es;
+    float* fb;
+    float* eb;
     double* epart;
 };
 
@@ -324,69 +329,85 @@ __device__ __forceinline__ v3 bdisp(const BondArgs& a, int from, int to) {
     return {(float)d[0] * LIMA_LIMA_TO_NANO_F, (float)d[1] * LIMA_LIMA_TO_NANO_F, (float)d[2] * LIMA_LIMA_TO_NANO_F};
 }
 
-__device__ __forceinline__ void addf(const BondArgs& a, int p, v3 f) {
-    const int s = a.slot_of[p];
-    atomicAdd(&a.fs[s], f.x);
-    atomicAdd(&a.fs[a.nslots + s], f.y);
-    atomicAdd(&a.fs[2 * a.nslots + s], f.z);
-}
-
+// Bonded terms, one lane per (atom, term) incidence.  An atom's incidences occupy consecutive
+// lanes of one wave (at most 64, checked at creation); each lane evaluates its term
+// (ForceComputations.cuh:17-376) and keeps the atom's share, a segmented shuffle sum gathers the
+// run into its first lane, and that lane stores the atom's bonded force in the slot's bonded
+// array (fb): one plain store per component, no atomics.  fb is cleared at every list rebuild, so
+// slots of atoms without terms read zero; the integrator adds fb to the atomically summed fs.  Every lane holds one short dependency chain, so long
+// incidence lists (chain atoms belong to ~10 terms) do not serialise; per-atom sums are in a
+// fixed order.  LIMA splits a term's potE equally over its atoms (EngineKernels.cuh:776-790);
+// the FP64 total counts each term once, at its role-0 lane.
 template <bool ENERGY>
-__device__ __forceinline__ void bonded_terms(const BondArgs& a, int blk) {
-    const int t = blk * blockDim.x + threadIdx.x;
-    float e = 0.f;
-    int ids[4];
-    int nat = 0;
-    if (t < a.nb) {   // calcSinglebondForces, ForceComputations.cuh:17-51
-        const int2 id = a.bid[t];
-        v3 f;
-        e = term_bond(bdisp(a, id.y, id.x), a.bp[t], &f);
-        addf(a, id.x, f);
-        addf(a, id.y, -f);
-        ids[0] = id.x; ids[1] = id.y; nat = 2;
-    } else if (t < a.nb + a.na) {   // calcAnglebondForces :53-115, harmonic in theta + Urey-Bradley
-        const int k = t - a.nb;
-        const int4 id = a.aid[k];
-        v3 fl, fr;
-        e = term_angle(bdisp(a, id.y, id.x), bdisp(a, id.y, id.z), bdisp(a, id.z, id.x), a.ap[k], &fl, &fr);
-        addf(a, id.x, fl);
-        addf(a, id.z, fr);
-        addf(a, id.y, -(fl + fr));
-        ids[0] = id.x; ids[1] = id.y; ids[2] = id.z; nat = 3;
-    } else if (t < a.nb + a.na + a.nd) {   // calcDihedralbondForces :119-210, phi_L = -atan2(sin, cos)
-        const int k = t - a.nb - a.na;
-        const int4 id = a.did[k];
-        v3 f[4];
-        e = term_dihedral(bdisp(a, id.x, id.y), bdisp(a, id.y, id.z), bdisp(a, id.z, id.w), a.dp[k], f);
-        addf(a, id.x, f[0]);
-        addf(a, id.y, f[1]);
-        addf(a, id.z, f[2]);
-        addf(a, id.w, f[3]);
-        ids[0] = id.x; ids[1] = id.y; ids[2] = id.z; ids[3] = id.w; nat = 4;
-    } else if (t < a.nb + a.na + a.nd + a.ni) {   // calcImproperdihedralbondForces :214-264
-        const int k = t - a.nb - a.na - a.nd;
-        const int4 id = a.iid[k];
-        v3 f[4];
-        e = term_improper(bdisp(a, id.x, id.y), bdisp(a, id.x, id.z), bdisp(a, id.x, id.w), a.ip[k], f);
-        addf(a, id.x, f[0]);
-        addf(a, id.y, f[1]);
-        addf(a, id.z, f[2]);
-        addf(a, id.w, f[3]);
-        ids[0] = id.x; ids[1] = id.y; ids[2] = id.z; ids[3] = id.w; nat = 4;
-    } else if (t < a.nb + a.na + a.nd + a.ni + a.np) {   // computePairbondForces :335-376
-        const int k = t - a.nb - a.na - a.nd - a.ni;
-        const int2 id = a.pid[k];
-        v3 f;
-        e = term_pair14(bdisp(a, id.x, id.y), a.pp[k], &f);
-        addf(a, id.x, f);
-        addf(a, id.y, -f);
-        ids[0] = id.x; ids[1] = id.y; nat = 2;
+__device__ __forceinline__ void bonded_lanes(const BondArgs& a, int blk) {
+    const int k = blk * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    uint32_t code = LANE_PAD;
+    int la = lane << 24;
+    if (k < a.n_lanes) { code = a.lane_code[k]; la = a.lane_atom[k]; }
+    const int last = (la >> 24) & 63, atom = la & 0xffffff;
+    v3 f = {0.f, 0.f, 0.f};
+    float e = 0.f, eshare = 0.f;
+    int role = -1;
+    if (code != LANE_PAD) {
+        const int type = (int)(code >> 29), t = (int)(code & (INC_MAX_TERMS - 1));
+        role = (int)((code >> 27) & 3u);
+        int nat;
+        if (type == T_BOND) {   // calcSinglebondForces :17-51
+            const int2 id = a.bid[t];
+            v3 fa;
+            e = term_bond(bdisp(a, id.y, id.x), a.bp[t], &fa);
+            f = role == 0 ? fa : -fa;
+            nat = 2;
+        } else if (type == T_ANGLE) {   // calcAnglebondForces :53-115, harmonic in theta + Urey-Bradley
+            const int4 id = a.aid[t];
+            v3 fl, fr;
+            e = term_angle(bdisp(a, id.y, id.x), bdisp(a, id.y, id.z), bdisp(a, id.z, id.x), a.ap[t], &fl, &fr);
+            f = role == 0 ? fl : (role == 2 ? fr : -(fl + fr));
+            nat = 3;
+        } else if (type == T_DIH) {   // calcDihedralbondForces :119-210, phi_L = -atan2(sin, cos)
+            const int4 id = a.did[t];
+            v3 fd[4];
+            e = term_dihedral(bdisp(a, id.x, id.y), bdisp(a, id.y, id.z), bdisp(a, id.z, id.w), a.dp[t], fd);
+            f = role == 0 ? fd[0] : (role == 1 ? fd[1] : (role == 2 ? fd[2] : fd[3]));
+            nat = 4;
+        } else if (type == T_IMP) {   // calcImproperdihedralbondForces :214-264
+            const int4 id = a.iid[t];
+            v3 fd[4];
+            e = term_improper(bdisp(a, id.x, id.y), bdisp(a, id.x, id.z), bdisp(a, id.x, id.w), a.ip[t], fd);
+            f = role == 0 ? fd[0] : (role == 1 ? fd[1] : (role == 2 ? fd[2] : fd[3]));
+            nat = 4;
+        } else {   // computePairbondForces :335-376
+            const int2 id = a.pid[t];
+            v3 fa;
+            e = term_pair14(bdisp(a, id.x, id.y), a.pp[t], &fa);
+            f = role == 0 ? fa : -fa;
+            nat = 2;
+        }
+        eshare = e / (float)nat;
+    }
+    // run sums into the run's first lane: after the step with offset o, lane i holds the sum of
+    // lanes i .. min(i + 2o - 1, last)
+    for (int o = 1; o < 64; o <<= 1) {
+        const float ox = __shfl_down(f.x, o, 64), oy = __shfl_down(f.y, o, 64), oz = __shfl_down(f.z, o, 64);
+        const float oe = ENERGY ? __shfl_down(eshare, o, 64) : 0.f;
+        if (lane + o <= last) {
+            f.x += ox; f.y += oy; f.z += oz;
+            if (ENERGY) eshare += oe;
+        }
+    }
+    const int prev = __shfl_up(la, 1, 64);
+    if (code != LANE_PAD && (lane == 0 || prev != la)) {   // the run's first lane: the atom's only run
+        const int s = a.slot_of[atom];
+        a.fb[s] = f.x;
+        a.fb[a.nslots + s] = f.y;
+        a.fb[2 * a.nslots + s] = f.z;
+        if (ENERGY) a.eb[s] = eshare;
     }
     if (ENERGY) {
-        for (int k = 0; k < nat; k++) atomicAdd(&a.es[a.slot_of[ids[k]]], e / (float)nat);
-        const double tot = wave_sum_d((double)e);
-        if ((threadIdx.x & 63) == 0 && (t & ~63) < a.nb + a.na + a.nd + a.ni + a.np) {   // waves inside the region
-            double* r = a.epart + (int64_t)(t >> 6) * ACC_N;
+        const double tot = wave_sum_d(role == 0 ? (double)e : 0.0);
+        if (lane == 0 && (k & ~63) < a.n_lanes) {   // waves inside the region
+            double* r = a.epart + (int64_t)(k >> 6) * ACC_N;
             r[ACC_LJ] = 0.0; r[ACC_COUL] = 0.0; r[ACC_BOND] = tot; r[ACC_PME] = 0.0; r[ACC_KIN] = 0.0; r[ACC_PAIRS] = 0.0;
         }
     }
@@ -399,7 +420,7 @@ template <bool ENERGY>
 __global__ __launch_bounds__(256) void k_local(IntraArgs ia, BondArgs ba, int nib) {
     __shared__ IntraLds L;
     if ((int)blockIdx.x < nib) intra_items<ENERGY>(ia, blockIdx.x, L);
-    else bonded_terms<ENERGY>(ba, blockIdx.x - nib);
+    else bonded_lanes<ENERGY>(ba, blockIdx.x - nib);
 }
 
 // Sums the per-wave energy records of all regions in a fixed order: acc[c] = sum_r epart[r][c].
@@ -437,8 +458,7 @@ ERegions energy_regions(const Engine& e) {
     r.pair = 0;
     r.intra = r.pair + e.nl.ncl * pair_split(e);
     r.bond = r.intra + e.n_intra;
-    const int nterms = e.bd.nb + e.bd.na + e.bd.nd + e.bd.ni + e.bd.np;
-    r.pme = r.bond + (nterms + 63) / 64;
+    r.pme = r.bond + e.bd.n_lanes / 64;   // one record per wave of the bonded incidence lanes
     // PME records: one per gather wave (4 per tile block when tiled)
     r.kin = r.pme + (e.pme.tiled ? e.pme.ntiles * 4 : (int)((e.N + 63) / 64));
     r.total = r.kin + (int)((e.N + 63) / 64);
@@ -492,16 +512,17 @@ BondArgs bond_args(Engine& e, const uint4* w, const int4* img) {
     a.pc = e.pc;
     a.bid = e.bd.bid; a.bp = e.bd.bp; a.aid = e.bd.aid; a.ap = e.bd.ap; a.did = e.bd.did; a.dp = e.bd.dp;
     a.iid = e.bd.iid; a.ip = e.bd.ip; a.pid = e.bd.pid; a.pp = e.bd.pp;
-    a.w = w; a.img = img; a.slot_of = e.nl.slot_of; a.fs = e.fs; a.es = e.es; a.epart = e.epart + (int64_t)e.ereg.bond * ACC_N;
+    a.n_lanes = e.bd.n_lanes; a.lane_code = e.bd.lane_code; a.lane_atom = e.bd.lane_atom;
+    a.w = w; a.img = img; a.slot_of = e.nl.slot_of; a.fs = e.fs; a.es = e.es; a.fb = e.fb; a.eb = e.eb;
+    a.epart = e.epart + (int64_t)e.ereg.bond * ACC_N;
     return a;
 }
 
 void launch_local_blocks(Engine& e, const uint4* w, const int4* img, bool energy, bool intra, bool bonded, hipStream_t s) {
     const IntraArgs ia = intra_args(e, w, img);
     const BondArgs ba = bond_args(e, w, img);
-    const int nterms = ba.nb + ba.na + ba.nd + ba.ni + ba.np;
     const int nib = intra ? nblk(ia.n_items, 4) : 0;
-    const int nbb = bonded ? nblk(nterms, 256) : 0;
+    const int nbb = bonded ? nblk(ba.n_lanes, 256) : 0;
     if (nib + nbb == 0) return;
     if (energy) hipLaunchKernelGGL(k_local<true>, dim3(nib + nbb), dim3(256), 0, s, ia, ba, nib);
     else hipLaunchKernelGGL(k_local<false>, dim3(nib + nbb), dim3(256), 0, s, ia, ba, nib);

@@ -6,6 +6,14 @@
 
 namespace lima {
 
+// Bonded-term incidence codes: (type << 29) | (role << 27) | term.  An atom's incidence list holds
+// one code per term it belongs to; role is its position in the term (0 .. 3).
+enum { T_BOND = 0, T_ANGLE = 1, T_DIH = 2, T_IMP = 3, T_PAIR = 4 };
+constexpr int INC_MAX_TERMS = 1 << 27;
+__host__ __device__ inline uint32_t inc_code(int type, int role, int term) {
+    return ((uint32_t)type << 29) | ((uint32_t)role << 27) | (uint32_t)term;
+}
+
 struct v3 {
     float x, y, z;
 };

@@ -587,6 +587,8 @@ void nlist_alloc(Engine& e) {
     HIP_CHECK(hipMalloc(&nl.eoff, (nl.ncl_max + 2) * sizeof(int)));
     HIP_CHECK(hipMalloc(&nl.flags, 4 * sizeof(int)));
     HIP_CHECK(hipMalloc(&e.fs, 3 * slots * sizeof(float)));
+    HIP_CHECK(hipMalloc(&e.fb, 3 * slots * sizeof(float)));
+    HIP_CHECK(hipMalloc(&e.eb, slots * sizeof(float)));
     HIP_CHECK(hipMalloc(&e.es, slots * sizeof(float)));
     nl.nent_cap = 0;
     nl.cap = 2048;
@@ -598,7 +600,7 @@ void nlist_free(Engine& e) {
     NList& nl = e.nl;
     void* ptrs[] = {nl.col_of, nl.col_count, nl.col_start, nl.col_fill, nl.keys, nl.col_ncl, nl.catoms, nl.slot_of,
                     nl.ccol, nl.cref, nl.cmin, nl.cmax, nl.ljs, nl.comps, nl.cw, nl.xb[0], nl.xb[1], nl.xq, nl.ecount, nl.eoff, nl.flags,
-                    nl.ent, nl.ecode, e.fs, e.es};
+                    nl.ent, nl.ecode, e.fs, e.es, e.fb, e.eb};
     for (void* p : ptrs) if (p) (void)hipFree(p);
 }
 
@@ -661,6 +663,11 @@ void nlist_build(Engine& e, const uint4* w) {
     hipLaunchKernelGGL(k_fill_int, dim3(nblk((int64_t)ncl * CLUSTER, 256)), dim3(256), 0, s, nl.catoms, ncl * CLUSTER, -1);
     hipLaunchKernelGGL(k_assign_slots, dim3(nblk(N, 256)), dim3(256), 0, s, N, nl.ncol, nl.col_start, col_cl_start, nl.keys,
                        nl.catoms, nl.slot_of);
+    {   // bonded slot forces: slots of atoms without bonded terms stay zero until the next rebuild
+        const int64_t ns = (int64_t)nl.ncl_max * CLUSTER;
+        hipLaunchKernelGGL(k_fill_int, dim3(nblk(3 * ns, 256)), dim3(256), 0, s, (int*)e.fb, (int)(3 * ns), 0);
+        hipLaunchKernelGGL(k_fill_int, dim3(nblk(ns, 256)), dim3(256), 0, s, (int*)e.eb, (int)ns, 0);
+    }
     hipLaunchKernelGGL(k_cluster_props, dim3(nblk((int64_t)ncl * CLUSTER, 256)), dim3(256), 0, s, ncl, nl.ncol, col_cl_start,
                        nl.catoms, w, e.p.prm, e.p.comp, nl.ccol, nl.cref, nl.cmin, nl.cmax, nl.ljs, nl.comps, nl.cw);
     NlArgs a{};
