@@ -127,6 +127,7 @@ struct Engine {
     float Lf[3]{};
     float rc = 1.2f, rc2 = 1.44f, skin = 0.1f;
     PairConst pc{};
+    float4* coul_tab = nullptr;   // [COUL_TAB] cubic Hermite coefficients of the Coulomb force factor
     double dt_ns = 2e-6;
     bool elec = true;
 

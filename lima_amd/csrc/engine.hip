@@ -470,6 +470,19 @@ void create(Engine& e, const lima_box_desc* b, const lima_simparams* p) {
     e.pc.kc = (float)(8.9875517873681764e9 / 1e-9 / 6.02214076e23 * 1000.f * 1000.f);   // PhysicsUtils.cuh:5
     e.pc.kappa = 3.f / e.rc;                                                             // PhysicsUtils.cuh:60-62
     e.pc.two_kappa_sqrtpi = 2.f * e.pc.kappa / sqrtf(LIMA_PI_F);
+    {   // H(r) = erfc(kappa r) + c r exp(-kappa^2 r^2), c = 2 kappa / sqrt(PI_L), and
+        // H'(r) = (c (1 - 2 kappa^2 r^2) - 2 kappa / sqrt(pi)) exp(-kappa^2 r^2), in double
+        const double kap = (double)e.pc.kappa, cc = (double)e.pc.two_kappa_sqrtpi, h = (double)e.rc / COUL_TAB;
+        auto H = [&](double r) { return std::erfc(kap * r) + cc * r * std::exp(-kap * kap * r * r); };
+        auto dH = [&](double r) { return (cc * (1.0 - 2.0 * kap * kap * r * r) - 2.0 * kap / std::sqrt(M_PI)) * std::exp(-kap * kap * r * r); };
+        std::vector<float4> tab(COUL_TAB);
+        for (int i = 0; i < COUL_TAB; i++) {
+            const double r0 = i * h, r1 = (i + 1) * h;
+            const double h0 = H(r0), h1 = H(r1), d0 = dH(r0) * h, d1 = dH(r1) * h;
+            tab[i] = make_float4((float)h0, (float)d0, (float)(3.0 * (h1 - h0) - 2.0 * d0 - d1), (float)(2.0 * (h0 - h1) + d0 + d1));
+        }
+        e.coul_tab = dalloc_copy(tab);
+    }
     e.dt_ns = (double)(p->dt_fs * 1e-6f);
     for (int d = 0; d < 3; d++) {
         e.Lq[d] = llround(b->box_nm[d] * 1e8);
@@ -709,7 +722,7 @@ void destroy(Engine& e) {
     pme_free(e);
     void* ptrs[] = {e.p.w, e.p.img, e.p.v, e.p.fprev, e.p.prm, e.p.comp, e.w_build, e.img_build, e.dmax2, e.thermo, e.dpart, e.epart, e.w_try, e.img_try, e.f_acc, e.sd, e.first,
                     e.bmask, e.intra_items, e.intra_lut, e.bd.bid, e.bd.bp, e.bd.aid, e.bd.ap, e.bd.did, e.bd.dp,
-                    e.bd.iid, e.bd.ip, e.bd.pid, e.bd.pp, e.bd.lane_code, e.bd.lane_atom, e.acc, e.fmax};
+                    e.bd.iid, e.bd.ip, e.bd.pid, e.bd.pp, e.bd.lane_code, e.bd.lane_atom, e.coul_tab, e.acc, e.fmax};
     for (void* p : ptrs) if (p) (void)hipFree(p);
     for (auto& ev : e.tev) if (ev) (void)hipEventDestroy(ev);
     if (e.step_graph) (void)hipGraphExecDestroy(e.step_graph);

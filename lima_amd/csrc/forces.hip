@@ -62,7 +62,9 @@ struct PairArgs {
     int ncl, nslots, cap;
     int split;           // waves per i-cluster (each takes a contiguous share of its entries)
     float rc2;
+    float tscale;        // COUL_TAB / cutoff
     PairConst pc;
+    const float4* coul_tab;
     const float4* xq;
     const float2* ljs;
     const int* ecount;
@@ -89,6 +91,9 @@ template <bool ENERGY>
 __global__ __launch_bounds__(256) void k_pair(PairArgs a) {
     __shared__ float4 sA[PAIR_WAVES][2][CHUNK];
     __shared__ float4 sB[PAIR_WAVES][2][CHUNK];
+    __shared__ float4 tab[COUL_TAB];
+    for (int i = threadIdx.x; i < COUL_TAB; i += blockDim.x) tab[i] = a.coul_tab[i];
+    __syncthreads();
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int wv = xcd_block(blockIdx.x, gridDim.x) * PAIR_WAVES + wid;
     const int I = wv / a.split, part = wv - I * a.split;
@@ -144,7 +149,8 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs a) {
             if (any == 0ull) continue;
             if (ENERGY) npairs += __popcll(any);
             float el = 0.f, ec = 0.f;
-            float S = pair_scalar_cut(inc ? r2 : 1.0f, pi.x + B.x, pi.y * B.y, qi_kc * A.w, a.pc, ENERGY ? &el : nullptr, &ec);
+            float S = pair_scalar_tab(inc ? r2 : 1.0f, pi.x + B.x, pi.y * B.y, qi_kc * A.w, a.pc, tab, a.tscale,
+                                      ENERGY ? &el : nullptr, &ec);
             S = inc ? S : 0.f;
             fx += dx * S; fy += dy * S; fz += dz * S;
             if (ENERGY && inc) {
@@ -482,7 +488,9 @@ void launch_pair(Engine& e, bool energy, hipStream_t s) {
     a.ncl = e.nl.ncl;
     a.nslots = e.nl.ncl * CLUSTER;
     a.rc2 = e.rc2;
+    a.tscale = (float)COUL_TAB / e.rc;
     a.pc = e.pc;
+    a.coul_tab = e.coul_tab;
     a.cap = e.nl.cap;
     a.xq = e.nl.xq; a.ljs = e.nl.ljs; a.ecount = e.nl.ecount; a.ent = e.nl.ent;
     a.fs = e.fs; a.es = e.es; a.epart = e.epart + (int64_t)e.ereg.pair * ACC_N;

@@ -106,4 +106,35 @@ __device__ __forceinline__ float pair_scalar_cut(float r2, float sig, float eps2
     return flj - fc;
 }
 
+// Coulomb force factor H(r) = erfc(kappa r) + (2 kappa / sqrt(PI_L)) r exp(-kappa^2 r^2) of the
+// pair force above (force on self = diff * kC qq H(r) / r^3), tabulated on [0, cutoff] in
+// COUL_TAB intervals as cubic Hermite polynomials in the local coordinate t in [0, 1)
+// (c0 + c1 t + c2 t^2 + c3 t^3; values and derivatives of H in double at the nodes).  The
+// interpolation error is below 1e-9 absolute (h^4 H''''/384, h = cutoff / 256).
+constexpr int COUL_TAB = 256;
+
+// pair_scalar_cut with the Coulomb force factor read from the LDS table (tab, tscale = COUL_TAB /
+// cutoff); the energies, when asked for, are evaluated as in pair_scalar_cut.
+__device__ __forceinline__ float pair_scalar_tab(float r2, float sig, float eps24, float qq_kc, const PairConst& c,
+                                                 const float4* tab, float tscale, float* e_lj, float* e_c) {
+    const float rinv = __builtin_amdgcn_rsqf(r2);
+    const float rinv2 = rinv * rinv;
+    float s = sig * sig * rinv2;
+    s = s * s * s;
+    const float flj = eps24 * s * rinv2 * (1.0f - 2.0f * s);
+    const float x = (r2 * rinv) * tscale;
+    const int i = min((int)x, COUL_TAB - 1);
+    const float t = x - (float)i;
+    const float4 k = tab[i];
+    const float Hr = fmaf(fmaf(fmaf(k.w, t, k.z), t, k.y), t, k.x);
+    const float fc = (qq_kc * rinv2) * (rinv * Hr);
+    if (e_lj) {
+        *e_lj = (eps24 * (1.0f / 6.0f)) * s * (s - 1.0f);
+        float el, ecl;
+        pair_scalar_cut(r2, sig, eps24, qq_kc, c, &el, &ecl);
+        *e_c = ecl;
+    }
+    return flj - fc;
+}
+
 }  // namespace lima
