@@ -254,8 +254,8 @@ def main():
                     help="only run the copy kernel used to calibrate the HBM byte counters")
     ap.add_argument("--no-em", action="store_true", help="skip the batched-minimisation measurement")
     ap.add_argument("--launch-mode", type=int, default=0,
-                    help="lima_simparams.launch_mode: 0 eager launches with PME on a second stream; "
-                         "+1 step graph replay (experimental), +2 PME in line on one stream")
+                    help="lima_simparams.launch_mode: 0 step graph replay with PME on a second stream; "
+                         "+1 eager launches, +2 PME in line on one stream")
     ap.add_argument("--em-systems", type=int, default=256)
     ap.add_argument("--em-train-systems", type=int, default=100)
     ap.add_argument("--em-max-iter", type=int, default=20000)
@@ -387,7 +387,7 @@ def main():
         "config": {"workload": workload, "n_atoms": box.n_particles, "box_nm": [float(x) for x in box.box_nm],
                    "chains": "128 x 50", "waters": (box.n_particles - 6400) // 3, "dt_fs": dt_fs, "cutoff_nm": 1.2,
                    "pme_spacing_nm": 0.12, "nlist_interval": 10, "skin_nm": 0.1, "energy_reductions": "fp64",
-                   "launch": ("step graph" if args.launch_mode & 1 else "eager") +
+                   "launch": ("eager" if args.launch_mode & 1 else "step graph") +
                              (", one stream" if args.launch_mode & 2 else ", PME on a second stream")},
         "roofline": roofline,
         "cpu_baseline": cpu,

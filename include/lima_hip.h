@@ -114,13 +114,14 @@ typedef struct lima_simparams {
     float   nlist_skin_nm;            /* Verlet buffer, default 0.1 */
     int32_t nlist_interval;           /* steps between list rebuilds, default 10 */
     int32_t energy_interval;          /* steps between FP64 energy reductions (0 = never) */
-    int32_t launch_mode;              /* 0 = kernels launched one by one, PME on a second stream; or-ed
-                                         flags LIMA_LAUNCH_GRAPH (each MD step replayed as a HIP graph,
-                                         re-captured after every list rebuild; experimental, off by
-                                         default) and LIMA_LAUNCH_ONE_STREAM (PME in line) */
+    int32_t launch_mode;              /* 0 = each MD step replayed as a HIP graph (captured again after
+                                         every list rebuild), PME on a second stream; or-ed flags
+                                         LIMA_LAUNCH_EAGER (launch the kernels one by one; the
+                                         integrator then also packs the next step's slots) and
+                                         LIMA_LAUNCH_ONE_STREAM (PME in line) */
 } lima_simparams;
 
-#define LIMA_LAUNCH_GRAPH 1
+#define LIMA_LAUNCH_EAGER 1
 #define LIMA_LAUNCH_ONE_STREAM 2
 
 /* FP64 energy totals [J/mol]. */

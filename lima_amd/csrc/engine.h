@@ -10,6 +10,7 @@
 
 #include "../../include/lima_hip.h"
 #include "lima_dev.h"
+#include "lima_terms.h"
 
 namespace lima {
 
@@ -98,7 +99,7 @@ struct Bonded {
 constexpr uint32_t LANE_PAD = 0xffffffffu;
 
 // Device accumulators (FP64 energy reductions and scalars).
-constexpr int LAUNCH_GRAPH = LIMA_LAUNCH_GRAPH, LAUNCH_ONE_STREAM = LIMA_LAUNCH_ONE_STREAM;
+constexpr int LAUNCH_EAGER = LIMA_LAUNCH_EAGER, LAUNCH_ONE_STREAM = LIMA_LAUNCH_ONE_STREAM;
 
 enum Acc { ACC_LJ = 0, ACC_COUL, ACC_BOND, ACC_PME, ACC_KIN, ACC_PAIRS, ACC_N };
 
@@ -184,6 +185,7 @@ struct Engine {
     int n_rebuilds = 0;
     lima_energy energy{};
     bool forces_valid = false;
+    bool packed = false;       // the integrator already packed the slots for the next evaluation
     bool critical = false;
 
     // timing: per step 7 events (before pack, after pack, pair, intra, bonded, pme, integrate),
@@ -222,6 +224,7 @@ int pair_split(const Engine& e);
 ERegions energy_regions(const Engine& e);
 void launch_reduce_energy(Engine& e, bool kinetic, hipStream_t s);
 void launch_pack(Engine& e, const uint4* w, hipStream_t s);
+BinArgs bin_args(const Engine& e);   // PME tile bins (bin_count == nullptr when PME is not tiled)
 void launch_pair(Engine& e, bool energy, hipStream_t s);
 void launch_intra(Engine& e, const uint4* w, const int4* img, bool energy, hipStream_t s);
 void launch_bonded(Engine& e, const uint4* w, const int4* img, bool energy, hipStream_t s);

@@ -45,13 +45,20 @@ struct IntegrateArgs {
     float* dpart;          // per-wave running max of the squared displacement since the list build
     double* epart;         // kinetic-energy region: one record per wave
     const int* slot_of;
-    const float* fs;
+    float* fs;
     const float* fb;
     uint4* w;
     int4* img;
     float4* v;
     float4* fprev;
     const float4* prm;
+    // pack != 0: also prepare the particle's slot for the next evaluation, as k_pack does (relative
+    // position and charge, accumulators cleared, PME tile bin); the list must stay unchanged
+    int pack;
+    const uint4* cref;
+    float4* xq;
+    float* es;
+    BinArgs bins;
 };
 
 // w + d wrapped back into [0, Lq) for any d (a step may exceed a box edge when a run explodes).
@@ -75,6 +82,8 @@ template <bool ENERGY>
 __global__ void k_integrate(IntegrateArgs a) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     float ke = 0.f, d2 = 0.f;
+    uint4 bin_w = make_uint4(0u, 0u, 0u, 0u);
+    bool bin_on = false;   // pack: this particle goes to a PME tile bin
     if (p < a.N) {
         const int s = a.slot_of[p];
         const float fx = a.fs[s] + a.fb[s], fy = a.fs[a.nslots + s] + a.fb[a.nslots + s], fz = a.fs[2 * a.nslots + s] + a.fb[2 * a.nslots + s];
@@ -93,6 +102,23 @@ __global__ void k_integrate(IntegrateArgs a) {
         wrap_add(w.z, im.z, coord_of(v.z * a.dt + fz * c2), a.Lq[2]);
         a.w[p] = w;
         a.img[p] = im;
+        if (a.pack) {
+            const uint4 r = a.cref[s >> 3];
+            long long d[3] = {(long long)w.x - (long long)r.x, (long long)w.y - (long long)r.y, (long long)w.z - (long long)r.z};
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                if (2 * d[k] > a.Lq[k]) d[k] -= a.Lq[k];
+                else if (2 * d[k] < -a.Lq[k]) d[k] += a.Lq[k];
+            }
+            const float q = a.prm[p].x;
+            a.xq[s] = make_float4((float)d[0] * LIMA_LIMA_TO_NANO_F, (float)d[1] * LIMA_LIMA_TO_NANO_F, (float)d[2] * LIMA_LIMA_TO_NANO_F, q);
+            a.fs[s] = 0.f;
+            a.fs[a.nslots + s] = 0.f;
+            a.fs[2 * a.nslots + s] = 0.f;
+            a.es[s] = 0.f;
+            bin_w = w;
+            bin_on = q != 0.f;
+        }
         {   // displacement since the list was built (unwrapped, exact integer difference)
             const uint4 wb = a.w_build[p];
             const int4 ib = a.img_build[p];
@@ -107,6 +133,7 @@ __global__ void k_integrate(IntegrateArgs a) {
         a.v[p] = v;
         a.fprev[p] = make_float4(fx, fy, fz, 0.f);
     }
+    if (a.pack && a.bins.bin_count) bin_wave(a.bins, bin_on, (int)p, bin_w);   // all lanes, converged
     d2 = wave_max(d2);
     const int64_t wv = p >> 6;
     const bool wave_in = (p & ~63ll) < a.N;   // waves inside the per-wave arrays
@@ -290,7 +317,8 @@ void evaluate(Engine& e, const uint4* w, const int4* img, bool energy, bool time
     hipStream_t s = e.s_main;
     hipEvent_t* ev = timed ? tslot(e) : nullptr;
     if (ev) HIP_CHECK(hipEventRecord(ev[0], s));
-    launch_pack(e, w, s);
+    if (!(e.packed && w == e.p.w)) launch_pack(e, w, s);
+    e.packed = false;
     if (ev) HIP_CHECK(hipEventRecord(ev[1], s));
     const bool split = !timed && !(e.prm.launch_mode & LAUNCH_ONE_STREAM) && e.elec && e.pme.ncharged > 0;
     if (split) {
@@ -338,7 +366,9 @@ void check_finite(Engine& e) {
     }
 }
 
-void integrate(Engine& e, bool energy) {
+// pack: the next evaluation uses the same list, so the integrator packs the slots for it and
+// the evaluation skips k_pack (one launch fewer per step).
+void integrate(Engine& e, bool energy, bool pack = false) {
     IntegrateArgs a{};
     a.N = e.N;
     a.nslots = e.nl.ncl * CLUSTER;
@@ -348,6 +378,10 @@ void integrate(Engine& e, bool energy) {
     a.slot_of = e.nl.slot_of; a.fs = e.fs; a.fb = e.fb; a.w = e.p.w; a.img = e.p.img; a.v = e.p.v; a.fprev = e.p.fprev;
     a.prm = e.p.prm; a.epart = e.epart + (int64_t)e.ereg.kin * ACC_N;
     a.w_build = e.w_build; a.img_build = e.img_build; a.dpart = e.dpart;
+    a.pack = pack ? 1 : 0;
+    a.cref = e.nl.cref; a.xq = e.nl.xq; a.es = e.es;
+    if (pack) a.bins = bin_args(e);
+    e.packed = pack;
     if (energy) hipLaunchKernelGGL(k_integrate<true>, dim3(nblk(e.N, 256)), dim3(256), 0, e.s_main, a);
     else hipLaunchKernelGGL(k_integrate<false>, dim3(nblk(e.N, 256)), dim3(256), 0, e.s_main, a);
 }
@@ -422,12 +456,16 @@ void md_steps(Engine& e, int64_t n) {
             }
         }
         const bool energy = (e.prm.energy_interval > 0 && e.step % e.prm.energy_interval == 0) || k == n - 1;
-        if (!energy && !e.timing && (e.prm.launch_mode & LAUNCH_GRAPH)) {
+        if (!energy && !e.timing && !(e.prm.launch_mode & LAUNCH_EAGER)) {
             if (!e.step_graph_valid) capture_step_graph(e);
             HIP_CHECK(hipGraphLaunch(e.step_graph, e.s_main));
         } else {
             evaluate(e, e.p.w, e.p.img, energy, e.timing);
-            integrate(e, energy);
+            // the integrator packs for the next step when that step keeps the list; the last step
+            // leaves the force slots intact for lima_get_state
+            const bool pack_next = !e.timing && (e.prm.launch_mode & LAUNCH_EAGER) && k + 1 < n &&
+                                   e.steps_since_build + 1 < interval;
+            integrate(e, energy, pack_next);
         }
         if (energy) launch_reduce_energy(e, true, e.s_main);
         if (e.timing) {
@@ -454,8 +492,8 @@ void create(Engine& e, const lima_box_desc* b, const lima_simparams* p) {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw LimaFail(LIMA_ERR_NO_DEVICE, "no HIP device visible");
     e.prm = *p;
-    if (p->launch_mode < 0 || p->launch_mode > (LAUNCH_GRAPH | LAUNCH_ONE_STREAM))
-        throw LimaFail(LIMA_ERR_BAD_INPUT, "launch_mode must be a combination of LIMA_LAUNCH_GRAPH and LIMA_LAUNCH_ONE_STREAM");
+    if (p->launch_mode < 0 || p->launch_mode > (LAUNCH_EAGER | LAUNCH_ONE_STREAM))
+        throw LimaFail(LIMA_ERR_BAD_INPUT, "launch_mode must be a combination of LIMA_LAUNCH_EAGER and LIMA_LAUNCH_ONE_STREAM");
     if (p->apply_thermostat && p->steps_per_temperature_measurement <= 0)
         throw LimaFail(LIMA_ERR_BAD_INPUT, "the thermostat needs steps_per_temperature_measurement > 0");
     if (p->bc_select != LIMA_BC_PBC) throw LimaFail(LIMA_ERR_UNSUPPORTED, "only periodic boundary conditions are implemented");

@@ -33,29 +33,36 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
     return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
 }
 
+// Slot k of the cluster list: position relative to the cluster reference point and charge,
+// force / energy accumulators cleared, and (tiled PME) the charged particle put in its tile bin.
 __global__ void k_pack(int nslots, const int* __restrict__ catoms, const uint4* __restrict__ w, const float4* __restrict__ prm,
                        const uint4* __restrict__ cref, long long Lqx, long long Lqy, long long Lqz,
-                       float4* __restrict__ xq, float* __restrict__ fs, float* __restrict__ es) {
+                       float4* __restrict__ xq, float* __restrict__ fs, float* __restrict__ es, BinArgs bins) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nslots) return;
-    const int a = catoms[k];
-    float4 out = make_float4(PAD_POS, PAD_POS, PAD_POS, 0.f);
-    if (a >= 0) {
-        const uint4 p = w[a];
-        const uint4 r = cref[k >> 3];
-        long long d[3] = {(long long)p.x - (long long)r.x, (long long)p.y - (long long)r.y, (long long)p.z - (long long)r.z};
-        const long long L[3] = {Lqx, Lqy, Lqz};
-        for (int i = 0; i < 3; i++) {
-            if (2 * d[i] > L[i]) d[i] -= L[i];
-            else if (2 * d[i] < -L[i]) d[i] += L[i];
+    const int a = k < nslots ? catoms[k] : -1;
+    uint4 p = make_uint4(0u, 0u, 0u, 0u);
+    float q = 0.f;
+    if (k < nslots) {
+        float4 out = make_float4(PAD_POS, PAD_POS, PAD_POS, 0.f);
+        if (a >= 0) {
+            p = w[a];
+            const uint4 r = cref[k >> 3];
+            long long d[3] = {(long long)p.x - (long long)r.x, (long long)p.y - (long long)r.y, (long long)p.z - (long long)r.z};
+            const long long L[3] = {Lqx, Lqy, Lqz};
+            for (int i = 0; i < 3; i++) {
+                if (2 * d[i] > L[i]) d[i] -= L[i];
+                else if (2 * d[i] < -L[i]) d[i] += L[i];
+            }
+            q = prm[a].x;
+            out = make_float4((float)d[0] * LIMA_LIMA_TO_NANO_F, (float)d[1] * LIMA_LIMA_TO_NANO_F, (float)d[2] * LIMA_LIMA_TO_NANO_F, q);
         }
-        out = make_float4((float)d[0] * LIMA_LIMA_TO_NANO_F, (float)d[1] * LIMA_LIMA_TO_NANO_F, (float)d[2] * LIMA_LIMA_TO_NANO_F, prm[a].x);
+        xq[k] = out;
+        fs[k] = 0.f;
+        fs[nslots + k] = 0.f;
+        fs[2 * nslots + k] = 0.f;
+        es[k] = 0.f;
     }
-    xq[k] = out;
-    fs[k] = 0.f;
-    fs[nslots + k] = 0.f;
-    fs[2 * nslots + k] = 0.f;
-    es[k] = 0.f;
+    if (bins.bin_count) bin_wave(bins, a >= 0 && q != 0.f, a, p);
 }
 
 struct PairArgs {
@@ -480,7 +487,7 @@ void launch_reduce_energy(Engine& e, bool kinetic, hipStream_t s) {
 void launch_pack(Engine& e, const uint4* w, hipStream_t s) {
     const int nslots = e.nl.ncl * CLUSTER;
     hipLaunchKernelGGL(k_pack, dim3(nblk(nslots, 256)), dim3(256), 0, s, nslots, e.nl.catoms, w, e.p.prm, e.nl.cref,
-                       (long long)e.Lq[0], (long long)e.Lq[1], (long long)e.Lq[2], e.nl.xq, e.fs, e.es);
+                       (long long)e.Lq[0], (long long)e.Lq[1], (long long)e.Lq[2], e.nl.xq, e.fs, e.es, bin_args(e));
 }
 
 void launch_pair(Engine& e, bool energy, hipStream_t s) {

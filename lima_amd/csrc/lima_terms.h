@@ -132,6 +132,59 @@ __device__ __forceinline__ void grid_coord(uint32_t w, long long Lq, int K, int*
     *f = (float)r / (float)Lq;
 }
 
+// PME tile bins of charged particles (tiled spreading).  A particle goes to the fixed-capacity
+// bin of the tile holding its base node; the bin order does not matter (the tile sums are
+// integer).  The counters start at zero each evaluation: the tiled gather clears them after use.
+// bin_count == nullptr: no tiled PME, nothing to bin.
+struct BinArgs {
+    long long Lq[3];
+    int K[3], T[3], nt[3];
+    int bincap;
+    int* bin_count;
+    int* bin_atoms;
+    int* flags;        // bit 0: a bin overflowed
+};
+
+// Called by all 64 lanes of a wave (converged); lanes with active set bin particle a at pw.
+// Neighbouring lanes mostly share a tile (slots follow the cluster order): ballots group the
+// lanes by tile (leader = lowest lane of the group, rank = lanes of the group below), then every
+// leader reserves its group's range with one atomic, all in the same instruction (one round trip
+// per wave), and each lane takes its base from its leader.
+__device__ __forceinline__ void bin_wave(const BinArgs& b, bool active, int a, uint4 pw) {
+    int tile = -1;
+    if (active) {
+        int ix[3];
+        float f;
+        grid_coord(pw.x, b.Lq[0], b.K[0], &ix[0], &f);
+        grid_coord(pw.y, b.Lq[1], b.K[1], &ix[1], &f);
+        grid_coord(pw.z, b.Lq[2], b.K[2], &ix[2], &f);
+        tile = ((ix[2] / b.T[2]) * b.nt[1] + ix[1] / b.T[1]) * b.nt[0] + ix[0] / b.T[0];
+    }
+    const int lane = threadIdx.x & 63;
+    int leader = lane, rank = 0, cnt = 0;
+    unsigned long long pending = __ballot(active);
+    while (pending) {   // wave-uniform: one pass per distinct tile, registers only
+        const int L = __builtin_ctzll(pending);
+        const int t = __shfl(tile, L, 64);
+        const bool mine = active && tile == t;
+        const unsigned long long same = __ballot(mine);
+        if (mine) {
+            leader = L;
+            rank = __popcll(same & ((1ull << lane) - 1ull));
+        }
+        if (lane == L) cnt = __popcll(same);
+        pending &= ~same;
+    }
+    int base = 0;
+    if (active && lane == leader) base = atomicAdd(&b.bin_count[tile], cnt);
+    base = __shfl(base, leader, 64);
+    if (active) {
+        const int pos = base + rank;
+        if (pos < b.bincap) b.bin_atoms[(int64_t)tile * b.bincap + pos] = a;
+        else atomicOr(b.flags, 1);
+    }
+}
+
 // Cubic B-spline weights at nodes ix-1 .. ix+2 (DeviceAlgorithmsPrivate.cuh:9-12).
 __device__ __forceinline__ void bspline(float f, float* w) {
     const float g = 1.f - f;

@@ -631,6 +631,10 @@ void nlist_build(Engine& e, const uint4* w) {
     NList& nl = e.nl;
     hipStream_t s = e.s_main;
     const int64_t N = e.N;
+    // cluster count, slots, list buffers and energy regions may change: a captured step graph holds
+    // the old kernel arguments (possibly freed buffers) and the integrator's packing is stale
+    e.step_graph_valid = false;
+    e.packed = false;
     {   // a list is stale once two particles may have closed the skin: 2 max|dx| > skin
         float d2 = 0.f;
         hipLaunchKernelGGL(k_max_reduce, dim3(1), dim3(256), 0, s, e.dpart, (int)((N + 63) / 64), e.dmax2);

@@ -177,51 +177,6 @@ struct TileArgs {
     int* flags;
 };
 
-constexpr int BIN_ITEMS = 1;          // particles per thread in the binning kernel
-constexpr int BIN_LDS_MAX = 16384;    // tiles counted in LDS; more tiles use global atomics directly
-
-// Charged particles -> fixed-capacity bin of the tile holding their current base node.  The order
-// inside a bin does not matter: the block sums are integer.  Each block ranks its particles per
-// tile in LDS and reserves one range per (block, tile) with a single global atomic.  The tile
-// counters start at zero: the tiled gather clears each counter after its last use.
-__global__ __launch_bounds__(256) void k_bin(TileArgs a) {
-    extern __shared__ int hist[];
-    const int ntiles = a.nt[0] * a.nt[1] * a.nt[2];
-    const bool lds = ntiles <= BIN_LDS_MAX;
-    if (lds) for (int t = threadIdx.x; t < ntiles; t += blockDim.x) hist[t] = 0;
-    __syncthreads();
-    const int64_t p0 = (int64_t)blockIdx.x * blockDim.x * BIN_ITEMS;
-    int tile[BIN_ITEMS], rank[BIN_ITEMS];
-#pragma unroll
-    for (int k = 0; k < BIN_ITEMS; k++) {
-        const int64_t p = p0 + (int64_t)k * blockDim.x + threadIdx.x;
-        tile[k] = -1;
-        rank[k] = 0;
-        if (p < a.N && a.prm[p].x != 0.f) {
-            const uint4 pw = a.w[p];
-            int ix[3];
-            float f;
-            grid_coord(pw.x, a.Lq[0], a.K[0], &ix[0], &f);
-            grid_coord(pw.y, a.Lq[1], a.K[1], &ix[1], &f);
-            grid_coord(pw.z, a.Lq[2], a.K[2], &ix[2], &f);
-            tile[k] = ((ix[2] / a.T[2]) * a.nt[1] + ix[1] / a.T[1]) * a.nt[0] + ix[0] / a.T[0];
-            rank[k] = lds ? atomicAdd(&hist[tile[k]], 1) : atomicAdd(&a.bin_count[tile[k]], 1);
-        }
-    }
-    __syncthreads();
-    if (lds)
-        for (int t = threadIdx.x; t < ntiles; t += blockDim.x)
-            if (hist[t]) hist[t] = atomicAdd(&a.bin_count[t], hist[t]);
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < BIN_ITEMS; k++) {
-        if (tile[k] < 0) continue;
-        const int pos = (lds ? hist[tile[k]] : 0) + rank[k];
-        if (pos < a.bincap) a.bin_atoms[(int64_t)tile[k] * a.bincap + pos] = (int)(p0 + (int64_t)k * blockDim.x + threadIdx.x);
-        else atomicOr(a.flags, 1);
-    }
-}
-
 // One block per tile: particles of the bin add q w_x w_y w_z (fixed point) into an E^3 LDS block
 // whose origin is the tile corner minus one node (E = T + 3); the block is stored as is.
 __global__ __launch_bounds__(256) void k_spread_tiles(TileArgs a) {
@@ -633,6 +588,18 @@ TileArgs tile_args(Engine& e, const uint4* w) {
 
 }  // namespace
 
+BinArgs bin_args(const Engine& e) {
+    const Pme& m = e.pme;
+    BinArgs b{};
+    if (!m.tiled || !e.elec || m.ncharged == 0) return b;
+    for (int d = 0; d < 3; d++) { b.Lq[d] = e.Lq[d]; b.K[d] = m.K[d]; b.T[d] = m.T[d]; b.nt[d] = m.nt[d]; }
+    b.bincap = m.bincap;
+    b.bin_count = m.bin_count;
+    b.bin_atoms = m.bin_atoms;
+    b.flags = m.flags;
+    return b;
+}
+
 void pme_bin(Engine& e, const uint4* w, hipStream_t s) {   // checked at every list rebuild
     (void)w;
     Pme& m = e.pme;
@@ -765,9 +732,7 @@ void launch_pme(Engine& e, const uint4* w, bool energy, hipStream_t s) {
     // LDS tiles stay <= 48 KB: 2 buffers x lines x length x 8 B
     const int TY = std::max(1, std::min(8, 3072 / Kx));   // lines per workgroup: a few hundred workgroups per pass
     if (m.tiled) {
-        TileArgs ta = tile_args(e, w);
-        hipLaunchKernelGGL(k_bin, dim3(nblk(e.N, 256 * BIN_ITEMS)), dim3(256),
-                           m.ntiles <= BIN_LDS_MAX ? (size_t)m.ntiles * sizeof(int) : 0, s, ta);
+        TileArgs ta = tile_args(e, w);   // bins filled by k_pack (or the integrator) from the same positions
         const int E3 = m.E[0] * m.E[1] * m.E[2];
         hipLaunchKernelGGL(k_spread_tiles, dim3(m.ntiles), dim3(256), (size_t)E3 * sizeof(long long), s, ta);
         hipLaunchKernelGGL(k_fft_x_fwd_tiled, dim3(nblk(Ky, TY), Kz), dim3(256), 2 * TY * Kx * sizeof(float2), s, Px, Ky, H,

@@ -159,16 +159,15 @@ def test_thermostat_trajectory_matches_oracle():
     assert err < 2e-4
 
 
-@pytest.mark.skip(reason="graph replay (LIMA_LAUNCH_GRAPH) is switched off: a bench run with it on faulted the GPU once "
-                         "and no cause has been found in the code yet; it stays off the GPU until one is")
 def test_graph_replay_matches_eager_launches():
     """The MD step replayed as a HIP graph (re-captured at each list rebuild) gives the trajectory
-    of the eager launches.  Float atomics make the force sums order dependent in the last bits,
+    of the eager launches.  The run crosses three list rebuilds, after each of which the graph must
+    be captured again (its kernel arguments name the list buffers and cluster counts).  Float atomics make the force sums order dependent in the last bits,
     so the comparison allows a few Coord units (1e-8 nm) of difference."""
     box = L.gen_em_system(1200, 1003, 0.0)
     runs = []
     for mode in (0, 1):
-        eng = L.Engine(box, L.default_params(dt_fs=1.0, launch_mode=1 - mode))
+        eng = L.Engine(box, L.default_params(dt_fs=1.0, launch_mode=mode))
         eng.step(35)                             # crosses three list rebuilds
         st = eng.state()
         runs.append((unwrapped_positions(box, st), _velocities(box, st), eng.energy().potential, eng.status()))
