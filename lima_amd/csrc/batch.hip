@@ -231,7 +231,8 @@ __device__ __forceinline__ void intra_atom(const BatchArgs& a, const SysMeta& m,
             const float r2 = dot(d, d);
             const float4 pj = a.prm[gj];
             float e1, e2;
-            const float S = pair_scalar(r2, pi.y + pj.y, pi.z * pj.z, pi.x * pj.x, a.pc, &e1, &e2);
+            // no cutoff: pair_scalar_cut's erfc polynomial stays within 1e-12 absolute beyond kappa r = 3.25
+            const float S = pair_scalar_cut(r2, pi.y + pj.y, 24.0f * (pi.z * pj.z), a.pc.kc * (pi.x * pj.x), a.pc, &e1, &e2);
             acc = acc + d * S;
             elj += 0.5f * e1;
             ec += 0.5f * e2;

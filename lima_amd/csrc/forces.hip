@@ -280,7 +280,10 @@ __device__ __forceinline__ void intra_items(const IntraArgs& a, int blk, IntraLd
             if (!(r2 > 0.f)) continue;
             const float4 pj = L.p[wid][j];
             float el = 0.f, ec = 0.f;
-            const float S = pair_scalar(r2, pi.y + pj.y, pi.z * pj.z, pi.x * pj.x, a.pc, ENERGY ? &el : nullptr, &ec);
+            // no cutoff here: pair_scalar_cut's erfc polynomial extrapolates beyond kappa r = 3.25 with an
+            // absolute error below 1e-12 (erfc itself is < 2e-6 there)
+            const float S = pair_scalar_cut(r2, pi.y + pj.y, 24.0f * (pi.z * pj.z), a.pc.kc * (pi.x * pj.x), a.pc,
+                                            ENERGY ? &el : nullptr, &ec);
             fx += dx * S; fy += dy * S; fz += dz * S;
             if (ENERGY) { el_s += 0.5f * el; ec_s += 0.5f * ec; }
         }
