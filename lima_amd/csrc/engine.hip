@@ -313,11 +313,13 @@ void timing_flush(Engine& e) {
 // after the pack kernel (which zeroes the force slots) and a join before the caller's next
 // kernel, overlapping the pair / intra / bonded kernels (LIMA overlaps PME the same way,
 // Engine.cu:333).  With timing on, everything runs on one stream so each phase is timed alone.
-void evaluate(Engine& e, const uint4* w, const int4* img, bool energy, bool timed = false) {
+// pack = false: the slots were already packed for these positions (by the integrator of the
+// previous step, or by an explicit k_pack before a replayed step graph).
+void evaluate(Engine& e, const uint4* w, const int4* img, bool energy, bool timed = false, bool pack = true) {
     hipStream_t s = e.s_main;
     hipEvent_t* ev = timed ? tslot(e) : nullptr;
     if (ev) HIP_CHECK(hipEventRecord(ev[0], s));
-    if (!(e.packed && w == e.p.w)) launch_pack(e, w, s);
+    if (pack) launch_pack(e, w, s);
     e.packed = false;
     if (ev) HIP_CHECK(hipEventRecord(ev[1], s));
     const bool split = !timed && !(e.prm.launch_mode & LAUNCH_ONE_STREAM) && e.elec && e.pme.ncharged > 0;
@@ -412,8 +414,8 @@ void capture_step_graph(Engine& e) {
     hipGraph_t g = nullptr;
     HIP_CHECK(hipStreamBeginCapture(e.s_main, hipStreamCaptureModeThreadLocal));
     try {
-        evaluate(e, e.p.w, e.p.img, false);
-        integrate(e, false);
+        evaluate(e, e.p.w, e.p.img, false, false, /*pack=*/false);
+        integrate(e, false, /*pack=*/true);
     } catch (...) {
         hipGraph_t junk = nullptr;
         (void)hipStreamEndCapture(e.s_main, &junk);
@@ -457,14 +459,18 @@ void md_steps(Engine& e, int64_t n) {
         }
         const bool energy = (e.prm.energy_interval > 0 && e.step % e.prm.energy_interval == 0) || k == n - 1;
         if (!energy && !e.timing && !(e.prm.launch_mode & LAUNCH_EAGER)) {
+            // the graph holds the evaluation without k_pack and an integrator that packs for the next
+            // step; after a list rebuild (or an unpacked eager step) k_pack runs first
+            const bool need_pack = !e.packed;
+            if (need_pack) launch_pack(e, e.p.w, e.s_main);
             if (!e.step_graph_valid) capture_step_graph(e);
             HIP_CHECK(hipGraphLaunch(e.step_graph, e.s_main));
+            e.packed = true;
         } else {
-            evaluate(e, e.p.w, e.p.img, energy, e.timing);
+            evaluate(e, e.p.w, e.p.img, energy, e.timing, !e.packed);
             // the integrator packs for the next step when that step keeps the list; the last step
             // leaves the force slots intact for lima_get_state
-            const bool pack_next = !e.timing && (e.prm.launch_mode & LAUNCH_EAGER) && k + 1 < n &&
-                                   e.steps_since_build + 1 < interval;
+            const bool pack_next = !e.timing && k + 1 < n && e.steps_since_build + 1 < interval;
             integrate(e, energy, pack_next);
         }
         if (energy) launch_reduce_energy(e, true, e.s_main);

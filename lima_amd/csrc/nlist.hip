@@ -634,6 +634,8 @@ void nlist_build(Engine& e, const uint4* w) {
     // cluster count, slots, list buffers and energy regions may change: a captured step graph holds
     // the old kernel arguments (possibly freed buffers) and the integrator's packing is stale
     e.step_graph_valid = false;
+    if (e.packed && e.pme.tiled && e.elec && e.pme.ncharged > 0)   // tile bins filled for the old slots
+        hipLaunchKernelGGL(k_fill_int, dim3(nblk(e.pme.ntiles, 256)), dim3(256), 0, s, e.pme.bin_count, e.pme.ntiles, 0);
     e.packed = false;
     {   // a list is stale once two particles may have closed the skin: 2 max|dx| > skin
         float d2 = 0.f;
