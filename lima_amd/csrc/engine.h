@@ -28,8 +28,11 @@ struct Particles {
 // Cluster-pair neighbour list (definition in oracle/lima_oracle.cpp build_clusters).
 struct NList {
     int ncx = 1, ncy = 1, ncol = 1;
-    int ncl = 0;               // clusters
-    int ncl_max = 0;
+    int ncl = 0;               // clusters of the current list (host copy, refreshed by nlist_count)
+    int ncl_max = 0;           // capacity: every grid, SoA stride and energy region is sized by it, and
+                               // kernels read the current count from ncl_dev, so a rebuild changes no
+                               // kernel argument
+    const int* ncl_dev = nullptr;   // = col_count + ncol after a rebuild (device)
     int64_t nent_cap = 0;      // entry slots allocated (ncl_max * cap)
     int cap = 2048;            // entry capacity per i-cluster (grows on overflow)
     int* col_of = nullptr;     // [N]
@@ -137,9 +140,8 @@ struct Engine {
     uint4* w_build = nullptr;  // positions at the last list rebuild (displacement check)
     int4* img_build = nullptr;
     float* dmax2 = nullptr;    // [1] max squared displacement since the last rebuild [nm^2]
+    int* lstats = nullptr;     // [2] device list statistics: rebuilds after which 2 max|dx| > skin, last max|dx| (float bits)
     float* dpart = nullptr;    // [N/64] per-wave running max of the squared displacement
-    int64_t list_violations = 0;
-    double last_dmax = 0;
     uint4* w_try = nullptr;
     int4* img_try = nullptr;
     float4* f_acc = nullptr;   // accepted forces, global order
@@ -225,6 +227,8 @@ ERegions energy_regions(const Engine& e);
 void launch_reduce_energy(Engine& e, bool kinetic, hipStream_t s);
 void launch_pack(Engine& e, const uint4* w, hipStream_t s);
 BinArgs bin_args(const Engine& e);   // PME tile bins (bin_count == nullptr when PME is not tiled)
+inline int slots_max(const Engine& e) { return e.nl.ncl_max * CLUSTER; }   // SoA stride of the slot arrays
+int nlist_count(Engine& e);          // current cluster count read back from the device (synchronises)
 void launch_pair(Engine& e, bool energy, hipStream_t s);
 void launch_intra(Engine& e, const uint4* w, const int4* img, bool energy, hipStream_t s);
 void launch_bonded(Engine& e, const uint4* w, const int4* img, bool energy, hipStream_t s);
@@ -234,7 +238,6 @@ void launch_local(Engine& e, const uint4* w, const int4* img, bool energy, hipSt
 void pme_alloc(Engine& e, const std::vector<float>& charges_global);
 void pme_free(Engine& e);
 void launch_pme(Engine& e, const uint4* w, bool energy, hipStream_t s);
-void pme_bin(Engine& e, const uint4* w, hipStream_t s);   // at list rebuild
 
 // ---- scan helper (nlist.hip) ----
 void launch_exclusive_scan(const int* in, int* out, int n, hipStream_t s);

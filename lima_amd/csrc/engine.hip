@@ -373,7 +373,7 @@ void check_finite(Engine& e) {
 void integrate(Engine& e, bool energy, bool pack = false) {
     IntegrateArgs a{};
     a.N = e.N;
-    a.nslots = e.nl.ncl * CLUSTER;
+    a.nslots = slots_max(e);
     for (int d = 0; d < 3; d++) a.Lq[d] = e.Lq[d];
     a.dt = (float)e.dt_ns;
     a.thermostat = e.thermo;
@@ -756,6 +756,13 @@ void create(Engine& e, const lima_box_desc* b, const lima_simparams* p) {
     for (auto& ev : e.tev) HIP_CHECK(hipEventCreate(&ev));
     nlist_alloc(e);
     pme_alloc(e, qv);
+    // energy records: regions sized by capacities, fixed for the engine's lifetime
+    e.ereg = energy_regions(e);
+    e.epart_cap = e.ereg.total;
+    HIP_CHECK(hipMalloc(&e.epart, e.epart_cap * ACC_N * sizeof(double)));
+    HIP_CHECK(hipMemset(e.epart, 0, e.epart_cap * ACC_N * sizeof(double)));
+    HIP_CHECK(hipMalloc(&e.lstats, 2 * sizeof(int)));
+    HIP_CHECK(hipMemset(e.lstats, 0, 2 * sizeof(int)));
     nlist_build(e, e.p.w);
     HIP_CHECK(hipStreamSynchronize(e.s_main));
 }
@@ -764,7 +771,7 @@ void destroy(Engine& e) {
     if (e.s_main) (void)hipStreamSynchronize(e.s_main);
     nlist_free(e);
     pme_free(e);
-    void* ptrs[] = {e.p.w, e.p.img, e.p.v, e.p.fprev, e.p.prm, e.p.comp, e.w_build, e.img_build, e.dmax2, e.thermo, e.dpart, e.epart, e.w_try, e.img_try, e.f_acc, e.sd, e.first,
+    void* ptrs[] = {e.p.w, e.p.img, e.p.v, e.p.fprev, e.p.prm, e.p.comp, e.w_build, e.img_build, e.dmax2, e.lstats, e.thermo, e.dpart, e.epart, e.w_try, e.img_try, e.f_acc, e.sd, e.first,
                     e.bmask, e.intra_items, e.intra_lut, e.bd.bid, e.bd.bp, e.bd.aid, e.bd.ap, e.bd.did, e.bd.dp,
                     e.bd.iid, e.bd.ip, e.bd.pid, e.bd.pp, e.bd.lane_code, e.bd.lane_atom, e.coul_tab, e.acc, e.fmax};
     for (void* p : ptrs) if (p) (void)hipFree(p);
@@ -779,7 +786,7 @@ void destroy(Engine& e) {
 
 void forces_global(Engine& e) {
     hipLaunchKernelGGL(k_zero_f, dim3(1), dim3(64), 0, e.s_main, e.fmax, 1);
-    hipLaunchKernelGGL(k_forces_global, dim3(nblk(e.N, 256)), dim3(256), 0, e.s_main, e.N, e.nl.ncl * CLUSTER, e.nl.slot_of,
+    hipLaunchKernelGGL(k_forces_global, dim3(nblk(e.N, 256)), dim3(256), 0, e.s_main, e.N, slots_max(e), e.nl.slot_of,
                        e.fs, e.es, e.fb, e.eb, e.f_acc, e.fmax);
 }
 
@@ -877,7 +884,7 @@ lima_status lima_minimize(lima_engine* h, double tol_kj, int64_t max_iter, doubl
             launch_reduce_energy(e, false, s);
             // trial forces in global order -> p.fprev used as scratch
             hipLaunchKernelGGL(k_zero_f, dim3(1), dim3(64), 0, s, e.fmax, 1);
-            hipLaunchKernelGGL(k_forces_global, dim3(nblk(e.N, 256)), dim3(256), 0, s, e.N, e.nl.ncl * CLUSTER, e.nl.slot_of,
+            hipLaunchKernelGGL(k_forces_global, dim3(nblk(e.N, 256)), dim3(256), 0, s, e.N, slots_max(e), e.nl.slot_of,
                                e.fs, e.es, e.fb, e.eb, e.p.fprev, e.fmax);
             hipLaunchKernelGGL(k_sd_accept, dim3(1), dim3(1), 0, s, e.sd, e.acc, e_const(e), e.fmax);
             hipLaunchKernelGGL(k_sd_commit, dim3(nblk(e.N, 256)), dim3(256), 0, s, e.N, e.sd, e.w_try, e.img_try, e.p.fprev,
@@ -970,7 +977,7 @@ lima_status lima_get_state(lima_engine* h, lima_state_out* out) {
 
 lima_status lima_get_status(lima_engine* h, lima_runstatus* out) {
     if (!h || !out) { set_err("null argument"); return LIMA_ERR_BAD_INPUT; }
-    const Engine& e = h->e;
+    Engine& e = h->e;
     std::memset(out, 0, sizeof(*out));
     out->current_step = e.step;
     out->current_temperature = NAN;   // RunStatus default until the first measurement (Engine.cuh:67)
@@ -981,13 +988,20 @@ lima_status lima_get_status(lima_engine* h, lima_runstatus* out) {
     out->critical_error_occured = e.critical ? 1 : 0;
     out->simulation_finished = (e.prm.n_steps > 0 && e.step >= e.prm.n_steps) ? 1 : 0;
     out->n_list_rebuilds = e.n_rebuilds;
-    if (e.nl.ncl > 0 && e.nl.ecount) {   // padded entry slots of the current list (summed here, not per rebuild)
-        std::vector<int> cnt(e.nl.ncl);
+    int ncl = 0;
+    if (e.nl.ncl_dev && hipMemcpy(&ncl, e.nl.ncl_dev, sizeof(int), hipMemcpyDeviceToHost) == hipSuccess && ncl > 0) {
+        e.nl.ncl = ncl;   // padded entry slots of the current list (summed here, not per rebuild)
+        std::vector<int> cnt(ncl);
         if (hipMemcpy(cnt.data(), e.nl.ecount, cnt.size() * sizeof(int), hipMemcpyDeviceToHost) == hipSuccess)
             for (int c : cnt) out->list_entries += c;
     }
-    out->list_violations = e.list_violations;
-    out->max_displacement_nm = (float)e.last_dmax;
+    int st[2] = {0, 0};
+    if (e.lstats && hipMemcpy(st, e.lstats, sizeof(st), hipMemcpyDeviceToHost) == hipSuccess) {
+        out->list_violations = st[0];
+        float d;
+        std::memcpy(&d, &st[1], sizeof(float));
+        out->max_displacement_nm = d;
+    }
     out->greatest_force = NAN;
     return LIMA_OK;
 }
@@ -998,7 +1012,7 @@ lima_status lima_get_neighbor_counts(lima_engine* h, int64_t* ncl, int64_t* nent
         Engine& e = h->e;
         nlist_build(e, e.p.w);
         HIP_CHECK(hipStreamSynchronize(e.s_main));
-        const int nc = e.nl.ncl;
+        const int nc = nlist_count(e);
         std::vector<int> cnt(nc);
         HIP_CHECK(hipMemcpy(cnt.data(), e.nl.ecount, nc * sizeof(int), hipMemcpyDeviceToHost));
         int64_t n = 0;
@@ -1020,7 +1034,7 @@ lima_status lima_get_neighbor_list(lima_engine* h, int32_t* cluster_atoms, int64
     if (!h) { set_err("null engine"); return LIMA_ERR_BAD_INPUT; }
     return guarded([&] {
         Engine& e = h->e;
-        const int ncl = e.nl.ncl;
+        const int ncl = nlist_count(e);
         const int64_t cap = e.nl.cap;
         std::vector<int> cat((size_t)ncl * CLUSTER), cnt(ncl);
         std::vector<float4> ent((size_t)ncl * cap);

@@ -35,14 +35,17 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
 
 // Slot k of the cluster list: position relative to the cluster reference point and charge,
 // force / energy accumulators cleared, and (tiled PME) the charged particle put in its tile bin.
-__global__ void k_pack(int nslots, const int* __restrict__ catoms, const uint4* __restrict__ w, const float4* __restrict__ prm,
+// nslots: the slot capacity (SoA stride); slots of clusters beyond the current count are left alone.
+__global__ void k_pack(const int* __restrict__ ncl_dev, int nslots, const int* __restrict__ catoms, const uint4* __restrict__ w,
+                       const float4* __restrict__ prm,
                        const uint4* __restrict__ cref, long long Lqx, long long Lqy, long long Lqz,
                        float4* __restrict__ xq, float* __restrict__ fs, float* __restrict__ es, BinArgs bins) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int a = k < nslots ? catoms[k] : -1;
+    const bool live = k < min(*ncl_dev * CLUSTER, nslots);
+    const int a = live ? catoms[k] : -1;
     uint4 p = make_uint4(0u, 0u, 0u, 0u);
     float q = 0.f;
-    if (k < nslots) {
+    if (live) {
         float4 out = make_float4(PAD_POS, PAD_POS, PAD_POS, 0.f);
         if (a >= 0) {
             p = w[a];
@@ -66,7 +69,8 @@ __global__ void k_pack(int nslots, const int* __restrict__ catoms, const uint4* 
 }
 
 struct PairArgs {
-    int ncl, nslots, cap;
+    const int* ncl_dev;  // clusters of the current list (device)
+    int ncl_max, nslots, cap;
     int split;           // waves per i-cluster (each takes a contiguous share of its entries)
     float rc2;
     float tscale;        // COUL_TAB / cutoff
@@ -104,7 +108,13 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs a) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int wv = xcd_block(blockIdx.x, gridDim.x) * PAIR_WAVES + wid;
     const int I = wv / a.split, part = wv - I * a.split;
-    if (I >= a.ncl) return;
+    if (I >= min(*a.ncl_dev, a.ncl_max)) {   // the grid covers the capacity
+        if (ENERGY && lane == 0 && wv < a.ncl_max * a.split) {   // no stale records in the energy sum
+            double* r = a.epart + (int64_t)wv * ACC_N;
+            for (int c = 0; c < ACC_N; c++) r[c] = 0.0;
+        }
+        return;
+    }
     const int ii = lane & 7, jj = lane >> 3;
     const int islot = I * CLUSTER + ii;
     const float4 xi = a.xq[islot];
@@ -465,14 +475,14 @@ __global__ __launch_bounds__(256) void k_reduce_energy(const double* __restrict_
 int pair_split(const Engine& e) {
     // enough waves to keep ~16 per SIMD in flight (256 CUs x 4 SIMDs) on small boxes
     int split = 1;
-    while (split < 8 && (int64_t)e.nl.ncl * split < 16384) split *= 2;
+    while (split < 8 && (int64_t)e.nl.ncl_max * split < 16384) split *= 2;
     return split;
 }
 
 ERegions energy_regions(const Engine& e) {
     ERegions r;
     r.pair = 0;
-    r.intra = r.pair + e.nl.ncl * pair_split(e);
+    r.intra = r.pair + e.nl.ncl_max * pair_split(e);
     r.bond = r.intra + e.n_intra;
     r.pme = r.bond + e.bd.n_lanes / 64;   // one record per wave of the bonded incidence lanes
     // PME records: one per gather wave (4 per tile block when tiled)
@@ -488,15 +498,16 @@ void launch_reduce_energy(Engine& e, bool kinetic, hipStream_t s) {
 }
 
 void launch_pack(Engine& e, const uint4* w, hipStream_t s) {
-    const int nslots = e.nl.ncl * CLUSTER;
-    hipLaunchKernelGGL(k_pack, dim3(nblk(nslots, 256)), dim3(256), 0, s, nslots, e.nl.catoms, w, e.p.prm, e.nl.cref,
+    const int nslots = slots_max(e);
+    hipLaunchKernelGGL(k_pack, dim3(nblk(nslots, 256)), dim3(256), 0, s, e.nl.ncl_dev, nslots, e.nl.catoms, w, e.p.prm, e.nl.cref,
                        (long long)e.Lq[0], (long long)e.Lq[1], (long long)e.Lq[2], e.nl.xq, e.fs, e.es, bin_args(e));
 }
 
 void launch_pair(Engine& e, bool energy, hipStream_t s) {
     PairArgs a{};
-    a.ncl = e.nl.ncl;
-    a.nslots = e.nl.ncl * CLUSTER;
+    a.ncl_dev = e.nl.ncl_dev;
+    a.ncl_max = e.nl.ncl_max;
+    a.nslots = slots_max(e);
     a.rc2 = e.rc2;
     a.tscale = (float)COUL_TAB / e.rc;
     a.pc = e.pc;
@@ -504,9 +515,8 @@ void launch_pair(Engine& e, bool energy, hipStream_t s) {
     a.cap = e.nl.cap;
     a.xq = e.nl.xq; a.ljs = e.nl.ljs; a.ecount = e.nl.ecount; a.ent = e.nl.ent;
     a.fs = e.fs; a.es = e.es; a.epart = e.epart + (int64_t)e.ereg.pair * ACC_N;
-    if (a.ncl == 0) return;
     a.split = pair_split(e);
-    const int64_t waves = (int64_t)a.ncl * a.split;
+    const int64_t waves = (int64_t)a.ncl_max * a.split;
     if (energy) hipLaunchKernelGGL(k_pair<true>, dim3(nblk(waves, PAIR_WAVES)), dim3(64 * PAIR_WAVES), 0, s, a);
     else hipLaunchKernelGGL(k_pair<false>, dim3(nblk(waves, PAIR_WAVES)), dim3(64 * PAIR_WAVES), 0, s, a);
 }
@@ -514,7 +524,7 @@ void launch_pair(Engine& e, bool energy, hipStream_t s) {
 IntraArgs intra_args(Engine& e, const uint4* w, const int4* img) {
     IntraArgs a{};
     a.n_items = e.n_intra;
-    a.nslots = e.nl.ncl * CLUSTER;
+    a.nslots = slots_max(e);
     for (int d = 0; d < 3; d++) a.Lq[d] = e.Lq[d];
     a.pc = e.pc;
     a.items = e.intra_items; a.lut = e.intra_lut; a.w = w; a.img = img; a.prm = e.p.prm;
@@ -525,7 +535,7 @@ IntraArgs intra_args(Engine& e, const uint4* w, const int4* img) {
 BondArgs bond_args(Engine& e, const uint4* w, const int4* img) {
     BondArgs a{};
     a.nb = e.bd.nb; a.na = e.bd.na; a.nd = e.bd.nd; a.ni = e.bd.ni; a.np = e.bd.np;
-    a.nslots = e.nl.ncl * CLUSTER;
+    a.nslots = slots_max(e);
     for (int d = 0; d < 3; d++) a.Lq[d] = e.Lq[d];
     a.pc = e.pc;
     a.bid = e.bd.bid; a.bp = e.bd.bp; a.aid = e.bd.aid; a.ap = e.bd.ap; a.did = e.bd.did; a.dp = e.bd.dp;

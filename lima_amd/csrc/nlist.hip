@@ -161,7 +161,7 @@ __global__ void k_assign_slots(int64_t N, int ncol, const int* __restrict__ col_
 
 // One thread per cluster slot; the 8 slots of a cluster are 8 consecutive lanes, which reduce the
 // cluster's bounding box with xor shuffles.
-__global__ __launch_bounds__(256) void k_cluster_props(int ncl, int ncol, const int* __restrict__ col_cl_start,
+__global__ __launch_bounds__(256) void k_cluster_props(const int* __restrict__ ncl_dev, int ncl_max, int ncol, const int* __restrict__ col_cl_start,
                                                        const int* __restrict__ catoms, const uint4* __restrict__ w,
                                                        const float4* __restrict__ prm, const int* __restrict__ comp,
                                                        int* __restrict__ ccol, uint4* __restrict__ cref, uint4* __restrict__ cmin,
@@ -169,7 +169,7 @@ __global__ __launch_bounds__(256) void k_cluster_props(int ncl, int ncol, const 
                                                        uint4* __restrict__ cw) {
     const int slot = blockIdx.x * blockDim.x + threadIdx.x;
     const int I = slot / CLUSTER;
-    const bool live = I < ncl;   // whole 8-lane groups share I
+    const bool live = I < min(*ncl_dev, ncl_max);   // whole 8-lane groups share I
     const int a = live ? catoms[slot] : -1;
     uint32_t mn[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, mx[3] = {0u, 0u, 0u};
     if (a >= 0) {
@@ -201,7 +201,8 @@ __global__ __launch_bounds__(256) void k_cluster_props(int ncl, int ncol, const 
 }
 
 struct NlArgs {
-    int ncl, ncx, ncy, nC, cap;
+    const int* ncl_dev;      // clusters of this list (device, written by the column scan)
+    int ncl_max, ncx, ncy, nC, cap;
     int* flags;
     long long Lq[3];
     long long R, R2;
@@ -315,7 +316,7 @@ __global__ __launch_bounds__(256) void k_nlist(NlArgs a) {
     __shared__ uint32_t flat_all[4][NL_FLAT], cand_all[4][NL_FLAT];
     const int lane = threadIdx.x & 63;
     const int I = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (I >= a.ncl) return;   // uniform per wave
+    if (I >= min(*a.ncl_dev, a.ncl_max)) return;   // uniform per wave
     uint32_t* flat = flat_all[threadIdx.x >> 6];   // candidates that passed the box test
     uint32_t* cand = cand_all[threadIdx.x >> 6];   // all candidates of the current chunk
     const uint4 bmn = a.cmin[I], bmx = a.cmax[I], refI = a.cref[I];
@@ -569,6 +570,8 @@ void nlist_alloc(Engine& e) {
     const int64_t slots = (int64_t)nl.ncl_max * CLUSTER;
     HIP_CHECK(hipMalloc(&nl.col_of, e.N * sizeof(int)));
     HIP_CHECK(hipMalloc(&nl.col_count, (nl.ncol + 1) * sizeof(int)));
+    HIP_CHECK(hipMemset(nl.col_count, 0, (nl.ncol + 1) * sizeof(int)));
+    nl.ncl_dev = nl.col_count + nl.ncol;
     HIP_CHECK(hipMalloc(&nl.col_start, (nl.ncol + 1) * sizeof(int)));
     HIP_CHECK(hipMalloc(&nl.col_fill, (nl.ncol + 1) * sizeof(int)));
     HIP_CHECK(hipMalloc(&nl.keys, e.N * sizeof(unsigned long long)));
@@ -625,26 +628,37 @@ __global__ __launch_bounds__(256) void k_max_reduce(const float* __restrict__ v,
     if (threadIdx.x == 0) *out = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
 }
 
+// List staleness statistics kept on the device: the largest displacement since the previous
+// rebuild, and a count of rebuilds after which 2 max|dx| exceeded the skin.
+__global__ void k_list_stats(const float* __restrict__ dmax2, float skin, int counted, int* stats) {
+    if (threadIdx.x != 0) return;
+    const float d = sqrtf(*dmax2);
+    if (counted && 2.f * d > skin) stats[0] += 1;
+    stats[1] = __float_as_int(d);
+}
+
 }  // namespace
+
+int nlist_count(Engine& e) {
+    int ncl = 0;
+    HIP_CHECK(hipMemcpy(&ncl, e.nl.ncl_dev, sizeof(int), hipMemcpyDeviceToHost));
+    e.nl.ncl = ncl;
+    return ncl;
+}
 
 void nlist_build(Engine& e, const uint4* w) {
     NList& nl = e.nl;
     hipStream_t s = e.s_main;
     const int64_t N = e.N;
-    // cluster count, slots, list buffers and energy regions may change: a captured step graph holds
-    // the old kernel arguments (possibly freed buffers) and the integrator's packing is stale
-    e.step_graph_valid = false;
+    // Every kernel argument stays the same across rebuilds (grids and strides follow ncl_max, the
+    // count is read from ncl_dev), so a captured step graph stays valid unless the entry buffer is
+    // reallocated.  The integrator's packing of the old slots is stale.
     if (e.packed && e.pme.tiled && e.elec && e.pme.ncharged > 0)   // tile bins filled for the old slots
         hipLaunchKernelGGL(k_fill_int, dim3(nblk(e.pme.ntiles, 256)), dim3(256), 0, s, e.pme.bin_count, e.pme.ntiles, 0);
     e.packed = false;
-    {   // a list is stale once two particles may have closed the skin: 2 max|dx| > skin
-        float d2 = 0.f;
-        hipLaunchKernelGGL(k_max_reduce, dim3(1), dim3(256), 0, s, e.dpart, (int)((N + 63) / 64), e.dmax2);
-        HIP_CHECK(hipMemcpyAsync(&d2, e.dmax2, sizeof(float), hipMemcpyDeviceToHost, s));
-        HIP_CHECK(hipStreamSynchronize(s));
-        e.last_dmax = std::sqrt((double)d2);
-        if (e.n_rebuilds > 0 && 2.0 * e.last_dmax > (double)e.skin) e.list_violations++;
-    }
+    // staleness statistics of the list being replaced, on the device (read by lima_get_status)
+    hipLaunchKernelGGL(k_max_reduce, dim3(1), dim3(256), 0, s, e.dpart, (int)((N + 63) / 64), e.dmax2);
+    hipLaunchKernelGGL(k_list_stats, dim3(1), dim3(64), 0, s, (const float*)e.dmax2, e.skin, e.n_rebuilds > 0 ? 1 : 0, e.lstats);
     if (w == e.p.w) hipLaunchKernelGGL(k_snapshot, dim3(nblk(N, 256)), dim3(256), 0, s, N, e.p.w, e.p.img, e.w_build, e.img_build, e.dpart);
     hipLaunchKernelGGL(k_fill_int, dim3(nblk(nl.ncol + 1, 256)), dim3(256), 0, s, nl.col_count, nl.ncol + 1, 0);
     hipLaunchKernelGGL(k_fill_int, dim3(nblk(nl.ncol + 1, 256)), dim3(256), 0, s, nl.col_fill, nl.ncol + 1, 0);
@@ -656,28 +670,22 @@ void nlist_build(Engine& e, const uint4* w) {
     hipLaunchKernelGGL(k_col_scatter, dim3(nblk(N, 256 * COL_ITEMS)), dim3(256), col_lds, s, N, nl.ncol, w, nl.col_of, nl.col_start,
                        nl.col_fill, nl.keys);
     hipLaunchKernelGGL(k_col_sort, dim3(nl.ncol), dim3(1024), 0, s, nl.col_start, nl.keys, nl.col_ncl, nl.flags);
-    launch_exclusive_scan(nl.col_ncl, nl.col_count, nl.ncol, s);   // col_count now holds the cluster start per column
-    int hflags[4];
-    int ncl = 0;
-    HIP_CHECK(hipMemcpyAsync(hflags, nl.flags, sizeof(hflags), hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipMemcpyAsync(&ncl, nl.col_count + nl.ncol, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipStreamSynchronize(s));
-    if (hflags[0]) throw LimaFail(LIMA_ERR_LIST_OVERFLOW, "an x-y column holds more than 2048 particles");
-    if (ncl > nl.ncl_max) throw LimaFail(LIMA_ERR_LIST_OVERFLOW, "cluster count exceeds capacity");
-    nl.ncl = ncl;
+    // col_count now holds the cluster start of every column and the total, ncl = col_count[ncol]
+    // (at most N / 8 + ncol <= ncl_max: each column pads its last cluster)
+    launch_exclusive_scan(nl.col_ncl, nl.col_count, nl.ncol, s);
     const int* col_cl_start = nl.col_count;
-    hipLaunchKernelGGL(k_fill_int, dim3(nblk((int64_t)ncl * CLUSTER, 256)), dim3(256), 0, s, nl.catoms, ncl * CLUSTER, -1);
+    const int ns = slots_max(e);
+    hipLaunchKernelGGL(k_fill_int, dim3(nblk(ns, 256)), dim3(256), 0, s, nl.catoms, ns, -1);
     hipLaunchKernelGGL(k_assign_slots, dim3(nblk(N, 256)), dim3(256), 0, s, N, nl.ncol, nl.col_start, col_cl_start, nl.keys,
                        nl.catoms, nl.slot_of);
     {   // bonded slot forces: slots of atoms without bonded terms stay zero until the next rebuild
-        const int64_t ns = (int64_t)nl.ncl_max * CLUSTER;
-        hipLaunchKernelGGL(k_fill_int, dim3(nblk(3 * ns, 256)), dim3(256), 0, s, (int*)e.fb, (int)(3 * ns), 0);
-        hipLaunchKernelGGL(k_fill_int, dim3(nblk(ns, 256)), dim3(256), 0, s, (int*)e.eb, (int)ns, 0);
+        hipLaunchKernelGGL(k_fill_int, dim3(nblk(3 * (int64_t)ns, 256)), dim3(256), 0, s, (int*)e.fb, 3 * ns, 0);
+        hipLaunchKernelGGL(k_fill_int, dim3(nblk(ns, 256)), dim3(256), 0, s, (int*)e.eb, ns, 0);
     }
-    hipLaunchKernelGGL(k_cluster_props, dim3(nblk((int64_t)ncl * CLUSTER, 256)), dim3(256), 0, s, ncl, nl.ncol, col_cl_start,
+    hipLaunchKernelGGL(k_cluster_props, dim3(nblk(ns, 256)), dim3(256), 0, s, nl.ncl_dev, nl.ncl_max, nl.ncol, col_cl_start,
                        nl.catoms, w, e.p.prm, e.p.comp, nl.ccol, nl.cref, nl.cmin, nl.cmax, nl.ljs, nl.comps, nl.cw);
     NlArgs a{};
-    a.ncl = ncl; a.ncx = nl.ncx; a.ncy = nl.ncy; a.nC = e.nC;
+    a.ncl_dev = nl.ncl_dev; a.ncl_max = nl.ncl_max; a.ncx = nl.ncx; a.ncy = nl.ncy; a.nC = e.nC;
     for (int d = 0; d < 3; d++) a.Lq[d] = e.Lq[d];
     a.R = llround(((double)e.rc + (double)e.skin) * 1e8);
     a.R2 = a.R * a.R;
@@ -693,26 +701,28 @@ void nlist_build(Engine& e, const uint4* w) {
             HIP_CHECK(hipMalloc(&nl.ent, need * sizeof(float4)));
             HIP_CHECK(hipMalloc(&nl.ecode, need));
             nl.nent_cap = need;
+            e.step_graph_valid = false;   // the captured pair kernel names the old entry buffer
         }
         a.cap = nl.cap; a.ent = nl.ent; a.ecode = nl.ecode;
         // 32-bit coordinate differences are exact when the box, the list radius and a column's
         // width stay below 2^31 Coord units together (boxes up to ~10 nm)
         const long long Lmax = std::max(a.Lq[0], std::max(a.Lq[1], a.Lq[2]));
-        if (2 * Lmax + 2 * a.R < (1ll << 31)) hipLaunchKernelGGL(k_nlist<true>, dim3(nblk(ncl, 4)), dim3(256), 0, s, a);
-        else hipLaunchKernelGGL(k_nlist<false>, dim3(nblk(ncl, 4)), dim3(256), 0, s, a);
+        if (2 * Lmax + 2 * a.R < (1ll << 31)) hipLaunchKernelGGL(k_nlist<true>, dim3(nblk(nl.ncl_max, 4)), dim3(256), 0, s, a);
+        else hipLaunchKernelGGL(k_nlist<false>, dim3(nblk(nl.ncl_max, 4)), dim3(256), 0, s, a);
+        // the one host synchronisation of a rebuild: overflow flags of the columns, the pair list
+        // and the PME tile bins of the steps since the last rebuild
+        int hflags[4];
+        int pflag = 0;
         HIP_CHECK(hipMemcpyAsync(hflags, nl.flags, sizeof(hflags), hipMemcpyDeviceToHost, s));
+        if (e.pme.tiled && e.elec && e.pme.ncharged > 0)
+            HIP_CHECK(hipMemcpyAsync(&pflag, e.pme.flags, sizeof(int), hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipStreamSynchronize(s));
+        if (hflags[0]) throw LimaFail(LIMA_ERR_LIST_OVERFLOW, "an x-y column holds more than 2048 particles");
+        if (pflag) throw LimaFail(LIMA_ERR_LIST_OVERFLOW, "a PME tile bin overflowed (more than 512 charged particles in one tile)");
         if (!hflags[1]) break;
         if (attempt >= 6) throw LimaFail(LIMA_ERR_LIST_OVERFLOW, "pair list capacity per cluster exceeded");
         nl.cap = (nl.cap * 3 / 2 + 7) & ~7;
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(4), 0, s, nl.flags, 4, 0);
-    }
-    pme_bin(e, w, s);
-    e.ereg = energy_regions(e);
-    if (e.ereg.total > e.epart_cap) {
-        if (e.epart) HIP_CHECK(hipFree(e.epart));
-        e.epart_cap = (int64_t)(e.ereg.total * 1.25) + 64;
-        HIP_CHECK(hipMalloc(&e.epart, e.epart_cap * ACC_N * sizeof(double)));
     }
     HIP_CHECK(hipGetLastError());
     e.steps_since_build = 0;

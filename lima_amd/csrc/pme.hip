@@ -600,15 +600,6 @@ BinArgs bin_args(const Engine& e) {
     return b;
 }
 
-void pme_bin(Engine& e, const uint4* w, hipStream_t s) {   // checked at every list rebuild
-    (void)w;
-    Pme& m = e.pme;
-    if (!m.tiled || !e.elec || m.ncharged == 0) return;
-    int hflag = 0;
-    HIP_CHECK(hipMemcpyAsync(&hflag, m.flags, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipStreamSynchronize(s));
-    if (hflag) throw LimaFail(LIMA_ERR_LIST_OVERFLOW, "a PME tile bin overflowed (more than 512 charged particles in one tile)");
-}
 
 void pme_alloc(Engine& e, const std::vector<float>& q) {
     Pme& m = e.pme;
@@ -766,7 +757,7 @@ void launch_pme(Engine& e, const uint4* w, bool energy, hipStream_t s) {
             gt.inv2h[d] = (float)(m.K[d] / (2.0 * e.L[d]));
         }
         gt.bincap = m.bincap;
-        gt.nslots = e.nl.ncl * CLUSTER;
+        gt.nslots = slots_max(e);
         gt.eself = (float)m.eself_total;
         gt.w = w; gt.prm = e.p.prm; gt.phi = m.phi; gt.slot_of = e.nl.slot_of; gt.bin_atoms = m.bin_atoms;
         gt.bin_count = m.bin_count; gt.fs = e.fs; gt.es = e.es;
@@ -778,7 +769,7 @@ void launch_pme(Engine& e, const uint4* w, bool energy, hipStream_t s) {
     }
     GatherArgs ga{};
     ga.N = e.N;
-    ga.nslots = e.nl.ncl * CLUSTER;
+    ga.nslots = slots_max(e);
     for (int d = 0; d < 3; d++) { ga.Lq[d] = e.Lq[d]; ga.K[d] = m.K[d]; ga.inv2h[d] = (float)(m.K[d] / (2.0 * e.L[d])); }
     ga.eself = (float)m.eself_total;
     ga.w = w; ga.prm = e.p.prm; ga.phi = m.phi; ga.slot_of = e.nl.slot_of; ga.fs = e.fs; ga.es = e.es;
