@@ -315,7 +315,9 @@ template <bool NARROW>
 __global__ __launch_bounds__(256) void k_nlist(NlArgs a) {
     __shared__ uint32_t flat_all[4][NL_FLAT], cand_all[4][NL_FLAT];
     const int lane = threadIdx.x & 63;
-    const int I = blockIdx.x * 4 + (threadIdx.x >> 6);
+    // the wave's cluster, marked wave-uniform so that everything derived from it (bounds, compound
+    // ids, column ranges) is loaded and kept in scalar registers
+    const int I = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
     if (I >= min(*a.ncl_dev, a.ncl_max)) return;   // uniform per wave
     uint32_t* flat = flat_all[threadIdx.x >> 6];   // candidates that passed the box test
     uint32_t* cand = cand_all[threadIdx.x >> 6];   // all candidates of the current chunk
