@@ -263,12 +263,12 @@ __device__ __forceinline__ void bonded_atom(const BatchArgs& a, const SysMeta& m
             const int4 id = a.did[t];
             v3 fd[4];
             e = term_dihedral(bdisp(u, id.x, id.y, m), bdisp(u, id.y, id.z, m), bdisp(u, id.z, id.w, m), a.dp[t], fd);
-            f = role == 0 ? fd[0] : (role == 1 ? fd[1] : (role == 2 ? fd[2] : fd[3]));
+            f = pick_role4(fd, role);
         } else if (type == T_IMP) {
             const int4 id = a.iid[t];
             v3 fd[4];
             e = term_improper(bdisp(u, id.x, id.y, m), bdisp(u, id.x, id.z, m), bdisp(u, id.x, id.w, m), a.ip[t], fd);
-            f = role == 0 ? fd[0] : (role == 1 ? fd[1] : (role == 2 ? fd[2] : fd[3]));
+            f = pick_role4(fd, role);
         } else {
             const int2 id = a.pid[t];
             v3 fa;

@@ -395,13 +395,13 @@ __device__ __forceinline__ void bonded_lanes(const BondArgs& a, int blk) {
             const int4 id = a.did[t];
             v3 fd[4];
             e = term_dihedral(bdisp(a, id.x, id.y), bdisp(a, id.y, id.z), bdisp(a, id.z, id.w), a.dp[t], fd);
-            f = role == 0 ? fd[0] : (role == 1 ? fd[1] : (role == 2 ? fd[2] : fd[3]));
+            f = pick_role4(fd, role);
             nat = 4;
         } else if (type == T_IMP) {   // calcImproperdihedralbondForces :214-264
             const int4 id = a.iid[t];
             v3 fd[4];
             e = term_improper(bdisp(a, id.x, id.y), bdisp(a, id.x, id.z), bdisp(a, id.x, id.w), a.ip[t], fd);
-            f = role == 0 ? fd[0] : (role == 1 ? fd[1] : (role == 2 ? fd[2] : fd[3]));
+            f = pick_role4(fd, role);
             nat = 4;
         } else {   // computePairbondForces :335-376
             const int2 id = a.pid[t];

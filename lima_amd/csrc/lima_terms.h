@@ -132,6 +132,18 @@ __device__ __forceinline__ void grid_coord(uint32_t w, long long Lq, int K, int*
     *f = (float)r / (float)Lq;
 }
 
+// f[role] for the 4 per-atom forces of a dihedral or improper, as a sum weighted by 0 / 1 so the
+// array stays in registers (a select between elements is folded into an indexed load, which puts
+// the array in scratch memory).
+__device__ __forceinline__ v3 pick_role4(const v3* f, int role) {
+    const float w0 = role == 0 ? 1.f : 0.f, w1 = role == 1 ? 1.f : 0.f, w2 = role == 2 ? 1.f : 0.f, w3 = role == 3 ? 1.f : 0.f;
+    v3 o;
+    o.x = fmaf(f[3].x, w3, fmaf(f[2].x, w2, fmaf(f[1].x, w1, f[0].x * w0)));
+    o.y = fmaf(f[3].y, w3, fmaf(f[2].y, w2, fmaf(f[1].y, w1, f[0].y * w0)));
+    o.z = fmaf(f[3].z, w3, fmaf(f[2].z, w2, fmaf(f[1].z, w1, f[0].z * w0)));
+    return o;
+}
+
 // PME tile bins of charged particles (tiled spreading).  A particle goes to the fixed-capacity
 // bin of the tile holding its base node; the bin order does not matter (the tile sums are
 // integer).  The counters start at zero each evaluation: the tiled gather clears them after use.
