@@ -119,6 +119,9 @@ typedef struct lima_simparams {
                                          LIMA_LAUNCH_EAGER (launch the kernels one by one; the
                                          integrator then also packs the next step's slots) and
                                          LIMA_LAUNCH_ONE_STREAM (PME in line) */
+    int32_t log_capacity;             /* frames the device data log holds (0 = no logging); with
+                                         data_logging_interval > 0 every step % interval == 0 is
+                                         evaluated with energies and logged (lima_read_log) */
 } lima_simparams;
 
 #define LIMA_LAUNCH_EAGER 1
@@ -212,6 +215,41 @@ lima_status lima_get_neighbor_counts(lima_engine* engine, int64_t* n_clusters, i
  * (cluster slot i, j) is computed in the cutoff kernel. */
 lima_status lima_get_neighbor_list(lima_engine* engine, int32_t* cluster_atoms, int64_t* entry_offsets,
                                    uint32_t* entries, uint8_t* entry_masks);
+
+/* ---- Data logging: LIMA's DatabuffersDeviceController (SimulationDevice.cuh:45-75), filled by
+ * LogCompoundData / LogSolventData (EngineUtils.cuh:143-173) at every step % data_logging_interval
+ * == 0 and offloaded by Engine::offloadLoggingData (Engine.cu:203-240).  Frames use LIMA's buffer
+ * layout of total_particles_upperbound = n_compounds * 32 + n_tinymols entries (compound c slot s
+ * at c * 32 + s, tinymol t at n_compounds * 32 + t; unused compound slots are zero):
+ *   position    [nm] after the step's integration, compound particles as origo + Coord with the
+ *               origo at the key particle's wrapped nm node, tinymols wrapped into the box
+ *               (LIMAPOSITIONSYSTEM::GetAbsolutePositionNM after applyBC);
+ *   pot_energy  the particle's LIMA potE of the step (as lima_state_out.particle_pot_energy);
+ *   speed       |v| after the step (thermostat scale applied);
+ *   force       the step's force [J/mol/nm].
+ * The device holds the last log_capacity frames (a ring); the host copies frames on request, so
+ * logging adds no host synchronisation to the step loop.  Frame k is the k-th logged step since
+ * lima_create. ---- */
+typedef struct lima_log_info {
+    int64_t frames_logged;        /* frames written since lima_create */
+    int64_t first_held;           /* oldest frame still held (frames_logged - held) */
+    int64_t particles_per_frame;  /* n_compounds * 32 + n_tinymols */
+    int32_t interval;             /* data_logging_interval */
+    int32_t capacity;             /* log_capacity */
+} lima_log_info;
+lima_status lima_get_log_info(lima_engine* engine, lima_log_info* out);
+/* Arrays the caller provides for n_frames frames (NULL = skip), P = particles_per_frame. */
+typedef struct lima_log_out {
+    int64_t*     step;            /* [n_frames] */
+    lima_energy* energy;          /* [n_frames] FP64 totals of the logged step */
+    float*       position;        /* [n_frames][P][3] */
+    float*       pot_energy;      /* [n_frames][P] */
+    float*       speed;           /* [n_frames][P] */
+    float*       force;           /* [n_frames][P][3] */
+} lima_log_out;
+/* Frames first_frame .. first_frame + n_frames - 1; LIMA_ERR_BAD_INPUT when any of them is no
+ * longer (or not yet) held. */
+lima_status lima_read_log(lima_engine* engine, int64_t first_frame, int64_t n_frames, lima_log_out* out);
 
 /* Thread-local message of the last error on this thread. */
 const char* lima_last_error(void);

@@ -15,7 +15,7 @@ from pathlib import Path
 
 import numpy as np
 
-from ._abi import BoxDesc, SimParams, Energy, RunStatus, StateOut, EmResult, Timings, BatchTuning
+from ._abi import BoxDesc, SimParams, Energy, RunStatus, StateOut, EmResult, Timings, BatchTuning, LogInfo, LogOut, EnergyAnalysis
 
 LIB_DIR = Path(__file__).resolve().parent / "lib"
 REPO_ROOT = Path(__file__).resolve().parent.parent
@@ -52,6 +52,13 @@ def load_host():
         lib.lima_box_n_particles.restype = C.c_int64
         lib.lima_gen_last_error.restype = C.c_char_p
         lib.lima_default_params.argtypes = [C.POINTER(SimParams), C.c_float, C.c_int64]
+        dp = C.POINTER(C.c_double)
+        lib.lima_analyze_energy.argtypes = [dp, dp, C.c_int64, C.POINTER(EnergyAnalysis)]
+        lib.lima_analyze_energy.restype = C.c_int
+        lib.lima_write_trr.argtypes = [C.c_char_p, C.c_int64, C.c_int64, dp, C.POINTER(C.c_float),
+                                       C.POINTER(C.c_int64), C.c_int32]
+        lib.lima_write_trr.restype = C.c_int
+        lib.lima_io_last_error.restype = C.c_char_p
         _host = lib
     return _host
 
@@ -80,6 +87,8 @@ def load_hip():
             "lima_get_timings": ([vp, C.POINTER(Timings)], C.c_int),
             "lima_set_timing": ([vp, C.c_int32], C.c_int),
             "lima_get_pair_count": ([vp], C.c_int64),
+            "lima_get_log_info": ([vp, C.POINTER(LogInfo)], C.c_int),
+            "lima_read_log": ([vp, C.c_int64, C.c_int64, C.POINTER(LogOut)], C.c_int),
             "lima_batch_create": ([C.POINTER(BoxDesc), C.c_int32, C.POINTER(SimParams), C.POINTER(vp)], C.c_int),
             "lima_batch_destroy": ([vp], C.c_int),
             "lima_batch_autotune": ([vp, C.c_int32, C.POINTER(BatchTuning)], C.c_int),
@@ -115,6 +124,39 @@ def load_hip():
         lib.lima_last_error.restype = C.c_char_p
         _hip = lib
     return _hip
+
+
+def _io_check(lib, st):
+    if st != 0:
+        raise LimaError(f"status {st}: {lib.lima_io_last_error().decode()}")
+
+
+def analyze_energy(pot, kin):
+    """LIMA's energy analysis (SimAnalysis::analyzeEnergy) of the per-logged-step potential and
+    kinetic energy series; see include/lima_io.h."""
+    lib = load_host()
+    p = np.ascontiguousarray(pot, np.float64)
+    k = np.ascontiguousarray(kin, np.float64)
+    if p.shape != k.shape or p.ndim != 1:
+        raise ValueError("pot and kin must be 1-D series of one length")
+    out = EnergyAnalysis()
+    dp = C.POINTER(C.c_double)
+    _io_check(lib, lib.lima_analyze_energy(p.ctypes.data_as(dp), k.ctypes.data_as(dp), len(p), C.byref(out)))
+    return out
+
+
+def write_trr(path, positions, box_nm, steps=None, host_order=False):
+    """Positions-only TRR trajectory (MDFiles::TrrFile::Dump): positions [frames, atoms, 3] nm."""
+    lib = load_host()
+    x = np.ascontiguousarray(positions, np.float32)
+    if x.ndim != 3 or x.shape[2] != 3:
+        raise ValueError("positions must be [frames, atoms, 3]")
+    b = np.ascontiguousarray(box_nm, np.float64)
+    st = None if steps is None else np.ascontiguousarray(steps, np.int64)
+    _io_check(lib, lib.lima_write_trr(str(path).encode(), x.shape[0], x.shape[1], b.ctypes.data_as(C.POINTER(C.c_double)),
+                                      x.ctypes.data_as(C.POINTER(C.c_float)),
+                                      None if st is None else st.ctypes.data_as(C.POINTER(C.c_int64)),
+                                      1 if host_order else 0))
 
 
 def default_params(dt_fs=2.0, n_steps=1000, **overrides):
@@ -314,6 +356,35 @@ class Engine:
 
     def pair_count(self):
         return int(self.lib.lima_get_pair_count(self._h))
+
+    def log_info(self):
+        i = LogInfo()
+        self._check(self.lib.lima_get_log_info(self._h, C.byref(i)))
+        return i
+
+    def read_log(self, first=None, n=None):
+        """Data-log frames (LIMA buffer layout, P = n_compounds * 32 + n_tinymols entries per frame):
+        dict of step [n], energy [n] (lima_energy), position [n, P, 3], pot_energy [n, P],
+        speed [n, P], force [n, P, 3].  Defaults: every frame still held."""
+        info = self.log_info()
+        if first is None:
+            first = info.first_held
+        if n is None:
+            n = info.frames_logged - first
+        P = info.particles_per_frame
+        out = {"step": np.zeros(max(n, 1), np.int64), "position": np.zeros((max(n, 1), P, 3), np.float32),
+               "pot_energy": np.zeros((max(n, 1), P), np.float32), "speed": np.zeros((max(n, 1), P), np.float32),
+               "force": np.zeros((max(n, 1), P, 3), np.float32)}
+        energies = (Energy * max(n, 1))()
+        lo = LogOut()
+        lo.step = out["step"].ctypes.data_as(C.POINTER(C.c_int64))
+        lo.energy = C.cast(energies, C.POINTER(Energy))
+        for k in ("position", "pot_energy", "speed", "force"):
+            setattr(lo, k, out[k].ctypes.data_as(C.POINTER(C.c_float)))
+        self._check(self.lib.lima_read_log(self._h, first, n, C.byref(lo)))
+        res = {k: v[:n] for k, v in out.items()}
+        res["energy"] = [energies[k] for k in range(n)]
+        return res
 
 
 class Batch:

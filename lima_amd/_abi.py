@@ -56,6 +56,7 @@ class SimParams(C.Structure):
         ("nlist_interval", C.c_int32),
         ("energy_interval", C.c_int32),
         ("launch_mode", C.c_int32),
+        ("log_capacity", C.c_int32),
     ]
 
 
@@ -129,3 +130,38 @@ class BatchTuning(C.Structure):
     def as_list(self):
         return [{"max_atoms": self.bucket_max_atoms[k], "threads": self.threads[k], "pair_unroll": self.pair_unroll[k],
                  "us_per_iter": round(self.us_per_iter[k], 2)} for k in range(self.n_buckets)]
+
+
+class LogInfo(C.Structure):
+    _fields_ = [
+        ("frames_logged", C.c_int64),
+        ("first_held", C.c_int64),
+        ("particles_per_frame", C.c_int64),
+        ("interval", C.c_int32),
+        ("capacity", C.c_int32),
+    ]
+
+
+class LogOut(C.Structure):
+    _fields_ = [
+        ("step", C.POINTER(C.c_int64)),
+        ("energy", C.POINTER(Energy)),
+        ("position", C.POINTER(C.c_float)),
+        ("pot_energy", C.POINTER(C.c_float)),
+        ("speed", C.POINTER(C.c_float)),
+        ("force", C.POINTER(C.c_float)),
+    ]
+
+
+class EnergyRow(C.Structure):
+    _fields_ = [("min", C.c_float), ("max", C.c_float), ("stddev", C.c_float), ("change", C.c_float)]
+
+
+class EnergyAnalysis(C.Structure):
+    _fields_ = [
+        ("n_entries", C.c_int64),
+        ("pot", EnergyRow), ("kin", EnergyRow), ("total", EnergyRow),
+        ("mean_energy", C.c_float),
+        ("energy_gradient", C.c_float),
+        ("variance_coefficient", C.c_float),
+    ]

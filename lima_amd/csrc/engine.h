@@ -200,6 +200,19 @@ struct Engine {
     int64_t pairs_last = 0;      // pairs inside the cutoff at the last energy evaluation
     double host_issue_ms = 0;    // host time spent issuing MD steps (excluding the final device wait)
     int64_t host_steps = 0;
+
+    // data log: a ring of log_cap frames written at every step % data_logging_interval == 0.  A
+    // frame holds, per entry of LIMA's buffer layout (compound c slot s at c*32 + s, tinymol t at
+    // nC*32 + t), (position [nm], potE) and (force, speed), plus the step's FP64 accumulators.
+    int log_cap = 0;
+    int64_t log_P = 0;             // entries per frame, nC*32 + nT
+    float4* log_pe = nullptr;      // [log_cap][log_P]
+    float4* log_fv = nullptr;      // [log_cap][log_P]
+    double* log_acc = nullptr;     // [log_cap][ACC_N]
+    int* log_index = nullptr;      // [N] buffer entry of each particle
+    int* log_key = nullptr;        // [N] global index of the particle's compound key particle, -1 for tinymols
+    int64_t log_frames = 0;        // frames written since create
+    std::vector<int64_t> log_step; // [log_cap] step of the frame held in each ring slot
 };
 
 #define HIP_CHECK(x)                                                                          \

@@ -205,6 +205,52 @@ __global__ void k_forces_global(int64_t N, int nslots, const int* __restrict__ s
     if ((threadIdx.x & 63) == 0) atomic_max_pos(fmax, m);
 }
 
+// One data-log frame (LogCompoundData / LogSolventData, EngineUtils.cuh:143-173): position after
+// the integration in LIMA's convention (compound particles: origo = the key particle's wrapped nm
+// node, Coord relative to it, so a compound stays whole; tinymols: wrapped into the box), potE and
+// force of the step's evaluation, speed after the step.  Written to the particle's entry of LIMA's
+// buffer layout.
+__global__ void k_log_frame(int64_t N, const int* __restrict__ lidx, const int* __restrict__ key, const uint4* __restrict__ w,
+                            const int4* __restrict__ img, const float4* __restrict__ v, const int* __restrict__ slot_of, int nslots,
+                            const float* __restrict__ fs, const float* __restrict__ fb, const float* __restrict__ es,
+                            const float* __restrict__ eb, long long Lqx, long long Lqy, long long Lqz,
+                            float4* __restrict__ pe, float4* __restrict__ fv) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= N) return;
+    const uint4 wp = w[p];
+    const int k = key[p];
+    const long long wpd[3] = {(long long)wp.x, (long long)wp.y, (long long)wp.z};
+    long long node[3], coord[3];
+    if (k >= 0) {
+        const uint4 wk = w[k];
+        const int4 ik = img[k], ip = img[p];
+        const long long wkd[3] = {(long long)wk.x, (long long)wk.y, (long long)wk.z};
+        const long long dimg[3] = {(long long)(ip.x - ik.x), (long long)(ip.y - ik.y), (long long)(ip.z - ik.z)};
+        const long long Lq[3] = {Lqx, Lqy, Lqz};
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            node[d] = wkd[d] / LIMA_NANO_TO_LIMA;
+            coord[d] = wpd[d] - wkd[d] + dimg[d] * Lq[d] + (wkd[d] - node[d] * LIMA_NANO_TO_LIMA);
+        }
+    } else {
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            node[d] = wpd[d] / LIMA_NANO_TO_LIMA;
+            coord[d] = wpd[d] - node[d] * LIMA_NANO_TO_LIMA;
+        }
+    }
+    // FP32 origo + Coord * 1e-8 and |v| (the multiply-adds contract to FMAs, as in LIMA's build)
+    float x[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) x[d] = (float)node[d] + (float)(int)coord[d] * LIMA_LIMA_TO_NANO_F;
+    const int s = slot_of[p];
+    const float4 vp = v[p];
+    const float speed = sqrtf(vp.x * vp.x + vp.y * vp.y + vp.z * vp.z);
+    const int64_t o = lidx[p];
+    pe[o] = make_float4(x[0], x[1], x[2], es[s] + eb[s]);
+    fv[o] = make_float4(fs[s] + fb[s], fs[nslots + s] + fb[nslots + s], fs[2 * nslots + s] + fb[2 * nslots + s], speed);
+}
+
 __global__ void k_zero_d(double* p, int n) {
     if ((int)threadIdx.x < n) p[threadIdx.x] = 0.0;
 }
@@ -344,10 +390,8 @@ void evaluate(Engine& e, const uint4* w, const int4* img, bool energy, bool time
     if (ev) HIP_CHECK(hipEventRecord(ev[5], s));
 }
 
-void fill_energy(Engine& e, bool with_kinetic) {
-    double acc[ACC_N];
-    HIP_CHECK(hipMemcpy(acc, e.acc, sizeof(acc), hipMemcpyDeviceToHost));
-    lima_energy& E = e.energy;
+// lima_energy from the FP64 accumulators of one evaluation (E.kinetic kept when acc has none).
+void energy_from_acc(const Engine& e, const double* acc, bool with_kinetic, lima_energy& E) {
     E.lj = acc[ACC_LJ];
     E.coulomb_sr = acc[ACC_COUL];
     E.bonded = acc[ACC_BOND];
@@ -358,7 +402,25 @@ void fill_energy(Engine& e, bool with_kinetic) {
     E.potential_lima = E.lj + E.coulomb_sr + E.bonded + E.pme_recip + E.pme_self_lima;
     if (with_kinetic) E.kinetic = acc[ACC_KIN];
     E.total = E.potential + E.kinetic;
+}
+
+void fill_energy(Engine& e, bool with_kinetic) {
+    double acc[ACC_N];
+    HIP_CHECK(hipMemcpy(acc, e.acc, sizeof(acc), hipMemcpyDeviceToHost));
+    energy_from_acc(e, acc, with_kinetic, e.energy);
     e.pairs_last = (int64_t)acc[ACC_PAIRS];
+}
+
+// Writes the current step's frame into the data-log ring (after the integration and the energy
+// reduction of a logged step, on the main stream).
+void log_frame(Engine& e) {
+    const int64_t slot = e.log_frames % e.log_cap;
+    hipLaunchKernelGGL(k_log_frame, dim3(nblk(e.N, 256)), dim3(256), 0, e.s_main, e.N, e.log_index, e.log_key, e.p.w, e.p.img,
+                       e.p.v, e.nl.slot_of, slots_max(e), e.fs, e.fb, e.es, e.eb, (long long)e.Lq[0], (long long)e.Lq[1],
+                       (long long)e.Lq[2], e.log_pe + slot * e.log_P, e.log_fv + slot * e.log_P);
+    HIP_CHECK(hipMemcpyAsync(e.log_acc + slot * ACC_N, e.acc, ACC_N * sizeof(double), hipMemcpyDeviceToDevice, e.s_main));
+    e.log_step[slot] = e.step;
+    e.log_frames++;
 }
 
 void check_finite(Engine& e) {
@@ -457,7 +519,11 @@ void md_steps(Engine& e, int64_t n) {
                 e.tacc.nlist_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
             }
         }
-        const bool energy = (e.prm.energy_interval > 0 && e.step % e.prm.energy_interval == 0) || k == n - 1;
+        // energy steps: every energy_interval-th step and the last step of the call (their totals are
+        // read back and checked), and the logged steps (totals kept on the device with the frame)
+        const bool energy_step = (e.prm.energy_interval > 0 && e.step % e.prm.energy_interval == 0) || k == n - 1;
+        const bool logged = e.log_cap > 0 && e.step % e.prm.data_logging_interval == 0;
+        const bool energy = energy_step || logged;
         if (!energy && !e.timing && !(e.prm.launch_mode & LAUNCH_EAGER)) {
             // the graph holds the evaluation without k_pack and an integrator that packs for the next
             // step; after a list rebuild (or an unpacked eager step) k_pack runs first
@@ -469,11 +535,12 @@ void md_steps(Engine& e, int64_t n) {
         } else {
             evaluate(e, e.p.w, e.p.img, energy, e.timing, !e.packed);
             // the integrator packs for the next step when that step keeps the list; the last step
-            // leaves the force slots intact for lima_get_state
-            const bool pack_next = !e.timing && k + 1 < n && e.steps_since_build + 1 < interval;
+            // leaves the force slots intact for lima_get_state, a logged step for its frame
+            const bool pack_next = !e.timing && !logged && k + 1 < n && e.steps_since_build + 1 < interval;
             integrate(e, energy, pack_next);
         }
         if (energy) launch_reduce_energy(e, true, e.s_main);
+        if (logged) log_frame(e);
         if (e.timing) {
             HIP_CHECK(hipEventRecord(tslot(e)[6], e.s_main));
             if (++e.tpos == Engine::TRING) timing_flush(e);
@@ -481,7 +548,7 @@ void md_steps(Engine& e, int64_t n) {
         e.step++;
         e.steps_since_build++;
         if (temperature_due(e)) measure_temperature(e);
-        if (energy) {
+        if (energy_step) {
             fill_energy(e, true);
             check_finite(e);
         }
@@ -502,6 +569,8 @@ void create(Engine& e, const lima_box_desc* b, const lima_simparams* p) {
         throw LimaFail(LIMA_ERR_BAD_INPUT, "launch_mode must be a combination of LIMA_LAUNCH_EAGER and LIMA_LAUNCH_ONE_STREAM");
     if (p->apply_thermostat && p->steps_per_temperature_measurement <= 0)
         throw LimaFail(LIMA_ERR_BAD_INPUT, "the thermostat needs steps_per_temperature_measurement > 0");
+    if (p->log_capacity < 0 || (p->log_capacity > 0 && p->data_logging_interval <= 0))
+        throw LimaFail(LIMA_ERR_BAD_INPUT, "the data log needs log_capacity >= 0 and, when enabled, data_logging_interval > 0");
     if (p->bc_select != LIMA_BC_PBC) throw LimaFail(LIMA_ERR_UNSUPPORTED, "only periodic boundary conditions are implemented");
     if (!(p->cutoff_nm > 0.f)) throw LimaFail(LIMA_ERR_BAD_INPUT, "cutoff must be positive");
     e.nC = b->n_compounds;
@@ -763,6 +832,27 @@ void create(Engine& e, const lima_box_desc* b, const lima_simparams* p) {
     HIP_CHECK(hipMemset(e.epart, 0, e.epart_cap * ACC_N * sizeof(double)));
     HIP_CHECK(hipMalloc(&e.lstats, 2 * sizeof(int)));
     HIP_CHECK(hipMemset(e.lstats, 0, 2 * sizeof(int)));
+    if (p->log_capacity > 0) {   // data log ring
+        e.log_cap = p->log_capacity;
+        e.log_P = (int64_t)e.nC * LIMA_MAX_COMPOUND_PARTICLES + e.nT;
+        const int64_t bytes = (int64_t)e.log_cap * e.log_P * 2 * (int64_t)sizeof(float4);
+        if (bytes > (int64_t)64 << 30) throw LimaFail(LIMA_ERR_BAD_INPUT, "the data log would exceed 64 GiB; lower log_capacity");
+        std::vector<int> lidx(N), lkey(N, -1);
+        for (int c = 0; c < e.nC; c++)
+            for (int64_t i = e.h_first[c]; i < e.h_first[c + 1]; i++) {
+                lidx[i] = c * LIMA_MAX_COMPOUND_PARTICLES + (int)(i - e.h_first[c]);
+                lkey[i] = (int)(e.h_first[c] + e.h_key[c]);
+            }
+        for (int t = 0; t < e.nT; t++) lidx[e.h_first[e.nC] + t] = e.nC * LIMA_MAX_COMPOUND_PARTICLES + t;
+        e.log_index = dalloc_copy(lidx);
+        e.log_key = dalloc_copy(lkey);
+        HIP_CHECK(hipMalloc(&e.log_pe, bytes / 2));
+        HIP_CHECK(hipMalloc(&e.log_fv, bytes / 2));
+        HIP_CHECK(hipMemset(e.log_pe, 0, bytes / 2));
+        HIP_CHECK(hipMemset(e.log_fv, 0, bytes / 2));
+        HIP_CHECK(hipMalloc(&e.log_acc, (size_t)e.log_cap * ACC_N * sizeof(double)));
+        e.log_step.assign(e.log_cap, -1);
+    }
     nlist_build(e, e.p.w);
     HIP_CHECK(hipStreamSynchronize(e.s_main));
 }
@@ -773,7 +863,8 @@ void destroy(Engine& e) {
     pme_free(e);
     void* ptrs[] = {e.p.w, e.p.img, e.p.v, e.p.fprev, e.p.prm, e.p.comp, e.w_build, e.img_build, e.dmax2, e.lstats, e.thermo, e.dpart, e.epart, e.w_try, e.img_try, e.f_acc, e.sd, e.first,
                     e.bmask, e.intra_items, e.intra_lut, e.bd.bid, e.bd.bp, e.bd.aid, e.bd.ap, e.bd.did, e.bd.dp,
-                    e.bd.iid, e.bd.ip, e.bd.pid, e.bd.pp, e.bd.lane_code, e.bd.lane_atom, e.coul_tab, e.acc, e.fmax};
+                    e.bd.iid, e.bd.ip, e.bd.pid, e.bd.pp, e.bd.lane_code, e.bd.lane_atom, e.coul_tab, e.acc, e.fmax,
+                    e.log_pe, e.log_fv, e.log_acc, e.log_index, e.log_key};
     for (void* p : ptrs) if (p) (void)hipFree(p);
     for (auto& ev : e.tev) if (ev) (void)hipEventDestroy(ev);
     if (e.step_graph) (void)hipGraphExecDestroy(e.step_graph);
@@ -971,6 +1062,53 @@ lima_status lima_get_state(lima_engine* h, lima_state_out* out) {
                 if (out->tinymol_force) out->tinymol_force[t * 3 + d] = ff[d];
             }
             if (out->tinymol_pot_energy) out->tinymol_pot_energy[t] = f[i].w;
+        }
+    });
+}
+
+lima_status lima_get_log_info(lima_engine* h, lima_log_info* out) {
+    if (!h || !out) { set_err("null argument"); return LIMA_ERR_BAD_INPUT; }
+    const Engine& e = h->e;
+    out->frames_logged = e.log_frames;
+    out->first_held = e.log_frames - std::min<int64_t>(e.log_frames, e.log_cap);
+    out->particles_per_frame = (int64_t)e.nC * LIMA_MAX_COMPOUND_PARTICLES + e.nT;
+    out->interval = e.prm.data_logging_interval;
+    out->capacity = e.log_cap;
+    return LIMA_OK;
+}
+
+lima_status lima_read_log(lima_engine* h, int64_t first, int64_t n, lima_log_out* out) {
+    if (!h || !out) { set_err("null argument"); return LIMA_ERR_BAD_INPUT; }
+    return guarded([&] {
+        Engine& e = h->e;
+        const int64_t held0 = e.log_frames - std::min<int64_t>(e.log_frames, e.log_cap);
+        if (n < 0 || first < held0 || first + n > e.log_frames)
+            throw LimaFail(LIMA_ERR_BAD_INPUT, "frames " + std::to_string(first) + " + " + std::to_string(n) + " are not held (held: " +
+                                                   std::to_string(held0) + " .. " + std::to_string(e.log_frames - 1) + ")");
+        HIP_CHECK(hipStreamSynchronize(e.s_main));
+        const int64_t P = e.log_P;
+        std::vector<float4> pe(P), fv(P);
+        for (int64_t k = 0; k < n; k++) {
+            const int64_t slot = (first + k) % e.log_cap;
+            if (out->step) out->step[k] = e.log_step[slot];
+            if (out->energy) {
+                double acc[ACC_N];
+                HIP_CHECK(hipMemcpy(acc, e.log_acc + slot * ACC_N, sizeof(acc), hipMemcpyDeviceToHost));
+                lima_energy E{};
+                energy_from_acc(e, acc, true, E);
+                out->energy[k] = E;
+            }
+            if (out->position || out->pot_energy)
+                HIP_CHECK(hipMemcpy(pe.data(), e.log_pe + slot * P, P * sizeof(float4), hipMemcpyDeviceToHost));
+            if (out->force || out->speed)
+                HIP_CHECK(hipMemcpy(fv.data(), e.log_fv + slot * P, P * sizeof(float4), hipMemcpyDeviceToHost));
+            for (int64_t i = 0; i < P; i++) {
+                const size_t o = (size_t)(k * P + i);
+                if (out->position) { out->position[3 * o] = pe[i].x; out->position[3 * o + 1] = pe[i].y; out->position[3 * o + 2] = pe[i].z; }
+                if (out->pot_energy) out->pot_energy[o] = pe[i].w;
+                if (out->force) { out->force[3 * o] = fv[i].x; out->force[3 * o + 1] = fv[i].y; out->force[3 * o + 2] = fv[i].z; }
+                if (out->speed) out->speed[o] = fv[i].w;
+            }
         }
     });
 }
