@@ -15,7 +15,7 @@ from pathlib import Path
 
 import numpy as np
 
-from ._abi import BoxDesc, SimParams, Energy, RunStatus, StateOut, EmResult, Timings, BatchTuning, LogInfo, LogOut, EnergyAnalysis
+from ._abi import BoxDesc, SimParams, Energy, RunStatus, StateOut, EmResult, Timings, BatchTuning, LogInfo, LogOut, EnergyAnalysis, SystemInfo
 
 LIB_DIR = Path(__file__).resolve().parent / "lib"
 REPO_ROOT = Path(__file__).resolve().parent.parent
@@ -59,6 +59,19 @@ def load_host():
                                        C.POINTER(C.c_int64), C.c_int32]
         lib.lima_write_trr.restype = C.c_int
         lib.lima_io_last_error.restype = C.c_char_p
+        lib.lima_load_system.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(C.c_char_p), C.c_int32, C.c_int32,
+                                         C.POINTER(C.c_void_p)]
+        lib.lima_load_system.restype = C.c_int
+        lib.lima_system_free.argtypes = [C.c_void_p]
+        lib.lima_system_get_info.argtypes = [C.c_void_p, C.POINTER(SystemInfo)]
+        lib.lima_system_get_info.restype = C.c_int
+        lib.lima_system_make_box.argtypes = [C.c_void_p, C.POINTER(C.c_void_p)]
+        lib.lima_system_make_box.restype = C.c_int
+        lib.lima_system_layout.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
+        lib.lima_system_layout.restype = C.c_int
+        lib.lima_system_write_gro.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.c_char_p]
+        lib.lima_system_write_gro.restype = C.c_int
+        lib.lima_topo_last_error.restype = C.c_char_p
         _host = lib
     return _host
 
@@ -157,6 +170,59 @@ def write_trr(path, positions, box_nm, steps=None, host_order=False):
                                       x.ctypes.data_as(C.POINTER(C.c_float)),
                                       None if st is None else st.ctypes.data_as(C.POINTER(C.c_int64)),
                                       1 if host_order else 0))
+
+
+class System:
+    """A .gro + .top system built into LIMA compounds (include/lima_topo.h)."""
+
+    def __init__(self, gro, top, include_dirs=(), keep_small_molecules=False):
+        self.lib = load_host()
+        dirs = [str(d).encode() for d in include_dirs]
+        arr = (C.c_char_p * max(len(dirs), 1))(*dirs)
+        h = C.c_void_p()
+        st = self.lib.lima_load_system(str(gro).encode(), str(top).encode(), arr, len(dirs),
+                                       1 if keep_small_molecules else 0, C.byref(h))
+        if st != 0:
+            raise LimaError(f"status {st}: {self.lib.lima_topo_last_error().decode()}")
+        self._h = h
+
+    def _check(self, st):
+        if st != 0:
+            raise LimaError(f"status {st}: {self.lib.lima_topo_last_error().decode()}")
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                self.lib.lima_system_free(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def info(self):
+        i = SystemInfo()
+        self._check(self.lib.lima_system_get_info(self._h, C.byref(i)))
+        return i
+
+    def box(self):
+        """A new Box (lima_box) holding the system."""
+        h = C.c_void_p()
+        self._check(self.lib.lima_system_make_box(self._h, C.byref(h)))
+        return Box(h.value)
+
+    def layout(self):
+        """.gro atom index of every entry of LIMA's buffer layout (-1 for unused slots)."""
+        i = self.info()
+        out = np.zeros(i.n_compounds * 32 + i.n_tinymols, np.int32)
+        self._check(self.lib.lima_system_layout(self._h, out.ctypes.data_as(C.POINTER(C.c_int32))))
+        return out
+
+    def write_gro(self, positions, path):
+        x = np.ascontiguousarray(positions, np.float32)
+        self._check(self.lib.lima_system_write_gro(self._h, x.ctypes.data_as(C.POINTER(C.c_float)), str(path).encode()))
+
+
+def load_system(gro, top, include_dirs=(), keep_small_molecules=False):
+    return System(gro, top, include_dirs, keep_small_molecules)
 
 
 def default_params(dt_fs=2.0, n_steps=1000, **overrides):

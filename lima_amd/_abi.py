@@ -165,3 +165,14 @@ class EnergyAnalysis(C.Structure):
         ("energy_gradient", C.c_float),
         ("variance_coefficient", C.c_float),
     ]
+
+
+class SystemInfo(C.Structure):
+    _fields_ = [
+        ("n_gro_atoms", C.c_int64),
+        ("n_molecules", C.c_int32),
+        ("n_compounds", C.c_int32),
+        ("n_tinymols", C.c_int32),
+        ("n_bonds", C.c_int32), ("n_angles", C.c_int32), ("n_dihedrals", C.c_int32),
+        ("n_impropers", C.c_int32), ("n_pairs", C.c_int32),
+    ]

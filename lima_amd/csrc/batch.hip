@@ -140,11 +140,12 @@ __device__ __forceinline__ int min_image(int d, int Lq) {   // representative of
 
 // x_b - x_a [nm] of bonded particles: the nearest image (LIMA's hyperpos result for particles a
 // few bonds apart; the Coords of different compounds may differ by whole boxes)
-__device__ __forceinline__ v3 bdisp(const int4* u, int a, int b, const SysMeta& m) {
+__device__ __forceinline__ Dq bdispq(const int4* u, int a, int b, const SysMeta& m) {
     const int4 ua = u[a], ub = u[b];
-    return {(float)min_image(ub.x - ua.x, m.Lq[0]) * LIMA_LIMA_TO_NANO_F, (float)min_image(ub.y - ua.y, m.Lq[1]) * LIMA_LIMA_TO_NANO_F,
-            (float)min_image(ub.z - ua.z, m.Lq[2]) * LIMA_LIMA_TO_NANO_F};
+    return {(long long)min_image(ub.x - ua.x, m.Lq[0]), (long long)min_image(ub.y - ua.y, m.Lq[1]),
+            (long long)min_image(ub.z - ua.z, m.Lq[2])};
 }
+__device__ __forceinline__ v3 bdisp(const int4* u, int a, int b, const SysMeta& m) { return dq_nm(bdispq(u, a, b, m)); }
 
 // One cutoff pair computed once: force on i += d S, on j -= d S, the whole pair energy.
 // pj = (q, sigma/2, sqrt(24 eps)) of j.
@@ -252,12 +253,12 @@ __device__ __forceinline__ void bonded_atom(const BatchArgs& a, const SysMeta& m
         if (type == T_BOND) {
             const int2 id = a.bid[t];
             v3 fa;
-            e = term_bond(bdisp(u, id.y, id.x, m), a.bp[t], &fa);
+            e = term_bond(bdispq(u, id.y, id.x, m), a.bp[t], &fa);
             f = role == 0 ? fa : -fa;
         } else if (type == T_ANGLE) {
             const int4 id = a.aid[t];
             v3 fl, fr;
-            e = term_angle(bdisp(u, id.y, id.x, m), bdisp(u, id.y, id.z, m), bdisp(u, id.z, id.x, m), a.ap[t], &fl, &fr);
+            e = term_angle(bdisp(u, id.y, id.x, m), bdisp(u, id.y, id.z, m), bdispq(u, id.z, id.x, m), a.ap[t], &fl, &fr);
             f = role == 0 ? fl : (role == 2 ? fr : -(fl + fr));
         } else if (type == T_DIH) {
             const int4 id = a.did[t];

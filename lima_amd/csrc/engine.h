@@ -98,6 +98,7 @@ struct Bonded {
     int n_lanes = 0;             // multiple of 64
     uint32_t* lane_code = nullptr;   // inc_code, or LANE_PAD
     int* lane_atom = nullptr;
+    int multi_lane0 = 0;     // first lane of the workgroups that each hold one atom with 65..256 incidences
 };
 constexpr uint32_t LANE_PAD = 0xffffffffu;
 

@@ -776,7 +776,7 @@ void create(Engine& e, const lima_box_desc* b, const lima_simparams* p) {
             for (auto& t : pid) { count(t.x); count(t.y); }
             std::vector<int> off(N + 1, 0);
             for (int64_t i = 0; i < N; i++) {
-                if (cnt[i] > 64) throw LimaFail(LIMA_ERR_UNSUPPORTED, "a particle belongs to more than 64 bonded terms");
+                if (cnt[i] > 256) throw LimaFail(LIMA_ERR_UNSUPPORTED, "a particle belongs to more than 256 bonded terms");
                 off[i + 1] = off[i] + cnt[i];
             }
             std::vector<uint32_t> inc(off[N]);
@@ -791,20 +791,32 @@ void create(Engine& e, const lima_box_desc* b, const lima_simparams* p) {
                 add(iid[k].x, T_IMP, 0, k); add(iid[k].y, T_IMP, 1, k); add(iid[k].z, T_IMP, 2, k); add(iid[k].w, T_IMP, 3, k);
             }
             for (int k = 0; k < b->n_pairs; k++) { add(pid[k].x, T_PAIR, 0, k); add(pid[k].y, T_PAIR, 1, k); }
-            // lanes: an atom's incidences in consecutive lanes of one wave (runs of at most 64)
+            // lanes: the incidences of an atom with at most 64 of them in consecutive lanes of one
+            // wave; after them, each atom with 65..256 incidences in a 256-lane workgroup of its own,
+            // one run of at most 64 per wave starting at the wave's first lane
             std::vector<uint32_t> lcode;
             std::vector<int> latom;
+            auto pad_to = [&](size_t m) {
+                while (lcode.size() % m) { lcode.push_back(LANE_PAD); latom.push_back((int)((lcode.size() - 1) & 63) << 24); }
+            };
             for (int64_t i = 0; i < N; i++) {
+                const int run = cnt[i];
+                if (run == 0 || run > 64) continue;
+                if ((int)(lcode.size() & 63) + run > 64) pad_to(64);
+                const int last = (int)((lcode.size() + run - 1) & 63);
+                for (int r = 0; r < run; r++) { lcode.push_back(inc[off[i] + r]); latom.push_back((int)i | (last << 24)); }
+            }
+            pad_to(256);
+            e.bd.multi_lane0 = (int)lcode.size();
+            for (int64_t i = 0; i < N; i++) {
+                if (cnt[i] <= 64) continue;
                 for (int q = off[i]; q < off[i + 1]; q += 64) {
                     const int run = std::min(64, off[i + 1] - q);
-                    const int inwave = (int)(lcode.size() & 63);
-                    if (inwave + run > 64)   // pad to the next wave
-                        while (lcode.size() & 63) { lcode.push_back(LANE_PAD); latom.push_back((int)((lcode.size() - 1) & 63) << 24); }
-                    const int last = (int)((lcode.size() + run - 1) & 63);
-                    for (int r = 0; r < run; r++) { lcode.push_back(inc[q + r]); latom.push_back((int)i | (last << 24)); }
+                    for (int r = 0; r < run; r++) { lcode.push_back(inc[q + r]); latom.push_back((int)i | ((run - 1) << 24)); }
+                    pad_to(64);
                 }
+                pad_to(256);
             }
-            while (lcode.size() & 63) { lcode.push_back(LANE_PAD); latom.push_back((int)((lcode.size() - 1) & 63) << 24); }
             e.bd.n_lanes = (int)lcode.size();
             e.bd.lane_code = dalloc_copy(lcode);
             e.bd.lane_atom = dalloc_copy(latom);

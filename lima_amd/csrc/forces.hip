@@ -329,6 +329,7 @@ struct BondArgs {
     const int4* iid; const float2* ip;
     const int2* pid; const float2* pp;
     int n_lanes;
+    int multi_lane0;     // workgroups from this lane on hold one atom each, one run per wave
     const uint32_t* lane_code;
     const int* lane_atom;
     const uint4* w;
@@ -344,7 +345,7 @@ struct BondArgs {
 // x_to - x_from [nm] of two bonded particles.  Bonded displacements are far shorter than half the
 // box, so the minimum image of the wrapped Coord difference is the same integer as the unwrapped
 // difference LIMA uses (no image counters needed).
-__device__ __forceinline__ v3 bdisp(const BondArgs& a, int from, int to) {
+__device__ __forceinline__ Dq bdispq(const BondArgs& a, int from, int to) {
     const uint4 p0 = a.w[from], p1 = a.w[to];
     long long d[3] = {(long long)p1.x - (long long)p0.x, (long long)p1.y - (long long)p0.y, (long long)p1.z - (long long)p0.z};
 #pragma unroll
@@ -352,8 +353,9 @@ __device__ __forceinline__ v3 bdisp(const BondArgs& a, int from, int to) {
         if (2 * d[k] > a.Lq[k]) d[k] -= a.Lq[k];
         else if (2 * d[k] < -a.Lq[k]) d[k] += a.Lq[k];
     }
-    return {(float)d[0] * LIMA_LIMA_TO_NANO_F, (float)d[1] * LIMA_LIMA_TO_NANO_F, (float)d[2] * LIMA_LIMA_TO_NANO_F};
+    return {d[0], d[1], d[2]};
 }
+__device__ __forceinline__ v3 bdisp(const BondArgs& a, int from, int to) { return dq_nm(bdispq(a, from, to)); }
 
 // Bonded terms, one lane per (atom, term) incidence.  An atom's incidences occupy consecutive
 // lanes of one wave (at most 64, checked at creation); each lane evaluates its term
@@ -382,13 +384,13 @@ __device__ __forceinline__ void bonded_lanes(const BondArgs& a, int blk) {
         if (type == T_BOND) {   // calcSinglebondForces :17-51
             const int2 id = a.bid[t];
             v3 fa;
-            e = term_bond(bdisp(a, id.y, id.x), a.bp[t], &fa);
+            e = term_bond(bdispq(a, id.y, id.x), a.bp[t], &fa);
             f = role == 0 ? fa : -fa;
             nat = 2;
         } else if (type == T_ANGLE) {   // calcAnglebondForces :53-115, harmonic in theta + Urey-Bradley
             const int4 id = a.aid[t];
             v3 fl, fr;
-            e = term_angle(bdisp(a, id.y, id.x), bdisp(a, id.y, id.z), bdisp(a, id.z, id.x), a.ap[t], &fl, &fr);
+            e = term_angle(bdisp(a, id.y, id.x), bdisp(a, id.y, id.z), bdispq(a, id.z, id.x), a.ap[t], &fl, &fr);
             f = role == 0 ? fl : (role == 2 ? fr : -(fl + fr));
             nat = 3;
         } else if (type == T_DIH) {   // calcDihedralbondForces :119-210, phi_L = -atan2(sin, cos)
@@ -422,13 +424,29 @@ __device__ __forceinline__ void bonded_lanes(const BondArgs& a, int blk) {
             if (ENERGY) eshare += oe;
         }
     }
-    const int prev = __shfl_up(la, 1, 64);
-    if (code != LANE_PAD && (lane == 0 || prev != la)) {   // the run's first lane: the atom's only run
-        const int s = a.slot_of[atom];
-        a.fb[s] = f.x;
-        a.fb[a.nslots + s] = f.y;
-        a.fb[2 * a.nslots + s] = f.z;
-        if (ENERGY) a.eb[s] = eshare;
+    if (blk * (int)blockDim.x < a.multi_lane0) {   // workgroup-uniform: every atom in one run
+        const int prev = __shfl_up(la, 1, 64);
+        if (code != LANE_PAD && (lane == 0 || prev != la)) {   // the run's first lane
+            const int s = a.slot_of[atom];
+            a.fb[s] = f.x;
+            a.fb[a.nslots + s] = f.y;
+            a.fb[2 * a.nslots + s] = f.z;
+            if (ENERGY) a.eb[s] = eshare;
+        }
+    } else {   // one atom, its runs starting at the waves' first lanes: the run sums meet in LDS
+        __shared__ float4 runs[4];
+        const int w = threadIdx.x >> 6;
+        if (lane == 0) runs[w] = code != LANE_PAD ? make_float4(f.x, f.y, f.z, eshare) : make_float4(0.f, 0.f, 0.f, 0.f);
+        __syncthreads();
+        if (threadIdx.x == 0 && code != LANE_PAD) {
+            float4 t = runs[0];
+            for (int r = 1; r < 4; r++) { t.x += runs[r].x; t.y += runs[r].y; t.z += runs[r].z; t.w += runs[r].w; }
+            const int s = a.slot_of[atom];
+            a.fb[s] = t.x;
+            a.fb[a.nslots + s] = t.y;
+            a.fb[2 * a.nslots + s] = t.z;
+            if (ENERGY) a.eb[s] = t.w;
+        }
     }
     if (ENERGY) {
         const double tot = wave_sum_d(role == 0 ? (double)e : 0.0);
@@ -540,7 +558,7 @@ BondArgs bond_args(Engine& e, const uint4* w, const int4* img) {
     a.pc = e.pc;
     a.bid = e.bd.bid; a.bp = e.bd.bp; a.aid = e.bd.aid; a.ap = e.bd.ap; a.did = e.bd.did; a.dp = e.bd.dp;
     a.iid = e.bd.iid; a.ip = e.bd.ip; a.pid = e.bd.pid; a.pp = e.bd.pp;
-    a.n_lanes = e.bd.n_lanes; a.lane_code = e.bd.lane_code; a.lane_atom = e.bd.lane_atom;
+    a.n_lanes = e.bd.n_lanes; a.multi_lane0 = e.bd.multi_lane0; a.lane_code = e.bd.lane_code; a.lane_atom = e.bd.lane_atom;
     a.w = w; a.img = img; a.slot_of = e.nl.slot_of; a.fs = e.fs; a.es = e.es; a.fb = e.fb; a.eb = e.eb;
     a.epart = e.epart + (int64_t)e.ereg.bond * ACC_N;
     return a;

@@ -26,21 +26,47 @@ __device__ __forceinline__ v3 cross(v3 a, v3 b) { return {a.y * b.z - a.z * b.y,
 __device__ __forceinline__ float len(v3 a) { return sqrtf(dot(a, a)); }
 __device__ __forceinline__ v3 nrm(v3 a) { return a * (1.0f / len(a)); }
 
+// A displacement in integer Coord units (1e-8 nm), exact.
+struct Dq {
+    long long x, y, z;
+};
+__device__ __forceinline__ v3 dq_nm(const Dq& d) {
+    return {(float)d.x * LIMA_LIMA_TO_NANO_F, (float)d.y * LIMA_LIMA_TO_NANO_F, (float)d.z * LIMA_LIMA_TO_NANO_F};
+}
+
+// |d| - r0 in Coord units, and |d| (*len_q), for a bond-like length close to its rest length r0 [nm].
+// Rounding |d| to FP32 costs about 6e-8 |d|, i.e. ~1e-5 of the deviation when a bond sits 1e-3 nm
+// from r0.  Here |d|^2 is summed exactly in integers and split into two floats, the square root
+// is corrected by its exact FMA residual, and r0 * 1e8 is an exact two-float product (1e8 is a
+// float), so the deviation keeps FP32 precision of its own size.
+__device__ __forceinline__ float length_deviation_q(const Dq& d, float r0_nm, float* len_q) {
+#pragma clang fp contract(off)
+    const long long r2 = d.x * d.x + d.y * d.y + d.z * d.z;
+    const float hi = (float)r2, lo = (float)(r2 - (long long)hi);
+    const float rh = sqrtf(hi);
+    const float rl = (fmaf(-rh, rh, hi) + lo) / (2.f * rh);
+    const float bh = r0_nm * 1e8f, bl = fmaf(r0_nm, 1e8f, -bh);
+    *len_q = rh;
+    return (rh - bh) + (rl - bl);
+}
+
 // ---- bonded terms (ForceComputations.cuh).  Each returns the term energy [J/mol] and the
 // forces [J/mol/nm] on its atoms; arguments are displacements in nm. ----
 
-// calcSinglebondForces, ForceComputations.cuh:17-51.  d = x_a - x_b; p = (b0, kb).
+// calcSinglebondForces, ForceComputations.cuh:17-51.  d = x_a - x_b in Coord units; p = (b0, kb).
 // Force on a is *fa, on b -*fa.
-__device__ __forceinline__ float term_bond(v3 d, float2 p, v3* fa) {
-    const float r = len(d), err = r - p.x;
-    *fa = d * (-p.y * err / r);
+__device__ __forceinline__ float term_bond(const Dq& d, float2 p, v3* fa) {
+    float rq;
+    const float err = length_deviation_q(d, p.x, &rq) * LIMA_LIMA_TO_NANO_F;
+    const float s = -p.y * err / rq;   // force along the unit vector d / |d|
+    *fa = {(float)d.x * s, (float)d.y * s, (float)d.z * s};
     return 0.5f * p.y * err * err;
 }
 
 // calcAnglebondForces, ForceComputations.cuh:53-115: harmonic in theta plus Urey-Bradley on r13.
-// r1 = x_l - x_m, r2 = x_r - x_m, d13 = x_l - x_r; p = (theta0, ktheta, ub0, kub).
+// r1 = x_l - x_m, r2 = x_r - x_m [nm], d13 = x_l - x_r [Coord units]; p = (theta0, ktheta, ub0, kub).
 // Forces on l and r; the middle atom takes -(fl + fr).
-__device__ __forceinline__ float term_angle(v3 r1, v3 r2, v3 d13, float4 p, v3* fl, v3* fr) {
+__device__ __forceinline__ float term_angle(v3 r1, v3 r2, const Dq& d13, float4 p, v3* fl, v3* fr) {
     const float l1 = len(r1), l2 = len(r2);
     const v3 v1 = r1 * (1.f / l1), v2 = r2 * (1.f / l2);
     const float c = fminf(1.f, fmaxf(-1.f, dot(v1, v2)));
@@ -51,8 +77,10 @@ __device__ __forceinline__ float term_angle(v3 r1, v3 r2, v3 d13, float4 p, v3* 
     v3 b = (v1 - v2 * c) * (tq / (l2 * s));
     float e = 0.5f * p.y * err * err;
     if (p.w != 0.f) {
-        const float r13 = len(d13), eu = r13 - p.z;
-        const v3 fu = d13 * (-p.w * eu / r13);
+        float r13q;
+        const float eu = length_deviation_q(d13, p.z, &r13q) * LIMA_LIMA_TO_NANO_F;
+        const float su = -p.w * eu / r13q;
+        const v3 fu = {(float)d13.x * su, (float)d13.y * su, (float)d13.z * su};
         a = a + fu;
         b = b - fu;
         e += 0.5f * p.w * eu * eu;
@@ -89,7 +117,9 @@ __device__ __forceinline__ float term_improper(v3 pj, v3 pk, v3 pl, float2 p, v3
     const v3 pi = {0.f, 0.f, 0.f};
     const v3 ij = nrm(pj - pi), ik = nrm(pk - pi), il = nrm(pl - pi), lj = nrm(pj - pl), lk = nrm(pk - pl);
     const v3 n1 = nrm(cross(ij, ik)), n2 = nrm(cross(lj, lk));
-    float ang = acosf(fminf(1.f, fmaxf(-1.f, dot(n1, n2))));
+    // the angle between the plane normals (LIMA: acos of their dot product), as atan2 of sine and
+    // cosine: acos loses ~1e-3 of a 0.01 rad angle in FP32, and impropers sit near their minimum
+    float ang = atan2f(len(cross(n1, n2)), dot(n1, n2));
     if (dot(n1, il) > 0.f) ang = -ang;
     const float err = ang - p.x;
     const float tq = p.y * err;

@@ -9,6 +9,7 @@
 // constants kJ -> J (Forcefield.cpp:342-347, Bodies.cpp:9-47) and dihedral k halved
 // (DihedralBond::Parameters::CreateFromCharmm, Bodies.cpp:36-38).
 #include "../../include/lima_gen.h"
+#include "lima_box_impl.h"
 
 #include <algorithm>
 #include <array>
@@ -241,18 +242,6 @@ struct Builder {
 };
 
 }  // namespace
-
-struct lima_box {
-    double box[3];
-    int32_t nC = 0, nT = 0;
-    std::vector<int32_t> cn, origo, key, coord;
-    std::vector<float> vel, q, sig, eps, mass;
-    std::vector<uint32_t> lut;
-    std::vector<int32_t> tblock, tcoord;
-    std::vector<float> tvel, tsig, teps, tmass;
-    std::vector<int32_t> bond_ids, angle_ids, dih_ids, imp_ids, pair_ids;
-    std::vector<float> bond_p, angle_p, dih_p, imp_p, pair_p;
-};
 
 namespace {
 
