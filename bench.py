@@ -164,8 +164,7 @@ def large_bench(args):
     box = L.gen_large_water()
     log(f"generated 1M-atom water box: {box.n_particles} atoms, {box.box_nm[0]:.3f} nm in {time.time() - t:.1f}s")
     dt_fs = 2.0
-    p = L.default_params(dt_fs=dt_fs, n_steps=args.steps, nlist_interval=10, energy_interval=0,
-                         launch_mode=args.launch_mode)
+    p = L.default_params(dt_fs=dt_fs, n_steps=args.steps, energy_interval=0, launch_mode=args.launch_mode)
     eng = L.Engine(box, p)
     t = time.time()
     r = eng.minimize(1000.0, args.em_iters, 0.01)
@@ -209,6 +208,10 @@ def large_bench(args):
         from tests.helpers import rel_rms, layout_forces
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         set_threads(threads)
+        if not args.no_accuracy:
+            from tests.helpers import max_list_age_error
+            stale = max_list_age_error(eng, box, p)
+            log(f"force rel RMS at list age {p.nlist_interval - 1} (the oldest list a step uses): {stale:.2e}")
         sbox = L.StateBox(box, eng.state())
         if not args.no_accuracy:
             e_gpu = eng.compute_forces()
@@ -217,7 +220,8 @@ def large_bench(args):
             f_ref, _, e_ref = Oracle(sbox, p).forces()
             err = rel_rms(f_gpu, f_ref)
             e_rel = abs(e_gpu.potential - e_ref.potential) / abs(e_ref.potential)
-            accuracy = {"force_rel_rms_vs_fp64": float(f"{err:.3e}"), "potential_rel_err_vs_fp64": float(f"{e_rel:.3e}")}
+            accuracy = {"force_rel_rms_vs_fp64": float(f"{err:.3e}"), "potential_rel_err_vs_fp64": float(f"{e_rel:.3e}"),
+                        "force_rel_rms_vs_fp64_max_list_age": float(f"{stale:.3e}")}
             log(f"accuracy vs FP64 oracle: force rel RMS {err:.2e}, potential rel {e_rel:.2e} ({time.time() - t:.1f}s)")
         if not args.no_cpu_baseline:
             orc = Oracle(sbox, p)
@@ -234,7 +238,7 @@ def large_bench(args):
         "dtype": "fp32", "data": "synthetic",
         "config": {"workload": "water_1m", "n_atoms": box.n_particles, "box_nm": [float(x) for x in box.box_nm],
                    "waters": 333333, "extra_lj_particles": 1, "dt_fs": dt_fs, "cutoff_nm": 1.2, "pme_grid": K,
-                   "nlist_interval": 10, "skin_nm": 0.1, "energy_reductions": "fp64"},
+                   "nlist_interval": p.nlist_interval, "skin_nm": round(p.nlist_skin_nm, 4), "energy_reductions": "fp64"},
         "roofline": roofline, "roofline_pme": roofline_pme, "cpu_baseline": cpu, "accuracy": accuracy,
         "phases_ms_per_step": {k: round(v, 5) for k, v in phases.items()}, "pairs_in_cutoff": pairs,
     }
@@ -287,8 +291,7 @@ def main():
     box = L.gen_chain_water()
     log(f"generated chain+water box: {box.n_particles} atoms, {box.n_compounds} compounds in {time.time() - t:.2f}s")
     dt_fs = 2.0
-    p = L.default_params(dt_fs=dt_fs, n_steps=args.steps, nlist_interval=10, energy_interval=0,
-                         launch_mode=args.launch_mode)
+    p = L.default_params(dt_fs=dt_fs, n_steps=args.steps, energy_interval=0, launch_mode=args.launch_mode)
     eng = L.Engine(box, p)
     log("engine created")
     t = time.time()
@@ -343,6 +346,10 @@ def main():
         from tests.helpers import rel_rms, layout_forces
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         set_threads(threads)
+        if not args.no_accuracy:
+            from tests.helpers import max_list_age_error
+            stale = max_list_age_error(eng, box, p)
+            log(f"force rel RMS at list age {p.nlist_interval - 1} (the oldest list a step uses): {stale:.2e}")
         st = eng.state()
         sbox = L.StateBox(box, st)
         if not args.no_accuracy:
@@ -353,7 +360,8 @@ def main():
             f_ref, _, e_ref = orc.forces()
             err = rel_rms(f_gpu, f_ref)
             e_rel = abs(e_gpu.potential - e_ref.potential) / abs(e_ref.potential)
-            accuracy = {"force_rel_rms_vs_fp64": float(f"{err:.3e}"), "potential_rel_err_vs_fp64": float(f"{e_rel:.3e}")}
+            accuracy = {"force_rel_rms_vs_fp64": float(f"{err:.3e}"), "potential_rel_err_vs_fp64": float(f"{e_rel:.3e}"),
+                        "force_rel_rms_vs_fp64_max_list_age": float(f"{stale:.3e}")}
             log(f"accuracy vs FP64 oracle: force rel RMS {err:.2e}, potential rel {e_rel:.2e} ({time.time() - t:.1f}s)")
         if not args.no_cpu_baseline:
             orc = Oracle(sbox, p)
@@ -386,7 +394,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": workload, "n_atoms": box.n_particles, "box_nm": [float(x) for x in box.box_nm],
                    "chains": "128 x 50", "waters": (box.n_particles - 6400) // 3, "dt_fs": dt_fs, "cutoff_nm": 1.2,
-                   "pme_spacing_nm": 0.12, "nlist_interval": 10, "skin_nm": 0.1, "energy_reductions": "fp64",
+                   "pme_spacing_nm": 0.12, "nlist_interval": p.nlist_interval, "skin_nm": round(p.nlist_skin_nm, 4),
+                   "energy_reductions": "fp64",
                    "launch": ("eager" if args.launch_mode & 1 else "step graph") +
                              (", one stream" if args.launch_mode & 2 else ", PME on a second stream")},
         "roofline": roofline,

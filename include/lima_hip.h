@@ -111,8 +111,8 @@ typedef struct lima_simparams {
     /* additions */
     int32_t pme_grid[3];              /* 0 = derived from pme_spacing_nm */
     float   pme_spacing_nm;           /* target PME spacing, default 0.12 */
-    float   nlist_skin_nm;            /* Verlet buffer, default 0.1 */
-    int32_t nlist_interval;           /* steps between list rebuilds, default 10 */
+    float   nlist_skin_nm;            /* Verlet buffer, default 0.15 */
+    int32_t nlist_interval;           /* steps between list rebuilds, default 20 */
     int32_t energy_interval;          /* steps between FP64 energy reductions (0 = never) */
     int32_t launch_mode;              /* 0 = each MD step replayed as a HIP graph (captured again after
                                          every list rebuild), PME on a second stream; or-ed flags

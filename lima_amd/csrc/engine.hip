@@ -509,7 +509,7 @@ void capture_step_graph(Engine& e) {
 
 void md_steps(Engine& e, int64_t n) {
     const auto t_issue = std::chrono::steady_clock::now();
-    const int interval = e.prm.nlist_interval > 0 ? e.prm.nlist_interval : 10;
+    const int interval = e.prm.nlist_interval > 0 ? e.prm.nlist_interval : 20;
     for (int64_t k = 0; k < n; k++) {
         if (e.steps_since_build >= interval) {
             const auto t0 = std::chrono::steady_clock::now();
@@ -578,7 +578,7 @@ void create(Engine& e, const lima_box_desc* b, const lima_simparams* p) {
     if (e.nC < 0 || e.nT < 0) throw LimaFail(LIMA_ERR_BAD_INPUT, "negative counts");
     e.rc = p->cutoff_nm;
     e.rc2 = e.rc * e.rc;
-    e.skin = p->nlist_skin_nm > 0 ? p->nlist_skin_nm : 0.1f;
+    e.skin = p->nlist_skin_nm > 0 ? p->nlist_skin_nm : 0.15f;
     e.elec = p->enable_electrostatics != 0;
     e.pc.kc = (float)(8.9875517873681764e9 / 1e-9 / 6.02214076e23 * 1000.f * 1000.f);   // PhysicsUtils.cuh:5
     e.pc.kappa = 3.f / e.rc;                                                             // PhysicsUtils.cuh:60-62

@@ -578,8 +578,8 @@ void lima_default_params(lima_simparams* p, float dt_fs, int64_t n_steps) {
     p->data_logging_interval = 5;
     p->steps_per_temperature_measurement = 200;
     p->pme_spacing_nm = 0.12f;
-    p->nlist_skin_nm = 0.1f;
-    p->nlist_interval = 10;
+    p->nlist_skin_nm = 0.15f;
+    p->nlist_interval = 20;
     p->energy_interval = 0;
 }
 

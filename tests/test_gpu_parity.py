@@ -8,7 +8,7 @@ import pytest
 
 import lima_amd as L
 from oracle.oracle import Oracle
-from tests.helpers import rel_rms, layout_forces, layout_pot, canonical_entries, unwrapped_positions, min_image
+from tests.helpers import rel_rms, layout_forces, layout_pot, canonical_entries, unwrapped_positions, min_image, max_list_age_error
 
 pytestmark = pytest.mark.gpu
 
@@ -159,6 +159,31 @@ def test_thermostat_trajectory_matches_oracle():
     assert err < 2e-4
 
 
+@pytest.mark.parametrize("name", ["chain_water_c1", "water_c0_hot_2fs"])
+def test_force_error_at_max_list_age(name):
+    """The neighbour list is rebuilt every nlist_interval steps with r <= r_c + skin; between rebuilds
+    the fastest particles (water H, ~7 nm/ps once these boxes heat to 500-800 K) move further than
+    skin / 2, so a pair can enter r_c unlisted.  What matters is the force error of the step with the
+    oldest list: against the FP64 oracle at the same positions it must stay within the 1e-5 bar with
+    the default skin and interval (measured ~6e-7 on the chain + water box, ~2e-6 on the hot water
+    box at 2 fs)."""
+    if name == "chain_water_c1":
+        box = L.gen_chain_water()
+        p = L.default_params(dt_fs=2.0)
+        eng = L.Engine(box, p)
+        eng.minimize(1000.0, 400, 0.01)
+        eng.step(300)
+    else:
+        box = L.gen_water_lattice(10, 3.104, 0, 300.0)
+        p = L.default_params(dt_fs=2.0)
+        eng = L.Engine(box, p)
+        eng.step(400)
+    errs = [max_list_age_error(eng, box, p) for _ in range(2)]
+    print(name, "skin", round(p.nlist_skin_nm, 3), "interval", p.nlist_interval, "force error at the oldest list",
+          ", ".join(f"{e:.3e}" for e in errs))
+    assert max(errs) <= FORCE_TOL
+
+
 def test_graph_replay_matches_eager_launches():
     """The MD step replayed as a HIP graph (re-captured at each list rebuild) gives the trajectory
     of the eager launches.  The run crosses three list rebuilds, after each of which the graph must
@@ -167,7 +192,7 @@ def test_graph_replay_matches_eager_launches():
     box = L.gen_em_system(1200, 1003, 0.0)
     runs = []
     for mode in (0, 1):
-        eng = L.Engine(box, L.default_params(dt_fs=1.0, launch_mode=mode))
+        eng = L.Engine(box, L.default_params(dt_fs=1.0, launch_mode=mode, nlist_interval=10))
         eng.step(35)                             # crosses three list rebuilds
         st = eng.state()
         runs.append((unwrapped_positions(box, st), _velocities(box, st), eng.energy().potential, eng.status()))

@@ -75,7 +75,8 @@ def test_log_ring_and_graph_replay():
     box = L.gen_em_system(1200, 1003, 0.0)
     logs = []
     for mode in (0, 1):                       # graph replay, eager launches
-        eng = L.Engine(box, L.default_params(dt_fs=1.0, data_logging_interval=5, log_capacity=3, launch_mode=mode))
+        eng = L.Engine(box, L.default_params(dt_fs=1.0, data_logging_interval=5, log_capacity=3, launch_mode=mode,
+                                            nlist_interval=10))
         eng.step(23)                          # logged steps 0, 5, 10, 15, 20; crosses two list rebuilds
         info = eng.log_info()
         assert (info.frames_logged, info.first_held) == (5, 2)
