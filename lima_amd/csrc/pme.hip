@@ -298,13 +298,21 @@ __global__ __launch_bounds__(256) void k_fft_x_inv(FftPlan P, int Ky, int H, int
     const int nl = min(TY, Ky - y0);
     float2* a = lds;
     float2* b = lds + TY * N;
-    for (int i = threadIdx.x; i < nl * N; i += blockDim.x) {
-        const int l = i / N, k = i - l * N;
-        const float2* row = cg + ((int64_t)z * Ky + y0 + l) * H;
-        float2 v;
-        if (k < H) v = row[k];
-        else { v = row[N - k]; v.y = -v.y; }
-        a[i] = v;
+    for (int i0 = 0; i0 < nl * N; i0 += 4 * (int)blockDim.x) {   // 4 loads in flight per thread
+        float2 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {   // no branch around the loads: past the end, reload the last element
+            const int i = min(i0 + u * (int)blockDim.x + (int)threadIdx.x, nl * N - 1);
+            const int l = i / N, k = i - l * N;
+            const float2* row = cg + ((int64_t)z * Ky + y0 + l) * H;
+            v[u] = row[k < H ? k : N - k];
+            if (k >= H) v[u].y = -v[u].y;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int i = i0 + u * (int)blockDim.x + (int)threadIdx.x;
+            if (i < nl * N) a[i] = v[u];
+        }
     }
     __syncthreads();
     const float2* r = fft_lines(a, b, nl, P, true);
@@ -325,9 +333,22 @@ __global__ __launch_bounds__(256) void k_fft_strided(FftPlan P, int H, int TK, i
     float2* a = lds;
     float2* b = lds + TK * N;
     float2* base = cg + (int64_t)o * outer_stride + k0;
-    for (int i = threadIdx.x; i < nl * N; i += blockDim.x) {
-        const int n = i / nl, t = i - n * nl;
-        a[t * N + n] = base[(int64_t)n * stride + t];
+    for (int i0 = 0; i0 < nl * N; i0 += 4 * (int)blockDim.x) {   // 4 loads in flight per thread
+        float2 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {   // no branch around the loads: past the end, reload the last element
+            const int i = min(i0 + u * (int)blockDim.x + (int)threadIdx.x, nl * N - 1);
+            const int n = i / nl, t = i - n * nl;
+            v[u] = base[(int64_t)n * stride + t];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int i = i0 + u * (int)blockDim.x + (int)threadIdx.x;
+            if (i < nl * N) {
+                const int n = i / nl, t = i - n * nl;
+                a[t * N + n] = v[u];
+            }
+        }
     }
     __syncthreads();
     float2* r = fft_lines(a, b, nl, P, mode == 1);
@@ -458,29 +479,53 @@ __global__ __launch_bounds__(256) void k_gather_tiles(GatherTileArgs a) {
     const int G0 = a.T[0] + 5, G1 = a.T[1] + 5, G2 = a.T[2] + 5;
     const int tx = t % a.nt[0], ty = (t / a.nt[0]) % a.nt[1], tz = t / (a.nt[0] * a.nt[1]);
     const int o0 = tx * a.T[0] - 2, o1 = ty * a.T[1] - 2, o2 = tz * a.T[2] - 2;
-    {   // rows of the block: 32 lanes per row (G0 <= 17)
-        const int lx = threadIdx.x & 31;
-        const float rG1 = 1.f / (float)G1;
-        for (int r = threadIdx.x >> 5; r < G1 * G2; r += blockDim.x >> 5) {
-            const int lz = (int)(((float)r + 0.5f) * rG1), ly = r - lz * G1;
-            if (lx < G0) {
+    {   // the block's nodes, flat; each thread issues up to 8 global loads before it stores any, so
+        // their latencies overlap (index divisions by float reciprocals, exact below 2^22)
+        const int n = G0 * G1 * G2;
+        const float rG0 = 1.f / (float)G0, rG1 = 1.f / (float)G1;
+        for (int b0 = 0; b0 < n; b0 += 8 * (int)blockDim.x) {
+            float v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {   // no branch around the loads: past the end, reload the last node
+                const int i = min(b0 + u * (int)blockDim.x + (int)threadIdx.x, n - 1);
+                const int r = (int)(((float)i + 0.5f) * rG0), lx = i - r * G0;
+                const int lz = (int)(((float)r + 0.5f) * rG1), ly = r - lz * G1;
                 const int X = wrapi(o0 + lx, a.K[0]), Y = wrapi(o1 + ly, a.K[1]), Z = wrapi(o2 + lz, a.K[2]);
-                phib[r * G0 + lx] = a.phi[((int64_t)Z * a.K[1] + Y) * a.K[0] + X];
+                v[u] = a.phi[((int64_t)Z * a.K[1] + Y) * a.K[0] + X];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int i = b0 + u * (int)blockDim.x + (int)threadIdx.x;
+                if (i < n) phib[i] = v[u];
             }
         }
     }
     __syncthreads();
+    // the bin's particles (index, charge, position) staged in LDS, 256 at a time, with one load
+    // round for all of them instead of two dependent loads per gather round
+    __shared__ int sp[256];
+    __shared__ float sq[256];
+    __shared__ uint4 sw[256];
     const int nb = min(a.bin_count[t], a.bincap);
     const int j = threadIdx.x & 7;
     float ephi = 0.f;
-    for (int k0 = 0; k0 < nb; k0 += blockDim.x >> 3) {   // uniform trip count per block
+    for (int c0 = 0; c0 < nb; c0 += 256) {   // uniform trip counts per block
+        const int nc = min(256, nb - c0);
+        if ((int)threadIdx.x < nc) {
+            const int p = a.bin_atoms[(int64_t)t * a.bincap + c0 + threadIdx.x];
+            sp[threadIdx.x] = p;
+            sq[threadIdx.x] = a.prm[p].x;
+            sw[threadIdx.x] = a.w[p];
+        }
+        __syncthreads();
+    for (int k0 = 0; k0 < nc; k0 += blockDim.x >> 3) {
         const int k = k0 + (threadIdx.x >> 3);
-        const bool live = k < nb;
-        const int p = live ? a.bin_atoms[(int64_t)t * a.bincap + k] : a.bin_atoms[(int64_t)t * a.bincap];
+        const bool live = k < nc;
+        const int p = live ? sp[k] : 0;
         float q = 0.f, ph = 0.f, ex = 0.f, ey = 0.f, ez = 0.f;
         if (live) {
-            q = a.prm[p].x;
-            const uint4 pw = a.w[p];
+            q = sq[k];
+            const uint4 pw = sw[k];
             int ix[3];
             float f[3], wx[4], wy[4], wz[4];
             grid_coord(pw.x, a.Lq[0], a.K[0], &ix[0], &f[0]);
@@ -528,6 +573,8 @@ __global__ __launch_bounds__(256) void k_gather_tiles(GatherTileArgs a) {
                 ephi += 0.5f * q * ph;
             }
         }
+    }
+        __syncthreads();   // the next chunk overwrites the staged particles
     }
     if (ENERGY) {
         const double e = wave_sum_d((double)ephi);
