@@ -150,7 +150,11 @@ typedef struct lima_runstatus {
     int32_t critical_error_occured;
     int32_t n_list_rebuilds;
     int64_t list_entries;         /* pair-list entries (8 pair slots each) of the last rebuild */
-    int64_t list_violations;      /* rebuilds after which 2 x max displacement exceeded the skin */
+    int64_t list_violations;      /* rebuilds after which 2 x max displacement exceeded the skin: a
+                                     conservative flag (the fastest particle, usually a water H,
+                                     could have brought an unlisted pair inside the cutoff); the
+                                     force error this causes is measured by comparing the step with
+                                     the oldest list against an exact evaluation, not by this count */
     float   max_displacement_nm;  /* max displacement over the last completed list interval */
 } lima_runstatus;
 
