@@ -86,31 +86,37 @@ __global__ void k_integrate(IntegrateArgs a) {
     bool bin_on = false;   // pack: this particle goes to a PME tile bin
     if (p < a.N) {
         const int s = a.slot_of[p];
+        // every load before the first store: the compiler cannot tell the arrays apart, and loads
+        // issued after a store would each add a memory round trip to this latency-bound kernel
         const float fx = a.fs[s] + a.fb[s], fy = a.fs[a.nslots + s] + a.fb[a.nslots + s], fz = a.fs[2 * a.nslots + s] + a.fb[2 * a.nslots + s];
-        const float m = a.prm[p].w;
+        const float4 prm = a.prm[p];
+        const float m = prm.w;
         const float4 fp = a.fprev[p];
         float4 v = a.v[p];
+        uint4 w = a.w[p];
+        int4 im = a.img[p];
+        const uint4 wb = a.w_build[p];
+        const int4 ib = a.img_build[p];
+        const uint4 r = a.pack ? a.cref[s >> 3] : make_uint4(0u, 0u, 0u, 0u);
+        const float ts = a.thermostat[0];
         const float kdt = a.dt * 0.5f / m;
         v.x = v.x + (fx + fp.x) * kdt;
         v.y = v.y + (fy + fp.y) * kdt;
         v.z = v.z + (fz + fp.z) * kdt;
         const float c2 = 0.5f / m * a.dt * a.dt;
-        uint4 w = a.w[p];
-        int4 im = a.img[p];
         wrap_add(w.x, im.x, coord_of(v.x * a.dt + fx * c2), a.Lq[0]);
         wrap_add(w.y, im.y, coord_of(v.y * a.dt + fy * c2), a.Lq[1]);
         wrap_add(w.z, im.z, coord_of(v.z * a.dt + fz * c2), a.Lq[2]);
         a.w[p] = w;
         a.img[p] = im;
         if (a.pack) {
-            const uint4 r = a.cref[s >> 3];
             long long d[3] = {(long long)w.x - (long long)r.x, (long long)w.y - (long long)r.y, (long long)w.z - (long long)r.z};
 #pragma unroll
             for (int k = 0; k < 3; k++) {
                 if (2 * d[k] > a.Lq[k]) d[k] -= a.Lq[k];
                 else if (2 * d[k] < -a.Lq[k]) d[k] += a.Lq[k];
             }
-            const float q = a.prm[p].x;
+            const float q = prm.x;
             a.xq[s] = make_float4((float)d[0] * LIMA_LIMA_TO_NANO_F, (float)d[1] * LIMA_LIMA_TO_NANO_F, (float)d[2] * LIMA_LIMA_TO_NANO_F, q);
             a.fs[s] = 0.f;
             a.fs[a.nslots + s] = 0.f;
@@ -120,15 +126,12 @@ __global__ void k_integrate(IntegrateArgs a) {
             bin_on = q != 0.f;
         }
         {   // displacement since the list was built (unwrapped, exact integer difference)
-            const uint4 wb = a.w_build[p];
-            const int4 ib = a.img_build[p];
             const float dx = (float)((long long)w.x - (long long)wb.x + (long long)(im.x - ib.x) * a.Lq[0]) * LIMA_LIMA_TO_NANO_F;
             const float dy = (float)((long long)w.y - (long long)wb.y + (long long)(im.y - ib.y) * a.Lq[1]) * LIMA_LIMA_TO_NANO_F;
             const float dz = (float)((long long)w.z - (long long)wb.z + (long long)(im.z - ib.z) * a.Lq[2]) * LIMA_LIMA_TO_NANO_F;
             d2 = dx * dx + dy * dy + dz * dz;
         }
         if (ENERGY) ke = 0.5f * m * (v.x * v.x + v.y * v.y + v.z * v.z);
-        const float ts = a.thermostat[0];
         v.x *= ts; v.y *= ts; v.z *= ts;
         a.v[p] = v;
         a.fprev[p] = make_float4(fx, fy, fz, 0.f);
