@@ -70,9 +70,7 @@ std::vector<std::string> words(const std::string& s) {
 }
 int to_int(const std::string& s, const std::string& where) {
     try {
-        size_t k = 0;
-        const int v = std::stoi(s, &k);
-        return v;
+        return std::stoi(s);
     } catch (...) {
         fail("expected an integer, got '" + s + "' in " + where);
     }
@@ -212,9 +210,9 @@ Sec section_of(const std::string& d, int& n_dih, int& n_dihtypes, const std::str
 }
 
 template <int N>
-bool map_ids(const Mtype& m, const std::vector<std::string>& w, std::array<int, N>& ids) {
+bool map_ids(const Mtype& m, const std::vector<std::string>& w, std::array<int, N>& ids, const std::string& where) {
     for (int i = 0; i < N; i++) {
-        const int nr = std::stoi(w[i]);
+        const int nr = to_int(w[i], where);
         const auto it = m.nr_to_local.find(nr);
         if (it == m.nr_to_local.end()) return false;   // LIMA skips terms naming unknown atoms
         ids[i] = it->second;
@@ -243,7 +241,7 @@ void parse_entry(Mtype& m, Sec s, const std::vector<std::string>& w, const std::
     case S_BONDS: {
         need(3);
         Term<2, BondP> t;
-        if (!map_ids<2>(m, w, t.ids)) break;
+        if (!map_ids<2>(m, w, t.ids, where)) break;
         if (w.size() >= 5) t.p = bond_from_gmx(f(3), f(4));
         m.bonds.push_back(t);
         break;
@@ -251,7 +249,7 @@ void parse_entry(Mtype& m, Sec s, const std::vector<std::string>& w, const std::
     case S_PAIRS: {
         need(3);
         Term<2, PairP> t;
-        if (!map_ids<2>(m, w, t.ids)) break;
+        if (!map_ids<2>(m, w, t.ids, where)) break;
         if (w.size() >= 5) t.p = pair_from_gmx(f(3), f(4));
         m.pairs.push_back(t);
         break;
@@ -259,7 +257,7 @@ void parse_entry(Mtype& m, Sec s, const std::vector<std::string>& w, const std::
     case S_ANGLES: {   // funct 5: theta0 k ub0 kub (Urey-Bradley); otherwise theta0 k [B-state values]
         need(4);
         Term<3, AngleP> t;
-        if (!map_ids<3>(m, w, t.ids)) break;
+        if (!map_ids<3>(m, w, t.ids, where)) break;
         const int funct = to_int(w[3], where);
         if (funct == 5 && w.size() >= 8) t.p = angle_from_gmx(f(4), f(5), f(6), f(7));
         else if (funct != 5 && w.size() >= 6) t.p = angle_from_gmx(f(4), f(5), 0.f, 0.f);
@@ -269,7 +267,7 @@ void parse_entry(Mtype& m, Sec s, const std::vector<std::string>& w, const std::
     case S_DIHS: {
         need(5);
         Term<4, DihP> t;
-        if (!map_ids<4>(m, w, t.ids)) break;
+        if (!map_ids<4>(m, w, t.ids, where)) break;
         if (w.size() >= 8) t.p = dih_from_gmx(f(5), f(6), to_int(w[7], where));
         m.dihs.push_back(t);
         break;
@@ -277,7 +275,7 @@ void parse_entry(Mtype& m, Sec s, const std::vector<std::string>& w, const std::
     case S_IMPS: {
         need(5);
         Term<4, ImpP> t;
-        if (!map_ids<4>(m, w, t.ids)) break;
+        if (!map_ids<4>(m, w, t.ids, where)) break;
         if (w.size() >= 7) t.p = imp_from_gmx(f(5), f(6));
         m.imps.push_back(t);
         break;
