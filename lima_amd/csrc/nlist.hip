@@ -611,32 +611,54 @@ void nlist_free(Engine& e) {
 
 namespace {
 
-__global__ void k_snapshot(int64_t N, const uint4* __restrict__ w, const int4* __restrict__ img, uint4* __restrict__ wb,
-                           int4* __restrict__ ib, float* dpart) {
+// The clears and the position snapshot that open a rebuild, in one launch: column counters and
+// fill cursors, the overflow flags, the PME tile bin counters when the integrator filled them for
+// the old slots (bins != nullptr), and, when the list is built from the engine's own positions,
+// the positions at the build and the per-wave displacement maxima (reset).
+__global__ void k_rebuild_open(int64_t N, int snapshot, const uint4* __restrict__ w, const int4* __restrict__ img,
+                               uint4* __restrict__ wb, int4* __restrict__ ib, float* __restrict__ dpart, int* __restrict__ col_count,
+                               int* __restrict__ col_fill, int ncol1, int* __restrict__ flags, int* __restrict__ bins, int nbins) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= N) return;
-    if ((p & 63) == 0) dpart[p >> 6] = 0.f;
-    wb[p] = w[p];
-    ib[p] = img[p];
+    if (snapshot && p < N) {
+        if ((p & 63) == 0) dpart[p >> 6] = 0.f;
+        wb[p] = w[p];
+        ib[p] = img[p];
+    }
+    if (p < ncol1) { col_count[p] = 0; col_fill[p] = 0; }
+    if (p < 4) flags[p] = 0;
+    if (bins && p < nbins) bins[p] = 0;
 }
 
-__global__ __launch_bounds__(256) void k_max_reduce(const float* __restrict__ v, int n, float* out) {
+// List staleness statistics kept on the device, from the per-wave displacement maxima of the
+// list being replaced: the largest displacement since the previous rebuild (*dmax2, squared;
+// stats[1], as float bits), and a count of rebuilds after which 2 max|dx| exceeded the skin.
+__global__ __launch_bounds__(256) void k_list_stats(const float* __restrict__ v, int n, float skin, int counted, float* dmax2,
+                                                    int* stats) {
     __shared__ float red[4];
     float m = 0.f;
     for (int i = threadIdx.x; i < n; i += blockDim.x) m = fmaxf(m, v[i]);
     m = wave_max(m);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
     __syncthreads();
-    if (threadIdx.x == 0) *out = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+    if (threadIdx.x == 0) {
+        const float m2 = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+        *dmax2 = m2;
+        const float d = sqrtf(m2);
+        if (counted && 2.f * d > skin) stats[0] += 1;
+        stats[1] = __float_as_int(d);
+    }
 }
 
-// List staleness statistics kept on the device: the largest displacement since the previous
-// rebuild, and a count of rebuilds after which 2 max|dx| exceeded the skin.
-__global__ void k_list_stats(const float* __restrict__ dmax2, float skin, int counted, int* stats) {
-    if (threadIdx.x != 0) return;
-    const float d = sqrtf(*dmax2);
-    if (counted && 2.f * d > skin) stats[0] += 1;
-    stats[1] = __float_as_int(d);
+// Slot arrays of the new list: no particle in any slot (catoms -1) until assigned, bonded forces
+// and energies zero (slots of particles without bonded terms keep them zero until the next rebuild).
+__global__ void k_clear_slots(int ns, int* __restrict__ catoms, float* __restrict__ fb, float* __restrict__ eb) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ns) return;
+    catoms[i] = -1;
+    fb[i] = 0.f;
+    fb[ns + i] = 0.f;
+    fb[2 * (int64_t)ns + i] = 0.f;
+    eb[i] = 0.f;
 }
 
 }  // namespace
@@ -655,16 +677,17 @@ void nlist_build(Engine& e, const uint4* w) {
     // Every kernel argument stays the same across rebuilds (grids and strides follow ncl_max, the
     // count is read from ncl_dev), so a captured step graph stays valid unless the entry buffer is
     // reallocated.  The integrator's packing of the old slots is stale.
-    if (e.packed && e.pme.tiled && e.elec && e.pme.ncharged > 0)   // tile bins filled for the old slots
-        hipLaunchKernelGGL(k_fill_int, dim3(nblk(e.pme.ntiles, 256)), dim3(256), 0, s, e.pme.bin_count, e.pme.ntiles, 0);
+    const bool clear_bins = e.packed && e.pme.tiled && e.elec && e.pme.ncharged > 0;   // tile bins filled for the old slots
     e.packed = false;
     // staleness statistics of the list being replaced, on the device (read by lima_get_status)
-    hipLaunchKernelGGL(k_max_reduce, dim3(1), dim3(256), 0, s, e.dpart, (int)((N + 63) / 64), e.dmax2);
-    hipLaunchKernelGGL(k_list_stats, dim3(1), dim3(64), 0, s, (const float*)e.dmax2, e.skin, e.n_rebuilds > 0 ? 1 : 0, e.lstats);
-    if (w == e.p.w) hipLaunchKernelGGL(k_snapshot, dim3(nblk(N, 256)), dim3(256), 0, s, N, e.p.w, e.p.img, e.w_build, e.img_build, e.dpart);
-    hipLaunchKernelGGL(k_fill_int, dim3(nblk(nl.ncol + 1, 256)), dim3(256), 0, s, nl.col_count, nl.ncol + 1, 0);
-    hipLaunchKernelGGL(k_fill_int, dim3(nblk(nl.ncol + 1, 256)), dim3(256), 0, s, nl.col_fill, nl.ncol + 1, 0);
-    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(4), 0, s, nl.flags, 4, 0);
+    hipLaunchKernelGGL(k_list_stats, dim3(1), dim3(256), 0, s, (const float*)e.dpart, (int)((N + 63) / 64), e.skin,
+                       e.n_rebuilds > 0 ? 1 : 0, e.dmax2, e.lstats);
+    {
+        const int64_t span = std::max<int64_t>(std::max<int64_t>(N, nl.ncol + 1), std::max<int64_t>(4, clear_bins ? e.pme.ntiles : 0));
+        hipLaunchKernelGGL(k_rebuild_open, dim3(nblk(span, 256)), dim3(256), 0, s, N, w == e.p.w ? 1 : 0, e.p.w, e.p.img, e.w_build,
+                           e.img_build, e.dpart, nl.col_count, nl.col_fill, nl.ncol + 1, nl.flags,
+                           clear_bins ? e.pme.bin_count : nullptr, e.pme.ntiles);
+    }
     const size_t col_lds = nl.ncol <= COL_LDS_MAX ? (size_t)nl.ncol * sizeof(int) : 0;
     hipLaunchKernelGGL(k_col_assign, dim3(nblk(N, 256 * COL_ITEMS)), dim3(256), col_lds, s, N, w, nl.xb[0], nl.xb[1], nl.rinv[0],
                        nl.rinv[1], nl.ncx, nl.ncy, nl.col_of, nl.col_count);
@@ -677,13 +700,9 @@ void nlist_build(Engine& e, const uint4* w) {
     launch_exclusive_scan(nl.col_ncl, nl.col_count, nl.ncol, s);
     const int* col_cl_start = nl.col_count;
     const int ns = slots_max(e);
-    hipLaunchKernelGGL(k_fill_int, dim3(nblk(ns, 256)), dim3(256), 0, s, nl.catoms, ns, -1);
+    hipLaunchKernelGGL(k_clear_slots, dim3(nblk(ns, 256)), dim3(256), 0, s, ns, nl.catoms, e.fb, e.eb);
     hipLaunchKernelGGL(k_assign_slots, dim3(nblk(N, 256)), dim3(256), 0, s, N, nl.ncol, nl.col_start, col_cl_start, nl.keys,
                        nl.catoms, nl.slot_of);
-    {   // bonded slot forces: slots of atoms without bonded terms stay zero until the next rebuild
-        hipLaunchKernelGGL(k_fill_int, dim3(nblk(3 * (int64_t)ns, 256)), dim3(256), 0, s, (int*)e.fb, 3 * ns, 0);
-        hipLaunchKernelGGL(k_fill_int, dim3(nblk(ns, 256)), dim3(256), 0, s, (int*)e.eb, ns, 0);
-    }
     hipLaunchKernelGGL(k_cluster_props, dim3(nblk(ns, 256)), dim3(256), 0, s, nl.ncl_dev, nl.ncl_max, nl.ncol, col_cl_start,
                        nl.catoms, w, e.p.prm, e.p.comp, nl.ccol, nl.cref, nl.cmin, nl.cmax, nl.ljs, nl.comps, nl.cw);
     NlArgs a{};
