@@ -515,6 +515,8 @@ void md_steps(Engine& e, int64_t n) {
     const int interval = e.prm.nlist_interval > 0 ? e.prm.nlist_interval : 20;
     for (int64_t k = 0; k < n; k++) {
         if (e.steps_since_build >= interval) {
+            // timing: drain earlier steps first, so the rebuild's clock holds the rebuild alone
+            if (e.timing) HIP_CHECK(hipDeviceSynchronize());
             const auto t0 = std::chrono::steady_clock::now();
             nlist_build(e, e.p.w);   // synchronises once (entry count read-back)
             if (e.timing) {
