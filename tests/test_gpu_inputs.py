@@ -10,7 +10,7 @@ import pytest
 
 import lima_amd as L
 from oracle.oracle import Oracle
-from tests.helpers import rel_rms, layout_forces
+from tests.helpers import rel_rms, layout_forces, unwrapped_positions
 
 pytestmark = pytest.mark.gpu
 
@@ -56,3 +56,18 @@ def test_bad_parameters_are_rejected(override, message):
     # the failed creation left nothing behind that breaks the next engine
     eng = L.Engine(box, L.default_params(dt_fs=1.0))
     assert np.isfinite(eng.compute_forces().total)
+
+
+def test_trajectory_without_electrostatics_matches_oracle():
+    """20 velocity-Verlet steps of a chain + water system with electrostatics off: the integer Coord
+    positions of the HIP path and of the FP64 oracle stay within 1e-5 nm RMSD."""
+    box = L.gen_em_system(600, 1010, 0.0)
+    p = L.default_params(dt_fs=1.0, enable_electrostatics=0)
+    eng = L.Engine(box, p)
+    orc = Oracle(box, p)
+    eng.step(20)
+    orc.step(20)
+    d = (unwrapped_positions(box, eng.state()) - orc.positions()) * 1e-8
+    rmsd = float(np.sqrt((d ** 2).sum(1).mean()))
+    print(f"electrostatics off: 20-step RMSD {rmsd:.3e} nm", flush=True)
+    assert rmsd < 1e-5, rmsd
