@@ -172,8 +172,10 @@ def large_bench(args):
     t = time.time()
     eng.step(args.warmup)
     log(f"warm-up {args.warmup} steps in {time.time() - t:.2f}s")
+    eng.synchronize()
     t0 = time.perf_counter()
     eng.step(args.steps)
+    eng.synchronize()
     elapsed = time.perf_counter() - t0
     ms_per_step = elapsed * 1e3 / args.steps
     ns_day = args.steps / elapsed * dt_fs * 86400.0 / 1e6
@@ -304,6 +306,7 @@ def main():
     host0 = eng.timings()
     t0 = time.perf_counter()
     eng.step(args.steps)
+    eng.synchronize()
     t1 = time.perf_counter()
     host1 = eng.timings()
     host_issue_us = (host1.host_issue_ms - host0.host_issue_ms) * 1e3 / max(host1.host_steps - host0.host_steps, 1)
@@ -311,7 +314,7 @@ def main():
     ms_per_step = elapsed * 1e3 / args.steps
     ns_day = args.steps / elapsed * dt_fs * 86400.0 / 1e6
     log(f"timed {args.steps} steps: {ms_per_step:.4f} ms/step, {ns_day:.1f} ns/day; host issue {host_issue_us:.1f} us/step")
-    e_end = eng.energy()
+    e_end = eng.compute_forces()
     pairs = eng.pair_count()
     stt = eng.status()
     list_slots = stt.list_entries * 8

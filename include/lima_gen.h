@@ -25,7 +25,9 @@ typedef struct lima_box lima_box;   /* owning container of every array a lima_bo
 lima_status lima_gen_water_lattice(int32_t n_per_dim, double box_nm, uint64_t seed, double temperature_K, lima_box** out);
 
 /* Chain slab + water: n_chains linear chains of chain_len atoms in two facing leaflets along z,
- * then waters filling the box up to n_waters (fewer if space runs out). */
+ * then exactly n_waters waters in the free volume: on a lattice at 1 g/cm^3 when they fit there,
+ * otherwise on successively finer lattices with shorter clash distances (a denser start that
+ * minimisation relaxes).  LIMA_ERR_BAD_INPUT when n_waters cannot be placed. */
 lima_status lima_gen_chain_water(double bx, double by, double bz, int32_t n_chains, int32_t chain_len,
                                  int32_t n_waters, uint64_t seed, double temperature_K, lima_box** out);
 

@@ -122,6 +122,8 @@ typedef struct lima_simparams {
     int32_t log_capacity;             /* frames the device data log holds (0 = no logging); with
                                          data_logging_interval > 0 every step % interval == 0 is
                                          evaluated with energies and logged (lima_read_log) */
+    int32_t nlist_capacity;           /* pair-list entries per i-cluster to start with (0 = 2048); a
+                                         build that needs more grows it to 1.5 x the largest count */
 } lima_simparams;
 
 #define LIMA_LAUNCH_EAGER 1
@@ -139,6 +141,9 @@ typedef struct lima_energy {
     double potential_lima;/* sum of LIMA's per-particle potE (uses pme_self_lima) */
     double kinetic;
     double total;         /* potential + kinetic */
+    int64_t step;         /* MD step whose evaluation produced these totals (lima_compute_forces: the
+                             current step; lima_minimize: the current step after minimising); -1
+                             before any energy evaluation */
 } lima_energy;
 
 /* LIMA RunStatus mirror (Engine.cuh:63-75). */
@@ -156,6 +161,8 @@ typedef struct lima_runstatus {
                                      force error this causes is measured by comparing the step with
                                      the oldest list against an exact evaluation, not by this count */
     float   max_displacement_nm;  /* max displacement over the last completed list interval */
+    int32_t capacity_rollbacks;   /* times issued steps were undone and run again after a capacity
+                                     overflow (see lima_step) */
 } lima_runstatus;
 
 /* State read-back.  Arrays the caller provides (NULL = skip) in LIMA's
@@ -191,14 +198,28 @@ lima_status lima_create(const lima_box_desc* box, const lima_simparams* params, 
 /* Engine::~Engine / terminateSimulation, Engine.cu:192-199 */
 lima_status lima_destroy(lima_engine* engine);
 
-/* ---- Engine::step(), Engine.cu:148-159: nsteps velocity-Verlet steps ---- */
+/* ---- Engine::step(), Engine.cu:148-159: nsteps velocity-Verlet steps ----
+ * Asynchronous, as LIMA's kernel launches are: the call issues the steps and returns; the device
+ * runs at most two neighbour-list intervals behind the host.  Conditions the device meets (a pair
+ * list, cell column or PME tile bin over capacity, a non-finite force) are found at a later list
+ * rebuild or at the next lima_synchronize: a capacity overflow is undone by restoring the state of
+ * the last list rebuild, growing the capacity and running those steps again, so callers never see
+ * its results; a non-finite force puts the engine in its critical state (LIMA_ERR_NONFINITE, as
+ * RunStatus::critical_error_occured).  Every call that returns data (lima_get_state,
+ * lima_get_energy, lima_get_status, lima_read_log, ...) synchronises first.  Energies are evaluated
+ * on every energy_interval-th step and on logged steps only. */
 lima_status lima_step(lima_engine* engine, int64_t nsteps);
+
+/* Waits for every issued step and reports what the device met (see lima_step).  No counterpart
+ * in LIMA, whose step() waits for the device itself (Engine.cu:297-300). */
+lima_status lima_synchronize(lima_engine* engine);
 
 /* One force (+ FP64 energy) evaluation at the current positions, no integration.
  * The force half of _deviceMaster (Engine.cu:297-352). */
 lima_status lima_compute_forces(lima_engine* engine, lima_energy* energy_out);
 
-/* Last FP64 energy totals (from lima_compute_forces, an energy step, or EM). */
+/* FP64 energy totals of the last energy evaluation: lima_compute_forces, lima_minimize, or an
+ * energy step of lima_step (every energy_interval-th step, every logged step); .step names it. */
 lima_status lima_get_energy(lima_engine* engine, lima_energy* energy_out);
 
 /* ---- Programs::EnergyMinimize, Programs.cpp:155-199 (EM entry, SimParams::em_variant):

@@ -89,6 +89,7 @@ def load_hip():
             "lima_create": ([C.POINTER(BoxDesc), C.POINTER(SimParams), C.POINTER(vp)], C.c_int),
             "lima_destroy": ([vp], C.c_int),
             "lima_step": ([vp, C.c_int64], C.c_int),
+            "lima_synchronize": ([vp], C.c_int),
             "lima_compute_forces": ([vp, C.POINTER(Energy)], C.c_int),
             "lima_get_energy": ([vp, C.POINTER(Energy)], C.c_int),
             "lima_minimize": ([vp, C.c_double, C.c_int64, C.c_double, C.POINTER(EmResult)], C.c_int),
@@ -372,7 +373,12 @@ class Engine:
             pass
 
     def step(self, n=1):
+        """Issues n MD steps (lima_step returns before the device has run them)."""
         self._check(self.lib.lima_step(self._h, n))
+
+    def synchronize(self):
+        """Waits for the issued steps; raises what the device met (lima_synchronize)."""
+        self._check(self.lib.lima_synchronize(self._h))
 
     def compute_forces(self):
         e = Energy()

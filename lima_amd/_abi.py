@@ -57,13 +57,14 @@ class SimParams(C.Structure):
         ("energy_interval", C.c_int32),
         ("launch_mode", C.c_int32),
         ("log_capacity", C.c_int32),
+        ("nlist_capacity", C.c_int32),
     ]
 
 
 class Energy(C.Structure):
     _fields_ = [(n, C.c_double) for n in (
         "lj", "coulomb_sr", "bonded", "pme_recip", "pme_self_lima", "pme_self_ewald",
-        "potential", "potential_lima", "kinetic", "total")]
+        "potential", "potential_lima", "kinetic", "total")] + [("step", C.c_int64)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -80,6 +81,7 @@ class RunStatus(C.Structure):
         ("list_entries", C.c_int64),
         ("list_violations", C.c_int64),
         ("max_displacement_nm", C.c_float),
+        ("capacity_rollbacks", C.c_int32),
     ]
 
 

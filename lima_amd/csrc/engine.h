@@ -57,7 +57,6 @@ struct NList {
     int* eoff = nullptr;       // [ncl_max+1]
     float4* ent = nullptr;     // [nent_cap] (shift nm xyz, bits: j slot | mask << 24)
     uint8_t* ecode = nullptr;  // [nent_cap] image shift code of each entry
-    int* flags = nullptr;      // [4] overflow flags
 };
 
 struct Pme {
@@ -83,7 +82,7 @@ struct Pme {
     int* bin_atoms = nullptr;              // [ntiles][bincap]
     long long* tblock = nullptr;           // [ntiles][E^3] per-tile blocks
     int4* cover[3] = {nullptr, nullptr, nullptr};   // per grid coordinate: (tile a, local a, tile b, local b)
-    int* flags = nullptr;                  // [1] particle left its tile block
+    int* flags = nullptr;                  // the engine's DF_PMEBIN word: a bin overflowed / a particle left its block
 };
 
 struct Bonded {
@@ -106,12 +105,39 @@ constexpr uint32_t LANE_PAD = 0xffffffffu;
 constexpr int LAUNCH_EAGER = LIMA_LAUNCH_EAGER, LAUNCH_ONE_STREAM = LIMA_LAUNCH_ONE_STREAM;
 
 enum Acc { ACC_LJ = 0, ACC_COUL, ACC_BOND, ACC_PME, ACC_KIN, ACC_PAIRS, ACC_N };
+constexpr int REDUCE_BLOCKS = 64;   // first stage of the energy reduction
 
 // FP64 energy reduction without same-address atomics: every wave of an energy-producing kernel
 // writes one record of ACC_N doubles into its kernel's region of `epart`; k_reduce_energy sums
 // the records in a fixed order (deterministic totals).
 struct ERegions {
     int pair = 0, intra = 0, bond = 0, pme = 0, kin = 0, total = 0;
+};
+
+// Sticky status words on the device (Engine::dflags).  Kernels only set them; the host clears
+// them after it has dealt with the condition.  Every list rebuild whose error words are still
+// clear snapshots the MD state (positions, velocities, previous forces, thermostat scale) and
+// counts the snapshot in DF_SNAPSHOTS, so a condition found later is undone by restoring the
+// last snapshot, growing the capacity that overflowed and issuing the steps again.
+enum DFlag {
+    DF_COLUMN = 0,     // an x-y column of the cell grid holds more particles than the sort can take
+    DF_PAIRLIST,       // an i-cluster's pair-list slice overflowed
+    DF_PMEBIN,         // a PME tile bin overflowed
+    DF_NONFINITE,      // the integrator met a NaN / inf force
+    DF_SNAPSHOTS,      // snapshots taken (ordinal of the last one)
+    DF_LIST_NEED,      // with DF_PAIRLIST: the largest entry count an i-cluster needed
+    DF_N = 8
+};
+constexpr int DF_ERRORS = 4;   // words 0 .. DF_ERRORS-1 are error conditions
+
+// Host side of a snapshot: the engine counters that belong with the device state it saved.
+struct SnapRecord {
+    int64_t ordinal = 0;       // value DF_SNAPSHOTS takes when this snapshot is taken
+    int64_t step = 0;
+    int64_t log_frames = 0;
+    int n_rebuilds = 0;        // rebuilds before this one
+    bool temp_measured = false;
+    int64_t energy_step = -1;
 };
 
 struct SdState {            // steepest descent state, device resident
@@ -147,6 +173,21 @@ struct Engine {
     int4* img_try = nullptr;
     float4* f_acc = nullptr;   // accepted forces, global order
     SdState* sd = nullptr;
+    // snapshot of the MD state at the last clean list rebuild (positions: w_build / img_build)
+    float4* v_snap = nullptr;
+    float4* fprev_snap = nullptr;
+    float* thermo_snap = nullptr;  // [2]
+    int* lstats_snap = nullptr;    // [2]
+    double* acc_snap = nullptr;    // [ACC_N]
+    bool stats_skip = false;       // the next rebuild does not count list statistics (after a rollback)
+    int* dflags = nullptr;         // [DF_N] sticky status words (device)
+    int* hflags = nullptr;         // [2][DF_N] pinned host copies, written after each deferred rebuild
+    hipEvent_t ev_rebuild[2] = {nullptr, nullptr};   // recorded after each deferred rebuild's flag copy
+    bool rb_pending[2] = {false, false};
+    int rb_parity = 0;
+    int64_t snap_issued = 0;       // snapshots the host has asked for (device count if none was skipped)
+    std::vector<SnapRecord> snaps; // records of snapshots not yet known to be superseded
+    int rollbacks = 0;             // rollbacks since create (lima_runstatus.capacity_rollbacks)
 
     // compounds
     int64_t* first = nullptr;  // [nC+1]
@@ -158,16 +199,29 @@ struct Engine {
     int n_intra = 0;
     int4* intra_items = nullptr;    // (A, B, first A, first B)
     uint32_t* intra_lut = nullptr;  // [n_intra*32]
+    int n_intra_groups = 0;         // compounds A with at least one item (their items are consecutive)
+    int2* intra_groups = nullptr;   // (first item, item count) per group
 
     Bonded bd;
     NList nl;
     Pme pme;
 
-    float* fs = nullptr;       // [3*slots] forces per cluster slot (SoA), summed with atomics
-    float* es = nullptr;       // [slots] per-slot potE
-    float* fb = nullptr;       // [3*slots] bonded force per slot, stored once by the atom's bonded lanes
-    float* eb = nullptr;       // [slots] bonded potE share per slot (energy evaluations)
+    // Per-kernel force (SoA [3][slots]) and potE ([slots]) buffers, each written with plain stores
+    // by the one wave that owns the slot; the integrator and the read-back sum them in a fixed order
+    // (pair + intra + bonded + PME), so forces are deterministic.  Slots a kernel never writes (no
+    // bonded terms, no intra pairs, no charge) are zeroed at each list rebuild.
+    float* fpair = nullptr;    // cutoff pairs
+    float* epair = nullptr;
+    float* fintra = nullptr;   // intra / bonded-compound pairs
+    float* eintra = nullptr;
+    float* fb = nullptr;       // bonded terms
+    float* eb = nullptr;
+    float* fpme = nullptr;     // PME gather
+    float* epme = nullptr;
+    float* ephi = nullptr;     // [slots] 1/2 q phi of charged slots (energy evaluations), summed in slot
+                               // order by the energy reduction (the gather's bin order is not fixed)
     double* acc = nullptr;     // [ACC_N]
+    double* acc_part = nullptr;// [REDUCE_BLOCKS][ACC_N] first-stage sums of the energy reduction
     double* epart = nullptr;   // [epart_cap * ACC_N] per-wave energy records
     int64_t epart_cap = 0;
     ERegions ereg;
@@ -187,6 +241,8 @@ struct Engine {
     int64_t steps_since_build = 0;
     int n_rebuilds = 0;
     lima_energy energy{};
+    int64_t energy_step = -1;      // MD step of the last energy evaluation issued (-1: none, -2: not an MD step)
+    bool energy_unread = false;    // its totals are still only on the device (e.acc)
     bool forces_valid = false;
     bool packed = false;       // the integrator already packed the slots for the next evaluation
     bool critical = false;
@@ -233,10 +289,17 @@ struct LimaFail : std::runtime_error {
 // ---- nlist.hip ----
 void nlist_alloc(Engine& e);
 void nlist_free(Engine& e);
-void nlist_build(Engine& e, const uint4* w);   // rebuilds clusters + lists from positions w
+// Rebuilds clusters + lists from positions w.  checked: wait for the build, read the status words,
+// grow a pair list that overflowed and build again (the one host synchronisation of a rebuild);
+// otherwise the words are copied to pinned host memory behind an event and examined later
+// (poll_rebuild_flags / settle in engine.hip).
+void nlist_build(Engine& e, const uint4* w, bool checked);
+void nlist_grow(Engine& e, int need);   // pair-list capacity >= 1.5 x need (the step graph is captured again)
+void reset_flags(Engine& e, int64_t snapshots, hipStream_t s);   // clear the error words, set DF_SNAPSHOTS
 
 // ---- forces.hip ----
 int pair_split(const Engine& e);
+void launch_clear_slots(Engine& e, hipStream_t s);   // zero the per-kernel force / potE buffers, no particle in any slot
 ERegions energy_regions(const Engine& e);
 void launch_reduce_energy(Engine& e, bool kinetic, hipStream_t s);
 void launch_pack(Engine& e, const uint4* w, hipStream_t s);
@@ -249,8 +312,10 @@ void launch_bonded(Engine& e, const uint4* w, const int4* img, bool energy, hipS
 void launch_local(Engine& e, const uint4* w, const int4* img, bool energy, hipStream_t s);   // intra + bonded, one launch
 
 // ---- pme.hip ----
-void pme_alloc(Engine& e, const std::vector<float>& charges_global);
+void pme_alloc(Engine& e, const std::vector<float>& charges_global, const std::vector<uint4>& w);
 void pme_free(Engine& e);
+void pme_grow_bins(Engine& e);       // tile-bin capacity x 2 (the step graph is captured again)
+void pme_clear_bins(Engine& e, hipStream_t s);
 void launch_pme(Engine& e, const uint4* w, bool energy, hipStream_t s);
 
 // ---- scan helper (nlist.hip) ----

@@ -45,20 +45,22 @@ struct IntegrateArgs {
     float* dpart;          // per-wave running max of the squared displacement since the list build
     double* epart;         // kinetic-energy region: one record per wave
     const int* slot_of;
-    float* fs;
+    const float* fpair;    // per-kernel force buffers [3][nslots], summed in this order
+    const float* fintra;
     const float* fb;
+    const float* fpme;
     uint4* w;
     int4* img;
     float4* v;
     float4* fprev;
     const float4* prm;
     // pack != 0: also prepare the particle's slot for the next evaluation, as k_pack does (relative
-    // position and charge, accumulators cleared, PME tile bin); the list must stay unchanged
+    // position and charge, PME tile bin); the list must stay unchanged
     int pack;
     const uint4* cref;
     float4* xq;
-    float* es;
     BinArgs bins;
+    int* flags;            // DF_NONFINITE is set when a force is NaN / inf
 };
 
 // w + d wrapped back into [0, Lq) for any d (a step may exceed a box edge when a run explodes).
@@ -84,11 +86,16 @@ __global__ void k_integrate(IntegrateArgs a) {
     float ke = 0.f, d2 = 0.f;
     uint4 bin_w = make_uint4(0u, 0u, 0u, 0u);
     bool bin_on = false;   // pack: this particle goes to a PME tile bin
+    bool bad = false;      // non-finite force
     if (p < a.N) {
         const int s = a.slot_of[p];
         // every load before the first store: the compiler cannot tell the arrays apart, and loads
         // issued after a store would each add a memory round trip to this latency-bound kernel
-        const float fx = a.fs[s] + a.fb[s], fy = a.fs[a.nslots + s] + a.fb[a.nslots + s], fz = a.fs[2 * a.nslots + s] + a.fb[2 * a.nslots + s];
+        const int n1 = a.nslots, n2 = 2 * a.nslots;
+        const float fx = ((a.fpair[s] + a.fintra[s]) + a.fb[s]) + a.fpme[s];
+        const float fy = ((a.fpair[n1 + s] + a.fintra[n1 + s]) + a.fb[n1 + s]) + a.fpme[n1 + s];
+        const float fz = ((a.fpair[n2 + s] + a.fintra[n2 + s]) + a.fb[n2 + s]) + a.fpme[n2 + s];
+        bad = !(fabsf(fx) <= 3.0e38f && fabsf(fy) <= 3.0e38f && fabsf(fz) <= 3.0e38f);
         const float4 prm = a.prm[p];
         const float m = prm.w;
         const float4 fp = a.fprev[p];
@@ -118,10 +125,6 @@ __global__ void k_integrate(IntegrateArgs a) {
             }
             const float q = prm.x;
             a.xq[s] = make_float4((float)d[0] * LIMA_LIMA_TO_NANO_F, (float)d[1] * LIMA_LIMA_TO_NANO_F, (float)d[2] * LIMA_LIMA_TO_NANO_F, q);
-            a.fs[s] = 0.f;
-            a.fs[a.nslots + s] = 0.f;
-            a.fs[2 * a.nslots + s] = 0.f;
-            a.es[s] = 0.f;
             bin_w = w;
             bin_on = q != 0.f;
         }
@@ -137,6 +140,7 @@ __global__ void k_integrate(IntegrateArgs a) {
         a.fprev[p] = make_float4(fx, fy, fz, 0.f);
     }
     if (a.pack && a.bins.bin_count) bin_wave(a.bins, bin_on, (int)p, bin_w);   // all lanes, converged
+    if (__ballot(bad) != 0ull && (threadIdx.x & 63) == 0) atomicOr(&a.flags[DF_NONFINITE], 1);
     d2 = wave_max(d2);
     const int64_t wv = p >> 6;
     const bool wave_in = (p & ~63ll) < a.N;   // waves inside the per-wave arrays
@@ -190,16 +194,29 @@ __global__ __launch_bounds__(256) void k_thermostat(const double* __restrict__ k
     }
 }
 
+// The per-kernel force and potE buffers of the last evaluation, summed in the integrator's order.
+struct SlotForces {
+    int nslots;
+    const float* f[4];   // pair, intra, bonded, PME: [3][nslots]
+    const float* e[4];   // [nslots]
+};
+__device__ __forceinline__ float4 slot_force(const SlotForces& b, int s) {
+    float4 r;
+    const int n1 = b.nslots, n2 = 2 * b.nslots;
+    r.x = ((b.f[0][s] + b.f[1][s]) + b.f[2][s]) + b.f[3][s];
+    r.y = ((b.f[0][n1 + s] + b.f[1][n1 + s]) + b.f[2][n1 + s]) + b.f[3][n1 + s];
+    r.z = ((b.f[0][n2 + s] + b.f[1][n2 + s]) + b.f[2][n2 + s]) + b.f[3][n2 + s];
+    r.w = ((b.e[0][s] + b.e[1][s]) + b.e[2][s]) + b.e[3][s];
+    return r;
+}
+
 // Forces of the last evaluation in global order (w = per-particle potE) and max |F|.
-__global__ void k_forces_global(int64_t N, int nslots, const int* __restrict__ slot_of, const float* __restrict__ fs,
-                                const float* __restrict__ es, const float* __restrict__ fb, const float* __restrict__ eb,
-                                float4* __restrict__ out, float* fmax) {
+__global__ void k_forces_global(int64_t N, const int* __restrict__ slot_of, SlotForces b, float4* __restrict__ out, float* fmax) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     float m = 0.f;
     if (p < N) {
         const int s = slot_of[p];
-        const float4 f = make_float4(fs[s] + fb[s], fs[nslots + s] + fb[nslots + s], fs[2 * nslots + s] + fb[2 * nslots + s],
-                                     es[s] + eb[s]);
+        const float4 f = slot_force(b, s);
         out[p] = f;
         m = sqrtf(f.x * f.x + f.y * f.y + f.z * f.z);
         if (!(m <= 3.0e38f)) m = 3.0e38f;   // NaN / inf -> flagged as huge
@@ -214,10 +231,8 @@ __global__ void k_forces_global(int64_t N, int nslots, const int* __restrict__ s
 // force of the step's evaluation, speed after the step.  Written to the particle's entry of LIMA's
 // buffer layout.
 __global__ void k_log_frame(int64_t N, const int* __restrict__ lidx, const int* __restrict__ key, const uint4* __restrict__ w,
-                            const int4* __restrict__ img, const float4* __restrict__ v, const int* __restrict__ slot_of, int nslots,
-                            const float* __restrict__ fs, const float* __restrict__ fb, const float* __restrict__ es,
-                            const float* __restrict__ eb, long long Lqx, long long Lqy, long long Lqz,
-                            float4* __restrict__ pe, float4* __restrict__ fv) {
+                            const int4* __restrict__ img, const float4* __restrict__ v, const int* __restrict__ slot_of, SlotForces b,
+                            long long Lqx, long long Lqy, long long Lqz, float4* __restrict__ pe, float4* __restrict__ fv) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= N) return;
     const uint4 wp = w[p];
@@ -250,8 +265,9 @@ __global__ void k_log_frame(int64_t N, const int* __restrict__ lidx, const int* 
     const float4 vp = v[p];
     const float speed = sqrtf(vp.x * vp.x + vp.y * vp.y + vp.z * vp.z);
     const int64_t o = lidx[p];
-    pe[o] = make_float4(x[0], x[1], x[2], es[s] + eb[s]);
-    fv[o] = make_float4(fs[s] + fb[s], fs[nslots + s] + fb[nslots + s], fs[2 * nslots + s] + fb[2 * nslots + s], speed);
+    const float4 f = slot_force(b, s);
+    pe[o] = make_float4(x[0], x[1], x[2], f.w);
+    fv[o] = make_float4(f.x, f.y, f.z, speed);
 }
 
 __global__ void k_zero_d(double* p, int n) {
@@ -278,11 +294,15 @@ __global__ void k_sd_trial(int64_t N, long long Lqx, long long Lqy, long long Lq
     it[p] = m;
 }
 
-__global__ void k_sd_accept(SdState* sd, const double* acc, double e_const, const float* fmax_try) {
+// flags[DF_PMEBIN] set: the trial's PME forces are incomplete, so it is refused and h kept (the
+// host grows the bins and repeats the iteration).
+__global__ void k_sd_accept(SdState* sd, const double* acc, double e_const, const float* fmax_try, const int* flags) {
     const double e = acc[ACC_LJ] + acc[ACC_COUL] + acc[ACC_BOND] + acc[ACC_PME] + e_const;
     sd->e_try = e;
     sd->fmax_try = *fmax_try;
-    if (e < sd->e_acc) {
+    if (flags[DF_PMEBIN]) {
+        sd->accepted = 0;
+    } else if (e < sd->e_acc) {
         sd->accepted = 1;
         sd->disp_since_build += sd->h;
         sd->e_acc = e;
@@ -407,11 +427,22 @@ void energy_from_acc(const Engine& e, const double* acc, bool with_kinetic, lima
     E.total = E.potential + E.kinetic;
 }
 
-void fill_energy(Engine& e, bool with_kinetic) {
+// Reads the totals of the last energy evaluation (e.acc) into e.energy; the device must be idle.
+void fill_energy(Engine& e, bool with_kinetic, int64_t step) {
     double acc[ACC_N];
     HIP_CHECK(hipMemcpy(acc, e.acc, sizeof(acc), hipMemcpyDeviceToHost));
     energy_from_acc(e, acc, with_kinetic, e.energy);
+    e.energy.step = step;
     e.pairs_last = (int64_t)acc[ACC_PAIRS];
+    e.energy_unread = false;
+}
+
+SlotForces slot_forces(const Engine& e) {
+    SlotForces b{};
+    b.nslots = slots_max(e);
+    b.f[0] = e.fpair; b.f[1] = e.fintra; b.f[2] = e.fb; b.f[3] = e.fpme;
+    b.e[0] = e.epair; b.e[1] = e.eintra; b.e[2] = e.eb; b.e[3] = e.epme;
+    return b;
 }
 
 // Writes the current step's frame into the data-log ring (after the integration and the energy
@@ -419,7 +450,7 @@ void fill_energy(Engine& e, bool with_kinetic) {
 void log_frame(Engine& e) {
     const int64_t slot = e.log_frames % e.log_cap;
     hipLaunchKernelGGL(k_log_frame, dim3(nblk(e.N, 256)), dim3(256), 0, e.s_main, e.N, e.log_index, e.log_key, e.p.w, e.p.img,
-                       e.p.v, e.nl.slot_of, slots_max(e), e.fs, e.fb, e.es, e.eb, (long long)e.Lq[0], (long long)e.Lq[1],
+                       e.p.v, e.nl.slot_of, slot_forces(e), (long long)e.Lq[0], (long long)e.Lq[1],
                        (long long)e.Lq[2], e.log_pe + slot * e.log_P, e.log_fv + slot * e.log_P);
     HIP_CHECK(hipMemcpyAsync(e.log_acc + slot * ACC_N, e.acc, ACC_N * sizeof(double), hipMemcpyDeviceToDevice, e.s_main));
     e.log_step[slot] = e.step;
@@ -442,12 +473,15 @@ void integrate(Engine& e, bool energy, bool pack = false) {
     for (int d = 0; d < 3; d++) a.Lq[d] = e.Lq[d];
     a.dt = (float)e.dt_ns;
     a.thermostat = e.thermo;
-    a.slot_of = e.nl.slot_of; a.fs = e.fs; a.fb = e.fb; a.w = e.p.w; a.img = e.p.img; a.v = e.p.v; a.fprev = e.p.fprev;
+    a.slot_of = e.nl.slot_of;
+    a.fpair = e.fpair; a.fintra = e.fintra; a.fb = e.fb; a.fpme = e.fpme;
+    a.w = e.p.w; a.img = e.p.img; a.v = e.p.v; a.fprev = e.p.fprev;
     a.prm = e.p.prm; a.epart = e.epart + (int64_t)e.ereg.kin * ACC_N;
     a.w_build = e.w_build; a.img_build = e.img_build; a.dpart = e.dpart;
     a.pack = pack ? 1 : 0;
-    a.cref = e.nl.cref; a.xq = e.nl.xq; a.es = e.es;
+    a.cref = e.nl.cref; a.xq = e.nl.xq;
     if (pack) a.bins = bin_args(e);
+    a.flags = e.dflags;
     e.packed = pack;
     if (energy) hipLaunchKernelGGL(k_integrate<true>, dim3(nblk(e.N, 256)), dim3(256), 0, e.s_main, a);
     else hipLaunchKernelGGL(k_integrate<false>, dim3(nblk(e.N, 256)), dim3(256), 0, e.s_main, a);
@@ -510,41 +544,112 @@ void capture_step_graph(Engine& e) {
     e.graph_captures++;
 }
 
-void md_steps(Engine& e, int64_t n) {
+// Undoes the steps since the last snapshot after a device condition: restores the snapshot's
+// state, grows the capacity that overflowed, and leaves e.step at the snapshot's step with a
+// rebuild due, so the caller issues the lost steps again.  A non-finite force or an overfull
+// column is not recoverable and throws.
+void rollback(Engine& e) {
+    HIP_CHECK(hipStreamSynchronize(e.s_pme));
+    HIP_CHECK(hipStreamSynchronize(e.s_main));
+    int f[DF_N];
+    HIP_CHECK(hipMemcpy(f, e.dflags, sizeof(f), hipMemcpyDeviceToHost));
+    e.rb_pending[0] = e.rb_pending[1] = false;
+    if (f[DF_NONFINITE]) {
+        e.critical = true;
+        throw LimaFail(LIMA_ERR_NONFINITE, "non-finite force in the MD step loop");
+    }
+    if (f[DF_COLUMN]) throw LimaFail(LIMA_ERR_LIST_OVERFLOW, "an x-y column holds more than 2048 particles");
+    const int64_t c = f[DF_SNAPSHOTS];
+    auto it = std::find_if(e.snaps.begin(), e.snaps.end(), [&](const SnapRecord& r) { return r.ordinal == c; });
+    if (it == e.snaps.end()) throw LimaFail(LIMA_ERR_HIP, "internal: no record of the last snapshot");
+    const SnapRecord r = *it;
+    if (++e.rollbacks > 64) throw LimaFail(LIMA_ERR_LIST_OVERFLOW, "capacity overflows persist after 64 rollbacks");
+    if (f[DF_PAIRLIST]) nlist_grow(e, f[DF_LIST_NEED]);
+    if (f[DF_PMEBIN]) pme_grow_bins(e);
+    hipStream_t s = e.s_main;
+    const int64_t N = e.N;
+    HIP_CHECK(hipMemcpyAsync(e.p.w, e.w_build, N * sizeof(uint4), hipMemcpyDeviceToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(e.p.img, e.img_build, N * sizeof(int4), hipMemcpyDeviceToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(e.p.v, e.v_snap, N * sizeof(float4), hipMemcpyDeviceToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(e.p.fprev, e.fprev_snap, N * sizeof(float4), hipMemcpyDeviceToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(e.thermo, e.thermo_snap, 2 * sizeof(float), hipMemcpyDeviceToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(e.lstats, e.lstats_snap, 2 * sizeof(int), hipMemcpyDeviceToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(e.acc, e.acc_snap, ACC_N * sizeof(double), hipMemcpyDeviceToDevice, s));
+    HIP_CHECK(hipMemsetAsync(e.dpart, 0, ((N + 63) / 64) * sizeof(float), s));
+    pme_clear_bins(e, s);
+    reset_flags(e, c - 1, s);
+    HIP_CHECK(hipStreamSynchronize(s));
+    e.snap_issued = c - 1;
+    e.snaps.erase(it, e.snaps.end());
+    e.step = r.step;
+    e.log_frames = r.log_frames;
+    e.n_rebuilds = r.n_rebuilds;
+    e.temp_measured = r.temp_measured;
+    e.energy_step = r.energy_step;
+    e.energy_unread = r.energy_step != -1;   // acc holds that evaluation's totals again
+    e.steps_since_build = INT64_MAX / 2;      // the next step rebuilds from the restored positions
+    e.stats_skip = true;
+    e.packed = false;
+    e.tpos = 0;                               // timing events of undone steps are dropped
+}
+
+// Reads the status words the device copied after the deferred rebuild issued two rebuilds ago,
+// waiting for that rebuild if it is still running (this bounds how far the host runs ahead of the
+// device).  Returns true when a condition was found and the engine rolled back.
+bool poll_rebuild_flags(Engine& e) {
+    const int par = e.rb_parity;
+    if (!e.rb_pending[par]) return false;
+    HIP_CHECK(hipEventSynchronize(e.ev_rebuild[par]));
+    e.rb_pending[par] = false;
+    const int* f = e.hflags + par * DF_N;
+    bool err = false;
+    for (int k = 0; k < DF_ERRORS; k++) err = err || f[k] != 0;
+    if (!err) return false;
+    rollback(e);
+    return true;
+}
+
+// Issues MD steps until e.step == target, without waiting for the device (it is at most two list
+// intervals behind, see poll_rebuild_flags).  Energies are evaluated on every energy_interval-th
+// step and on logged steps; their totals stay on the device until read (settle).
+void issue_steps(Engine& e, int64_t target) {
     const auto t_issue = std::chrono::steady_clock::now();
+    const int64_t step0 = e.step;
     const int interval = e.prm.nlist_interval > 0 ? e.prm.nlist_interval : 20;
-    for (int64_t k = 0; k < n; k++) {
+    while (e.step < target) {
         if (e.steps_since_build >= interval) {
+            if (poll_rebuild_flags(e)) continue;   // rolled back to an earlier step: issue from there
             // timing: drain earlier steps first, so the rebuild's clock holds the rebuild alone
             if (e.timing) HIP_CHECK(hipDeviceSynchronize());
             const auto t0 = std::chrono::steady_clock::now();
-            nlist_build(e, e.p.w);   // synchronises once (entry count read-back)
+            nlist_build(e, e.p.w, /*checked=*/false);
             if (e.timing) {
                 HIP_CHECK(hipStreamSynchronize(e.s_main));
                 e.tacc.nlist_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
             }
         }
-        // energy steps: every energy_interval-th step and the last step of the call (their totals are
-        // read back and checked), and the logged steps (totals kept on the device with the frame)
-        const bool energy_step = (e.prm.energy_interval > 0 && e.step % e.prm.energy_interval == 0) || k == n - 1;
+        const bool energy_step = e.prm.energy_interval > 0 && e.step % e.prm.energy_interval == 0;
         const bool logged = e.log_cap > 0 && e.step % e.prm.data_logging_interval == 0;
         const bool energy = energy_step || logged;
         if (!energy && !e.timing && !(e.prm.launch_mode & LAUNCH_EAGER)) {
             // the graph holds the evaluation without k_pack and an integrator that packs for the next
             // step; after a list rebuild (or an unpacked eager step) k_pack runs first
-            const bool need_pack = !e.packed;
-            if (need_pack) launch_pack(e, e.p.w, e.s_main);
+            if (!e.packed) launch_pack(e, e.p.w, e.s_main);
             if (!e.step_graph_valid) capture_step_graph(e);
             HIP_CHECK(hipGraphLaunch(e.step_graph, e.s_main));
             e.packed = true;
         } else {
             evaluate(e, e.p.w, e.p.img, energy, e.timing, !e.packed);
-            // the integrator packs for the next step when that step keeps the list; the last step
-            // leaves the force slots intact for lima_get_state, a logged step for its frame
-            const bool pack_next = !e.timing && !logged && k + 1 < n && e.steps_since_build + 1 < interval;
+            // the integrator packs for the next step when that step keeps the list (the force
+            // buffers are not touched by packing, so the last evaluation stays readable)
+            const bool pack_next = !e.timing && e.steps_since_build + 1 < interval;
             integrate(e, energy, pack_next);
         }
-        if (energy) launch_reduce_energy(e, true, e.s_main);
+        if (energy) {
+            launch_reduce_energy(e, true, e.s_main);
+            e.energy_step = e.step;
+            e.energy_unread = true;
+        }
         if (logged) log_frame(e);
         if (e.timing) {
             HIP_CHECK(hipEventRecord(tslot(e)[6], e.s_main));
@@ -553,16 +658,34 @@ void md_steps(Engine& e, int64_t n) {
         e.step++;
         e.steps_since_build++;
         if (temperature_due(e)) measure_temperature(e);
-        if (energy_step) {
-            fill_energy(e, true);
-            check_finite(e);
-        }
     }
     e.forces_valid = true;
     e.host_issue_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_issue).count();
-    e.host_steps += n;
-    HIP_CHECK(hipStreamSynchronize(e.s_main));
+    e.host_steps += std::max<int64_t>(0, e.step - step0);
     HIP_CHECK(hipGetLastError());
+}
+
+// Waits for every issued step, resolves device conditions (rolling back and issuing again as
+// often as needed), and reads unread energy totals.  Every call that returns data runs it first.
+void settle(Engine& e) {
+    for (;;) {
+        HIP_CHECK(hipStreamSynchronize(e.s_pme));
+        HIP_CHECK(hipStreamSynchronize(e.s_main));
+        HIP_CHECK(hipGetLastError());
+        int f[DF_N];
+        HIP_CHECK(hipMemcpy(f, e.dflags, sizeof(f), hipMemcpyDeviceToHost));
+        e.rb_pending[0] = e.rb_pending[1] = false;
+        bool err = false;
+        for (int k = 0; k < DF_ERRORS; k++) err = err || f[k] != 0;
+        if (!err) break;
+        const int64_t target = e.step;
+        rollback(e);
+        issue_steps(e, target);
+    }
+    if (e.energy_unread) {
+        fill_energy(e, true, e.energy_step);
+        check_finite(e);
+    }
 }
 
 void create(Engine& e, const lima_box_desc* b, const lima_simparams* p) {
@@ -731,6 +854,15 @@ void create(Engine& e, const lima_box_desc* b, const lima_simparams* p) {
     e.n_intra = (int)items.size();
     e.intra_items = dalloc_copy(items);
     e.intra_lut = dalloc_copy(ilut);
+    {   // one group per compound with items (its items are consecutive)
+        std::vector<int2> groups;
+        for (int k = 0; k < e.n_intra; k++) {
+            if (groups.empty() || items[groups.back().x].x != items[k].x) groups.push_back(make_int2(k, 0));
+            groups.back().y++;
+        }
+        e.n_intra_groups = (int)groups.size();
+        e.intra_groups = dalloc_copy(groups);
+    }
     e.first = dalloc_copy(e.h_first);
     e.bmask = dalloc_copy(bm);
     // ---- bonded terms ----
@@ -834,14 +966,27 @@ void create(Engine& e, const lima_box_desc* b, const lima_simparams* p) {
         e.bd.pid = dalloc_copy(pid); e.bd.pp = dalloc_copy(pp);
     }
     HIP_CHECK(hipMalloc(&e.acc, ACC_N * sizeof(double)));
+    HIP_CHECK(hipMemset(e.acc, 0, ACC_N * sizeof(double)));
+    HIP_CHECK(hipMalloc(&e.acc_part, REDUCE_BLOCKS * ACC_N * sizeof(double)));
     HIP_CHECK(hipMalloc(&e.fmax, sizeof(float)));
+    // status words, their pinned host copies and the snapshot buffers
+    HIP_CHECK(hipMalloc(&e.dflags, DF_N * sizeof(int)));
+    HIP_CHECK(hipMemset(e.dflags, 0, DF_N * sizeof(int)));
+    HIP_CHECK(hipHostMalloc((void**)&e.hflags, 2 * DF_N * sizeof(int), hipHostMallocDefault));
+    std::memset(e.hflags, 0, 2 * DF_N * sizeof(int));
+    HIP_CHECK(hipMalloc(&e.v_snap, N * sizeof(float4)));
+    HIP_CHECK(hipMalloc(&e.fprev_snap, N * sizeof(float4)));
+    HIP_CHECK(hipMalloc(&e.thermo_snap, 2 * sizeof(float)));
+    HIP_CHECK(hipMalloc(&e.lstats_snap, 2 * sizeof(int)));
+    HIP_CHECK(hipMalloc(&e.acc_snap, ACC_N * sizeof(double)));
     HIP_CHECK(hipStreamCreate(&e.s_main));
     HIP_CHECK(hipStreamCreate(&e.s_pme));
     HIP_CHECK(hipEventCreateWithFlags(&e.ev_fork, hipEventDisableTiming));
     HIP_CHECK(hipEventCreateWithFlags(&e.ev_join, hipEventDisableTiming));
+    for (auto& ev : e.ev_rebuild) HIP_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     for (auto& ev : e.tev) HIP_CHECK(hipEventCreate(&ev));
     nlist_alloc(e);
-    pme_alloc(e, qv);
+    pme_alloc(e, qv, w);
     // energy records: regions sized by capacities, fixed for the engine's lifetime
     e.ereg = energy_regions(e);
     e.epart_cap = e.ereg.total;
@@ -870,8 +1015,9 @@ void create(Engine& e, const lima_box_desc* b, const lima_simparams* p) {
         HIP_CHECK(hipMalloc(&e.log_acc, (size_t)e.log_cap * ACC_N * sizeof(double)));
         e.log_step.assign(e.log_cap, -1);
     }
-    nlist_build(e, e.p.w);
+    nlist_build(e, e.p.w, /*checked=*/true);
     HIP_CHECK(hipStreamSynchronize(e.s_main));
+    e.energy.step = -1;
 }
 
 void destroy(Engine& e) {
@@ -879,10 +1025,13 @@ void destroy(Engine& e) {
     nlist_free(e);
     pme_free(e);
     void* ptrs[] = {e.p.w, e.p.img, e.p.v, e.p.fprev, e.p.prm, e.p.comp, e.w_build, e.img_build, e.dmax2, e.lstats, e.thermo, e.dpart, e.epart, e.w_try, e.img_try, e.f_acc, e.sd, e.first,
-                    e.bmask, e.intra_items, e.intra_lut, e.bd.bid, e.bd.bp, e.bd.aid, e.bd.ap, e.bd.did, e.bd.dp,
-                    e.bd.iid, e.bd.ip, e.bd.pid, e.bd.pp, e.bd.lane_code, e.bd.lane_atom, e.coul_tab, e.acc, e.fmax,
-                    e.log_pe, e.log_fv, e.log_acc, e.log_index, e.log_key};
+                    e.bmask, e.intra_items, e.intra_lut, e.intra_groups, e.bd.bid, e.bd.bp, e.bd.aid, e.bd.ap, e.bd.did, e.bd.dp,
+                    e.bd.iid, e.bd.ip, e.bd.pid, e.bd.pp, e.bd.lane_code, e.bd.lane_atom, e.coul_tab, e.acc, e.acc_part, e.fmax,
+                    e.log_pe, e.log_fv, e.log_acc, e.log_index, e.log_key, e.dflags, e.v_snap, e.fprev_snap, e.thermo_snap,
+                    e.lstats_snap, e.acc_snap};
     for (void* p : ptrs) if (p) (void)hipFree(p);
+    if (e.hflags) (void)hipHostFree(e.hflags);
+    for (auto& ev : e.ev_rebuild) if (ev) (void)hipEventDestroy(ev);
     for (auto& ev : e.tev) if (ev) (void)hipEventDestroy(ev);
     if (e.step_graph) (void)hipGraphExecDestroy(e.step_graph);
     e.step_graph = nullptr;
@@ -892,10 +1041,11 @@ void destroy(Engine& e) {
     if (e.s_pme) (void)hipStreamDestroy(e.s_pme);
 }
 
-void forces_global(Engine& e) {
+// Forces (and potE) of the last evaluation in global order into out, max |F| into e.fmax.
+void forces_global(Engine& e, float4* out) {
     hipLaunchKernelGGL(k_zero_f, dim3(1), dim3(64), 0, e.s_main, e.fmax, 1);
-    hipLaunchKernelGGL(k_forces_global, dim3(nblk(e.N, 256)), dim3(256), 0, e.s_main, e.N, slots_max(e), e.nl.slot_of,
-                       e.fs, e.es, e.fb, e.eb, e.f_acc, e.fmax);
+    hipLaunchKernelGGL(k_forces_global, dim3(nblk(e.N, 256)), dim3(256), 0, e.s_main, e.N, e.nl.slot_of, slot_forces(e), out,
+                       e.fmax);
 }
 
 }  // namespace
@@ -928,21 +1078,47 @@ lima_status lima_destroy(lima_engine* h) {
 lima_status lima_step(lima_engine* h, int64_t nsteps) {
     if (!h) { set_err("null engine"); return LIMA_ERR_BAD_INPUT; }
     if (h->e.critical) { set_err("engine is in a critical error state"); return LIMA_ERR_NONFINITE; }
-    return guarded([&] { md_steps(h->e, nsteps); });
+    if (nsteps < 0) { set_err("negative step count"); return LIMA_ERR_BAD_INPUT; }
+    return guarded([&] { issue_steps(h->e, h->e.step + nsteps); });
+}
+
+lima_status lima_synchronize(lima_engine* h) {
+    if (!h) { set_err("null engine"); return LIMA_ERR_BAD_INPUT; }
+    return guarded([&] { settle(h->e); });
+}
+
+// One evaluation with energies at the current positions into e.energy; a PME tile bin that
+// overflows (the only capacity an evaluation can exceed after a checked rebuild) is grown and the
+// evaluation repeated.
+void compute_forces_now(Engine& e, bool with_kinetic) {
+    for (int attempt = 0;; attempt++) {
+        nlist_build(e, e.p.w, /*checked=*/true);
+        evaluate(e, e.p.w, e.p.img, true);
+        if (with_kinetic)
+            hipLaunchKernelGGL(k_kinetic, dim3(nblk(e.N, 256)), dim3(256), 0, e.s_main, e.N, e.p.v, e.p.prm,
+                               e.epart + (int64_t)e.ereg.kin * ACC_N);
+        launch_reduce_energy(e, with_kinetic, e.s_main);
+        HIP_CHECK(hipStreamSynchronize(e.s_main));
+        HIP_CHECK(hipGetLastError());
+        int f[DF_N];
+        HIP_CHECK(hipMemcpy(f, e.dflags, sizeof(f), hipMemcpyDeviceToHost));
+        if (!f[DF_PMEBIN]) break;
+        if (attempt >= 6) throw LimaFail(LIMA_ERR_LIST_OVERFLOW, "PME tile bin capacity exceeded");
+        pme_grow_bins(e);
+        reset_flags(e, f[DF_SNAPSHOTS], e.s_main);
+        pme_clear_bins(e, e.s_main);
+    }
+    e.packed = false;
 }
 
 lima_status lima_compute_forces(lima_engine* h, lima_energy* out) {
     if (!h) { set_err("null engine"); return LIMA_ERR_BAD_INPUT; }
     return guarded([&] {
         Engine& e = h->e;
-        nlist_build(e, e.p.w);
-        evaluate(e, e.p.w, e.p.img, true);
-        hipLaunchKernelGGL(k_kinetic, dim3(nblk(e.N, 256)), dim3(256), 0, e.s_main, e.N, e.p.v, e.p.prm,
-                           e.epart + (int64_t)e.ereg.kin * ACC_N);
-        launch_reduce_energy(e, true, e.s_main);
-        HIP_CHECK(hipStreamSynchronize(e.s_main));
-        HIP_CHECK(hipGetLastError());
-        fill_energy(e, true);
+        settle(e);
+        compute_forces_now(e, true);
+        fill_energy(e, true, e.step);
+        e.energy_step = e.step;
         e.forces_valid = true;
         if (out) *out = e.energy;
     });
@@ -950,21 +1126,22 @@ lima_status lima_compute_forces(lima_engine* h, lima_energy* out) {
 
 lima_status lima_get_energy(lima_engine* h, lima_energy* out) {
     if (!h || !out) { set_err("null argument"); return LIMA_ERR_BAD_INPUT; }
-    *out = h->e.energy;
-    return LIMA_OK;
+    return guarded([&] {
+        settle(h->e);
+        *out = h->e.energy;
+    });
 }
 
 lima_status lima_minimize(lima_engine* h, double tol_kj, int64_t max_iter, double h0, lima_em_result* res) {
     if (!h) { set_err("null engine"); return LIMA_ERR_BAD_INPUT; }
     return guarded([&] {
         Engine& e = h->e;
+        settle(e);
         const auto t0 = std::chrono::steady_clock::now();
         hipStream_t s = e.s_main;
         // initial evaluation at the current positions
-        nlist_build(e, e.p.w);
-        evaluate(e, e.p.w, e.p.img, true);
-        launch_reduce_energy(e, false, s);
-        forces_global(e);
+        compute_forces_now(e, false);
+        forces_global(e, e.f_acc);
         SdState sd{};
         {
             double acc[ACC_N];
@@ -982,7 +1159,7 @@ lima_status lima_minimize(lima_engine* h, double tol_kj, int64_t max_iter, doubl
         bool conv = sd.fmax_acc < tol;
         while (!conv && it < max_iter) {
             if (sd.disp_since_build + sd.h > 0.5 * e.skin) {
-                nlist_build(e, e.p.w);
+                nlist_build(e, e.p.w, /*checked=*/true);
                 sd.disp_since_build = 0;
                 HIP_CHECK(hipMemcpyAsync(&e.sd->disp_since_build, &sd.disp_since_build, sizeof(double), hipMemcpyHostToDevice, s));
             }
@@ -991,25 +1168,33 @@ lima_status lima_minimize(lima_engine* h, double tol_kj, int64_t max_iter, doubl
             evaluate(e, e.w_try, e.img_try, true);
             launch_reduce_energy(e, false, s);
             // trial forces in global order -> p.fprev used as scratch
-            hipLaunchKernelGGL(k_zero_f, dim3(1), dim3(64), 0, s, e.fmax, 1);
-            hipLaunchKernelGGL(k_forces_global, dim3(nblk(e.N, 256)), dim3(256), 0, s, e.N, slots_max(e), e.nl.slot_of,
-                               e.fs, e.es, e.fb, e.eb, e.p.fprev, e.fmax);
-            hipLaunchKernelGGL(k_sd_accept, dim3(1), dim3(1), 0, s, e.sd, e.acc, e_const(e), e.fmax);
+            forces_global(e, e.p.fprev);
+            hipLaunchKernelGGL(k_sd_accept, dim3(1), dim3(1), 0, s, e.sd, e.acc, e_const(e), e.fmax, (const int*)e.dflags);
             hipLaunchKernelGGL(k_sd_commit, dim3(nblk(e.N, 256)), dim3(256), 0, s, e.N, e.sd, e.w_try, e.img_try, e.p.fprev,
                                e.p.w, e.p.img, e.f_acc);
+            int f[DF_N];
             HIP_CHECK(hipMemcpyAsync(&sd, e.sd, sizeof(sd), hipMemcpyDeviceToHost, s));
+            HIP_CHECK(hipMemcpyAsync(f, e.dflags, sizeof(f), hipMemcpyDeviceToHost, s));
             HIP_CHECK(hipStreamSynchronize(s));
+            if (f[DF_PMEBIN]) {   // the trial was refused without a change of h: grow the bins and repeat it
+                if (e.pme.bincap > (1 << 20)) throw LimaFail(LIMA_ERR_LIST_OVERFLOW, "PME tile bin capacity exceeded");
+                pme_grow_bins(e);
+                reset_flags(e, f[DF_SNAPSHOTS], s);
+                pme_clear_bins(e, s);
+                continue;
+            }
             it++;
             conv = sd.fmax_acc < tol;
             if (!std::isfinite(sd.e_acc)) throw LimaFail(LIMA_ERR_NONFINITE, "non-finite energy in minimisation");
             if (sd.h < 1e-12f) break;
         }
-        // leave forces / energies of the accepted state in the engine
-        evaluate(e, e.p.w, e.p.img, true);
-        launch_reduce_energy(e, false, s);
+        // leave forces / energies of the accepted state in the engine, with a list built at the
+        // final positions (MD that follows starts with its full skin)
+        compute_forces_now(e, false);
         HIP_CHECK(hipMemsetAsync(e.p.fprev, 0, e.N * sizeof(float4), s));
         HIP_CHECK(hipStreamSynchronize(s));
-        fill_energy(e, false);
+        fill_energy(e, false, e.step);
+        e.energy_step = e.step;
         e.forces_valid = true;
         if (res) {
             res->iterations = it;
@@ -1026,6 +1211,7 @@ lima_status lima_get_state(lima_engine* h, lima_state_out* out) {
     if (!h || !out) { set_err("null argument"); return LIMA_ERR_BAD_INPUT; }
     return guarded([&] {
         Engine& e = h->e;
+        settle(e);
         const int64_t N = e.N;
         std::vector<uint4> w(N);
         std::vector<int4> img(N);
@@ -1034,7 +1220,7 @@ lima_status lima_get_state(lima_engine* h, lima_state_out* out) {
         HIP_CHECK(hipMemcpy(img.data(), e.p.img, N * sizeof(int4), hipMemcpyDeviceToHost));
         HIP_CHECK(hipMemcpy(v.data(), e.p.v, N * sizeof(float4), hipMemcpyDeviceToHost));
         if (e.forces_valid) {
-            forces_global(e);
+            forces_global(e, e.f_acc);
             HIP_CHECK(hipStreamSynchronize(e.s_main));
             HIP_CHECK(hipMemcpy(f.data(), e.f_acc, N * sizeof(float4), hipMemcpyDeviceToHost));
         } else {
@@ -1085,19 +1271,22 @@ lima_status lima_get_state(lima_engine* h, lima_state_out* out) {
 
 lima_status lima_get_log_info(lima_engine* h, lima_log_info* out) {
     if (!h || !out) { set_err("null argument"); return LIMA_ERR_BAD_INPUT; }
-    const Engine& e = h->e;
-    out->frames_logged = e.log_frames;
-    out->first_held = e.log_frames - std::min<int64_t>(e.log_frames, e.log_cap);
-    out->particles_per_frame = (int64_t)e.nC * LIMA_MAX_COMPOUND_PARTICLES + e.nT;
-    out->interval = e.prm.data_logging_interval;
-    out->capacity = e.log_cap;
-    return LIMA_OK;
+    return guarded([&] {
+        Engine& e = h->e;
+        settle(e);
+        out->frames_logged = e.log_frames;
+        out->first_held = e.log_frames - std::min<int64_t>(e.log_frames, e.log_cap);
+        out->particles_per_frame = (int64_t)e.nC * LIMA_MAX_COMPOUND_PARTICLES + e.nT;
+        out->interval = e.prm.data_logging_interval;
+        out->capacity = e.log_cap;
+    });
 }
 
 lima_status lima_read_log(lima_engine* h, int64_t first, int64_t n, lima_log_out* out) {
     if (!h || !out) { set_err("null argument"); return LIMA_ERR_BAD_INPUT; }
     return guarded([&] {
         Engine& e = h->e;
+        settle(e);
         const int64_t held0 = e.log_frames - std::min<int64_t>(e.log_frames, e.log_cap);
         if (n < 0 || first < held0 || first + n > e.log_frames)
             throw LimaFail(LIMA_ERR_BAD_INPUT, "frames " + std::to_string(first) + " + " + std::to_string(n) + " are not held (held: " +
@@ -1134,6 +1323,7 @@ lima_status lima_get_status(lima_engine* h, lima_runstatus* out) {
     if (!h || !out) { set_err("null argument"); return LIMA_ERR_BAD_INPUT; }
     Engine& e = h->e;
     std::memset(out, 0, sizeof(*out));
+    const lima_status status = e.critical ? LIMA_OK : guarded([&] { settle(e); });
     out->current_step = e.step;
     out->current_temperature = NAN;   // RunStatus default until the first measurement (Engine.cuh:67)
     if (e.temp_measured) {
@@ -1158,14 +1348,17 @@ lima_status lima_get_status(lima_engine* h, lima_runstatus* out) {
         out->max_displacement_nm = d;
     }
     out->greatest_force = NAN;
-    return LIMA_OK;
+    out->capacity_rollbacks = e.rollbacks;
+    return status;
 }
 
 lima_status lima_get_neighbor_counts(lima_engine* h, int64_t* ncl, int64_t* nent) {
     if (!h || !ncl || !nent) { set_err("null argument"); return LIMA_ERR_BAD_INPUT; }
     return guarded([&] {
         Engine& e = h->e;
-        nlist_build(e, e.p.w);
+        settle(e);
+        nlist_build(e, e.p.w, /*checked=*/true);
+        e.packed = false;
         HIP_CHECK(hipStreamSynchronize(e.s_main));
         const int nc = nlist_count(e);
         std::vector<int> cnt(nc);
@@ -1189,6 +1382,7 @@ lima_status lima_get_neighbor_list(lima_engine* h, int32_t* cluster_atoms, int64
     if (!h) { set_err("null engine"); return LIMA_ERR_BAD_INPUT; }
     return guarded([&] {
         Engine& e = h->e;
+        settle(e);
         const int ncl = nlist_count(e);
         const int64_t cap = e.nl.cap;
         std::vector<int> cat((size_t)ncl * CLUSTER), cnt(ncl);
@@ -1222,6 +1416,7 @@ lima_status lima_get_neighbor_list(lima_engine* h, int32_t* cluster_atoms, int64
 lima_status lima_get_timings(lima_engine* h, lima_timings* out) {
     if (!h || !out) { set_err("null argument"); return LIMA_ERR_BAD_INPUT; }
     return guarded([&] {
+        settle(h->e);
         timing_flush(h->e);
         *out = h->e.tacc;
         out->host_issue_ms = h->e.host_issue_ms;
@@ -1232,6 +1427,7 @@ lima_status lima_get_timings(lima_engine* h, lima_timings* out) {
 lima_status lima_set_timing(lima_engine* h, int32_t enabled) {
     if (!h) { set_err("null engine"); return LIMA_ERR_BAD_INPUT; }
     return guarded([&] {
+        settle(h->e);
         timing_flush(h->e);
         h->e.timing = enabled != 0;
         h->e.tacc = lima_timings{};

@@ -5,12 +5,14 @@
 // A wave owns one i-cluster (8 particles) and walks its entry list 8 entries at a time:
 // lane = 8*jj + ii computes pair (i-slot ii, entry jj), so one wave instruction covers 64 pairs.
 // The list is full (both sides of every pair, as LIMA evaluates them): only i forces are
-// accumulated, so the loop has no atomics and the pair forces are deterministic.
+// accumulated, so the loop has no atomics; a cluster split over several waves of one workgroup
+// sums their shares in LDS in a fixed order, and one wave stores the result (deterministic).
 // FP32 VALU bound: about 50 VALU slots per evaluated pair (pair_scalar_cut in lima_dev.h).
 //
 // Intra kernel: LIMA's immediate-neighbour path (EngineKernels.cuh:150-263,
 // LennardJonesInteractions.cuh:121-161): pairs of the same or bonded compounds, BondedParticlesLUT
-// exclusions, no cutoff, force on self only (as LIMA evaluates it).  Within a compound the
+// exclusions, no cutoff, force on self only (as LIMA evaluates it); one wave per compound walks its
+// items (itself and its bonded compounds) and stores the compound's forces once.  Within a compound the
 // displacement is unwrapped; a bonded compound is first moved to the image whose key particle is
 // nearest (LIMA's hyperpos of the neighbour's origo, EngineKernels.cuh:235-236).
 //
@@ -33,13 +35,14 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
     return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
 }
 
-// Slot k of the cluster list: position relative to the cluster reference point and charge,
-// force / energy accumulators cleared, and (tiled PME) the charged particle put in its tile bin.
-// nslots: the slot capacity (SoA stride); slots of clusters beyond the current count are left alone.
+// Slot k of the cluster list: position relative to the cluster reference point and charge, and
+// (tiled PME) the charged particle put in its tile bin.  nslots: the slot capacity (SoA stride);
+// slots of clusters beyond the current count are left alone.  The force buffers need no clearing:
+// every kernel overwrites the slots it owns.
 __global__ void k_pack(const int* __restrict__ ncl_dev, int nslots, const int* __restrict__ catoms, const uint4* __restrict__ w,
                        const float4* __restrict__ prm,
                        const uint4* __restrict__ cref, long long Lqx, long long Lqy, long long Lqz,
-                       float4* __restrict__ xq, float* __restrict__ fs, float* __restrict__ es, BinArgs bins) {
+                       float4* __restrict__ xq, BinArgs bins) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = k < min(*ncl_dev * CLUSTER, nslots);
     const int a = live ? catoms[k] : -1;
@@ -60,12 +63,25 @@ __global__ void k_pack(const int* __restrict__ ncl_dev, int nslots, const int* _
             out = make_float4((float)d[0] * LIMA_LIMA_TO_NANO_F, (float)d[1] * LIMA_LIMA_TO_NANO_F, (float)d[2] * LIMA_LIMA_TO_NANO_F, q);
         }
         xq[k] = out;
-        fs[k] = 0.f;
-        fs[nslots + k] = 0.f;
-        fs[2 * nslots + k] = 0.f;
-        es[k] = 0.f;
     }
     if (bins.bin_count) bin_wave(bins, a >= 0 && q != 0.f, a, p);
+}
+
+// The per-kernel force and potE buffers of a new list: slots that a kernel does not own keep zero
+// until the next rebuild; no particle in any slot (catoms -1) until assigned.
+__global__ void k_clear_slots(int ns, int* __restrict__ catoms, float* __restrict__ fintra, float* __restrict__ eintra,
+                              float* __restrict__ fb, float* __restrict__ eb, float* __restrict__ fpme, float* __restrict__ epme,
+                              float* __restrict__ epair, float* __restrict__ ephi) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ns) return;
+    catoms[i] = -1;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        fintra[(int64_t)d * ns + i] = 0.f;
+        fb[(int64_t)d * ns + i] = 0.f;
+        fpme[(int64_t)d * ns + i] = 0.f;
+    }
+    eintra[i] = 0.f; eb[i] = 0.f; epme[i] = 0.f; epair[i] = 0.f; ephi[i] = 0.f;
 }
 
 struct PairArgs {
@@ -80,8 +96,8 @@ struct PairArgs {
     const float2* ljs;
     const int* ecount;
     const float4* ent;
-    float* fs;
-    float* es;
+    float* fpair;    // [3][nslots]
+    float* epair;    // [nslots]
     double* epart;   // this kernel's energy region: one record per wave
 };
 
@@ -103,27 +119,24 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs a) {
     __shared__ float4 sA[PAIR_WAVES][2][CHUNK];
     __shared__ float4 sB[PAIR_WAVES][2][CHUNK];
     __shared__ float4 tab[COUL_TAB];
+    __shared__ float4 part_f[PAIR_WAVES][CLUSTER];   // each wave's share of its cluster's i forces (+ potE)
     for (int i = threadIdx.x; i < COUL_TAB; i += blockDim.x) tab[i] = a.coul_tab[i];
     __syncthreads();
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    // a workgroup holds PAIR_WAVES / split whole clusters: the waves of one cluster are consecutive
     const int wv = xcd_block(blockIdx.x, gridDim.x) * PAIR_WAVES + wid;
     const int I = wv / a.split, part = wv - I * a.split;
-    if (I >= min(*a.ncl_dev, a.ncl_max)) {   // the grid covers the capacity
-        if (ENERGY && lane == 0 && wv < a.ncl_max * a.split) {   // no stale records in the energy sum
-            double* r = a.epart + (int64_t)wv * ACC_N;
-            for (int c = 0; c < ACC_N; c++) r[c] = 0.0;
-        }
-        return;
-    }
+    const bool live = I < min(*a.ncl_dev, a.ncl_max);   // the grid covers the capacity (uniform per wave)
     const int ii = lane & 7, jj = lane >> 3;
     const int islot = I * CLUSTER + ii;
-    const float4 xi = a.xq[islot];
-    const float2 pi = a.ljs[islot];
+    float4 xi = make_float4(0.f, 0.f, 0.f, 0.f);
+    float2 pi = make_float2(0.f, 0.f);
+    if (live) { xi = a.xq[islot]; pi = a.ljs[islot]; }
     const float qi_kc = xi.w * a.pc.kc;
     float fx = 0.f, fy = 0.f, fz = 0.f;
     double elj = 0.0, ecl = 0.0;   // FP64 energy accumulation (energy steps only)
     int npairs = 0;
-    const int ntot = a.ecount[I] / CLUSTER;
+    const int ntot = live ? a.ecount[I] / CLUSTER : 0;
     const int it0 = (int)((int64_t)ntot * part / a.split), it1 = (int)((int64_t)ntot * (part + 1) / a.split);
     const int64_t e0 = (int64_t)I * a.cap + (int64_t)it0 * CLUSTER;
     const int nent = (it1 - it0) * CLUSTER;          // entries of this wave (multiple of 8)
@@ -187,18 +200,36 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs a) {
         fy += __shfl_xor(fy, o, 64);
         fz += __shfl_xor(fz, o, 64);
     }
-    if (jj == 0) {
-        // atomics: the intra, bonded and PME kernels (PME on its own stream) add to the same slots
-        atomicAdd(&a.fs[islot], fx);
-        atomicAdd(&a.fs[a.nslots + islot], fy);
-        atomicAdd(&a.fs[2 * a.nslots + islot], fz);
+    float ep = 0.f;
+    if (ENERGY) {
+        ep = (float)(elj + ecl);
+        for (int o = 8; o < 64; o <<= 1) ep += __shfl_xor(ep, o, 64);
+    }
+    if (a.split == 1) {
+        if (live && jj == 0) {
+            a.fpair[islot] = fx;
+            a.fpair[a.nslots + islot] = fy;
+            a.fpair[2 * a.nslots + islot] = fz;
+            if (ENERGY) a.epair[islot] = ep;
+        }
+    } else {   // the cluster's waves meet in LDS; its first wave adds the shares in part order
+        if (jj == 0) part_f[wid][ii] = make_float4(fx, fy, fz, ep);
+        __syncthreads();
+        if (live && part == 0 && jj == 0) {
+            float4 t = part_f[wid][ii];
+            for (int q = 1; q < a.split; q++) {
+                const float4 u = part_f[wid + q][ii];
+                t.x += u.x; t.y += u.y; t.z += u.z; t.w += u.w;
+            }
+            a.fpair[islot] = t.x;
+            a.fpair[a.nslots + islot] = t.y;
+            a.fpair[2 * a.nslots + islot] = t.z;
+            if (ENERGY) a.epair[islot] = t.w;
+        }
     }
     if (ENERGY) {
-        float ep = (float)(elj + ecl);
-        for (int o = 8; o < 64; o <<= 1) ep += __shfl_xor(ep, o, 64);
-        if (jj == 0) atomicAdd(&a.es[islot], ep);
         const double tl = wave_sum_d(elj), tc = wave_sum_d(ecl);
-        if (lane == 0) {
+        if (lane == 0 && wv < a.ncl_max * a.split) {   // waves past the live count write zero records
             double* r = a.epart + (int64_t)wv * ACC_N;
             r[ACC_LJ] = tl; r[ACC_COUL] = tc; r[ACC_BOND] = 0.0; r[ACC_PME] = 0.0; r[ACC_KIN] = 0.0;
             r[ACC_PAIRS] = (double)npairs * 0.5;   // both sides counted
@@ -207,18 +238,19 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs a) {
 }
 
 struct IntraArgs {
-    int n_items, nslots;
+    int n_groups, nslots;
     long long Lq[3];
     PairConst pc;
-    const int4* items;
+    const int2* groups;   // (first item, item count) per compound A
+    const int4* items;    // (A, B, key particle of A, key particle of B)
     const uint32_t* lut;
     const uint4* w;
     const int4* img;
     const float4* prm;
     const int* slot_of;
     const int64_t* first;
-    float* fs;
-    float* es;
+    float* fintra;   // [3][nslots]
+    float* eintra;   // [nslots]
     double* epart;
 };
 
@@ -230,9 +262,10 @@ __device__ __forceinline__ void unwrapped3(const uint4* w, const int4* img, cons
     u[2] = (long long)p.z + (long long)m.z * Lq[2];
 }
 
-// One wave per (A, B) item: compound B's particles (unwrapped Coord, parameters) are staged in the
-// wave's LDS slice, then lanes 0..31 take i = lane with j in [0,16) and lanes 32..63 the same i
-// with j in [16,32).
+// One wave per compound A: for each of A's items (A itself and its bonded compounds B), B's
+// particles (unwrapped Coord, parameters) are staged in the wave's LDS slice, then lanes 0..31 take
+// i = lane with j in [0,16) and lanes 32..63 the same i with j in [16,32).  A's forces accumulate
+// in registers over its items and are stored once.
 struct IntraLds {
     long long u[4][32][3];
     float4 p[4][32];
@@ -241,61 +274,69 @@ struct IntraLds {
 template <bool ENERGY>
 __device__ __forceinline__ void intra_items(const IntraArgs& a, int blk, IntraLds& L) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const int it = blk * 4 + wid;
-    if (it >= a.n_items) return;
-    const int4 item = a.items[it];
-    const int A = item.x, B = item.y;
-    const int nA = (int)(a.first[A + 1] - a.first[A]), nB = (int)(a.first[B + 1] - a.first[B]);
-    long long sh[3] = {0, 0, 0};   // B's image: its key particle nearest to A's (LIMA's hyperpos)
-    if (A != B) {
-        long long ka[3], kb[3];
-        unwrapped3(a.w, a.img, a.Lq, item.z, ka);
-        unwrapped3(a.w, a.img, a.Lq, item.w, kb);
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-            const long long dk = kb[d] - ka[d];
-            long long r = dk % a.Lq[d];
-            r += r < 0 ? a.Lq[d] : 0;
-            r -= 2 * r > a.Lq[d] ? a.Lq[d] : 0;
-            sh[d] = r - dk;
-        }
-    }
-    if (lane < nB) {
-        const int bj = (int)a.first[B] + lane;
-        long long ub[3];
-        unwrapped3(a.w, a.img, a.Lq, bj, ub);
-        L.u[wid][lane][0] = ub[0] + sh[0];
-        L.u[wid][lane][1] = ub[1] + sh[1];
-        L.u[wid][lane][2] = ub[2] + sh[2];
-        L.p[wid][lane] = a.prm[bj];
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int g = blk * 4 + wid;
+    if (g >= a.n_groups) return;
+    const int2 grp = a.groups[g];
+    const int A = a.items[grp.x].x;
+    const int nA = (int)(a.first[A + 1] - a.first[A]);
     const int i = lane & 31, half = lane >> 5;
-    float fx = 0.f, fy = 0.f, fz = 0.f, el_s = 0.f, ec_s = 0.f;
     const int ai = (int)a.first[A] + i;
-    if (i < nA) {
-        long long ui[3];
-        unwrapped3(a.w, a.img, a.Lq, ai, ui);
-        const float4 pi = a.prm[ai];
-        const uint32_t row = a.lut[it * 32 + i];
-        for (int j = half * 16; j < half * 16 + 16 && j < nB; j++) {
-            if ((row >> j) & 1u) continue;
-            const long long* uj = L.u[wid][j];
-            const float dx = (float)(uj[0] - ui[0]) * LIMA_LIMA_TO_NANO_F;
-            const float dy = (float)(uj[1] - ui[1]) * LIMA_LIMA_TO_NANO_F;
-            const float dz = (float)(uj[2] - ui[2]) * LIMA_LIMA_TO_NANO_F;
-            const float r2 = dx * dx + dy * dy + dz * dz;
-            if (!(r2 > 0.f)) continue;
-            const float4 pj = L.p[wid][j];
-            float el = 0.f, ec = 0.f;
-            // no cutoff here: pair_scalar_cut's erfc polynomial extrapolates beyond kappa r = 3.25 with an
-            // absolute error below 1e-12 (erfc itself is < 2e-6 there)
-            const float S = pair_scalar_cut(r2, pi.y + pj.y, 24.0f * (pi.z * pj.z), a.pc.kc * (pi.x * pj.x), a.pc,
-                                            ENERGY ? &el : nullptr, &ec);
-            fx += dx * S; fy += dy * S; fz += dz * S;
-            if (ENERGY) { el_s += 0.5f * el; ec_s += 0.5f * ec; }
+    long long ui[3] = {0, 0, 0};
+    float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (i < nA) { unwrapped3(a.w, a.img, a.Lq, ai, ui); pi = a.prm[ai]; }
+    float fx = 0.f, fy = 0.f, fz = 0.f, el_s = 0.f, ec_s = 0.f;
+    for (int it = grp.x; it < grp.x + grp.y; it++) {
+        const int4 item = a.items[it];
+        const int B = item.y;
+        const int nB = (int)(a.first[B + 1] - a.first[B]);
+        long long sh[3] = {0, 0, 0};   // B's image: its key particle nearest to A's (LIMA's hyperpos)
+        if (A != B) {
+            long long ka[3], kb[3];
+            unwrapped3(a.w, a.img, a.Lq, item.z, ka);
+            unwrapped3(a.w, a.img, a.Lq, item.w, kb);
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const long long dk = kb[d] - ka[d];
+                long long r = dk % a.Lq[d];
+                r += r < 0 ? a.Lq[d] : 0;
+                r -= 2 * r > a.Lq[d] ? a.Lq[d] : 0;
+                sh[d] = r - dk;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");   // the previous item's reads are done
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (lane < nB) {
+            const int bj = (int)a.first[B] + lane;
+            long long ub[3];
+            unwrapped3(a.w, a.img, a.Lq, bj, ub);
+            L.u[wid][lane][0] = ub[0] + sh[0];
+            L.u[wid][lane][1] = ub[1] + sh[1];
+            L.u[wid][lane][2] = ub[2] + sh[2];
+            L.p[wid][lane] = a.prm[bj];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (i < nA) {
+            const uint32_t row = a.lut[it * 32 + i];
+            for (int j = half * 16; j < half * 16 + 16 && j < nB; j++) {
+                if ((row >> j) & 1u) continue;
+                const long long* uj = L.u[wid][j];
+                const float dx = (float)(uj[0] - ui[0]) * LIMA_LIMA_TO_NANO_F;
+                const float dy = (float)(uj[1] - ui[1]) * LIMA_LIMA_TO_NANO_F;
+                const float dz = (float)(uj[2] - ui[2]) * LIMA_LIMA_TO_NANO_F;
+                const float r2 = dx * dx + dy * dy + dz * dz;
+                if (!(r2 > 0.f)) continue;
+                const float4 pj = L.p[wid][j];
+                float el = 0.f, ec = 0.f;
+                // no cutoff here: pair_scalar_cut's erfc polynomial extrapolates beyond kappa r = 3.25 with an
+                // absolute error below 1e-12 (erfc itself is < 2e-6 there)
+                const float S = pair_scalar_cut(r2, pi.y + pj.y, 24.0f * (pi.z * pj.z), a.pc.kc * (pi.x * pj.x), a.pc,
+                                                ENERGY ? &el : nullptr, &ec);
+                fx += dx * S; fy += dy * S; fz += dz * S;
+                if (ENERGY) { el_s += 0.5f * el; ec_s += 0.5f * ec; }
+            }
         }
     }
     fx += __shfl_xor(fx, 32, 64);
@@ -305,15 +346,15 @@ __device__ __forceinline__ void intra_items(const IntraArgs& a, int blk, IntraLd
     if (ENERGY) e_i = __shfl_xor(el_s + ec_s, 32, 64) + el_s + ec_s;
     if (half == 0 && i < nA) {
         const int s = a.slot_of[ai];
-        atomicAdd(&a.fs[s], fx);
-        atomicAdd(&a.fs[a.nslots + s], fy);
-        atomicAdd(&a.fs[2 * a.nslots + s], fz);
-        if (ENERGY) atomicAdd(&a.es[s], e_i);
+        a.fintra[s] = fx;
+        a.fintra[a.nslots + s] = fy;
+        a.fintra[2 * a.nslots + s] = fz;
+        if (ENERGY) a.eintra[s] = e_i;
     }
     if (ENERGY) {
         const double tl = wave_sum_d((double)el_s), tc = wave_sum_d((double)ec_s);
         if (lane == 0) {
-            double* r = a.epart + (int64_t)it * ACC_N;
+            double* r = a.epart + (int64_t)g * ACC_N;
             r[ACC_LJ] = tl; r[ACC_COUL] = tc; r[ACC_BOND] = 0.0; r[ACC_PME] = 0.0; r[ACC_KIN] = 0.0; r[ACC_PAIRS] = 0.0;
         }
     }
@@ -335,8 +376,6 @@ struct BondArgs {
     const uint4* w;
     const int4* img;
     const int* slot_of;
-    float* fs;
-    float* es;
     float* fb;
     float* eb;
     double* epart;
@@ -362,7 +401,7 @@ __device__ __forceinline__ v3 bdisp(const BondArgs& a, int from, int to) { retur
 // (ForceComputations.cuh:17-376) and keeps the atom's share, a segmented shuffle sum gathers the
 // run into its first lane, and that lane stores the atom's bonded force in the slot's bonded
 // array (fb): one plain store per component, no atomics.  fb is cleared at every list rebuild, so
-// slots of atoms without terms read zero; the integrator adds fb to the atomically summed fs.  Every lane holds one short dependency chain, so long
+// slots of atoms without terms read zero.  Every lane holds one short dependency chain, so long
 // incidence lists (chain atoms belong to ~10 terms) do not serialise; per-atom sums are in a
 // fixed order.  LIMA splits a term's potE equally over its atoms (EngineKernels.cuh:776-790);
 // the FP64 total counts each term once, at its role-0 lane.
@@ -467,17 +506,24 @@ __global__ __launch_bounds__(256) void k_local(IntraArgs ia, BondArgs ba, int ni
     else bonded_lanes<ENERGY>(ba, blockIdx.x - nib);
 }
 
-// Sums the per-wave energy records of all regions in a fixed order: acc[c] = sum_r epart[r][c].
-// Records in [0, n0) are always present; [n0, n1) (PME) and [n1, n2) (kinetic) only when valid.
-__global__ __launch_bounds__(256) void k_reduce_energy(const double* __restrict__ epart, int n0, int n1, int n2, int pme_valid,
-                                                       int kin_valid, double* __restrict__ acc) {
+// Sums the per-wave energy records of all regions in a fixed order, in two stages: block b adds
+// the records of its contiguous share ([b n / B, (b+1) n / B)) into part[b], then one block adds the
+// parts in block order into acc.  Records in [0, n1) are always present, [n1, n2) (kinetic) only
+// when kin_valid.  The PME reciprocal energy comes from the per-slot 1/2 q phi array (ephi, when
+// pme_valid), summed the same way.
+__global__ __launch_bounds__(256) void k_reduce_energy_part(const double* __restrict__ epart, int n1, int n2, int kin_valid,
+                                                            const float* __restrict__ ephi, int nslots, int pme_valid,
+                                                            double* __restrict__ part) {
     __shared__ double red[ACC_N][4];
     double s[ACC_N];
     for (int c = 0; c < ACC_N; c++) s[c] = 0.0;
-    const int nrec = kin_valid ? n2 : (pme_valid ? n1 : n0);
-    for (int r = threadIdx.x; r < nrec; r += blockDim.x) {
-        if (r >= n0 && r < n1 && !pme_valid) continue;
+    const int nrec = kin_valid ? n2 : n1;
+    const int r0 = (int)((int64_t)nrec * blockIdx.x / gridDim.x), r1 = (int)((int64_t)nrec * (blockIdx.x + 1) / gridDim.x);
+    for (int r = r0 + threadIdx.x; r < r1; r += blockDim.x)
         for (int c = 0; c < ACC_N; c++) s[c] += epart[(int64_t)r * ACC_N + c];
+    if (pme_valid) {
+        const int q0 = (int)((int64_t)nslots * blockIdx.x / gridDim.x), q1 = (int)((int64_t)nslots * (blockIdx.x + 1) / gridDim.x);
+        for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) s[ACC_PME] += (double)ephi[q];
     }
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     for (int c = 0; c < ACC_N; c++) {
@@ -485,40 +531,56 @@ __global__ __launch_bounds__(256) void k_reduce_energy(const double* __restrict_
         if (lane == 0) red[c][wid] = v;
     }
     __syncthreads();
-    if (threadIdx.x < ACC_N) acc[threadIdx.x] = ((red[threadIdx.x][0] + red[threadIdx.x][1]) + red[threadIdx.x][2]) + red[threadIdx.x][3];
+    if (threadIdx.x < ACC_N)
+        part[blockIdx.x * ACC_N + threadIdx.x] = ((red[threadIdx.x][0] + red[threadIdx.x][1]) + red[threadIdx.x][2]) + red[threadIdx.x][3];
+}
+
+__global__ void k_reduce_energy_final(const double* __restrict__ part, int nparts, double* __restrict__ acc) {
+    const int c = threadIdx.x;
+    if (c >= ACC_N) return;
+    double t = 0.0;
+    for (int b = 0; b < nparts; b++) t += part[b * ACC_N + c];
+    acc[c] = t;
 }
 
 }  // namespace
 
 int pair_split(const Engine& e) {
-    // enough waves to keep ~16 per SIMD in flight (256 CUs x 4 SIMDs) on small boxes
+    // enough waves to keep ~16 per SIMD in flight (256 CUs x 4 SIMDs) on small boxes; a cluster's
+    // waves stay inside one workgroup (split divides PAIR_WAVES)
     int split = 1;
-    while (split < 8 && (int64_t)e.nl.ncl_max * split < 16384) split *= 2;
+    while (split < PAIR_WAVES && (int64_t)e.nl.ncl_max * split < 16384) split *= 2;
     return split;
+}
+
+void launch_clear_slots(Engine& e, hipStream_t s) {
+    const int ns = slots_max(e);
+    hipLaunchKernelGGL(k_clear_slots, dim3(nblk(ns, 256)), dim3(256), 0, s, ns, e.nl.catoms, e.fintra, e.eintra, e.fb, e.eb,
+                       e.fpme, e.epme, e.epair, e.ephi);
 }
 
 ERegions energy_regions(const Engine& e) {
     ERegions r;
     r.pair = 0;
     r.intra = r.pair + e.nl.ncl_max * pair_split(e);
-    r.bond = r.intra + e.n_intra;
+    r.bond = r.intra + e.n_intra_groups;
     r.pme = r.bond + e.bd.n_lanes / 64;   // one record per wave of the bonded incidence lanes
-    // PME records: one per gather wave (4 per tile block when tiled)
-    r.kin = r.pme + (e.pme.tiled ? e.pme.ntiles * 4 : (int)((e.N + 63) / 64));
+    r.kin = r.pme;                        // PME energy is summed from the per-slot ephi array instead
     r.total = r.kin + (int)((e.N + 63) / 64);
     return r;
 }
 
 void launch_reduce_energy(Engine& e, bool kinetic, hipStream_t s) {
     const bool pme_valid = e.elec && e.pme.ncharged > 0;
-    hipLaunchKernelGGL(k_reduce_energy, dim3(1), dim3(256), 0, s, e.epart, e.ereg.pme, e.ereg.kin, e.ereg.total,
-                       pme_valid ? 1 : 0, kinetic ? 1 : 0, e.acc);
+    hipLaunchKernelGGL(k_reduce_energy_part, dim3(REDUCE_BLOCKS), dim3(256), 0, s, e.epart, e.ereg.kin, e.ereg.total,
+                       kinetic ? 1 : 0, (const float*)e.ephi, slots_max(e), pme_valid ? 1 : 0, e.acc_part);
+    hipLaunchKernelGGL(k_reduce_energy_final, dim3(1), dim3(64), 0, s, (const double*)e.acc_part, REDUCE_BLOCKS, e.acc);
 }
 
 void launch_pack(Engine& e, const uint4* w, hipStream_t s) {
     const int nslots = slots_max(e);
     hipLaunchKernelGGL(k_pack, dim3(nblk(nslots, 256)), dim3(256), 0, s, e.nl.ncl_dev, nslots, e.nl.catoms, w, e.p.prm, e.nl.cref,
-                       (long long)e.Lq[0], (long long)e.Lq[1], (long long)e.Lq[2], e.nl.xq, e.fs, e.es, bin_args(e));
+                       (long long)e.Lq[0], (long long)e.Lq[1], (long long)e.Lq[2], e.nl.xq, bin_args(e));
 }
 
 void launch_pair(Engine& e, bool energy, hipStream_t s) {
@@ -532,7 +594,7 @@ void launch_pair(Engine& e, bool energy, hipStream_t s) {
     a.coul_tab = e.coul_tab;
     a.cap = e.nl.cap;
     a.xq = e.nl.xq; a.ljs = e.nl.ljs; a.ecount = e.nl.ecount; a.ent = e.nl.ent;
-    a.fs = e.fs; a.es = e.es; a.epart = e.epart + (int64_t)e.ereg.pair * ACC_N;
+    a.fpair = e.fpair; a.epair = e.epair; a.epart = e.epart + (int64_t)e.ereg.pair * ACC_N;
     a.split = pair_split(e);
     const int64_t waves = (int64_t)a.ncl_max * a.split;
     if (energy) hipLaunchKernelGGL(k_pair<true>, dim3(nblk(waves, PAIR_WAVES)), dim3(64 * PAIR_WAVES), 0, s, a);
@@ -541,12 +603,13 @@ void launch_pair(Engine& e, bool energy, hipStream_t s) {
 
 IntraArgs intra_args(Engine& e, const uint4* w, const int4* img) {
     IntraArgs a{};
-    a.n_items = e.n_intra;
+    a.n_groups = e.n_intra_groups;
     a.nslots = slots_max(e);
     for (int d = 0; d < 3; d++) a.Lq[d] = e.Lq[d];
     a.pc = e.pc;
-    a.items = e.intra_items; a.lut = e.intra_lut; a.w = w; a.img = img; a.prm = e.p.prm;
-    a.slot_of = e.nl.slot_of; a.first = e.first; a.fs = e.fs; a.es = e.es; a.epart = e.epart + (int64_t)e.ereg.intra * ACC_N;
+    a.groups = e.intra_groups; a.items = e.intra_items; a.lut = e.intra_lut; a.w = w; a.img = img; a.prm = e.p.prm;
+    a.slot_of = e.nl.slot_of; a.first = e.first; a.fintra = e.fintra; a.eintra = e.eintra;
+    a.epart = e.epart + (int64_t)e.ereg.intra * ACC_N;
     return a;
 }
 
@@ -559,7 +622,7 @@ BondArgs bond_args(Engine& e, const uint4* w, const int4* img) {
     a.bid = e.bd.bid; a.bp = e.bd.bp; a.aid = e.bd.aid; a.ap = e.bd.ap; a.did = e.bd.did; a.dp = e.bd.dp;
     a.iid = e.bd.iid; a.ip = e.bd.ip; a.pid = e.bd.pid; a.pp = e.bd.pp;
     a.n_lanes = e.bd.n_lanes; a.multi_lane0 = e.bd.multi_lane0; a.lane_code = e.bd.lane_code; a.lane_atom = e.bd.lane_atom;
-    a.w = w; a.img = img; a.slot_of = e.nl.slot_of; a.fs = e.fs; a.es = e.es; a.fb = e.fb; a.eb = e.eb;
+    a.w = w; a.img = img; a.slot_of = e.nl.slot_of; a.fb = e.fb; a.eb = e.eb;
     a.epart = e.epart + (int64_t)e.ereg.bond * ACC_N;
     return a;
 }
@@ -567,7 +630,7 @@ BondArgs bond_args(Engine& e, const uint4* w, const int4* img) {
 void launch_local_blocks(Engine& e, const uint4* w, const int4* img, bool energy, bool intra, bool bonded, hipStream_t s) {
     const IntraArgs ia = intra_args(e, w, img);
     const BondArgs ba = bond_args(e, w, img);
-    const int nib = intra ? nblk(ia.n_items, 4) : 0;
+    const int nib = intra ? nblk(ia.n_groups, 4) : 0;
     const int nbb = bonded ? nblk(ba.n_lanes, 256) : 0;
     if (nib + nbb == 0) return;
     if (energy) hipLaunchKernelGGL(k_local<true>, dim3(nib + nbb), dim3(256), 0, s, ia, ba, nib);

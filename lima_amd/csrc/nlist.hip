@@ -23,11 +23,6 @@ namespace {
 
 constexpr int MAXCOL = 2048;   // particles per x-y column held in LDS for the z sort
 
-__global__ void k_fill_int(int* p, int n, int v) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
-}
-
 // Column of coordinate v in [0, Lq): floor(v n / Lq), from a double estimate corrected against
 // the exact column starts xb[c] = ceil(c Lq / n) (no 64-bit integer division).
 __device__ __forceinline__ int col_of(long long v, const long long* __restrict__ xb, int n, double rinv) {
@@ -114,7 +109,7 @@ __global__ __launch_bounds__(1024) void k_col_sort(const int* __restrict__ col_s
     const int b = col_start[c], n = col_start[c + 1] - b;
     if (threadIdx.x == 0) col_ncl[c] = (n + CLUSTER - 1) / CLUSTER;
     if (n > MAXCOL) {
-        if (threadIdx.x == 0) atomicOr(&flags[0], 1);
+        if (threadIdx.x == 0) atomicOr(&flags[DF_COLUMN], 1);
         return;
     }
     int P = 1;
@@ -309,8 +304,8 @@ __device__ __forceinline__ bool jbox_near(const NlArgs& a, int J, int code, cons
 // of the i box and pack the survivors into a second table.  The wave takes 8 survivors x 8 slots
 // per pass, with the next pass's loads in flight, and tests each particle against the i box.  Entries go to the cluster's
 // fixed-capacity slice [I cap, (I+1) cap); the count (padded to a multiple of 8 with mask-0
-// entries) goes to ecount[I].  A slice that would overflow sets flags[1] and the host rebuilds
-// with a larger capacity.
+// entries) goes to ecount[I].  A slice that would overflow sets the DF_PAIRLIST word and the host
+// rebuilds with a larger capacity (at once, or by a rollback when the rebuild was not waited for).
 template <bool NARROW>
 __global__ __launch_bounds__(256) void k_nlist(NlArgs a) {
     __shared__ uint32_t flat_all[4][NL_FLAT], cand_all[4][NL_FLAT];
@@ -498,7 +493,7 @@ __global__ __launch_bounds__(256) void k_nlist(NlArgs a) {
     }
     const int padded = (count + CLUSTER - 1) & ~(CLUSTER - 1);
     if (padded > a.cap) {
-        if (lane == 0) { atomicOr(&a.flags[1], 1); a.ecount[I] = 0; }
+        if (lane == 0) { atomicOr(&a.flags[DF_PAIRLIST], 1); atomicMax(&a.flags[DF_LIST_NEED], padded); a.ecount[I] = 0; }
         return;
     }
     for (int k = count + lane; k < padded; k += 64) {
@@ -590,13 +585,16 @@ void nlist_alloc(Engine& e) {
     HIP_CHECK(hipMalloc(&nl.xq, slots * sizeof(float4)));
     HIP_CHECK(hipMalloc(&nl.ecount, (nl.ncl_max + 1) * sizeof(int)));
     HIP_CHECK(hipMalloc(&nl.eoff, (nl.ncl_max + 2) * sizeof(int)));
-    HIP_CHECK(hipMalloc(&nl.flags, 4 * sizeof(int)));
-    HIP_CHECK(hipMalloc(&e.fs, 3 * slots * sizeof(float)));
-    HIP_CHECK(hipMalloc(&e.fb, 3 * slots * sizeof(float)));
-    HIP_CHECK(hipMalloc(&e.eb, slots * sizeof(float)));
-    HIP_CHECK(hipMalloc(&e.es, slots * sizeof(float)));
+    for (float** f : {&e.fpair, &e.fintra, &e.fb, &e.fpme}) {
+        HIP_CHECK(hipMalloc(f, 3 * slots * sizeof(float)));
+        HIP_CHECK(hipMemset(*f, 0, 3 * slots * sizeof(float)));
+    }
+    for (float** f : {&e.epair, &e.eintra, &e.eb, &e.epme, &e.ephi}) {
+        HIP_CHECK(hipMalloc(f, slots * sizeof(float)));
+        HIP_CHECK(hipMemset(*f, 0, slots * sizeof(float)));
+    }
     nl.nent_cap = 0;
-    nl.cap = 2048;
+    nl.cap = e.prm.nlist_capacity > 0 ? (e.prm.nlist_capacity + 7) & ~7 : 2048;
     nl.ent = nullptr;
     nl.ecode = nullptr;
 }
@@ -604,29 +602,58 @@ void nlist_alloc(Engine& e) {
 void nlist_free(Engine& e) {
     NList& nl = e.nl;
     void* ptrs[] = {nl.col_of, nl.col_count, nl.col_start, nl.col_fill, nl.keys, nl.col_ncl, nl.catoms, nl.slot_of,
-                    nl.ccol, nl.cref, nl.cmin, nl.cmax, nl.ljs, nl.comps, nl.cw, nl.xb[0], nl.xb[1], nl.xq, nl.ecount, nl.eoff, nl.flags,
-                    nl.ent, nl.ecode, e.fs, e.es, e.fb, e.eb};
+                    nl.ccol, nl.cref, nl.cmin, nl.cmax, nl.ljs, nl.comps, nl.cw, nl.xb[0], nl.xb[1], nl.xq, nl.ecount, nl.eoff,
+                    nl.ent, nl.ecode, e.fpair, e.epair, e.fintra, e.eintra, e.fb, e.eb, e.fpme, e.epme, e.ephi};
     for (void* p : ptrs) if (p) (void)hipFree(p);
 }
 
 namespace {
 
-// The clears and the position snapshot that open a rebuild, in one launch: column counters and
-// fill cursors, the overflow flags, the PME tile bin counters when the integrator filled them for
-// the old slots (bins != nullptr), and, when the list is built from the engine's own positions,
-// the positions at the build and the per-wave displacement maxima (reset).
-__global__ void k_rebuild_open(int64_t N, int snapshot, const uint4* __restrict__ w, const int4* __restrict__ img,
-                               uint4* __restrict__ wb, int4* __restrict__ ib, float* __restrict__ dpart, int* __restrict__ col_count,
-                               int* __restrict__ col_fill, int ncol1, int* __restrict__ flags, int* __restrict__ bins, int nbins) {
+// The clears and the snapshot that open a rebuild, in one launch: column counters and fill cursors,
+// and the PME tile bin counters when the integrator filled them for the old slots (bins !=
+// nullptr).  When the list is built from the engine's own positions (snapshot != 0) the per-wave
+// displacement maxima restart, and, if no error word is set, the MD state is saved (positions at
+// the build, velocities, previous forces, thermostat words, list statistics, energy totals) and
+// DF_SNAPSHOTS counts it.  Error words are sticky: a rebuild after an unresolved condition takes no
+// snapshot, so the last one stays the state the host can restore.
+struct OpenArgs {
+    int64_t N;
+    int snapshot;
+    const uint4* w; const int4* img; const float4* v; const float4* fprev;
+    const float* thermo; const int* lstats; const double* acc;
+    uint4* wb; int4* ib; float4* vs; float4* fps; float* ths; int* lss; double* accs;
+    float* dpart;
+    int* col_count; int* col_fill; int ncol1;
+    int* flags;
+    int* bins; int nbins;
+};
+__global__ void k_rebuild_open(OpenArgs a) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (snapshot && p < N) {
-        if ((p & 63) == 0) dpart[p >> 6] = 0.f;
-        wb[p] = w[p];
-        ib[p] = img[p];
+    bool clean = a.snapshot != 0;
+#pragma unroll
+    for (int k = 0; k < DF_ERRORS; k++) clean = clean && a.flags[k] == 0;
+    if (a.snapshot && p < a.N && (p & 63) == 0) a.dpart[p >> 6] = 0.f;
+    if (clean && p < a.N) {
+        a.wb[p] = a.w[p];
+        a.ib[p] = a.img[p];
+        a.vs[p] = a.v[p];
+        a.fps[p] = a.fprev[p];
     }
-    if (p < ncol1) { col_count[p] = 0; col_fill[p] = 0; }
-    if (p < 4) flags[p] = 0;
-    if (bins && p < nbins) bins[p] = 0;
+    if (clean && p == 0) {
+        a.ths[0] = a.thermo[0]; a.ths[1] = a.thermo[1];
+        a.lss[0] = a.lstats[0]; a.lss[1] = a.lstats[1];
+        for (int k = 0; k < ACC_N; k++) a.accs[k] = a.acc[k];
+        a.flags[DF_SNAPSHOTS] += 1;   // only this thread touches the word; the others read words 0..3
+    }
+    if (p < a.ncol1) { a.col_count[p] = 0; a.col_fill[p] = 0; }
+    if (a.bins && p < a.nbins) a.bins[p] = 0;
+}
+
+// Clears the error words and sets the snapshot count (after the host has dealt with a condition).
+__global__ void k_reset_flags(int* flags, int snapshots) {
+    const int k = threadIdx.x;
+    if (k < DF_ERRORS || k == DF_LIST_NEED) flags[k] = 0;
+    if (k == DF_SNAPSHOTS) flags[k] = snapshots;
 }
 
 // List staleness statistics kept on the device, from the per-wave displacement maxima of the
@@ -649,18 +676,6 @@ __global__ __launch_bounds__(256) void k_list_stats(const float* __restrict__ v,
     }
 }
 
-// Slot arrays of the new list: no particle in any slot (catoms -1) until assigned, bonded forces
-// and energies zero (slots of particles without bonded terms keep them zero until the next rebuild).
-__global__ void k_clear_slots(int ns, int* __restrict__ catoms, float* __restrict__ fb, float* __restrict__ eb) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ns) return;
-    catoms[i] = -1;
-    fb[i] = 0.f;
-    fb[ns + i] = 0.f;
-    fb[2 * (int64_t)ns + i] = 0.f;
-    eb[i] = 0.f;
-}
-
 }  // namespace
 
 int nlist_count(Engine& e) {
@@ -670,7 +685,16 @@ int nlist_count(Engine& e) {
     return ncl;
 }
 
-void nlist_build(Engine& e, const uint4* w) {
+void nlist_grow(Engine& e, int need) {
+    // the next build reallocates the entry buffer
+    e.nl.cap = std::max((e.nl.cap * 3 / 2 + 7) & ~7, (int)(((int64_t)need * 3 / 2 + 7) & ~7));
+}
+
+void reset_flags(Engine& e, int64_t snapshots, hipStream_t s) {
+    hipLaunchKernelGGL(k_reset_flags, dim3(1), dim3(64), 0, s, e.dflags, (int)snapshots);
+}
+
+void nlist_build(Engine& e, const uint4* w, bool checked) {
     NList& nl = e.nl;
     hipStream_t s = e.s_main;
     const int64_t N = e.N;
@@ -679,43 +703,64 @@ void nlist_build(Engine& e, const uint4* w) {
     // reallocated.  The integrator's packing of the old slots is stale.
     const bool clear_bins = e.packed && e.pme.tiled && e.elec && e.pme.ncharged > 0;   // tile bins filled for the old slots
     e.packed = false;
-    // staleness statistics of the list being replaced, on the device (read by lima_get_status)
-    hipLaunchKernelGGL(k_list_stats, dim3(1), dim3(256), 0, s, (const float*)e.dpart, (int)((N + 63) / 64), e.skin,
-                       e.n_rebuilds > 0 ? 1 : 0, e.dmax2, e.lstats);
-    {
-        const int64_t span = std::max<int64_t>(std::max<int64_t>(N, nl.ncol + 1), std::max<int64_t>(4, clear_bins ? e.pme.ntiles : 0));
-        hipLaunchKernelGGL(k_rebuild_open, dim3(nblk(span, 256)), dim3(256), 0, s, N, w == e.p.w ? 1 : 0, e.p.w, e.p.img, e.w_build,
-                           e.img_build, e.dpart, nl.col_count, nl.col_fill, nl.ncol + 1, nl.flags,
-                           clear_bins ? e.pme.bin_count : nullptr, e.pme.ntiles);
-    }
-    const size_t col_lds = nl.ncol <= COL_LDS_MAX ? (size_t)nl.ncol * sizeof(int) : 0;
-    hipLaunchKernelGGL(k_col_assign, dim3(nblk(N, 256 * COL_ITEMS)), dim3(256), col_lds, s, N, w, nl.xb[0], nl.xb[1], nl.rinv[0],
-                       nl.rinv[1], nl.ncx, nl.ncy, nl.col_of, nl.col_count);
-    launch_exclusive_scan(nl.col_count, nl.col_start, nl.ncol, s);
-    hipLaunchKernelGGL(k_col_scatter, dim3(nblk(N, 256 * COL_ITEMS)), dim3(256), col_lds, s, N, nl.ncol, w, nl.col_of, nl.col_start,
-                       nl.col_fill, nl.keys);
-    hipLaunchKernelGGL(k_col_sort, dim3(nl.ncol), dim3(1024), 0, s, nl.col_start, nl.keys, nl.col_ncl, nl.flags);
-    // col_count now holds the cluster start of every column and the total, ncl = col_count[ncol]
-    // (at most N / 8 + ncol <= ncl_max: each column pads its last cluster)
-    launch_exclusive_scan(nl.col_ncl, nl.col_count, nl.ncol, s);
-    const int* col_cl_start = nl.col_count;
-    const int ns = slots_max(e);
-    hipLaunchKernelGGL(k_clear_slots, dim3(nblk(ns, 256)), dim3(256), 0, s, ns, nl.catoms, e.fb, e.eb);
-    hipLaunchKernelGGL(k_assign_slots, dim3(nblk(N, 256)), dim3(256), 0, s, N, nl.ncol, nl.col_start, col_cl_start, nl.keys,
-                       nl.catoms, nl.slot_of);
-    hipLaunchKernelGGL(k_cluster_props, dim3(nblk(ns, 256)), dim3(256), 0, s, nl.ncl_dev, nl.ncl_max, nl.ncol, col_cl_start,
-                       nl.catoms, w, e.p.prm, e.p.comp, nl.ccol, nl.cref, nl.cmin, nl.cmax, nl.ljs, nl.comps, nl.cw);
-    NlArgs a{};
-    a.ncl_dev = nl.ncl_dev; a.ncl_max = nl.ncl_max; a.ncx = nl.ncx; a.ncy = nl.ncy; a.nC = e.nC;
-    for (int d = 0; d < 3; d++) a.Lq[d] = e.Lq[d];
-    a.R = llround(((double)e.rc + (double)e.skin) * 1e8);
-    a.R2 = a.R * a.R;
-    a.col_cl_start = col_cl_start; a.catoms = nl.catoms; a.comps = nl.comps; a.bmask = e.bmask; a.cw = nl.cw;
-    a.xb[0] = nl.xb[0]; a.xb[1] = nl.xb[1]; a.rinv[0] = nl.rinv[0]; a.rinv[1] = nl.rinv[1];
-    a.cref = nl.cref; a.cmin = nl.cmin; a.cmax = nl.cmax; a.ecount = nl.ecount; a.flags = nl.flags;
+    const bool snapshot = w == e.p.w;
     for (int attempt = 0;; attempt++) {
+        if (snapshot) {   // the host half of the snapshot this rebuild takes when no error word is set
+            SnapRecord r;
+            r.ordinal = ++e.snap_issued;
+            r.step = e.step;
+            r.log_frames = e.log_frames;
+            r.n_rebuilds = e.n_rebuilds;
+            r.temp_measured = e.temp_measured;
+            r.energy_step = e.energy_step;
+            e.snaps.push_back(r);
+            if (e.snaps.size() > 16) e.snaps.erase(e.snaps.begin());
+        }
+        // staleness statistics of the list being replaced, on the device (read by lima_get_status)
+        hipLaunchKernelGGL(k_list_stats, dim3(1), dim3(256), 0, s, (const float*)e.dpart, (int)((N + 63) / 64), e.skin,
+                           (e.n_rebuilds > 0 && !e.stats_skip) ? 1 : 0, e.dmax2, e.lstats);
+        e.stats_skip = false;
+        {
+            OpenArgs o{};
+            o.N = N; o.snapshot = snapshot ? 1 : 0;
+            o.w = e.p.w; o.img = e.p.img; o.v = e.p.v; o.fprev = e.p.fprev; o.thermo = e.thermo; o.lstats = e.lstats; o.acc = e.acc;
+            o.wb = e.w_build; o.ib = e.img_build; o.vs = e.v_snap; o.fps = e.fprev_snap; o.ths = e.thermo_snap; o.lss = e.lstats_snap;
+            o.accs = e.acc_snap; o.dpart = e.dpart;
+            o.col_count = nl.col_count; o.col_fill = nl.col_fill; o.ncol1 = nl.ncol + 1;
+            o.flags = e.dflags;
+            o.bins = clear_bins ? e.pme.bin_count : nullptr; o.nbins = e.pme.ntiles;
+            const int64_t span = std::max<int64_t>(std::max<int64_t>(N, nl.ncol + 1), std::max<int64_t>(1, clear_bins ? e.pme.ntiles : 0));
+            hipLaunchKernelGGL(k_rebuild_open, dim3(nblk(span, 256)), dim3(256), 0, s, o);
+        }
+        const size_t col_lds = nl.ncol <= COL_LDS_MAX ? (size_t)nl.ncol * sizeof(int) : 0;
+        hipLaunchKernelGGL(k_col_assign, dim3(nblk(N, 256 * COL_ITEMS)), dim3(256), col_lds, s, N, w, nl.xb[0], nl.xb[1], nl.rinv[0],
+                           nl.rinv[1], nl.ncx, nl.ncy, nl.col_of, nl.col_count);
+        launch_exclusive_scan(nl.col_count, nl.col_start, nl.ncol, s);
+        hipLaunchKernelGGL(k_col_scatter, dim3(nblk(N, 256 * COL_ITEMS)), dim3(256), col_lds, s, N, nl.ncol, w, nl.col_of, nl.col_start,
+                           nl.col_fill, nl.keys);
+        hipLaunchKernelGGL(k_col_sort, dim3(nl.ncol), dim3(1024), 0, s, nl.col_start, nl.keys, nl.col_ncl, e.dflags);
+        // col_count now holds the cluster start of every column and the total, ncl = col_count[ncol]
+        // (at most N / 8 + ncol <= ncl_max: each column pads its last cluster)
+        launch_exclusive_scan(nl.col_ncl, nl.col_count, nl.ncol, s);
+        const int* col_cl_start = nl.col_count;
+        const int ns = slots_max(e);
+        launch_clear_slots(e, s);
+        hipLaunchKernelGGL(k_assign_slots, dim3(nblk(N, 256)), dim3(256), 0, s, N, nl.ncol, nl.col_start, col_cl_start, nl.keys,
+                           nl.catoms, nl.slot_of);
+        hipLaunchKernelGGL(k_cluster_props, dim3(nblk(ns, 256)), dim3(256), 0, s, nl.ncl_dev, nl.ncl_max, nl.ncol, col_cl_start,
+                           nl.catoms, w, e.p.prm, e.p.comp, nl.ccol, nl.cref, nl.cmin, nl.cmax, nl.ljs, nl.comps, nl.cw);
+        NlArgs a{};
+        a.ncl_dev = nl.ncl_dev; a.ncl_max = nl.ncl_max; a.ncx = nl.ncx; a.ncy = nl.ncy; a.nC = e.nC;
+        for (int d = 0; d < 3; d++) a.Lq[d] = e.Lq[d];
+        a.R = llround(((double)e.rc + (double)e.skin) * 1e8);
+        a.R2 = a.R * a.R;
+        a.col_cl_start = col_cl_start; a.catoms = nl.catoms; a.comps = nl.comps; a.bmask = e.bmask; a.cw = nl.cw;
+        a.xb[0] = nl.xb[0]; a.xb[1] = nl.xb[1]; a.rinv[0] = nl.rinv[0]; a.rinv[1] = nl.rinv[1];
+        a.cref = nl.cref; a.cmin = nl.cmin; a.cmax = nl.cmax; a.ecount = nl.ecount; a.flags = e.dflags;
         const int64_t need = (int64_t)nl.ncl_max * nl.cap;
         if (need > nl.nent_cap) {
+            HIP_CHECK(hipStreamSynchronize(s));   // kernels of earlier steps may still read the old buffer
+            HIP_CHECK(hipStreamSynchronize(e.s_pme));
             if (nl.ent) HIP_CHECK(hipFree(nl.ent));
             if (nl.ecode) HIP_CHECK(hipFree(nl.ecode));
             nl.ent = nullptr; nl.ecode = nullptr;
@@ -730,20 +775,29 @@ void nlist_build(Engine& e, const uint4* w) {
         const long long Lmax = std::max(a.Lq[0], std::max(a.Lq[1], a.Lq[2]));
         if (2 * Lmax + 2 * a.R < (1ll << 31)) hipLaunchKernelGGL(k_nlist<true>, dim3(nblk(nl.ncl_max, 4)), dim3(256), 0, s, a);
         else hipLaunchKernelGGL(k_nlist<false>, dim3(nblk(nl.ncl_max, 4)), dim3(256), 0, s, a);
-        // the one host synchronisation of a rebuild: overflow flags of the columns, the pair list
-        // and the PME tile bins of the steps since the last rebuild
-        int hflags[4];
-        int pflag = 0;
-        HIP_CHECK(hipMemcpyAsync(hflags, nl.flags, sizeof(hflags), hipMemcpyDeviceToHost, s));
-        if (e.pme.tiled && e.elec && e.pme.ncharged > 0)
-            HIP_CHECK(hipMemcpyAsync(&pflag, e.pme.flags, sizeof(int), hipMemcpyDeviceToHost, s));
+        if (!checked) {
+            // the status words go to pinned host memory behind an event; issue_steps reads them two
+            // rebuilds later (bounded run-ahead), lima_synchronize at once
+            const int par = e.rb_parity;
+            HIP_CHECK(hipMemcpyAsync(e.hflags + par * DF_N, e.dflags, DF_N * sizeof(int), hipMemcpyDeviceToHost, s));
+            HIP_CHECK(hipEventRecord(e.ev_rebuild[par], s));
+            e.rb_pending[par] = true;
+            e.rb_parity ^= 1;
+            break;
+        }
+        // checked: the one host synchronisation of this rebuild
+        int hf[DF_N];
+        HIP_CHECK(hipMemcpyAsync(hf, e.dflags, sizeof(hf), hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipStreamSynchronize(s));
-        if (hflags[0]) throw LimaFail(LIMA_ERR_LIST_OVERFLOW, "an x-y column holds more than 2048 particles");
-        if (pflag) throw LimaFail(LIMA_ERR_LIST_OVERFLOW, "a PME tile bin overflowed (more than 512 charged particles in one tile)");
-        if (!hflags[1]) break;
+        if (hf[DF_COLUMN]) throw LimaFail(LIMA_ERR_LIST_OVERFLOW, "an x-y column holds more than 2048 particles");
+        if (hf[DF_PMEBIN] || hf[DF_NONFINITE])
+            throw LimaFail(LIMA_ERR_HIP, "internal: a checked rebuild found an unresolved PME or force condition");
+        if (!hf[DF_PAIRLIST]) break;
         if (attempt >= 6) throw LimaFail(LIMA_ERR_LIST_OVERFLOW, "pair list capacity per cluster exceeded");
-        nl.cap = (nl.cap * 3 / 2 + 7) & ~7;
-        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(4), 0, s, nl.flags, 4, 0);
+        nlist_grow(e, hf[DF_LIST_NEED]);
+        // this attempt's snapshot (taken before the overflow) is superseded by the next attempt's
+        if (snapshot) { e.snap_issued--; e.snaps.pop_back(); }
+        reset_flags(e, e.snap_issued, s);
     }
     HIP_CHECK(hipGetLastError());
     e.steps_since_build = 0;

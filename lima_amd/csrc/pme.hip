@@ -11,6 +11,7 @@
 //            rearranged into one 6x6x6 stencil: -sum_n w(n) (phi(n+1) - phi(n-1)) =
 //            sum_m phi(m) (w(m+1) - w(m-1)).  Force q E, potE 1/2 (q phi + Eself) (PME.cuh:414-420).
 // HBM / L2 bound: per step B = 48 N_q + 34 N_grid bytes of algorithmic traffic.
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -380,15 +381,14 @@ struct GatherArgs {
     const float4* prm;
     const float* phi;
     const int* slot_of;
-    float* fs;
-    float* es;
-    double* epart;
+    float* fpme;     // [3][nslots], the particle's slot written once
+    float* epme;
+    float* ephi;     // [nslots] 1/2 q phi (energy evaluations)
 };
 
 template <bool ENERGY>
 __global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    float ephi = 0.f;
     if (p < a.N) {
         const float q = a.prm[p].x;
         if (q != 0.f) {
@@ -424,20 +424,13 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
                 }
             }
             const int s = a.slot_of[p];
-            atomicAdd(&a.fs[s], q * ex * a.inv2h[0]);
-            atomicAdd(&a.fs[a.nslots + s], q * ey * a.inv2h[1]);
-            atomicAdd(&a.fs[2 * a.nslots + s], q * ez * a.inv2h[2]);
+            a.fpme[s] = q * ex * a.inv2h[0];
+            a.fpme[a.nslots + s] = q * ey * a.inv2h[1];
+            a.fpme[2 * a.nslots + s] = q * ez * a.inv2h[2];
             if (ENERGY) {
-                atomicAdd(&a.es[s], 0.5f * (q * ph + a.eself));
-                ephi = 0.5f * q * ph;
+                a.epme[s] = 0.5f * (q * ph + a.eself);
+                a.ephi[s] = 0.5f * q * ph;
             }
-        }
-    }
-    if (ENERGY) {
-        const double t = wave_sum_d((double)ephi);
-        if ((threadIdx.x & 63) == 0 && (p & ~63ll) < a.N) {   // waves inside the region
-            double* r = a.epart + (p >> 6) * ACC_N;
-            r[ACC_LJ] = 0.0; r[ACC_COUL] = 0.0; r[ACC_BOND] = 0.0; r[ACC_PME] = t; r[ACC_KIN] = 0.0; r[ACC_PAIRS] = 0.0;
         }
     }
 }
@@ -455,9 +448,9 @@ struct GatherTileArgs {
     const int* slot_of;
     const int* bin_atoms;
     int* bin_count;
-    float* fs;
-    float* es;
-    double* epart;
+    float* fpme;
+    float* epme;
+    float* ephi;
 };
 
 __device__ __forceinline__ float pick6(const float* v, int i) {   // v[i] for a per-lane i in 0..5, as a masked sum (no stack array)
@@ -508,7 +501,6 @@ __global__ __launch_bounds__(256) void k_gather_tiles(GatherTileArgs a) {
     __shared__ uint4 sw[256];
     const int nb = min(a.bin_count[t], a.bincap);
     const int j = threadIdx.x & 7;
-    float ephi = 0.f;
     for (int c0 = 0; c0 < nb; c0 += 256) {   // uniform trip counts per block
         const int nc = min(256, nb - c0);
         if ((int)threadIdx.x < nc) {
@@ -565,23 +557,16 @@ __global__ __launch_bounds__(256) void k_gather_tiles(GatherTileArgs a) {
         }
         if (live && j == 0) {
             const int s = a.slot_of[p];
-            atomicAdd(&a.fs[s], q * ex * a.inv2h[0]);
-            atomicAdd(&a.fs[a.nslots + s], q * ey * a.inv2h[1]);
-            atomicAdd(&a.fs[2 * a.nslots + s], q * ez * a.inv2h[2]);
+            a.fpme[s] = q * ex * a.inv2h[0];
+            a.fpme[a.nslots + s] = q * ey * a.inv2h[1];
+            a.fpme[2 * a.nslots + s] = q * ez * a.inv2h[2];
             if (ENERGY) {
-                atomicAdd(&a.es[s], 0.5f * (q * ph + a.eself));
-                ephi += 0.5f * q * ph;
+                a.epme[s] = 0.5f * (q * ph + a.eself);
+                a.ephi[s] = 0.5f * q * ph;
             }
         }
     }
         __syncthreads();   // the next chunk overwrites the staged particles
-    }
-    if (ENERGY) {
-        const double e = wave_sum_d((double)ephi);
-        if ((threadIdx.x & 63) == 0) {
-            double* r = a.epart + ((int64_t)t * (blockDim.x >> 6) + (threadIdx.x >> 6)) * ACC_N;
-            r[ACC_LJ] = 0.0; r[ACC_COUL] = 0.0; r[ACC_BOND] = 0.0; r[ACC_PME] = e; r[ACC_KIN] = 0.0; r[ACC_PAIRS] = 0.0;
-        }
     }
     __syncthreads();
     if (threadIdx.x == 0) a.bin_count[t] = 0;
@@ -648,7 +633,7 @@ BinArgs bin_args(const Engine& e) {
 }
 
 
-void pme_alloc(Engine& e, const std::vector<float>& q) {
+void pme_alloc(Engine& e, const std::vector<float>& q, const std::vector<uint4>& w) {
     Pme& m = e.pme;
     const double spacing = e.prm.pme_spacing_nm > 0 ? e.prm.pme_spacing_nm : 0.12;
     for (int d = 0; d < 3; d++) {
@@ -720,12 +705,26 @@ void pme_alloc(Engine& e, const std::vector<float>& q) {
         if ((int64_t)m.E[0] * m.E[1] * m.E[2] * 8 > 56 * 1024) m.tiled = false;   // LDS block size
         m.ntiles = m.nt[0] * m.nt[1] * m.nt[2];
         if (m.tiled) {
+            // bin capacity: twice the fullest bin of the starting positions (a multiple of 64, at
+            // least 128); a bin that overflows later sets DF_PMEBIN and the capacity doubles
+            {
+                std::vector<int> cnt(m.ntiles, 0);
+                for (size_t i = 0; i < q.size(); i++) {
+                    if (q[i] == 0.f) continue;
+                    const uint32_t wc[3] = {w[i].x, w[i].y, w[i].z};
+                    int ix[3];
+                    for (int d = 0; d < 3; d++) ix[d] = (int)(((uint64_t)wc[d] * (uint64_t)m.K[d]) / (uint64_t)e.Lq[d]);
+                    cnt[((ix[2] / m.T[2]) * m.nt[1] + ix[1] / m.T[1]) * m.nt[0] + ix[0] / m.T[0]]++;
+                }
+                int mx = 0;
+                for (int c : cnt) mx = std::max(mx, c);
+                m.bincap = std::max(128, (2 * mx + 63) / 64 * 64);
+            }
             HIP_CHECK(hipMalloc(&m.bin_count, (m.ntiles + 1) * sizeof(int)));
             HIP_CHECK(hipMemset(m.bin_count, 0, (m.ntiles + 1) * sizeof(int)));   // later cleared by the gather
             HIP_CHECK(hipMalloc(&m.bin_atoms, (int64_t)m.ntiles * m.bincap * sizeof(int)));
             HIP_CHECK(hipMalloc(&m.tblock, (int64_t)m.ntiles * m.E[0] * m.E[1] * m.E[2] * sizeof(long long)));
-            HIP_CHECK(hipMalloc(&m.flags, sizeof(int)));
-            HIP_CHECK(hipMemset(m.flags, 0, sizeof(int)));
+            m.flags = e.dflags + DF_PMEBIN;
             // per grid coordinate: the (at most two) tiles whose block covers it and the local index
             for (int d = 0; d < 3; d++) {
                 std::vector<int4> cov(m.K[d]);
@@ -756,8 +755,23 @@ void pme_alloc(Engine& e, const std::vector<float>& q) {
 void pme_free(Engine& e) {
     Pme& m = e.pme;
     void* ptrs[] = {m.qgrid, m.phi, m.cgrid, m.green, m.tw[0], m.tw[1], m.tw[2], m.bin_count, m.bin_atoms, m.tblock,
-                    m.cover[0], m.cover[1], m.cover[2], m.flags};
+                    m.cover[0], m.cover[1], m.cover[2]};
     for (void* p : ptrs) if (p) (void)hipFree(p);
+}
+
+void pme_grow_bins(Engine& e) {
+    Pme& m = e.pme;
+    if (!m.tiled) return;
+    HIP_CHECK(hipDeviceSynchronize());   // no kernel may still use the old bins
+    if (m.bin_atoms) HIP_CHECK(hipFree(m.bin_atoms));
+    m.bin_atoms = nullptr;
+    m.bincap *= 2;
+    HIP_CHECK(hipMalloc(&m.bin_atoms, (int64_t)m.ntiles * m.bincap * sizeof(int)));
+    e.step_graph_valid = false;   // bin arguments are part of the captured kernels
+}
+
+void pme_clear_bins(Engine& e, hipStream_t s) {
+    if (e.pme.tiled && e.pme.bin_count) HIP_CHECK(hipMemsetAsync(e.pme.bin_count, 0, (e.pme.ntiles + 1) * sizeof(int), s));
 }
 
 void launch_pme(Engine& e, const uint4* w, bool energy, hipStream_t s) {
@@ -807,8 +821,7 @@ void launch_pme(Engine& e, const uint4* w, bool energy, hipStream_t s) {
         gt.nslots = slots_max(e);
         gt.eself = (float)m.eself_total;
         gt.w = w; gt.prm = e.p.prm; gt.phi = m.phi; gt.slot_of = e.nl.slot_of; gt.bin_atoms = m.bin_atoms;
-        gt.bin_count = m.bin_count; gt.fs = e.fs; gt.es = e.es;
-        gt.epart = e.epart + (int64_t)e.ereg.pme * ACC_N;
+        gt.bin_count = m.bin_count; gt.fpme = e.fpme; gt.epme = e.epme; gt.ephi = e.ephi;
         const size_t lds = (size_t)(m.T[0] + 5) * (m.T[1] + 5) * (m.T[2] + 5) * sizeof(float);
         if (energy) hipLaunchKernelGGL(k_gather_tiles<true>, dim3(m.ntiles), dim3(256), lds, s, gt);
         else hipLaunchKernelGGL(k_gather_tiles<false>, dim3(m.ntiles), dim3(256), lds, s, gt);
@@ -819,8 +832,7 @@ void launch_pme(Engine& e, const uint4* w, bool energy, hipStream_t s) {
     ga.nslots = slots_max(e);
     for (int d = 0; d < 3; d++) { ga.Lq[d] = e.Lq[d]; ga.K[d] = m.K[d]; ga.inv2h[d] = (float)(m.K[d] / (2.0 * e.L[d])); }
     ga.eself = (float)m.eself_total;
-    ga.w = w; ga.prm = e.p.prm; ga.phi = m.phi; ga.slot_of = e.nl.slot_of; ga.fs = e.fs; ga.es = e.es;
-    ga.epart = e.epart + (int64_t)e.ereg.pme * ACC_N;
+    ga.w = w; ga.prm = e.p.prm; ga.phi = m.phi; ga.slot_of = e.nl.slot_of; ga.fpme = e.fpme; ga.epme = e.epme; ga.ephi = e.ephi;
     if (energy) hipLaunchKernelGGL(k_gather<true>, dim3(nblk(e.N, 256)), dim3(256), 0, s, ga);
     else hipLaunchKernelGGL(k_gather<false>, dim3(nblk(e.N, 256)), dim3(256), 0, s, ga);
 }

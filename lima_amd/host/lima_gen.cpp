@@ -374,7 +374,9 @@ lima_box* assemble(const Builder& B, Rng& r, double temperature) {
     return box;
 }
 
-void fill_waters(Builder& B, Rng& r, int n_target, const Hash& solute, double o_min, double h_min, double spacing) {
+// Places up to n_target waters on a shuffled lattice of the given spacing, skipping sites whose
+// O or H would come closer than o_min / h_min to the solute; returns the number placed.
+int fill_waters(Builder& B, Rng& r, int n_target, const Hash& solute, double o_min, double h_min, double spacing) {
     int n[3];
     for (int d = 0; d < 3; d++) n[d] = std::max(1, (int)std::lround(B.box[d] / spacing));
     std::vector<int64_t> sites((size_t)n[0] * n[1] * n[2]);
@@ -391,6 +393,7 @@ void fill_waters(Builder& B, Rng& r, int n_target, const Hash& solute, double o_
         B.mols.push_back(std::move(w));
         placed++;
     }
+    return placed;
 }
 
 double total_mass_g(int n_chain_atoms, int n_water) { return n_chain_atoms * C_M + n_water * WATER_MASS; }
@@ -452,7 +455,22 @@ lima_status lima_gen_chain_water(double bx, double by, double bz, int32_t n_chai
             for (auto& p : pos) solute.add(p);
             B.mols.push_back(std::move(m));
         }
-        fill_waters(B, r, n_waters, solute, 0.30, 0.22, std::cbrt(WATER_MASS / GMOL_PER_NM3_AT_1GCC));
+        // Waters on a lattice at 1 g/cm^3 first; when the free volume next to the chains cannot hold
+        // n_waters there, the lattice is made finer and the clash distances shorter, pass by pass
+        // (the structure is relaxed by minimisation before any dynamics).  Fewer than n_waters
+        // after the last pass is an error, never a silently smaller box.
+        const size_t before = B.mols.size();
+        double omin = 0.30, hmin = 0.22, wsp = std::cbrt(WATER_MASS / GMOL_PER_NM3_AT_1GCC);
+        int placed = 0;
+        for (int pass = 0; pass < 12; pass++) {
+            B.mols.resize(before);
+            placed = fill_waters(B, r, n_waters, solute, omin, hmin, wsp);
+            if (placed >= n_waters) break;
+            omin *= 0.97; hmin *= 0.97; wsp *= 0.97;
+        }
+        if (placed < n_waters)
+            throw std::runtime_error("chain + water box: placed " + std::to_string(placed) + " of " + std::to_string(n_waters) +
+                                     " waters; enlarge the box or request fewer");
         *out = assemble(B, r, T);
         return LIMA_OK;
     } catch (const std::exception& e) { g_err = e.what(); return LIMA_ERR_BAD_INPUT; }

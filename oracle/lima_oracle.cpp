@@ -690,15 +690,19 @@ ClusterList build_clusters(const lima_oracle* o) {
             const int64_t J = clof[b];
             for (int code = 0; code < 27; code++) {
                 const int s[3] = {code % 3 - 1, (code / 3) % 3 - 1, code / 9 - 1};
+                // gaps beyond R end the test early: the squares of three gaps of up to two box
+                // lengths would exceed int64 in boxes wider than ~9 nm
                 int64_t d2 = 0;
-                for (int d = 0; d < 3; d++) {
+                bool far = false;
+                for (int d = 0; d < 3 && !far; d++) {
                     const int64_t p = W[b * 3 + d] + s[d] * o->Lq[d];
                     int64_t g = 0;
                     if (p < bmin[I * 3 + d]) g = bmin[I * 3 + d] - p;
                     else if (p > bmax[I * 3 + d]) g = p - bmax[I * 3 + d];
-                    d2 += g * g;
+                    if (g > R) far = true;
+                    else d2 += g * g;
                 }
-                if (d2 > R2) continue;
+                if (far || d2 > R2) continue;
                 uint8_t m = 0;
                 for (int i = 0; i < 8; i++) {
                     const int32_t a = cl.atoms[I * 8 + i];
@@ -1002,11 +1006,13 @@ int64_t lima_oracle_pair_count(lima_oracle* o, double radius_nm) {
             for (int code = 0; code < 27; code++) {
                 const int s[3] = {code % 3 - 1, (code / 3) % 3 - 1, code / 9 - 1};
                 int64_t d2 = 0;
-                for (int d = 0; d < 3; d++) {
+                bool far = false;   // a component beyond R: its square could overflow int64 in wide boxes
+                for (int d = 0; d < 3 && !far; d++) {
                     const int64_t dd = o->wrapped(j, d) + s[d] * o->Lq[d] - o->wrapped(i, d);
-                    d2 += dd * dd;
+                    if (dd > R || dd < -R) far = true;
+                    else d2 += dd * dd;
                 }
-                if (d2 <= R2) { g_pi.push_back((int32_t)i); g_pj.push_back((int32_t)j); g_ps.push_back(code); }
+                if (!far && d2 <= R2) { g_pi.push_back((int32_t)i); g_pj.push_back((int32_t)j); g_ps.push_back(code); }
             }
         }
     return (int64_t)g_pi.size();

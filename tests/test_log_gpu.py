@@ -87,12 +87,11 @@ def test_log_ring_and_graph_replay():
         logs.append(eng.read_log())
     g, e = logs
     assert list(g["step"]) == list(e["step"]) == [10, 15, 20]
-    live = np.abs(e["position"]).sum(axis=2) > 0
-    # float atomics order the force sums differently in the last bits between launch modes
-    assert np.abs(g["position"] - e["position"])[live].max() <= 1e-5
-    assert np.abs(g["force"] - e["force"]).max() <= 1e-3 * np.abs(e["force"]).max()
+    # force sums are in a fixed order, so both launch modes log the same bits
+    assert np.array_equal(g["position"], e["position"])
+    assert np.array_equal(g["force"], e["force"])
     for a, b in zip(g["energy"], e["energy"]):
-        assert abs(a.potential - b.potential) <= 1e-6 * abs(b.potential)
+        assert a.potential == b.potential and a.kinetic == b.kinetic
 
 
 def test_nve_stability_by_energy_analysis():

@@ -13,7 +13,8 @@ mistake (dropped term, wrong sign, transposed index, wrong unit) fails:
   PME total          direct Ewald sum within LIMA's own PME tolerances (ElectrostaticsTests.h:385-392)
   neighbour list     expanded cluster list == brute-force O(N^2) pair set
   integrator         harmonic bond frequency sqrt(k/mu)/2pi within 1% (ForceCorrectness.h:169-230)
-  minimiser          monotone energy; a stretched bond relaxes to b0
+  minimiser          the GROMACS steep rule in closed form on two bonds (accept / refuse sequence,
+                     energies, max |F|); monotone energy; a stretched bond relaxes to b0
   thermostat         force-free gas: velocities v0 s^k with LIMA's clamp and measurement gating
   improper           axis geometry: closed-form energy, rotational gradient on i and l, zero net force
 """
@@ -479,6 +480,66 @@ def test_cluster_list_covers_exactly_the_bruteforce_pairs(which):
     assert len(ent) > 0
 
 
+def test_cluster_list_in_a_wide_box_matches_numpy_bruteforce():
+    """A 12 nm box (Coord differences past 2^31, squared distances of shifted images past 2^63):
+    400 LJ particles at random positions, a quarter of them within 0.5 nm of a box face.  The
+    expanded cluster list equals a brute-force pair set computed here in numpy (per-component
+    screening keeps every square inside int64), and the oracle's own brute force agrees."""
+    rng = np.random.default_rng(12)
+    Lb = 12.0
+    pos = rng.uniform(0.0, Lb, (400, 3))
+    edge = rng.integers(0, 3, 100)
+    pos[:100, :][np.arange(100), edge] = rng.uniform(0.0, 0.5, 100) * np.where(rng.random(100) < 0.5, 1, -1) % Lb
+    box = CustomBox([Lb] * 3, pos, np.zeros(400), np.full(400, 0.3), np.full(400, 0.5), np.full(400, 12.0),
+                    molecule=np.arange(400))
+    p = params(enable_electrostatics=0)
+    o = Oracle(box, p)
+    atoms, off, ent, msk = o.neighbor_list()
+    radius = 1.2 + p.nlist_skin_nm
+    got, n_items = _expand(box, atoms, off, ent, msk, radius)
+    W = np.mod(o.positions(), np.round(box.box_nm * 1e8).astype(np.int64))
+    Lq = int(round(Lb * 1e8))
+    R = int(round(radius * 1e8))
+    brute = set()
+    for code in range(27):
+        s = np.array([code % 3 - 1, (code // 3) % 3 - 1, code // 9 - 1], np.int64)
+        d = W[None, :, :] + s * Lq - W[:, None, :]          # d[i, j] = w_j + s L - w_i
+        near = np.all(np.abs(d) <= R, axis=2)
+        d2 = np.where(near[..., None], d, 0)
+        ok = near & ((d2 * d2).sum(2) <= R * R)
+        for i, j in zip(*np.nonzero(ok)):
+            if i < j:
+                brute.add((int(i), int(j), code))
+    assert len(brute) > 50
+    assert got == brute
+    assert n_items == 2 * len(brute)
+    pi, pj, ps = o.pairs(radius)
+    assert set(zip(pi.tolist(), pj.tolist(), ps.tolist())) == brute
+    # the entries themselves: (cluster, particle, image) with the image within R of the cluster's
+    # bounding box, from the list definition evaluated here (no self pair; no intra pairs exist)
+    ncl = len(off) - 1
+    want = set()
+    for I in range(ncl):
+        mem = [int(a) for a in atoms[I * 8:(I + 1) * 8] if a >= 0]
+        lo, hi = W[mem].min(0), W[mem].max(0)
+        for code in range(27):
+            s = np.array([code % 3 - 1, (code // 3) % 3 - 1, code // 9 - 1], np.int64)
+            q = W + s * Lq
+            g = np.maximum(np.maximum(lo - q, q - hi), 0)
+            near = np.all(g <= R, axis=1)
+            g = np.where(near[:, None], g, 0)
+            inside = near & ((g * g).sum(1) <= R * R)
+            for b in np.nonzero(inside)[0]:
+                if code == 13 and len(mem) == 1 and mem[0] == b:
+                    continue
+                want.add((I, int(b), code))
+    have = set()
+    for I in range(ncl):
+        for k in range(off[I], off[I + 1]):
+            have.add((I, int(ent[k] & 0xFFFFFF), int(ent[k] >> 24)))
+    assert have == want
+
+
 # ---------------------------------------------------------------------------------------------
 def test_integrator_bond_frequency():
     k_kj, b0 = 3e5, 0.1
@@ -585,6 +646,73 @@ def test_thermostat_unclamped_scale_is_target_over_temperature():
     assert T == pytest.approx(T0, rel=1e-5)
     assert scale == pytest.approx(310.0 / T0, rel=1e-6)
     assert abs(scale - 1.0) < 0.001
+
+
+def _sd_closed_form(pos_q, bonds, kb, b0, h0, n_iter, grow=1.2, shrink=0.2, per_atom=False):
+    """Steepest descent on harmonic bonds, derived here from the GROMACS steep rule: every atom moves
+    by h F_i / max_j |F_j| (per component, truncated to integer Coord units of 1e-8 nm); the trial is
+    kept when the potential decreases, after which h grows by `grow`, otherwise h shrinks by
+    `shrink`.  per_atom=True normalises each atom by its own |F_i| instead (a wrong rule the test
+    must tell apart).  Returns the accepted energy [J/mol] and max |F| [kJ/mol/nm] after each
+    iteration, as the oracle's trace holds them."""
+    def energy_forces(p):
+        F = np.zeros((len(p), 3))
+        E = 0.0
+        for a, b in bonds:
+            d = (p[a] - p[b]).astype(np.float64) * 1e-8
+            r = np.sqrt((d * d).sum())
+            E += 0.5 * kb * (r - b0) ** 2
+            f = -kb * (r - b0) * d / r
+            F[a] += f
+            F[b] -= f
+        return E, F
+    pos = pos_q.copy()
+    E, F = energy_forces(pos)
+    h = h0
+    te, tf = [], []
+    for _ in range(n_iter):
+        norm = np.sqrt((F * F).sum(1))
+        scale = norm[:, None] if per_atom else norm.max()
+        trial = pos + np.trunc(h * F / scale * 1e8).astype(np.int64)
+        Et, Ft = energy_forces(trial)
+        if Et < E:
+            pos, E, F = trial, Et, Ft
+            h *= grow
+        else:
+            h *= shrink
+        te.append(E)
+        tf.append(np.sqrt((F * F).sum(1)).max() / 1000.0)
+    return np.array(te), np.array(tf)
+
+
+def test_steepest_descent_follows_the_steep_rule():
+    """Three atoms A-B-C with bonds A-B along x (stretched 0.06 nm) and B-C along y (0.02 nm): B's
+    force has two components and differs in size from A's and C's, so the step normalisation by
+    the largest |F| (not per atom, not per component) decides every displacement.  The oracle's
+    accept / reject sequence, energies and max |F| over 40 iterations equal the closed-form steep
+    rule (h x 1.2 after an accepted trial, x 0.2 after a refused one); swapped factors or per-atom
+    normalisation give a different sequence, so they would fail."""
+    b0 = float(np.float32(0.15))
+    kb = float(np.float32(2e5 * 1000.0))
+    pos_nm = np.array([[1.0 - (0.15 + 0.06), 1.0, 1.0], [1.0, 1.0, 1.0], [1.0, 1.0 + 0.15 + 0.02, 1.0]])
+    box = CustomBox([4.0] * 3, pos_nm, [0, 0, 0], [0, 0, 0], [0, 0, 0], [12.0] * 3, bonds=[(0, 1, 0.15, 2e5), (1, 2, 0.15, 2e5)])
+    o = Oracle(box, params(enable_electrostatics=0))
+    pos_q = o.positions()
+    n_iter = 40
+    r, te, tf = o.minimize(tol=1e-9, max_iter=n_iter, h0=0.01, trace=True)
+    assert r.iterations == n_iter
+    ce, cf = _sd_closed_form(pos_q, [(0, 1), (1, 2)], kb, b0, 0.01, n_iter)
+    acc_oracle = np.diff(np.concatenate([[np.inf], te])) < 0
+    acc_closed = np.diff(np.concatenate([[np.inf], ce])) < 0
+    assert acc_oracle.sum() >= 5 and (~acc_oracle).sum() >= 5, "the run must both accept and refuse trials"
+    assert np.array_equal(acc_oracle, acc_closed)
+    assert np.allclose(te, ce, rtol=1e-6, atol=1e-3)
+    assert np.allclose(tf, cf, rtol=1e-6, atol=1e-6)
+    assert r.accepted == int(acc_closed.sum())
+    # the rules a plausible mistake would implement instead differ from the oracle's trace
+    for kw in (dict(grow=0.2, shrink=1.2), dict(per_atom=True), dict(grow=1.2, shrink=0.5)):
+        we, _ = _sd_closed_form(pos_q, [(0, 1), (1, 2)], kb, b0, 0.01, n_iter, **kw)
+        assert not np.allclose(we, te, rtol=1e-6, atol=1e-3), kw
 
 
 def test_minimizer_monotone_and_bond_reaches_b0():

@@ -58,7 +58,8 @@ def _check(box, tag):
     for k in ("lj", "coulomb_sr", "bonded", "pme_recip"):
         assert abs(getattr(e_gpu, k) - getattr(e_ref, k)) <= 1e-5 * scale + 1e-3, k
     eng.step(20)
-    assert np.isfinite(eng.energy().total)
+    eng.synchronize()                     # raises if a force went non-finite
+    assert np.isfinite(eng.compute_forces().total)
 
 
 def test_dmpc_vacuum_matches_oracle(tmp_path):
