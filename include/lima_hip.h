@@ -163,6 +163,8 @@ typedef struct lima_runstatus {
     float   max_displacement_nm;  /* max displacement over the last completed list interval */
     int32_t capacity_rollbacks;   /* times issued steps were undone and run again after a capacity
                                      overflow (see lima_step) */
+    int32_t list_capacity;        /* pair-list entry slots per i-cluster */
+    int32_t list_max_entries;     /* slots the fullest i-cluster of the current list uses */
 } lima_runstatus;
 
 /* State read-back.  Arrays the caller provides (NULL = skip) in LIMA's

@@ -82,6 +82,8 @@ class RunStatus(C.Structure):
         ("list_violations", C.c_int64),
         ("max_displacement_nm", C.c_float),
         ("capacity_rollbacks", C.c_int32),
+        ("list_capacity", C.c_int32),
+        ("list_max_entries", C.c_int32),
     ]
 
 

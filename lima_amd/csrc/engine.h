@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+#include <cmath>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -53,10 +55,15 @@ struct NList {
     long long* xb[2] = {nullptr, nullptr};   // [n+1] first Coord of each x / y column: ceil(c Lq / n)
     double rinv[2] = {0.0, 0.0};             // n / Lq per axis
     float4* xq = nullptr;      // [slots] position relative to cluster ref [nm] + charge
-    int* ecount = nullptr;     // [ncl_max+1]
-    int* eoff = nullptr;       // [ncl_max+1]
-    float4* ent = nullptr;     // [nent_cap] (shift nm xyz, bits: j slot | mask << 24)
-    uint8_t* ecode = nullptr;  // [nent_cap] image shift code of each entry
+    // Each i-cluster owns the slice [I cap, (I+1) cap) of the entry arrays.  Entries whose j image lay
+    // within the cutoff of the cluster's box at the build fill it from the front (ecount[I] of them,
+    // padded to whole passes of 8), the farther ones (up to cutoff + skin) from the back (efar[I]):
+    // the pair kernel's passes over the far segment mostly find every pair outside the cutoff and
+    // skip their arithmetic.
+    int* ecount = nullptr;     // [ncl_max+1] near entries (padded)
+    int* efar = nullptr;       // [ncl_max+1] far entries (padded), at the slice's end
+    float4* ent = nullptr;     // [nent_cap] (shift between the reference points nm xyz, bits: j slot | mask << 24)
+    uint8_t* ecode = nullptr;  // [nent_cap] image shift code (sx + 3 sy + 9 sz, each 0..2 for -1, 0, +1 box)
 };
 
 struct Pme {
@@ -158,7 +165,7 @@ struct Engine {
     float Lf[3]{};
     float rc = 1.2f, rc2 = 1.44f, skin = 0.1f;
     PairConst pc{};
-    float4* coul_tab = nullptr;   // [COUL_TAB] cubic Hermite coefficients of the Coulomb force factor
+    float4* coul_tab = nullptr;   // [COUL_TAB + 1] cubic Hermite coefficients of the Coulomb force factor in r^2
     double dt_ns = 2e-6;
     bool elec = true;
 
@@ -305,6 +312,13 @@ void launch_reduce_energy(Engine& e, bool kinetic, hipStream_t s);
 void launch_pack(Engine& e, const uint4* w, hipStream_t s);
 BinArgs bin_args(const Engine& e);   // PME tile bins (bin_count == nullptr when PME is not tiled)
 inline int slots_max(const Engine& e) { return e.nl.ncl_max * CLUSTER; }   // SoA stride of the slot arrays
+// 32-bit Coord differences are exact when the box, the list radius and a column's width stay below
+// 2^31 Coord units together (boxes up to ~9.4 nm); the list and pair kernels pick their path by it
+inline bool narrow_coords(const Engine& e) {
+    const long long R = llround(((double)e.rc + (double)e.skin) * 1e8);
+    const long long Lmax = std::max(e.Lq[0], std::max(e.Lq[1], e.Lq[2]));
+    return 2 * Lmax + 2 * R < (1ll << 31);
+}
 int nlist_count(Engine& e);          // current cluster count read back from the device (synchronises)
 void launch_pair(Engine& e, bool energy, hipStream_t s);
 void launch_intra(Engine& e, const uint4* w, const int4* img, bool energy, hipStream_t s);

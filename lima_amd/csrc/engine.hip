@@ -712,16 +712,23 @@ void create(Engine& e, const lima_box_desc* b, const lima_simparams* p) {
     e.pc.kappa = 3.f / e.rc;                                                             // PhysicsUtils.cuh:60-62
     e.pc.two_kappa_sqrtpi = 2.f * e.pc.kappa / sqrtf(LIMA_PI_F);
     {   // H(r) = erfc(kappa r) + c r exp(-kappa^2 r^2), c = 2 kappa / sqrt(PI_L), and
-        // H'(r) = (c (1 - 2 kappa^2 r^2) - 2 kappa / sqrt(pi)) exp(-kappa^2 r^2), in double
-        const double kap = (double)e.pc.kappa, cc = (double)e.pc.two_kappa_sqrtpi, h = (double)e.rc / COUL_TAB;
-        auto H = [&](double r) { return std::erfc(kap * r) + cc * r * std::exp(-kap * kap * r * r); };
-        auto dH = [&](double r) { return (cc * (1.0 - 2.0 * kap * kap * r * r) - 2.0 * kap / std::sqrt(M_PI)) * std::exp(-kap * kap * r * r); };
-        std::vector<float4> tab(COUL_TAB);
+        // H'(r) = (c (1 - 2 kappa^2 r^2) - 2 kappa / sqrt(pi)) exp(-kappa^2 r^2), in double, tabulated
+        // in z = r^2 with dH/dz = H'(r) / (2 r) (its limit at r = 0 is 0)
+        const double kap = (double)e.pc.kappa, cc = (double)e.pc.two_kappa_sqrtpi;
+        const double hz = (double)e.rc * (double)e.rc / COUL_TAB;
+        auto H = [&](double z) { const double r = std::sqrt(z); return std::erfc(kap * r) + cc * r * std::exp(-kap * kap * z); };
+        auto dH = [&](double z) {
+            const double r = std::sqrt(z);
+            if (r == 0.0) return 0.0;
+            return (cc * (1.0 - 2.0 * kap * kap * z) - 2.0 * kap / std::sqrt(M_PI)) * std::exp(-kap * kap * z) / (2.0 * r);
+        };
+        std::vector<float4> tab(COUL_TAB + 1);
         for (int i = 0; i < COUL_TAB; i++) {
-            const double r0 = i * h, r1 = (i + 1) * h;
-            const double h0 = H(r0), h1 = H(r1), d0 = dH(r0) * h, d1 = dH(r1) * h;
+            const double z0 = i * hz, z1 = (i + 1) * hz;
+            const double h0 = H(z0), h1 = H(z1), d0 = dH(z0) * hz, d1 = dH(z1) * hz;
             tab[i] = make_float4((float)h0, (float)d0, (float)(3.0 * (h1 - h0) - 2.0 * d0 - d1), (float)(2.0 * (h0 - h1) + d0 + d1));
         }
+        tab[COUL_TAB] = make_float4((float)H(COUL_TAB * hz), 0.f, 0.f, 0.f);   // r2 == cutoff^2 exactly
         e.coul_tab = dalloc_copy(tab);
     }
     e.dt_ns = (double)(p->dt_fs * 1e-6f);
@@ -1336,9 +1343,13 @@ lima_status lima_get_status(lima_engine* h, lima_runstatus* out) {
     int ncl = 0;
     if (e.nl.ncl_dev && hipMemcpy(&ncl, e.nl.ncl_dev, sizeof(int), hipMemcpyDeviceToHost) == hipSuccess && ncl > 0) {
         e.nl.ncl = ncl;   // padded entry slots of the current list (summed here, not per rebuild)
-        std::vector<int> cnt(ncl);
-        if (hipMemcpy(cnt.data(), e.nl.ecount, cnt.size() * sizeof(int), hipMemcpyDeviceToHost) == hipSuccess)
-            for (int c : cnt) out->list_entries += c;
+        std::vector<int> cnt(ncl), far(ncl);
+        if (hipMemcpy(cnt.data(), e.nl.ecount, cnt.size() * sizeof(int), hipMemcpyDeviceToHost) == hipSuccess &&
+            hipMemcpy(far.data(), e.nl.efar, far.size() * sizeof(int), hipMemcpyDeviceToHost) == hipSuccess)
+            for (int c = 0; c < ncl; c++) {
+                out->list_entries += cnt[c] + far[c];
+                out->list_max_entries = std::max(out->list_max_entries, cnt[c] + far[c]);
+            }
     }
     int st[2] = {0, 0};
     if (e.lstats && hipMemcpy(st, e.lstats, sizeof(st), hipMemcpyDeviceToHost) == hipSuccess) {
@@ -1349,7 +1360,49 @@ lima_status lima_get_status(lima_engine* h, lima_runstatus* out) {
     }
     out->greatest_force = NAN;
     out->capacity_rollbacks = e.rollbacks;
+    out->list_capacity = e.nl.cap;
     return status;
+}
+
+// The current list read back in cluster order: per cluster its near, then its far entries, mask-0
+// padding dropped; entries as global particle | shift code << 24.
+struct HostList {
+    std::vector<int> catoms;
+    std::vector<int64_t> off;
+    std::vector<uint32_t> ent;
+    std::vector<uint8_t> mask;
+};
+static HostList read_list(Engine& e) {
+    HostList L;
+    const int ncl = nlist_count(e);
+    const int64_t cap = e.nl.cap;
+    std::vector<int> cnt(ncl), far(ncl);
+    std::vector<float4> ent((size_t)ncl * cap);
+    std::vector<uint8_t> code((size_t)ncl * cap);
+    L.catoms.resize((size_t)ncl * CLUSTER);
+    HIP_CHECK(hipMemcpy(L.catoms.data(), e.nl.catoms, L.catoms.size() * sizeof(int), hipMemcpyDeviceToHost));
+    if (ncl) {
+        HIP_CHECK(hipMemcpy(cnt.data(), e.nl.ecount, ncl * sizeof(int), hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(far.data(), e.nl.efar, ncl * sizeof(int), hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(ent.data(), e.nl.ent, ent.size() * sizeof(float4), hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(code.data(), e.nl.ecode, code.size(), hipMemcpyDeviceToHost));
+    }
+    L.off.assign(ncl + 1, 0);
+    for (int I = 0; I < ncl; I++) {
+        L.off[I] = (int64_t)L.ent.size();
+        auto take = [&](int64_t k) {
+            uint32_t bits;
+            std::memcpy(&bits, &ent[k].w, 4);
+            const uint32_t m = bits >> 24;
+            if (!m) return;
+            L.ent.push_back((uint32_t)L.catoms[bits & 0xffffffu] | ((uint32_t)code[k] << 24));
+            L.mask.push_back((uint8_t)m);
+        };
+        for (int64_t k = I * cap; k < I * cap + cnt[I]; k++) take(k);
+        for (int64_t k = I * cap + cap - far[I]; k < (I + 1) * cap; k++) take(k);
+    }
+    L.off[ncl] = (int64_t)L.ent.size();
+    return L;
 }
 
 lima_status lima_get_neighbor_counts(lima_engine* h, int64_t* ncl, int64_t* nent) {
@@ -1360,21 +1413,9 @@ lima_status lima_get_neighbor_counts(lima_engine* h, int64_t* ncl, int64_t* nent
         nlist_build(e, e.p.w, /*checked=*/true);
         e.packed = false;
         HIP_CHECK(hipStreamSynchronize(e.s_main));
-        const int nc = nlist_count(e);
-        std::vector<int> cnt(nc);
-        HIP_CHECK(hipMemcpy(cnt.data(), e.nl.ecount, nc * sizeof(int), hipMemcpyDeviceToHost));
-        int64_t n = 0;
-        for (int I = 0; I < nc; I++) {
-            std::vector<float4> ent(cnt[I]);
-            if (cnt[I]) HIP_CHECK(hipMemcpy(ent.data(), e.nl.ent + (int64_t)I * e.nl.cap, cnt[I] * sizeof(float4), hipMemcpyDeviceToHost));
-            for (auto& r : ent) {
-                uint32_t bits;
-                std::memcpy(&bits, &r.w, 4);
-                if (bits >> 24) n++;
-            }
-        }
-        *ncl = nc;
-        *nent = n;
+        const HostList L = read_list(e);
+        *ncl = (int64_t)L.off.size() - 1;
+        *nent = (int64_t)L.ent.size();
     });
 }
 
@@ -1383,33 +1424,11 @@ lima_status lima_get_neighbor_list(lima_engine* h, int32_t* cluster_atoms, int64
     return guarded([&] {
         Engine& e = h->e;
         settle(e);
-        const int ncl = nlist_count(e);
-        const int64_t cap = e.nl.cap;
-        std::vector<int> cat((size_t)ncl * CLUSTER), cnt(ncl);
-        std::vector<float4> ent((size_t)ncl * cap);
-        std::vector<uint8_t> code((size_t)ncl * cap);
-        HIP_CHECK(hipMemcpy(cat.data(), e.nl.catoms, cat.size() * sizeof(int), hipMemcpyDeviceToHost));
-        HIP_CHECK(hipMemcpy(cnt.data(), e.nl.ecount, ncl * sizeof(int), hipMemcpyDeviceToHost));
-        if (ncl) {
-            HIP_CHECK(hipMemcpy(ent.data(), e.nl.ent, ent.size() * sizeof(float4), hipMemcpyDeviceToHost));
-            HIP_CHECK(hipMemcpy(code.data(), e.nl.ecode, code.size(), hipMemcpyDeviceToHost));
-        }
-        if (cluster_atoms) std::memcpy(cluster_atoms, cat.data(), cat.size() * sizeof(int));
-        int64_t n = 0;
-        for (int I = 0; I < ncl; I++) {
-            if (offsets) offsets[I] = n;
-            for (int64_t k = I * cap; k < I * cap + cnt[I]; k++) {
-                uint32_t bits;
-                std::memcpy(&bits, &ent[k].w, 4);
-                const uint32_t m = bits >> 24;
-                if (!m) continue;
-                const int slot = (int)(bits & 0xffffffu);
-                if (entries) entries[n] = (uint32_t)cat[slot] | ((uint32_t)code[k] << 24);
-                if (masks) masks[n] = (uint8_t)m;
-                n++;
-            }
-        }
-        if (offsets) offsets[ncl] = n;
+        const HostList L = read_list(e);
+        if (cluster_atoms) std::memcpy(cluster_atoms, L.catoms.data(), L.catoms.size() * sizeof(int));
+        if (offsets) std::memcpy(offsets, L.off.data(), L.off.size() * sizeof(int64_t));
+        if (entries) std::memcpy(entries, L.ent.data(), L.ent.size() * sizeof(uint32_t));
+        if (masks) std::memcpy(masks, L.mask.data(), L.mask.size());
     });
 }
 

@@ -89,12 +89,13 @@ struct PairArgs {
     int ncl_max, nslots, cap;
     int split;           // waves per i-cluster (each takes a contiguous share of its entries)
     float rc2;
-    float tscale;        // COUL_TAB / cutoff
+    float zscale;        // COUL_TAB / cutoff^2
     PairConst pc;
     const float4* coul_tab;
     const float4* xq;
     const float2* ljs;
-    const int* ecount;
+    const int* ecount;   // near entries of each cluster (front of its slice)
+    const int* efar;     // far entries (back of its slice)
     const float4* ent;
     float* fpair;    // [3][nslots]
     float* epair;    // [nslots]
@@ -111,6 +112,9 @@ struct PairArgs {
 // (x, y, z, q) and (sigma/2, sqrt(24 eps), mask bits) in the wave's LDS slice.  The 8 lanes of a
 // group then read their entry with two broadcast ds_read_b128, so the texture path sees each
 // datum once instead of eight times.  Two LDS buffers: chunk c + 1 is loaded while c is computed.
+// A cluster's near entries come first, its far entries (beyond the cutoff from its box at the
+// build) last, so whole passes of the far segment usually fail the cutoff test and skip the
+// arithmetic.
 constexpr int PAIR_WAVES = 4;
 constexpr int CHUNK = 64;
 
@@ -118,9 +122,9 @@ template <bool ENERGY>
 __global__ __launch_bounds__(256) void k_pair(PairArgs a) {
     __shared__ float4 sA[PAIR_WAVES][2][CHUNK];
     __shared__ float4 sB[PAIR_WAVES][2][CHUNK];
-    __shared__ float4 tab[COUL_TAB];
+    __shared__ float4 tab[COUL_TAB + 1];
     __shared__ float4 part_f[PAIR_WAVES][CLUSTER];   // each wave's share of its cluster's i forces (+ potE)
-    for (int i = threadIdx.x; i < COUL_TAB; i += blockDim.x) tab[i] = a.coul_tab[i];
+    for (int i = threadIdx.x; i <= COUL_TAB; i += blockDim.x) tab[i] = a.coul_tab[i];
     __syncthreads();
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     // a workgroup holds PAIR_WAVES / split whole clusters: the waves of one cluster are consecutive
@@ -128,6 +132,7 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs a) {
     const int I = wv / a.split, part = wv - I * a.split;
     const bool live = I < min(*a.ncl_dev, a.ncl_max);   // the grid covers the capacity (uniform per wave)
     const int ii = lane & 7, jj = lane >> 3;
+    const uint32_t ibit = 1u << ii;
     const int islot = I * CLUSTER + ii;
     float4 xi = make_float4(0.f, 0.f, 0.f, 0.f);
     float2 pi = make_float2(0.f, 0.f);
@@ -136,9 +141,11 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs a) {
     float fx = 0.f, fy = 0.f, fz = 0.f;
     double elj = 0.0, ecl = 0.0;   // FP64 energy accumulation (energy steps only)
     int npairs = 0;
-    const int ntot = live ? a.ecount[I] / CLUSTER : 0;
+    const int nn = live ? a.ecount[I] : 0, nf = live ? a.efar[I] : 0;
+    const int ntot = (nn + nf) / CLUSTER;
     const int it0 = (int)((int64_t)ntot * part / a.split), it1 = (int)((int64_t)ntot * (part + 1) / a.split);
-    const int64_t e0 = (int64_t)I * a.cap + (int64_t)it0 * CLUSTER;
+    const int64_t base = (int64_t)I * a.cap;
+    const int v0 = it0 * CLUSTER;                    // this wave's first entry in the near + far order
     const int nent = (it1 - it0) * CLUSTER;          // entries of this wave (multiple of 8)
     const int nchunk = (nent + CHUNK - 1) / CHUNK;   // wave-uniform
 
@@ -146,7 +153,8 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs a) {
     auto fetch = [&](int c) {   // global loads of chunk c into registers (lane l: entry l)
         const int k = c * CHUNK + lane;
         if (k < nent) {
-            const float4 rec = a.ent[e0 + k];
+            const int v = v0 + k;
+            const float4 rec = a.ent[base + (v < nn ? v : a.cap - nf + (v - nn))];
             const uint32_t bits = __float_as_uint(rec.w);
             const int b = (int)(bits & 0xffffffu);
             const float4 xj = a.xq[b];
@@ -174,18 +182,19 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs a) {
             const uint32_t m = __float_as_uint(B.z);
             const float dx = A.x - xi.x, dy = A.y - xi.y, dz = A.z - xi.z;
             const float r2 = dx * dx + dy * dy + dz * dz;
-            const bool inc = ((m >> ii) & 1u) && (r2 <= a.rc2);   // the list masks exclude self and padding pairs
-            const unsigned long long any = __ballot(inc);
-            if (any == 0ull) continue;
-            if (ENERGY) npairs += __popcll(any);
-            float el = 0.f, ec = 0.f;
-            float S = pair_scalar_tab(inc ? r2 : 1.0f, pi.x + B.x, pi.y * B.y, qi_kc * A.w, a.pc, tab, a.tscale,
-                                      ENERGY ? &el : nullptr, &ec);
-            S = inc ? S : 0.f;
-            fx += dx * S; fy += dy * S; fz += dz * S;
-            if (ENERGY && inc) {
-                elj += 0.5 * (double)el;   // each pair is evaluated from both sides: half per side
-                ecl += 0.5 * (double)ec;
+            const bool inc = (m & ibit) && (r2 <= a.rc2);   // the list masks exclude self and padding pairs
+            // the force arithmetic runs under the lanes' own mask (no selects); a pass where no lane
+            // is inside the cutoff branches over it
+            if (inc) {
+                if (ENERGY) npairs++;
+                float el = 0.f, ec = 0.f;
+                const float S = pair_scalar_tab(r2, pi.x + B.x, pi.y * B.y, qi_kc * A.w, a.pc, tab, a.zscale,
+                                                ENERGY ? &el : nullptr, &ec);
+                fx = fmaf(dx, S, fx); fy = fmaf(dy, S, fy); fz = fmaf(dz, S, fz);
+                if (ENERGY) {
+                    elj += 0.5 * (double)el;   // each pair is evaluated from both sides: half per side
+                    ecl += 0.5 * (double)ec;
+                }
             }
         }
         if (c + 1 < nchunk) {
@@ -228,11 +237,11 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs a) {
         }
     }
     if (ENERGY) {
-        const double tl = wave_sum_d(elj), tc = wave_sum_d(ecl);
+        const double tl = wave_sum_d(elj), tc = wave_sum_d(ecl), np = wave_sum_d((double)npairs);
         if (lane == 0 && wv < a.ncl_max * a.split) {   // waves past the live count write zero records
             double* r = a.epart + (int64_t)wv * ACC_N;
             r[ACC_LJ] = tl; r[ACC_COUL] = tc; r[ACC_BOND] = 0.0; r[ACC_PME] = 0.0; r[ACC_KIN] = 0.0;
-            r[ACC_PAIRS] = (double)npairs * 0.5;   // both sides counted
+            r[ACC_PAIRS] = np * 0.5;   // both sides counted
         }
     }
 }
@@ -589,16 +598,17 @@ void launch_pair(Engine& e, bool energy, hipStream_t s) {
     a.ncl_max = e.nl.ncl_max;
     a.nslots = slots_max(e);
     a.rc2 = e.rc2;
-    a.tscale = (float)COUL_TAB / e.rc;
+    a.zscale = (float)COUL_TAB / e.rc2;
     a.pc = e.pc;
     a.coul_tab = e.coul_tab;
     a.cap = e.nl.cap;
-    a.xq = e.nl.xq; a.ljs = e.nl.ljs; a.ecount = e.nl.ecount; a.ent = e.nl.ent;
+    a.xq = e.nl.xq; a.ljs = e.nl.ljs; a.ecount = e.nl.ecount; a.efar = e.nl.efar; a.ent = e.nl.ent;
     a.fpair = e.fpair; a.epair = e.epair; a.epart = e.epart + (int64_t)e.ereg.pair * ACC_N;
     a.split = pair_split(e);
     const int64_t waves = (int64_t)a.ncl_max * a.split;
-    if (energy) hipLaunchKernelGGL(k_pair<true>, dim3(nblk(waves, PAIR_WAVES)), dim3(64 * PAIR_WAVES), 0, s, a);
-    else hipLaunchKernelGGL(k_pair<false>, dim3(nblk(waves, PAIR_WAVES)), dim3(64 * PAIR_WAVES), 0, s, a);
+    const dim3 grid(nblk(waves, PAIR_WAVES)), block(64 * PAIR_WAVES);
+    if (energy) hipLaunchKernelGGL(k_pair<true>, grid, block, 0, s, a);
+    else hipLaunchKernelGGL(k_pair<false>, grid, block, 0, s, a);
 }
 
 IntraArgs intra_args(Engine& e, const uint4* w, const int4* img) {

@@ -17,6 +17,15 @@ constexpr float PAD_POS = 1.0e4f;   // position of padding slots, far outside ev
 
 struct f3 { float x, y, z; };
 
+// Image shift of one axis, code 0..2 for -1, 0, +1 box lengths; codes of a list entry are
+// sx + 3 sy + 9 sz.  shift_code_axes splits a code < 27 without a division.
+__host__ __device__ __forceinline__ long long shift_of(int c, long long Lq) { return c == 0 ? -Lq : (c == 2 ? Lq : 0ll); }
+__device__ __forceinline__ void shift_code_axes(int cd, int* cx, int* cy, int* cz) {
+    const int z = (cd * 57) >> 9, rem = cd - 9 * z, y = (rem * 11) >> 5;
+    *cz = z; *cy = y; *cx = rem - 3 * y;
+}
+
+
 __device__ __forceinline__ int64_t unwrapped(uint32_t w, int32_t img, int64_t Lq) { return (int64_t)w + (int64_t)img * Lq; }
 
 // Sum over aligned groups of 8 lanes: quad_perm xor1, quad_perm xor2, row_half_mirror.
@@ -107,34 +116,36 @@ __device__ __forceinline__ float pair_scalar_cut(float r2, float sig, float eps2
 }
 
 // Coulomb force factor H(r) = erfc(kappa r) + (2 kappa / sqrt(PI_L)) r exp(-kappa^2 r^2) of the
-// pair force above (force on self = diff * kC qq H(r) / r^3), tabulated on [0, cutoff] in
-// COUL_TAB intervals as cubic Hermite polynomials in the local coordinate t in [0, 1)
-// (c0 + c1 t + c2 t^2 + c3 t^3; values and derivatives of H in double at the nodes).  The
-// interpolation error is below 1e-9 absolute (h^4 H''''/384, h = cutoff / 256).
+// pair force above (force on self = diff * kC qq H(r) / r^3), tabulated as a function of z = r^2
+// on [0, cutoff^2] in COUL_TAB intervals as cubic Hermite polynomials in the local coordinate
+// t in [0, 1) (c0 + c1 t + c2 t^2 + c3 t^3; values and z-derivatives of H in double at the nodes),
+// plus one entry for z = cutoff^2 itself.  H(sqrt z) = 1 - (4 kappa^3 / 3 sqrt(pi)) z^1.5 + ...
+// is smooth enough in z that the interpolation error stays below 3e-7 of H for r >= 0.1 nm
+// (h = cutoff^2 / 256); indexing by r^2 saves the square root and the index clamp.
 constexpr int COUL_TAB = 256;
 
-// pair_scalar_cut with the Coulomb force factor read from the LDS table (tab, tscale = COUL_TAB /
-// cutoff); the energies, when asked for, are evaluated as in pair_scalar_cut.
+// pair_scalar_cut with the Coulomb force factor read from the LDS table (tab[COUL_TAB + 1], zscale =
+// COUL_TAB / cutoff^2); r2 <= cutoff^2.  The energies, when asked for, are evaluated as in
+// pair_scalar_cut.
 __device__ __forceinline__ float pair_scalar_tab(float r2, float sig, float eps24, float qq_kc, const PairConst& c,
-                                                 const float4* tab, float tscale, float* e_lj, float* e_c) {
+                                                 const float4* tab, float zscale, float* e_lj, float* e_c) {
     const float rinv = __builtin_amdgcn_rsqf(r2);
     const float rinv2 = rinv * rinv;
     float s = sig * sig * rinv2;
     s = s * s * s;
-    const float flj = eps24 * s * rinv2 * (1.0f - 2.0f * s);
-    const float x = (r2 * rinv) * tscale;
-    const int i = min((int)x, COUL_TAB - 1);
-    const float t = x - (float)i;
-    const float4 k = tab[i];
-    const float Hr = fmaf(fmaf(fmaf(k.w, t, k.z), t, k.y), t, k.x);
-    const float fc = (qq_kc * rinv2) * (rinv * Hr);
+    const float x = r2 * zscale;
+    const float4 k = tab[(int)x];
+    const float t = __builtin_amdgcn_fractf(x);
+    const float H = fmaf(fmaf(fmaf(k.w, t, k.z), t, k.y), t, k.x);
+    // LJ eps24 s rinv2 (1 - 2 s) minus Coulomb qq H rinv^3, with the common rinv2 factored out
+    const float S = rinv2 * fmaf(eps24 * s, fmaf(-2.f, s, 1.f), -((qq_kc * rinv) * H));
     if (e_lj) {
         *e_lj = (eps24 * (1.0f / 6.0f)) * s * (s - 1.0f);
         float el, ecl;
         pair_scalar_cut(r2, sig, eps24, qq_kc, c, &el, &ecl);
         *e_c = ecl;
     }
-    return flj - fc;
+    return S;
 }
 
 }  // namespace lima

@@ -211,8 +211,9 @@ struct NlArgs {
     const uint4* cref;
     const uint4* cmin;
     const uint4* cmax;
-    int* ecount;
-    const int* eoff;
+    int* ecount;     // near entries of each cluster (padded)
+    int* efar;       // far entries (padded), at the end of the slice
+    long long F2;    // cutoff^2 in Coord^2: an entry whose image lies farther from the i box is far
     float4* ent;
     uint8_t* ecode;
 };
@@ -282,9 +283,6 @@ __device__ __forceinline__ void axis_pick(const AxisRanges& r, int k, int* col, 
 
 constexpr int NL_FLAT = 512;   // flat j-cluster table per wave (LDS), filled in chunks
 
-// Image shift of code = sx + 3 sy + 9 sz (each 0..2 for -1, 0, +1 box lengths).
-__device__ __forceinline__ long long shift_of(int c, long long Lq) { return c == 0 ? -Lq : (c == 2 ? Lq : 0ll); }
-
 // Is the shifted box of j-cluster J within R of the i box?  A particle is never nearer than its
 // cluster's box, so dropping J here never drops an entry.
 __device__ __forceinline__ bool jbox_near(const NlArgs& a, int J, int code, const long long* mn, const long long* mx) {
@@ -303,9 +301,12 @@ __device__ __forceinline__ bool jbox_near(const NlArgs& a, int J, int code, cons
 // into an LDS table of (J, shift); lanes test 64 table entries at a time for a shifted box within R
 // of the i box and pack the survivors into a second table.  The wave takes 8 survivors x 8 slots
 // per pass, with the next pass's loads in flight, and tests each particle against the i box.  Entries go to the cluster's
-// fixed-capacity slice [I cap, (I+1) cap); the count (padded to a multiple of 8 with mask-0
-// entries) goes to ecount[I].  A slice that would overflow sets the DF_PAIRLIST word and the host
-// rebuilds with a larger capacity (at once, or by a rollback when the rebuild was not waited for).
+// fixed-capacity slice [I cap, (I+1) cap): an entry whose image lies within the cutoff of the i box
+// goes to the front, one farther away (within cutoff + skin) to the back, and each segment is
+// padded to a multiple of 8 with mask-0 entries (counts in ecount[I], efar[I]).  The order does not
+// change which entries exist (the integer bounding-box test with cutoff + skin).  A slice that
+// would overflow sets the DF_PAIRLIST word and the host rebuilds with a larger capacity (at once, or
+// by a rollback when the rebuild was not waited for).
 template <bool NARROW>
 __global__ __launch_bounds__(256) void k_nlist(NlArgs a) {
     __shared__ uint32_t flat_all[4][NL_FLAT], cand_all[4][NL_FLAT];
@@ -348,7 +349,7 @@ __global__ __launch_bounds__(256) void k_nlist(NlArgs a) {
     const AxisRanges rx = axis_ranges(mn[0], mx[0], a.R, a.Lq[0], a.ncx, a.xb[0], a.rinv[0]);
     const int nunits = ry.n * rx.n * 3;
     const int jsub = lane >> 3, sl = lane & 7;
-    int count = 0;
+    int cnear = 0, cfar = 0;   // entries of the near (front) and far (back) segments
     const int64_t base = (int64_t)I * a.cap;
     for (int ub = 0; ub < nunits; ub += 64) {
         // ---- this lane's unit: a z range of one shifted column ----
@@ -412,24 +413,24 @@ __global__ __launch_bounds__(256) void k_nlist(NlArgs a) {
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             // ---- 8 j-clusters x 8 slots per pass; next pass's loads in flight ----
-            auto fetch = [&](int f, int& J, int& cd, uint4& pb, int& cb, uint4& rj) {
+            auto fetch = [&](int f, int& J, int& cd, uint4& pb, int& cb) {
                 const bool ok = f < c1;
                 const uint32_t t = flat[ok ? f - c0 : 0];
                 J = ok ? (int)(t & 0xffffffu) : I;   // past the end: a valid cluster, never tested
                 cd = ok ? (int)(t >> 24) : -1;
                 pb = a.cw[J * CLUSTER + sl];
                 cb = a.comps[J * CLUSTER + sl];
-                rj = a.cref[J];
             };
             int J, cd, cb, nJ, ncd, ncb;
-            uint4 pb, rj, npb, nrj;
-            fetch(c0 + jsub, J, cd, pb, cb, rj);
+            uint4 pb, npb;
+            fetch(c0 + jsub, J, cd, pb, cb);
             for (int f0 = c0; f0 < c1; f0 += CLUSTER) {
-                fetch(f0 + CLUSTER + jsub, nJ, ncd, npb, ncb, nrj);
+                fetch(f0 + CLUSTER + jsub, nJ, ncd, npb, ncb);
                 uint32_t mask = 0;
+                bool far = false;      // the j image is farther than the cutoff from the whole i box
                 if (cd >= 0 && pb.w == 0u) {
-                    // shift code cd = sx + 3 sy + 9 sz, each 0..2 (division-free for cd < 27)
-                    const int czi = (cd * 57) >> 9, rem = cd - 9 * czi, cyi = (rem * 11) >> 5, cxi = rem - 3 * cyi;
+                    int cxi, cyi, czi;
+                    shift_code_axes(cd, &cxi, &cyi, &czi);
                     bool in;
                     if (NARROW) {
                         // Lq + R + cluster extents < 2^31: every difference below is an exact int32
@@ -439,13 +440,17 @@ __global__ __launch_bounds__(256) void k_nlist(NlArgs a) {
                         const uint32_t gx = min((uint32_t)max(0, max((int)bmn.x - px, px - (int)bmx.x)), R1);
                         const uint32_t gy = min((uint32_t)max(0, max((int)bmn.y - py, py - (int)bmx.y)), R1);
                         const uint32_t gz = min((uint32_t)max(0, max((int)bmn.z - pz, pz - (int)bmx.z)), R1);
-                        in = (unsigned long long)gx * gx + (unsigned long long)gy * gy + (unsigned long long)gz * gz <= (unsigned long long)a.R2;
+                        const unsigned long long g2 = (unsigned long long)gx * gx + (unsigned long long)gy * gy + (unsigned long long)gz * gz;
+                        in = g2 <= (unsigned long long)a.R2;
+                        far = g2 > (unsigned long long)a.F2;
                     } else {
                         const long long sx = shift_of(cxi, a.Lq[0]), sy = shift_of(cyi, a.Lq[1]), sz = shift_of(czi, a.Lq[2]);
                         const long long gx = min(gap1((long long)pb.x + sx, (long long)pb.x + sx, mn[0], mx[0]), a.R + 1);
                         const long long gy = min(gap1((long long)pb.y + sy, (long long)pb.y + sy, mn[1], mx[1]), a.R + 1);
                         const long long gz = min(gap1((long long)pb.z + sz, (long long)pb.z + sz, mn[2], mx[2]), a.R + 1);
-                        in = gx * gx + gy * gy + gz * gz <= a.R2;
+                        const long long g2 = gx * gx + gy * gy + gz * gz;
+                        in = g2 <= a.R2;
+                        far = g2 > a.F2;
                     }
                     if (in) {
                         if (cb < ci_lo - 2 || cb > ci_hi + 2) {
@@ -463,12 +468,16 @@ __global__ __launch_bounds__(256) void k_nlist(NlArgs a) {
                     }
                 }
                 const bool emit = mask != 0;
-                const unsigned long long bal = __ballot(emit);
+                const unsigned long long bn = __ballot(emit && !far), bf = __ballot(emit && far);
                 if (emit) {
-                    const int pos = count + __popcll(bal & ((1ull << lane) - 1ull));
-                    if (pos < a.cap) {
+                    const unsigned long long lt = (1ull << lane) - 1ull;
+                    const int pos = far ? cfar + __popcll(bf & lt) : cnear + __popcll(bn & lt);
+                    if (pos < a.cap) {   // near from the front, far from the back; a collision is an overflow
+                        const int64_t idx = base + (far ? a.cap - 1 - pos : pos);
                         // shift between the cluster reference points (exact integer, then one rounding)
-                        const int czi = (cd * 57) >> 9, rem = cd - 9 * czi, cyi = (rem * 11) >> 5, cxi = rem - 3 * cyi;
+                        int cxi, cyi, czi;
+                        shift_code_axes(cd, &cxi, &cyi, &czi);
+                        const uint4 rj = a.cref[J];
                         float sxf, syf, szf;
                         if (NARROW) {
                             sxf = (float)(int)(rj.x - refI.x + (uint32_t)shift_of(cxi, a.Lq[0])) * LIMA_LIMA_TO_NANO_F;
@@ -479,28 +488,42 @@ __global__ __launch_bounds__(256) void k_nlist(NlArgs a) {
                             syf = (float)(double)((long long)rj.y - (long long)refI.y + shift_of(cyi, a.Lq[1])) * LIMA_LIMA_TO_NANO_F;
                             szf = (float)(double)((long long)rj.z - (long long)refI.z + shift_of(czi, a.Lq[2])) * LIMA_LIMA_TO_NANO_F;
                         }
-                        a.ent[base + pos] = make_float4(sxf, syf, szf, __uint_as_float((uint32_t)(J * CLUSTER + sl) | (mask << 24)));
-                        a.ecode[base + pos] = (uint8_t)cd;
+                        a.ent[idx] = make_float4(sxf, syf, szf, __uint_as_float((uint32_t)(J * CLUSTER + sl) | (mask << 24)));
+                        a.ecode[idx] = (uint8_t)cd;
                     }
                 }
-                count += __popcll(bal);
-                J = nJ; cd = ncd; pb = npb; cb = ncb; rj = nrj;
+                cnear += __popcll(bn);
+                cfar += __popcll(bf);
+                J = nJ; cd = ncd; pb = npb; cb = ncb;
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
     }
-    const int padded = (count + CLUSTER - 1) & ~(CLUSTER - 1);
-    if (padded > a.cap) {
-        if (lane == 0) { atomicOr(&a.flags[DF_PAIRLIST], 1); atomicMax(&a.flags[DF_LIST_NEED], padded); a.ecount[I] = 0; }
+    const int pn = (cnear + CLUSTER - 1) & ~(CLUSTER - 1), pf = (cfar + CLUSTER - 1) & ~(CLUSTER - 1);
+    if (pn + pf > a.cap) {
+        if (lane == 0) {
+            atomicOr(&a.flags[DF_PAIRLIST], 1);
+            atomicMax(&a.flags[DF_LIST_NEED], pn + pf);
+            a.ecount[I] = 0;
+            a.efar[I] = 0;
+        }
         return;
     }
-    for (int k = count + lane; k < padded; k += 64) {
+    // padding entries: the cluster's own first slot, mask 0, no shift
+    for (int k = cnear + lane; k < pn; k += 64) {
         a.ent[base + k] = make_float4(0.f, 0.f, 0.f, __uint_as_float((uint32_t)(I * CLUSTER)));
         a.ecode[base + k] = 13;
     }
-    if (lane == 0) a.ecount[I] = padded;
+    for (int k = cfar + lane; k < pf; k += 64) {
+        a.ent[base + a.cap - 1 - k] = make_float4(0.f, 0.f, 0.f, __uint_as_float((uint32_t)(I * CLUSTER)));
+        a.ecode[base + a.cap - 1 - k] = 13;
+    }
+    if (lane == 0) {
+        a.ecount[I] = pn;
+        a.efar[I] = pf;
+    }
 }
 
 // Single-block exclusive scan of n ints; out[n] = total.
@@ -584,7 +607,7 @@ void nlist_alloc(Engine& e) {
     HIP_CHECK(hipMalloc(&nl.cw, slots * sizeof(uint4)));
     HIP_CHECK(hipMalloc(&nl.xq, slots * sizeof(float4)));
     HIP_CHECK(hipMalloc(&nl.ecount, (nl.ncl_max + 1) * sizeof(int)));
-    HIP_CHECK(hipMalloc(&nl.eoff, (nl.ncl_max + 2) * sizeof(int)));
+    HIP_CHECK(hipMalloc(&nl.efar, (nl.ncl_max + 1) * sizeof(int)));
     for (float** f : {&e.fpair, &e.fintra, &e.fb, &e.fpme}) {
         HIP_CHECK(hipMalloc(f, 3 * slots * sizeof(float)));
         HIP_CHECK(hipMemset(*f, 0, 3 * slots * sizeof(float)));
@@ -602,7 +625,7 @@ void nlist_alloc(Engine& e) {
 void nlist_free(Engine& e) {
     NList& nl = e.nl;
     void* ptrs[] = {nl.col_of, nl.col_count, nl.col_start, nl.col_fill, nl.keys, nl.col_ncl, nl.catoms, nl.slot_of,
-                    nl.ccol, nl.cref, nl.cmin, nl.cmax, nl.ljs, nl.comps, nl.cw, nl.xb[0], nl.xb[1], nl.xq, nl.ecount, nl.eoff,
+                    nl.ccol, nl.cref, nl.cmin, nl.cmax, nl.ljs, nl.comps, nl.cw, nl.xb[0], nl.xb[1], nl.xq, nl.ecount, nl.efar,
                     nl.ent, nl.ecode, e.fpair, e.epair, e.fintra, e.eintra, e.fb, e.eb, e.fpme, e.epme, e.ephi};
     for (void* p : ptrs) if (p) (void)hipFree(p);
 }
@@ -756,7 +779,11 @@ void nlist_build(Engine& e, const uint4* w, bool checked) {
         a.R2 = a.R * a.R;
         a.col_cl_start = col_cl_start; a.catoms = nl.catoms; a.comps = nl.comps; a.bmask = e.bmask; a.cw = nl.cw;
         a.xb[0] = nl.xb[0]; a.xb[1] = nl.xb[1]; a.rinv[0] = nl.rinv[0]; a.rinv[1] = nl.rinv[1];
-        a.cref = nl.cref; a.cmin = nl.cmin; a.cmax = nl.cmax; a.ecount = nl.ecount; a.flags = e.dflags;
+        a.cref = nl.cref; a.cmin = nl.cmin; a.cmax = nl.cmax; a.ecount = nl.ecount; a.efar = nl.efar; a.flags = e.dflags;
+        {
+            const long long F = llround((double)e.rc * 1e8);
+            a.F2 = F * F;
+        }
         const int64_t need = (int64_t)nl.ncl_max * nl.cap;
         if (need > nl.nent_cap) {
             HIP_CHECK(hipStreamSynchronize(s));   // kernels of earlier steps may still read the old buffer
@@ -770,10 +797,7 @@ void nlist_build(Engine& e, const uint4* w, bool checked) {
             e.step_graph_valid = false;   // the captured pair kernel names the old entry buffer
         }
         a.cap = nl.cap; a.ent = nl.ent; a.ecode = nl.ecode;
-        // 32-bit coordinate differences are exact when the box, the list radius and a column's
-        // width stay below 2^31 Coord units together (boxes up to ~10 nm)
-        const long long Lmax = std::max(a.Lq[0], std::max(a.Lq[1], a.Lq[2]));
-        if (2 * Lmax + 2 * a.R < (1ll << 31)) hipLaunchKernelGGL(k_nlist<true>, dim3(nblk(nl.ncl_max, 4)), dim3(256), 0, s, a);
+        if (narrow_coords(e)) hipLaunchKernelGGL(k_nlist<true>, dim3(nblk(nl.ncl_max, 4)), dim3(256), 0, s, a);
         else hipLaunchKernelGGL(k_nlist<false>, dim3(nblk(nl.ncl_max, 4)), dim3(256), 0, s, a);
         if (!checked) {
             // the status words go to pinned host memory behind an event; issue_steps reads them two

@@ -153,12 +153,11 @@ def test_rollback_after_pair_list_overflow_is_exact():
     bit, that of an engine whose capacity never overflows."""
     box = L.gen_water_lattice(10, 3.104, 0, 600.0)
     probe = L.Engine(box, L.default_params(dt_fs=2.0, nlist_interval=10))
-    _, off, _, _ = probe.neighbor_list()
+    need = probe.status().list_max_entries     # slots the fullest cluster of the first list uses
     del probe
-    # the largest entry count of a cluster as the device stores it (padded to whole passes of 8)
-    need = int(((np.diff(off) + 7) // 8 * 8).max())
-    ref = L.Engine(box, L.default_params(dt_fs=2.0, nlist_interval=10))
+    ref = L.Engine(box, L.default_params(dt_fs=2.0, nlist_interval=10, nlist_capacity=4 * need))
     tight = L.Engine(box, L.default_params(dt_fs=2.0, nlist_interval=10, nlist_capacity=need))
+    assert tight.status().list_capacity == need
     steps = 0
     while tight.status().capacity_rollbacks == 0 and steps < 400:
         tight.step(20)
