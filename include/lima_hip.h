@@ -349,6 +349,16 @@ lima_status lima_mlp_get_params(lima_mlp* model, float* params);
 lima_status lima_mlp_set_params(lima_mlp* model, const float* params);
 /* s for n feature rows [n][32] (BF16 MFMA inference, FP32 accumulation) */
 lima_status lima_mlp_forward(lima_mlp* model, const float* features, int64_t n, float* scale_out);
+/* Allocation-free variant of lima_mlp_forward.  No counterpart in LIMA: it is for callers that
+ * cannot allocate device memory (code under stream capture, tight serving loops).  The features and
+ * scales are staged in a fixed device array of LIMA_MLP_NOALLOC_ROWS rows (8.65 MB per GPU, held until
+ * the process ends) instead of buffers allocated per call; n > LIMA_MLP_NOALLOC_ROWS returns
+ * LIMA_ERR_BAD_INPUT.  The array is shared by every call on a GPU and keeps the last call's contents;
+ * the call holds a process-wide lock until its results are on the host, so calls from several host
+ * threads run one after another (calls that overlapped on the array would race).  Results equal
+ * lima_mlp_forward's bit for bit. */
+#define LIMA_MLP_NOALLOC_ROWS 65536
+lima_status lima_mlp_forward_noalloc(lima_mlp* model, const float* features, int64_t n, float* scale_out);
 /* Adam on 1/2 (s - t)^2, BF16 MFMA forward and backward with FP32 accumulation and master weights;
  * batch a multiple of 16.  loss_per_epoch[epochs] (may be NULL), *seconds (may be NULL). */
 lima_status lima_mlp_train(lima_mlp* model, const float* features, const float* targets, int64_t n, int32_t epochs,

@@ -126,6 +126,7 @@ def load_hip():
             "lima_mlp_get_params": ([vp, C.POINTER(C.c_float)], C.c_int),
             "lima_mlp_set_params": ([vp, C.POINTER(C.c_float)], C.c_int),
             "lima_mlp_forward": ([vp, C.POINTER(C.c_float), C.c_int64, C.POINTER(C.c_float)], C.c_int),
+            "lima_mlp_forward_noalloc": ([vp, C.POINTER(C.c_float), C.c_int64, C.POINTER(C.c_float)], C.c_int),
             "lima_mlp_gradient": ([vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int64, C.POINTER(C.c_float),
                                    C.POINTER(C.c_double)], C.c_int),
             "lima_mlp_train": ([vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int64, C.c_int32, C.c_int32, C.c_float,
@@ -567,6 +568,8 @@ class Batch:
 class Mlp:
     """Handle of a lima_mlp: the per-atom step-scale model (BF16 MFMA inference and training)."""
 
+    NOALLOC_ROWS = 65536        # LIMA_MLP_NOALLOC_ROWS
+
     def __init__(self, seed=0):
         self.lib = load_hip()
         h = C.c_void_p()
@@ -603,6 +606,15 @@ class Mlp:
         out = np.zeros(len(x), np.float32)
         fp = C.POINTER(C.c_float)
         self._check(self.lib.lima_mlp_forward(self._h, x.ctypes.data_as(fp), len(x), out.ctypes.data_as(fp)))
+        return out
+
+    def forward_noalloc(self, x):
+        """lima_mlp_forward_noalloc: the same scales, staged in the library's fixed per-GPU array
+        (at most NOALLOC_ROWS rows) instead of memory allocated per call."""
+        x = np.ascontiguousarray(x, np.float32)
+        out = np.zeros(len(x), np.float32)
+        fp = C.POINTER(C.c_float)
+        self._check(self.lib.lima_mlp_forward_noalloc(self._h, x.ctypes.data_as(fp), len(x), out.ctypes.data_as(fp)))
         return out
 
     def gradient(self, x, t):

@@ -17,6 +17,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -231,6 +232,12 @@ struct lima_mlp {
     lima::Mlp m;
 };
 
+// Staging array of lima_mlp_forward_noalloc: feature rows [LIMA_MLP_NOALLOC_ROWS][32], then the
+// step scales [LIMA_MLP_NOALLOC_ROWS] (8.65 MB).  Every GPU the library runs on holds its own copy
+// until the process ends.  The name is a host-side stand-in: host code uses the per-device address
+// that scratch_address() obtains, never the name.
+static __device__ __align__(256) float scratch[(size_t)LIMA_MLP_NOALLOC_ROWS * (lima::MLP_IN + 1)];
+
 namespace {
 template <class F>
 lima_status mguard(F&& f) {
@@ -247,6 +254,24 @@ lima_status mguard(F&& f) {
         lima::g_err = x.what();
         return LIMA_ERR_BAD_INPUT;
     }
+}
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;                            // guards g_scratch and every use of the array
+float* g_scratch[SCRATCH_MAX_DEVICES] = {};         // device address of `scratch`, per device
+
+// Address of the current device's copy of `scratch`, looked up once per device with that device
+// current.  Call with g_scratch_mu held.
+float* scratch_address() {
+    int dev = -1;
+    HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) throw lima::LimaFail(LIMA_ERR_NO_DEVICE, "device index out of range");
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) throw lima::LimaFail(LIMA_ERR_HIP, "hipGetSymbolAddress returned a null address");
+        g_scratch[dev] = static_cast<float*>(p);
+    }
+    return g_scratch[dev];
 }
 void mlp_free(lima::Mlp& M) {
     void* ptrs[] = {M.p, M.m, M.v, M.w1t, M.w2t, M.w2};
@@ -333,6 +358,30 @@ lima_status lima_mlp_forward(lima_mlp* h, const float* features, int64_t n, floa
         (void)hipFree(dx);
         (void)hipFree(dy);
         HIP_CHECK(e);
+    });
+}
+
+lima_status lima_mlp_forward_noalloc(lima_mlp* h, const float* features, int64_t n, float* scale_out) {
+    if (!h || !features || !scale_out || n < 0) { lima::g_err = "bad argument"; return LIMA_ERR_BAD_INPUT; }
+    if (n > LIMA_MLP_NOALLOC_ROWS) {
+        lima::g_err = "lima_mlp_forward_noalloc: " + std::to_string(n) + " rows exceed the static staging array of " +
+                      std::to_string(LIMA_MLP_NOALLOC_ROWS) + " rows; use lima_mlp_forward or split the call";
+        return LIMA_ERR_BAD_INPUT;
+    }
+    if (n == 0) return LIMA_OK;
+    return mguard([&] {
+        using namespace lima;
+        // Held until the results are on the host: calls from several host threads run one after
+        // another instead of sharing the array at once.
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        float* dx = scratch_address();
+        float* dy = dx + (size_t)LIMA_MLP_NOALLOC_ROWS * MLP_IN;
+        // every row the kernel reads (< n) is written here and every scale it writes is copied back:
+        // nothing of an earlier call is read, so nothing needs clearing
+        HIP_CHECK(hipMemcpy(dx, features, n * MLP_IN * sizeof(float), hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_mlp_forward, dim3(nblk(n, 16)), dim3(64), 0, 0, h->m.view(), dx, n, dy);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipMemcpy(scale_out, dy, n * sizeof(float), hipMemcpyDeviceToHost));
     });
 }
 
