@@ -141,6 +141,10 @@ typedef struct lima_energy {
     double potential_lima;/* sum of LIMA's per-particle potE (uses pme_self_lima) */
     double kinetic;
     double total;         /* potential + kinetic */
+    double virial[6];     /* potential virial W = -dU/d(strain) = sum over pairs of r_ij (x) F_ij, plus the
+                             bonded terms' sum_a (x_a - x_ref) (x) F_a and the PME reciprocal-space sum;
+                             components xx, yy, zz, xy, xz, yz [J/mol].  Pressure tensor
+                             P = (sum_i m_i v_i (x) v_i + W) / V.  No counterpart in LIMA. */
     int64_t step;         /* MD step whose evaluation produced these totals (lima_compute_forces: the
                              current step; lima_minimize: the current step after minimising); -1
                              before any energy evaluation */

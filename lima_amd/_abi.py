@@ -64,10 +64,18 @@ class SimParams(C.Structure):
 class Energy(C.Structure):
     _fields_ = [(n, C.c_double) for n in (
         "lj", "coulomb_sr", "bonded", "pme_recip", "pme_self_lima", "pme_self_ewald",
-        "potential", "potential_lima", "kinetic", "total")] + [("step", C.c_int64)]
+        "potential", "potential_lima", "kinetic", "total")] + [("virial", C.c_double * 6), ("step", C.c_int64)]
+
+    VIRIAL_COMPONENTS = ("xx", "yy", "zz", "xy", "xz", "yz")
 
     def as_dict(self):
-        return {n: getattr(self, n) for n, _ in self._fields_}
+        return {n: (list(getattr(self, n)) if n == "virial" else getattr(self, n)) for n, _ in self._fields_}
+
+    def virial_tensor(self):
+        """W as a symmetric 3x3 numpy array [J/mol]."""
+        import numpy as np
+        v = list(self.virial)
+        return np.array([[v[0], v[3], v[4]], [v[3], v[1], v[5]], [v[4], v[5], v[2]]])
 
 
 class RunStatus(C.Structure):

@@ -111,7 +111,9 @@ constexpr uint32_t LANE_PAD = 0xffffffffu;
 // Device accumulators (FP64 energy reductions and scalars).
 constexpr int LAUNCH_EAGER = LIMA_LAUNCH_EAGER, LAUNCH_ONE_STREAM = LIMA_LAUNCH_ONE_STREAM;
 
-enum Acc { ACC_LJ = 0, ACC_COUL, ACC_BOND, ACC_PME, ACC_KIN, ACC_PAIRS, ACC_N };
+// ACC_VXX .. ACC_VYZ: the virial W = -dU/d(strain), components xx yy zz xy xz yz [J/mol]
+enum Acc { ACC_LJ = 0, ACC_COUL, ACC_BOND, ACC_PME, ACC_KIN, ACC_PAIRS, ACC_VXX, ACC_VYY, ACC_VZZ, ACC_VXY, ACC_VXZ, ACC_VYZ,
+           ACC_N };
 constexpr int REDUCE_BLOCKS = 64;   // first stage of the energy reduction
 
 // FP64 energy reduction without same-address atomics: every wave of an energy-producing kernel
@@ -120,6 +122,15 @@ constexpr int REDUCE_BLOCKS = 64;   // first stage of the energy reduction
 struct ERegions {
     int pair = 0, intra = 0, bond = 0, pme = 0, kin = 0, total = 0;
 };
+
+// One energy record: a kernel's sums, zero in the fields it does not produce (vir: six virial
+// components or nullptr).
+__device__ __forceinline__ void write_record(double* r, double lj, double coul, double bond, double kin, double pairs,
+                                             const double* vir) {
+    r[ACC_LJ] = lj; r[ACC_COUL] = coul; r[ACC_BOND] = bond; r[ACC_PME] = 0.0; r[ACC_KIN] = kin; r[ACC_PAIRS] = pairs;
+#pragma unroll
+    for (int k = 0; k < 6; k++) r[ACC_VXX + k] = vir ? vir[k] : 0.0;
+}
 
 // Sticky status words on the device (Engine::dflags).  Kernels only set them; the host clears
 // them after it has dealt with the condition.  Every list rebuild whose error words are still
@@ -329,6 +340,8 @@ void launch_local(Engine& e, const uint4* w, const int4* img, bool energy, hipSt
 void pme_alloc(Engine& e, const std::vector<float>& charges_global, const std::vector<uint4>& w);
 void pme_free(Engine& e);
 void pme_grow_bins(Engine& e);       // tile-bin capacity x 2 (the step graph is captured again)
+int pme_zpass_lines(const Pme& m);             // kx lines per workgroup of the PME z pass
+int pme_virial_records(const Engine& e);       // energy records of the z pass (one per workgroup)
 void pme_clear_bins(Engine& e, hipStream_t s);
 void launch_pme(Engine& e, const uint4* w, bool energy, hipStream_t s);
 

@@ -149,7 +149,7 @@ __global__ void k_integrate(IntegrateArgs a) {
         const double t = wave_sum_d((double)ke);
         if ((threadIdx.x & 63) == 0 && wave_in) {
             double* r = a.epart + wv * ACC_N;
-            r[ACC_LJ] = 0.0; r[ACC_COUL] = 0.0; r[ACC_BOND] = 0.0; r[ACC_PME] = 0.0; r[ACC_KIN] = t; r[ACC_PAIRS] = 0.0;
+            write_record(r, 0.0, 0.0, 0.0, t, 0.0, nullptr);
         }
     }
 }
@@ -164,7 +164,7 @@ __global__ void k_kinetic(int64_t N, const float4* __restrict__ v, const float4*
     const double t = wave_sum_d((double)ke);
     if ((threadIdx.x & 63) == 0 && (p & ~63ll) < N) {
         double* r = epart + (p >> 6) * ACC_N;
-        r[ACC_LJ] = 0.0; r[ACC_COUL] = 0.0; r[ACC_BOND] = 0.0; r[ACC_PME] = 0.0; r[ACC_KIN] = t; r[ACC_PAIRS] = 0.0;
+        write_record(r, 0.0, 0.0, 0.0, t, 0.0, nullptr);
     }
 }
 
@@ -425,6 +425,7 @@ void energy_from_acc(const Engine& e, const double* acc, bool with_kinetic, lima
     E.potential_lima = E.lj + E.coulomb_sr + E.bonded + E.pme_recip + E.pme_self_lima;
     if (with_kinetic) E.kinetic = acc[ACC_KIN];
     E.total = E.potential + E.kinetic;
+    for (int k = 0; k < 6; k++) E.virial[k] = acc[ACC_VXX + k];
 }
 
 // Reads the totals of the last energy evaluation (e.acc) into e.energy; the device must be idle.

@@ -140,6 +140,7 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs a) {
     const float qi_kc = xi.w * a.pc.kc;
     float fx = 0.f, fy = 0.f, fz = 0.f;
     double elj = 0.0, ecl = 0.0;   // FP64 energy accumulation (energy steps only)
+    double vir[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};   // -1/2 S d (x) d per evaluated pair (energy steps only)
     int npairs = 0;
     const int nn = live ? a.ecount[I] : 0, nf = live ? a.efar[I] : 0;
     const int ntot = (nn + nf) / CLUSTER;
@@ -194,6 +195,9 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs a) {
                 if (ENERGY) {
                     elj += 0.5 * (double)el;   // each pair is evaluated from both sides: half per side
                     ecl += 0.5 * (double)ec;
+                    const float hx = -0.5f * S * dx, hy = -0.5f * S * dy, hz = -0.5f * S * dz;
+                    vir[0] += (double)(hx * dx); vir[1] += (double)(hy * dy); vir[2] += (double)(hz * dz);
+                    vir[3] += (double)(hx * dy); vir[4] += (double)(hx * dz); vir[5] += (double)(hy * dz);
                 }
             }
         }
@@ -238,11 +242,10 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs a) {
     }
     if (ENERGY) {
         const double tl = wave_sum_d(elj), tc = wave_sum_d(ecl), np = wave_sum_d((double)npairs);
-        if (lane == 0 && wv < a.ncl_max * a.split) {   // waves past the live count write zero records
-            double* r = a.epart + (int64_t)wv * ACC_N;
-            r[ACC_LJ] = tl; r[ACC_COUL] = tc; r[ACC_BOND] = 0.0; r[ACC_PME] = 0.0; r[ACC_KIN] = 0.0;
-            r[ACC_PAIRS] = np * 0.5;   // both sides counted
-        }
+#pragma unroll
+        for (int k = 0; k < 6; k++) vir[k] = wave_sum_d(vir[k]);
+        if (lane == 0 && wv < a.ncl_max * a.split)   // waves past the live count write zero records
+            write_record(a.epart + (int64_t)wv * ACC_N, tl, tc, 0.0, 0.0, np * 0.5, vir);   // pairs: both sides counted
     }
 }
 
@@ -294,6 +297,7 @@ __device__ __forceinline__ void intra_items(const IntraArgs& a, int blk, IntraLd
     float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
     if (i < nA) { unwrapped3(a.w, a.img, a.Lq, ai, ui); pi = a.prm[ai]; }
     float fx = 0.f, fy = 0.f, fz = 0.f, el_s = 0.f, ec_s = 0.f;
+    float vir[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};   // -1/2 S d (x) d per evaluated pair (energy evaluations)
     for (int it = grp.x; it < grp.x + grp.y; it++) {
         const int4 item = a.items[it];
         const int B = item.y;
@@ -344,7 +348,12 @@ __device__ __forceinline__ void intra_items(const IntraArgs& a, int blk, IntraLd
                 const float S = pair_scalar_cut(r2, pi.y + pj.y, 24.0f * (pi.z * pj.z), a.pc.kc * (pi.x * pj.x), a.pc,
                                                 ENERGY ? &el : nullptr, &ec);
                 fx += dx * S; fy += dy * S; fz += dz * S;
-                if (ENERGY) { el_s += 0.5f * el; ec_s += 0.5f * ec; }
+                if (ENERGY) {
+                    el_s += 0.5f * el; ec_s += 0.5f * ec;
+                    const float hx = -0.5f * S * dx, hy = -0.5f * S * dy, hz = -0.5f * S * dz;
+                    vir[0] += hx * dx; vir[1] += hy * dy; vir[2] += hz * dz;
+                    vir[3] += hx * dy; vir[4] += hx * dz; vir[5] += hy * dz;
+                }
             }
         }
     }
@@ -362,10 +371,10 @@ __device__ __forceinline__ void intra_items(const IntraArgs& a, int blk, IntraLd
     }
     if (ENERGY) {
         const double tl = wave_sum_d((double)el_s), tc = wave_sum_d((double)ec_s);
-        if (lane == 0) {
-            double* r = a.epart + (int64_t)g * ACC_N;
-            r[ACC_LJ] = tl; r[ACC_COUL] = tc; r[ACC_BOND] = 0.0; r[ACC_PME] = 0.0; r[ACC_KIN] = 0.0; r[ACC_PAIRS] = 0.0;
-        }
+        double vd[6];
+#pragma unroll
+        for (int k = 0; k < 6; k++) vd[k] = wave_sum_d((double)vir[k]);
+        if (lane == 0) write_record(a.epart + (int64_t)g * ACC_N, tl, tc, 0.0, 0.0, 0.0, vd);
     }
 }
 
@@ -424,43 +433,56 @@ __device__ __forceinline__ void bonded_lanes(const BondArgs& a, int blk) {
     const int last = (la >> 24) & 63, atom = la & 0xffffff;
     v3 f = {0.f, 0.f, 0.f};
     float e = 0.f, eshare = 0.f;
+    float vir[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};   // (x_atom - x_ref) (x) f, symmetric part (energy evaluations)
     int role = -1;
     if (code != LANE_PAD) {
         const int type = (int)(code >> 29), t = (int)(code & (INC_MAX_TERMS - 1));
         role = (int)((code >> 27) & 3u);
-        int nat;
+        int nat, ref;   // ref: the term's role-0 particle, the origin of its virial positions
         if (type == T_BOND) {   // calcSinglebondForces :17-51
             const int2 id = a.bid[t];
+            ref = id.x;
             v3 fa;
             e = term_bond(bdispq(a, id.y, id.x), a.bp[t], &fa);
             f = role == 0 ? fa : -fa;
             nat = 2;
         } else if (type == T_ANGLE) {   // calcAnglebondForces :53-115, harmonic in theta + Urey-Bradley
             const int4 id = a.aid[t];
+            ref = id.x;
             v3 fl, fr;
             e = term_angle(bdisp(a, id.y, id.x), bdisp(a, id.y, id.z), bdispq(a, id.z, id.x), a.ap[t], &fl, &fr);
             f = role == 0 ? fl : (role == 2 ? fr : -(fl + fr));
             nat = 3;
         } else if (type == T_DIH) {   // calcDihedralbondForces :119-210, phi_L = -atan2(sin, cos)
             const int4 id = a.did[t];
+            ref = id.x;
             v3 fd[4];
             e = term_dihedral(bdisp(a, id.x, id.y), bdisp(a, id.y, id.z), bdisp(a, id.z, id.w), a.dp[t], fd);
             f = pick_role4(fd, role);
             nat = 4;
         } else if (type == T_IMP) {   // calcImproperdihedralbondForces :214-264
             const int4 id = a.iid[t];
+            ref = id.x;
             v3 fd[4];
             e = term_improper(bdisp(a, id.x, id.y), bdisp(a, id.x, id.z), bdisp(a, id.x, id.w), a.ip[t], fd);
             f = pick_role4(fd, role);
             nat = 4;
         } else {   // computePairbondForces :335-376
             const int2 id = a.pid[t];
+            ref = id.x;
             v3 fa;
             e = term_pair14(bdisp(a, id.x, id.y), a.pp[t], &fa);
             f = role == 0 ? fa : -fa;
             nat = 2;
         }
         eshare = e / (float)nat;
+        if (ENERGY && role != 0) {   // the role-0 lane sits at the origin
+            const v3 r = bdisp(a, ref, atom);
+            vir[0] = r.x * f.x; vir[1] = r.y * f.y; vir[2] = r.z * f.z;
+            vir[3] = 0.5f * (r.x * f.y + r.y * f.x);
+            vir[4] = 0.5f * (r.x * f.z + r.z * f.x);
+            vir[5] = 0.5f * (r.y * f.z + r.z * f.y);
+        }
     }
     // run sums into the run's first lane: after the step with offset o, lane i holds the sum of
     // lanes i .. min(i + 2o - 1, last)
@@ -498,10 +520,11 @@ __device__ __forceinline__ void bonded_lanes(const BondArgs& a, int blk) {
     }
     if (ENERGY) {
         const double tot = wave_sum_d(role == 0 ? (double)e : 0.0);
-        if (lane == 0 && (k & ~63) < a.n_lanes) {   // waves inside the region
-            double* r = a.epart + (int64_t)(k >> 6) * ACC_N;
-            r[ACC_LJ] = 0.0; r[ACC_COUL] = 0.0; r[ACC_BOND] = tot; r[ACC_PME] = 0.0; r[ACC_KIN] = 0.0; r[ACC_PAIRS] = 0.0;
-        }
+        double vd[6];
+#pragma unroll
+        for (int c = 0; c < 6; c++) vd[c] = wave_sum_d((double)vir[c]);
+        if (lane == 0 && (k & ~63) < a.n_lanes)   // waves inside the region
+            write_record(a.epart + (int64_t)(k >> 6) * ACC_N, 0.0, 0.0, tot, 0.0, 0.0, vd);
     }
 }
 
@@ -574,7 +597,7 @@ ERegions energy_regions(const Engine& e) {
     r.intra = r.pair + e.nl.ncl_max * pair_split(e);
     r.bond = r.intra + e.n_intra_groups;
     r.pme = r.bond + e.bd.n_lanes / 64;   // one record per wave of the bonded incidence lanes
-    r.kin = r.pme;                        // PME energy is summed from the per-slot ephi array instead
+    r.kin = r.pme + pme_virial_records(e);   // PME: the z pass's virial records (its energy comes from ephi)
     r.total = r.kin + (int)((e.N + 63) / 64);
     return r;
 }

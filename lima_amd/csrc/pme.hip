@@ -323,10 +323,24 @@ __global__ __launch_bounds__(256) void k_fft_x_inv(FftPlan P, int Ky, int H, int
     }
 }
 
+// Reciprocal-space virial, taken in the z pass of energy evaluations (rec != nullptr).  With
+// E_rec = V_cell / (2 NG) sum_m G(m) |rho^(m)|^2 (the gather's 1/2 sum q phi, by Parseval) and the
+// spline weights and sinc(pi m / K) fixed under a strain of the box, each wave vector adds
+// W_ab = E_m (delta_ab - 2 (1 / k^2 + 1 / (4 kappa^2)) k_a k_b); the half-complex grid holds the
+// x frequencies 1 .. (Kx - 1) / 2 for their conjugates too (weight 2).
+struct PmeVir {
+    float kf[3];     // 2 PI_L / L_d
+    int K[3];
+    float inv4k2;    // 1 / (4 kappa^2)
+    float efac;      // V_cell / (2 NG)
+    double* rec;     // one record per workgroup of the z pass, or nullptr
+};
+
 // Strided pass over lines (n * stride) for TK consecutive kx of one outer index.
 // mode 0: forward, 1: inverse, 2: forward * green, inverse (z pass).
 __global__ __launch_bounds__(256) void k_fft_strided(FftPlan P, int H, int TK, int64_t outer_stride, int64_t stride, int mode,
-                                                     int green_y_count, float2* __restrict__ cg, const float* __restrict__ green) {
+                                                     int green_y_count, float2* __restrict__ cg, const float* __restrict__ green,
+                                                     PmeVir v) {
     extern __shared__ float2 lds[];
     const int N = P.N;
     const int o = blockIdx.y, k0 = blockIdx.x * TK;
@@ -354,11 +368,45 @@ __global__ __launch_bounds__(256) void k_fft_strided(FftPlan P, int H, int TK, i
     __syncthreads();
     float2* r = fft_lines(a, b, nl, P, mode == 1);
     if (mode == 2) {
+        float w[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        const bool vir = v.rec != nullptr;   // kernel-uniform
         for (int i = threadIdx.x; i < nl * N; i += blockDim.x) {
             const int t = i / N, n = i - t * N;
             // z pass: line index n is z, outer o is y
             const float g = green[((int64_t)n * green_y_count + o) * H + k0 + t];
+            if (vir) {
+                const int kx = k0 + t;
+                const int my = 2 * o <= v.K[1] ? o : o - v.K[1], mz = 2 * n <= v.K[2] ? n : n - v.K[2];
+                const float ka = v.kf[0] * (float)kx, kb = v.kf[1] * (float)my, kc = v.kf[2] * (float)mz;
+                const float k2 = ka * ka + kb * kb + kc * kc;
+                if (k2 > 0.f) {
+                    const float mult = (kx == 0 || 2 * kx == v.K[0]) ? 1.f : 2.f;
+                    const float Em = v.efac * mult * g * (r[i].x * r[i].x + r[i].y * r[i].y);
+                    const float c = 2.f * (1.f / k2 + v.inv4k2);
+                    w[0] += Em * (1.f - c * ka * ka);
+                    w[1] += Em * (1.f - c * kb * kb);
+                    w[2] += Em * (1.f - c * kc * kc);
+                    w[3] -= Em * c * ka * kb;
+                    w[4] -= Em * c * ka * kc;
+                    w[5] -= Em * c * kb * kc;
+                }
+            }
             r[i] = cscale(r[i], g);
+        }
+        if (vir) {   // FP64 workgroup sums in a fixed order, one record per workgroup
+            __shared__ double red[6][4];
+            const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+            for (int c = 0; c < 6; c++) {
+                const double sw = wave_sum_d((double)w[c]);
+                if (lane == 0) red[c][wid] = sw;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                double vd[6];
+                for (int c = 0; c < 6; c++) vd[c] = ((red[c][0] + red[c][1]) + red[c][2]) + red[c][3];
+                write_record(v.rec + ((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * ACC_N, 0.0, 0.0, 0.0, 0.0, 0.0, vd);
+            }
         }
         __syncthreads();
         float2* other = (r == a) ? b : a;
@@ -770,6 +818,13 @@ void pme_grow_bins(Engine& e) {
     e.step_graph_valid = false;   // bin arguments are part of the captured kernels
 }
 
+int pme_zpass_lines(const Pme& m) { return std::max(1, std::min(8, 3072 / m.K[2])); }
+
+int pme_virial_records(const Engine& e) {
+    if (!e.elec || e.pme.ncharged == 0) return 0;
+    return (int)nblk(e.pme.H, pme_zpass_lines(e.pme)) * e.pme.K[1];
+}
+
 void pme_clear_bins(Engine& e, hipStream_t s) {
     if (e.pme.tiled && e.pme.bin_count) HIP_CHECK(hipMemsetAsync(e.pme.bin_count, 0, (e.pme.ntiles + 1) * sizeof(int), s));
 }
@@ -800,15 +855,24 @@ void launch_pme(Engine& e, const uint4* w, bool energy, hipStream_t s) {
         hipLaunchKernelGGL(k_fft_x_fwd, dim3(nblk(Ky, TY), Kz), dim3(256), 2 * TY * Kx * sizeof(float2), s, Px, Ky, H, TY,
                            (float)(1.0 / vcell), m.qgrid, m.cgrid);
     }
-    const int TKy = std::max(1, std::min(8, 3072 / Ky)), TKz = std::max(1, std::min(8, 3072 / Kz));
+    const int TKy = std::max(1, std::min(8, 3072 / Ky)), TKz = pme_zpass_lines(m);
+    PmeVir none{};
+    PmeVir vir{};
+    if (energy) {
+        const double PI_L = (double)LIMA_PI_F;
+        for (int d = 0; d < 3; d++) { vir.kf[d] = (float)(2.0 * PI_L / e.L[d]); vir.K[d] = m.K[d]; }
+        vir.inv4k2 = (float)(1.0 / (4.0 * (double)e.pc.kappa * (double)e.pc.kappa));
+        vir.efac = (float)(vcell / (2.0 * (double)NG));
+        vir.rec = e.epart + (int64_t)e.ereg.pme * ACC_N;
+    }
     // y pass: outer z, lines along y (stride H)
     hipLaunchKernelGGL(k_fft_strided, dim3(nblk(H, TKy), Kz), dim3(256), 2 * TKy * Ky * sizeof(float2), s, Py, H, TKy,
-                       (int64_t)Ky * H, (int64_t)H, 0, Ky, m.cgrid, m.green);
+                       (int64_t)Ky * H, (int64_t)H, 0, Ky, m.cgrid, m.green, none);
     // z pass with the influence function: outer y, lines along z (stride Ky*H)
     hipLaunchKernelGGL(k_fft_strided, dim3(nblk(H, TKz), Ky), dim3(256), 2 * TKz * Kz * sizeof(float2), s, Pz, H, TKz,
-                       (int64_t)H, (int64_t)Ky * H, 2, Ky, m.cgrid, m.green);
+                       (int64_t)H, (int64_t)Ky * H, 2, Ky, m.cgrid, m.green, vir);
     hipLaunchKernelGGL(k_fft_strided, dim3(nblk(H, TKy), Kz), dim3(256), 2 * TKy * Ky * sizeof(float2), s, Py, H, TKy,
-                       (int64_t)Ky * H, (int64_t)H, 1, Ky, m.cgrid, m.green);
+                       (int64_t)Ky * H, (int64_t)H, 1, Ky, m.cgrid, m.green, none);
     hipLaunchKernelGGL(k_fft_x_inv, dim3(nblk(Ky, TY), Kz), dim3(256), 2 * TY * Kx * sizeof(float2), s, Px, Ky, H, TY,
                        (float)(1.0 / (double)NG), m.cgrid, m.phi);
     if (m.tiled) {

@@ -156,6 +156,18 @@ inline Vec pair_force(const Vec& diff, double sig, double eps, double qq, double
     return f;
 }
 
+/* Symmetric part of scale * r (x) f added to a virial tensor W (xx yy zz xy xz yz).  The virial
+ * is W = -dU/d(strain) = sum_i x_i (x) F_i for a finite system; for pair terms evaluated from
+ * both sides each ordered pair adds 1/2 (x_i - x_j) (x) F_i = -1/2 d (x) F_i with d = x_j - x_i. */
+inline void add_virial(double* W, const Vec& r, const Vec& f, double scale) {
+    W[0] += scale * r.x * f.x;
+    W[1] += scale * r.y * f.y;
+    W[2] += scale * r.z * f.z;
+    W[3] += scale * 0.5 * (r.x * f.y + r.y * f.x);
+    W[4] += scale * 0.5 * (r.x * f.z + r.z * f.x);
+    W[5] += scale * 0.5 * (r.y * f.z + r.z * f.y);
+}
+
 /* ---- mixed-radix complex FFT in double (radix 2,3,4,5, generic odd prime by DFT) ---- */
 using cd = std::complex<double>;
 
@@ -290,7 +302,7 @@ Vec bdisp(const lima_oracle* o, int64_t a, int64_t b) {  // nearest image of x_b
  * includes r == rc).  LIMA evaluates each ordered pair from the self side
  * (EngineKernels.cuh:96-139); the oracle does the same: force on i, half energy on i.
  * ------------------------------------------------------------------------------------- */
-void nb_cutoff(lima_oracle* o, std::vector<double>& F, std::vector<double>& pot, double& elj, double& ec) {
+void nb_cutoff(lima_oracle* o, std::vector<double>& F, std::vector<double>& pot, double& elj, double& ec, double* W) {
     const int64_t N = o->N;
     int nc[3];
     for (int d = 0; d < 3; d++) { nc[d] = std::max(1, (int)std::floor(o->L[d] / o->rc)); }
@@ -308,8 +320,8 @@ void nb_cutoff(lima_oracle* o, std::vector<double>& F, std::vector<double>& pot,
         for (int64_t i = 0; i < N; i++) order[fill[cell[i]]++] = i;
     }
     const double rc2 = o->rc * o->rc;
-    double elj_t = 0, ec_t = 0;
-#pragma omp parallel for schedule(dynamic, 64) reduction(+ : elj_t, ec_t) num_threads(g_threads > 0 ? g_threads : omp_get_max_threads())
+    double elj_t = 0, ec_t = 0, w_t[6] = {0, 0, 0, 0, 0, 0};
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : elj_t, ec_t, w_t[:6]) num_threads(g_threads > 0 ? g_threads : omp_get_max_threads())
     for (int64_t i = 0; i < N; i++) {
         int64_t ci[3] = {cell[i] % nc[0], (cell[i] / nc[0]) % nc[1], cell[i] / ((int64_t)nc[0] * nc[1])};
         // candidate (cell index, shift) per dim
@@ -340,7 +352,9 @@ void nb_cutoff(lima_oracle* o, std::vector<double>& F, std::vector<double>& pot,
                 if (d.dot(d) > rc2) continue;
                 const double sg = (o->sig[i] + o->sig[j]) * 0.5, ep = std::sqrt(o->eps[i] * o->eps[j]);
                 double a = 0, b = 0;
-                fi += pair_force(d, sg, ep, o->q[i] * o->q[j], o->kappa, a, b);
+                const Vec f = pair_force(d, sg, ep, o->q[i] * o->q[j], o->kappa, a, b);
+                fi += f;
+                add_virial(w_t, d, f, -0.5);
                 el += a; e2 += b;
             }
         }
@@ -350,13 +364,14 @@ void nb_cutoff(lima_oracle* o, std::vector<double>& F, std::vector<double>& pot,
         elj_t += el; ec_t += e2;
     }
     elj += elj_t; ec += ec_t;
+    for (int k = 0; k < 6; k++) W[k] += w_t[k];
 }
 
 /* Intra / bonded-compound pairs, no cutoff, LUT exclusions, compound-frame displacement.
  * EngineKernels.cuh:197-263 with LennardJonesInteractions.cuh:121-161. */
-void nb_intra(lima_oracle* o, std::vector<double>& F, std::vector<double>& pot, double& elj, double& ec) {
-    double elj_t = 0, ec_t = 0;
-#pragma omp parallel for schedule(dynamic, 4) reduction(+ : elj_t, ec_t) num_threads(g_threads > 0 ? g_threads : omp_get_max_threads())
+void nb_intra(lima_oracle* o, std::vector<double>& F, std::vector<double>& pot, double& elj, double& ec, double* W) {
+    double elj_t = 0, ec_t = 0, w_t[6] = {0, 0, 0, 0, 0, 0};
+#pragma omp parallel for schedule(dynamic, 4) reduction(+ : elj_t, ec_t, w_t[:6]) num_threads(g_threads > 0 ? g_threads : omp_get_max_threads())
     for (int A = 0; A < o->nC; A++) {
         const int nA = (int)(o->first[A + 1] - o->first[A]);
         for (int k = 0; k < 5; k++) {
@@ -382,7 +397,9 @@ void nb_intra(lima_oracle* o, std::vector<double>& F, std::vector<double>& pot, 
                                 (double)(o->u[j * 3 + 1] - o->u[i * 3 + 1] + sh[1]) / NANO_TO_LIMA,
                                 (double)(o->u[j * 3 + 2] - o->u[i * 3 + 2] + sh[2]) / NANO_TO_LIMA);
                     const double sg = (o->sig[i] + o->sig[j]) * 0.5, ep = std::sqrt(o->eps[i] * o->eps[j]);
-                    fi += pair_force(d, sg, ep, o->q[i] * o->q[j], o->kappa, el, e2);
+                    const Vec f = pair_force(d, sg, ep, o->q[i] * o->q[j], o->kappa, el, e2);
+                    fi += f;
+                    add_virial(w_t, d, f, -0.5);
                 }
                 F[i * 3 + 0] += fi.x; F[i * 3 + 1] += fi.y; F[i * 3 + 2] += fi.z;
                 pot[i] += el + e2;
@@ -391,13 +408,15 @@ void nb_intra(lima_oracle* o, std::vector<double>& F, std::vector<double>& pot, 
         }
     }
     elj += elj_t; ec += ec_t;
+    for (int k = 0; k < 6; k++) W[k] += w_t[k];
 }
 
 inline void addF(std::vector<double>& F, int64_t i, const Vec& f) { F[i * 3 + 0] += f.x; F[i * 3 + 1] += f.y; F[i * 3 + 2] += f.z; }
 
 /* Bonded terms, ForceComputations.cuh.  Energies are split over the term's particles
  * (ForceComputations.cuh:317-325). */
-double bonded(lima_oracle* o, std::vector<double>& F, std::vector<double>& pot) {
+double bonded(lima_oracle* o, std::vector<double>& F, std::vector<double>& pot, double* W) {
+    // each term adds sum_a (x_a - x_a0) (x) F_a, positions taken relative to its first particle
     double etot = 0;
     // bonds, calcSinglebondForces :17-51 (MD form; SD uses the exact gradient)
     for (auto& t : o->bonds) {
@@ -406,6 +425,7 @@ double bonded(lima_oracle* o, std::vector<double>& F, std::vector<double>& pot) 
         const double e = 0.5 * t.p[1] * err * err;
         const Vec f = dif * (-t.p[1] * err / r);
         addF(F, t.a[0], f); addF(F, t.a[1], -f);
+        add_virial(W, -dif, -f, 1.0);
         pot[t.a[0]] += 0.5 * e; pot[t.a[1]] += 0.5 * e; etot += e;
     }
     // angles + Urey-Bradley, calcAnglebondForces :53-115 (harmonic in theta; exact gradient)
@@ -429,6 +449,8 @@ double bonded(lima_oracle* o, std::vector<double>& F, std::vector<double>& pot) 
             fl += fu; fr += -fu;
         }
         addF(F, t.a[0], fl); addF(F, t.a[2], fr); addF(F, t.a[1], -(fl + fr));
+        add_virial(W, bdisp(o, t.a[0], t.a[1]), -(fl + fr), 1.0);
+        add_virial(W, bdisp(o, t.a[0], t.a[2]), fr, 1.0);
         for (int k = 0; k < 3; k++) pot[t.a[k]] += e / 3.0;
         etot += e;
     }
@@ -451,6 +473,9 @@ double bonded(lima_oracle* o, std::vector<double>& F, std::vector<double>& pot) 
         // F = -dV/dphi_L * dphi_L/dx = +dV/dphi_L * dphi_bk/dx
         addF(F, t.a[0], g1 * dVdphi); addF(F, t.a[1], g2 * dVdphi);
         addF(F, t.a[2], g3 * dVdphi); addF(F, t.a[3], g4 * dVdphi);
+        add_virial(W, b1, g2 * dVdphi, 1.0);                   // x_1 - x_0 = b1
+        add_virial(W, b1 + b2, g3 * dVdphi, 1.0);
+        add_virial(W, b1 + b2 + b3, g4 * dVdphi, 1.0);
         for (int k = 0; k < 4; k++) pot[t.a[k]] += e / 4.0;
         etot += e;
     }
@@ -471,6 +496,9 @@ double bonded(lima_oracle* o, std::vector<double>& F, std::vector<double>& pot) 
         const Vec res = -(fi + fl);
         const double ratio = (pj - pi).len() / ((pj - pi).len() + (pk - pi).len());
         addF(F, t.a[0], fi); addF(F, t.a[1], res * (1.0 - ratio)); addF(F, t.a[2], res * ratio); addF(F, t.a[3], fl);
+        add_virial(W, pj, res * (1.0 - ratio), 1.0);
+        add_virial(W, pk, res * ratio, 1.0);
+        add_virial(W, pl, fl, 1.0);
         for (int k = 0; k < 4; k++) pot[t.a[k]] += e / 4.0;
         etot += e;
     }
@@ -480,6 +508,7 @@ double bonded(lima_oracle* o, std::vector<double>& F, std::vector<double>& pot) 
         double el = 0, ec = 0;
         const Vec f = pair_force(d, t.p[0], t.p[1], 0.0, o->kappa, el, ec);
         addF(F, t.a[0], f); addF(F, t.a[1], -f);
+        add_virial(W, d, -f, 1.0);
         pot[t.a[0]] += el; pot[t.a[1]] += el; etot += 2 * el;
     }
     return etot;
@@ -532,9 +561,10 @@ void pme(lima_oracle* o, std::vector<double>& F, std::vector<double>& pot, lima_
     const double twopi = 2.0 * (double)PI_L;
     const double fourpi = 4.0 * (double)PI_L;
     // per-axis wave numbers and spline factors; G is separable apart from exp(-k^2 / 4 kappa^2)
-    std::vector<double> kk[3], sf[3];
+    std::vector<double> kk[3], kv[3], sf[3];
     for (int d = 0; d < 3; d++) {
         kk[d].resize(o->K[d]);
+        kv[d].resize(o->K[d]);
         sf[d].resize(o->K[d]);
         const int half = o->K[d] / 2;
         for (int i = 0; i < o->K[d]; i++) {
@@ -542,10 +572,17 @@ void pme(lima_oracle* o, std::vector<double>& F, std::vector<double>& pot, lima_
             const double k = twopi * m / o->L[d];
             const double kh = k * h[d] * 0.5;
             kk[d][i] = k * k;
+            kv[d][i] = k;
             sf[d][i] = (std::fabs(kh) < 1e-14) ? 1.0 : std::pow(std::sin(kh) / kh, 4);
         }
     }
-#pragma omp parallel for schedule(static) num_threads(nthreads())
+    // Reciprocal-space virial.  E_rec = V_cell / (2 NG) sum_m G(m) |rho^(m)|^2 (the gather's
+    // 1/2 sum q phi by Parseval), with |rho^|^2 V_cell^2 fixed under a strain of the box (the spline
+    // weights depend on fractional coordinates only, and sinc(k h / 2) = sinc(pi m / K) does not
+    // depend on L), so per wave vector W_ab = E_m (delta_ab - 2 (1/k^2 + 1/(4 kappa^2)) k_a k_b).
+    const double vfac = 1.0 / (inv_vcell * 2.0 * (double)NG), inv4k2 = 1.0 / (4.0 * o->kappa * o->kappa);
+    double wr[6] = {0, 0, 0, 0, 0, 0};
+#pragma omp parallel for schedule(static) reduction(+ : wr[:6]) num_threads(nthreads())
     for (int z = 0; z < Kz; z++) for (int y = 0; y < Ky; y++) for (int x = 0; x < Kx; x++) {
         const double k2 = kk[0][x] + kk[1][y] + kk[2][z];
         const double S = sf[0][x] * sf[1][y] * sf[2][z];
@@ -556,6 +593,15 @@ void pme(lima_oracle* o, std::vector<double>& F, std::vector<double>& pot, lima_
         if (k2 > 1e-14) {
             G = fourpi / k2 * std::exp(-k2 / (4.0 * o->kappa * o->kappa)) * KC;
             G = o->pme_divide_modulus ? G / (S * S) : G * S * S;
+            const double Em = vfac * G * std::norm(grid[((size_t)z * Ky + y) * Kx + x]);
+            const double c = 2.0 * (1.0 / k2 + inv4k2);
+            const double ka[3] = {kv[0][x], kv[1][y], kv[2][z]};
+            wr[0] += Em * (1.0 - c * ka[0] * ka[0]);
+            wr[1] += Em * (1.0 - c * ka[1] * ka[1]);
+            wr[2] += Em * (1.0 - c * ka[2] * ka[2]);
+            wr[3] -= Em * c * ka[0] * ka[1];
+            wr[4] -= Em * c * ka[0] * ka[2];
+            wr[5] -= Em * c * ka[1] * ka[2];
         }
         grid[((size_t)z * Ky + y) * Kx + x] *= G;
     }
@@ -589,6 +635,7 @@ void pme(lima_oracle* o, std::vector<double>& F, std::vector<double>& pot, lima_
         erec += 0.5 * o->q[i] * ph;
     }
     E.pme_recip = erec;
+    for (int k = 0; k < 6; k++) E.virial[k] += wr[k];
     E.pme_self_lima = 0.5 * eself * (double)ncharged;
     E.pme_self_ewald = eself;
 }
@@ -598,10 +645,10 @@ void evaluate(lima_oracle* o, int comps, std::vector<double>& F, std::vector<dou
     pot.assign(o->N, 0.0);
     E = lima_energy{};
     double elj = 0, ec = 0;
-    if (comps & LIMA_ORC_NB_CUTOFF) nb_cutoff(o, F, pot, elj, ec);
-    if (comps & LIMA_ORC_NB_INTRA) nb_intra(o, F, pot, elj, ec);
+    if (comps & LIMA_ORC_NB_CUTOFF) nb_cutoff(o, F, pot, elj, ec, E.virial);
+    if (comps & LIMA_ORC_NB_INTRA) nb_intra(o, F, pot, elj, ec, E.virial);
     E.lj = elj; E.coulomb_sr = ec;
-    if (comps & LIMA_ORC_BONDED) E.bonded = bonded(o, F, pot);
+    if (comps & LIMA_ORC_BONDED) E.bonded = bonded(o, F, pot, E.virial);
     if ((comps & LIMA_ORC_PME) && o->elec) pme(o, F, pot, E);
     E.potential = E.lj + E.coulomb_sr + E.bonded + E.pme_recip + E.pme_self_ewald;
     E.potential_lima = E.lj + E.coulomb_sr + E.bonded + E.pme_recip + E.pme_self_lima;
