@@ -112,7 +112,10 @@ typedef struct lima_simparams {
     int32_t pme_grid[3];              /* 0 = derived from pme_spacing_nm */
     float   pme_spacing_nm;           /* target PME spacing, default 0.12 */
     float   nlist_skin_nm;            /* Verlet buffer, default 0.15 */
-    int32_t nlist_interval;           /* steps between list rebuilds, default 20 */
+    int32_t nlist_interval;           /* most steps per list, default 20.  An interval after which the
+                                         two largest displacements since the build sum to more than the
+                                         skin is undone and run again with lists a quarter shorter; after
+                                         50 rebuilds without one the interval grows by a step again */
     int32_t energy_interval;          /* steps between FP64 energy reductions (0 = never) */
     int32_t launch_mode;              /* 0 = each MD step replayed as a HIP graph (captured again after
                                          every list rebuild), PME on a second stream; or-ed flags
@@ -159,16 +162,19 @@ typedef struct lima_runstatus {
     int32_t critical_error_occured;
     int32_t n_list_rebuilds;
     int64_t list_entries;         /* pair-list entries (8 pair slots each) of the last rebuild */
-    int64_t list_violations;      /* rebuilds after which 2 x max displacement exceeded the skin: a
-                                     conservative flag (the fastest particle, usually a water H,
-                                     could have brought an unlisted pair inside the cutoff); the
-                                     force error this causes is measured by comparing the step with
-                                     the oldest list against an exact evaluation, not by this count */
+    int64_t list_violations;      /* kept list intervals whose largest plus second-largest
+                                     displacement since the build exceeded the skin (a pair could
+                                     have come inside the cutoff unlisted).  Such an interval is
+                                     undone and run again with shorter lists, so this stays 0 in MD;
+                                     see list_stale_rollbacks */
     float   max_displacement_nm;  /* max displacement over the last completed list interval */
     int32_t capacity_rollbacks;   /* times issued steps were undone and run again after a capacity
                                      overflow (see lima_step) */
     int32_t list_capacity;        /* pair-list entry slots per i-cluster */
     int32_t list_max_entries;     /* slots the fullest i-cluster of the current list uses */
+    int32_t list_interval;        /* steps per list now in use (at most nlist_interval) */
+    int32_t list_stale_rollbacks; /* list intervals undone and run again because the list could have
+                                     missed a pair */
 } lima_runstatus;
 
 /* State read-back.  Arrays the caller provides (NULL = skip) in LIMA's
@@ -298,7 +304,8 @@ lima_status lima_get_timings(lima_engine* engine, lima_timings* out);
  * summed when lima_get_timings is called). */
 lima_status lima_set_timing(lima_engine* engine, int32_t enabled);
 /* Particle pairs inside the cutoff counted by the pair kernel at the last energy evaluation
- * (the algorithmic work unit of the pair kernel); -1 for a null engine. */
+ * (the algorithmic work unit of the pair kernel), after the issued steps have finished; -1 for a
+ * null engine or an error (lima_last_error). */
 int64_t lima_get_pair_count(lima_engine* engine);
 
 /* ---- Batched minimisation: Programs::EnergyMinimize (Programs.cpp:155-199) applied to many

@@ -92,6 +92,8 @@ class RunStatus(C.Structure):
         ("capacity_rollbacks", C.c_int32),
         ("list_capacity", C.c_int32),
         ("list_max_entries", C.c_int32),
+        ("list_interval", C.c_int32),
+        ("list_stale_rollbacks", C.c_int32),
     ]
 
 

@@ -142,11 +142,13 @@ enum DFlag {
     DF_PAIRLIST,       // an i-cluster's pair-list slice overflowed
     DF_PMEBIN,         // a PME tile bin overflowed
     DF_NONFINITE,      // the integrator met a NaN / inf force
+    DF_STALE,          // a list interval ended with two displacements since the build summing to more
+                       // than the skin: a pair could have come inside the cutoff unlisted
     DF_SNAPSHOTS,      // snapshots taken (ordinal of the last one)
     DF_LIST_NEED,      // with DF_PAIRLIST: the largest entry count an i-cluster needed
     DF_N = 8
 };
-constexpr int DF_ERRORS = 4;   // words 0 .. DF_ERRORS-1 are error conditions
+constexpr int DF_ERRORS = 5;   // words 0 .. DF_ERRORS-1 are error conditions
 
 // Host side of a snapshot: the engine counters that belong with the device state it saved.
 struct SnapRecord {
@@ -156,6 +158,8 @@ struct SnapRecord {
     int n_rebuilds = 0;        // rebuilds before this one
     bool temp_measured = false;
     int64_t energy_step = -1;
+    int list_interval = 20;    // steps of the list this rebuild starts
+    int builds_clean = 0;
 };
 
 struct SdState {            // steepest descent state, device resident
@@ -186,7 +190,11 @@ struct Engine {
     int4* img_build = nullptr;
     float* dmax2 = nullptr;    // [1] max squared displacement since the last rebuild [nm^2]
     int* lstats = nullptr;     // [2] device list statistics: rebuilds after which 2 max|dx| > skin, last max|dx| (float bits)
-    float* dpart = nullptr;    // [N/64] per-wave running max of the squared displacement
+    float2* dpart = nullptr;   // [N/64] per wave: running maxima of the largest and second-largest squared
+                               // displacement since the list build among the wave's particles
+    int list_interval = 20;        // steps per list in use: nlist_interval, lowered after a stale interval
+    int builds_clean = 0;          // rebuilds issued since the last stale interval
+    int64_t stale_rollbacks = 0;   // intervals undone because the list could have missed a pair
     uint4* w_try = nullptr;
     int4* img_try = nullptr;
     float4* f_acc = nullptr;   // accepted forces, global order
@@ -247,8 +255,8 @@ struct Engine {
     float* thermo = nullptr;   // [2] velocity scale factor applied by the integrator, last measured temperature [K]
     bool temp_measured = false;
 
-    hipStream_t s_main = nullptr, s_pme = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipStream_t s_main = nullptr, s_pme = nullptr, s_local = nullptr;   // all at the default priority
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_join_local = nullptr;
     // one MD step (evaluation without energies + integration) as an instantiated HIP graph; its
     // kernel arguments depend on the list, so every list rebuild invalidates it
     hipGraphExec_t step_graph = nullptr;
@@ -311,6 +319,7 @@ void nlist_free(Engine& e);
 // grow a pair list that overflowed and build again (the one host synchronisation of a rebuild);
 // otherwise the words are copied to pinned host memory behind an event and examined later
 // (poll_rebuild_flags / settle in engine.hip).
+void list_interval_check(Engine& e, hipStream_t s);   // raises DF_STALE if the open interval's list may miss a pair
 void nlist_build(Engine& e, const uint4* w, bool checked);
 void nlist_grow(Engine& e, int need);   // pair-list capacity >= 1.5 x need (the step graph is captured again)
 void reset_flags(Engine& e, int64_t snapshots, hipStream_t s);   // clear the error words, set DF_SNAPSHOTS

@@ -42,7 +42,7 @@ struct IntegrateArgs {
     const float* thermostat;   // [0] velocity scale factor (Thermostat.cuh:56-67), set on the device
     const uint4* w_build;
     const int4* img_build;
-    float* dpart;          // per-wave running max of the squared displacement since the list build
+    float2* dpart;         // per wave: running maxima of the two largest squared displacements since the list build
     double* epart;         // kinetic-energy region: one record per wave
     const int* slot_of;
     const float* fpair;    // per-kernel force buffers [3][nslots], summed in this order
@@ -141,10 +141,16 @@ __global__ void k_integrate(IntegrateArgs a) {
     }
     if (a.pack && a.bins.bin_count) bin_wave(a.bins, bin_on, (int)p, bin_w);   // all lanes, converged
     if (__ballot(bad) != 0ull && (threadIdx.x & 63) == 0) atomicOr(&a.flags[DF_NONFINITE], 1);
-    d2 = wave_max(d2);
+    // the wave's largest and second-largest squared displacement (lanes past N hold 0)
+    const float m1 = wave_max(d2);
+    const bool top = d2 == m1;
+    const float m2 = __popcll(__ballot(top)) > 1 ? m1 : wave_max(top ? 0.f : d2);
     const int64_t wv = p >> 6;
     const bool wave_in = (p & ~63ll) < a.N;   // waves inside the per-wave arrays
-    if ((threadIdx.x & 63) == 0 && wave_in) a.dpart[wv] = fmaxf(a.dpart[wv], d2);
+    if ((threadIdx.x & 63) == 0 && wave_in) {
+        const float2 o = a.dpart[wv];
+        a.dpart[wv] = make_float2(fmaxf(o.x, m1), fmaxf(o.y, m2));
+    }
     if (ENERGY) {
         const double t = wave_sum_d((double)ke);
         if ((threadIdx.x & 63) == 0 && wave_in) {
@@ -391,12 +397,22 @@ void evaluate(Engine& e, const uint4* w, const int4* img, bool energy, bool time
     if (pack) launch_pack(e, w, s);
     e.packed = false;
     if (ev) HIP_CHECK(hipEventRecord(ev[1], s));
-    const bool split = !timed && !(e.prm.launch_mode & LAUNCH_ONE_STREAM) && e.elec && e.pme.ncharged > 0;
+    // Three streams after the pack: PME on s_pme, the intra-compound pairs and bonded terms
+    // (k_local, latency-bound) on s_local beside the VALU-bound pair kernel on the main stream.
+    // They write disjoint force and energy buffers; the main stream joins both before the
+    // integrator.
+    const bool split = !timed && !(e.prm.launch_mode & LAUNCH_ONE_STREAM);
+    const bool pme_on = e.elec && e.pme.ncharged > 0;
     if (split) {
         HIP_CHECK(hipEventRecord(e.ev_fork, s));
-        HIP_CHECK(hipStreamWaitEvent(e.s_pme, e.ev_fork, 0));
-        launch_pme(e, w, energy, e.s_pme);
-        HIP_CHECK(hipEventRecord(e.ev_join, e.s_pme));
+        if (pme_on) {
+            HIP_CHECK(hipStreamWaitEvent(e.s_pme, e.ev_fork, 0));
+            launch_pme(e, w, energy, e.s_pme);
+            HIP_CHECK(hipEventRecord(e.ev_join, e.s_pme));
+        }
+        HIP_CHECK(hipStreamWaitEvent(e.s_local, e.ev_fork, 0));
+        launch_local(e, w, img, energy, e.s_local);
+        HIP_CHECK(hipEventRecord(e.ev_join_local, e.s_local));
     }
     launch_pair(e, energy, s);
     if (ev) HIP_CHECK(hipEventRecord(ev[2], s));
@@ -405,11 +421,15 @@ void evaluate(Engine& e, const uint4* w, const int4* img, bool energy, bool time
         HIP_CHECK(hipEventRecord(ev[3], s));
         launch_bonded(e, w, img, energy, s);
         HIP_CHECK(hipEventRecord(ev[4], s));
-    } else {
+    } else if (!split) {
         launch_local(e, w, img, energy, s);
     }
-    if (split) HIP_CHECK(hipStreamWaitEvent(s, e.ev_join, 0));
-    else launch_pme(e, w, energy, s);
+    if (split) {
+        HIP_CHECK(hipStreamWaitEvent(s, e.ev_join_local, 0));
+        if (pme_on) HIP_CHECK(hipStreamWaitEvent(s, e.ev_join, 0));
+    } else {
+        launch_pme(e, w, energy, s);
+    }
     if (ev) HIP_CHECK(hipEventRecord(ev[5], s));
 }
 
@@ -550,6 +570,7 @@ void capture_step_graph(Engine& e) {
 // rebuild due, so the caller issues the lost steps again.  A non-finite force or an overfull
 // column is not recoverable and throws.
 void rollback(Engine& e) {
+    HIP_CHECK(hipStreamSynchronize(e.s_local));
     HIP_CHECK(hipStreamSynchronize(e.s_pme));
     HIP_CHECK(hipStreamSynchronize(e.s_main));
     int f[DF_N];
@@ -564,9 +585,20 @@ void rollback(Engine& e) {
     auto it = std::find_if(e.snaps.begin(), e.snaps.end(), [&](const SnapRecord& r) { return r.ordinal == c; });
     if (it == e.snaps.end()) throw LimaFail(LIMA_ERR_HIP, "internal: no record of the last snapshot");
     const SnapRecord r = *it;
-    if (++e.rollbacks > 64) throw LimaFail(LIMA_ERR_LIST_OVERFLOW, "capacity overflows persist after 64 rollbacks");
-    if (f[DF_PAIRLIST]) nlist_grow(e, f[DF_LIST_NEED]);
-    if (f[DF_PMEBIN]) pme_grow_bins(e);
+    if (f[DF_PAIRLIST] || f[DF_PMEBIN]) {
+        if (++e.rollbacks > 64) throw LimaFail(LIMA_ERR_LIST_OVERFLOW, "capacity overflows persist after 64 rollbacks");
+        if (f[DF_PAIRLIST]) nlist_grow(e, f[DF_LIST_NEED]);
+        if (f[DF_PMEBIN]) pme_grow_bins(e);
+    }
+    e.list_interval = r.list_interval;   // the schedule in force when the snapshot was taken
+    e.builds_clean = r.builds_clean;
+    if (f[DF_STALE]) {   // run the interval again with lists that live a quarter shorter
+        if (e.list_interval <= 1)
+            throw LimaFail(LIMA_ERR_LIST_OVERFLOW, "particles move more than half the list skin in one step");
+        e.list_interval -= std::max(1, e.list_interval / 4);
+        e.builds_clean = 0;
+        e.stale_rollbacks++;
+    }
     hipStream_t s = e.s_main;
     const int64_t N = e.N;
     HIP_CHECK(hipMemcpyAsync(e.p.w, e.w_build, N * sizeof(uint4), hipMemcpyDeviceToDevice, s));
@@ -576,7 +608,7 @@ void rollback(Engine& e) {
     HIP_CHECK(hipMemcpyAsync(e.thermo, e.thermo_snap, 2 * sizeof(float), hipMemcpyDeviceToDevice, s));
     HIP_CHECK(hipMemcpyAsync(e.lstats, e.lstats_snap, 2 * sizeof(int), hipMemcpyDeviceToDevice, s));
     HIP_CHECK(hipMemcpyAsync(e.acc, e.acc_snap, ACC_N * sizeof(double), hipMemcpyDeviceToDevice, s));
-    HIP_CHECK(hipMemsetAsync(e.dpart, 0, ((N + 63) / 64) * sizeof(float), s));
+    HIP_CHECK(hipMemsetAsync(e.dpart, 0, ((N + 63) / 64) * sizeof(float2), s));
     pme_clear_bins(e, s);
     reset_flags(e, c - 1, s);
     HIP_CHECK(hipStreamSynchronize(s));
@@ -616,10 +648,16 @@ bool poll_rebuild_flags(Engine& e) {
 void issue_steps(Engine& e, int64_t target) {
     const auto t_issue = std::chrono::steady_clock::now();
     const int64_t step0 = e.step;
-    const int interval = e.prm.nlist_interval > 0 ? e.prm.nlist_interval : 20;
+    const int max_interval = e.prm.nlist_interval > 0 ? e.prm.nlist_interval : 20;
     while (e.step < target) {
+        const int interval = e.list_interval;   // a rollback may shorten it
         if (e.steps_since_build >= interval) {
             if (poll_rebuild_flags(e)) continue;   // rolled back to an earlier step: issue from there
+            // after 50 rebuilds without a stale interval, try lists one step longer again
+            if (++e.builds_clean >= 50 && e.list_interval < max_interval) {
+                e.list_interval++;
+                e.builds_clean = 0;
+            }
             // timing: drain earlier steps first, so the rebuild's clock holds the rebuild alone
             if (e.timing) HIP_CHECK(hipDeviceSynchronize());
             const auto t0 = std::chrono::steady_clock::now();
@@ -670,9 +708,12 @@ void issue_steps(Engine& e, int64_t target) {
 // often as needed), and reads unread energy totals.  Every call that returns data runs it first.
 void settle(Engine& e) {
     for (;;) {
+        HIP_CHECK(hipStreamSynchronize(e.s_local));
         HIP_CHECK(hipStreamSynchronize(e.s_pme));
         HIP_CHECK(hipStreamSynchronize(e.s_main));
         HIP_CHECK(hipGetLastError());
+        // the interval still open has had no rebuild to check it: its completeness bound now
+        if (e.steps_since_build > 0 && e.steps_since_build < INT64_MAX / 4) list_interval_check(e, e.s_main);
         int f[DF_N];
         HIP_CHECK(hipMemcpy(f, e.dflags, sizeof(f), hipMemcpyDeviceToHost));
         e.rb_pending[0] = e.rb_pending[1] = false;
@@ -708,6 +749,7 @@ void create(Engine& e, const lima_box_desc* b, const lima_simparams* p) {
     e.rc = p->cutoff_nm;
     e.rc2 = e.rc * e.rc;
     e.skin = p->nlist_skin_nm > 0 ? p->nlist_skin_nm : 0.15f;
+    e.list_interval = p->nlist_interval > 0 ? p->nlist_interval : 20;
     e.elec = p->enable_electrostatics != 0;
     e.pc.kc = (float)(8.9875517873681764e9 / 1e-9 / 6.02214076e23 * 1000.f * 1000.f);   // PhysicsUtils.cuh:5
     e.pc.kappa = 3.f / e.rc;                                                             // PhysicsUtils.cuh:60-62
@@ -816,8 +858,8 @@ void create(Engine& e, const lima_box_desc* b, const lima_simparams* p) {
         HIP_CHECK(hipMalloc(&e.thermo, sizeof(t0)));
         HIP_CHECK(hipMemcpy(e.thermo, t0, sizeof(t0), hipMemcpyHostToDevice));
     }
-    HIP_CHECK(hipMalloc(&e.dpart, ((N + 63) / 64) * sizeof(float)));
-    HIP_CHECK(hipMemset(e.dpart, 0, ((N + 63) / 64) * sizeof(float)));
+    HIP_CHECK(hipMalloc(&e.dpart, ((N + 63) / 64) * sizeof(float2)));
+    HIP_CHECK(hipMemset(e.dpart, 0, ((N + 63) / 64) * sizeof(float2)));
     HIP_CHECK(hipMalloc(&e.w_try, N * sizeof(uint4)));
     HIP_CHECK(hipMalloc(&e.img_try, N * sizeof(int4)));
     e.f_acc = dalloc_copy(zero4);
@@ -991,6 +1033,8 @@ void create(Engine& e, const lima_box_desc* b, const lima_simparams* p) {
     HIP_CHECK(hipStreamCreate(&e.s_pme));
     HIP_CHECK(hipEventCreateWithFlags(&e.ev_fork, hipEventDisableTiming));
     HIP_CHECK(hipEventCreateWithFlags(&e.ev_join, hipEventDisableTiming));
+    HIP_CHECK(hipStreamCreate(&e.s_local));
+    HIP_CHECK(hipEventCreateWithFlags(&e.ev_join_local, hipEventDisableTiming));
     for (auto& ev : e.ev_rebuild) HIP_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     for (auto& ev : e.tev) HIP_CHECK(hipEventCreate(&ev));
     nlist_alloc(e);
@@ -1045,6 +1089,8 @@ void destroy(Engine& e) {
     e.step_graph = nullptr;
     if (e.ev_fork) (void)hipEventDestroy(e.ev_fork);
     if (e.ev_join) (void)hipEventDestroy(e.ev_join);
+    if (e.ev_join_local) (void)hipEventDestroy(e.ev_join_local);
+    if (e.s_local) (void)hipStreamDestroy(e.s_local);
     if (e.s_main) (void)hipStreamDestroy(e.s_main);
     if (e.s_pme) (void)hipStreamDestroy(e.s_pme);
 }
@@ -1361,6 +1407,8 @@ lima_status lima_get_status(lima_engine* h, lima_runstatus* out) {
     }
     out->greatest_force = NAN;
     out->capacity_rollbacks = e.rollbacks;
+    out->list_interval = e.list_interval;
+    out->list_stale_rollbacks = (int32_t)e.stale_rollbacks;
     out->list_capacity = e.nl.cap;
     return status;
 }
@@ -1454,6 +1502,14 @@ lima_status lima_set_timing(lima_engine* h, int32_t enabled) {
     });
 }
 
-int64_t lima_get_pair_count(lima_engine* h) { return h ? h->e.pairs_last : -1; }
+int64_t lima_get_pair_count(lima_engine* h) {
+    if (!h) return -1;
+    int64_t n = -1;
+    const lima_status st = guarded([&] {
+        settle(h->e);
+        n = h->e.pairs_last;
+    });
+    return st == LIMA_OK ? n : -1;
+}
 
 }  // extern "C"

@@ -113,8 +113,11 @@ struct PairArgs {
 // group then read their entry with two broadcast ds_read_b128, so the texture path sees each
 // datum once instead of eight times.  Two LDS buffers: chunk c + 1 is loaded while c is computed.
 // A cluster's near entries come first, its far entries (beyond the cutoff from its box at the
-// build) last, so whole passes of the far segment usually fail the cutoff test and skip the
-// arithmetic.
+// build) last.  Before an entry is staged, the gap between its j position and the bounding box of
+// the cluster's own particles at this step is compared with the cutoff: an entry farther away
+// cannot reach any i, so it is dropped and the chunk's survivors are compacted (ballot prefix)
+// into fewer passes.  The test is exact at every step, so the list's skin costs fetches, not
+// passes.
 constexpr int PAIR_WAVES = 4;
 constexpr int CHUNK = 64;
 
@@ -138,6 +141,20 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs a) {
     float2 pi = make_float2(0.f, 0.f);
     if (live) { xi = a.xq[islot]; pi = a.ljs[islot]; }
     const float qi_kc = xi.w * a.pc.kc;
+    // bounding box of the cluster's real particles at this step (padding slots sit at PAD_POS):
+    // centre and half extent per axis, and the cutoff with 1e-4 nm to spare for rounding
+    float bc[3], bh[3];
+    {
+        const bool real = fabsf(xi.x) < 0.5f * PAD_POS;
+        const float p3[3] = {xi.x, xi.y, xi.z};
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const float lo = -wave_max(real ? -p3[d] : -1e30f), hi = wave_max(real ? p3[d] : -1e30f);
+            bc[d] = 0.5f * (lo + hi);
+            bh[d] = 0.5f * (hi - lo);
+        }
+    }
+    const float rcut = sqrtf(a.rc2) + 1e-4f, rcut2 = rcut * rcut;
     float fx = 0.f, fy = 0.f, fz = 0.f;
     double elj = 0.0, ecl = 0.0;   // FP64 energy accumulation (energy steps only)
     double vir[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};   // -1/2 S d (x) d per evaluated pair (energy steps only)
@@ -166,17 +183,36 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs a) {
             ga = make_float4(PAD_POS, PAD_POS, PAD_POS, 0.f);
             gb = make_float4(0.f, 0.f, __uint_as_float(0u), 0.f);
         }
+    };    // the fetched chunk's entries inside the cluster's reach, compacted into LDS buffer buf and padded
+    // to whole passes; returns how many passes they fill (wave-uniform)
+    auto stage = [&](int buf) {
+        const float gx = fmaxf(fabsf(ga.x - bc[0]) - bh[0], 0.f), gy = fmaxf(fabsf(ga.y - bc[1]) - bh[1], 0.f),
+                    gz = fmaxf(fabsf(ga.z - bc[2]) - bh[2], 0.f);
+        const bool keep = gx * gx + gy * gy + gz * gz <= rcut2;   // entries past the end sit at PAD_POS
+        const uint64_t bal = __ballot(keep);
+        const int cnt = __popcll(bal);
+        const int pos = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+        if (keep) {
+            sA[wid][buf][pos] = ga;
+            sB[wid][buf][pos] = gb;
+        }
+        const int npad = ((cnt + CLUSTER - 1) & ~(CLUSTER - 1)) - cnt;
+        if (lane < npad) {
+            sA[wid][buf][cnt + lane] = make_float4(PAD_POS, PAD_POS, PAD_POS, 0.f);
+            sB[wid][buf][cnt + lane] = make_float4(0.f, 0.f, __uint_as_float(0u), 0.f);
+        }
+        return (cnt + CLUSTER - 1) / CLUSTER;
     };
+    int nit_next = 0;
     if (nchunk > 0) {
         fetch(0);
-        sA[wid][0][lane] = ga;
-        sB[wid][0][lane] = gb;
+        nit_next = stage(0);
         __builtin_amdgcn_wave_barrier();   // the wave's own LDS slice: ordering only, no s_barrier
     }
     for (int c = 0; c < nchunk; c++) {
         const int buf = c & 1;
+        const int nit = __builtin_amdgcn_readfirstlane(nit_next);   // wave-uniform
         if (c + 1 < nchunk) fetch(c + 1);   // in flight during the compute below
-        const int nit = __builtin_amdgcn_readfirstlane(min(CHUNK, nent - c * CHUNK) / CLUSTER);   // wave-uniform
         for (int k = 0; k < nit; k++) {
             const float4 A = sA[wid][buf][k * CLUSTER + jj];
             const float4 B = sB[wid][buf][k * CLUSTER + jj];
@@ -201,10 +237,7 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs a) {
                 }
             }
         }
-        if (c + 1 < nchunk) {
-            sA[wid][buf ^ 1][lane] = ga;
-            sB[wid][buf ^ 1][lane] = gb;
-        }
+        if (c + 1 < nchunk) nit_next = stage(buf ^ 1);
         __builtin_amdgcn_wave_barrier();
     }
     // i force: lanes ii, ii+8, ..., ii+56 hold partial sums

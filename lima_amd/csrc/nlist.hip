@@ -645,7 +645,7 @@ struct OpenArgs {
     const uint4* w; const int4* img; const float4* v; const float4* fprev;
     const float* thermo; const int* lstats; const double* acc;
     uint4* wb; int4* ib; float4* vs; float4* fps; float* ths; int* lss; double* accs;
-    float* dpart;
+    float2* dpart;
     int* col_count; int* col_fill; int ncol1;
     int* flags;
     int* bins; int nbins;
@@ -655,7 +655,7 @@ __global__ void k_rebuild_open(OpenArgs a) {
     bool clean = a.snapshot != 0;
 #pragma unroll
     for (int k = 0; k < DF_ERRORS; k++) clean = clean && a.flags[k] == 0;
-    if (a.snapshot && p < a.N && (p & 63) == 0) a.dpart[p >> 6] = 0.f;
+    if (a.snapshot && p < a.N && (p & 63) == 0) a.dpart[p >> 6] = make_float2(0.f, 0.f);
     if (clean && p < a.N) {
         a.wb[p] = a.w[p];
         a.ib[p] = a.img[p];
@@ -679,23 +679,47 @@ __global__ void k_reset_flags(int* flags, int snapshots) {
     if (k == DF_SNAPSHOTS) flags[k] = snapshots;
 }
 
-// List staleness statistics kept on the device, from the per-wave displacement maxima of the
-// list being replaced: the largest displacement since the previous rebuild (*dmax2, squared;
-// stats[1], as float bits), and a count of rebuilds after which 2 max|dx| exceeded the skin.
-__global__ __launch_bounds__(256) void k_list_stats(const float* __restrict__ v, int n, float skin, int counted, float* dmax2,
-                                                    int* stats) {
-    __shared__ float red[4];
-    float m = 0.f;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) m = fmaxf(m, v[i]);
-    m = wave_max(m);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+// Completeness of the list being replaced, from the per-wave displacement maxima: a pair (i, j)
+// inside the cutoff at some step of the interval is in the list when |dx_i| + |dx_j| <= skin at that
+// step (the build listed every particle within cutoff + skin of the i-cluster's box).  Every
+// step's largest plus second-largest displacement is at most G1 + G2, the two largest values of
+// the multiset {running wave maximum, running wave second maximum} over all waves.  mode bit 0:
+// statistics of a finished interval (*dmax2 = G1, stats[0] counts intervals with G1 + G2 > skin,
+// stats[1] = sqrt(G1) as float bits); bit 1: raise DF_STALE when G1 + G2 > skin.
+__device__ __forceinline__ void top2_insert(float& t1, float& t2, float v) {
+    if (v > t1) { t2 = t1; t1 = v; }
+    else if (v > t2) t2 = v;
+}
+
+// (t1 >= t2) and (u1 >= u2) -> the two largest of the four
+__device__ __forceinline__ void top2_merge(float& t1, float& t2, float u1, float u2) {
+    const float hi = fmaxf(t1, u1), lo = fminf(t1, u1);
+    t1 = hi;
+    t2 = fmaxf(lo, fmaxf(t2, u2));
+}
+
+__global__ __launch_bounds__(256) void k_list_stats(const float2* __restrict__ v, int n, float skin, int mode, float* dmax2,
+                                                    int* stats, int* flags) {
+    __shared__ float r1[4], r2[4];
+    float t1 = 0.f, t2 = 0.f;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const float2 m = v[i];
+        top2_insert(t1, t2, m.x);
+        top2_insert(t1, t2, m.y);
+    }
+    for (int o = 1; o < 64; o <<= 1) top2_merge(t1, t2, __shfl_xor(t1, o, 64), __shfl_xor(t2, o, 64));
+    if ((threadIdx.x & 63) == 0) { r1[threadIdx.x >> 6] = t1; r2[threadIdx.x >> 6] = t2; }
     __syncthreads();
     if (threadIdx.x == 0) {
-        const float m2 = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
-        *dmax2 = m2;
-        const float d = sqrtf(m2);
-        if (counted && 2.f * d > skin) stats[0] += 1;
-        stats[1] = __float_as_int(d);
+        float g1 = r1[0], g2 = r2[0];
+        for (int k = 1; k < 4; k++) top2_merge(g1, g2, r1[k], r2[k]);
+        const bool stale = sqrtf(g1) + sqrtf(g2) > skin;
+        if (mode & 1) {
+            *dmax2 = g1;
+            if (stale) stats[0] += 1;
+            stats[1] = __float_as_int(sqrtf(g1));
+        }
+        if ((mode & 2) && stale) atomicOr(&flags[DF_STALE], 1);
     }
 }
 
@@ -717,6 +741,11 @@ void reset_flags(Engine& e, int64_t snapshots, hipStream_t s) {
     hipLaunchKernelGGL(k_reset_flags, dim3(1), dim3(64), 0, s, e.dflags, (int)snapshots);
 }
 
+void list_interval_check(Engine& e, hipStream_t s) {
+    hipLaunchKernelGGL(k_list_stats, dim3(1), dim3(256), 0, s, (const float2*)e.dpart, (int)((e.N + 63) / 64), e.skin, 2,
+                       e.dmax2, e.lstats, e.dflags);
+}
+
 void nlist_build(Engine& e, const uint4* w, bool checked) {
     NList& nl = e.nl;
     hipStream_t s = e.s_main;
@@ -736,12 +765,20 @@ void nlist_build(Engine& e, const uint4* w, bool checked) {
             r.n_rebuilds = e.n_rebuilds;
             r.temp_measured = e.temp_measured;
             r.energy_step = e.energy_step;
+            r.list_interval = e.list_interval;
+            r.builds_clean = e.builds_clean;
             e.snaps.push_back(r);
             if (e.snaps.size() > 16) e.snaps.erase(e.snaps.begin());
         }
-        // staleness statistics of the list being replaced, on the device (read by lima_get_status)
-        hipLaunchKernelGGL(k_list_stats, dim3(1), dim3(256), 0, s, (const float*)e.dpart, (int)((N + 63) / 64), e.skin,
-                           (e.n_rebuilds > 0 && !e.stats_skip) ? 1 : 0, e.dmax2, e.lstats);
+        // completeness of the list being replaced, on the device (statistics read by lima_get_status);
+        // an MD rebuild (deferred) raises DF_STALE before this rebuild's snapshot, so the snapshot of
+        // the interval's start survives for the rollback that runs the interval again
+        {
+            const bool counted = e.n_rebuilds > 0 && !e.stats_skip;
+            const int mode = counted ? (checked ? 1 : 3) : 0;
+            hipLaunchKernelGGL(k_list_stats, dim3(1), dim3(256), 0, s, (const float2*)e.dpart, (int)((N + 63) / 64), e.skin,
+                               mode, e.dmax2, e.lstats, e.dflags);
+        }
         e.stats_skip = false;
         {
             OpenArgs o{};
@@ -788,6 +825,7 @@ void nlist_build(Engine& e, const uint4* w, bool checked) {
         if (need > nl.nent_cap) {
             HIP_CHECK(hipStreamSynchronize(s));   // kernels of earlier steps may still read the old buffer
             HIP_CHECK(hipStreamSynchronize(e.s_pme));
+            HIP_CHECK(hipStreamSynchronize(e.s_local));
             if (nl.ent) HIP_CHECK(hipFree(nl.ent));
             if (nl.ecode) HIP_CHECK(hipFree(nl.ecode));
             nl.ent = nullptr; nl.ecode = nullptr;

@@ -160,6 +160,10 @@ class Oracle:
                                         msk.ctypes.data_as(C.POINTER(C.c_uint8)))
         return atoms, off, ent[:nent.value], msk[:nent.value]
 
+    def pair_count(self, radius_nm):
+        """Unordered non-intra particle pairs (over all images) with distance <= radius_nm, by brute force."""
+        return int(lib().lima_oracle_pair_count(self._h, radius_nm))
+
     def pairs(self, radius_nm):
         n = lib().lima_oracle_pair_count(self._h, radius_nm)
         pi = np.zeros(max(n, 1), np.int32)

@@ -204,3 +204,32 @@ def test_water_1m_forces_match_oracle():
     print(f"water 1M: rel RMS force error {err:.3e}, E {e_gpu.potential:.6e} / {e_ref.potential:.6e}", flush=True)
     assert err <= FORCE_TOL
     assert abs(e_gpu.potential - e_ref.potential) <= 1e-6 * abs(e_ref.potential)
+
+
+def test_list_stays_complete_when_atoms_move_fast():
+    """Water at 900 K with lists meant to live 40 steps: intervals whose two largest displacements
+    since the build exceed the skin are undone and run again with shorter lists, so every step
+    evaluates every pair inside the cutoff.  The pair kernel's count of in-cutoff pairs at each
+    checked step lies between the oracle's brute-force counts at cutoff -+ 2e-6 nm (the kernel's
+    FP32 distances of pairs that close to the cutoff may round either way; the band holds ~5 of
+    the ~1.08M pairs), and no kept interval breaks the bound."""
+    box = L.gen_water_lattice(10, 3.104, 0, 900.0)
+    p = L.default_params(dt_fs=2.0, nlist_interval=40, energy_interval=1)
+    eng = L.Engine(box, p)
+    rc = float(p.cutoff_nm)
+    checked = 0
+    for k in range(120):
+        st = eng.state()
+        eng.step(1)
+        if k % 12 != 11:
+            continue
+        n_gpu = eng.pair_count()          # pairs inside the cutoff evaluated at the positions st
+        orc = Oracle(L.StateBox(box, st), p)
+        lo, hi = orc.pair_count(rc - 2e-6), orc.pair_count(rc + 2e-6)
+        assert lo <= n_gpu <= hi, (k, lo, n_gpu, hi)
+        checked += 1
+    s = eng.status()
+    print(f"{checked} steps checked; list interval now {s.list_interval} (max 40), {s.list_stale_rollbacks} stale "
+          f"intervals undone, {s.list_violations} kept intervals over the bound, {s.n_list_rebuilds} rebuilds", flush=True)
+    assert s.list_stale_rollbacks >= 1 and s.list_interval < 40
+    assert s.list_violations == 0
