@@ -4,8 +4,8 @@
 Workload: the chain + water box of the mixed-precision config (8 x 8 x 7 nm, 128 chains of 50
 atoms in two leaflets along z, 12,000 waters filling the rest, 42,400 atoms; seed 1), cutoff
 1.2 nm, PME spacing 0.12 nm (72 x 72 x 60 grid), dt = 2 fs, FP32 forces with FP64 energy
-reductions.  The generated structure is relaxed with steepest descent (untimed), W warm-up steps
-run, then K steps are timed: lima_step only issues work, so the host clock runs from an idle
+reductions.  The generated structure is relaxed with steepest descent and equilibrated with 1,000 MD steps
+(both untimed), W warm-up steps run, then K steps are timed: lima_step only issues work, so the host clock runs from an idle
 device to lima_synchronize after the K steps.
 
 The line also carries: the same K steps issued one lima_step(h, 1) call at a time (LIMA's
@@ -383,6 +383,7 @@ def water_1m_leg(args):
     p = L.default_params(dt_fs=dt_fs, n_steps=args.steps_1m, energy_interval=0, launch_mode=args.launch_mode)
     eng = L.Engine(box, p)
     r = eng.minimize(1000.0, args.em_iters_1m, 0.01)
+    eng.step(args.equil_steps_1m)   # equilibration, untimed
     eng.step(args.warmup_1m)
     log(f"water_1m: {box.n_particles} atoms, {r.iterations} SD iterations, set up in {time.time() - t:.1f}s")
     elapsed = timed_window(eng, args.steps_1m)
@@ -412,6 +413,9 @@ def main():
     ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--warmup", type=int, default=500)
     ap.add_argument("--em-iters", type=int, default=400)
+    ap.add_argument("--equil-steps", type=int, default=1000,
+                    help="untimed MD steps after the relaxation, before the warm-up")
+    ap.add_argument("--equil-steps-1m", type=int, default=200)
     ap.add_argument("--timing-steps", type=int, default=1000)
     ap.add_argument("--steps-1m", type=int, default=300)
     ap.add_argument("--warmup-1m", type=int, default=60)
@@ -456,6 +460,14 @@ def main():
     t = time.time()
     r = eng.minimize(1000.0, args.em_iters, 0.01)
     log(f"relaxed: {r.iterations} SD iterations, fmax {r.final_fmax:.1f} kJ/mol/nm, {time.time() - t:.1f}s")
+    # equilibration MD (untimed, part of preparing the workload): the relaxed structure heats up and
+    # the list lifetime settles at the longest interval that keeps the lists complete
+    t = time.time()
+    eng.step(args.equil_steps)
+    eng.synchronize()
+    s0 = eng.status()
+    log(f"equilibrated {args.equil_steps} steps in {time.time() - t:.2f}s: lists live {s0.list_interval} steps, "
+        f"{s0.list_stale_rollbacks} intervals rerun")
     t = time.time()
     eng.step(args.warmup)
     eng.synchronize()
@@ -473,8 +485,8 @@ def main():
     pc_ms = pc * 1e3 / args.steps
     log(f"per-call: {args.steps} x lima_step(h, 1): {pc_ms:.4f} ms/step ({pc_ms / ms_per_step:.3f} x the multi-step call)")
     stt = eng.status()
-    log(f"list: {stt.n_list_rebuilds} rebuilds, {stt.list_violations} intervals with 2 max|dx| > skin, "
-        f"last max |dx| {stt.max_displacement_nm:.4f} nm")
+    log(f"list: {stt.n_list_rebuilds} rebuilds, lists live {stt.list_interval} steps, {stt.list_stale_rollbacks} intervals "
+        f"rerun, {stt.list_violations} kept intervals over the bound, last max |dx| {stt.max_displacement_nm:.4f} nm")
 
     phases, rebuild_ms, rebuilds = phase_times(eng, args.timing_steps, p.nlist_interval)
     log("phase ms/step: " + ", ".join(f"{k} {v:.4f}" for k, v in phases.items()) + f"; {rebuild_ms:.4f} ms per rebuild")
@@ -532,7 +544,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": workload, "n_atoms": box.n_particles, "box_nm": [float(x) for x in box.box_nm],
                    "chains": "128 x 50", "waters": (box.n_particles - 6400) // 3, "dt_fs": dt_fs, "cutoff_nm": 1.2,
-                   "pme_spacing_nm": 0.12, "pme_grid": pme_grid_dims(box.box_nm), "nlist_interval": p.nlist_interval,
+                   "pme_spacing_nm": 0.12, "pme_grid": pme_grid_dims(box.box_nm), "nlist_interval_max": p.nlist_interval,
+                   "equilibration_steps": args.equil_steps,
                    "skin_nm": round(p.nlist_skin_nm, 4), "energy_reductions": "fp64",
                    "launch": ("eager" if args.launch_mode & 1 else "step graph") +
                              (", one stream" if args.launch_mode & 2 else ", PME on a second stream")},
@@ -551,7 +564,9 @@ def main():
         "pair_slots_evaluated": list_slots,
         "cluster_pair_efficiency": round(2 * pairs / list_slots, 4) if list_slots else None,
         "list_rebuilds": stt.n_list_rebuilds,
-        "list_stale_intervals": stt.list_violations,
+        "list_interval": stt.list_interval,
+        "list_intervals_rerun": stt.list_stale_rollbacks,
+        "list_kept_intervals_over_bound": stt.list_violations,
         "final_energy_kj": round(e_end.total / 1000.0, 3),
         "water_1m": w1m,
         "em_batch": em,

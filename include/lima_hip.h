@@ -111,11 +111,12 @@ typedef struct lima_simparams {
     /* additions */
     int32_t pme_grid[3];              /* 0 = derived from pme_spacing_nm */
     float   pme_spacing_nm;           /* target PME spacing, default 0.12 */
-    float   nlist_skin_nm;            /* Verlet buffer, default 0.15 */
-    int32_t nlist_interval;           /* most steps per list, default 20.  An interval after which the
-                                         two largest displacements since the build sum to more than the
-                                         skin is undone and run again with lists a quarter shorter; after
-                                         50 rebuilds without one the interval grows by a step again */
+    float   nlist_skin_nm;            /* Verlet buffer, default 0.30 */
+    int32_t nlist_interval;           /* most steps per list, default 40.  Lists start at 10 steps (or
+                                         this, if less) and grow by a step after 20 rebuilds in a row whose
+                                         lists stayed complete.  An interval after which the two largest
+                                         displacements since the build sum to more than the skin is undone
+                                         and run again with lists a sixth shorter */
     int32_t energy_interval;          /* steps between FP64 energy reductions (0 = never) */
     int32_t launch_mode;              /* 0 = each MD step replayed as a HIP graph (captured again after
                                          every list rebuild), PME on a second stream; or-ed flags

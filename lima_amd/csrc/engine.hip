@@ -592,10 +592,10 @@ void rollback(Engine& e) {
     }
     e.list_interval = r.list_interval;   // the schedule in force when the snapshot was taken
     e.builds_clean = r.builds_clean;
-    if (f[DF_STALE]) {   // run the interval again with lists that live a quarter shorter
+    if (f[DF_STALE]) {   // run the interval again with lists that live a sixth shorter
         if (e.list_interval <= 1)
             throw LimaFail(LIMA_ERR_LIST_OVERFLOW, "particles move more than half the list skin in one step");
-        e.list_interval -= std::max(1, e.list_interval / 4);
+        e.list_interval -= std::max(1, e.list_interval / 6);
         e.builds_clean = 0;
         e.stale_rollbacks++;
     }
@@ -653,8 +653,8 @@ void issue_steps(Engine& e, int64_t target) {
         const int interval = e.list_interval;   // a rollback may shorten it
         if (e.steps_since_build >= interval) {
             if (poll_rebuild_flags(e)) continue;   // rolled back to an earlier step: issue from there
-            // after 50 rebuilds without a stale interval, try lists one step longer again
-            if (++e.builds_clean >= 50 && e.list_interval < max_interval) {
+            // after 20 rebuilds without a stale interval, try lists one step longer
+            if (++e.builds_clean >= 20 && e.list_interval < max_interval) {
                 e.list_interval++;
                 e.builds_clean = 0;
             }
@@ -749,7 +749,8 @@ void create(Engine& e, const lima_box_desc* b, const lima_simparams* p) {
     e.rc = p->cutoff_nm;
     e.rc2 = e.rc * e.rc;
     e.skin = p->nlist_skin_nm > 0 ? p->nlist_skin_nm : 0.15f;
-    e.list_interval = p->nlist_interval > 0 ? p->nlist_interval : 20;
+    // lists start short and grow while they stay complete (a stale interval costs a rerun)
+    e.list_interval = std::min(p->nlist_interval > 0 ? p->nlist_interval : 40, 10);
     e.elec = p->enable_electrostatics != 0;
     e.pc.kc = (float)(8.9875517873681764e9 / 1e-9 / 6.02214076e23 * 1000.f * 1000.f);   // PhysicsUtils.cuh:5
     e.pc.kappa = 3.f / e.rc;                                                             // PhysicsUtils.cuh:60-62

@@ -596,8 +596,8 @@ void lima_default_params(lima_simparams* p, float dt_fs, int64_t n_steps) {
     p->data_logging_interval = 5;
     p->steps_per_temperature_measurement = 200;
     p->pme_spacing_nm = 0.12f;
-    p->nlist_skin_nm = 0.15f;
-    p->nlist_interval = 20;
+    p->nlist_skin_nm = 0.30f;   // lists stay complete for ~10-15 steps of 2 fs water (see nlist_interval)
+    p->nlist_interval = 40;
     p->energy_interval = 0;
 }
 

@@ -57,7 +57,7 @@ def canonical_entries(atoms, off, ent, msk):
 
 def max_list_age_error(eng, box, params):
     """Relative RMS force error against the FP64 oracle of a step whose neighbour list is
-    nlist_interval - 1 steps old (the oldest list any step uses): steps to the next rebuild, then to
+    list_interval - 1 steps old (the oldest list any step uses): steps to the next rebuild, then to
     the last step before the following one, and compares that step's forces with the oracle's at
     the same positions."""
     import lima_amd as L
@@ -65,8 +65,9 @@ def max_list_age_error(eng, box, params):
     n0 = eng.status().n_list_rebuilds
     while eng.status().n_list_rebuilds == n0:
         eng.step(1)                      # this call rebuilt the list at its start
-    if params.nlist_interval > 2:
-        eng.step(params.nlist_interval - 2)
+    interval = eng.status().list_interval   # lists now live this many steps (at most nlist_interval)
+    if interval > 2:
+        eng.step(interval - 2)
     st = eng.state()                     # the next step evaluates forces here with the oldest list
     eng.step(1)
     f_gpu = layout_forces(box, eng.state())

@@ -214,7 +214,7 @@ def test_list_stays_complete_when_atoms_move_fast():
     FP32 distances of pairs that close to the cutoff may round either way; the band holds ~5 of
     the ~1.08M pairs), and no kept interval breaks the bound."""
     box = L.gen_water_lattice(10, 3.104, 0, 900.0)
-    p = L.default_params(dt_fs=2.0, nlist_interval=40, energy_interval=1)
+    p = L.default_params(dt_fs=2.0, nlist_interval=40, nlist_skin_nm=0.15, energy_interval=1)
     eng = L.Engine(box, p)
     rc = float(p.cutoff_nm)
     checked = 0
