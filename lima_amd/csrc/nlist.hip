@@ -320,30 +320,17 @@ __global__ __launch_bounds__(256) void k_nlist(NlArgs a) {
     const uint4 bmn = a.cmin[I], bmx = a.cmax[I], refI = a.cref[I];
     const long long mn[3] = {(long long)bmn.x, (long long)bmn.y, (long long)bmn.z};
     const long long mx[3] = {(long long)bmx.x, (long long)bmx.y, (long long)bmx.z};
+    // the i slots' compounds and their bonded-compound bits, wave-uniform (scalar registers); an
+    // empty slot gets a compound id no particle has, so it never matches
     int ci[CLUSTER];
     uint32_t bmi[CLUSTER];   // exclusion bits of slot s's compound, 0 when it has none
-    int ci_lo = 0x7fffffff, ci_hi = -0x7fffffff;
     uint32_t valid = 0;      // i slots holding a particle
 #pragma unroll
     for (int s = 0; s < CLUSTER; s++) {
-        ci[s] = a.comps[I * CLUSTER + s];
-        bmi[s] = (ci[s] >= 0 && ci[s] < a.nC) ? (uint32_t)a.bmask[ci[s]] : 0u;
-        if (ci[s] >= 0) {
-            valid |= 1u << s;
-            ci_lo = min(ci_lo, ci[s]);
-            ci_hi = max(ci_hi, ci[s]);
-        }
-    }
-    // the i-cluster's distinct compounds: id, the slots holding it, its bonded-compound bits
-    int dc_id[CLUSTER], ndc = 0;
-    uint32_t dc_slots[CLUSTER], dc_bm[CLUSTER];
-#pragma unroll
-    for (int s = 0; s < CLUSTER; s++) {
-        if (ci[s] < 0) continue;
-        bool seen = false;
-        for (int k = 0; k < ndc; k++)
-            if (dc_id[k] == ci[s]) { dc_slots[k] |= 1u << s; seen = true; }
-        if (!seen) { dc_id[ndc] = ci[s]; dc_slots[ndc] = 1u << s; dc_bm[ndc] = bmi[s]; ndc++; }
+        const int c = a.comps[I * CLUSTER + s];
+        bmi[s] = (c >= 0 && c < a.nC) ? (uint32_t)a.bmask[c] : 0u;
+        if (c >= 0) valid |= 1u << s;
+        ci[s] = c >= 0 ? c : -0x40000000;
     }
     const AxisRanges ry = axis_ranges(mn[1], mx[1], a.R, a.Lq[1], a.ncy, a.xb[1], a.rinv[1]);
     const AxisRanges rx = axis_ranges(mn[0], mx[0], a.R, a.Lq[0], a.ncx, a.xb[0], a.rinv[0]);
@@ -453,14 +440,19 @@ __global__ __launch_bounds__(256) void k_nlist(NlArgs a) {
                         far = g2 > a.F2;
                     }
                     if (in) {
-                        if (cb < ci_lo - 2 || cb > ci_hi + 2) {
-                            mask = valid;   // no slot's compound is cb or bonded to it
-                        } else {
+                        // a slot excludes cb only when its compound is within 2 of cb (LIMA's bonded
+                        // compounds are neighbours in the compound order): test that first
+                        bool close = false;
+#pragma unroll
+                        for (int s = 0; s < CLUSTER; s++) close = close || (uint32_t)(cb - ci[s] + 2) <= 4u;
+                        mask = valid;
+                        if (close) {
                             const bool cbin = cb >= 0 && cb < a.nC;
                             uint32_t excl = 0;
-                            for (int k = 0; k < ndc; k++) {
-                                const int dd = cb - dc_id[k];
-                                if (dd == 0 || (cbin && dd >= -2 && dd <= 2 && ((dc_bm[k] >> (dd + 2)) & 1u))) excl |= dc_slots[k];
+#pragma unroll
+                            for (int s = 0; s < CLUSTER; s++) {
+                                const int dd = cb - ci[s];
+                                if (dd == 0 || (cbin && (uint32_t)(dd + 2) <= 4u && ((bmi[s] >> (dd + 2)) & 1u))) excl |= 1u << s;
                             }
                             mask = valid & ~excl;
                         }
