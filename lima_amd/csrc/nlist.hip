@@ -442,17 +442,19 @@ __global__ __launch_bounds__(256) void k_nlist(NlArgs a) {
                     if (in) {
                         // a slot excludes cb only when its compound is within 2 of cb (LIMA's bonded
                         // compounds are neighbours in the compound order): test that first
-                        bool close = false;
+                        // (branch-free: bitwise, not short-circuit, logic)
+                        uint32_t close = 0u;
 #pragma unroll
-                        for (int s = 0; s < CLUSTER; s++) close = close || (uint32_t)(cb - ci[s] + 2) <= 4u;
+                        for (int s = 0; s < CLUSTER; s++) close |= (uint32_t)((uint32_t)(cb - ci[s] + 2) <= 4u) << s;
                         mask = valid;
                         if (close) {
-                            const bool cbin = cb >= 0 && cb < a.nC;
-                            uint32_t excl = 0;
+                            const uint32_t cbin = (uint32_t)(cb >= 0) & (uint32_t)(cb < a.nC);
+                            uint32_t excl = 0u;
 #pragma unroll
                             for (int s = 0; s < CLUSTER; s++) {
-                                const int dd = cb - ci[s];
-                                if (dd == 0 || (cbin && (uint32_t)(dd + 2) <= 4u && ((bmi[s] >> (dd + 2)) & 1u))) excl |= 1u << s;
+                                const uint32_t dd2 = (uint32_t)(cb - ci[s] + 2);   // 0..4 when close
+                                const uint32_t bonded = (bmi[s] >> (dd2 & 7u)) & cbin & ((close >> s) & 1u);
+                                excl |= ((uint32_t)(dd2 == 2u) | bonded) << s;
                             }
                             mask = valid & ~excl;
                         }
