@@ -37,16 +37,18 @@ def log(msg):
 
 
 def load_traffic(kernel, workload):
-    """HBM bytes per launch of `kernel`, from the committed counter summary
-    profiles/traffic_<kernel>_<workload>.txt (line 'traffic_bytes_per_launch = N'), or None."""
-    path = ROOT / "profiles" / f"traffic_{kernel}_{workload}.txt"
+    """HBM bytes per launch of `kernel` (k_pair, pme_chain per step, nlist_rebuild per rebuild) from the
+    committed counter summary profiles/traffic_<workload>.txt (line 'traffic_bytes_per_launch <kernel> = N',
+    FETCH_SIZE and WRITE_SIZE scaled by a copy of known size), or None."""
+    path = ROOT / "profiles" / f"traffic_{workload}.txt"
     if not path.exists():
         return None
     for line in path.read_text().splitlines():
-        if line.startswith("traffic_bytes_per_launch"):
+        head, _, val = line.partition("=")
+        if head.split() == ["traffic_bytes_per_launch", kernel]:
             try:
-                return int(float(line.split("=")[1]))
-            except ValueError:
+                return int(float(val.split()[0]))
+            except (ValueError, IndexError):
                 return None
     return None
 
