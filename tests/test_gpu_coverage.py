@@ -233,3 +233,26 @@ def test_list_stays_complete_when_atoms_move_fast():
           f"intervals undone, {s.list_violations} kept intervals over the bound, {s.n_list_rebuilds} rebuilds", flush=True)
     assert s.list_stale_rollbacks >= 1 and s.list_interval < 40
     assert s.list_violations == 0
+
+
+def test_coarse_pme_grid_matches_oracle():
+    """A coarse PME grid (0.3 nm spacing: 12^3 on the 3.104 nm water box, tiles of 6 grid points
+    holding ~375 charges each, where a 0.12 nm grid's tiles hold ~50): forces and energy of the start
+    state and a 10-step trajectory against the oracle, which derives the same grid."""
+    box = L.gen_water_lattice(10, 3.104, 0, 300.0)
+    p = L.default_params(dt_fs=1.0, pme_spacing_nm=0.3)
+    eng = L.Engine(box, p)
+    orc = Oracle(box, p)
+    assert list(orc.pme_dims()) == [12, 12, 12]
+    e_gpu = eng.compute_forces()
+    f_ref, _, e_ref = orc.forces()
+    err = rel_rms(layout_forces(box, eng.state()), f_ref)
+    eng.step(10)
+    orc.step(10)
+    d = (unwrapped_positions(box, eng.state()) - orc.positions()) * 1e-8
+    rmsd = float(np.sqrt((d ** 2).sum(1).mean()))
+    print(f"PME 12^3: force rel RMS {err:.3e}, E {e_gpu.potential:.6e} / {e_ref.potential:.6e}, 10-step RMSD {rmsd:.3e} nm",
+          flush=True)
+    assert err <= FORCE_TOL
+    assert abs(e_gpu.potential - e_ref.potential) <= 1e-6 * abs(e_ref.potential)
+    assert rmsd < 1e-5
