@@ -750,6 +750,23 @@ void pme_alloc(Engine& e, const std::vector<float>& q, const std::vector<uint4>&
             m.nt[d] = m.K[d] / best;
             if (m.K[d] < m.E[d] || m.nt[d] < 2) m.tiled = false;
         }
+        // large grids: grow the tiles (less halo per tile in the spread and gather blocks) while at
+        // least 2048 tiles remain to spread the work over the CUs
+        for (bool grew = m.tiled; grew;) {
+            grew = false;
+            for (int d = 0; d < 3; d++) {
+                int t = m.T[d] + 1;
+                while (t <= 12 && m.K[d] % t != 0) t++;
+                if (t > 12) continue;
+                const int nt = m.K[d] / t, E = t + 3 + 2 * m.margin;
+                const int64_t tiles = (int64_t)m.nt[0] * m.nt[1] * m.nt[2] / m.nt[d] * nt;
+                int64_t e3 = (int64_t)E;
+                for (int k = 0; k < 3; k++) if (k != d) e3 *= m.E[k];
+                if (tiles < 2048 || nt < 2 || m.K[d] < E || e3 * 8 > 56 * 1024) continue;
+                m.T[d] = t; m.E[d] = E; m.nt[d] = nt;
+                grew = true;
+            }
+        }
         if ((int64_t)m.E[0] * m.E[1] * m.E[2] * 8 > 56 * 1024) m.tiled = false;   // LDS block size
         m.ntiles = m.nt[0] * m.nt[1] * m.nt[2];
         if (m.tiled) {
