@@ -62,8 +62,10 @@ struct NList {
     // skip their arithmetic.
     int* ecount = nullptr;     // [ncl_max+1] near entries (padded)
     int* efar = nullptr;       // [ncl_max+1] far entries (padded), at the slice's end
-    float4* ent = nullptr;     // [nent_cap] (shift between the reference points nm xyz, bits: j slot | mask << 24)
-    uint8_t* ecode = nullptr;  // [nent_cap] image shift code (sx + 3 sy + 9 sz, each 0..2 for -1, 0, +1 box)
+    // [nent_cap] x: j slot | mask << 24;  y: image shift code sx + 3 sy + 9 sz (each 0..2 for -1, 0, +1
+    // box) in bits 0-7, and per axis d a -1 box bit (8 + 2d) and a +1 box bit (9 + 2d).  The pair
+    // kernel forms the shift between the two clusters' reference points from these bits and cref.
+    uint2* ent = nullptr;
 };
 
 struct Pme {

@@ -1427,25 +1427,21 @@ static HostList read_list(Engine& e) {
     const int ncl = nlist_count(e);
     const int64_t cap = e.nl.cap;
     std::vector<int> cnt(ncl), far(ncl);
-    std::vector<float4> ent((size_t)ncl * cap);
-    std::vector<uint8_t> code((size_t)ncl * cap);
+    std::vector<uint2> ent((size_t)ncl * cap);
     L.catoms.resize((size_t)ncl * CLUSTER);
     HIP_CHECK(hipMemcpy(L.catoms.data(), e.nl.catoms, L.catoms.size() * sizeof(int), hipMemcpyDeviceToHost));
     if (ncl) {
         HIP_CHECK(hipMemcpy(cnt.data(), e.nl.ecount, ncl * sizeof(int), hipMemcpyDeviceToHost));
         HIP_CHECK(hipMemcpy(far.data(), e.nl.efar, ncl * sizeof(int), hipMemcpyDeviceToHost));
-        HIP_CHECK(hipMemcpy(ent.data(), e.nl.ent, ent.size() * sizeof(float4), hipMemcpyDeviceToHost));
-        HIP_CHECK(hipMemcpy(code.data(), e.nl.ecode, code.size(), hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(ent.data(), e.nl.ent, ent.size() * sizeof(uint2), hipMemcpyDeviceToHost));
     }
     L.off.assign(ncl + 1, 0);
     for (int I = 0; I < ncl; I++) {
         L.off[I] = (int64_t)L.ent.size();
         auto take = [&](int64_t k) {
-            uint32_t bits;
-            std::memcpy(&bits, &ent[k].w, 4);
-            const uint32_t m = bits >> 24;
+            const uint32_t bits = ent[k].x, m = bits >> 24;
             if (!m) return;
-            L.ent.push_back((uint32_t)L.catoms[bits & 0xffffffu] | ((uint32_t)code[k] << 24));
+            L.ent.push_back((uint32_t)L.catoms[bits & 0xffffffu] | ((ent[k].y & 0xffu) << 24));
             L.mask.push_back((uint8_t)m);
         };
         for (int64_t k = I * cap; k < I * cap + cnt[I]; k++) take(k);

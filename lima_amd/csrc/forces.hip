@@ -96,7 +96,9 @@ struct PairArgs {
     const float2* ljs;
     const int* ecount;   // near entries of each cluster (front of its slice)
     const int* efar;     // far entries (back of its slice)
-    const float4* ent;
+    const uint2* ent;    // list entries: x = j slot | mask << 24, y = shift code | -1 / +1 box bits of axis d at 8 + 2d / 9 + 2d
+    const uint4* cref;   // cluster reference points (integer)
+    uint32_t lq[3];      // box lengths, integer units (mod 2^32)
     float* fpair;    // [3][nslots]
     float* epair;    // [nslots]
     double* epart;   // this kernel's energy region: one record per wave
@@ -107,8 +109,10 @@ struct PairArgs {
 // force is accumulated: no atomics and no cross-lane traffic inside the loop, and the pair
 // forces are deterministic.
 //
-// Entries are staged through LDS 64 at a time: lane l loads entry l of the chunk and its j
-// particle with one coalesced load each, adds the image shift to the j position and stores
+// Entries are staged through LDS 64 at a time: lane l loads entry l of the chunk, its j particle
+// and the j cluster's reference point, forms the shift between the two clusters' reference
+// points (an exact integer difference, rounded once to float: the value the list build would
+// compute), adds it to the j position (stored relative to its own reference point) and stores
 // (x, y, z, q) and (sigma/2, sqrt(24 eps), mask bits) in the wave's LDS slice.  The 8 lanes of a
 // group then read their entry with two broadcast ds_read_b128, so the texture path sees each
 // datum once instead of eight times.  Two LDS buffers: chunk c + 1 is loaded while c is computed.
@@ -155,6 +159,14 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs a) {
         }
     }
     const float rcut = sqrtf(a.rc2) + 1e-4f, rcut2 = rcut * rcut;
+    // this cluster's reference point; the shift of an entry is ref_J - ref_I + (p - n) L per axis
+    // from its +1 / -1 box bits, in 32-bit wrap-around arithmetic, which is exact since every
+    // wanted difference is far below 2^31 units.  Masks, not selects: a select between values
+    // the lambdas capture by reference becomes a load through a selected pointer, which forces
+    // them into scratch memory.
+    uint4 ri = live ? a.cref[I] : make_uint4(0u, 0u, 0u, 0u);
+    ri = make_uint4(__builtin_amdgcn_readfirstlane(ri.x), __builtin_amdgcn_readfirstlane(ri.y), __builtin_amdgcn_readfirstlane(ri.z), 0u);   // wave-uniform
+    const uint32_t lqx = a.lq[0], lqy = a.lq[1], lqz = a.lq[2];
     float fx = 0.f, fy = 0.f, fz = 0.f;
     double elj = 0.0, ecl = 0.0;   // FP64 energy accumulation (energy steps only)
     double vir[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};   // -1/2 S d (x) d per evaluated pair (energy steps only)
@@ -167,25 +179,37 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs a) {
     const int nent = (it1 - it0) * CLUSTER;          // entries of this wave (multiple of 8)
     const int nchunk = (nent + CHUNK - 1) / CHUNK;   // wave-uniform
 
-    float4 ga = make_float4(0.f, 0.f, 0.f, 0.f), gb = make_float4(0.f, 0.f, 0.f, 0.f);
+    // raw loads of the chunk in flight: j position, LJ pair, j cluster reference point (xyz) and
+    // the entry's shift code | mask << 24; the shift is formed in stage(), after the compute
+    // that the loads overlap
+    float4 gxj = make_float4(0.f, 0.f, 0.f, 0.f);
+    float2 gpj = make_float2(0.f, 0.f);
+    uint4 grj = make_uint4(0u, 0u, 0u, 0u);
     auto fetch = [&](int c) {   // global loads of chunk c into registers (lane l: entry l)
         const int k = c * CHUNK + lane;
         if (k < nent) {
             const int v = v0 + k;
-            const float4 rec = a.ent[base + (v < nn ? v : a.cap - nf + (v - nn))];
-            const uint32_t bits = __float_as_uint(rec.w);
-            const int b = (int)(bits & 0xffffffu);
-            const float4 xj = a.xq[b];
-            const float2 pj = a.ljs[b];
-            ga = make_float4(xj.x + rec.x, xj.y + rec.y, xj.z + rec.z, xj.w);
-            gb = make_float4(pj.x, pj.y, __uint_as_float(bits >> 24), 0.f);
-        } else {
-            ga = make_float4(PAD_POS, PAD_POS, PAD_POS, 0.f);
-            gb = make_float4(0.f, 0.f, __uint_as_float(0u), 0.f);
+            const uint2 rec = a.ent[base + (v < nn ? v : a.cap - nf + (v - nn))];
+            const int b = (int)(rec.x & 0xffffffu);
+            const uint4 rj = a.cref[b / CLUSTER];
+            gxj = a.xq[b];
+            gpj = a.ljs[b];
+            grj = make_uint4(rj.x, rj.y, rj.z, (rec.y & 0x3fffu) | (rec.x & 0xff000000u));
+        } else {   // past the end: far away, mask 0, no shift
+            gxj = make_float4(PAD_POS, PAD_POS, PAD_POS, 0.f);
+            gpj = make_float2(0.f, 0.f);
+            grj = make_uint4(ri.x, ri.y, ri.z, 13u);
         }
     };    // the fetched chunk's entries inside the cluster's reach, compacted into LDS buffer buf and padded
     // to whole passes; returns how many passes they fill (wave-uniform)
     auto stage = [&](int buf) {
+        const uint32_t cw = grj.w;
+        auto bitmask = [cw](int b) { return (uint32_t)((int32_t)(cw << (31 - b)) >> 31); };   // bit b: 0 or all ones
+        const float shx = (float)(int)(grj.x - ri.x + (lqx & bitmask(9)) - (lqx & bitmask(8))) * LIMA_LIMA_TO_NANO_F;
+        const float shy = (float)(int)(grj.y - ri.y + (lqy & bitmask(11)) - (lqy & bitmask(10))) * LIMA_LIMA_TO_NANO_F;
+        const float shz = (float)(int)(grj.z - ri.z + (lqz & bitmask(13)) - (lqz & bitmask(12))) * LIMA_LIMA_TO_NANO_F;
+        const float4 ga = make_float4(gxj.x + shx, gxj.y + shy, gxj.z + shz, gxj.w);
+        const float4 gb = make_float4(gpj.x, gpj.y, __uint_as_float(cw >> 24), 0.f);
         const float gx = fmaxf(fabsf(ga.x - bc[0]) - bh[0], 0.f), gy = fmaxf(fabsf(ga.y - bc[1]) - bh[1], 0.f),
                     gz = fmaxf(fabsf(ga.z - bc[2]) - bh[2], 0.f);
         const bool keep = gx * gx + gy * gy + gz * gz <= rcut2;   // entries past the end sit at PAD_POS
@@ -659,6 +683,8 @@ void launch_pair(Engine& e, bool energy, hipStream_t s) {
     a.coul_tab = e.coul_tab;
     a.cap = e.nl.cap;
     a.xq = e.nl.xq; a.ljs = e.nl.ljs; a.ecount = e.nl.ecount; a.efar = e.nl.efar; a.ent = e.nl.ent;
+    a.cref = e.nl.cref;
+    for (int d = 0; d < 3; d++) a.lq[d] = (uint32_t)e.Lq[d];
     a.fpair = e.fpair; a.epair = e.epair; a.epart = e.epart + (int64_t)e.ereg.pair * ACC_N;
     a.split = pair_split(e);
     const int64_t waves = (int64_t)a.ncl_max * a.split;

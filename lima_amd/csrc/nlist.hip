@@ -214,8 +214,7 @@ struct NlArgs {
     int* ecount;     // near entries of each cluster (padded)
     int* efar;       // far entries (padded), at the end of the slice
     long long F2;    // cutoff^2 in Coord^2: an entry whose image lies farther from the i box is far
-    float4* ent;
-    uint8_t* ecode;
+    uint2* ent;
 };
 
 // range of column indices whose [lo, hi] interval (after shift) is non-empty
@@ -317,7 +316,7 @@ __global__ __launch_bounds__(256) void k_nlist(NlArgs a) {
     if (I >= min(*a.ncl_dev, a.ncl_max)) return;   // uniform per wave
     uint32_t* flat = flat_all[threadIdx.x >> 6];   // candidates that passed the box test
     uint32_t* cand = cand_all[threadIdx.x >> 6];   // all candidates of the current chunk
-    const uint4 bmn = a.cmin[I], bmx = a.cmax[I], refI = a.cref[I];
+    const uint4 bmn = a.cmin[I], bmx = a.cmax[I];
     const long long mn[3] = {(long long)bmn.x, (long long)bmn.y, (long long)bmn.z};
     const long long mx[3] = {(long long)bmx.x, (long long)bmx.y, (long long)bmx.z};
     // the i slots' compounds and their bonded-compound bits, wave-uniform (scalar registers); an
@@ -468,22 +467,12 @@ __global__ __launch_bounds__(256) void k_nlist(NlArgs a) {
                     const int pos = far ? cfar + __popcll(bf & lt) : cnear + __popcll(bn & lt);
                     if (pos < a.cap) {   // near from the front, far from the back; a collision is an overflow
                         const int64_t idx = base + (far ? a.cap - 1 - pos : pos);
-                        // shift between the cluster reference points (exact integer, then one rounding)
                         int cxi, cyi, czi;
                         shift_code_axes(cd, &cxi, &cyi, &czi);
-                        const uint4 rj = a.cref[J];
-                        float sxf, syf, szf;
-                        if (NARROW) {
-                            sxf = (float)(int)(rj.x - refI.x + (uint32_t)shift_of(cxi, a.Lq[0])) * LIMA_LIMA_TO_NANO_F;
-                            syf = (float)(int)(rj.y - refI.y + (uint32_t)shift_of(cyi, a.Lq[1])) * LIMA_LIMA_TO_NANO_F;
-                            szf = (float)(int)(rj.z - refI.z + (uint32_t)shift_of(czi, a.Lq[2])) * LIMA_LIMA_TO_NANO_F;
-                        } else {
-                            sxf = (float)(double)((long long)rj.x - (long long)refI.x + shift_of(cxi, a.Lq[0])) * LIMA_LIMA_TO_NANO_F;
-                            syf = (float)(double)((long long)rj.y - (long long)refI.y + shift_of(cyi, a.Lq[1])) * LIMA_LIMA_TO_NANO_F;
-                            szf = (float)(double)((long long)rj.z - (long long)refI.z + shift_of(czi, a.Lq[2])) * LIMA_LIMA_TO_NANO_F;
-                        }
-                        a.ent[idx] = make_float4(sxf, syf, szf, __uint_as_float((uint32_t)(J * CLUSTER + sl) | (mask << 24)));
-                        a.ecode[idx] = (uint8_t)cd;
+                        a.ent[idx] = make_uint2((uint32_t)(J * CLUSTER + sl) | (mask << 24),
+                                                (uint32_t)cd | (uint32_t)(cxi == 0) << 8 | (uint32_t)(cxi == 2) << 9 |
+                                                    (uint32_t)(cyi == 0) << 10 | (uint32_t)(cyi == 2) << 11 |
+                                                    (uint32_t)(czi == 0) << 12 | (uint32_t)(czi == 2) << 13);
                     }
                 }
                 cnear += __popcll(bn);
@@ -505,15 +494,10 @@ __global__ __launch_bounds__(256) void k_nlist(NlArgs a) {
         }
         return;
     }
-    // padding entries: the cluster's own first slot, mask 0, no shift
-    for (int k = cnear + lane; k < pn; k += 64) {
-        a.ent[base + k] = make_float4(0.f, 0.f, 0.f, __uint_as_float((uint32_t)(I * CLUSTER)));
-        a.ecode[base + k] = 13;
-    }
-    for (int k = cfar + lane; k < pf; k += 64) {
-        a.ent[base + a.cap - 1 - k] = make_float4(0.f, 0.f, 0.f, __uint_as_float((uint32_t)(I * CLUSTER)));
-        a.ecode[base + a.cap - 1 - k] = 13;
-    }
+    // padding entries: the cluster's own first slot, mask 0, no image shift
+    const uint2 pad = make_uint2((uint32_t)(I * CLUSTER), 13u);
+    for (int k = cnear + lane; k < pn; k += 64) a.ent[base + k] = pad;
+    for (int k = cfar + lane; k < pf; k += 64) a.ent[base + a.cap - 1 - k] = pad;
     if (lane == 0) {
         a.ecount[I] = pn;
         a.efar[I] = pf;
@@ -613,14 +597,13 @@ void nlist_alloc(Engine& e) {
     nl.nent_cap = 0;
     nl.cap = e.prm.nlist_capacity > 0 ? (e.prm.nlist_capacity + 7) & ~7 : 2048;
     nl.ent = nullptr;
-    nl.ecode = nullptr;
 }
 
 void nlist_free(Engine& e) {
     NList& nl = e.nl;
     void* ptrs[] = {nl.col_of, nl.col_count, nl.col_start, nl.col_fill, nl.keys, nl.col_ncl, nl.catoms, nl.slot_of,
                     nl.ccol, nl.cref, nl.cmin, nl.cmax, nl.ljs, nl.comps, nl.cw, nl.xb[0], nl.xb[1], nl.xq, nl.ecount, nl.efar,
-                    nl.ent, nl.ecode, e.fpair, e.epair, e.fintra, e.eintra, e.fb, e.eb, e.fpme, e.epme, e.ephi};
+                    nl.ent, e.fpair, e.epair, e.fintra, e.eintra, e.fb, e.eb, e.fpme, e.epme, e.ephi};
     for (void* p : ptrs) if (p) (void)hipFree(p);
 }
 
@@ -821,14 +804,12 @@ void nlist_build(Engine& e, const uint4* w, bool checked) {
             HIP_CHECK(hipStreamSynchronize(e.s_pme));
             HIP_CHECK(hipStreamSynchronize(e.s_local));
             if (nl.ent) HIP_CHECK(hipFree(nl.ent));
-            if (nl.ecode) HIP_CHECK(hipFree(nl.ecode));
-            nl.ent = nullptr; nl.ecode = nullptr;
-            HIP_CHECK(hipMalloc(&nl.ent, need * sizeof(float4)));
-            HIP_CHECK(hipMalloc(&nl.ecode, need));
+            nl.ent = nullptr;
+            HIP_CHECK(hipMalloc(&nl.ent, need * sizeof(uint2)));
             nl.nent_cap = need;
             e.step_graph_valid = false;   // the captured pair kernel names the old entry buffer
         }
-        a.cap = nl.cap; a.ent = nl.ent; a.ecode = nl.ecode;
+        a.cap = nl.cap; a.ent = nl.ent;
         if (narrow_coords(e)) hipLaunchKernelGGL(k_nlist<true>, dim3(nblk(nl.ncl_max, 4)), dim3(256), 0, s, a);
         else hipLaunchKernelGGL(k_nlist<false>, dim3(nblk(nl.ncl_max, 4)), dim3(256), 0, s, a);
         if (!checked) {
