@@ -83,6 +83,10 @@ __device__ __forceinline__ int coord_of(float nm) {   // Coord{Float3}: static_c
 template <bool ENERGY>
 __global__ void k_integrate(IntegrateArgs a) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t wv = p >> 6;
+    const bool lead = (threadIdx.x & 63) == 0 && (p & ~63ll) < a.N;   // lane 0 of a wave inside the per-wave arrays
+    // the wave's running displacement maxima, loaded first: independent of every other load
+    const float2 dp_old = lead ? a.dpart[wv] : make_float2(0.f, 0.f);
     float ke = 0.f, d2 = 0.f;
     uint4 bin_w = make_uint4(0u, 0u, 0u, 0u);
     bool bin_on = false;   // pack: this particle goes to a PME tile bin
@@ -145,15 +149,10 @@ __global__ void k_integrate(IntegrateArgs a) {
     const float m1 = wave_max(d2);
     const bool top = d2 == m1;
     const float m2 = __popcll(__ballot(top)) > 1 ? m1 : wave_max(top ? 0.f : d2);
-    const int64_t wv = p >> 6;
-    const bool wave_in = (p & ~63ll) < a.N;   // waves inside the per-wave arrays
-    if ((threadIdx.x & 63) == 0 && wave_in) {
-        const float2 o = a.dpart[wv];
-        a.dpart[wv] = make_float2(fmaxf(o.x, m1), fmaxf(o.y, m2));
-    }
+    if (lead) a.dpart[wv] = make_float2(fmaxf(dp_old.x, m1), fmaxf(dp_old.y, m2));
     if (ENERGY) {
         const double t = wave_sum_d((double)ke);
-        if ((threadIdx.x & 63) == 0 && wave_in) {
+        if (lead) {
             double* r = a.epart + wv * ACC_N;
             write_record(r, 0.0, 0.0, 0.0, t, 0.0, nullptr);
         }

@@ -127,6 +127,13 @@ __device__ float2* fft_lines(float2* a, float2* b, int nl, const FftPlan& P, boo
     return a;
 }
 
+// round(y) as int64 (nearest, ties to even) for |y| < 2^51: y + 1.5 2^52 in FP64 leaves the rounded
+// integer in the low mantissa bits.  Four instructions instead of the generic float -> int64 sequence.
+__device__ __forceinline__ long long fixed_round(float y) {
+    const double m = 6755399441055744.0;   // 1.5 * 2^52
+    return __double_as_longlong((double)y + m) - __double_as_longlong(m);
+}
+
 struct SpreadArgs {
     int64_t N;
     long long Lq[3];
@@ -156,7 +163,7 @@ __global__ void k_spread(SpreadArgs a) {
             long long* row = a.qgrid + ((int64_t)Z * a.K[1] + Y) * a.K[0];
             for (int x = 0; x < 4; x++) {
                 const int X = wrapi(ix[0] - 1 + x, a.K[0]);
-                const long long v = __float2ll_rn(qyz * wx[x] * 4294967296.0f);
+                const long long v = fixed_round(qyz * wx[x] * 4294967296.0f);
                 atomicAdd((unsigned long long*)&row[X], (unsigned long long)v);
             }
         }
@@ -213,7 +220,7 @@ __global__ __launch_bounds__(256) void k_spread_tiles(TileArgs a) {
                 const float qyz = q * wy[b] * wz[c];
                 unsigned long long* row = blk + ((l[2] + c) * Ey + (l[1] + b)) * Ex + l[0];
                 for (int x = 0; x < 4; x++)
-                    atomicAdd(&row[x], (unsigned long long)__float2ll_rn(qyz * wx[x] * 4294967296.0f));
+                    atomicAdd(&row[x], (unsigned long long)fixed_round(qyz * wx[x] * 4294967296.0f));
             }
     }
     __syncthreads();
@@ -461,14 +468,17 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
                     const float wyz = w6y[my] * w6z[mz];
                     const float dyz = d6y[my] * w6z[mz];
                     const float wdz = w6y[my] * d6z[mz];
+                    float rw = 0.f, rd = 0.f;   // the same per-row order as the tiled gather
                     for (int mx = 0; mx < 6; mx++) {
                         const int X = wrapi(wrapi(ix[0] - 2 + mx, a.K[0]), a.K[0]);
                         const float v = row[X];
-                        ph += w6x[mx] * wyz * v;
-                        ex += d6x[mx] * wyz * v;
-                        ey += w6x[mx] * dyz * v;
-                        ez += w6x[mx] * wdz * v;
+                        rw = fmaf(w6x[mx], v, rw);
+                        rd = fmaf(d6x[mx], v, rd);
                     }
+                    if (ENERGY) ph = fmaf(wyz, rw, ph);
+                    ex = fmaf(wyz, rd, ex);
+                    ey = fmaf(dyz, rw, ey);
+                    ez = fmaf(wdz, rw, ez);
                 }
             }
             const int s = a.slot_of[p];
@@ -581,18 +591,23 @@ __global__ __launch_bounds__(256) void k_gather_tiles(GatherTileArgs a) {
             const int l0 = ix[0] - tx * a.T[0], l1 = ix[1] - ty * a.T[1], l2 = ix[2] - tz * a.T[2];
             if (j < 6) {
                 const float wz6 = pick6(w6z, j), dz6 = pick6(d6z, j);
+                // per row: the x sums with the weights and their differences (two FMAs per node),
+                // then the row's y and z factors
 #pragma unroll
                 for (int my = 0; my < 6; my++) {
                     const float wyz = w6y[my] * wz6, dyz = d6y[my] * wz6, wdz = w6y[my] * dz6;
                     const float* rowp = phib + ((l2 + j) * G1 + (l1 + my)) * G0 + l0;
+                    float rw = 0.f, rd = 0.f;
 #pragma unroll
                     for (int mx = 0; mx < 6; mx++) {
                         const float v = rowp[mx];
-                        ph += w6x[mx] * wyz * v;
-                        ex += d6x[mx] * wyz * v;
-                        ey += w6x[mx] * dyz * v;
-                        ez += w6x[mx] * wdz * v;
+                        rw = fmaf(w6x[mx], v, rw);
+                        rd = fmaf(d6x[mx], v, rd);
                     }
+                    if (ENERGY) ph = fmaf(wyz, rw, ph);
+                    ex = fmaf(wyz, rd, ex);
+                    ey = fmaf(dyz, rw, ey);
+                    ez = fmaf(wdz, rw, ez);
                 }
             }
         }
