@@ -102,6 +102,8 @@ struct BatchArgs {
     float2* spec;
     float4* part;
     const float2* twid;
+    const float4* ctab;          // Coulomb factors of the cutoff pairs in r (see CTAB_N), staged in LDS
+    float ctab_scale;            // CTAB_N / cutoff
     SysOut* out;
     PairConst pc;
     float rc, rc2;
@@ -124,6 +126,11 @@ constexpr int REC_W = 48;              // floats per record: 32 features, step (
 constexpr float HIST_R = 1.0f;         // neighbour histogram radius [nm]
 constexpr int HIST_BINS = 14;          // bins per charge sign
 constexpr int HIST_MAX_ATOMS = 4096;   // byte counters in LDS: 28 per atom
+// Cutoff pairs read erfc(kappa r) and H(r) = erfc(kappa r) + (2 kappa / sqrt(PI_L)) r exp(-kappa^2 r^2)
+// from a table in r: CTAB_N intervals on [0, cutoff], per interval two cubic Hermite polynomials in
+// the local coordinate t in [0, 1) (erfc, then H; values and derivatives in double at the nodes),
+// plus the values at r = cutoff.  Both are analytic in r, so the interpolation error is ~1e-10.
+constexpr int CTAB_N = 256;
 
 
 __device__ __forceinline__ int fmodi(int x, int L) {
@@ -147,12 +154,32 @@ __device__ __forceinline__ Dq bdispq(const int4* u, int a, int b, const SysMeta&
 }
 __device__ __forceinline__ v3 bdisp(const int4* u, int a, int b, const SysMeta& m) { return dq_nm(bdispq(u, a, b, m)); }
 
+// pair_scalar_cut with the Coulomb factors read from the LDS table in r (r2 <= cutoff^2):
+// force scalar S (force on i = d S) and the LJ and Coulomb energies of the pair.
+__device__ __forceinline__ float pair_scalar_rtab(float r2, float sig, float eps24, float qq_kc, const float4* __restrict__ tab,
+                                                  float rscale, float* e_lj, float* e_c) {
+    const float rinv = __builtin_amdgcn_rsqf(r2);
+    const float rinv2 = rinv * rinv;
+    float s = sig * sig * rinv2;
+    s = s * s * s;
+    const float x = (r2 * rinv) * rscale;
+    const int k = (int)x;
+    const float t = __builtin_amdgcn_fractf(x);
+    const float4 ka = tab[2 * k], kb = tab[2 * k + 1];
+    const float A = fmaf(fmaf(fmaf(ka.w, t, ka.z), t, ka.y), t, ka.x);   // erfc(kappa r)
+    const float H = fmaf(fmaf(fmaf(kb.w, t, kb.z), t, kb.y), t, kb.x);
+    const float qr = qq_kc * rinv;
+    *e_lj = (eps24 * (1.0f / 6.0f)) * s * (s - 1.0f);
+    *e_c = qr * A;
+    return rinv2 * fmaf(eps24 * s, fmaf(-2.f, s, 1.f), -(qr * H));
+}
+
 // One cutoff pair computed once: force on i += d S, on j -= d S, the whole pair energy.
 // pj = (q, sigma/2, sqrt(24 eps)) of j.
 __device__ __forceinline__ void pair_eval(float x, float y, float z, float r2, const float4& pi, float qj, float sj, float ej,
-                                          const PairConst& pc, v3& fi, v3& fj, float& elj, float& ec) {
+                                          const PairConst& pc, const float4* tab, float rscale, v3& fi, v3& fj, float& elj, float& ec) {
     float e1, e2;
-    const float S = pair_scalar_cut(r2, pi.y + sj, pi.w * ej, pi.x * qj * pc.kc, pc, &e1, &e2);
+    const float S = pair_scalar_rtab(r2, pi.y + sj, pi.w * ej, pi.x * qj * pc.kc, tab, rscale, &e1, &e2);
     // explicit FMAs: the same rounding whatever the unrolling or scheduling
     fi.x = fmaf(x, S, fi.x); fi.y = fmaf(y, S, fi.y); fi.z = fmaf(z, S, fi.z);
     fj.x = fmaf(-x, S, fj.x); fj.y = fmaf(-y, S, fj.y); fj.z = fmaf(-z, S, fj.z);
@@ -163,23 +190,26 @@ __device__ __forceinline__ void pair_eval(float x, float y, float z, float r2, c
 // Box constants of one system held in registers for the pair loop.
 struct PairBox {
     int Lx, Ly, Lz, hx, hy, hz;
-    float fx, fy, fz;
+    float fx, fy, fz;      // box edges [nm]
+    float ix, iy, iz;      // their reciprocals
 };
 
 // Cutoff pair (i, j) over every periodic image within the cutoff.  `excl` marks a pair of one
 // compound or of bonded compounds (left to the intra pass).  One predicate guards the pair
 // arithmetic, so a wave branches once per step.
 template <bool MULTI>
-__device__ __forceinline__ void pair_cut(const int4& wi, const float4& pi, bool excl, int jx, int jy, int jz, float qj, float sj,
-                                         float ej, const PairBox& bx, const BatchArgs& a, v3& fi, v3& fj, float& elj, float& ec) {
-    int dx = jx - wi.x, dy = jy - wi.y, dz = jz - wi.z;
-    dx += dx < -bx.hx ? bx.Lx : 0; dx -= dx >= bx.hx ? bx.Lx : 0;
-    dy += dy < -bx.hy ? bx.Ly : 0; dy -= dy >= bx.hy ? bx.Ly : 0;
-    dz += dz < -bx.hz ? bx.Lz : 0; dz -= dz >= bx.hz ? bx.Lz : 0;
-    const float x = (float)dx * LIMA_LIMA_TO_NANO_F, y = (float)dy * LIMA_LIMA_TO_NANO_F, z = (float)dz * LIMA_LIMA_TO_NANO_F;
+// Positions are the wrapped Coord in nm as floats (rounding ~2e-7 nm at these box sizes); the
+// nearest image is d - L rint(d / L), four instructions per axis.
+__device__ __forceinline__ void pair_cut(const float3& wi, const float4& pi, bool excl, float jx, float jy, float jz, float qj,
+                                         float sj, float ej, const PairBox& bx, const BatchArgs& a, const float4* tab, v3& fi, v3& fj,
+                                         float& elj, float& ec) {
+    float x = jx - wi.x, y = jy - wi.y, z = jz - wi.z;
+    x = fmaf(-bx.fx, rintf(x * bx.ix), x);
+    y = fmaf(-bx.fy, rintf(y * bx.iy), y);
+    z = fmaf(-bx.fz, rintf(z * bx.iz), z);
     if (!MULTI) {
         const float r2 = x * x + y * y + z * z;
-        if (!excl && r2 <= a.rc2) pair_eval(x, y, z, r2, pi, qj, sj, ej, a.pc, fi, fj, elj, ec);
+        if (!excl && r2 <= a.rc2) pair_eval(x, y, z, r2, pi, qj, sj, ej, a.pc, tab, a.ctab_scale, fi, fj, elj, ec);
     } else if (!excl) {
         // second candidate image per axis: the one on the other side, when within the cutoff
         const float ax = x - copysignf(bx.fx, x), ay = y - copysignf(bx.fy, y), az = z - copysignf(bx.fz, z);
@@ -191,7 +221,7 @@ __device__ __forceinline__ void pair_cut(const int4& wi, const float4& pi, bool 
                 for (int kx = 0; kx < nx; kx++) {
                     const float xx = kx ? ax : x;
                     const float r2 = xx * xx + yy * yy + zz * zz;
-                    if (r2 <= a.rc2) pair_eval(xx, yy, zz, r2, pi, qj, sj, ej, a.pc, fi, fj, elj, ec);
+                    if (r2 <= a.rc2) pair_eval(xx, yy, zz, r2, pi, qj, sj, ej, a.pc, tab, a.ctab_scale, fi, fj, elj, ec);
                 }
             }
         }
@@ -309,6 +339,7 @@ struct BlockShared {
     float fres;
     float2 tw[3][KMAX];
     unsigned long long ph[7];    // phase ticks seen by thread 0
+    float4 ctab[2 * (CTAB_N + 1)];   // the Coulomb table of the cutoff pairs
 };
 
 // FP64 sums of v[0..3] and max of fm over the workgroup, in a fixed order; every thread gets them.
@@ -494,7 +525,8 @@ __device__ __forceinline__ Eval evaluate(const BatchArgs& a, const SysMeta& m, B
         const int NI = NT * (NT + 1) / 2;
         float* home = reinterpret_cast<float*>(lds) + wave * 192;   // 64 x 3 floats per wave
         float4* part = a.part + m.part0;
-        const PairBox bx = {m.Lq[0], m.Lq[1], m.Lq[2], m.hLq[0], m.hLq[1], m.hLq[2], m.L[0], m.L[1], m.L[2]};
+        const PairBox bx = {m.Lq[0], m.Lq[1], m.Lq[2], m.hLq[0], m.hLq[1], m.hLq[2], m.L[0], m.L[1], m.L[2],
+                            1.f / m.L[0], 1.f / m.L[1], 1.f / m.L[2]};
         const int q0 = (int)((long long)NI * wave / W), q1 = (int)((long long)NI * (wave + 1) / W);
         int I = 0, J = 0;
         {
@@ -503,7 +535,7 @@ __device__ __forceinline__ Eval evaluate(const BatchArgs& a, const SysMeta& m, B
             J = I + q;
         }
         int curI = -1;
-        int4 wi = make_int4(0, 0, 0, 0);
+        float3 wi = make_float3(0.f, 0.f, 0.f);
         float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
         int ci = 0, bmi = 0;
         bool vi = false;
@@ -515,20 +547,24 @@ __device__ __forceinline__ Eval evaluate(const BatchArgs& a, const SysMeta& m, B
                 if (vi) {
                     const int4 uu = u[m.atom0 + li];
                     ci = a.comp[m.atom0 + li];
-                    wi = make_int4(fmodi(uu.x, bx.Lx), fmodi(uu.y, bx.Ly), fmodi(uu.z, bx.Lz), 0);
+                    wi = make_float3((float)fmodi(uu.x, bx.Lx) * LIMA_LIMA_TO_NANO_F, (float)fmodi(uu.y, bx.Ly) * LIMA_LIMA_TO_NANO_F,
+                                     (float)fmodi(uu.z, bx.Lz) * LIMA_LIMA_TO_NANO_F);
                     pi = a.prm[m.atom0 + li];
                     bmi = a.cbm[m.comp0 + ci];
                 }
             }
             // atom J*64+lane and its accumulator start in this lane
-            int jx = 0, jy = 0, jz = 0, jc = -1;
+            float jx = 0.f, jy = 0.f, jz = 0.f;
+            int jc = -1;
             float jq = 0.f, js = 0.f, je = 0.f;
             {
                 const int lj = J * 64 + lane;
                 if (lj < n) {
                     const int4 uu = u[m.atom0 + lj];
                     const float4 pj = a.prm[m.atom0 + lj];
-                    jx = fmodi(uu.x, bx.Lx); jy = fmodi(uu.y, bx.Ly); jz = fmodi(uu.z, bx.Lz);
+                    jx = (float)fmodi(uu.x, bx.Lx) * LIMA_LIMA_TO_NANO_F;
+                    jy = (float)fmodi(uu.y, bx.Ly) * LIMA_LIMA_TO_NANO_F;
+                    jz = (float)fmodi(uu.z, bx.Lz) * LIMA_LIMA_TO_NANO_F;
                     jc = a.comp[m.atom0 + lj];
                     jq = pj.x; js = pj.y; je = pj.w;
                 }
@@ -538,16 +574,30 @@ __device__ __forceinline__ Eval evaluate(const BatchArgs& a, const SysMeta& m, B
             const bool diag = I == J;
             const bool near = J - I <= 2;   // compounds hold <= 32 atoms: intra pairs lie within two tiles
             const int ns = diag ? 33 : 64;  // diagonal tile: steps 1..32 (step 32 on half the lanes)
+            const float4* tab = sh.ctab;
+            if (near) {
 #pragma unroll UNR
-            for (int st = 0; st < ns; st++) {
-                const int dc = jc - ci;
-                const bool excl = !vi || jc < 0 || (diag && (st == 0 || (st == 32 && lane >= 32))) ||
-                                  (near && (unsigned)(dc + 2) <= 4u && ((bmi >> (dc + 2)) & 1));
-                pair_cut<MULTI>(wi, pi, excl, jx, jy, jz, jq, js, je, bx, a, fi, fj, elj, ec);
-                if (st + 1 < ns) {
-                    jx = rot1(jx); jy = rot1(jy); jz = rot1(jz); jc = rot1(jc);
-                    jq = rot1f(jq); js = rot1f(js); je = rot1f(je);
-                    fj.x = rot1f(fj.x); fj.y = rot1f(fj.y); fj.z = rot1f(fj.z);
+                for (int st = 0; st < ns; st++) {
+                    const int dc = jc - ci;
+                    const bool excl = !vi || jc < 0 || (diag && (st == 0 || (st == 32 && lane >= 32))) ||
+                                      ((unsigned)(dc + 2) <= 4u && ((bmi >> (dc + 2)) & 1));
+                    pair_cut<MULTI>(wi, pi, excl, jx, jy, jz, jq, js, je, bx, a, tab, fi, fj, elj, ec);
+                    if (st + 1 < ns) {
+                        jx = rot1f(jx); jy = rot1f(jy); jz = rot1f(jz); jc = rot1(jc);
+                        jq = rot1f(jq); js = rot1f(js); je = rot1f(je);
+                        fj.x = rot1f(fj.x); fj.y = rot1f(fj.y); fj.z = rot1f(fj.z);
+                    }
+                }
+            } else {   // tiles more than two apart: no intra or bonded-compound pairs, 64 full steps
+                // (jc < 0 marks padding atoms; it rotates with the other j data)
+#pragma unroll UNR
+                for (int st = 0; st < 64; st++) {
+                    pair_cut<MULTI>(wi, pi, !vi || jc < 0, jx, jy, jz, jq, js, je, bx, a, tab, fi, fj, elj, ec);
+                    if (st + 1 < 64) {
+                        jx = rot1f(jx); jy = rot1f(jy); jz = rot1f(jz); jc = rot1(jc);
+                        jq = rot1f(jq); js = rot1f(js); je = rot1f(je);
+                        fj.x = rot1f(fj.x); fj.y = rot1f(fj.y); fj.z = rot1f(fj.z);
+                    }
                 }
             }
             // after ns - 1 rotations lane l holds the accumulator of atom J*64 + (l + ns - 1) mod 64
@@ -907,6 +957,8 @@ __global__ __launch_bounds__(T) void k_em_batch(BatchArgs a) {
     const SysMeta m = a.meta[s];
     SysOut* out = a.out + s;
     if (threadIdx.x == 0) out->t0 = __builtin_amdgcn_s_memrealtime();
+    for (int i = threadIdx.x; i < 2 * (CTAB_N + 1); i += T) sh.ctab[i] = a.ctab[i];
+    __syncthreads();
     if (m.multi) minimize_system<T, UNR, true>(a, m, s, out, sh, lds);
     else minimize_system<T, UNR, false>(a, m, s, out, sh, lds);
     __syncthreads();
@@ -986,6 +1038,7 @@ struct Batch {
     float2* spec = nullptr;
     float4* part = nullptr;
     float2* twid = nullptr;
+    float4* ctab = nullptr;
     SysOut* out = nullptr;
     unsigned short* feat = nullptr;   // [N][32] BF16 features (learned and recording modes)
     float* scale = nullptr;           // [N]
@@ -1005,7 +1058,7 @@ namespace {
 void batch_free(Batch& b) {
     for (auto& s : b.streams) if (s) { (void)hipStreamSynchronize(s); (void)hipStreamDestroy(s); s = nullptr; }
     void* ptrs[] = {b.d_meta, b.d_order, b.u0, b.u1, b.u_start, b.f0, b.f1, b.prm4, b.comp, b.cinfo, b.cbm, b.clut, b.inc_off, b.inc,
-                    b.bid, b.bp, b.aid, b.ap, b.did, b.dp, b.iid, b.ip, b.pid, b.pp, b.green, b.spec, b.part, b.twid, b.out,
+                    b.bid, b.bp, b.aid, b.ap, b.did, b.dp, b.iid, b.ip, b.pid, b.pp, b.green, b.spec, b.part, b.twid, b.ctab, b.out,
                     b.feat, b.scale, b.rec, b.rec_info};
     for (void* p : ptrs) if (p) (void)hipFree(p);
 }
@@ -1023,6 +1076,28 @@ void batch_create(Batch& B, const lima_box_desc* boxes, int nsys, const lima_sim
     B.pc.kc = (float)(8.9875517873681764e9 / 1e-9 / 6.02214076e23 * 1000.f * 1000.f);   // PhysicsUtils.cuh:5
     B.pc.kappa = 3.f / B.rc;                                                             // PhysicsUtils.cuh:60-62
     B.pc.two_kappa_sqrtpi = 2.f * B.pc.kappa / sqrtf(LIMA_PI_F);
+    {   // erfc(kappa r) and H(r) with their r-derivatives, in double (the explicit term uses LIMA's
+        // PI_L as the pair force does; the derivative of erfc is exact); cubic Hermite per interval
+        const double kap = (double)B.pc.kappa, cc = (double)B.pc.two_kappa_sqrtpi, hr = (double)B.rc / CTAB_N;
+        const double de = 2.0 * kap / std::sqrt(M_PI);
+        auto A = [&](double r) { return std::erfc(kap * r); };
+        auto dA = [&](double r) { return -de * std::exp(-kap * kap * r * r); };
+        auto H = [&](double r) { return std::erfc(kap * r) + cc * r * std::exp(-kap * kap * r * r); };
+        auto dH = [&](double r) { return (cc * (1.0 - 2.0 * kap * kap * r * r) - de) * std::exp(-kap * kap * r * r); };
+        auto herm = [&](double f0, double f1, double d0, double d1) {
+            d0 *= hr; d1 *= hr;
+            return make_float4((float)f0, (float)d0, (float)(3.0 * (f1 - f0) - 2.0 * d0 - d1), (float)(2.0 * (f0 - f1) + d0 + d1));
+        };
+        std::vector<float4> tab(2 * (CTAB_N + 1));
+        for (int i = 0; i < CTAB_N; i++) {
+            const double r0 = i * hr, r1 = (i + 1) * hr;
+            tab[2 * i] = herm(A(r0), A(r1), dA(r0), dA(r1));
+            tab[2 * i + 1] = herm(H(r0), H(r1), dH(r0), dH(r1));
+        }
+        tab[2 * CTAB_N] = make_float4((float)A(B.rc), 0.f, 0.f, 0.f);   // r == cutoff exactly
+        tab[2 * CTAB_N + 1] = make_float4((float)H(B.rc), 0.f, 0.f, 0.f);
+        B.ctab = dcopy(tab);
+    }
     const double spacing = p->pme_spacing_nm > 0 ? p->pme_spacing_nm : 0.12;
 
     std::vector<int4> u;
@@ -1283,6 +1358,7 @@ BatchArgs make_args(const Batch& B, double tol_j, int max_iter, double h0) {
     a.bid = B.bid; a.bp = B.bp; a.aid = B.aid; a.ap = B.ap; a.did = B.did; a.dp = B.dp; a.iid = B.iid; a.ip = B.ip;
     a.pid = B.pid; a.pp = B.pp;
     a.green = B.green; a.spec = B.spec; a.part = B.part; a.twid = B.twid; a.out = B.out;
+    a.ctab = B.ctab; a.ctab_scale = (float)CTAB_N / B.rc;
     a.pc = B.pc; a.rc = B.rc; a.rc2 = B.rc * B.rc;
     a.tol = tol_j; a.h0 = h0; a.max_iter = max_iter;
     a.mode = MODE_PLAIN;
