@@ -42,7 +42,7 @@
 namespace lima {
 namespace {
 
-constexpr int KMAX = 32;                       // largest PME grid edge handled in registers
+constexpr int KMAX = 32;                       // largest PME grid edge (twiddle tables, LDS grid)
 constexpr int MAX_ATOMS = 8192;                // atoms per system
 constexpr float FIX_SCALE = 1048576.0f;        // 2^20 fixed-point units per kC/mol of spread charge
 constexpr int NBUCKET = 4;
@@ -100,6 +100,7 @@ struct BatchArgs {
     const int2* pid; const float2* pp;
     const float* green;
     float2* spec;
+    float2* spec2;               // second half-spectrum buffer (the transforms ping-pong)
     float4* part;
     const float2* twid;
     const float4* ctab;          // Coulomb factors of the cutoff pairs in r (see CTAB_N), staged in LDS
@@ -309,26 +310,6 @@ __device__ __forceinline__ void bonded_atom(const BatchArgs& a, const SysMeta& m
         acc = acc + f;
         if (role == 0) eb += e;
     }
-}
-
-// Length-K DFT of a line held in registers, one output at a time:
-// out[k] = sum_j in[j] W^(j k), W = exp(-+2 pi i / K) from the twiddle table.
-template <bool INV>
-__device__ __forceinline__ float2 dft_out(const float2* in, int K, int k, const float2* tw) {
-    float re = 0.f, im = 0.f;
-    int idx = 0;
-#pragma unroll
-    for (int j = 0; j < KMAX; j++) {
-        if (j < K) {
-            const float2 w = tw[idx];
-            const float wi = INV ? -w.y : w.y;
-            re += in[j].x * w.x - in[j].y * wi;
-            im += in[j].x * wi + in[j].y * w.x;
-            idx += k;
-            idx -= idx >= K ? K : 0;
-        }
-    }
-    return make_float2(re, im);
 }
 
 template <int T>
@@ -681,82 +662,82 @@ __device__ __forceinline__ Eval evaluate(const BatchArgs& a, const SysMeta& m, B
             }
         }
         __syncthreads();
+        // The transforms compute one output coefficient per thread (a length-K sum with the
+        // twiddles from LDS), so no thread holds a whole line, and every pass has H K K' work
+        // items for the workgroup.  The half spectrum ping-pongs between spec and spec2.
         float2* spec = a.spec + m.spec0;
-        // x forward, real -> half spectrum: lines (y, z)
-        for (int l = tid; l < K1 * K2; l += T) {
-            float2 line[KMAX];
+        float2* spec2 = a.spec2 + m.spec0;
+        const float rH = 1.f / (float)H, rK0 = 1.f / (float)K0;
+        // item it = row * n + col, split with a float reciprocal (exact below 2^22 items)
+        auto split = [](int it, float rn, int n, int* col) { const int r = (int)(((float)it + 0.5f) * rn); *col = it - r * n; return r; };
+        // x forward, real -> half spectrum: out (row l = (z, y), k) = sum_j rho(l, j) W0^(j k)
+        for (int it = tid; it < K1 * K2 * H; it += T) {
+            int k;
+            const int l = split(it, rH, H, &k);
             const int* src = qg + l * K0;
-#pragma unroll
-            for (int j = 0; j < KMAX; j++) line[j] = make_float2(j < K0 ? (float)src[j] * m.fix_to_rho : 0.f, 0.f);
-            float2* dst = spec + l * H;
-            for (int k = 0; k < H; k++) dst[k] = dft_out<false>(line, K0, k, sh.tw[0]);
-        }
-        __syncthreads();
-        // y forward: lines (x < H, z), stride H
-        for (int l = tid; l < H * K2; l += T) {
-            float2 line[KMAX];
-            const int x = l % H, z = l / H;
-            float2* base = spec + z * K1 * H + x;
-#pragma unroll
-            for (int j = 0; j < KMAX; j++) line[j] = j < K1 ? base[j * H] : make_float2(0.f, 0.f);
-            for (int k = 0; k < K1; k++) base[k * H] = dft_out<false>(line, K1, k, sh.tw[1]);
-        }
-        __syncthreads();
-        // z forward times the influence function, then z inverse: lines (x < H, y), stride K1 H
-        for (int l = tid; l < H * K1; l += T) {
-            float2 line[KMAX];
-            const int x = l % H, y = l / H;
-            float2* base = spec + y * H + x;
-            const float* g = a.green + m.spec0 + y * H + x;
-#pragma unroll
-            for (int j = 0; j < KMAX; j++) line[j] = j < K2 ? base[j * K1 * H] : make_float2(0.f, 0.f);
-            for (int k = 0; k < K2; k++) {
-                const float2 v = dft_out<false>(line, K2, k, sh.tw[2]);
-                const float gk = g[k * K1 * H];
-                base[k * K1 * H] = make_float2(v.x * gk, v.y * gk);
+            float re = 0.f, im = 0.f;
+            int idx = 0;
+            for (int j = 0; j < K0; j++) {
+                const float v = (float)src[j] * m.fix_to_rho;
+                const float2 w = sh.tw[0][idx];
+                re = fmaf(v, w.x, re);
+                im = fmaf(v, w.y, im);
+                idx += k;
+                idx -= idx >= K0 ? K0 : 0;
             }
+            spec[l * H + k] = make_float2(re, im);
         }
         __syncthreads();
-        for (int l = tid; l < H * K1; l += T) {
-            float2 line[KMAX];
-            const int x = l % H, y = l / H;
-            float2* base = spec + y * H + x;
-#pragma unroll
-            for (int j = 0; j < KMAX; j++) line[j] = j < K2 ? base[j * K1 * H] : make_float2(0.f, 0.f);
-            for (int k = 0; k < K2; k++) base[k * K1 * H] = dft_out<true>(line, K2, k, sh.tw[2]);
-        }
-        __syncthreads();
-        for (int l = tid; l < H * K2; l += T) {
-            float2 line[KMAX];
-            const int x = l % H, z = l / H;
-            float2* base = spec + z * K1 * H + x;
-#pragma unroll
-            for (int j = 0; j < KMAX; j++) line[j] = j < K1 ? base[j * H] : make_float2(0.f, 0.f);
-            for (int k = 0; k < K1; k++) base[k * H] = dft_out<true>(line, K1, k, sh.tw[1]);
-        }
-        __syncthreads();
-        // x inverse, half spectrum -> real potential (Hermitian sum) / NG into LDS
-        for (int l = tid; l < K1 * K2; l += T) {
-            float2 line[KMAX];
-            const float2* src = spec + l * H;
-#pragma unroll
-            for (int j = 0; j < KMAX / 2 + 1; j++) line[j] = j < H ? src[j] : make_float2(0.f, 0.f);
-            float* dst = phi + l * K0;
-            const bool even = (K0 & 1) == 0;
-            for (int xj = 0; xj < K0; xj++) {
-                float s = line[0].x;
-                int idx = xj;
-#pragma unroll
-                for (int k = 1; k < KMAX / 2 + 1; k++) {
-                    if (k < H) {
-                        const float2 w = sh.tw[0][idx];   // exp(-i ...); the inverse uses the conjugate
-                        const float re = line[k].x * w.x + line[k].y * w.y;
-                        s += (even && k == H - 1) ? re : 2.f * re;
-                        idx += xj;
-                        idx -= idx >= K0 ? K0 : 0;
-                    }
+        // one complex pass of length K along y (stride H, outer z with stride K1 H) or z (stride
+        // K1 H, outer y with stride H): item (outer o, k, x), out[o ostride + x + k stride] =
+        // sum_j in[o ostride + x + j stride] W^(j k)
+        auto pass = [&](const float2* in, float2* out, const float2* tw, int K, int stride, int nout, int ostride, bool inv,
+                        bool green) {
+            const float rK = 1.f / (float)K;
+            for (int it = tid; it < H * K * nout; it += T) {
+                int x, k;
+                const int r = split(it, rH, H, &x);
+                const int o = split(r, rK, K, &k);
+                const int line0 = o * ostride + x;
+                float re = 0.f, im = 0.f;
+                int idx = 0;
+                for (int j = 0; j < K; j++) {
+                    const float2 v = in[line0 + j * stride];
+                    const float2 w = tw[idx];
+                    const float wi = inv ? -w.y : w.y;
+                    re = fmaf(v.x, w.x, fmaf(-v.y, wi, re));
+                    im = fmaf(v.x, wi, fmaf(v.y, w.x, im));
+                    idx += k;
+                    idx -= idx >= K ? K : 0;
                 }
-                dst[xj] = s * m.inv_ng;
+                const int oi = line0 + k * stride;
+                if (green) { const float gk = a.green[m.spec0 + oi]; re *= gk; im *= gk; }
+                out[oi] = make_float2(re, im);
+            }
+            __syncthreads();
+        };
+        pass(spec, spec2, sh.tw[1], K1, H, K2, K1 * H, false, false);    // y forward
+        pass(spec2, spec, sh.tw[2], K2, K1 * H, K1, H, false, true);     // z forward times the influence function
+        pass(spec, spec2, sh.tw[2], K2, K1 * H, K1, H, true, false);     // z inverse
+        pass(spec2, spec, sh.tw[1], K1, H, K2, K1 * H, true, false);     // y inverse
+        // x inverse, half spectrum -> real potential (Hermitian sum) / NG into LDS
+        {
+            const bool even = (K0 & 1) == 0;
+            for (int it = tid; it < K1 * K2 * K0; it += T) {
+                int xj;
+                const int l = split(it, rK0, K0, &xj);
+                const float2* src = spec + l * H;
+                float s = src[0].x;
+                int idx = xj;
+                for (int k = 1; k < H; k++) {
+                    const float2 v = src[k];
+                    const float2 w = sh.tw[0][idx];   // exp(-i ...); the inverse uses the conjugate
+                    const float re = v.x * w.x + v.y * w.y;
+                    s += (even && k == H - 1) ? re : 2.f * re;
+                    idx += xj;
+                    idx -= idx >= K0 ? K0 : 0;
+                }
+                phi[l * K0 + xj] = s * m.inv_ng;
             }
         }
         __syncthreads();
@@ -775,23 +756,37 @@ __device__ __forceinline__ Eval evaluate(const BatchArgs& a, const SysMeta& m, B
             gather_weights(wx, w6x, d6x);
             gather_weights(wy, w6y, d6y);
             gather_weights(wz, w6z, d6z);
-            float ph = 0.f, ex = 0.f, ey = 0.f, ez = 0.f;
+            // per row: the x sums with the weights and their differences (two FMAs per node), then
+            // the row's y and z factors; the z planes in a loop (weights picked, not indexed)
+            int xs[6];
 #pragma unroll
+            for (int mx = 0; mx < 6; mx++) xs[mx] = wrapi(wrapi(ix[0] - 2 + mx, K0), K0);
+            float ph = 0.f, ex = 0.f, ey = 0.f, ez = 0.f;
+#pragma unroll 1
             for (int mz = 0; mz < 6; mz++) {
                 const int Z = wrapi(wrapi(ix[2] - 2 + mz, K2), K2);
+                float wz = 0.f, dz = 0.f;
+#pragma unroll
+                for (int c = 0; c < 6; c++) {
+                    wz = mz == c ? w6z[c] : wz;
+                    dz = mz == c ? d6z[c] : dz;
+                }
 #pragma unroll
                 for (int my = 0; my < 6; my++) {
                     const int Y = wrapi(wrapi(ix[1] - 2 + my, K1), K1);
                     const float* row = phi + (Z * K1 + Y) * K0;
-                    const float wyz = w6y[my] * w6z[mz], dyz = d6y[my] * w6z[mz], wdz = w6y[my] * d6z[mz];
+                    float rw = 0.f, rd = 0.f;
 #pragma unroll
                     for (int mx = 0; mx < 6; mx++) {
-                        const float v = row[wrapi(wrapi(ix[0] - 2 + mx, K0), K0)];
-                        ph += w6x[mx] * wyz * v;
-                        ex += d6x[mx] * wyz * v;
-                        ey += w6x[mx] * dyz * v;
-                        ez += w6x[mx] * wdz * v;
+                        const float v = row[xs[mx]];
+                        rw = fmaf(w6x[mx], v, rw);
+                        rd = fmaf(d6x[mx], v, rd);
                     }
+                    const float wyz = w6y[my] * wz, dyz = d6y[my] * wz, wdz = w6y[my] * dz;
+                    ph = fmaf(wyz, rw, ph);
+                    ex = fmaf(wyz, rd, ex);
+                    ey = fmaf(dyz, rw, ey);
+                    ez = fmaf(wdz, rw, ez);
                 }
             }
             float4 fo = f[m.atom0 + li];
@@ -1036,6 +1031,7 @@ struct Batch {
     int2* pid = nullptr; float2* pp = nullptr;
     float* green = nullptr;
     float2* spec = nullptr;
+    float2* spec2 = nullptr;
     float4* part = nullptr;
     float2* twid = nullptr;
     float4* ctab = nullptr;
@@ -1058,7 +1054,7 @@ namespace {
 void batch_free(Batch& b) {
     for (auto& s : b.streams) if (s) { (void)hipStreamSynchronize(s); (void)hipStreamDestroy(s); s = nullptr; }
     void* ptrs[] = {b.d_meta, b.d_order, b.u0, b.u1, b.u_start, b.f0, b.f1, b.prm4, b.comp, b.cinfo, b.cbm, b.clut, b.inc_off, b.inc,
-                    b.bid, b.bp, b.aid, b.ap, b.did, b.dp, b.iid, b.ip, b.pid, b.pp, b.green, b.spec, b.part, b.twid, b.ctab, b.out,
+                    b.bid, b.bp, b.aid, b.ap, b.did, b.dp, b.iid, b.ip, b.pid, b.pp, b.green, b.spec, b.spec2, b.part, b.twid, b.ctab, b.out,
                     b.feat, b.scale, b.rec, b.rec_info};
     for (void* p : ptrs) if (p) (void)hipFree(p);
 }
@@ -1323,6 +1319,7 @@ void batch_create(Batch& B, const lima_box_desc* boxes, int nsys, const lima_sim
     B.pid = dcopy(pid); B.pp = dcopy(pp);
     B.green = dcopy(green);
     HIP_CHECK(hipMalloc(&B.spec, std::max(spec_total, 1) * sizeof(float2)));
+    HIP_CHECK(hipMalloc(&B.spec2, std::max(spec_total, 1) * sizeof(float2)));
     HIP_CHECK(hipMalloc(&B.part, std::max<int64_t>(part_total, 1) * sizeof(float4)));
     B.twid = dcopy(twid);
     HIP_CHECK(hipMalloc(&B.out, nsys * sizeof(SysOut)));
@@ -1357,7 +1354,7 @@ BatchArgs make_args(const Batch& B, double tol_j, int max_iter, double h0) {
     a.inc_off = B.inc_off; a.inc = B.inc;
     a.bid = B.bid; a.bp = B.bp; a.aid = B.aid; a.ap = B.ap; a.did = B.did; a.dp = B.dp; a.iid = B.iid; a.ip = B.ip;
     a.pid = B.pid; a.pp = B.pp;
-    a.green = B.green; a.spec = B.spec; a.part = B.part; a.twid = B.twid; a.out = B.out;
+    a.green = B.green; a.spec = B.spec; a.spec2 = B.spec2; a.part = B.part; a.twid = B.twid; a.out = B.out;
     a.ctab = B.ctab; a.ctab_scale = (float)CTAB_N / B.rc;
     a.pc = B.pc; a.rc = B.rc; a.rc2 = B.rc * B.rc;
     a.tol = tol_j; a.h0 = h0; a.max_iter = max_iter;
