@@ -516,7 +516,9 @@ __device__ __forceinline__ Eval evaluate(const BatchArgs& a, const SysMeta& m, B
             J = I + q;
         }
         int curI = -1;
-        float3 wi = make_float3(0.f, 0.f, 0.f);
+        // padding atoms (past n) sit at NaN: every distance test with them is false
+        const float pad = __builtin_nanf("");
+        float3 wi = make_float3(pad, pad, pad);
         float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
         int ci = 0, bmi = 0;
         bool vi = false;
@@ -525,6 +527,7 @@ __device__ __forceinline__ Eval evaluate(const BatchArgs& a, const SysMeta& m, B
                 curI = I;
                 const int li = I * 64 + lane;
                 vi = li < n;
+                wi = make_float3(pad, pad, pad);
                 if (vi) {
                     const int4 uu = u[m.atom0 + li];
                     ci = a.comp[m.atom0 + li];
@@ -535,7 +538,7 @@ __device__ __forceinline__ Eval evaluate(const BatchArgs& a, const SysMeta& m, B
                 }
             }
             // atom J*64+lane and its accumulator start in this lane
-            float jx = 0.f, jy = 0.f, jz = 0.f;
+            float jx = pad, jy = pad, jz = pad;
             int jc = -1;
             float jq = 0.f, js = 0.f, je = 0.f;
             {
@@ -570,12 +573,12 @@ __device__ __forceinline__ Eval evaluate(const BatchArgs& a, const SysMeta& m, B
                     }
                 }
             } else {   // tiles more than two apart: no intra or bonded-compound pairs, 64 full steps
-                // (jc < 0 marks padding atoms; it rotates with the other j data)
+                // (padding atoms drop out through their NaN coordinates)
 #pragma unroll UNR
                 for (int st = 0; st < 64; st++) {
-                    pair_cut<MULTI>(wi, pi, !vi || jc < 0, jx, jy, jz, jq, js, je, bx, a, tab, fi, fj, elj, ec);
+                    pair_cut<MULTI>(wi, pi, false, jx, jy, jz, jq, js, je, bx, a, tab, fi, fj, elj, ec);
                     if (st + 1 < 64) {
-                        jx = rot1f(jx); jy = rot1f(jy); jz = rot1f(jz); jc = rot1(jc);
+                        jx = rot1f(jx); jy = rot1f(jy); jz = rot1f(jz);
                         jq = rot1f(jq); js = rot1f(js); je = rot1f(je);
                         fj.x = rot1f(fj.x); fj.y = rot1f(fj.y); fj.z = rot1f(fj.z);
                     }
