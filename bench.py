@@ -103,8 +103,9 @@ def em_bench(args):
                                             grow_on_learned=True)
     cnt_l = learned.counters()
     log(f"em: learned {wall_l:.3f}s, {sum(r.iterations for r in res_l)} iterations")
-    # control: the same learned path with a model whose output is a constant scale, which makes every
-    # proposal the plain step; it separates the path's overhead from the effect of the trained scales
+    # control: the same learned path with a model whose output is a constant scale; such a model
+    # proposes the plain step, which the path then takes as such (one evaluation per iteration), so
+    # the control's extra time is the path's overhead (feature histograms, inference)
     const = L.Mlp(seed=0)
     prm = np.zeros(const.n_params, np.float32)
     const.set_params(prm)     # 4 sigmoid(0) = 2 everywhere

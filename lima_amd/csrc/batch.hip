@@ -120,6 +120,7 @@ struct BatchArgs {
     const int2* rec_info;        // per system (offset in floats, sampled atoms)
     int rec_stride, rec_cap;     // sampled atoms li = k * rec_stride; states per system
     int grow_on_learned;         // learned mode: h *= 1.2 after an accepted proposal (else h is kept)
+    int wlds;                    // learned mode: byte offset of the BF16 weights in dynamic LDS, staged once
 };
 
 enum { MODE_PLAIN = 0, MODE_LEARNED = 1, MODE_RECORD = 2 };
@@ -316,8 +317,10 @@ template <int T>
 struct BlockShared {
     double red[T / 64][5];
     float fred[T / 64];
+    float fmin_[T / 64];
     double res[5];
     float fres;
+    int uniform;                 // infer_scales: every atom got the same scale
     float2 tw[3][KMAX];
     unsigned long long ph[7];    // phase ticks seen by thread 0
     float4 ctab[2 * (CTAB_N + 1)];   // the Coulomb table of the cutoff pairs
@@ -442,12 +445,17 @@ __device__ __forceinline__ float force_feature(float r) { return 1.f + log10f(fm
 // Learned step scale of every atom: features = (F / Fmax, log |F| / Fmax, histogram), one MFMA forward
 // per 16 atoms per wave.  Returns S = max_i s_i |F_i| (every thread), the normaliser of the proposal.
 template <int T>
-__device__ __forceinline__ float infer_scales(const BatchArgs& a, const SysMeta& m, const float4* F, float fmax, BlockShared<T>& sh) {
+__device__ __forceinline__ float infer_scales(const BatchArgs& a, const SysMeta& m, const float4* F, float fmax, BlockShared<T>& sh,
+                                              const unsigned char* lds) {
+    // the MFMA A operands from the copy in LDS (staged at the start of the launch)
+    MlpWeights wl = a.mlp;
+    wl.w1t = reinterpret_cast<const unsigned short*>(lds + a.wlds);
+    wl.w2t = wl.w1t + MLP_H * MLP_IN;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
     constexpr int W = T / 64;
     const float inv = 1.f / fmax;
     const bf16x4 zero = pack4(0.f, 0.f, 0.f, 0.f);
-    float smax = 0.f;
+    float smax = 0.f, slo = 3.0e38f, shi = -3.0e38f;
     for (int base = wave * 16; base < m.n; base += W * 16) {
         const int li = base + r;
         const bool ok = li < m.n;
@@ -460,19 +468,33 @@ __device__ __forceinline__ float infer_scales(const BatchArgs& a, const SysMeta&
         xb[1] = ok ? *reinterpret_cast<const bf16x4*>(fr + 16 + 4 * g) : zero;
         f32x4 z1[4], z2[4];
         bf16x4 h1[4];
-        const float sc = mlp_scale(mlp_forward16(a.mlp, xb, z1, h1, z2));
+        const float sc = mlp_scale(mlp_forward16(wl, xb, z1, h1, z2));
         if (g == 0 && ok) {
             a.scale[m.atom0 + li] = sc;
             smax = fmaxf(smax, sc * fl);
+            slo = fminf(slo, sc);
+            shi = fmaxf(shi, sc);
         }
     }
     smax = wave_max(smax);
-    if (lane == 0) sh.fred[wave] = smax;
+    slo = -wave_max(-slo);
+    shi = wave_max(shi);
+    if (lane == 0) { sh.fred[wave] = smax; sh.fmin_[wave] = slo; }
+    __syncthreads();
+    // the scale range, wave by wave (the same lanes reuse fmin_ after the first barrier)
+    if (threadIdx.x == 0) {
+        float v = 0.f, lo = 3.0e38f;
+        for (int w = 0; w < W; w++) { v = fmaxf(v, sh.fred[w]); lo = fminf(lo, sh.fmin_[w]); }
+        sh.fres = v;
+        sh.res[0] = (double)lo;
+    }
+    __syncthreads();
+    if (lane == 0) sh.fmin_[wave] = shi;
     __syncthreads();
     if (threadIdx.x == 0) {
-        float v = 0.f;
-        for (int w = 0; w < W; w++) v = fmaxf(v, sh.fred[w]);
-        sh.fres = v;
+        float hi = -3.0e38f;
+        for (int w = 0; w < W; w++) hi = fmaxf(hi, sh.fmin_[w]);
+        sh.uniform = (float)sh.res[0] == hi ? 1 : 0;
     }
     __syncthreads();
     return sh.fres;
@@ -879,8 +901,12 @@ __device__ __forceinline__ void minimize_system(const BatchArgs& a, const SysMet
                     __syncthreads();
                     if (tid == 0) { const unsigned long long t1 = __builtin_amdgcn_s_memrealtime(); sh.ph[5] += t1 - t0; t0 = t1; }
                 }
-                snorm = infer_scales<T>(a, m, cur ? a.f1 : a.f0, acc_e.fmax, sh);
+                snorm = infer_scales<T>(a, m, cur ? a.f1 : a.f0, acc_e.fmax, sh, lds);
                 if (tid == 0) sh.ph[6] += __builtin_amdgcn_s_memrealtime() - t0;
+                // every atom got the same scale: the proposal is the plain step's direction with
+                // the same largest displacement, so the plain step is taken (one evaluation, and a
+                // refusal needs no second one)
+                if (sh.uniform) kind = 2;
             }
             const unsigned long long ts = tid == 0 ? __builtin_amdgcn_s_memrealtime() : 0ull;
             // plain: h F / Fmax; learned: h s F / max(s |F|), the same largest displacement h
@@ -956,6 +982,13 @@ __global__ __launch_bounds__(T) void k_em_batch(BatchArgs a) {
     SysOut* out = a.out + s;
     if (threadIdx.x == 0) out->t0 = __builtin_amdgcn_s_memrealtime();
     for (int i = threadIdx.x; i < 2 * (CTAB_N + 1); i += T) sh.ctab[i] = a.ctab[i];
+    if (a.mode == MODE_LEARNED) {   // model weights, 8 bytes per thread and step
+        uint2* wd = reinterpret_cast<uint2*>(lds + a.wlds);
+        const uint2* w1 = reinterpret_cast<const uint2*>(a.mlp.w1t);
+        const uint2* w2 = reinterpret_cast<const uint2*>(a.mlp.w2t);
+        constexpr int n1 = MLP_H * MLP_IN / 4, n2 = MLP_H * MLP_H / 4;
+        for (int i = threadIdx.x; i < n1 + n2; i += T) wd[i] = i < n1 ? w1[i] : w2[i - n1];
+    }
     __syncthreads();
     if (m.multi) minimize_system<T, UNR, true>(a, m, s, out, sh, lds);
     else minimize_system<T, UNR, false>(a, m, s, out, sh, lds);
@@ -1362,7 +1395,7 @@ BatchArgs make_args(const Batch& B, double tol_j, int max_iter, double h0) {
     a.pc = B.pc; a.rc = B.rc; a.rc2 = B.rc * B.rc;
     a.tol = tol_j; a.h0 = h0; a.max_iter = max_iter;
     a.mode = MODE_PLAIN;
-    a.feat = B.feat; a.scale = B.scale; a.feat_interval = 1;
+    a.feat = B.feat; a.scale = B.scale; a.feat_interval = 1; a.wlds = 0;
     a.rec = B.rec; a.rec_info = B.rec_info; a.rec_stride = B.rec_stride; a.rec_cap = B.rec_cap;
     return a;
 }
@@ -1401,6 +1434,10 @@ void launch_groups(Batch& B, const std::vector<int>& systems, const std::vector<
             if (a.mode != MODE_PLAIN) lds = std::max(lds, m.n * 28);   // histogram byte counters
         }
         BatchArgs ag = a;
+        if (a.mode == MODE_LEARNED) {   // the model's BF16 weights after everything else, for the whole launch
+            ag.wlds = (lds + 15) & ~15;
+            lds = ag.wlds + (MLP_H * MLP_IN + MLP_H * MLP_H) * (int)sizeof(unsigned short);
+        }
         ag.order = B.d_order + ranges[sh].first;
         KernelFn fn = shape_kernel(sh);
         HIP_CHECK(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, lds));

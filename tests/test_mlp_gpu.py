@@ -126,10 +126,10 @@ def test_recording_and_training_set():
 
 
 def test_constant_scale_model_reproduces_plain_minimiser():
-    """A model whose output is the same scale for every atom makes each proposal the plain step (the
-    proposal is normalised by max s |F|), so the learned path follows the plain minimiser: step for
-    step over the first iterations (until FP32 roundings of the two step formulas part them), and to
-    the same convergence over a full run."""
+    """A model whose output is the same scale for every atom proposes the plain step's direction
+    with the plain step's largest displacement, so the learned path takes the plain step itself
+    (one evaluation per iteration, no fallback): the run is the plain minimiser's, bit for bit,
+    while the histogram and inference passes still run."""
     boxes = [L.gen_em_system(n, s, 0.02) for n, s in ((400, 1020), (1100, 1021))]
     p = L.default_params()
     m = L.Mlp(seed=0)
@@ -139,16 +139,17 @@ def test_constant_scale_model_reproduces_plain_minimiser():
         rp, _ = plain.minimize(tol, iters, 0.01)
         learned = L.Batch(boxes, p)
         rl, _ = learned.minimize_learned(m, tol, iters, 0.01, feature_interval=4, grow_on_learned=True)
-        cnt = learned.counters()
+        cnt_p, cnt_l = plain.counters(), learned.counters()
+        ph = learned.phase_times()
         for k in range(len(boxes)):
             print(f"max_iter {iters} system {k}: plain {rp[k].iterations} it {rp[k].accepted} acc E {rp[k].final_energy:.10e} | "
-                  f"constant-scale {rl[k].iterations} it {rl[k].accepted} acc {cnt[k, 1]} learned E {rl[k].final_energy:.10e}")
-            if iters == 25:
-                assert rl[k].iterations == rp[k].iterations == 25
-                assert rl[k].accepted == rp[k].accepted and cnt[k, 1] == rl[k].accepted
-                assert rl[k].final_energy == pytest.approx(rp[k].final_energy, rel=2e-5)   # FP32 |F| of the two norms
-            else:
+                  f"constant-scale {rl[k].iterations} it {rl[k].accepted} acc {cnt_l[k, 1]} learned E {rl[k].final_energy:.10e}")
+            assert rl[k].iterations == rp[k].iterations
+            assert rl[k].accepted == rp[k].accepted
+            assert rl[k].final_energy == rp[k].final_energy
+            assert rl[k].converged == rp[k].converged
+            assert cnt_l[k, 0] == cnt_p[k, 0]        # force evaluations: no fallback evaluations
+            assert cnt_l[k, 1] == 0                  # no learned proposal taken
+            assert ph[k, 6] > 0.0                    # inference ran
+            if iters == 5000:
                 assert rl[k].converged == 1
-                assert abs(rl[k].iterations - rp[k].iterations) <= 0.15 * rp[k].iterations
-                assert cnt[k, 1] >= 0.8 * rl[k].accepted     # nearly every accepted step is a proposal
-                assert rl[k].final_energy == pytest.approx(rp[k].final_energy, rel=3e-3)
