@@ -121,6 +121,8 @@ struct BatchArgs {
     int rec_stride, rec_cap;     // sampled atoms li = k * rec_stride; states per system
     int grow_on_learned;         // learned mode: h *= 1.2 after an accepted proposal (else h is kept)
     int wlds;                    // learned mode: byte offset of the BF16 weights in dynamic LDS, staged once
+    unsigned* hnext;             // learned mode: [N][7] packed neighbour-histogram bytes of the last trial
+                                 // evaluated before a feature refresh (counted in its pair pass)
 };
 
 enum { MODE_PLAIN = 0, MODE_LEARNED = 1, MODE_RECORD = 2 };
@@ -199,20 +201,40 @@ struct PairBox {
 // Cutoff pair (i, j) over every periodic image within the cutoff.  `excl` marks a pair of one
 // compound or of bonded compounds (left to the intra pass).  One predicate guards the pair
 // arithmetic, so a wave branches once per step.
+// Neighbour-histogram counting folded into a pair pass (learned mode): byte counters in LDS, bin b
+// of the distance, offset by HIST_BINS when the partner is negative; every pair within HIST_R is
+// counted, excluded (intra / bonded) pairs included, once per image.
+struct HistCtx {
+    unsigned* hc;     // [n][7] packed byte counters
+    int li, lj;       // local atoms of the pair
+    int ineg, jneg;   // HIST_BINS when the atom's charge is negative, else 0
+    bool on;          // counting in this pass (wave-uniform)
+    bool skip;        // self pair, padding or the pair's second visit in a diagonal tile
+};
+__device__ __forceinline__ void hist_count(const HistCtx& h, float r2) {
+    if (r2 <= HIST_R * HIST_R && !h.skip) {
+        const int b = min(HIST_BINS - 1, (int)(sqrtf(r2) * ((float)HIST_BINS / HIST_R)));
+        const int bi = b + h.jneg, bj = b + h.ineg;
+        atomicAdd(&h.hc[h.li * 7 + (bi >> 2)], 1u << ((bi & 3) * 8));
+        atomicAdd(&h.hc[h.lj * 7 + (bj >> 2)], 1u << ((bj & 3) * 8));
+    }
+}
+
 template <bool MULTI>
 // Positions are the wrapped Coord in nm as floats (rounding ~2e-7 nm at these box sizes); the
 // nearest image is d - L rint(d / L), four instructions per axis.
 __device__ __forceinline__ void pair_cut(const float3& wi, const float4& pi, bool excl, float jx, float jy, float jz, float qj,
                                          float sj, float ej, const PairBox& bx, const BatchArgs& a, const float4* tab, v3& fi, v3& fj,
-                                         float& elj, float& ec) {
+                                         float& elj, float& ec, const HistCtx& hist) {
     float x = jx - wi.x, y = jy - wi.y, z = jz - wi.z;
     x = fmaf(-bx.fx, rintf(x * bx.ix), x);
     y = fmaf(-bx.fy, rintf(y * bx.iy), y);
     z = fmaf(-bx.fz, rintf(z * bx.iz), z);
     if (!MULTI) {
         const float r2 = x * x + y * y + z * z;
+        if (hist.on) hist_count(hist, r2);
         if (!excl && r2 <= a.rc2) pair_eval(x, y, z, r2, pi, qj, sj, ej, a.pc, tab, a.ctab_scale, fi, fj, elj, ec);
-    } else if (!excl) {
+    } else if (!excl || hist.on) {
         // second candidate image per axis: the one on the other side, when within the cutoff
         const float ax = x - copysignf(bx.fx, x), ay = y - copysignf(bx.fy, y), az = z - copysignf(bx.fz, z);
         const int nx = fabsf(ax) <= a.rc ? 2 : 1, ny = fabsf(ay) <= a.rc ? 2 : 1, nz = fabsf(az) <= a.rc ? 2 : 1;
@@ -223,7 +245,8 @@ __device__ __forceinline__ void pair_cut(const float3& wi, const float4& pi, boo
                 for (int kx = 0; kx < nx; kx++) {
                     const float xx = kx ? ax : x;
                     const float r2 = xx * xx + yy * yy + zz * zz;
-                    if (r2 <= a.rc2) pair_eval(xx, yy, zz, r2, pi, qj, sj, ej, a.pc, tab, a.ctab_scale, fi, fj, elj, ec);
+                    if (hist.on) hist_count(hist, r2);
+                    if (!excl && r2 <= a.rc2) pair_eval(xx, yy, zz, r2, pi, qj, sj, ej, a.pc, tab, a.ctab_scale, fi, fj, elj, ec);
                 }
             }
         }
@@ -351,6 +374,18 @@ __device__ __forceinline__ void block_reduce(BlockShared<T>& sh, double* v, floa
     for (int c = 0; c < 4; c++) v[c] = sh.res[c];
 }
 
+// Features 4..31 from packed byte counters: count / 8 as BF16.
+template <int T>
+__device__ __forceinline__ void hist_features(const BatchArgs& a, const SysMeta& m, const unsigned* hc) {
+    for (int k = threadIdx.x; k < m.n; k += T) {
+        unsigned short* fo = a.feat + (int64_t)(m.atom0 + k) * 32;
+        for (int b = 0; b < 2 * HIST_BINS; b++) {
+            const unsigned c = (hc[k * 7 + (b >> 2)] >> ((b & 3) * 8)) & 255u;
+            fo[4 + b] = f2bf((float)c * 0.125f);
+        }
+    }
+}
+
 // Neighbour histograms (features 4..31): pairs within HIST_R counted in HIST_BINS distance bins per
 // charge sign of the neighbour, every periodic image, each pair once through the same tile-pair walk
 // as the forces (j positions rotate across the wave).  Byte counters live in LDS ([n][28]); integer
@@ -428,13 +463,7 @@ __device__ __forceinline__ void hist_pass(const BatchArgs& a, const SysMeta& m, 
         if (++J == NT) { ++I; J = I; }
     }
     __syncthreads();
-    for (int k = tid; k < n; k += T) {
-        unsigned short* fo = a.feat + (int64_t)(m.atom0 + k) * 32;
-        for (int b = 0; b < 2 * HIST_BINS; b++) {
-            const unsigned c = (hc[k * 7 + (b >> 2)] >> ((b & 3) * 8)) & 255u;
-            fo[4 + b] = f2bf((float)c * 0.125f);
-        }
-    }
+    hist_features<T>(a, m, hc);
     __syncthreads();
 }
 
@@ -510,11 +539,19 @@ struct Eval {
 // energy parts and Fmax (identical in every thread).
 template <int T, int UNR, bool MULTI>
 __device__ __forceinline__ Eval evaluate(const BatchArgs& a, const SysMeta& m, BlockShared<T>& sh, unsigned char* lds,
-                                         const int4* u, float4* f) {
+                                         const int4* u, float4* f, bool want_hist) {
     const int tid = threadIdx.x;
     const int n = m.n;
     double elj_d = 0.0, ec_d = 0.0, eb_d = 0.0, ep_d = 0.0;
     unsigned long long tk = tid == 0 ? __builtin_amdgcn_s_memrealtime() : 0ull;
+    // learned mode, before a feature refresh: the pair pass also counts the neighbour histogram
+    // (byte counters in LDS after the waves' j-force slices), copied to hnext afterwards
+    const bool hon = __builtin_amdgcn_readfirstlane(want_hist ? 1 : 0) != 0;
+    unsigned* hcl = reinterpret_cast<unsigned*>(lds + (T / 64) * 192 * sizeof(float));
+    if (hon) {
+        for (int k = tid; k < n * 7; k += T) hcl[k] = 0u;
+        __syncthreads();
+    }
     // ---- cutoff pairs: every unordered pair of 64-atom tiles once (Newton's third law) ----
     // Wave w takes a fixed contiguous range of tile pairs (I <= J).  Lane l holds atom I*64+l;
     // the J tile's atoms and their force accumulators rotate one lane per step (DPP), so at
@@ -544,11 +581,13 @@ __device__ __forceinline__ Eval evaluate(const BatchArgs& a, const SysMeta& m, B
         float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
         int ci = 0, bmi = 0;
         bool vi = false;
+        HistCtx hh{hcl, 0, 0, 0, 0, hon, false};
         for (int q = q0; q < q1; q++) {
             if (I != curI) {
                 curI = I;
                 const int li = I * 64 + lane;
                 vi = li < n;
+                hh.li = li;
                 wi = make_float3(pad, pad, pad);
                 if (vi) {
                     const int4 uu = u[m.atom0 + li];
@@ -558,6 +597,7 @@ __device__ __forceinline__ Eval evaluate(const BatchArgs& a, const SysMeta& m, B
                     pi = a.prm[m.atom0 + li];
                     bmi = a.cbm[m.comp0 + ci];
                 }
+                hh.ineg = vi && pi.x < 0.f ? HIST_BINS : 0;
             }
             // atom J*64+lane and its accumulator start in this lane
             float jx = pad, jy = pad, jz = pad;
@@ -585,9 +625,12 @@ __device__ __forceinline__ Eval evaluate(const BatchArgs& a, const SysMeta& m, B
 #pragma unroll UNR
                 for (int st = 0; st < ns; st++) {
                     const int dc = jc - ci;
-                    const bool excl = !vi || jc < 0 || (diag && (st == 0 || (st == 32 && lane >= 32))) ||
-                                      ((unsigned)(dc + 2) <= 4u && ((bmi >> (dc + 2)) & 1));
-                    pair_cut<MULTI>(wi, pi, excl, jx, jy, jz, jq, js, je, bx, a, tab, fi, fj, elj, ec);
+                    const bool skip = !vi || jc < 0 || (diag && (st == 0 || (st == 32 && lane >= 32)));
+                    const bool excl = skip || ((unsigned)(dc + 2) <= 4u && ((bmi >> (dc + 2)) & 1));
+                    hh.lj = J * 64 + ((lane + st) & 63);
+                    hh.jneg = jq < 0.f ? HIST_BINS : 0;
+                    hh.skip = skip;
+                    pair_cut<MULTI>(wi, pi, excl, jx, jy, jz, jq, js, je, bx, a, tab, fi, fj, elj, ec, hh);
                     if (st + 1 < ns) {
                         jx = rot1f(jx); jy = rot1f(jy); jz = rot1f(jz); jc = rot1(jc);
                         jq = rot1f(jq); js = rot1f(js); je = rot1f(je);
@@ -596,9 +639,12 @@ __device__ __forceinline__ Eval evaluate(const BatchArgs& a, const SysMeta& m, B
                 }
             } else {   // tiles more than two apart: no intra or bonded-compound pairs, 64 full steps
                 // (padding atoms drop out through their NaN coordinates)
+                hh.skip = false;
 #pragma unroll UNR
                 for (int st = 0; st < 64; st++) {
-                    pair_cut<MULTI>(wi, pi, false, jx, jy, jz, jq, js, je, bx, a, tab, fi, fj, elj, ec);
+                    hh.lj = J * 64 + ((lane + st) & 63);
+                    hh.jneg = jq < 0.f ? HIST_BINS : 0;
+                    pair_cut<MULTI>(wi, pi, false, jx, jy, jz, jq, js, je, bx, a, tab, fi, fj, elj, ec, hh);
                     if (st + 1 < 64) {
                         jx = rot1f(jx); jy = rot1f(jy); jz = rot1f(jz);
                         jq = rot1f(jq); js = rot1f(js); je = rot1f(je);
@@ -624,6 +670,8 @@ __device__ __forceinline__ Eval evaluate(const BatchArgs& a, const SysMeta& m, B
         }
     }
     __syncthreads();
+    if (hon)   // before the PME pass reuses the LDS
+        for (int k = tid; k < n * 7; k += T) a.hnext[(int64_t)m.atom0 * 7 + k] = hcl[k];
     if (tid == 0) { const unsigned long long t = __builtin_amdgcn_s_memrealtime(); sh.ph[0] += t - tk; tk = t; }
     // ---- per atom: tile-pair partials in a fixed order, intra pairs, bonded terms ----
     {
@@ -887,6 +935,7 @@ __device__ __forceinline__ void minimize_system(const BatchArgs& a, const SysMet
     int kind = 0;
     float snorm = 1.f;
     bool conv = false;
+    bool hist_ok = false;   // hnext holds the neighbour histogram of the current accepted state
     for (;;) {
         const int tgt = kind == 0 ? 0 : 1 - cur;
         int4* ut = tgt ? a.u1 : a.u0;
@@ -897,7 +946,10 @@ __device__ __forceinline__ void minimize_system(const BatchArgs& a, const SysMet
             if (kind == 1) {
                 unsigned long long t0 = tid == 0 ? __builtin_amdgcn_s_memrealtime() : 0ull;
                 if (it % a.feat_interval == 0) {
-                    hist_pass<T, MULTI>(a, m, lds, cur ? a.u1 : a.u0);
+                    // the histogram counted in the pair pass of the accepted state's evaluation, or
+                    // (when that evaluation counted none) a pass of its own
+                    if (hist_ok) hist_features<T>(a, m, a.hnext + (int64_t)m.atom0 * 7);
+                    else hist_pass<T, MULTI>(a, m, lds, cur ? a.u1 : a.u0);
                     __syncthreads();
                     if (tid == 0) { const unsigned long long t1 = __builtin_amdgcn_s_memrealtime(); sh.ph[5] += t1 - t0; t0 = t1; }
                 }
@@ -914,7 +966,9 @@ __device__ __forceinline__ void minimize_system(const BatchArgs& a, const SysMet
             __syncthreads();
             if (tid == 0) sh.ph[4] += __builtin_amdgcn_s_memrealtime() - ts;
         }
-        const Eval ev = evaluate<T, UNR, MULTI>(a, m, sh, lds, ut, ft);
+        // the evaluations that may produce the state of the next feature refresh count its histogram
+        const bool want_hist = learned && (kind == 0 || (it + 1) % a.feat_interval == 0);
+        const Eval ev = evaluate<T, UNR, MULTI>(a, m, sh, lds, ut, ft, want_hist);
         nevals++;
         bool step_done = false, accepted = false;
         if (kind == 0) {
@@ -929,6 +983,8 @@ __device__ __forceinline__ void minimize_system(const BatchArgs& a, const SysMet
         } else if (kind == 2) {
             step_done = true;
         }
+        if (want_hist) hist_ok = accepted;   // hnext now holds this trial's histogram
+        else if (accepted) hist_ok = false;  // the current state changed without one
         if (kind != 0 && step_done) {
             if (accepted) {
                 if (kind == 2 || a.grow_on_learned) h *= 1.2;
@@ -1073,6 +1129,7 @@ struct Batch {
     float4* ctab = nullptr;
     SysOut* out = nullptr;
     unsigned short* feat = nullptr;   // [N][32] BF16 features (learned and recording modes)
+    unsigned* hnext = nullptr;        // [N][7] histogram bytes counted in a pair pass (learned mode)
     float* scale = nullptr;           // [N]
     float* rec = nullptr;             // recorded states
     int2* rec_info = nullptr;         // per system (offset, sampled atoms)
@@ -1091,7 +1148,7 @@ void batch_free(Batch& b) {
     for (auto& s : b.streams) if (s) { (void)hipStreamSynchronize(s); (void)hipStreamDestroy(s); s = nullptr; }
     void* ptrs[] = {b.d_meta, b.d_order, b.u0, b.u1, b.u_start, b.f0, b.f1, b.prm4, b.comp, b.cinfo, b.cbm, b.clut, b.inc_off, b.inc,
                     b.bid, b.bp, b.aid, b.ap, b.did, b.dp, b.iid, b.ip, b.pid, b.pp, b.green, b.spec, b.spec2, b.part, b.twid, b.ctab, b.out,
-                    b.feat, b.scale, b.rec, b.rec_info};
+                    b.feat, b.hnext, b.scale, b.rec, b.rec_info};
     for (void* p : ptrs) if (p) (void)hipFree(p);
 }
 
@@ -1395,12 +1452,13 @@ BatchArgs make_args(const Batch& B, double tol_j, int max_iter, double h0) {
     a.pc = B.pc; a.rc = B.rc; a.rc2 = B.rc * B.rc;
     a.tol = tol_j; a.h0 = h0; a.max_iter = max_iter;
     a.mode = MODE_PLAIN;
-    a.feat = B.feat; a.scale = B.scale; a.feat_interval = 1; a.wlds = 0;
+    a.feat = B.feat; a.scale = B.scale; a.feat_interval = 1; a.wlds = 0; a.hnext = B.hnext;
     a.rec = B.rec; a.rec_info = B.rec_info; a.rec_stride = B.rec_stride; a.rec_cap = B.rec_cap;
     return a;
 }
 
 void ensure_features(Batch& B) {
+    if (!B.hnext) HIP_CHECK(hipMalloc(&B.hnext, B.N * 7 * sizeof(unsigned)));
     if (!B.feat) {
         HIP_CHECK(hipMalloc(&B.feat, B.N * 32 * sizeof(unsigned short)));
         HIP_CHECK(hipMemset(B.feat, 0, B.N * 32 * sizeof(unsigned short)));
@@ -1432,6 +1490,8 @@ void launch_groups(Batch& B, const std::vector<int>& systems, const std::vector<
             const SysMeta& m = B.meta[order[ranges[sh].first + k]];
             lds = std::max(lds, m.lds_bytes);
             if (a.mode != MODE_PLAIN) lds = std::max(lds, m.n * 28);   // histogram byte counters
+            if (a.mode == MODE_LEARNED)   // counters after the j-force slices, in the pair pass
+                lds = std::max(lds, SHAPES[sh].threads / 64 * 192 * (int)sizeof(float) + m.n * 28);
         }
         BatchArgs ag = a;
         if (a.mode == MODE_LEARNED) {   // the model's BF16 weights after everything else, for the whole launch
