@@ -485,16 +485,27 @@ __device__ __forceinline__ float infer_scales(const BatchArgs& a, const SysMeta&
     const float inv = 1.f / fmax;
     const bf16x4 zero = pack4(0.f, 0.f, 0.f, 0.f);
     float smax = 0.f, slo = 3.0e38f, shi = -3.0e38f;
-    for (int base = wave * 16; base < m.n; base += W * 16) {
+    // the operands of a group of 16 atoms: force, and this lane's feature words (group g's slice);
+    // the next group's loads are issued before the current group's MFMA chain
+    auto load = [&](int base, float4& f, bf16x4& x0, bf16x4& x1) {
         const int li = base + r;
         const bool ok = li < m.n;
         const unsigned short* fr = a.feat + (int64_t)(m.atom0 + (ok ? li : 0)) * 32;
-        const float4 f = ok ? F[m.atom0 + li] : make_float4(0.f, 0.f, 0.f, 0.f);
+        f = ok ? F[m.atom0 + li] : make_float4(0.f, 0.f, 0.f, 0.f);
+        x0 = ok ? *reinterpret_cast<const bf16x4*>(fr + 4 * g) : zero;   // g == 0 replaces it below
+        x1 = ok ? *reinterpret_cast<const bf16x4*>(fr + 16 + 4 * g) : zero;
+    };
+    float4 fn;
+    bf16x4 x0n, x1n;
+    if (wave * 16 < m.n) load(wave * 16, fn, x0n, x1n);
+    for (int base = wave * 16; base < m.n; base += W * 16) {
+        const int li = base + r;
+        const bool ok = li < m.n;
+        const float4 f = fn;
+        bf16x4 xb[2] = {x0n, x1n};
+        if (base + W * 16 < m.n) load(base + W * 16, fn, x0n, x1n);
         const float fl = sqrtf(f.x * f.x + f.y * f.y + f.z * f.z);
-        bf16x4 xb[2];
         if (g == 0) xb[0] = pack4(f.x * inv, f.y * inv, f.z * inv, force_feature(fl * inv));
-        else xb[0] = ok ? *reinterpret_cast<const bf16x4*>(fr + 4 * g) : zero;
-        xb[1] = ok ? *reinterpret_cast<const bf16x4*>(fr + 16 + 4 * g) : zero;
         f32x4 z1[4], z2[4];
         bf16x4 h1[4];
         const float sc = mlp_scale(mlp_forward16(wl, xb, z1, h1, z2));
