@@ -93,6 +93,31 @@ def test_neighbour_histogram_features_match_brute_force():
     assert mism.max() <= 1 and np.count_nonzero(mism) <= 0.002 * mism.size
 
 
+def test_histogram_counted_in_the_pair_pass_matches_brute_force():
+    """In learned runs the histograms are counted inside the pair pass of the evaluation before a
+    feature refresh.  With one feature refresh per iteration and one iteration, the features are
+    those counted while evaluating the starting state."""
+    box = L.gen_em_system(700, 1001, 0.02)
+    start = L.Batch([box], L.default_params())
+    start.minimize(100.0, 0, 0.01)                   # evaluates the starting state only
+    st = start.state(0)
+    b = L.Batch([box], L.default_params())
+    m = L.Mlp(seed=3)                                # scales not all equal: proposals are made
+    b.minimize_learned(m, 100.0, 1, 0.01, feature_interval=1)
+    f = b.features(0)[:, 4:]
+    li = box.layout_index()
+    comp = li // 32
+    coord = st["particle_coord"].reshape(-1, 3)
+    pos = np.mod((st["compound_origo"][comp].astype(np.int64) * 100000000 + coord[li]).astype(np.float64) * 1e-8,
+                 box.box_nm[0])
+    h = _images_hist(pos, box.charge.reshape(-1)[li], box.box_nm[0])
+    counts = np.rint(f * 8).astype(np.int64)
+    mism = np.abs(counts - h)
+    print(f"histogram mismatches: {np.count_nonzero(mism)} of {mism.size}, total count {h.sum()}")
+    assert h.sum() > 0
+    assert mism.max() <= 1 and np.count_nonzero(mism) <= 0.002 * mism.size
+
+
 def test_learned_minimisation_converges_with_fallback():
     boxes = [L.gen_em_system(n, s, 0.02) for n, s in ((300, 1010), (900, 1011), (1700, 1012))]
     p = L.default_params()
