@@ -88,7 +88,8 @@ struct Pme {
     int ntiles = 0;
     int bincap = 512;                      // charged particles per tile bin
     int* bin_count = nullptr;              // [ntiles+1]
-    int* bin_atoms = nullptr;              // [ntiles][bincap]
+    int* bin_slot = nullptr;               // [ntiles][bincap] slot of each binned particle
+    uint4* bin_pos = nullptr;              // [ntiles][bincap] its wrapped Coord and charge bits
     long long* tblock = nullptr;           // [ntiles][E^3] per-tile blocks
     int4* cover[3] = {nullptr, nullptr, nullptr};   // per grid coordinate: (tile a, local a, tile b, local b)
     int* flags = nullptr;                  // the engine's DF_PMEBIN word: a bin overflowed / a particle left its block

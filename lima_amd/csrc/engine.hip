@@ -90,6 +90,8 @@ __global__ void k_integrate(IntegrateArgs a) {
     float ke = 0.f, d2 = 0.f;
     uint4 bin_w = make_uint4(0u, 0u, 0u, 0u);
     bool bin_on = false;   // pack: this particle goes to a PME tile bin
+    int bin_s = 0;
+    float bin_q = 0.f;
     bool bad = false;      // non-finite force
     if (p < a.N) {
         const int s = a.slot_of[p];
@@ -131,6 +133,8 @@ __global__ void k_integrate(IntegrateArgs a) {
             a.xq[s] = make_float4((float)d[0] * LIMA_LIMA_TO_NANO_F, (float)d[1] * LIMA_LIMA_TO_NANO_F, (float)d[2] * LIMA_LIMA_TO_NANO_F, q);
             bin_w = w;
             bin_on = q != 0.f;
+            bin_s = s;
+            bin_q = q;
         }
         {   // displacement since the list was built (unwrapped, exact integer difference)
             const float dx = (float)((long long)w.x - (long long)wb.x + (long long)(im.x - ib.x) * a.Lq[0]) * LIMA_LIMA_TO_NANO_F;
@@ -143,7 +147,7 @@ __global__ void k_integrate(IntegrateArgs a) {
         a.v[p] = v;
         a.fprev[p] = make_float4(fx, fy, fz, 0.f);
     }
-    if (a.pack && a.bins.bin_count) bin_wave(a.bins, bin_on, (int)p, bin_w);   // all lanes, converged
+    if (a.pack && a.bins.bin_count) bin_wave(a.bins, bin_on, bin_s, bin_w, bin_q);   // all lanes, converged
     if (__ballot(bad) != 0ull && (threadIdx.x & 63) == 0) atomicOr(&a.flags[DF_NONFINITE], 1);
     // the wave's largest and second-largest squared displacement (lanes past N hold 0)
     const float m1 = wave_max(d2);

@@ -64,7 +64,7 @@ __global__ void k_pack(const int* __restrict__ ncl_dev, int nslots, const int* _
         }
         xq[k] = out;
     }
-    if (bins.bin_count) bin_wave(bins, a >= 0 && q != 0.f, a, p);
+    if (bins.bin_count) bin_wave(bins, a >= 0 && q != 0.f, k, p, q);
 }
 
 // The per-kernel force and potE buffers of a new list: slots that a kernel does not own keep zero

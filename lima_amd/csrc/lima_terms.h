@@ -183,16 +183,19 @@ struct BinArgs {
     int K[3], T[3], nt[3];
     int bincap;
     int* bin_count;
-    int* bin_atoms;
+    int* bin_slot;     // [tile][bincap] the particle's slot (where the gather writes its force)
+    uint4* bin_pos;    // [tile][bincap] its wrapped Coord and charge bits: the spread and gather read
+                       // their tile's particles contiguously instead of gathering them by index
     int* flags;        // bit 0: a bin overflowed
 };
 
-// Called by all 64 lanes of a wave (converged); lanes with active set bin particle a at pw.
+// Called by all 64 lanes of a wave (converged); lanes with active set bin the particle of slot s at
+// pw with charge q.
 // Neighbouring lanes mostly share a tile (slots follow the cluster order): ballots group the
 // lanes by tile (leader = lowest lane of the group, rank = lanes of the group below), then every
 // leader reserves its group's range with one atomic, all in the same instruction (one round trip
 // per wave), and each lane takes its base from its leader.
-__device__ __forceinline__ void bin_wave(const BinArgs& b, bool active, int a, uint4 pw) {
+__device__ __forceinline__ void bin_wave(const BinArgs& b, bool active, int s, uint4 pw, float q) {
     int tile = -1;
     if (active) {
         int ix[3];
@@ -222,8 +225,13 @@ __device__ __forceinline__ void bin_wave(const BinArgs& b, bool active, int a, u
     base = __shfl(base, leader, 64);
     if (active) {
         const int pos = base + rank;
-        if (pos < b.bincap) b.bin_atoms[(int64_t)tile * b.bincap + pos] = a;
-        else atomicOr(b.flags, 1);
+        if (pos < b.bincap) {
+            const int64_t e = (int64_t)tile * b.bincap + pos;
+            b.bin_slot[e] = s;
+            b.bin_pos[e] = make_uint4(pw.x, pw.y, pw.z, __float_as_uint(q));
+        } else {
+            atomicOr(b.flags, 1);
+        }
     }
 }
 

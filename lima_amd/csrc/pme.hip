@@ -177,9 +177,7 @@ struct TileArgs {
     int T[3], margin, E[3];
     int nt[3];
     int bincap;
-    const uint4* w;
-    const float4* prm;
-    int* bin_atoms;
+    const uint4* bin_pos;   // the tile's particles: wrapped Coord, charge bits
     int* bin_count;
     long long* tblock;
     int* flags;
@@ -197,9 +195,8 @@ __global__ __launch_bounds__(256) void k_spread_tiles(TileArgs a) {
     __syncthreads();
     const int nb = min(a.bin_count[t], a.bincap);
     for (int k = threadIdx.x; k < nb; k += blockDim.x) {
-        const int p = a.bin_atoms[(int64_t)t * a.bincap + k];
-        const float q = a.prm[p].x;
-        const uint4 pw = a.w[p];
+        const uint4 pw = a.bin_pos[(int64_t)t * a.bincap + k];
+        const float q = __uint_as_float(pw.w);
         int ix[3];
         float f[3], wx[4], wy[4], wz[4];
         grid_coord(pw.x, a.Lq[0], a.K[0], &ix[0], &f[0]);
@@ -500,11 +497,9 @@ struct GatherTileArgs {
     int nslots;
     float inv2h[3];
     float eself;
-    const uint4* w;
-    const float4* prm;
     const float* phi;
-    const int* slot_of;
-    const int* bin_atoms;
+    const int* bin_slot;    // the tile's particles: slot, and wrapped Coord with charge bits
+    const uint4* bin_pos;
     int* bin_count;
     float* fpme;
     float* epme;
@@ -552,8 +547,8 @@ __global__ __launch_bounds__(256) void k_gather_tiles(GatherTileArgs a) {
         }
     }
     __syncthreads();
-    // the bin's particles (index, charge, position) staged in LDS, 256 at a time, with one load
-    // round for all of them instead of two dependent loads per gather round
+    // the bin's particles (slot, charge, position) staged in LDS, 256 at a time, read contiguously
+    // from the bin records
     __shared__ int sp[256];
     __shared__ float sq[256];
     __shared__ uint4 sw[256];
@@ -562,10 +557,11 @@ __global__ __launch_bounds__(256) void k_gather_tiles(GatherTileArgs a) {
     for (int c0 = 0; c0 < nb; c0 += 256) {   // uniform trip counts per block
         const int nc = min(256, nb - c0);
         if ((int)threadIdx.x < nc) {
-            const int p = a.bin_atoms[(int64_t)t * a.bincap + c0 + threadIdx.x];
-            sp[threadIdx.x] = p;
-            sq[threadIdx.x] = a.prm[p].x;
-            sw[threadIdx.x] = a.w[p];
+            const int64_t e = (int64_t)t * a.bincap + c0 + threadIdx.x;
+            const uint4 pw = a.bin_pos[e];
+            sp[threadIdx.x] = a.bin_slot[e];
+            sq[threadIdx.x] = __uint_as_float(pw.w);
+            sw[threadIdx.x] = pw;
         }
         __syncthreads();
     for (int k0 = 0; k0 < nc; k0 += blockDim.x >> 3) {
@@ -619,7 +615,7 @@ __global__ __launch_bounds__(256) void k_gather_tiles(GatherTileArgs a) {
             ez += __shfl_xor(ez, o, 64);
         }
         if (live && j == 0) {
-            const int s = a.slot_of[p];
+            const int s = p;   // the bin holds slots
             a.fpme[s] = q * ex * a.inv2h[0];
             a.fpme[a.nslots + s] = q * ey * a.inv2h[1];
             a.fpme[2 * a.nslots + s] = q * ez * a.inv2h[2];
@@ -676,7 +672,7 @@ TileArgs tile_args(Engine& e, const uint4* w) {
     for (int d = 0; d < 3; d++) { a.T[d] = m.T[d]; a.E[d] = m.E[d]; }
     a.margin = m.margin;
     a.bincap = m.bincap;
-    a.w = w; a.prm = e.p.prm; a.bin_atoms = m.bin_atoms; a.bin_count = m.bin_count; a.tblock = m.tblock; a.flags = m.flags;
+    a.bin_pos = m.bin_pos; a.bin_count = m.bin_count; a.tblock = m.tblock; a.flags = m.flags;
     return a;
 }
 
@@ -690,7 +686,8 @@ BinArgs bin_args(const Engine& e) {
     for (int d = 0; d < 3; d++) { b.Lq[d] = e.Lq[d]; b.K[d] = m.K[d]; b.T[d] = m.T[d]; b.nt[d] = m.nt[d]; }
     b.bincap = m.bincap;
     b.bin_count = m.bin_count;
-    b.bin_atoms = m.bin_atoms;
+    b.bin_slot = m.bin_slot;
+    b.bin_pos = m.bin_pos;
     b.flags = m.flags;
     return b;
 }
@@ -802,7 +799,8 @@ void pme_alloc(Engine& e, const std::vector<float>& q, const std::vector<uint4>&
             }
             HIP_CHECK(hipMalloc(&m.bin_count, (m.ntiles + 1) * sizeof(int)));
             HIP_CHECK(hipMemset(m.bin_count, 0, (m.ntiles + 1) * sizeof(int)));   // later cleared by the gather
-            HIP_CHECK(hipMalloc(&m.bin_atoms, (int64_t)m.ntiles * m.bincap * sizeof(int)));
+            HIP_CHECK(hipMalloc(&m.bin_slot, (int64_t)m.ntiles * m.bincap * sizeof(int)));
+            HIP_CHECK(hipMalloc(&m.bin_pos, (int64_t)m.ntiles * m.bincap * sizeof(uint4)));
             HIP_CHECK(hipMalloc(&m.tblock, (int64_t)m.ntiles * m.E[0] * m.E[1] * m.E[2] * sizeof(long long)));
             m.flags = e.dflags + DF_PMEBIN;
             // per grid coordinate: the (at most two) tiles whose block covers it and the local index
@@ -834,7 +832,7 @@ void pme_alloc(Engine& e, const std::vector<float>& q, const std::vector<uint4>&
 
 void pme_free(Engine& e) {
     Pme& m = e.pme;
-    void* ptrs[] = {m.qgrid, m.phi, m.cgrid, m.green, m.tw[0], m.tw[1], m.tw[2], m.bin_count, m.bin_atoms, m.tblock,
+    void* ptrs[] = {m.qgrid, m.phi, m.cgrid, m.green, m.tw[0], m.tw[1], m.tw[2], m.bin_count, m.bin_slot, m.bin_pos, m.tblock,
                     m.cover[0], m.cover[1], m.cover[2]};
     for (void* p : ptrs) if (p) (void)hipFree(p);
 }
@@ -843,10 +841,13 @@ void pme_grow_bins(Engine& e) {
     Pme& m = e.pme;
     if (!m.tiled) return;
     HIP_CHECK(hipDeviceSynchronize());   // no kernel may still use the old bins
-    if (m.bin_atoms) HIP_CHECK(hipFree(m.bin_atoms));
-    m.bin_atoms = nullptr;
+    if (m.bin_slot) HIP_CHECK(hipFree(m.bin_slot));
+    if (m.bin_pos) HIP_CHECK(hipFree(m.bin_pos));
+    m.bin_slot = nullptr;
+    m.bin_pos = nullptr;
     m.bincap *= 2;
-    HIP_CHECK(hipMalloc(&m.bin_atoms, (int64_t)m.ntiles * m.bincap * sizeof(int)));
+    HIP_CHECK(hipMalloc(&m.bin_slot, (int64_t)m.ntiles * m.bincap * sizeof(int)));
+    HIP_CHECK(hipMalloc(&m.bin_pos, (int64_t)m.ntiles * m.bincap * sizeof(uint4)));
     e.step_graph_valid = false;   // bin arguments are part of the captured kernels
 }
 
@@ -916,7 +917,7 @@ void launch_pme(Engine& e, const uint4* w, bool energy, hipStream_t s) {
         gt.bincap = m.bincap;
         gt.nslots = slots_max(e);
         gt.eself = (float)m.eself_total;
-        gt.w = w; gt.prm = e.p.prm; gt.phi = m.phi; gt.slot_of = e.nl.slot_of; gt.bin_atoms = m.bin_atoms;
+        gt.phi = m.phi; gt.bin_slot = m.bin_slot; gt.bin_pos = m.bin_pos;
         gt.bin_count = m.bin_count; gt.fpme = e.fpme; gt.epme = e.epme; gt.ephi = e.ephi;
         const size_t lds = (size_t)(m.T[0] + 5) * (m.T[1] + 5) * (m.T[2] + 5) * sizeof(float);
         if (energy) hipLaunchKernelGGL(k_gather_tiles<true>, dim3(m.ntiles), dim3(256), lds, s, gt);
