@@ -27,14 +27,6 @@ namespace {
 
 inline int nblk(int64_t n, int b) { return (int)((n + b - 1) / b); }
 
-// block index remap so each XCD (blocks dispatched round-robin over 8 XCDs) gets a contiguous,
-// spatially coherent range of clusters and so shares j data in its own L2.
-__device__ __forceinline__ int xcd_block(int b, int nb) {
-    const int q = nb >> 3, r = nb & 7;
-    const int x = b & 7, k = b >> 3;
-    return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
-}
-
 // Slot k of the cluster list: position relative to the cluster reference point and charge, and
 // (tiled PME) the charged particle put in its tile bin.  nslots: the slot capacity (SoA stride);
 // slots of clusters beyond the current count are left alone.  The force buffers need no clearing:

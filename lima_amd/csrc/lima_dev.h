@@ -28,6 +28,22 @@ __device__ __forceinline__ void shift_code_axes(int cd, int* cx, int* cy, int* c
 
 __device__ __forceinline__ int64_t unwrapped(uint32_t w, int32_t img, int64_t Lq) { return (int64_t)w + (int64_t)img * Lq; }
 
+// Block index remap: workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
+// Block b runs on XCD b mod 8; the remap gives each XCD a contiguous range of logical blocks
+// (a bijection on [0, nb)), so neighbouring logical blocks, which share data, share an L2.
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+    const int q = nb >> 3, r = nb & 7;
+    const int x = b & 7, k = b >> 3;
+    return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+// The same for a 2D grid: (x, y) of the logical block, x fastest.
+__device__ __forceinline__ int2 xcd_block2() {
+    const int nb = gridDim.x * gridDim.y;
+    const int l = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, nb);
+    const int y = l / gridDim.x;
+    return make_int2(l - y * gridDim.x, y);
+}
+
 // Sum over aligned groups of 8 lanes: quad_perm xor1, quad_perm xor2, row_half_mirror.
 // Every lane of the group ends up with the group sum.
 __device__ __forceinline__ float sum8(float v) {

@@ -187,7 +187,7 @@ struct TileArgs {
 // whose origin is the tile corner minus one node (E = T + 3); the block is stored as is.
 __global__ __launch_bounds__(256) void k_spread_tiles(TileArgs a) {
     extern __shared__ unsigned long long blk[];
-    const int t = blockIdx.x;
+    const int t = xcd_block(blockIdx.x, gridDim.x);
     const int Ex = a.E[0], Ey = a.E[1], E3 = a.E[0] * a.E[1] * a.E[2];
     const int tx = t % a.nt[0], ty = (t / a.nt[0]) % a.nt[1], tz = t / (a.nt[0] * a.nt[1]);
     const int o[3] = {tx * a.T[0] - 1 - a.margin, ty * a.T[1] - 1 - a.margin, tz * a.T[2] - 1 - a.margin};
@@ -255,7 +255,8 @@ __global__ __launch_bounds__(256) void k_fft_x_fwd_tiled(FftPlan P, int Ky, int 
                                                          float2* __restrict__ cg) {
     extern __shared__ float2 lds[];
     const int N = P.N;
-    const int z = blockIdx.y, y0 = blockIdx.x * TY;
+    const int2 lb = xcd_block2();
+    const int z = lb.y, y0 = lb.x * TY;
     const int nl = min(TY, Ky - y0);
     float2* a = lds;
     float2* b = lds + TY * N;
@@ -277,7 +278,8 @@ __global__ __launch_bounds__(256) void k_fft_x_fwd(FftPlan P, int Ky, int H, int
                                                    const long long* __restrict__ qgrid, float2* __restrict__ cg) {
     extern __shared__ float2 lds[];
     const int N = P.N;
-    const int z = blockIdx.y, y0 = blockIdx.x * TY;
+    const int2 lb = xcd_block2();
+    const int z = lb.y, y0 = lb.x * TY;
     const int nl = min(TY, Ky - y0);
     float2* a = lds;
     float2* b = lds + TY * N;
@@ -299,7 +301,8 @@ __global__ __launch_bounds__(256) void k_fft_x_inv(FftPlan P, int Ky, int H, int
                                                    const float2* __restrict__ cg, float* __restrict__ phi) {
     extern __shared__ float2 lds[];
     const int N = P.N;
-    const int z = blockIdx.y, y0 = blockIdx.x * TY;
+    const int2 lb = xcd_block2();
+    const int z = lb.y, y0 = lb.x * TY;
     const int nl = min(TY, Ky - y0);
     float2* a = lds;
     float2* b = lds + TY * N;
@@ -347,7 +350,8 @@ __global__ __launch_bounds__(256) void k_fft_strided(FftPlan P, int H, int TK, i
                                                      PmeVir v) {
     extern __shared__ float2 lds[];
     const int N = P.N;
-    const int o = blockIdx.y, k0 = blockIdx.x * TK;
+    const int2 lb = xcd_block2();
+    const int o = lb.y, k0 = lb.x * TK;
     const int nl = min(TK, H - k0);
     float2* a = lds;
     float2* b = lds + TK * N;
@@ -409,7 +413,7 @@ __global__ __launch_bounds__(256) void k_fft_strided(FftPlan P, int H, int TK, i
             if (threadIdx.x == 0) {
                 double vd[6];
                 for (int c = 0; c < 6; c++) vd[c] = ((red[c][0] + red[c][1]) + red[c][2]) + red[c][3];
-                write_record(v.rec + ((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * ACC_N, 0.0, 0.0, 0.0, 0.0, 0.0, vd);
+                write_record(v.rec + ((int64_t)o * gridDim.x + lb.x) * ACC_N, 0.0, 0.0, 0.0, 0.0, 0.0, vd);
             }
         }
         __syncthreads();
@@ -521,7 +525,7 @@ __device__ __forceinline__ float pick6(const float* v, int i) {   // v[i] for a 
 template <bool ENERGY>
 __global__ __launch_bounds__(256) void k_gather_tiles(GatherTileArgs a) {
     extern __shared__ float phib[];
-    const int t = blockIdx.x;
+    const int t = xcd_block(blockIdx.x, gridDim.x);
     const int G0 = a.T[0] + 5, G1 = a.T[1] + 5, G2 = a.T[2] + 5;
     const int tx = t % a.nt[0], ty = (t / a.nt[0]) % a.nt[1], tz = t / (a.nt[0] * a.nt[1]);
     const int o0 = tx * a.T[0] - 2, o1 = ty * a.T[1] - 2, o2 = tz * a.T[2] - 2;
