@@ -31,7 +31,7 @@ struct Particles {
 struct NList {
     int ncx = 1, ncy = 1, ncol = 1;
     int ncl = 0;               // clusters of the current list (host copy, refreshed by nlist_count)
-    int ncl_max = 0;           // capacity: every grid, SoA stride and energy region is sized by it, and
+    int ncl_max = 0;           // capacity: every grid, slot array and energy region is sized by it, and
                                // kernels read the current count from ncl_dev, so a rebuild changes no
                                // kernel argument
     const int* ncl_dev = nullptr;   // = col_count + ncol after a rebuild (device)
@@ -235,18 +235,16 @@ struct Engine {
     NList nl;
     Pme pme;
 
-    // Per-kernel force (SoA [3][slots]) and potE ([slots]) buffers, each written with plain stores
-    // by the one wave that owns the slot; the integrator and the read-back sum them in a fixed order
-    // (pair + intra + bonded + PME), so forces are deterministic.  Slots a kernel never writes (no
-    // bonded terms, no intra pairs, no charge) are zeroed at each list rebuild.
-    float* fpair = nullptr;    // cutoff pairs
-    float* epair = nullptr;
-    float* fintra = nullptr;   // intra / bonded-compound pairs
-    float* eintra = nullptr;
-    float* fb = nullptr;       // bonded terms
-    float* eb = nullptr;
-    float* fpme = nullptr;     // PME gather
-    float* epme = nullptr;
+    // Per-kernel force buffers, one (fx, fy, fz, potE share) record per slot, each written with plain
+    // stores by the one wave that owns the slot; the integrator and the read-back sum them in a fixed
+    // order (pair + intra + bonded + PME), so forces are deterministic.  Slots a kernel never writes
+    // (no bonded terms, no intra pairs, no charge) are zeroed at each list rebuild.  One 16-byte
+    // record per slot: the integrator, which reads them by slot in particle order, gathers four
+    // records instead of twelve scattered floats.
+    float4* fpair = nullptr;   // cutoff pairs
+    float4* fintra = nullptr;  // intra / bonded-compound pairs
+    float4* fb = nullptr;      // bonded terms
+    float4* fpme = nullptr;    // PME gather
     float* ephi = nullptr;     // [slots] 1/2 q phi of charged slots (energy evaluations), summed in slot
                                // order by the energy reduction (the gather's bin order is not fixed)
     double* acc = nullptr;     // [ACC_N]
@@ -334,7 +332,7 @@ ERegions energy_regions(const Engine& e);
 void launch_reduce_energy(Engine& e, bool kinetic, hipStream_t s);
 void launch_pack(Engine& e, const uint4* w, hipStream_t s);
 BinArgs bin_args(const Engine& e);   // PME tile bins (bin_count == nullptr when PME is not tiled)
-inline int slots_max(const Engine& e) { return e.nl.ncl_max * CLUSTER; }   // SoA stride of the slot arrays
+inline int slots_max(const Engine& e) { return e.nl.ncl_max * CLUSTER; }   // length of the slot arrays
 // 32-bit Coord differences are exact when the box, the list radius and a column's width stay below
 // 2^31 Coord units together (boxes up to ~9.4 nm); the list and pair kernels pick their path by it
 inline bool narrow_coords(const Engine& e) {

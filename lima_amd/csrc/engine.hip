@@ -36,7 +36,6 @@ int64_t fdiv(int64_t a, int64_t b) { int64_t q = a / b; if ((a % b != 0) && ((a 
 // v(t) = v(t-dt) + (F(t) + F(t-dt)) dt/2m ;  x += Coord{ v(t) dt + F(t) dt^2 / 2m } (truncating)
 struct IntegrateArgs {
     int64_t N;
-    int nslots;
     long long Lq[3];
     float dt;
     const float* thermostat;   // [0] velocity scale factor (Thermostat.cuh:56-67), set on the device
@@ -45,10 +44,10 @@ struct IntegrateArgs {
     float2* dpart;         // per wave: running maxima of the two largest squared displacements since the list build
     double* epart;         // kinetic-energy region: one record per wave
     const int* slot_of;
-    const float* fpair;    // per-kernel force buffers [3][nslots], summed in this order
-    const float* fintra;
-    const float* fb;
-    const float* fpme;
+    const float4* fpair;   // per-kernel force records [nslots], summed in this order
+    const float4* fintra;
+    const float4* fb;
+    const float4* fpme;
     uint4* w;
     int4* img;
     float4* v;
@@ -97,10 +96,10 @@ __global__ void k_integrate(IntegrateArgs a) {
         const int s = a.slot_of[p];
         // every load before the first store: the compiler cannot tell the arrays apart, and loads
         // issued after a store would each add a memory round trip to this latency-bound kernel
-        const int n1 = a.nslots, n2 = 2 * a.nslots;
-        const float fx = ((a.fpair[s] + a.fintra[s]) + a.fb[s]) + a.fpme[s];
-        const float fy = ((a.fpair[n1 + s] + a.fintra[n1 + s]) + a.fb[n1 + s]) + a.fpme[n1 + s];
-        const float fz = ((a.fpair[n2 + s] + a.fintra[n2 + s]) + a.fb[n2 + s]) + a.fpme[n2 + s];
+        const float4 f0 = a.fpair[s], f1 = a.fintra[s], f2 = a.fb[s], f3 = a.fpme[s];
+        const float fx = ((f0.x + f1.x) + f2.x) + f3.x;
+        const float fy = ((f0.y + f1.y) + f2.y) + f3.y;
+        const float fz = ((f0.z + f1.z) + f2.z) + f3.z;
         bad = !(fabsf(fx) <= 3.0e38f && fabsf(fy) <= 3.0e38f && fabsf(fz) <= 3.0e38f);
         const float4 prm = a.prm[p];
         const float m = prm.w;
@@ -205,18 +204,12 @@ __global__ __launch_bounds__(256) void k_thermostat(const double* __restrict__ k
 
 // The per-kernel force and potE buffers of the last evaluation, summed in the integrator's order.
 struct SlotForces {
-    int nslots;
-    const float* f[4];   // pair, intra, bonded, PME: [3][nslots]
-    const float* e[4];   // [nslots]
+    const float4* f[4];   // pair, intra, bonded, PME: [nslots] (fx, fy, fz, potE share)
 };
 __device__ __forceinline__ float4 slot_force(const SlotForces& b, int s) {
-    float4 r;
-    const int n1 = b.nslots, n2 = 2 * b.nslots;
-    r.x = ((b.f[0][s] + b.f[1][s]) + b.f[2][s]) + b.f[3][s];
-    r.y = ((b.f[0][n1 + s] + b.f[1][n1 + s]) + b.f[2][n1 + s]) + b.f[3][n1 + s];
-    r.z = ((b.f[0][n2 + s] + b.f[1][n2 + s]) + b.f[2][n2 + s]) + b.f[3][n2 + s];
-    r.w = ((b.e[0][s] + b.e[1][s]) + b.e[2][s]) + b.e[3][s];
-    return r;
+    const float4 f0 = b.f[0][s], f1 = b.f[1][s], f2 = b.f[2][s], f3 = b.f[3][s];
+    return make_float4(((f0.x + f1.x) + f2.x) + f3.x, ((f0.y + f1.y) + f2.y) + f3.y, ((f0.z + f1.z) + f2.z) + f3.z,
+                       ((f0.w + f1.w) + f2.w) + f3.w);
 }
 
 // Forces of the last evaluation in global order (w = per-particle potE) and max |F|.
@@ -463,9 +456,7 @@ void fill_energy(Engine& e, bool with_kinetic, int64_t step) {
 
 SlotForces slot_forces(const Engine& e) {
     SlotForces b{};
-    b.nslots = slots_max(e);
     b.f[0] = e.fpair; b.f[1] = e.fintra; b.f[2] = e.fb; b.f[3] = e.fpme;
-    b.e[0] = e.epair; b.e[1] = e.eintra; b.e[2] = e.eb; b.e[3] = e.epme;
     return b;
 }
 
@@ -493,7 +484,6 @@ void check_finite(Engine& e) {
 void integrate(Engine& e, bool energy, bool pack = false) {
     IntegrateArgs a{};
     a.N = e.N;
-    a.nslots = slots_max(e);
     for (int d = 0; d < 3; d++) a.Lq[d] = e.Lq[d];
     a.dt = (float)e.dt_ns;
     a.thermostat = e.thermo;

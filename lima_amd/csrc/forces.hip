@@ -28,7 +28,7 @@ namespace {
 inline int nblk(int64_t n, int b) { return (int)((n + b - 1) / b); }
 
 // Slot k of the cluster list: position relative to the cluster reference point and charge, and
-// (tiled PME) the charged particle put in its tile bin.  nslots: the slot capacity (SoA stride);
+// (tiled PME) the charged particle put in its tile bin.  nslots: the slot capacity;
 // slots of clusters beyond the current count are left alone.  The force buffers need no clearing:
 // every kernel overwrites the slots it owns.
 __global__ void k_pack(const int* __restrict__ ncl_dev, int nslots, const int* __restrict__ catoms, const uint4* __restrict__ w,
@@ -61,24 +61,19 @@ __global__ void k_pack(const int* __restrict__ ncl_dev, int nslots, const int* _
 
 // The per-kernel force and potE buffers of a new list: slots that a kernel does not own keep zero
 // until the next rebuild; no particle in any slot (catoms -1) until assigned.
-__global__ void k_clear_slots(int ns, int* __restrict__ catoms, float* __restrict__ fintra, float* __restrict__ eintra,
-                              float* __restrict__ fb, float* __restrict__ eb, float* __restrict__ fpme, float* __restrict__ epme,
-                              float* __restrict__ epair, float* __restrict__ ephi) {
+__global__ void k_clear_slots(int ns, int* __restrict__ catoms, float4* __restrict__ fpair, float4* __restrict__ fintra,
+                              float4* __restrict__ fb, float4* __restrict__ fpme, float* __restrict__ ephi) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ns) return;
     catoms[i] = -1;
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-        fintra[(int64_t)d * ns + i] = 0.f;
-        fb[(int64_t)d * ns + i] = 0.f;
-        fpme[(int64_t)d * ns + i] = 0.f;
-    }
-    eintra[i] = 0.f; eb[i] = 0.f; epme[i] = 0.f; epair[i] = 0.f; ephi[i] = 0.f;
+    const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+    fpair[i] = z; fintra[i] = z; fb[i] = z; fpme[i] = z;
+    ephi[i] = 0.f;
 }
 
 struct PairArgs {
     const int* ncl_dev;  // clusters of the current list (device)
-    int ncl_max, nslots, cap;
+    int ncl_max, cap;
     int split;           // waves per i-cluster (each takes a contiguous share of its entries)
     float rc2;
     float zscale;        // COUL_TAB / cutoff^2
@@ -91,8 +86,7 @@ struct PairArgs {
     const uint2* ent;    // list entries: x = j slot | mask << 24, y = shift code | -1 / +1 box bits of axis d at 8 + 2d / 9 + 2d
     const uint4* cref;   // cluster reference points (integer)
     uint32_t lq[3];      // box lengths, integer units (mod 2^32)
-    float* fpair;    // [3][nslots]
-    float* epair;    // [nslots]
+    float4* fpair;   // [nslots] (fx, fy, fz, potE share)
     double* epart;   // this kernel's energy region: one record per wave
 };
 
@@ -269,10 +263,7 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs a) {
     }
     if (a.split == 1) {
         if (live && jj == 0) {
-            a.fpair[islot] = fx;
-            a.fpair[a.nslots + islot] = fy;
-            a.fpair[2 * a.nslots + islot] = fz;
-            if (ENERGY) a.epair[islot] = ep;
+            put_force<ENERGY>(a.fpair, islot, fx, fy, fz, ep);
         }
     } else {   // the cluster's waves meet in LDS; its first wave adds the shares in part order
         if (jj == 0) part_f[wid][ii] = make_float4(fx, fy, fz, ep);
@@ -283,10 +274,7 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs a) {
                 const float4 u = part_f[wid + q][ii];
                 t.x += u.x; t.y += u.y; t.z += u.z; t.w += u.w;
             }
-            a.fpair[islot] = t.x;
-            a.fpair[a.nslots + islot] = t.y;
-            a.fpair[2 * a.nslots + islot] = t.z;
-            if (ENERGY) a.epair[islot] = t.w;
+            put_force<ENERGY>(a.fpair, islot, t.x, t.y, t.z, t.w);
         }
     }
     if (ENERGY) {
@@ -299,7 +287,7 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs a) {
 }
 
 struct IntraArgs {
-    int n_groups, nslots;
+    int n_groups;
     long long Lq[3];
     PairConst pc;
     const int2* groups;   // (first item, item count) per compound A
@@ -310,8 +298,7 @@ struct IntraArgs {
     const float4* prm;
     const int* slot_of;
     const int64_t* first;
-    float* fintra;   // [3][nslots]
-    float* eintra;   // [nslots]
+    float4* fintra;  // [nslots] (fx, fy, fz, potE share)
     double* epart;
 };
 
@@ -413,10 +400,7 @@ __device__ __forceinline__ void intra_items(const IntraArgs& a, int blk, IntraLd
     if (ENERGY) e_i = __shfl_xor(el_s + ec_s, 32, 64) + el_s + ec_s;
     if (half == 0 && i < nA) {
         const int s = a.slot_of[ai];
-        a.fintra[s] = fx;
-        a.fintra[a.nslots + s] = fy;
-        a.fintra[2 * a.nslots + s] = fz;
-        if (ENERGY) a.eintra[s] = e_i;
+        put_force<ENERGY>(a.fintra, s, fx, fy, fz, e_i);
     }
     if (ENERGY) {
         const double tl = wave_sum_d((double)el_s), tc = wave_sum_d((double)ec_s);
@@ -428,7 +412,7 @@ __device__ __forceinline__ void intra_items(const IntraArgs& a, int blk, IntraLd
 }
 
 struct BondArgs {
-    int nb, na, nd, ni, np, nslots;
+    int nb, na, nd, ni, np;
     long long Lq[3];
     PairConst pc;
     const int2* bid; const float2* bp;
@@ -443,8 +427,7 @@ struct BondArgs {
     const uint4* w;
     const int4* img;
     const int* slot_of;
-    float* fb;
-    float* eb;
+    float4* fb;      // [nslots] (fx, fy, fz, potE share)
     double* epart;
 };
 
@@ -547,10 +530,7 @@ __device__ __forceinline__ void bonded_lanes(const BondArgs& a, int blk) {
         const int prev = __shfl_up(la, 1, 64);
         if (code != LANE_PAD && (lane == 0 || prev != la)) {   // the run's first lane
             const int s = a.slot_of[atom];
-            a.fb[s] = f.x;
-            a.fb[a.nslots + s] = f.y;
-            a.fb[2 * a.nslots + s] = f.z;
-            if (ENERGY) a.eb[s] = eshare;
+            put_force<ENERGY>(a.fb, s, f.x, f.y, f.z, eshare);
         }
     } else {   // one atom, its runs starting at the waves' first lanes: the run sums meet in LDS
         __shared__ float4 runs[4];
@@ -561,10 +541,7 @@ __device__ __forceinline__ void bonded_lanes(const BondArgs& a, int blk) {
             float4 t = runs[0];
             for (int r = 1; r < 4; r++) { t.x += runs[r].x; t.y += runs[r].y; t.z += runs[r].z; t.w += runs[r].w; }
             const int s = a.slot_of[atom];
-            a.fb[s] = t.x;
-            a.fb[a.nslots + s] = t.y;
-            a.fb[2 * a.nslots + s] = t.z;
-            if (ENERGY) a.eb[s] = t.w;
+            put_force<ENERGY>(a.fb, s, t.x, t.y, t.z, t.w);
         }
     }
     if (ENERGY) {
@@ -636,8 +613,8 @@ int pair_split(const Engine& e) {
 
 void launch_clear_slots(Engine& e, hipStream_t s) {
     const int ns = slots_max(e);
-    hipLaunchKernelGGL(k_clear_slots, dim3(nblk(ns, 256)), dim3(256), 0, s, ns, e.nl.catoms, e.fintra, e.eintra, e.fb, e.eb,
-                       e.fpme, e.epme, e.epair, e.ephi);
+    hipLaunchKernelGGL(k_clear_slots, dim3(nblk(ns, 256)), dim3(256), 0, s, ns, e.nl.catoms, e.fpair, e.fintra, e.fb, e.fpme,
+                       e.ephi);
 }
 
 ERegions energy_regions(const Engine& e) {
@@ -668,7 +645,6 @@ void launch_pair(Engine& e, bool energy, hipStream_t s) {
     PairArgs a{};
     a.ncl_dev = e.nl.ncl_dev;
     a.ncl_max = e.nl.ncl_max;
-    a.nslots = slots_max(e);
     a.rc2 = e.rc2;
     a.zscale = (float)COUL_TAB / e.rc2;
     a.pc = e.pc;
@@ -677,7 +653,7 @@ void launch_pair(Engine& e, bool energy, hipStream_t s) {
     a.xq = e.nl.xq; a.ljs = e.nl.ljs; a.ecount = e.nl.ecount; a.efar = e.nl.efar; a.ent = e.nl.ent;
     a.cref = e.nl.cref;
     for (int d = 0; d < 3; d++) a.lq[d] = (uint32_t)e.Lq[d];
-    a.fpair = e.fpair; a.epair = e.epair; a.epart = e.epart + (int64_t)e.ereg.pair * ACC_N;
+    a.fpair = e.fpair; a.epart = e.epart + (int64_t)e.ereg.pair * ACC_N;
     a.split = pair_split(e);
     const int64_t waves = (int64_t)a.ncl_max * a.split;
     const dim3 grid(nblk(waves, PAIR_WAVES)), block(64 * PAIR_WAVES);
@@ -688,11 +664,10 @@ void launch_pair(Engine& e, bool energy, hipStream_t s) {
 IntraArgs intra_args(Engine& e, const uint4* w, const int4* img) {
     IntraArgs a{};
     a.n_groups = e.n_intra_groups;
-    a.nslots = slots_max(e);
     for (int d = 0; d < 3; d++) a.Lq[d] = e.Lq[d];
     a.pc = e.pc;
     a.groups = e.intra_groups; a.items = e.intra_items; a.lut = e.intra_lut; a.w = w; a.img = img; a.prm = e.p.prm;
-    a.slot_of = e.nl.slot_of; a.first = e.first; a.fintra = e.fintra; a.eintra = e.eintra;
+    a.slot_of = e.nl.slot_of; a.first = e.first; a.fintra = e.fintra;
     a.epart = e.epart + (int64_t)e.ereg.intra * ACC_N;
     return a;
 }
@@ -700,13 +675,12 @@ IntraArgs intra_args(Engine& e, const uint4* w, const int4* img) {
 BondArgs bond_args(Engine& e, const uint4* w, const int4* img) {
     BondArgs a{};
     a.nb = e.bd.nb; a.na = e.bd.na; a.nd = e.bd.nd; a.ni = e.bd.ni; a.np = e.bd.np;
-    a.nslots = slots_max(e);
     for (int d = 0; d < 3; d++) a.Lq[d] = e.Lq[d];
     a.pc = e.pc;
     a.bid = e.bd.bid; a.bp = e.bd.bp; a.aid = e.bd.aid; a.ap = e.bd.ap; a.did = e.bd.did; a.dp = e.bd.dp;
     a.iid = e.bd.iid; a.ip = e.bd.ip; a.pid = e.bd.pid; a.pp = e.bd.pp;
     a.n_lanes = e.bd.n_lanes; a.multi_lane0 = e.bd.multi_lane0; a.lane_code = e.bd.lane_code; a.lane_atom = e.bd.lane_atom;
-    a.w = w; a.img = img; a.slot_of = e.nl.slot_of; a.fb = e.fb; a.eb = e.eb;
+    a.w = w; a.img = img; a.slot_of = e.nl.slot_of; a.fb = e.fb;
     a.epart = e.epart + (int64_t)e.ereg.bond * ACC_N;
     return a;
 }

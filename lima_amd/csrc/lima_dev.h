@@ -28,6 +28,19 @@ __device__ __forceinline__ void shift_code_axes(int cd, int* cx, int* cy, int* c
 
 __device__ __forceinline__ int64_t unwrapped(uint32_t w, int32_t img, int64_t Lq) { return (int64_t)w + (int64_t)img * Lq; }
 
+// Per-slot force records (fx, fy, fz, potential-energy share).  The energy share is written only by
+// energy evaluations: the other evaluations store 12 bytes and leave it as the last energy
+// evaluation wrote it.
+template <bool ENERGY>
+__device__ __forceinline__ void put_force(float4* buf, int s, float fx, float fy, float fz, float e) {
+    if (ENERGY) {
+        buf[s] = make_float4(fx, fy, fz, e);
+    } else {
+        float* p = reinterpret_cast<float*>(buf + s);
+        p[0] = fx; p[1] = fy; p[2] = fz;
+    }
+}
+
 // Block index remap: workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
 // Block b runs on XCD b mod 8; the remap gives each XCD a contiguous range of logical blocks
 // (a bijection on [0, nb)), so neighbouring logical blocks, which share data, share an L2.

@@ -586,14 +586,12 @@ void nlist_alloc(Engine& e) {
     HIP_CHECK(hipMalloc(&nl.xq, slots * sizeof(float4)));
     HIP_CHECK(hipMalloc(&nl.ecount, (nl.ncl_max + 1) * sizeof(int)));
     HIP_CHECK(hipMalloc(&nl.efar, (nl.ncl_max + 1) * sizeof(int)));
-    for (float** f : {&e.fpair, &e.fintra, &e.fb, &e.fpme}) {
-        HIP_CHECK(hipMalloc(f, 3 * slots * sizeof(float)));
-        HIP_CHECK(hipMemset(*f, 0, 3 * slots * sizeof(float)));
+    for (float4** f : {&e.fpair, &e.fintra, &e.fb, &e.fpme}) {
+        HIP_CHECK(hipMalloc(f, slots * sizeof(float4)));
+        HIP_CHECK(hipMemset(*f, 0, slots * sizeof(float4)));
     }
-    for (float** f : {&e.epair, &e.eintra, &e.eb, &e.epme, &e.ephi}) {
-        HIP_CHECK(hipMalloc(f, slots * sizeof(float)));
-        HIP_CHECK(hipMemset(*f, 0, slots * sizeof(float)));
-    }
+    HIP_CHECK(hipMalloc(&e.ephi, slots * sizeof(float)));
+    HIP_CHECK(hipMemset(e.ephi, 0, slots * sizeof(float)));
     nl.nent_cap = 0;
     nl.cap = e.prm.nlist_capacity > 0 ? (e.prm.nlist_capacity + 7) & ~7 : 2048;
     nl.ent = nullptr;
@@ -603,7 +601,7 @@ void nlist_free(Engine& e) {
     NList& nl = e.nl;
     void* ptrs[] = {nl.col_of, nl.col_count, nl.col_start, nl.col_fill, nl.keys, nl.col_ncl, nl.catoms, nl.slot_of,
                     nl.ccol, nl.cref, nl.cmin, nl.cmax, nl.ljs, nl.comps, nl.cw, nl.xb[0], nl.xb[1], nl.xq, nl.ecount, nl.efar,
-                    nl.ent, e.fpair, e.epair, e.fintra, e.eintra, e.fb, e.eb, e.fpme, e.epme, e.ephi};
+                    nl.ent, e.fpair, e.fintra, e.fb, e.fpme, e.ephi};
     for (void* p : ptrs) if (p) (void)hipFree(p);
 }
 

@@ -428,7 +428,6 @@ __global__ __launch_bounds__(256) void k_fft_strided(FftPlan P, int H, int TK, i
 
 struct GatherArgs {
     int64_t N;
-    int nslots;
     long long Lq[3];
     int K[3];
     float inv2h[3];
@@ -437,8 +436,7 @@ struct GatherArgs {
     const float4* prm;
     const float* phi;
     const int* slot_of;
-    float* fpme;     // [3][nslots], the particle's slot written once
-    float* epme;
+    float4* fpme;    // [nslots] (fx, fy, fz, potE share), the particle's slot written once
     float* ephi;     // [nslots] 1/2 q phi (energy evaluations)
 };
 
@@ -483,13 +481,8 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
                 }
             }
             const int s = a.slot_of[p];
-            a.fpme[s] = q * ex * a.inv2h[0];
-            a.fpme[a.nslots + s] = q * ey * a.inv2h[1];
-            a.fpme[2 * a.nslots + s] = q * ez * a.inv2h[2];
-            if (ENERGY) {
-                a.epme[s] = 0.5f * (q * ph + a.eself);
-                a.ephi[s] = 0.5f * q * ph;
-            }
+            put_force<ENERGY>(a.fpme, s, q * ex * a.inv2h[0], q * ey * a.inv2h[1], q * ez * a.inv2h[2], 0.5f * (q * ph + a.eself));
+            if (ENERGY) a.ephi[s] = 0.5f * q * ph;
         }
     }
 }
@@ -498,15 +491,13 @@ struct GatherTileArgs {
     int K[3], T[3], nt[3];
     long long Lq[3];
     int bincap;
-    int nslots;
     float inv2h[3];
     float eself;
     const float* phi;
     const int* bin_slot;    // the tile's particles: slot, and wrapped Coord with charge bits
     const uint4* bin_pos;
     int* bin_count;
-    float* fpme;
-    float* epme;
+    float4* fpme;
     float* ephi;
 };
 
@@ -620,13 +611,8 @@ __global__ __launch_bounds__(256) void k_gather_tiles(GatherTileArgs a) {
         }
         if (live && j == 0) {
             const int s = p;   // the bin holds slots
-            a.fpme[s] = q * ex * a.inv2h[0];
-            a.fpme[a.nslots + s] = q * ey * a.inv2h[1];
-            a.fpme[2 * a.nslots + s] = q * ez * a.inv2h[2];
-            if (ENERGY) {
-                a.epme[s] = 0.5f * (q * ph + a.eself);
-                a.ephi[s] = 0.5f * q * ph;
-            }
+            put_force<ENERGY>(a.fpme, s, q * ex * a.inv2h[0], q * ey * a.inv2h[1], q * ez * a.inv2h[2], 0.5f * (q * ph + a.eself));
+            if (ENERGY) a.ephi[s] = 0.5f * q * ph;
         }
     }
         __syncthreads();   // the next chunk overwrites the staged particles
@@ -919,10 +905,9 @@ void launch_pme(Engine& e, const uint4* w, bool energy, hipStream_t s) {
             gt.inv2h[d] = (float)(m.K[d] / (2.0 * e.L[d]));
         }
         gt.bincap = m.bincap;
-        gt.nslots = slots_max(e);
         gt.eself = (float)m.eself_total;
         gt.phi = m.phi; gt.bin_slot = m.bin_slot; gt.bin_pos = m.bin_pos;
-        gt.bin_count = m.bin_count; gt.fpme = e.fpme; gt.epme = e.epme; gt.ephi = e.ephi;
+        gt.bin_count = m.bin_count; gt.fpme = e.fpme; gt.ephi = e.ephi;
         const size_t lds = (size_t)(m.T[0] + 5) * (m.T[1] + 5) * (m.T[2] + 5) * sizeof(float);
         if (energy) hipLaunchKernelGGL(k_gather_tiles<true>, dim3(m.ntiles), dim3(256), lds, s, gt);
         else hipLaunchKernelGGL(k_gather_tiles<false>, dim3(m.ntiles), dim3(256), lds, s, gt);
@@ -930,10 +915,9 @@ void launch_pme(Engine& e, const uint4* w, bool energy, hipStream_t s) {
     }
     GatherArgs ga{};
     ga.N = e.N;
-    ga.nslots = slots_max(e);
     for (int d = 0; d < 3; d++) { ga.Lq[d] = e.Lq[d]; ga.K[d] = m.K[d]; ga.inv2h[d] = (float)(m.K[d] / (2.0 * e.L[d])); }
     ga.eself = (float)m.eself_total;
-    ga.w = w; ga.prm = e.p.prm; ga.phi = m.phi; ga.slot_of = e.nl.slot_of; ga.fpme = e.fpme; ga.epme = e.epme; ga.ephi = e.ephi;
+    ga.w = w; ga.prm = e.p.prm; ga.phi = m.phi; ga.slot_of = e.nl.slot_of; ga.fpme = e.fpme; ga.ephi = e.ephi;
     if (energy) hipLaunchKernelGGL(k_gather<true>, dim3(nblk(e.N, 256)), dim3(256), 0, s, ga);
     else hipLaunchKernelGGL(k_gather<false>, dim3(nblk(e.N, 256)), dim3(256), 0, s, ga);
 }
