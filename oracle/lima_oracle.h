@@ -1,8 +1,8 @@
 /*
  * lima_oracle.h -- host FP64 oracle of LIMA's per-step force path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path and
- * the CPU baseline.  Only tests/, __graft_entry__.smoke() and bench.py's CPU
+ * TEST INFRASTRUCTURE ONLY.  This library is the reference the HIP path is tested
+ * against, and the CPU baseline.  Only tests/, __graft_entry__.smoke() and bench.py's CPU
  * baseline leg may load it; the product library never links or calls it.
  *
  * It restates LIMA's force-field formulas in double precision (OpenMP over

@@ -1,7 +1,7 @@
 """ctypes wrapper of the FP64 oracle (oracle/_build/liblima_oracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
-cpu_baseline leg as the checker / CPU baseline.  The product (lima_amd) never imports it.
+cpu_baseline leg, as the reference the tests compare with and as the CPU baseline.  The product (lima_amd) never imports it.
 """
 import ctypes as C
 from pathlib import Path
