@@ -90,7 +90,7 @@ struct Pme {
     int* bin_count = nullptr;              // [ntiles+1]
     int* bin_slot = nullptr;               // [ntiles][bincap] slot of each binned particle
     uint4* bin_pos = nullptr;              // [ntiles][bincap] its wrapped Coord and charge bits
-    long long* tblock = nullptr;           // [ntiles][E^3] per-tile blocks
+    float* tblock = nullptr;               // [ntiles][E^3] per-tile blocks (charge density, float)
     int4* cover[3] = {nullptr, nullptr, nullptr};   // per grid coordinate: (tile a, local a, tile b, local b)
     int* flags = nullptr;                  // the engine's DF_PMEBIN word: a bin overflowed / a particle left its block
 };

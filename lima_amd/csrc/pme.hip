@@ -179,12 +179,14 @@ struct TileArgs {
     int bincap;
     const uint4* bin_pos;   // the tile's particles: wrapped Coord, charge bits
     int* bin_count;
-    long long* tblock;
+    float* tblock;
     int* flags;
 };
 
 // One block per tile: particles of the bin add q w_x w_y w_z (fixed point) into an E^3 LDS block
-// whose origin is the tile corner minus one node (E = T + 3); the block is stored as is.
+// whose origin is the tile corner minus one node (E = T + 3).  The block's sums are exact and
+// independent of the order of the atomics; it is stored as float (rounded once), half the bytes
+// of the int64 sums, and the x pass adds the at most 8 blocks covering a node in a fixed order.
 __global__ __launch_bounds__(256) void k_spread_tiles(TileArgs a) {
     extern __shared__ unsigned long long blk[];
     const int t = xcd_block(blockIdx.x, gridDim.x);
@@ -221,16 +223,16 @@ __global__ __launch_bounds__(256) void k_spread_tiles(TileArgs a) {
             }
     }
     __syncthreads();
-    long long* out = a.tblock + (int64_t)t * E3;
-    for (int i = threadIdx.x; i < E3; i += blockDim.x) out[i] = (long long)blk[i];
+    float* out = a.tblock + (int64_t)t * E3;
+    for (int i = threadIdx.x; i < E3; i += blockDim.x) out[i] = __ll2float_rn((long long)blk[i]) * (1.0f / 4294967296.0f);
 }
 
-// fixed-point charge at grid point (x, y, z) summed over the tile blocks that cover it
-__device__ __forceinline__ long long tiled_q(const long long* __restrict__ tblock, const int4* __restrict__ cx,
+// charge at grid point (x, y, z) summed over the tile blocks that cover it, in a fixed order
+__device__ __forceinline__ float tiled_q(const float* __restrict__ tblock, const int4* __restrict__ cx,
                                             const int4* __restrict__ cy, const int4* __restrict__ cz, int x, int y, int z,
                                             int Ex, int Ey, int E3, int ntx, int nty) {
     const int4 X = cx[x], Y = cy[y], Z = cz[z];
-    long long acc = 0;
+    float acc = 0.f;
     const int zt[2] = {Z.x, Z.z}, zl[2] = {Z.y, Z.w};
     const int yt[2] = {Y.x, Y.z}, yl[2] = {Y.y, Y.w};
     const int xt[2] = {X.x, X.z}, xl[2] = {X.y, X.w};
@@ -249,7 +251,7 @@ __device__ __forceinline__ long long tiled_q(const long long* __restrict__ tbloc
 }
 
 // x pass, forward, reading the tile blocks instead of a global grid.
-__global__ __launch_bounds__(256) void k_fft_x_fwd_tiled(FftPlan P, int Ky, int H, int TY, float scale, const long long* __restrict__ tblock,
+__global__ __launch_bounds__(256) void k_fft_x_fwd_tiled(FftPlan P, int Ky, int H, int TY, float scale, const float* __restrict__ tblock,
                                                          const int4* __restrict__ cx, const int4* __restrict__ cy,
                                                          const int4* __restrict__ cz, int Ex, int Ey, int E3, int ntx, int nty,
                                                          float2* __restrict__ cg) {
@@ -262,8 +264,7 @@ __global__ __launch_bounds__(256) void k_fft_x_fwd_tiled(FftPlan P, int Ky, int 
     float2* b = lds + TY * N;
     for (int i = threadIdx.x; i < nl * N; i += blockDim.x) {
         const int l = i / N, x = i - l * N;
-        const long long v = tiled_q(tblock, cx, cy, cz, x, y0 + l, z, Ex, Ey, E3, ntx, nty);
-        a[i] = make_float2(__ll2float_rn(v) * (1.0f / 4294967296.0f) * scale, 0.f);
+        a[i] = make_float2(tiled_q(tblock, cx, cy, cz, x, y0 + l, z, Ex, Ey, E3, ntx, nty) * scale, 0.f);
     }
     __syncthreads();
     const float2* r = fft_lines(a, b, nl, P, false);
@@ -791,7 +792,7 @@ void pme_alloc(Engine& e, const std::vector<float>& q, const std::vector<uint4>&
             HIP_CHECK(hipMemset(m.bin_count, 0, (m.ntiles + 1) * sizeof(int)));   // later cleared by the gather
             HIP_CHECK(hipMalloc(&m.bin_slot, (int64_t)m.ntiles * m.bincap * sizeof(int)));
             HIP_CHECK(hipMalloc(&m.bin_pos, (int64_t)m.ntiles * m.bincap * sizeof(uint4)));
-            HIP_CHECK(hipMalloc(&m.tblock, (int64_t)m.ntiles * m.E[0] * m.E[1] * m.E[2] * sizeof(long long)));
+            HIP_CHECK(hipMalloc(&m.tblock, (int64_t)m.ntiles * m.E[0] * m.E[1] * m.E[2] * sizeof(float)));
             m.flags = e.dflags + DF_PMEBIN;
             // per grid coordinate: the (at most two) tiles whose block covers it and the local index
             for (int d = 0; d < 3; d++) {
