@@ -543,21 +543,32 @@ __global__ __launch_bounds__(256) void k_gather_tiles(GatherTileArgs a) {
         }
     }
     __syncthreads();
-    // the bin's particles (slot, charge, position) staged in LDS, 256 at a time, read contiguously
-    // from the bin records
-    __shared__ int sp[256];
-    __shared__ float sq[256];
-    __shared__ uint4 sw[256];
+    // the bin's particles staged in LDS, GCHUNK at a time, read contiguously from the bin records:
+    // one thread per particle computes its slot, charge, node-local base and fractional grid
+    // coordinates once (the grid-coordinate division is FP64), and the 8 lanes that gather it
+    // read them back
+    constexpr int GCHUNK = 256;
+    __shared__ int sp[GCHUNK];
+    __shared__ float sq[GCHUNK];
+    __shared__ float4 sf[GCHUNK];   // fractional coordinates, and the local base node l0 | l1 << 8 | l2 << 16
     const int nb = min(a.bin_count[t], a.bincap);
     const int j = threadIdx.x & 7;
-    for (int c0 = 0; c0 < nb; c0 += 256) {   // uniform trip counts per block
-        const int nc = min(256, nb - c0);
+    for (int c0 = 0; c0 < nb; c0 += GCHUNK) {   // uniform trip counts per block
+        const int nc = min(GCHUNK, nb - c0);
         if ((int)threadIdx.x < nc) {
             const int64_t e = (int64_t)t * a.bincap + c0 + threadIdx.x;
             const uint4 pw = a.bin_pos[e];
             sp[threadIdx.x] = a.bin_slot[e];
             sq[threadIdx.x] = __uint_as_float(pw.w);
-            sw[threadIdx.x] = pw;
+            int ix[3];
+            float f[3];
+            grid_coord(pw.x, a.Lq[0], a.K[0], &ix[0], &f[0]);
+            grid_coord(pw.y, a.Lq[1], a.K[1], &ix[1], &f[1]);
+            grid_coord(pw.z, a.Lq[2], a.K[2], &ix[2], &f[2]);
+            // the particle's base node lies in this tile (binned from the same positions), so its
+            // stencil ix-2 .. ix+3 is local nodes ix - corner .. ix - corner + 5
+            const int lb = (ix[0] - tx * a.T[0]) | ((ix[1] - ty * a.T[1]) << 8) | ((ix[2] - tz * a.T[2]) << 16);
+            sf[threadIdx.x] = make_float4(f[0], f[1], f[2], __int_as_float(lb));
         }
         __syncthreads();
     for (int k0 = 0; k0 < nc; k0 += blockDim.x >> 3) {
@@ -567,20 +578,15 @@ __global__ __launch_bounds__(256) void k_gather_tiles(GatherTileArgs a) {
         float q = 0.f, ph = 0.f, ex = 0.f, ey = 0.f, ez = 0.f;
         if (live) {
             q = sq[k];
-            const uint4 pw = sw[k];
-            int ix[3];
-            float f[3], wx[4], wy[4], wz[4];
-            grid_coord(pw.x, a.Lq[0], a.K[0], &ix[0], &f[0]);
-            grid_coord(pw.y, a.Lq[1], a.K[1], &ix[1], &f[1]);
-            grid_coord(pw.z, a.Lq[2], a.K[2], &ix[2], &f[2]);
-            bspline(f[0], wx); bspline(f[1], wy); bspline(f[2], wz);
+            const float4 fk = sf[k];
+            float wx[4], wy[4], wz[4];
+            bspline(fk.x, wx); bspline(fk.y, wy); bspline(fk.z, wz);
             float w6x[6], w6y[6], w6z[6], d6x[6], d6y[6], d6z[6];
             gather_weights(wx, w6x, d6x);
             gather_weights(wy, w6y, d6y);
             gather_weights(wz, w6z, d6z);
-            // the particle's base node lies in this tile (binned from the same positions), so its
-            // stencil ix-2 .. ix+3 is local nodes ix - corner .. ix - corner + 5
-            const int l0 = ix[0] - tx * a.T[0], l1 = ix[1] - ty * a.T[1], l2 = ix[2] - tz * a.T[2];
+            const int lb = __float_as_int(fk.w);
+            const int l0 = lb & 0xff, l1 = (lb >> 8) & 0xff, l2 = lb >> 16;
             if (j < 6) {
                 const float wz6 = pick6(w6z, j), dz6 = pick6(d6z, j);
                 // per row: the x sums with the weights and their differences (two FMAs per node),
