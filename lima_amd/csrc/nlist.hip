@@ -282,14 +282,20 @@ __device__ __forceinline__ void axis_pick(const AxisRanges& r, int k, int* col, 
 
 constexpr int NL_FLAT = 512;   // flat j-cluster table per wave (LDS), filled in chunks
 
+// The i box moved by minus one image shift: a j box or particle shifted by s lies as far from the
+// i box as the unshifted one from this box (exact integer arithmetic either way).  One per shift
+// code, built once per wave, so the tests below need no shift decoding.
+struct ShiftedBox {
+    long long lo[3], hi[3];
+};
+
 // Is the shifted box of j-cluster J within R of the i box?  A particle is never nearer than its
 // cluster's box, so dropping J here never drops an entry.
-__device__ __forceinline__ bool jbox_near(const NlArgs& a, int J, int code, const long long* mn, const long long* mx) {
+__device__ __forceinline__ bool jbox_near(const NlArgs& a, int J, const ShiftedBox& sb) {
     const uint4 lo = a.cmin[J], hi = a.cmax[J];
-    const long long sx = shift_of(code % 3, a.Lq[0]), sy = shift_of((code / 3) % 3, a.Lq[1]), sz = shift_of(code / 9, a.Lq[2]);
-    const long long bx = min(gap1((long long)lo.x + sx, (long long)hi.x + sx, mn[0], mx[0]), a.R + 1);
-    const long long by = min(gap1((long long)lo.y + sy, (long long)hi.y + sy, mn[1], mx[1]), a.R + 1);
-    const long long bz = min(gap1((long long)lo.z + sz, (long long)hi.z + sz, mn[2], mx[2]), a.R + 1);
+    const long long bx = min(gap1((long long)lo.x, (long long)hi.x, sb.lo[0], sb.hi[0]), a.R + 1);
+    const long long by = min(gap1((long long)lo.y, (long long)hi.y, sb.lo[1], sb.hi[1]), a.R + 1);
+    const long long bz = min(gap1((long long)lo.z, (long long)hi.z, sb.lo[2], sb.hi[2]), a.R + 1);
     return bx * bx + by * by + bz * bz <= a.R2;
 }
 
@@ -309,6 +315,8 @@ __device__ __forceinline__ bool jbox_near(const NlArgs& a, int J, int code, cons
 template <bool NARROW>
 __global__ __launch_bounds__(256) void k_nlist(NlArgs a) {
     __shared__ uint32_t flat_all[4][NL_FLAT], cand_all[4][NL_FLAT];
+    __shared__ ShiftedBox sbox_all[4][27];   // per wave: the i box moved by minus each image shift
+    __shared__ uint32_t yword_all[4][27];    // per wave: an entry's y word for each shift code
     const int lane = threadIdx.x & 63;
     // the wave's cluster, marked wave-uniform so that everything derived from it (bounds, compound
     // ids, column ranges) is loaded and kept in scalar registers
@@ -319,6 +327,26 @@ __global__ __launch_bounds__(256) void k_nlist(NlArgs a) {
     const uint4 bmn = a.cmin[I], bmx = a.cmax[I];
     const long long mn[3] = {(long long)bmn.x, (long long)bmn.y, (long long)bmn.z};
     const long long mx[3] = {(long long)bmx.x, (long long)bmx.y, (long long)bmx.z};
+    ShiftedBox* sbox = sbox_all[threadIdx.x >> 6];
+    uint32_t* yword = yword_all[threadIdx.x >> 6];
+    if (lane < 27) {   // code = sx + 3 sy + 9 sz, each 0..2 for -1, 0, +1 box
+        int cs[3];
+        shift_code_axes(lane, &cs[0], &cs[1], &cs[2]);
+        ShiftedBox b;
+        uint32_t yw = (uint32_t)lane;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const long long sh = shift_of(cs[d], a.Lq[d]);
+            b.lo[d] = mn[d] - sh;
+            b.hi[d] = mx[d] - sh;
+            yw |= (uint32_t)(cs[d] == 0) << (8 + 2 * d) | (uint32_t)(cs[d] == 2) << (9 + 2 * d);
+        }
+        sbox[lane] = b;
+        yword[lane] = yw;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     // the i slots' compounds and their bonded-compound bits, wave-uniform (scalar registers); an
     // empty slot gets a compound id no particle has, so it never matches
     int ci[CLUSTER];
@@ -388,7 +416,7 @@ __global__ __launch_bounds__(256) void k_nlist(NlArgs a) {
                 uint32_t t = 0;
                 if (k < k1) {
                     t = cand[k - k0];
-                    near = jbox_near(a, (int)(t & 0xffffffu), (int)(t >> 24), mn, mx);
+                    near = jbox_near(a, (int)(t & 0xffffffu), sbox[t >> 24]);
                 }
                 const unsigned long long bal = __ballot(near);
                 if (near) flat[c1 + __popcll(bal & ((1ull << lane) - 1ull))] = t;
@@ -415,25 +443,22 @@ __global__ __launch_bounds__(256) void k_nlist(NlArgs a) {
                 uint32_t mask = 0;
                 bool far = false;      // the j image is farther than the cutoff from the whole i box
                 if (cd >= 0 && pb.w == 0u) {
-                    int cxi, cyi, czi;
-                    shift_code_axes(cd, &cxi, &cyi, &czi);
+                    const ShiftedBox& sb = sbox[cd];
                     bool in;
                     if (NARROW) {
                         // Lq + R + cluster extents < 2^31: every difference below is an exact int32
-                        const int px = (int)(pb.x + (uint32_t)shift_of(cxi, a.Lq[0])), py = (int)(pb.y + (uint32_t)shift_of(cyi, a.Lq[1]));
-                        const int pz = (int)(pb.z + (uint32_t)shift_of(czi, a.Lq[2]));
+                        const int px = (int)pb.x, py = (int)pb.y, pz = (int)pb.z;
                         const uint32_t R1 = (uint32_t)a.R + 1u;
-                        const uint32_t gx = min((uint32_t)max(0, max((int)bmn.x - px, px - (int)bmx.x)), R1);
-                        const uint32_t gy = min((uint32_t)max(0, max((int)bmn.y - py, py - (int)bmx.y)), R1);
-                        const uint32_t gz = min((uint32_t)max(0, max((int)bmn.z - pz, pz - (int)bmx.z)), R1);
+                        const uint32_t gx = min((uint32_t)max(0, max((int)sb.lo[0] - px, px - (int)sb.hi[0])), R1);
+                        const uint32_t gy = min((uint32_t)max(0, max((int)sb.lo[1] - py, py - (int)sb.hi[1])), R1);
+                        const uint32_t gz = min((uint32_t)max(0, max((int)sb.lo[2] - pz, pz - (int)sb.hi[2])), R1);
                         const unsigned long long g2 = (unsigned long long)gx * gx + (unsigned long long)gy * gy + (unsigned long long)gz * gz;
                         in = g2 <= (unsigned long long)a.R2;
                         far = g2 > (unsigned long long)a.F2;
                     } else {
-                        const long long sx = shift_of(cxi, a.Lq[0]), sy = shift_of(cyi, a.Lq[1]), sz = shift_of(czi, a.Lq[2]);
-                        const long long gx = min(gap1((long long)pb.x + sx, (long long)pb.x + sx, mn[0], mx[0]), a.R + 1);
-                        const long long gy = min(gap1((long long)pb.y + sy, (long long)pb.y + sy, mn[1], mx[1]), a.R + 1);
-                        const long long gz = min(gap1((long long)pb.z + sz, (long long)pb.z + sz, mn[2], mx[2]), a.R + 1);
+                        const long long gx = min(gap1((long long)pb.x, (long long)pb.x, sb.lo[0], sb.hi[0]), a.R + 1);
+                        const long long gy = min(gap1((long long)pb.y, (long long)pb.y, sb.lo[1], sb.hi[1]), a.R + 1);
+                        const long long gz = min(gap1((long long)pb.z, (long long)pb.z, sb.lo[2], sb.hi[2]), a.R + 1);
                         const long long g2 = gx * gx + gy * gy + gz * gz;
                         in = g2 <= a.R2;
                         far = g2 > a.F2;
@@ -467,12 +492,7 @@ __global__ __launch_bounds__(256) void k_nlist(NlArgs a) {
                     const int pos = far ? cfar + __popcll(bf & lt) : cnear + __popcll(bn & lt);
                     if (pos < a.cap) {   // near from the front, far from the back; a collision is an overflow
                         const int64_t idx = base + (far ? a.cap - 1 - pos : pos);
-                        int cxi, cyi, czi;
-                        shift_code_axes(cd, &cxi, &cyi, &czi);
-                        a.ent[idx] = make_uint2((uint32_t)(J * CLUSTER + sl) | (mask << 24),
-                                                (uint32_t)cd | (uint32_t)(cxi == 0) << 8 | (uint32_t)(cxi == 2) << 9 |
-                                                    (uint32_t)(cyi == 0) << 10 | (uint32_t)(cyi == 2) << 11 |
-                                                    (uint32_t)(czi == 0) << 12 | (uint32_t)(czi == 2) << 13);
+                        a.ent[idx] = make_uint2((uint32_t)(J * CLUSTER + sl) | (mask << 24), yword[cd]);
                     }
                 }
                 cnear += __popcll(bn);
