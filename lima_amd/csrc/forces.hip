@@ -99,9 +99,10 @@ struct PairArgs {
 // and the j cluster's reference point, forms the shift between the two clusters' reference
 // points (an exact integer difference, rounded once to float: the value the list build would
 // compute), adds it to the j position (stored relative to its own reference point) and stores
-// (x, y, z, q) and (sigma/2, sqrt(24 eps), mask bits) in the wave's LDS slice.  The 8 lanes of a
-// group then read their entry with two broadcast ds_read_b128, so the texture path sees each
-// datum once instead of eight times.  Two LDS buffers: chunk c + 1 is loaded while c is computed.
+// (x, y, z, mask bits) and (sigma/2, sqrt(24 eps), q) in the wave's LDS slice.  The 8 lanes of a
+// group then read their entry with broadcast LDS reads, the first record in every pass and the
+// second only in passes with a lane inside the cutoff, so the texture path sees each datum once
+// instead of eight times.  Two LDS buffers: chunk c + 1 is loaded while c is computed.
 // A cluster's near entries come first, its far entries (beyond the cutoff from its box at the
 // build) last.  Before an entry is staged, the gap between its j position and the bounding box of
 // the cluster's own particles at this step is compared with the cutoff: an entry farther away
@@ -194,8 +195,9 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs a) {
         const float shx = (float)(int)(grj.x - ri.x + (lqx & bitmask(9)) - (lqx & bitmask(8))) * LIMA_LIMA_TO_NANO_F;
         const float shy = (float)(int)(grj.y - ri.y + (lqy & bitmask(11)) - (lqy & bitmask(10))) * LIMA_LIMA_TO_NANO_F;
         const float shz = (float)(int)(grj.z - ri.z + (lqz & bitmask(13)) - (lqz & bitmask(12))) * LIMA_LIMA_TO_NANO_F;
-        const float4 ga = make_float4(gxj.x + shx, gxj.y + shy, gxj.z + shz, gxj.w);
-        const float4 gb = make_float4(gpj.x, gpj.y, __uint_as_float(cw >> 24), 0.f);
+        // (x, y, z, mask bits) for the test every pass makes, (sigma/2, sqrt(24 eps), q) for the force
+        const float4 ga = make_float4(gxj.x + shx, gxj.y + shy, gxj.z + shz, __uint_as_float(cw >> 24));
+        const float4 gb = make_float4(gpj.x, gpj.y, gxj.w, 0.f);
         const float gx = fmaxf(fabsf(ga.x - bc[0]) - bh[0], 0.f), gy = fmaxf(fabsf(ga.y - bc[1]) - bh[1], 0.f),
                     gz = fmaxf(fabsf(ga.z - bc[2]) - bh[2], 0.f);
         const bool keep = gx * gx + gy * gy + gz * gz <= rcut2;   // entries past the end sit at PAD_POS
@@ -208,8 +210,8 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs a) {
         }
         const int npad = ((cnt + CLUSTER - 1) & ~(CLUSTER - 1)) - cnt;
         if (lane < npad) {
-            sA[wid][buf][cnt + lane] = make_float4(PAD_POS, PAD_POS, PAD_POS, 0.f);
-            sB[wid][buf][cnt + lane] = make_float4(0.f, 0.f, __uint_as_float(0u), 0.f);
+            sA[wid][buf][cnt + lane] = make_float4(PAD_POS, PAD_POS, PAD_POS, __uint_as_float(0u));
+            sB[wid][buf][cnt + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
         }
         return (cnt + CLUSTER - 1) / CLUSTER;
     };
@@ -225,8 +227,7 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs a) {
         if (c + 1 < nchunk) fetch(c + 1);   // in flight during the compute below
         for (int k = 0; k < nit; k++) {
             const float4 A = sA[wid][buf][k * CLUSTER + jj];
-            const float4 B = sB[wid][buf][k * CLUSTER + jj];
-            const uint32_t m = __float_as_uint(B.z);
+            const uint32_t m = __float_as_uint(A.w);
             const float dx = A.x - xi.x, dy = A.y - xi.y, dz = A.z - xi.z;
             const float r2 = dx * dx + dy * dy + dz * dz;
             const bool inc = (m & ibit) && (r2 <= a.rc2);   // the list masks exclude self and padding pairs
@@ -235,7 +236,8 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs a) {
             if (inc) {
                 if (ENERGY) npairs++;
                 float el = 0.f, ec = 0.f;
-                const float S = pair_scalar_tab(r2, pi.x + B.x, pi.y * B.y, qi_kc * A.w, a.pc, tab, a.zscale,
+                const float4 B = sB[wid][buf][k * CLUSTER + jj];
+                const float S = pair_scalar_tab(r2, pi.x + B.x, pi.y * B.y, qi_kc * B.z, a.pc, tab, a.zscale,
                                                 ENERGY ? &el : nullptr, &ec);
                 fx = fmaf(dx, S, fx); fy = fmaf(dy, S, fy); fz = fmaf(dz, S, fz);
                 if (ENERGY) {
