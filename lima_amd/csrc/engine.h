@@ -197,6 +197,9 @@ struct Engine {
                                // displacement since the list build among the wave's particles
     int list_interval = 20;        // steps per list in use: nlist_interval, lowered after a stale interval
     int builds_clean = 0;          // rebuilds issued since the last stale interval
+    int grow_after = 20;           // clean rebuilds before lists are tried one step longer; doubles
+                                   // (up to 640) with every stale interval, so a length that fails
+                                   // now and then is retried ever more rarely
     int64_t stale_rollbacks = 0;   // intervals undone because the list could have missed a pair
     uint4* w_try = nullptr;
     int4* img_try = nullptr;

@@ -590,6 +590,7 @@ void rollback(Engine& e) {
             throw LimaFail(LIMA_ERR_LIST_OVERFLOW, "particles move more than half the list skin in one step");
         e.list_interval -= std::max(1, e.list_interval / 6);
         e.builds_clean = 0;
+        e.grow_after = std::min(2 * e.grow_after, 640);
         e.stale_rollbacks++;
     }
     hipStream_t s = e.s_main;
@@ -646,8 +647,8 @@ void issue_steps(Engine& e, int64_t target) {
         const int interval = e.list_interval;   // a rollback may shorten it
         if (e.steps_since_build >= interval) {
             if (poll_rebuild_flags(e)) continue;   // rolled back to an earlier step: issue from there
-            // after 20 rebuilds without a stale interval, try lists one step longer
-            if (++e.builds_clean >= 20 && e.list_interval < max_interval) {
+            // after grow_after rebuilds without a stale interval, try lists one step longer
+            if (++e.builds_clean >= e.grow_after && e.list_interval < max_interval) {
                 e.list_interval++;
                 e.builds_clean = 0;
             }
