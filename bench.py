@@ -430,6 +430,8 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run the copy kernel used to calibrate the HBM byte counters")
     ap.add_argument("--no-em", action="store_true", help="skip the batched-minimisation measurement")
+    ap.add_argument("--per-call-steps", type=int, default=1000,
+                    help="steps of the per-call comparison (at least --steps)")
     ap.add_argument("--launch-mode", type=int, default=0,
                     help="lima_simparams.launch_mode: 0 step graph replay with PME on a second stream; "
                          "+1 eager launches, +2 PME in line on one stream")
@@ -485,7 +487,7 @@ def main():
     log(f"timed {args.steps} steps: {ms_per_step:.4f} ms/step, {ns_day:.1f} ns/day; host issue {host_issue_us:.1f} us/step")
     # the per-call path against the multi-step path, on a sample long enough that list rebuilds and
     # the occasional rerun of an interval land in both alike
-    n_pc = max(args.steps, 1000)
+    n_pc = max(args.steps, args.per_call_steps)
     ms_multi = timed_window(eng, n_pc) * 1e3 / n_pc
     pc_ms = per_call_window(eng, n_pc) * 1e3 / n_pc
     log(f"per-call: {n_pc} x lima_step(h, 1): {pc_ms:.4f} ms/step ({pc_ms / ms_multi:.3f} x one {n_pc}-step call, "
