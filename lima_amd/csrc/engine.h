@@ -330,7 +330,6 @@ void reset_flags(Engine& e, int64_t snapshots, hipStream_t s);   // clear the er
 
 // ---- forces.hip ----
 int pair_split(const Engine& e);
-void launch_clear_slots(Engine& e, hipStream_t s);   // zero the per-kernel force / potE buffers, no particle in any slot
 ERegions energy_regions(const Engine& e);
 void launch_reduce_energy(Engine& e, bool kinetic, hipStream_t s);
 void launch_pack(Engine& e, const uint4* w, hipStream_t s);

@@ -59,18 +59,6 @@ __global__ void k_pack(const int* __restrict__ ncl_dev, int nslots, const int* _
     if (bins.bin_count) bin_wave(bins, a >= 0 && q != 0.f, k, p, q);
 }
 
-// The per-kernel force and potE buffers of a new list: slots that a kernel does not own keep zero
-// until the next rebuild; no particle in any slot (catoms -1) until assigned.
-__global__ void k_clear_slots(int ns, int* __restrict__ catoms, float4* __restrict__ fpair, float4* __restrict__ fintra,
-                              float4* __restrict__ fb, float4* __restrict__ fpme, float* __restrict__ ephi) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ns) return;
-    catoms[i] = -1;
-    const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
-    fpair[i] = z; fintra[i] = z; fb[i] = z; fpme[i] = z;
-    ephi[i] = 0.f;
-}
-
 struct PairArgs {
     const int* ncl_dev;  // clusters of the current list (device)
     int ncl_max, cap;
@@ -611,12 +599,6 @@ int pair_split(const Engine& e) {
     int split = 1;
     while (split < PAIR_WAVES && (int64_t)e.nl.ncl_max * split < 16384) split *= 2;
     return split;
-}
-
-void launch_clear_slots(Engine& e, hipStream_t s) {
-    const int ns = slots_max(e);
-    hipLaunchKernelGGL(k_clear_slots, dim3(nblk(ns, 256)), dim3(256), 0, s, ns, e.nl.catoms, e.fpair, e.fintra, e.fb, e.fpme,
-                       e.ephi);
 }
 
 ERegions energy_regions(const Engine& e) {

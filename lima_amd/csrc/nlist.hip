@@ -644,6 +644,9 @@ struct OpenArgs {
     int* col_count; int* col_fill; int ncol1;
     int* flags;
     int* bins; int nbins;
+    // the new list's slots: no particle (catoms -1) until assigned, per-kernel force records and
+    // 1/2 q phi zero (a slot a kernel does not own keeps zero until the next rebuild)
+    int* catoms; float4* fpair; float4* fintra; float4* fb; float4* fpme; float* ephi; int nslots;
 };
 __global__ void k_rebuild_open(OpenArgs a) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -665,6 +668,12 @@ __global__ void k_rebuild_open(OpenArgs a) {
     }
     if (p < a.ncol1) { a.col_count[p] = 0; a.col_fill[p] = 0; }
     if (a.bins && p < a.nbins) a.bins[p] = 0;
+    if (p < a.nslots) {
+        const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+        a.catoms[p] = -1;
+        a.fpair[p] = z; a.fintra[p] = z; a.fb[p] = z; a.fpme[p] = z;
+        a.ephi[p] = 0.f;
+    }
 }
 
 // Clears the error words and sets the snapshot count (after the host has dealt with a condition).
@@ -784,7 +793,10 @@ void nlist_build(Engine& e, const uint4* w, bool checked) {
             o.col_count = nl.col_count; o.col_fill = nl.col_fill; o.ncol1 = nl.ncol + 1;
             o.flags = e.dflags;
             o.bins = clear_bins ? e.pme.bin_count : nullptr; o.nbins = e.pme.ntiles;
-            const int64_t span = std::max<int64_t>(std::max<int64_t>(N, nl.ncol + 1), std::max<int64_t>(1, clear_bins ? e.pme.ntiles : 0));
+            o.catoms = nl.catoms; o.fpair = e.fpair; o.fintra = e.fintra; o.fb = e.fb; o.fpme = e.fpme; o.ephi = e.ephi;
+            o.nslots = slots_max(e);
+            const int64_t span = std::max<int64_t>(std::max<int64_t>(std::max<int64_t>(N, nl.ncol + 1), o.nslots),
+                                                   std::max<int64_t>(1, clear_bins ? e.pme.ntiles : 0));
             hipLaunchKernelGGL(k_rebuild_open, dim3(nblk(span, 256)), dim3(256), 0, s, o);
         }
         const size_t col_lds = nl.ncol <= COL_LDS_MAX ? (size_t)nl.ncol * sizeof(int) : 0;
@@ -799,7 +811,6 @@ void nlist_build(Engine& e, const uint4* w, bool checked) {
         launch_exclusive_scan(nl.col_ncl, nl.col_count, nl.ncol, s);
         const int* col_cl_start = nl.col_count;
         const int ns = slots_max(e);
-        launch_clear_slots(e, s);
         hipLaunchKernelGGL(k_assign_slots, dim3(nblk(N, 256)), dim3(256), 0, s, N, nl.ncol, nl.col_start, col_cl_start, nl.keys,
                            nl.catoms, nl.slot_of);
         hipLaunchKernelGGL(k_cluster_props, dim3(nblk(ns, 256)), dim3(256), 0, s, nl.ncl_dev, nl.ncl_max, nl.ncol, col_cl_start,
