@@ -594,10 +594,13 @@ __global__ void k_reduce_energy_final(const double* __restrict__ part, int npart
 }  // namespace
 
 int pair_split(const Engine& e) {
-    // enough waves to keep ~16 per SIMD in flight (256 CUs x 4 SIMDs) on small boxes; a cluster's
-    // waves stay inside one workgroup (split divides PAIR_WAVES)
+    // small boxes: split clusters until there are ~4 waves per SIMD (256 CUs x 4 SIMDs); a cluster's
+    // waves stay inside one workgroup (split divides PAIR_WAVES).  Past that, one wave per cluster:
+    // at 42k atoms all pair workgroups are then resident at once, and the slots they leave free take
+    // the PME and intra / bonded kernels that run beside them (splitting further kept queueing pair
+    // workgroups into every freed slot: the step was 3 % slower with the pair kernel's own time equal).
     int split = 1;
-    while (split < PAIR_WAVES && (int64_t)e.nl.ncl_max * split < 16384) split *= 2;
+    while (split < PAIR_WAVES && (int64_t)e.nl.ncl_max * split < 4096) split *= 2;
     return split;
 }
 
