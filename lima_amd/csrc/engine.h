@@ -259,8 +259,8 @@ struct Engine {
     float* thermo = nullptr;   // [2] velocity scale factor applied by the integrator, last measured temperature [K]
     bool temp_measured = false;
 
-    hipStream_t s_main = nullptr, s_pme = nullptr, s_local = nullptr;   // all at the default priority
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_join_local = nullptr;
+    hipStream_t s_main = nullptr, s_pme = nullptr;   // both at the default priority; s_pme: k_local, then PME
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     // one MD step (evaluation without energies + integration) as an instantiated HIP graph; its
     // kernel arguments depend on the list, so every list rebuild invalidates it
     hipGraphExec_t step_graph = nullptr;

@@ -380,9 +380,9 @@ void timing_flush(Engine& e) {
     e.tpos = 0;
 }
 
-// One force evaluation.  PME (spread, FFT, gather) runs on its own stream between a fork
-// after the pack kernel (which zeroes the force slots) and a join before the caller's next
-// kernel, overlapping the pair / intra / bonded kernels (LIMA overlaps PME the same way,
+// One force evaluation.  The intra / bonded kernel and PME (spread, FFT, gather) run on a second
+// stream between a fork after the pack kernel (which zeroes the force slots) and a join before
+// the caller's next kernel, overlapping the pair kernel (LIMA overlaps PME the same way,
 // Engine.cu:333).  With timing on, everything runs on one stream so each phase is timed alone.
 // pack = false: the slots were already packed for these positions (by the integrator of the
 // previous step, or by an explicit k_pack before a replayed step graph).
@@ -393,22 +393,19 @@ void evaluate(Engine& e, const uint4* w, const int4* img, bool energy, bool time
     if (pack) launch_pack(e, w, s);
     e.packed = false;
     if (ev) HIP_CHECK(hipEventRecord(ev[1], s));
-    // Three streams after the pack: PME on s_pme, the intra-compound pairs and bonded terms
-    // (k_local, latency-bound) on s_local beside the VALU-bound pair kernel on the main stream.
-    // They write disjoint force and energy buffers; the main stream joins both before the
-    // integrator.
+    // Two streams after the pack: the intra-compound pairs and bonded terms (k_local,
+    // latency-bound), then PME, on s_pme beside the VALU-bound pair kernel on the main stream.
+    // They write disjoint force and energy buffers; the main stream joins s_pme before the
+    // integrator.  The side branch is captured first: a replayed graph puts the first branch
+    // after a fork on the launching queue, and a third branch costs one more cross-queue join.
     const bool split = !timed && !(e.prm.launch_mode & LAUNCH_ONE_STREAM);
     const bool pme_on = e.elec && e.pme.ncharged > 0;
     if (split) {
         HIP_CHECK(hipEventRecord(e.ev_fork, s));
-        if (pme_on) {
-            HIP_CHECK(hipStreamWaitEvent(e.s_pme, e.ev_fork, 0));
-            launch_pme(e, w, energy, e.s_pme);
-            HIP_CHECK(hipEventRecord(e.ev_join, e.s_pme));
-        }
-        HIP_CHECK(hipStreamWaitEvent(e.s_local, e.ev_fork, 0));
-        launch_local(e, w, img, energy, e.s_local);
-        HIP_CHECK(hipEventRecord(e.ev_join_local, e.s_local));
+        HIP_CHECK(hipStreamWaitEvent(e.s_pme, e.ev_fork, 0));
+        launch_local(e, w, img, energy, e.s_pme);
+        if (pme_on) launch_pme(e, w, energy, e.s_pme);
+        HIP_CHECK(hipEventRecord(e.ev_join, e.s_pme));
     }
     launch_pair(e, energy, s);
     if (ev) HIP_CHECK(hipEventRecord(ev[2], s));
@@ -421,8 +418,7 @@ void evaluate(Engine& e, const uint4* w, const int4* img, bool energy, bool time
         launch_local(e, w, img, energy, s);
     }
     if (split) {
-        HIP_CHECK(hipStreamWaitEvent(s, e.ev_join_local, 0));
-        if (pme_on) HIP_CHECK(hipStreamWaitEvent(s, e.ev_join, 0));
+        HIP_CHECK(hipStreamWaitEvent(s, e.ev_join, 0));
     } else {
         launch_pme(e, w, energy, s);
     }
@@ -519,8 +515,8 @@ void measure_temperature(Engine& e) {
     e.temp_measured = true;
 }
 
-// Captures one MD step without energies (pack, PME on the second stream, pair, local terms,
-// join, integrator) into a graph.  Replaying it replaces ~16 host launch and event calls per step
+// Captures one MD step without energies (local terms and PME on the second stream, pair, join,
+// integrator and the next step's pack) into a graph.  Replaying it replaces ~16 host launch and event calls per step
 // by one graph launch; the kernels, their arguments and the fork/join are those of the eager path.
 // An existing executable graph is updated in place when the topology is unchanged.
 void capture_step_graph(Engine& e) {
@@ -563,7 +559,6 @@ void capture_step_graph(Engine& e) {
 // rebuild due, so the caller issues the lost steps again.  A non-finite force or an overfull
 // column is not recoverable and throws.
 void rollback(Engine& e) {
-    HIP_CHECK(hipStreamSynchronize(e.s_local));
     HIP_CHECK(hipStreamSynchronize(e.s_pme));
     HIP_CHECK(hipStreamSynchronize(e.s_main));
     int f[DF_N];
@@ -702,7 +697,6 @@ void issue_steps(Engine& e, int64_t target) {
 // often as needed), and reads unread energy totals.  Every call that returns data runs it first.
 void settle(Engine& e) {
     for (;;) {
-        HIP_CHECK(hipStreamSynchronize(e.s_local));
         HIP_CHECK(hipStreamSynchronize(e.s_pme));
         HIP_CHECK(hipStreamSynchronize(e.s_main));
         HIP_CHECK(hipGetLastError());
@@ -1028,8 +1022,6 @@ void create(Engine& e, const lima_box_desc* b, const lima_simparams* p) {
     HIP_CHECK(hipStreamCreate(&e.s_pme));
     HIP_CHECK(hipEventCreateWithFlags(&e.ev_fork, hipEventDisableTiming));
     HIP_CHECK(hipEventCreateWithFlags(&e.ev_join, hipEventDisableTiming));
-    HIP_CHECK(hipStreamCreate(&e.s_local));
-    HIP_CHECK(hipEventCreateWithFlags(&e.ev_join_local, hipEventDisableTiming));
     for (auto& ev : e.ev_rebuild) HIP_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     for (auto& ev : e.tev) HIP_CHECK(hipEventCreate(&ev));
     nlist_alloc(e);
@@ -1084,8 +1076,6 @@ void destroy(Engine& e) {
     e.step_graph = nullptr;
     if (e.ev_fork) (void)hipEventDestroy(e.ev_fork);
     if (e.ev_join) (void)hipEventDestroy(e.ev_join);
-    if (e.ev_join_local) (void)hipEventDestroy(e.ev_join_local);
-    if (e.s_local) (void)hipStreamDestroy(e.s_local);
     if (e.s_main) (void)hipStreamDestroy(e.s_main);
     if (e.s_pme) (void)hipStreamDestroy(e.s_pme);
 }

@@ -831,7 +831,6 @@ void nlist_build(Engine& e, const uint4* w, bool checked) {
         if (need > nl.nent_cap) {
             HIP_CHECK(hipStreamSynchronize(s));   // kernels of earlier steps may still read the old buffer
             HIP_CHECK(hipStreamSynchronize(e.s_pme));
-            HIP_CHECK(hipStreamSynchronize(e.s_local));
             if (nl.ent) HIP_CHECK(hipFree(nl.ent));
             nl.ent = nullptr;
             HIP_CHECK(hipMalloc(&nl.ent, need * sizeof(uint2)));
