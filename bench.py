@@ -433,7 +433,7 @@ def main():
     ap.add_argument("--per-call-steps", type=int, default=1000,
                     help="steps of the per-call comparison (at least --steps)")
     ap.add_argument("--launch-mode", type=int, default=0,
-                    help="lima_simparams.launch_mode: 0 step graph replay with PME on a second stream; "
+                    help="lima_simparams.launch_mode: 0 step graph replay with k_local and PME on a second stream; "
                          "+1 eager launches, +2 PME in line on one stream")
     ap.add_argument("--em-systems", type=int, default=256)
     ap.add_argument("--em-train-systems", type=int, default=100)
@@ -556,7 +556,7 @@ def main():
                    "equilibration_steps": args.equil_steps,
                    "skin_nm": round(p.nlist_skin_nm, 4), "energy_reductions": "fp64",
                    "launch": ("eager" if args.launch_mode & 1 else "step graph") +
-                             (", one stream" if args.launch_mode & 2 else ", PME on a second stream")},
+                             (", one stream" if args.launch_mode & 2 else ", k_local and PME on a second stream")},
         "roofline": roof,
         "roofline_pme": roof_pme,
         "roofline_nlist": roof_nl,
