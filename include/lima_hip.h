@@ -120,8 +120,7 @@ typedef struct lima_simparams {
     int32_t energy_interval;          /* steps between FP64 energy reductions (0 = never) */
     int32_t launch_mode;              /* 0 = each MD step replayed as a HIP graph (captured again only
                                          when the list or the PME bins are reallocated), intra /
-                                         bonded terms and PME
-                                         on a second stream; or-ed flags
+                                         bonded terms and PME on a second stream; or-ed flags
                                          LIMA_LAUNCH_EAGER (launch the kernels one by one; the
                                          integrator then also packs the next step's slots) and
                                          LIMA_LAUNCH_ONE_STREAM (PME in line) */
